@@ -1,0 +1,11 @@
+"""MI355X (gfx950) FP64 solver for ASTR's per-timestep right-hand side.
+
+The compute path is the HIP library astr_gpu_amd/libastr_hip.so behind the
+C ABI of include/astr_hip.h; this package is its host-side binding.
+"""
+from ._lib import AstrError, AstrParams, load  # noqa: F401
+from .solver import (AstrSolver, box_metrics, default_params, haloed_shape,  # noqa: F401
+                     interior, tgv_state)
+
+__all__ = ["AstrError", "AstrParams", "AstrSolver", "box_metrics", "default_params",
+           "haloed_shape", "interior", "load", "tgv_state"]

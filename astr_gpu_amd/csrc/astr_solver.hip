@@ -1,0 +1,1269 @@
+// MI355X (gfx950) FP64 solver for ASTR's per-timestep right-hand side.
+//
+// Device layout.  Every field is one padded 3-D array of doubles holding
+// nodes -HM..n+HM on each axis (the reference's halo width, so the qswap
+// halo semantics can be reproduced exactly):
+//     index(i,j,k) = org + i + j*s1 + k*s2,
+// with i fastest; node i=0 sits at a 128-byte boundary and the row pitch s1
+// is a multiple of 16 doubles, so 16 consecutive i form one 128-byte line.
+//
+// Line sweeps.  A workgroup of 512 threads (8 wavefronts) owns a tile of 16
+// lines.  For each quantity it stages the 16 lines (nodes -HM..n+HM) in LDS
+// with coalesced loads, each wavefront solves 2 lines with the chunked-scan
+// Thomas solver of line_engine.hpp, and the results go back through an LDS
+// output tile with coalesced stores.  x-lines are contiguous; y/z tiles are
+// 16 consecutive i, i.e. one 128-byte segment per node.
+//
+// Per RK3 stage (src/mainloop.F90:392-476) the device runs:
+//   filter x,y,z (in place)           filterq      src/comsolver.F90:514
+//   qswap halo + seam average          qswap        src/parallel.F90:4849
+//   conv+grad sweeps x,y,z             gradcal + convrsdcal6 (+ qrhs=-qrhs)
+//   sigma/qflux pointwise              diffrsdcal6 first half
+//   viscous sweeps x,y,z               diffrsdcal6 second half
+//   RK3 combine + q2fvar               src/mainloop.F90:435-476
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/astr_hip.h"
+#include "line_engine.hpp"
+
+using namespace astr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(ASTR_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int TX = 16;     // lines per tile
+constexpr int WGT = 512;   // threads per sweep workgroup
+constexpr int NWAVE = WGT / 64;
+constexpr int OI = 16;     // element offset of node i=0 in a row
+constexpr int LMAX = 8;    // rows per lane supported (line length <= 64*LMAX)
+
+struct Geo {
+    int n[3];    // cells per axis (nodes 0..n)
+    long s[3];   // element stride per axis (s[0] = 1)
+    long org;    // element offset of node (0,0,0)
+};
+
+__device__ __forceinline__ long gidx(const Geo& g, int i, int j, int k)
+{
+    return g.org + i + j * g.s[1] + k * g.s[2];
+}
+
+// ---------------------------------------------------------------- sweeps
+
+enum Pass : int {
+    PASS_FILTER = 0,       // filterq along one axis, q(1:5) in place
+    PASS_CONV = 1,         // gradcal + convrsdcal6 along one axis
+    PASS_VISC = 2,         // viscous fluxes along one axis
+    PASS_DERIV_FIELD = 3,  // fds%central of one field (halos as given)
+    PASS_FILTER_FIELD = 4  // compact_filter of one field (halos as given)
+};
+
+struct SweepArgs {
+    Geo g;
+    int dir;          // 0,1,2
+    int pass;
+    LineTab tab;      // line system of this axis (derivative or filter)
+    const FiltCoef* fc;
+    int na, nb;       // line index ranges: a in [0,na), b in [0,nb)
+    int ntile_a;      // tiles along a
+    int rs_in;
+    // fields
+    double* q[5];
+    const double* prim[6];  // rho,u,v,w,p,T
+    double* dvel[9];
+    double* dtmp[3];
+    double* qrhs[5];
+    const double* sig[6];
+    const double* qfl[3];
+    const double* jac;
+    const double* xi;       // metric along dir, xi[n+HM]
+    const double* fin;      // field in (PASS_*_FIELD)
+    double* fout;           // field out (PASS_*_FIELD)
+};
+
+// node -> element offset along the line, with periodic wrap (dataswap:
+// plain copy of hm planes, src/parallel.F90:4170-4175) or direct.
+__device__ __forceinline__ int wrap_node(int n, int dim)
+{
+    return n < 0 ? n + dim : (n > dim ? n - dim : n);
+}
+
+template <int L, int PASS>
+__global__ __launch_bounds__(WGT) void k_sweep(SweepArgs a)
+{
+    // One LDS tile: line l, staged node n (node n-HM) at tile[l*rs_in + n].
+    // The solution of a line overwrites that line's slots in place, which is
+    // safe because only the wavefront solving a line reads it.
+    extern __shared__ double tile[];
+
+    const int ta = blockIdx.x % a.ntile_a;
+    const int b = blockIdx.x / a.ntile_a;
+    const int a0 = ta * TX;
+    const int dir = a.dir;
+    const int dim = a.g.n[dir];
+    const int nn = dim + 1 + 2 * HM;   // nodes per staged line
+    const int no = dim + 1;            // output nodes per line
+    const long sd = a.g.s[dir];
+
+    // element offset of node 0 of line l of this tile
+    auto line_off = [&](int l) -> long {
+        const long av = a0 + l;
+        if (dir == 0) return a.g.org + av * a.g.s[1] + (long)b * a.g.s[2];
+        if (dir == 1) return a.g.org + av + (long)b * a.g.s[2];
+        return a.g.org + av + (long)b * a.g.s[1];
+    };
+    const int nvalid = min(TX, a.na - a0);
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+
+    // loader element mapping: x-lines node-fastest, y/z lines line-fastest
+    const int nel_in = TX * nn;
+    constexpr int EMAX = (TX * (64 * L + 2 * HM + 1) + WGT - 1) / WGT;
+    constexpr int NKEEP = PASS == PASS_CONV ? 3 : PASS == PASS_VISC ? 2 : 0;
+    constexpr int EK = NKEEP ? EMAX : 1;
+    constexpr int NQ = PASS == PASS_FILTER ? 5 : PASS == PASS_CONV ? 9 : PASS == PASS_VISC ? 4 : 1;
+
+    // per-element values kept in registers across the quantities of a pass:
+    // conv: U = dxi(d,d)*u_d, p, J;  visc: dxi(d,d), J
+    double k0[EK], k1[EK], k2[EK];
+    if constexpr (NKEEP > 0) {
+#pragma unroll
+        for (int e = 0; e < EMAX; ++e) {
+            k0[e] = 0.0; k1[e] = 0.0; k2[e] = 0.0;
+            const int el = tid + e * WGT;
+            if (el < nel_in) {
+                int l, n;
+                if (dir == 0) { l = el / nn; n = el - l * nn; }
+                else { n = el / TX; l = el - n * TX; }
+                if (l < nvalid) {
+                    const long gi = line_off(l) + (long)(n - HM) * sd;
+                    k2[e] = a.jac[gi];
+                    if constexpr (PASS == PASS_CONV) {
+                        // uu = dxi(d,1)*u + dxi(d,2)*v + dxi(d,3)*w with the
+                        // off-diagonal metrics zero (src/solver.F90:2201)
+                        k0[e] = a.xi[n] * a.prim[1 + dir][gi];
+                        k1[e] = a.prim[4][gi];
+                    } else {
+                        k0[e] = a.xi[n];
+                    }
+                }
+            }
+        }
+    }
+
+    for (int qi = 0; qi < NQ; ++qi) {
+        // ---------------- stage the 16 lines of this quantity in LDS
+#pragma unroll
+        for (int e = 0; e < EMAX; ++e) {
+            const int el = tid + e * WGT;
+            if (el < nel_in) {
+                int l, n;
+                if (dir == 0) { l = el / nn; n = el - l * nn; }
+                else { n = el / TX; l = el - n * TX; }
+                double v = 0.0;
+                if (l < nvalid) {
+                    const int node = n - HM;
+                    const long lo = line_off(l);
+                    if constexpr (PASS == PASS_FILTER) {
+                        v = a.q[qi][lo + (long)wrap_node(node, dim) * sd];
+                    } else if constexpr (PASS == PASS_CONV) {
+                        const long gi = lo + (long)node * sd;
+                        if (qi < 4) {
+                            v = a.prim[qi < 3 ? 1 + qi : 5][gi];
+                        } else {
+                            const int m = qi - 4;
+                            const double qm = a.q[m][gi];
+                            const double uu = k0[e], pr = k1[e], jc = k2[e];
+                            // fcs(:,1..5), src/solver.F90:2203-2207
+                            if (m == 0) v = jc * qm * uu;
+                            else if (m == 4) v = jc * (qm + pr) * uu;
+                            else if (m == dir + 1) v = jc * (qm * uu + a.xi[n] * pr);
+                            else v = jc * (qm * uu);
+                        }
+                    } else if constexpr (PASS == PASS_VISC) {
+                        // ff(:,2..5), src/solver.F90:2628-2639: the diagonal
+                        // metric picks sigma(d,c) and qflux(d)
+                        const long gw = lo + (long)wrap_node(node, dim) * sd;
+                        const double s = qi < 3 ? a.sig[qi][gw] : a.qfl[0][gw];
+                        v = (s * k0[e]) * k2[e];
+                    } else {
+                        v = a.fin[lo + (long)node * sd];
+                    }
+                }
+                tile[l * a.rs_in + n] = v;
+            }
+        }
+        __syncthreads();
+
+        // ---------------- solve: wavefront w takes lines w and w+8
+        for (int l = wave; l < nvalid; l += NWAVE) {
+            double x[L];
+            double* ln = tile + l * a.rs_in;
+            solve_line<L>(ln, a.tab, a.fc, x);
+            const int lane = tid & 63;
+            double keep0 = ln[HM], keepd = ln[HM + dim];
+#pragma unroll
+            for (int t = 0; t < L; ++t) {
+                const int node = a.tab.first + lane * L + t;
+                if (lane * L + t < a.tab.R && node >= 0 && node <= dim) {
+                    double v = x[t];
+                    if (a.tab.op == OP_FILTER) {
+                        // compact_filter keeps the boundary node
+                        // (src/filter.F90:141-142)
+                        if (node == 0 && lo_boundary(a.tab.ntype)) v = keep0;
+                        if (node == dim && hi_boundary(a.tab.ntype)) v = keepd;
+                    }
+                    ln[node + HM] = v;
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---------------- write back
+        const int nel_out = TX * no;
+        for (int el = tid; el < nel_out; el += WGT) {
+            int l, n;
+            if (dir == 0) { l = el / no; n = el - l * no; }
+            else { n = el / TX; l = el - n * TX; }
+            if (l >= nvalid) continue;
+            const double o = tile[l * a.rs_in + n + HM];
+            const long gi = line_off(l) + (long)n * sd;
+            if constexpr (PASS == PASS_FILTER) {
+                a.q[qi][gi] = o;
+            } else if constexpr (PASS == PASS_CONV) {
+                if (qi < 3) {
+                    a.dvel[qi * 3 + dir][gi] = o * a.xi[n + HM];
+                } else if (qi == 3) {
+                    a.dtmp[dir][gi] = o * a.xi[n + HM];
+                } else {
+                    const int m = qi - 4;
+                    if (dir == 0) a.qrhs[m][gi] = o;
+                    else if (dir == 1) a.qrhs[m][gi] = a.qrhs[m][gi] + o;
+                    else a.qrhs[m][gi] = -(a.qrhs[m][gi] + o);  // qrhs=-qrhs
+                }
+            } else if constexpr (PASS == PASS_VISC) {
+                a.qrhs[qi + 1][gi] = a.qrhs[qi + 1][gi] + o;
+            } else {
+                a.fout[gi] = o;
+            }
+        }
+        if (qi + 1 < NQ) __syncthreads();  // the next staging overwrites the tile
+    }
+}
+
+// ---------------------------------------------------------------- pointwise
+
+struct PtArgs {
+    Geo g;
+    double* q[5];
+    double* prim[6];
+    double* qsave[5];
+    const double* qrhs[5];
+    const double* dvel[9];
+    const double* dtmp[3];
+    double* sig[6];
+    double* qfl[3];
+    const double* jac;
+    double const1, const2, const5, const6;
+    double reynolds, prandtl, tempconst, tempconst1;
+    double rka, rkb, rkc, dt;
+    int stage;
+    double* part;
+};
+
+__device__ __forceinline__ void interior_ijk(const Geo& g, long p, int& i, int& j, int& k)
+{
+    const long ni = g.n[0] + 1, nj = g.n[1] + 1;
+    i = (int)(p % ni);
+    const long r = p / ni;
+    j = (int)(r % nj);
+    k = (int)(r / nj);
+}
+
+// q2fvar_3da, src/fludyna.F90:545-634 (nondimensional, numq = 5)
+__device__ __forceinline__ void q2fvar(const PtArgs& a, const double qv[5], long gi)
+{
+    const double rho = qv[0];
+    const double u = qv[1] / rho, v = qv[2] / rho, w = qv[3] / rho;
+    const double p = (qv[4] - 0.5 * rho * (u * u + v * v + w * w)) / a.const6;
+    a.prim[0][gi] = rho;
+    a.prim[1][gi] = u;
+    a.prim[2][gi] = v;
+    a.prim[3][gi] = w;
+    a.prim[4][gi] = p;
+    a.prim[5][gi] = p / rho * a.const2;
+}
+
+// miucal, src/fludyna.F90:791-820
+__device__ __forceinline__ double miucal(const PtArgs& a, double t)
+{
+    return t * sqrt(t) * a.tempconst1 / (t + a.tempconst);
+}
+
+// qswap halos (src/parallel.F90:4877-5200, single rank, periodic): the
+// reference copies the i-halos before any averaging, the j-halos after the
+// i-average, the k-halos after the i- and j-averages, and refreshes the
+// primitives on every halo cell it wrote.  Each halo cell is computed here
+// from the unaveraged interior.  Reads interior only, writes halos only.
+__global__ void k_qswap_halo(PtArgs a)
+{
+    const Geo& g = a.g;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    // face sizes: i-halo 2*HM*(n1+1)*(n2+1), j-halo 2*HM*(n0+1)*(n2+1), ...
+    const long fi = 2L * HM * (n1 + 1) * (n2 + 1);
+    const long fj = 2L * HM * (n0 + 1) * (n2 + 1);
+    const long fk = 2L * HM * (n0 + 1) * (n1 + 1);
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= fi + fj + fk) return;
+    int axis;
+    if (p < fi) axis = 0;
+    else if (p < fi + fj) { axis = 1; p -= fi; }
+    else { axis = 2; p -= fi + fj; }
+    // p -> (h in 0..2HM-1, u, v) with u fastest
+    const int nu = axis == 0 ? n1 + 1 : n0 + 1;
+    const int nv = axis == 2 ? n1 + 1 : n2 + 1;
+    const int h = (int)(p % (2 * HM));
+    const long r = p / (2 * HM);
+    const int uu = (int)(r % nu);
+    const int vv = (int)(r / nu);
+    (void)nv;
+    const int dimh = g.n[axis];
+    // halo node and its periodic source
+    const int hn = h < HM ? -(h + 1) : dimh + (h - HM + 1);
+    const int sn = h < HM ? dimh - (h + 1) : (h - HM + 1);
+    int ci[3];  // halo cell
+    int si[3];  // source cell
+    if (axis == 0) { ci[0] = hn; ci[1] = uu; ci[2] = vv; }
+    else if (axis == 1) { ci[0] = uu; ci[1] = hn; ci[2] = vv; }
+    else { ci[0] = uu; ci[1] = vv; ci[2] = hn; }
+    si[0] = ci[0]; si[1] = ci[1]; si[2] = ci[2];
+    si[axis] = sn;
+    double qv[5];
+    // the source value after the averages of the axes before `axis`
+    for (int m = 0; m < 5; ++m) {
+        if (axis == 0) {
+            qv[m] = a.q[m][gidx(g, si[0], si[1], si[2])];
+        } else {
+            // x-average (only if i is a seam node), then y-average (axis 2)
+            auto xavg = [&](int j, int k) -> double {
+                const int i = si[0];
+                if (i == 0 || i == n0)
+                    return 0.5 * (a.q[m][gidx(g, 0, j, k)] + a.q[m][gidx(g, n0, j, k)]);
+                return a.q[m][gidx(g, i, j, k)];
+            };
+            if (axis == 1) {
+                qv[m] = xavg(si[1], si[2]);
+            } else {
+                const int j = si[1];
+                if (j == 0 || j == n1) qv[m] = 0.5 * (xavg(0, si[2]) + xavg(n1, si[2]));
+                else qv[m] = xavg(j, si[2]);
+            }
+        }
+    }
+    const long gi = gidx(g, ci[0], ci[1], ci[2]);
+    for (int m = 0; m < 5; ++m) a.q[m][gi] = qv[m];
+    q2fvar(a, qv, gi);
+}
+
+// qswap duplicate-node averages q(0)=q(n)=(q(0)+q(n))/2, in the order i, j,
+// k (src/parallel.F90:4884-4885, 5029-5030, 5168-5169), and the primitive
+// refresh of every seam node.  One thread owns one group of duplicates: the
+// canonical node has coordinates in [0,n) with at least one zero.
+__global__ void k_qswap_seam(PtArgs a)
+{
+    const Geo& g = a.g;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const long tot = (long)n0 * n1 * n2;
+    const long inner = (long)(n0 - 1) * (n1 - 1) * (n2 - 1);
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= tot - inner) return;
+    // enumerate canonical nodes with some coordinate zero:
+    //   i == 0: n1*n2 nodes; i > 0, j == 0: (n0-1)*n2; i,j > 0, k == 0
+    int i, j, k;
+    const long c0 = (long)n1 * n2, c1 = (long)(n0 - 1) * n2;
+    if (p < c0) { i = 0; j = (int)(p % n1); k = (int)(p / n1); }
+    else if (p < c0 + c1) { p -= c0; i = 1 + (int)(p % (n0 - 1)); j = 0; k = (int)(p / (n0 - 1)); }
+    else { p -= c0 + c1; i = 1 + (int)(p % (n0 - 1)); j = 1 + (int)(p / (n0 - 1)); k = 0; }
+    const bool sx = i == 0, sy = j == 0, sz = k == 0;
+    // member (bx,by,bz): coordinate n if the bit is set
+    for (int m = 0; m < 5; ++m) {
+        double v[2][2][2];
+        for (int bz = 0; bz <= (int)sz; ++bz)
+            for (int by = 0; by <= (int)sy; ++by)
+                for (int bx = 0; bx <= (int)sx; ++bx)
+                    v[bz][by][bx] = a.q[m][gidx(g, bx ? n0 : i, by ? n1 : j, bz ? n2 : k)];
+        if (sx)
+            for (int bz = 0; bz <= (int)sz; ++bz)
+                for (int by = 0; by <= (int)sy; ++by)
+                    v[bz][by][0] = v[bz][by][1] = 0.5 * (v[bz][by][0] + v[bz][by][1]);
+        if (sy)
+            for (int bz = 0; bz <= (int)sz; ++bz)
+                for (int bx = 0; bx <= (int)sx; ++bx)
+                    v[bz][0][bx] = v[bz][1][bx] = 0.5 * (v[bz][0][bx] + v[bz][1][bx]);
+        if (sz)
+            for (int by = 0; by <= (int)sy; ++by)
+                for (int bx = 0; bx <= (int)sx; ++bx)
+                    v[0][by][bx] = v[1][by][bx] = 0.5 * (v[0][by][bx] + v[1][by][bx]);
+        for (int bz = 0; bz <= (int)sz; ++bz)
+            for (int by = 0; by <= (int)sy; ++by)
+                for (int bx = 0; bx <= (int)sx; ++bx)
+                    a.q[m][gidx(g, bx ? n0 : i, by ? n1 : j, bz ? n2 : k)] = v[bz][by][bx];
+    }
+    for (int bz = 0; bz <= (int)sz; ++bz)
+        for (int by = 0; by <= (int)sy; ++by)
+            for (int bx = 0; bx <= (int)sx; ++bx) {
+                const long gi = gidx(g, bx ? n0 : i, by ? n1 : j, bz ? n2 : k);
+                double qv[5];
+                for (int m = 0; m < 5; ++m) qv[m] = a.q[m][gi];
+                q2fvar(a, qv, gi);
+            }
+}
+
+// sigma and qflux of diffrsdcal6, src/solver.F90:2434-2546 (turbmode none)
+__global__ void k_sigma(PtArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ntot = (long)(a.g.n[0] + 1) * (a.g.n[1] + 1) * (a.g.n[2] + 1);
+    if (p >= ntot) return;
+    int i, j, k;
+    interior_ijk(a.g, p, i, j, k);
+    const long gi = gidx(a.g, i, j, k);
+    const double miu = miucal(a, a.prim[5][gi]) / a.reynolds;
+    const double d11 = a.dvel[0][gi], d12 = a.dvel[1][gi], d13 = a.dvel[2][gi];
+    const double d21 = a.dvel[3][gi], d22 = a.dvel[4][gi], d23 = a.dvel[5][gi];
+    const double d31 = a.dvel[6][gi], d32 = a.dvel[7][gi], d33 = a.dvel[8][gi];
+    const double s11 = d11, s12 = 0.5 * (d12 + d21), s13 = 0.5 * (d13 + d31);
+    const double s22 = d22, s23 = 0.5 * (d23 + d32), s33 = d33;
+    const double skk = (1.0 / 3.0) * (s11 + s22 + s33);
+    const double miu2 = 2.0 * miu;
+    const double hcc = (miu / a.prandtl) / a.const5;
+    const double g1 = miu2 * (s11 - skk), g2 = miu2 * s12, g3 = miu2 * s13;
+    const double g4 = miu2 * (s22 - skk), g5 = miu2 * s23, g6 = miu2 * (s33 - skk);
+    a.sig[0][gi] = g1; a.sig[1][gi] = g2; a.sig[2][gi] = g3;
+    a.sig[3][gi] = g4; a.sig[4][gi] = g5; a.sig[5][gi] = g6;
+    const double u = a.prim[1][gi], v = a.prim[2][gi], w = a.prim[3][gi];
+    a.qfl[0][gi] = hcc * a.dtmp[0][gi] + g1 * u + g2 * v + g3 * w;
+    a.qfl[1][gi] = hcc * a.dtmp[1][gi] + g2 * u + g4 * v + g5 * w;
+    a.qfl[2][gi] = hcc * a.dtmp[2][gi] + g3 * u + g5 * v + g6 * w;
+}
+
+// RK3 combine (src/mainloop.F90:423-445) and updatefvar (src/fludyna.F90:
+// 191-242) on the interior.
+__global__ void k_rk(PtArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ntot = (long)(a.g.n[0] + 1) * (a.g.n[1] + 1) * (a.g.n[2] + 1);
+    if (p >= ntot) return;
+    int i, j, k;
+    interior_ijk(a.g, p, i, j, k);
+    const long gi = gidx(a.g, i, j, k);
+    const double jc = a.jac[gi];
+    double qv[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const double qo = a.q[m][gi];
+        double qs;
+        if (a.stage == 1) {
+            qs = qo * jc;
+            a.qsave[m][gi] = qs;
+        } else {
+            qs = a.qsave[m][gi];
+        }
+        const double vn = a.rka * qs + a.rkb * qo * jc + a.rkc * a.qrhs[m][gi] * a.dt;
+        qv[m] = vn / jc;
+        a.q[m][gi] = qv[m];
+    }
+    q2fvar(a, qv, gi);
+}
+
+// Partial sums of kenergycal / enstophycal / diss_rate_cal over 1..n
+// (src/statistic.F90:870-1042).
+__global__ __launch_bounds__(256) void k_stats(PtArgs a)
+{
+    __shared__ double sh[3][256];
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ntot = (long)a.g.n[0] * a.g.n[1] * a.g.n[2];
+    double ke = 0.0, en = 0.0, di = 0.0;
+    if (p < ntot) {
+        const long n0 = a.g.n[0], n1 = a.g.n[1];
+        const int i = 1 + (int)(p % n0);
+        const long r = p / n0;
+        const int j = 1 + (int)(r % n1);
+        const int k = 1 + (int)(r / n1);
+        const long gi = gidx(a.g, i, j, k);
+        const double rho = a.prim[0][gi];
+        const double u = a.prim[1][gi], v = a.prim[2][gi], w = a.prim[3][gi];
+        ke = rho * (u * u + v * v + w * w);
+        const double d11 = a.dvel[0][gi], d12 = a.dvel[1][gi], d13 = a.dvel[2][gi];
+        const double d21 = a.dvel[3][gi], d22 = a.dvel[4][gi], d23 = a.dvel[5][gi];
+        const double d31 = a.dvel[6][gi], d32 = a.dvel[7][gi], d33 = a.dvel[8][gi];
+        const double o1 = d32 - d23, o2 = d13 - d31, o3 = d21 - d12;
+        en = rho * (o1 * o1 + o2 * o2 + o3 * o3);
+        const double miu = miucal(a, a.prim[5][gi]) / a.reynolds;
+        const double s12 = 0.5 * (d12 + d21), s13 = 0.5 * (d13 + d31), s23 = 0.5 * (d23 + d32);
+        const double div = d11 + d22 + d33;
+        di = 2.0 * miu * (d11 * d11 + d22 * d22 + d33 * d33 +
+                          2.0 * (s12 * s12 + s13 * s13 + s23 * s23) - (1.0 / 3.0) * (div * div));
+    }
+    sh[0][threadIdx.x] = ke;
+    sh[1][threadIdx.x] = en;
+    sh[2][threadIdx.x] = di;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) a.part[3 * blockIdx.x + c] = sh[c][0];
+}
+
+}  // namespace
+
+// ====================================================================== host
+
+struct astr_ctx {
+    astr_params p;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Geo g{};
+    long pi = 0, pj = 0, pk = 0, nfield = 0;  // padded dims, elements per field
+    double* base = nullptr;
+    double *q[5], *prim[6], *qsave[5], *qrhs[5], *dvel[9], *dtmp[3], *sig[6], *qfl[3], *jac;
+    double* scratch = nullptr;  // field in of the field-level operators
+    double* scratch2 = nullptr; // field out of the field-level operators
+    double* xi = nullptr;       // 3 metric lines, each 64*LMAX+2*HM+1
+    long xi_stride = 0;
+    double* tabs = nullptr;
+    LineTab tab[3][2];
+    int L[3];
+    FiltCoef* fc = nullptr;
+    double* part = nullptr;
+    long npart = 0;
+    std::vector<double> host_part;
+    double const1, const2, const3, const4, const5, const6, const7, tempconst, tempconst1;
+    bool have_geom = false, have_state = false;
+    bool timing = false;
+    hipEvent_t ev[16] = {};
+    double tms[7] = {0, 0, 0, 0, 0, 0, 0};
+};
+
+namespace {
+
+// ---- line systems (host): fd_scheme_initiate, compact_filter_initiate and
+// tridiagonal_thomas_proprocess restated, then the chunk products.
+struct HostTab {
+    int R, first, last;
+    std::vector<double> t;  // 5 * 64 * L
+};
+
+void node_range(int ntype, int dim, int ext, int& first, int& last)
+{
+    first = (ntype == 1 || ntype == 4) ? 0 : -ext;
+    last = (ntype == 2 || ntype == 4) ? dim : dim + ext;
+}
+
+HostTab build_tab(int op, int ntype, int dim, double alfa, int L)
+{
+    HostTab h;
+    node_range(ntype, dim, op == OP_DERIV ? 1 : 3, h.first, h.last);
+    const int R = h.last - h.first + 1;
+    h.R = R;
+    std::vector<double> a(R), c(R);
+    if (op == OP_DERIV) {
+        // src/derivative.F90:104-139 (643c)
+        for (int r = 0; r < R; ++r) a[r] = c[r] = 1.0 / 3.0;
+        a[0] = c[0] = 0.0;
+        a[R - 1] = c[R - 1] = 0.0;
+        if (ntype == 1 || ntype == 4) { a[0] = c[0] = 2.0; a[1] = c[1] = 0.25; }
+        if (ntype == 2 || ntype == 4) { a[R - 1] = c[R - 1] = 2.0; a[R - 2] = c[R - 2] = 0.25; }
+    } else {
+        // src/filter.F90:44-96
+        const double b0 = (ntype == 1 || ntype == 4) ? 0.98 : 1.11;
+        const double bm = (ntype == 2 || ntype == 4) ? 0.98 : 1.11;
+        for (int r = 0; r < R; ++r) a[r] = c[r] = alfa;
+        a[0] = c[0] = b0;
+        a[R - 1] = c[R - 1] = bm;
+    }
+    const int S = 64 * L;
+    h.t.assign(5 * S, 0.0);
+    double* ac1 = h.t.data();
+    double* ac2 = ac1 + S;
+    double* ac3 = ac1 + 2 * S;
+    double* gp = ac1 + 3 * S;
+    double* hp = ac1 + 4 * S;
+    // src/commfunc.F90:760-770
+    ac1[0] = c[0];
+    ac2[0] = 1.0;
+    ac3[0] = 0.0;
+    for (int r = 1; r < R; ++r) {
+        const double den = 1.0 - a[r] * ac1[r - 1];
+        ac1[r] = c[r] / den;
+        ac2[r] = 1.0 / den;
+        ac3[r] = a[r] / den;
+    }
+    ac1[R - 1] = 0.0;  // the solver starts the back substitution at x(n)=d(n)
+    for (int l = 0; l < 64; ++l) {
+        const int r0 = l * L;
+        double pr = 1.0;
+        for (int t = 0; t < L; ++t) {
+            pr *= -ac3[r0 + t];
+            gp[r0 + t] = pr;
+        }
+        pr = 1.0;
+        for (int t = L - 1; t >= 0; --t) {
+            pr *= -ac1[r0 + t];
+            hp[r0 + t] = pr;
+        }
+    }
+    return h;
+}
+
+// filter_coefficient_cal closed forms, src/filter.F90:303-431
+FiltCoef build_coef(double al, double bh, double bb)
+{
+    FiltCoef f{};
+    f.c6[0] = (11.0 + 10.0 * al) / 32.0;
+    f.c6[1] = (15.0 + 34.0 * al) / 64.0;
+    f.c6[2] = (-3.0 + 6.0 * al) / 32.0;
+    f.c6[3] = (1.0 - 2.0 * al) / 64.0;
+    f.c8[0] = (93.0 + 70.0 * al) / 256.0;
+    f.c8[1] = (7.0 + 18.0 * al) / 32.0;
+    f.c8[2] = (-7.0 + 14.0 * al) / 64.0;
+    f.c8[3] = (1.0 - 2.0 * al) / 32.0;
+    f.c8[4] = (-1.0 + 2.0 * al) / 256.0;
+    f.c10[0] = (193.0 + 126.0 * al) / 512.0;
+    f.c10[1] = (105.0 + 302.0 * al) / 512.0;
+    f.c10[2] = (-15.0 + 30.0 * al) / 128.0;
+    f.c10[3] = (45.0 - 90.0 * al) / 1024.0;
+    f.c10[4] = (-5.0 + 10.0 * al) / 512.0;
+    f.c10[5] = (1.0 - 2.0 * al) / 1024.0;
+    const double cb[3][7] = {
+        {(63.0 + 1.0 * bb) / 64.0, (3.0 + 29.0 * bb) / 32.0, (-15.0 + 15.0 * bb) / 64.0,
+         (5.0 - 5.0 * bb) / 16.0, (-15.0 + 15.0 * bb) / 64.0, (3.0 - 3.0 * bb) / 32.0,
+         (-1.0 + 1.0 * bb) / 64.0},
+        {(1.0 + 62.0 * al) / 64.0, (29.0 + 6.0 * al) / 32.0, (15.0 + 34.0 * al) / 64.0,
+         (-5.0 + 10.0 * al) / 16.0, (15.0 - 30.0 * al) / 64.0, (-3.0 + 6.0 * al) / 32.0,
+         (1.0 - 2.0 * al) / 64.0},
+        {(-1.0 + 2.0 * al) / 64.0, (3.0 + 26.0 * al) / 32.0, (49.0 + 30.0 * al) / 64.0,
+         (5.0 + 6.0 * al) / 16.0, (-15.0 + 30.0 * al) / 64.0, (3.0 - 6.0 * al) / 32.0,
+         (-1.0 + 2.0 * al) / 64.0}};
+    const double ch[3][11] = {
+        {(-1.0 + 1.0 * bh) / 1024.0, (5.0 - 5.0 * bh) / 512.0, (979.0 + 45.0 * bh) / 1024.0,
+         (15.0 + 113.0 * bh) / 128.0, (-105.0 + 105.0 * bh) / 512.0, (63.0 - 63.0 * bh) / 256.0,
+         (-105.0 + 105.0 * bh) / 512.0, (15.0 - 15.0 * bh) / 128.0, (-45.0 + 45.0 * bh) / 1024.0,
+         (5.0 - 5.0 * bh) / 512.0, (-1.0 + 1.0 * bh) / 1024.0},
+        {(1.0 - 2.0 * al) / 1024.0, (-5.0 + 10.0 * al) / 512.0, (45.0 + 934.0 * al) / 1024.0,
+         (113.0 + 30.0 * al) / 128.0, (105.0 + 302.0 * al) / 512.0, (-63.0 + 126.0 * al) / 256.0,
+         (105.0 - 210.0 * al) / 512.0, (-15.0 + 30.0 * al) / 128.0, (45.0 - 90.0 * al) / 1024.0,
+         (-5.0 + 10.0 * al) / 512.0, (1.0 - 2.0 * al) / 1024.0},
+        {(-1.0 + 2.0 * al) / 1024.0, (5.0 - 10.0 * al) / 512.0, (-45.0 + 90.0 * al) / 1024.0,
+         (15.0 + 98.0 * al) / 128.0, (407.0 + 210.0 * al) / 512.0, (63.0 + 130.0 * al) / 256.0,
+         (-105.0 + 210.0 * al) / 512.0, (15.0 - 30.0 * al) / 128.0, (-45.0 + 90.0 * al) / 1024.0,
+         (5.0 - 10.0 * al) / 512.0, (-1.0 + 2.0 * al) / 1024.0}};
+    memcpy(f.cb, cb, sizeof cb);
+    memcpy(f.ch, ch, sizeof ch);
+    return f;
+}
+
+PtArgs pt_args(astr_ctx* c)
+{
+    PtArgs a{};
+    a.g = c->g;
+    for (int m = 0; m < 5; ++m) { a.q[m] = c->q[m]; a.qsave[m] = c->qsave[m]; a.qrhs[m] = c->qrhs[m]; }
+    for (int m = 0; m < 6; ++m) { a.prim[m] = c->prim[m]; a.sig[m] = c->sig[m]; }
+    for (int m = 0; m < 9; ++m) a.dvel[m] = c->dvel[m];
+    for (int m = 0; m < 3; ++m) { a.dtmp[m] = c->dtmp[m]; a.qfl[m] = c->qfl[m]; }
+    a.jac = c->jac;
+    a.const1 = c->const1; a.const2 = c->const2; a.const5 = c->const5; a.const6 = c->const6;
+    a.reynolds = c->p.reynolds; a.prandtl = c->p.prandtl;
+    a.tempconst = c->tempconst; a.tempconst1 = c->tempconst1;
+    a.part = c->part;
+    return a;
+}
+
+SweepArgs sweep_args(astr_ctx* c, int dir, int pass)
+{
+    SweepArgs a{};
+    a.g = c->g;
+    a.dir = dir;
+    a.pass = pass;
+    const bool filt = pass == PASS_FILTER || pass == PASS_FILTER_FIELD;
+    a.tab = c->tab[dir][filt ? 1 : 0];
+    a.fc = c->fc;
+    const int n0 = c->g.n[0], n1 = c->g.n[1], n2 = c->g.n[2];
+    if (dir == 0) { a.na = n1 + 1; a.nb = n2 + 1; }
+    else if (dir == 1) { a.na = n0 + 1; a.nb = n2 + 1; }
+    else { a.na = n0 + 1; a.nb = n1 + 1; }
+    a.ntile_a = (a.na + TX - 1) / TX;
+    const int dim = c->g.n[dir];
+    a.rs_in = (dim + 1 + 2 * HM) | 1;
+    for (int m = 0; m < 5; ++m) { a.q[m] = c->q[m]; a.qrhs[m] = c->qrhs[m]; }
+    for (int m = 0; m < 6; ++m) a.prim[m] = c->prim[m];
+    for (int m = 0; m < 9; ++m) a.dvel[m] = c->dvel[m];
+    for (int m = 0; m < 3; ++m) a.dtmp[m] = c->dtmp[m];
+    static const int sidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    for (int cc = 0; cc < 3; ++cc) a.sig[cc] = c->sig[sidx[dir][cc]];
+    a.qfl[0] = c->qfl[dir];
+    a.jac = c->jac;
+    a.xi = c->xi + dir * c->xi_stride;
+    return a;
+}
+
+template <int PASS>
+int launch_sweep_p(astr_ctx* c, const SweepArgs& a)
+{
+    const int L = c->L[a.dir];
+    const size_t lds = sizeof(double) * TX * (size_t)a.rs_in;
+    const int nblk = a.ntile_a * a.nb;
+    switch (L) {
+#define CASE(LL)                                                                          \
+    case LL:                                                                              \
+        hipLaunchKernelGGL((k_sweep<LL, PASS>), dim3(nblk), dim3(WGT), lds, c->stream, a); \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default:
+        return fail(ASTR_E_UNSUPPORTED, "line too long");
+    }
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
+}
+
+int launch_sweep(astr_ctx* c, const SweepArgs& a)
+{
+    switch (a.pass) {
+    case PASS_FILTER: return launch_sweep_p<PASS_FILTER>(c, a);
+    case PASS_CONV: return launch_sweep_p<PASS_CONV>(c, a);
+    case PASS_VISC: return launch_sweep_p<PASS_VISC>(c, a);
+    case PASS_DERIV_FIELD: return launch_sweep_p<PASS_DERIV_FIELD>(c, a);
+    case PASS_FILTER_FIELD: return launch_sweep_p<PASS_FILTER_FIELD>(c, a);
+    }
+    return fail(ASTR_E_ARG, "unknown pass");
+}
+
+long n_interior(const astr_ctx* c)
+{
+    return (long)(c->g.n[0] + 1) * (c->g.n[1] + 1) * (c->g.n[2] + 1);
+}
+
+int launch_pt(astr_ctx* c, void (*k)(PtArgs), long nthreads, const PtArgs& a, int block = 256)
+{
+    if (nthreads <= 0) return ASTR_OK;
+    const long nb = (nthreads + block - 1) / block;
+    hipLaunchKernelGGL(k, dim3((unsigned)nb), dim3(block), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
+}
+
+int set_dev(astr_ctx* c)
+{
+    HIPCHK(hipSetDevice(c->device));
+    return ASTR_OK;
+}
+
+#define TRY(x)                      \
+    do {                            \
+        int rc_ = (x);              \
+        if (rc_ != ASTR_OK) return rc_; \
+    } while (0)
+
+int do_filterq(astr_ctx* c)
+{
+    for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_FILTER)));
+    return ASTR_OK;
+}
+
+int do_qswap(astr_ctx* c)
+{
+    PtArgs a = pt_args(c);
+    const int n0 = c->g.n[0], n1 = c->g.n[1], n2 = c->g.n[2];
+    const long nh = 2L * HM * ((long)(n1 + 1) * (n2 + 1) + (long)(n0 + 1) * (n2 + 1) +
+                               (long)(n0 + 1) * (n1 + 1));
+    TRY(launch_pt(c, k_qswap_halo, nh, a));
+    const long ns = (long)n0 * n1 * n2 - (long)(n0 - 1) * (n1 - 1) * (n2 - 1);
+    TRY(launch_pt(c, k_qswap_seam, ns, a));
+    return ASTR_OK;
+}
+
+int do_rhscal(astr_ctx* c)
+{
+    for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_CONV)));
+    TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
+    for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_VISC)));
+    return ASTR_OK;
+}
+
+const double RKC[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+
+int do_rk(astr_ctx* c, int stage, double dt)
+{
+    PtArgs a = pt_args(c);
+    a.stage = stage;
+    a.rka = RKC[stage - 1][0];
+    a.rkb = RKC[stage - 1][1];
+    a.rkc = RKC[stage - 1][2];
+    a.dt = dt;
+    return launch_pt(c, k_rk, n_interior(c), a);
+}
+
+int do_stats(astr_ctx* c, double* out)
+{
+    PtArgs a = pt_args(c);
+    const long n = (long)c->g.n[0] * c->g.n[1] * c->g.n[2];
+    TRY(launch_pt(c, k_stats, n, a, 256));
+    const long nb = (n + 255) / 256;
+    HIPCHK(hipMemcpyAsync(c->host_part.data(), c->part, sizeof(double) * 3 * nb,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double s[3] = {0, 0, 0};
+    for (long b = 0; b < nb; ++b)
+        for (int k = 0; k < 3; ++k) s[k] += c->host_part[3 * b + k];
+    const double n3 = (double)c->g.n[0] * (double)c->g.n[1] * (double)c->g.n[2];
+    // roinf = uinf = 1 and l_0 = 1 in the nondimensional box
+    out[0] = 0.5 * s[0] / n3;
+    out[1] = 0.5 * s[1] / n3;
+    out[2] = s[2] / n3;
+    return ASTR_OK;
+}
+
+// host haloed Fortran array <-> device padded field
+int upload_field(astr_ctx* c, double* dev, const double* host, std::vector<double>& stage)
+{
+    // work queued on the solver's stream may still read the field
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int ni = c->g.n[0] + 1 + 2 * HM, nj = c->g.n[1] + 1 + 2 * HM, nk = c->g.n[2] + 1 + 2 * HM;
+    stage.assign(c->nfield, 0.0);
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            memcpy(&stage[(OI - HM) + (long)j * c->pi + (long)k * c->pi * c->pj],
+                   host + (long)ni * (j + (long)nj * k), sizeof(double) * ni);
+    HIPCHK(hipMemcpy(dev, stage.data(), sizeof(double) * c->nfield, hipMemcpyHostToDevice));
+    return ASTR_OK;
+}
+
+int download_field(astr_ctx* c, const double* dev, double* host, std::vector<double>& stage,
+                   bool interior)
+{
+    stage.resize(c->nfield);
+    HIPCHK(hipMemcpy(stage.data(), dev, sizeof(double) * c->nfield, hipMemcpyDeviceToHost));
+    if (interior) {
+        const int ni = c->g.n[0] + 1, nj = c->g.n[1] + 1, nk = c->g.n[2] + 1;
+        for (int k = 0; k < nk; ++k)
+            for (int j = 0; j < nj; ++j)
+                memcpy(host + (long)ni * (j + (long)nj * k),
+                       &stage[c->g.org + (long)j * c->pi + (long)k * c->pi * c->pj],
+                       sizeof(double) * ni);
+    } else {
+        const int ni = c->g.n[0] + 1 + 2 * HM, nj = c->g.n[1] + 1 + 2 * HM, nk = c->g.n[2] + 1 + 2 * HM;
+        for (int k = 0; k < nk; ++k)
+            for (int j = 0; j < nj; ++j)
+                memcpy(host + (long)ni * (j + (long)nj * k),
+                       &stage[(OI - HM) + (long)j * c->pi + (long)k * c->pi * c->pj],
+                       sizeof(double) * ni);
+    }
+    return ASTR_OK;
+}
+
+int time_mark(astr_ctx* c, int slot)
+{
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[slot], c->stream));
+    return ASTR_OK;
+}
+
+int do_step(astr_ctx* c, double dt, double* stats, bool timed)
+{
+    for (int s = 1; s <= 3; ++s) {
+        if (timed) TRY(time_mark(c, 0));
+        TRY(do_filterq(c));
+        if (timed) TRY(time_mark(c, 1));
+        TRY(do_qswap(c));
+        if (timed) TRY(time_mark(c, 2));
+        for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_CONV)));
+        if (timed) TRY(time_mark(c, 3));
+        TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
+        if (timed) TRY(time_mark(c, 4));
+        for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_VISC)));
+        if (timed) TRY(time_mark(c, 5));
+        if (s == 1 && stats) TRY(do_stats(c, stats));
+        TRY(do_rk(c, s, dt));
+        if (timed) {
+            TRY(time_mark(c, 6));
+            HIPCHK(hipEventSynchronize(c->ev[6]));
+            for (int f = 0; f < 6; ++f) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, c->ev[f], c->ev[f + 1]));
+                c->tms[f] += ms;
+                c->tms[6] += ms;
+            }
+        }
+    }
+    return ASTR_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char* astr_last_error(void) { return g_err.c_str(); }
+
+void astr_default_params(astr_params* p, int32_t n)
+{
+    memset(p, 0, sizeof *p);
+    p->im = p->jm = p->km = n;
+    p->hm = HM;
+    p->numq = 5;
+    p->npdci = p->npdcj = p->npdck = 3;
+    p->variant = 0;
+    p->alfa_filter = 0.49;
+    p->reynolds = 1600.0;
+    p->mach = 0.1;
+    p->prandtl = 0.72;
+    p->gamma = 1.4;
+    p->ref_tem = 273.15;
+    p->sutherland = 110.3;
+    p->beta_halo = 1.11;
+    p->beta_bound = 0.98;
+}
+
+int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
+{
+    if (!p || !out) return fail(ASTR_E_ARG, "null argument");
+    *out = nullptr;
+    if (p->hm != HM) return fail(ASTR_E_UNSUPPORTED, "hm must be 5");
+    if (p->numq != 5) return fail(ASTR_E_UNSUPPORTED, "numq must be 5");
+    if (p->variant != 0) return fail(ASTR_E_UNSUPPORTED, "only variant 0 is implemented");
+    const int dims[3] = {p->im, p->jm, p->km};
+    const int nts[3] = {p->npdci, p->npdcj, p->npdck};
+    for (int d = 0; d < 3; ++d) {
+        if (dims[d] < 16) return fail(ASTR_E_UNSUPPORTED, "each axis needs at least 16 cells");
+        if (nts[d] < 1 || nts[d] > 4) return fail(ASTR_E_ARG, "npdc must be 1..4");
+        if ((dims[d] + 7 + 63) / 64 > LMAX || TX * (dims[d] + 12) * 8 > 65536)
+            return fail(ASTR_E_UNSUPPORTED, "axis longer than 500 cells");
+    }
+    auto* c = new astr_ctx();
+    c->p = *p;
+    c->device = device;
+    auto cleanup = [&](int rc) {
+        astr_destroy(c);
+        return rc;
+    };
+    if (set_dev(c) != ASTR_OK) return cleanup(ASTR_E_HIP);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(ASTR_E_HIP, "hipStreamCreate failed"));
+    for (int d = 0; d < 3; ++d) c->g.n[d] = dims[d];
+    c->pi = ((OI + dims[0] + 1 + HM) + 15) / 16 * 16;
+    c->pj = dims[1] + 1 + 2 * HM;
+    c->pk = dims[2] + 1 + 2 * HM;
+    c->nfield = c->pi * c->pj * c->pk;
+    c->g.s[0] = 1;
+    c->g.s[1] = c->pi;
+    c->g.s[2] = c->pi * c->pj;
+    c->g.org = OI + HM * c->g.s[1] + HM * c->g.s[2];
+    const int nfields = 5 + 6 + 5 + 5 + 9 + 3 + 6 + 3 + 1 + 2;
+    if (hipMalloc(&c->base, sizeof(double) * c->nfield * nfields) != hipSuccess)
+        return cleanup(fail(ASTR_E_NOMEM, "device allocation of the fields failed"));
+    double* f = c->base;
+    auto take = [&]() { double* r = f; f += c->nfield; return r; };
+    for (int m = 0; m < 5; ++m) c->q[m] = take();
+    for (int m = 0; m < 6; ++m) c->prim[m] = take();
+    for (int m = 0; m < 5; ++m) c->qsave[m] = take();
+    for (int m = 0; m < 5; ++m) c->qrhs[m] = take();
+    for (int m = 0; m < 9; ++m) c->dvel[m] = take();
+    for (int m = 0; m < 3; ++m) c->dtmp[m] = take();
+    for (int m = 0; m < 6; ++m) c->sig[m] = take();
+    for (int m = 0; m < 3; ++m) c->qfl[m] = take();
+    c->jac = take();
+    c->scratch = take();
+    c->scratch2 = take();
+    if (hipMemset(c->base, 0, sizeof(double) * c->nfield * nfields) != hipSuccess)
+        return cleanup(fail(ASTR_E_HIP, "hipMemset failed"));
+
+    // constants of refcal, src/solver.F90:105-122
+    const double g = p->gamma, ma = p->mach;
+    c->const1 = 1.0 / (g * (g - 1.0) * (ma * ma));
+    c->const2 = g * (ma * ma);
+    c->const3 = (g - 1.0) / 3.0 * p->prandtl * (ma * ma);
+    c->const4 = (g - 1.0) * (ma * ma) * p->reynolds * p->prandtl;
+    c->const5 = (g - 1.0) * (ma * ma);
+    c->const6 = 1.0 / (g - 1.0);
+    c->const7 = c->const4;
+    c->tempconst = p->sutherland / p->ref_tem;
+    c->tempconst1 = 1.0 + c->tempconst;
+
+    // line systems
+    std::vector<double> tabs;
+    std::vector<long> offs;
+    for (int d = 0; d < 3; ++d) {
+        c->L[d] = (dims[d] + 7 + 63) / 64;
+        for (int op = 0; op < 2; ++op) {
+            HostTab h = build_tab(op, nts[d], dims[d], p->alfa_filter, c->L[d]);
+            offs.push_back((long)tabs.size());
+            tabs.insert(tabs.end(), h.t.begin(), h.t.end());
+            LineTab& t = c->tab[d][op];
+            t.R = h.R;
+            t.first = h.first;
+            t.last = h.last;
+            t.ntype = nts[d];
+            t.dim = dims[d];
+            t.op = op;
+            t.L = c->L[d];
+        }
+    }
+    if (hipMalloc(&c->tabs, sizeof(double) * tabs.size()) != hipSuccess)
+        return cleanup(fail(ASTR_E_NOMEM, "table allocation failed"));
+    if (hipMemcpy(c->tabs, tabs.data(), sizeof(double) * tabs.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(ASTR_E_HIP, "table copy failed"));
+    for (int d = 0, s = 0; d < 3; ++d)
+        for (int op = 0; op < 2; ++op, ++s) c->tab[d][op].t = c->tabs + offs[s];
+
+    FiltCoef fc = build_coef(p->alfa_filter, p->beta_halo, p->beta_bound);
+    if (hipMalloc(&c->fc, sizeof(FiltCoef)) != hipSuccess ||
+        hipMemcpy(c->fc, &fc, sizeof fc, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(ASTR_E_HIP, "coefficient upload failed"));
+
+    c->xi_stride = 64 * LMAX + 2 * HM + 1;
+    if (hipMalloc(&c->xi, sizeof(double) * 3 * c->xi_stride) != hipSuccess)
+        return cleanup(fail(ASTR_E_NOMEM, "metric allocation failed"));
+    (void)hipMemset(c->xi, 0, sizeof(double) * 3 * c->xi_stride);
+
+    c->npart = ((long)dims[0] * dims[1] * dims[2] + 255) / 256;
+    c->host_part.resize(3 * c->npart);
+    if (hipMalloc(&c->part, sizeof(double) * 3 * c->npart) != hipSuccess)
+        return cleanup(fail(ASTR_E_NOMEM, "partials allocation failed"));
+    for (int e = 0; e < 16; ++e)
+        if (hipEventCreate(&c->ev[e]) != hipSuccess) return cleanup(fail(ASTR_E_HIP, "event"));
+    *out = c;
+    return ASTR_OK;
+}
+
+int astr_destroy(astr_ctx* c)
+{
+    if (!c) return ASTR_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->base);
+    (void)hipFree(c->tabs);
+    (void)hipFree(c->fc);
+    (void)hipFree(c->xi);
+    (void)hipFree(c->part);
+    for (int e = 0; e < 16; ++e)
+        if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ASTR_OK;
+}
+
+int astr_upload_geom(astr_ctx* c, const double* jacob, const double* dxi)
+{
+    if (!c || !jacob || !dxi) return fail(ASTR_E_ARG, "null argument");
+    TRY(set_dev(c));
+    const int n[3] = {c->g.n[0], c->g.n[1], c->g.n[2]};
+    const long ni = n[0] + 1 + 2 * HM, nj = n[1] + 1 + 2 * HM, nk = n[2] + 1 + 2 * HM;
+    const long nt = ni * nj * nk;
+    // rectilinear check over the whole haloed array
+    std::vector<double> xi(3 * c->xi_stride, 0.0);
+    for (int d = 0; d < 3; ++d)
+        for (int e = 0; e < 3; ++e) {
+            const double* m = dxi + (long)(d + 3 * e) * nt;  // dxi(:,:,:,d+1,e+1)
+            for (long k = 0; k < nk; ++k)
+                for (long j = 0; j < nj; ++j)
+                    for (long i = 0; i < ni; ++i) {
+                        const double v = m[i + ni * (j + nj * k)];
+                        if (d != e) {
+                            if (v != 0.0)
+                                return fail(ASTR_E_UNSUPPORTED, "curvilinear metrics (off-diagonal dxi)");
+                            continue;
+                        }
+                        const long idx = d == 0 ? i : d == 1 ? j : k;
+                        double& slot = xi[d * c->xi_stride + idx];
+                        const bool first = (d == 0 ? (j == 0 && k == 0)
+                                                   : d == 1 ? (i == 0 && k == 0) : (i == 0 && j == 0));
+                        if (first) slot = v;
+                        else if (slot != v)
+                            return fail(ASTR_E_UNSUPPORTED, "dxi(d,d) varies across the other axes");
+                    }
+        }
+    HIPCHK(hipMemcpy(c->xi, xi.data(), sizeof(double) * xi.size(), hipMemcpyHostToDevice));
+    std::vector<double> stage;
+    TRY(upload_field(c, c->jac, jacob, stage));
+    c->have_geom = true;
+    return ASTR_OK;
+}
+
+int astr_upload_state(astr_ctx* c, const double* q, const double* rho, const double* vel,
+                      const double* prs, const double* tmp)
+{
+    if (!c || !q || !rho || !vel || !prs || !tmp) return fail(ASTR_E_ARG, "null argument");
+    TRY(set_dev(c));
+    const long nt = (long)(c->g.n[0] + 1 + 2 * HM) * (c->g.n[1] + 1 + 2 * HM) * (c->g.n[2] + 1 + 2 * HM);
+    std::vector<double> stage;
+    for (int m = 0; m < 5; ++m) TRY(upload_field(c, c->q[m], q + m * nt, stage));
+    TRY(upload_field(c, c->prim[0], rho, stage));
+    for (int m = 0; m < 3; ++m) TRY(upload_field(c, c->prim[1 + m], vel + m * nt, stage));
+    TRY(upload_field(c, c->prim[4], prs, stage));
+    TRY(upload_field(c, c->prim[5], tmp, stage));
+    c->have_state = true;
+    return ASTR_OK;
+}
+
+int astr_download_state(astr_ctx* c, double* q, double* rho, double* vel, double* prs, double* tmp)
+{
+    if (!c || !q || !rho || !vel || !prs || !tmp) return fail(ASTR_E_ARG, "null argument");
+    TRY(set_dev(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const long nt = (long)(c->g.n[0] + 1 + 2 * HM) * (c->g.n[1] + 1 + 2 * HM) * (c->g.n[2] + 1 + 2 * HM);
+    std::vector<double> stage;
+    for (int m = 0; m < 5; ++m) TRY(download_field(c, c->q[m], q + m * nt, stage, false));
+    TRY(download_field(c, c->prim[0], rho, stage, false));
+    for (int m = 0; m < 3; ++m) TRY(download_field(c, c->prim[1 + m], vel + m * nt, stage, false));
+    TRY(download_field(c, c->prim[4], prs, stage, false));
+    TRY(download_field(c, c->prim[5], tmp, stage, false));
+    return ASTR_OK;
+}
+
+int astr_download_rhs(astr_ctx* c, double* qrhs, double* dvel, double* dtmp)
+{
+    if (!c) return fail(ASTR_E_ARG, "null argument");
+    TRY(set_dev(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const long ni = n_interior(c);
+    std::vector<double> stage;
+    if (qrhs)
+        for (int m = 0; m < 5; ++m) TRY(download_field(c, c->qrhs[m], qrhs + m * ni, stage, true));
+    if (dvel)
+        for (int m = 0; m < 3; ++m)
+            for (int d = 0; d < 3; ++d)
+                // dvel(:,:,:,m,d) is column (m-1) + 3*(d-1) of the Fortran array
+                TRY(download_field(c, c->dvel[m * 3 + d], dvel + (m + 3 * d) * ni, stage, true));
+    if (dtmp)
+        for (int d = 0; d < 3; ++d) TRY(download_field(c, c->dtmp[d], dtmp + d * ni, stage, true));
+    return ASTR_OK;
+}
+
+static int ready(astr_ctx* c)
+{
+    if (!c) return fail(ASTR_E_ARG, "null context");
+    if (!c->have_geom || !c->have_state) return fail(ASTR_E_STATE, "upload geometry and state first");
+    for (int d = 0; d < 3; ++d) {
+        const int nt = d == 0 ? c->p.npdci : d == 1 ? c->p.npdcj : c->p.npdck;
+        if (nt != 3) return fail(ASTR_E_UNSUPPORTED, "the solver steps periodic boxes only");
+    }
+    return set_dev(c);
+}
+
+int astr_filterq(astr_ctx* c)
+{
+    TRY(ready(c));
+    return do_filterq(c);
+}
+
+int astr_qswap(astr_ctx* c)
+{
+    TRY(ready(c));
+    return do_qswap(c);
+}
+
+int astr_rhscal(astr_ctx* c)
+{
+    TRY(ready(c));
+    return do_rhscal(c);
+}
+
+int astr_rk3_update(astr_ctx* c, int32_t stage, double dt)
+{
+    TRY(ready(c));
+    if (stage < 1 || stage > 3) return fail(ASTR_E_ARG, "stage must be 1..3");
+    return do_rk(c, stage, dt);
+}
+
+int astr_stats(astr_ctx* c, double* out3)
+{
+    TRY(ready(c));
+    if (!out3) return fail(ASTR_E_ARG, "null argument");
+    return do_stats(c, out3);
+}
+
+int astr_rk3_step(astr_ctx* c, double dt, double* stats)
+{
+    TRY(ready(c));
+    TRY(do_step(c, dt, stats, c->timing));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ASTR_OK;
+}
+
+int astr_rk3_steps(astr_ctx* c, int32_t nsteps, double dt, double* stats)
+{
+    TRY(ready(c));
+    if (nsteps < 0) return fail(ASTR_E_ARG, "nsteps < 0");
+    for (int f = 0; f < 7; ++f) c->tms[f] = 0.0;
+    for (int s = 0; s < nsteps; ++s) TRY(do_step(c, dt, stats ? stats + 3 * s : nullptr, c->timing));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ASTR_OK;
+}
+
+int astr_set_timing(astr_ctx* c, int32_t on)
+{
+    if (!c) return fail(ASTR_E_ARG, "null context");
+    c->timing = on != 0;
+    return ASTR_OK;
+}
+
+int astr_get_timing(astr_ctx* c, double* ms7)
+{
+    if (!c || !ms7) return fail(ASTR_E_ARG, "null argument");
+    for (int f = 0; f < 7; ++f) ms7[f] = c->tms[f];
+    return ASTR_OK;
+}
+
+static int field_op(astr_ctx* c, int dir, const double* f, double* df, int pass)
+{
+    if (!c || !f || !df) return fail(ASTR_E_ARG, "null argument");
+    if (dir < 1 || dir > 3) return fail(ASTR_E_ARG, "dir must be 1..3");
+    TRY(set_dev(c));
+    std::vector<double> stage;
+    TRY(upload_field(c, c->scratch, f, stage));
+    SweepArgs a = sweep_args(c, dir - 1, pass);
+    a.fin = c->scratch;
+    a.fout = c->scratch2;
+    TRY(launch_sweep(c, a));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    TRY(download_field(c, c->scratch2, df, stage, true));
+    return ASTR_OK;
+}
+
+int astr_deriv_field(astr_ctx* c, int32_t dir, const double* f, double* df)
+{
+    return field_op(c, dir, f, df, PASS_DERIV_FIELD);
+}
+
+int astr_filter_field(astr_ctx* c, int32_t dir, const double* f, double* ff)
+{
+    return field_op(c, dir, f, ff, PASS_FILTER_FIELD);
+}
+
+}  // extern "C"

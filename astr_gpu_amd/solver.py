@@ -1,0 +1,210 @@
+"""Host side of the HIP solver: the reference's solver interface over the C ABI.
+
+Method names follow the reference routines they replace, so a parity test
+reads like the reference's own call sequence (src/mainloop.F90:392-476):
+
+    solver.filterq(); solver.qswap(); solver.rhscal(); solver.rk3_update(s, dt)
+
+Host arrays use the reference layout: Fortran order, i fastest, halo hm=5 on
+every axis, variable index last, e.g. q has shape (im+11, jm+11, km+11, 5).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import AstrParams, check, dptr
+
+HM = 5
+
+
+def default_params(n: int | tuple[int, int, int], **over) -> AstrParams:
+    """Taylor-Green deck values (examples/Taylor_Green_Vortex/datin/input.tgv)."""
+    lib = _lib.load()
+    p = AstrParams()
+    if isinstance(n, int):
+        n = (n, n, n)
+    lib.astr_default_params(ctypes.byref(p), int(n[0]))
+    p.im, p.jm, p.km = (int(v) for v in n)
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+def haloed_shape(p: AstrParams, extra: tuple = ()) -> tuple:
+    return (p.im + 1 + 2 * HM, p.jm + 1 + 2 * HM, p.km + 1 + 2 * HM) + tuple(extra)
+
+
+class AstrSolver:
+    """One periodic box resident on one GPU (one astr_ctx)."""
+
+    def __init__(self, params: AstrParams, device: int = 0):
+        self.lib = _lib.load()
+        self.p = params
+        ctx = ctypes.c_void_p()
+        check(self.lib.astr_create(ctypes.byref(params), int(device), ctypes.byref(ctx)),
+              "astr_create")
+        self.ctx = ctx
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.astr_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- data
+    def _f(self, a, shape):
+        a = np.asarray(a, dtype=np.float64)
+        if a.shape != shape:
+            raise ValueError(f"expected shape {shape}, got {a.shape}")
+        return np.asfortranarray(a)
+
+    def upload_geom(self, jacob, dxi) -> None:
+        jacob = self._f(jacob, haloed_shape(self.p))
+        dxi = self._f(dxi, haloed_shape(self.p, (3, 3)))
+        check(self.lib.astr_upload_geom(self.ctx, dptr(jacob), dptr(dxi)), "astr_upload_geom")
+
+    def upload_state(self, q, rho, vel, prs, tmp) -> None:
+        s = haloed_shape(self.p)
+        arrs = [self._f(q, s + (5,)), self._f(rho, s), self._f(vel, s + (3,)),
+                self._f(prs, s), self._f(tmp, s)]
+        check(self.lib.astr_upload_state(self.ctx, *[dptr(a) for a in arrs]),
+              "astr_upload_state")
+
+    def download_state(self) -> dict:
+        s = haloed_shape(self.p)
+        out = {"q": np.zeros(s + (5,), order="F"), "rho": np.zeros(s, order="F"),
+               "vel": np.zeros(s + (3,), order="F"), "prs": np.zeros(s, order="F"),
+               "tmp": np.zeros(s, order="F")}
+        check(self.lib.astr_download_state(self.ctx, dptr(out["q"]), dptr(out["rho"]),
+                                           dptr(out["vel"]), dptr(out["prs"]),
+                                           dptr(out["tmp"])), "astr_download_state")
+        return out
+
+    def download_rhs(self) -> dict:
+        s = (self.p.im + 1, self.p.jm + 1, self.p.km + 1)
+        out = {"qrhs": np.zeros(s + (5,), order="F"), "dvel": np.zeros(s + (3, 3), order="F"),
+               "dtmp": np.zeros(s + (3,), order="F")}
+        check(self.lib.astr_download_rhs(self.ctx, dptr(out["qrhs"]), dptr(out["dvel"]),
+                                         dptr(out["dtmp"])), "astr_download_rhs")
+        return out
+
+    # ------------------------------------------------------- stage pieces
+    def filterq(self) -> None:
+        check(self.lib.astr_filterq(self.ctx), "astr_filterq")
+
+    def qswap(self) -> None:
+        check(self.lib.astr_qswap(self.ctx), "astr_qswap")
+
+    def rhscal(self) -> None:
+        check(self.lib.astr_rhscal(self.ctx), "astr_rhscal")
+
+    def rk3_update(self, stage: int, dt: float) -> None:
+        check(self.lib.astr_rk3_update(self.ctx, int(stage), float(dt)), "astr_rk3_update")
+
+    def statcal(self) -> tuple[float, float, float]:
+        out = np.zeros(3)
+        check(self.lib.astr_stats(self.ctx, dptr(out)), "astr_stats")
+        return float(out[0]), float(out[1]), float(out[2])
+
+    # ---------------------------------------------------------- steps
+    def time_integration_rk(self, dt: float, stats: bool = False):
+        out = np.zeros(3) if stats else None
+        check(self.lib.astr_rk3_step(self.ctx, float(dt), dptr(out)), "astr_rk3_step")
+        return None if out is None else tuple(float(v) for v in out)
+
+    def steploop(self, nsteps: int, dt: float, stats: bool = False):
+        out = np.zeros((nsteps, 3)) if stats else None
+        check(self.lib.astr_rk3_steps(self.ctx, int(nsteps), float(dt), dptr(out)),
+              "astr_rk3_steps")
+        return out
+
+    def set_timing(self, on: bool) -> None:
+        check(self.lib.astr_set_timing(self.ctx, 1 if on else 0), "astr_set_timing")
+
+    def timing(self) -> np.ndarray:
+        out = np.zeros(7)
+        check(self.lib.astr_get_timing(self.ctx, dptr(out)), "astr_get_timing")
+        return out
+
+    # ------------------------------------------------- field operators
+    def central(self, direction: int, f) -> np.ndarray:
+        """fds%central along `direction` (1..3) for every line of field f."""
+        f = self._f(f, haloed_shape(self.p))
+        df = np.zeros((self.p.im + 1, self.p.jm + 1, self.p.km + 1), order="F")
+        check(self.lib.astr_deriv_field(self.ctx, int(direction), dptr(f), dptr(df)),
+              "astr_deriv_field")
+        return df
+
+    def compact_filter(self, direction: int, f) -> np.ndarray:
+        """compact_filter along `direction` (1..3) for every line of field f."""
+        f = self._f(f, haloed_shape(self.p))
+        ff = np.zeros((self.p.im + 1, self.p.jm + 1, self.p.km + 1), order="F")
+        check(self.lib.astr_filter_field(self.ctx, int(direction), dptr(f), dptr(ff)),
+              "astr_filter_field")
+        return ff
+
+
+# -------------------------------------------------------------- set-up data
+
+def box_metrics(p: AstrParams, lengths=(2 * math.pi,) * 3):
+    """Metrics of the uniform box of gridcube (src/gridgeneration.F90:233-267):
+    dxi = diag(1/dx, 1/dy, 1/dz), jacob = dx*dy*dz, halos included (the
+    single-rank grid exchange extends the grid linearly)."""
+    dx = [lengths[0] / p.im, lengths[1] / p.jm, lengths[2] / p.km]
+    s = haloed_shape(p)
+    jacob = np.full(s, dx[0] * dx[1] * dx[2], order="F")
+    dxi = np.zeros(s + (3, 3), order="F")
+    for d in range(3):
+        dxi[..., d, d] = 1.0 / dx[d]
+    return jacob, dxi
+
+
+def tgv_state(p: AstrParams, lengths=(2 * math.pi,) * 3):
+    """Taylor-Green initial field of tgvini (src/initialisation.F90:1079-1149)
+    with q from updateq/fvar2q (temperature form, src/fludyna.F90:449-533).
+    Interior nodes only; halos are zero (qswap fills them)."""
+    gamma, mach = p.gamma, p.mach
+    const1 = 1.0 / (gamma * (gamma - 1.0) * (mach * mach))
+    const2 = gamma * (mach * mach)
+    pinf = 1.0 / const2
+    s = haloed_shape(p)
+    x = (lengths[0] / p.im) * np.arange(p.im + 1)
+    y = (lengths[1] / p.jm) * np.arange(p.jm + 1)
+    z = (lengths[2] / p.km) * np.arange(p.km + 1)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    rho = np.ones_like(X)
+    u = np.sin(X) * np.cos(Y) * np.cos(Z)
+    v = -(np.cos(X) * np.sin(Y) * np.cos(Z))
+    w = np.zeros_like(X)
+    prs = pinf + rho / 16.0 * 1.0 * (np.cos(2.0 * X) + np.cos(2.0 * Y)) * (np.cos(2.0 * Z) + 2.0)
+    tmp = prs / rho * const2
+    out = {"q": np.zeros(s + (5,), order="F"), "rho": np.zeros(s, order="F"),
+           "vel": np.zeros(s + (3,), order="F"), "prs": np.zeros(s, order="F"),
+           "tmp": np.zeros(s, order="F")}
+    I = (slice(HM, HM + p.im + 1), slice(HM, HM + p.jm + 1), slice(HM, HM + p.km + 1))
+    out["rho"][I] = rho
+    out["vel"][I + (0,)] = u
+    out["vel"][I + (1,)] = v
+    out["vel"][I + (2,)] = w
+    out["prs"][I] = prs
+    out["tmp"][I] = tmp
+    out["q"][I + (0,)] = rho
+    out["q"][I + (1,)] = rho * u
+    out["q"][I + (2,)] = rho * v
+    out["q"][I + (3,)] = rho * w
+    out["q"][I + (4,)] = rho * (tmp * const1 + 0.5 * (u * u + v * v + w * w))
+    return out
+
+
+def interior(a, p: AstrParams):
+    """Interior (0:im,0:jm,0:km) view of a haloed array."""
+    return a[HM:HM + p.im + 1, HM:HM + p.jm + 1, HM:HM + p.km + 1]

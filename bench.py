@@ -1,0 +1,143 @@
+"""Throughput of the HIP solver on the Taylor-Green box (FP64, RK3).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+
+Workload: 256^3 periodic Taylor-Green box (Re 1600, Ma 0.1, 643c compact
+derivatives, alpha 0.49 compact filter every stage, RK3, dt 0.004), the
+whole domain resident in one GPU's HBM.  A step is one full RK3 step (three
+stages of filter, qswap, right-hand side, update).  The metric counts unique
+periodic grid points (n^3) advanced one RK3 step per second.
+
+Prints one JSON line.  roofline: the dominant kernel family, its algorithmic
+HBM bytes per launch over its average launch time (HIP events on the
+solver's stream, over the timed region).  cpu_baseline: the checker's C
+restatement (oracle/, OpenMP over lines) on the host cores, one RK3 step at
+the same size.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+ALGO_BYTES_PER_PT_STEP = 4224.0  # 176 doubles per point per stage, 3 stages
+# algorithmic doubles per interior node per launch of each kernel family
+# (fields each launch must read and write once)
+FAMILY_DOUBLES = {
+    "filter": 10.0,        # q(1:5) in and out, per direction
+    "qswap": None,         # boundary planes only
+    "conv": (20.0 + 25.0 + 25.0) / 3.0,  # x: 11 in 9 out; y,z: +qrhs in
+    "sigma": 25.0,         # dvel 9, dtmp 3, T, u, v, w in; sigma 6, qflux 3 out
+    "visc": 13.0,          # sigma 3, qflux 1, J, qrhs 4 in; qrhs 4 out
+    "rk": 27.0,            # q, qrhs, qsave, J in; q, rho,u,v,w,p,T out
+}
+FAMILY_LAUNCHES = {"filter": 3, "qswap": 2, "conv": 3, "sigma": 1, "visc": 3, "rk": 1}
+FAMILY_NAMES = ["filter", "qswap", "conv", "sigma", "visc", "rk"]
+
+
+def cpu_baseline(n: int, dt: float):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    b = pyoracle.OracleBox(n, variant=0)
+    b.init_tgv()
+    t0 = time.perf_counter()
+    b.step(dt)
+    el = time.perf_counter() - t0
+    thr = pyoracle.oracle_lib().orc_omp_threads()
+    b.close()
+    return {"value": n ** 3 / el, "unit": "grid-point-RK3-steps/s", "cores": int(thr),
+            "kind": "port",
+            "sample": f"one RK3 step of the {n}^3 Taylor-Green box by the C restatement "
+                      f"(OpenMP, {thr} threads), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the solver keeps the whole box on one device "
+              "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import astr_gpu_amd as ag
+
+    n, dt = args.n, 0.004
+    p = ag.default_params(n)
+    s = ag.AstrSolver(p, device=0)
+    s.upload_geom(*ag.box_metrics(p))
+    st = ag.tgv_state(p)
+    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+    del st
+
+    s.steploop(args.warmup, dt)
+    # timed region: K steps, kernel families timed with events on the stream
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.steploop(args.steps, dt)   # synchronizes before returning
+    el = time.perf_counter() - t0
+    fam = s.timing()
+    s.set_timing(False)
+    # untimed-by-events pass for the wall clock without event overhead
+    t0 = time.perf_counter()
+    s.steploop(args.steps, dt)
+    el_noev = time.perf_counter() - t0
+    el = min(el, el_noev)
+    s.close()
+
+    pts = float(n) ** 3
+    value = pts * args.steps / el
+    ms_per_step = 1e3 * el / args.steps
+    nodes = float(n + 1) ** 3
+    stages = 3 * args.steps
+    fam_ms = {nm: float(fam[i]) for i, nm in enumerate(FAMILY_NAMES)}
+    dom = max((k for k in fam_ms if FAMILY_DOUBLES[k]), key=lambda k: fam_ms[k])
+    launches = FAMILY_LAUNCHES[dom] * stages
+    avg_s = fam_ms[dom] / 1e3 / launches
+    algo = FAMILY_DOUBLES[dom] * 8.0 * nodes
+    achieved = algo / avg_s / 1e9
+    out = {
+        "metric": "FP64 grid-point-RK3-steps/s at 256^3 (plain)",
+        "value": value,
+        "unit": "grid-point-RK3-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Taylor-Green initial field)",
+        "config": {"workload": f"Taylor-Green vortex {n}^3 periodic, Re 1600, Ma 0.1, 643c, "
+                               f"filter alpha 0.49, RK3 dt 0.004", "n": n,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_sweep {dom} family, {FAMILY_LAUNCHES[dom]} launches/stage",
+                     "algo_bytes_per_launch": algo, "avg_launch_ms": avg_s * 1e3},
+        "step_hbm_frac": value * ALGO_BYTES_PER_PT_STEP / (HBM_PEAK_GBS * 1e9),
+        "family_ms_per_step": {k: v / args.steps for k, v in fam_ms.items()},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, dt)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

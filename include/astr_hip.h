@@ -1,0 +1,137 @@
+/*
+ * astr_hip.h - C ABI of the MI355X (gfx950) FP64 solver for ASTR's
+ * per-timestep right-hand-side path.
+ *
+ * The Fortran host keeps its module state (src/commarray.F90) and calls
+ * these entry points through iso_c_binding (astr_gpu_amd/fortran/
+ * astr_hip_mod.f90); Python binds the same symbols with ctypes.  Host
+ * arrays are passed in the reference's own layout: Fortran column-major,
+ * i fastest, lower bound -hm (hm = 5) on every haloed axis, variable index
+ * last, e.g. q(-hm:im+hm,-hm:jm+hm,-hm:km+hm,1:5).  Arrays without halo
+ * (qrhs, dvel, dtmp) are dimensioned (0:im,0:jm,0:km,...).
+ *
+ * Every entry point returns 0 on success or an ASTR_E_* code; the Fortran
+ * shim turns a nonzero code into mpistop, which is what the reference does
+ * on any error (e.g. src/mainloop.F90:381).
+ *
+ * Reference citations are relative to the ASTR source tree.
+ */
+#ifndef ASTR_HIP_H
+#define ASTR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ASTR_OK = 0,
+    ASTR_E_ARG = 1,         /* bad argument (null pointer, bad direction) */
+    ASTR_E_UNSUPPORTED = 2, /* configuration outside what the solver does */
+    ASTR_E_HIP = 3,         /* a HIP runtime call failed */
+    ASTR_E_NOMEM = 4,       /* device allocation failed */
+    ASTR_E_STATE = 5        /* call out of order (e.g. step before upload) */
+};
+
+/* Solver configuration: the values refcal/readinput/solvrinit put in
+ * module commvar (src/solver.F90:28-172, src/comsolver.F90:47-146).
+ * Layout is fixed so that a Fortran bind(c) derived type can mirror it. */
+typedef struct astr_params {
+    int32_t im, jm, km;          /* nodes 0..im etc. (commvar im,jm,km)   */
+    int32_t hm;                  /* halo width of host arrays, must be 5  */
+    int32_t numq;                /* must be 5 (no species, no k-omega)    */
+    int32_t npdci, npdcj, npdck; /* closure type per axis; the solver
+                                    steps periodic boxes (3 on all axes)  */
+    int32_t variant;             /* 0: src/mainloop.F90 stage order       */
+    int32_t reserved;
+    double alfa_filter;          /* 0.49 in every deck                    */
+    double reynolds, mach, prandtl, gamma;
+    double ref_tem;              /* tempconst = sutherland / ref_tem      */
+    double sutherland;           /* 110.3 (src/solver.F90:121)            */
+    double beta_halo;            /* 1.11 (src/comsolver.F90:121)          */
+    double beta_bound;           /* 0.98 (src/comsolver.F90:121)          */
+} astr_params;
+
+typedef struct astr_ctx astr_ctx;
+
+/* Fill *p with the Taylor-Green deck values (examples/Taylor_Green_Vortex/
+ * datin/input.tgv, Re 1600, Ma 0.1) for an n x n x n periodic box. */
+void astr_default_params(astr_params *p, int32_t n);
+
+/* Replaces solvrinit (src/comsolver.F90:47-146) and allocommarray: builds
+ * the pre-factored line systems (fd_scheme_initiate, src/derivative.F90:
+ * 63-158; compact_filter_initiate + filter_coefficient_cal, src/filter.F90:
+ * 31-100,299-432) and allocates the device fields on HIP device `device`. */
+int astr_create(const astr_params *p, int32_t device, astr_ctx **out);
+int astr_destroy(astr_ctx *ctx);
+
+/* Metrics from gridgeom (src/geom.F90:99-917): jacob(-hm:im+hm,...) and
+ * dxi(-hm:im+hm,...,1:3,1:3).  The solver keeps rectilinear metrics only:
+ * off-diagonal dxi must be zero and dxi(d,d) must depend on the d-th index
+ * alone; otherwise ASTR_E_UNSUPPORTED. */
+int astr_upload_geom(astr_ctx *ctx, const double *jacob, const double *dxi);
+
+/* Module state of src/commarray.F90 in and out: q(.,.,.,1:5), rho, vel
+ * (.,.,.,1:3), prs, tmp, all haloed.  Download writes the full haloed
+ * arrays as the device holds them. */
+int astr_upload_state(astr_ctx *ctx, const double *q, const double *rho,
+                      const double *vel, const double *prs, const double *tmp);
+int astr_download_state(astr_ctx *ctx, double *q, double *rho, double *vel,
+                        double *prs, double *tmp);
+
+/* --- stage pieces: one per call site of time_integration_rk ---------- */
+
+/* call filterq, src/mainloop.F90:395 -> src/comsolver.F90:514-632 */
+int astr_filterq(astr_ctx *ctx);
+/* call qswap, src/mainloop.F90:415 -> src/parallel.F90:4849-5319 */
+int astr_qswap(astr_ctx *ctx);
+/* qrhs=0 (src/mainloop.F90:404) then call rhscal (src/solver.F90:184-283:
+ * gradcal, convrsdcal6, qrhs=-qrhs, diffrsdcal6) */
+int astr_rhscal(astr_ctx *ctx);
+/* RK3 combine of stage s (1..3) and updatefvar, src/mainloop.F90:423-476;
+ * stage 1 also saves qsave = q*jacob. */
+int astr_rk3_update(astr_ctx *ctx, int32_t stage, double dt);
+/* kenergycal, enstophycal, diss_rate_cal (src/statistic.F90:870-1042) of
+ * the current device state -> out[0..2]. */
+int astr_stats(astr_ctx *ctx, double *out3);
+
+/* --- whole steps: time_integration_rk, src/mainloop.F90:298-595 ------- */
+
+/* One RK3 step.  If stats is non-null it receives (ke, enstrophy,
+ * dissipation) evaluated at stage 1 after rhscal, where rkfirst
+ * (src/mainloop.F90:608-688) evaluates them. */
+int astr_rk3_step(astr_ctx *ctx, double dt, double *stats);
+/* nsteps RK3 steps back to back; stats (nullable) receives 3 per step. */
+int astr_rk3_steps(astr_ctx *ctx, int32_t nsteps, double dt, double *stats);
+
+/* --- results of the last rhscal, for parity checks --------------------- */
+/* qrhs(0:im,0:jm,0:km,1:5), dvel(0:im,0:jm,0:km,1:3,1:3),
+ * dtmp(0:im,0:jm,0:km,1:3) */
+int astr_download_rhs(astr_ctx *ctx, double *qrhs, double *dvel, double *dtmp);
+
+/* --- field-level line operators --------------------------------------- */
+
+/* fds%central along axis dir (1,2,3) for every line of a haloed field:
+ * f(-hm:im+hm,-hm:jm+hm,-hm:km+hm) -> df(0:im,0:jm,0:km), index-space
+ * derivative (src/derivative.F90:16-46,171-306), closure npdc{i,j,k} of
+ * the context.  Halos are used as given. */
+int astr_deriv_field(astr_ctx *ctx, int32_t dir, const double *f, double *df);
+/* compact_filter along axis dir for every line (src/filter.F90:112-285):
+ * f(-hm:...) haloed in, ff(0:im,0:jm,0:km) out.  Halos used as given. */
+int astr_filter_field(astr_ctx *ctx, int32_t dir, const double *f, double *ff);
+
+/* Kernel time of the last astr_rk3_steps call, in milliseconds, per kernel
+ * family: [0] filter, [1] qswap, [2] convective+gradient sweeps,
+ * [3] sigma, [4] viscous sweeps, [5] rk update, [6] total; only filled when
+ * timing was enabled with astr_set_timing(ctx, 1). */
+int astr_set_timing(astr_ctx *ctx, int32_t on);
+int astr_get_timing(astr_ctx *ctx, double *ms7);
+
+/* Human-readable text of the last error on this thread. */
+const char *astr_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,105 @@
+/*
+ * CPU restatement of the ASTR per-timestep right-hand-side path, in FP64.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP solver:
+ * it may be loaded by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg, and by nothing else.  The product path never calls it.
+ *
+ * Every function cites the reference routine it restates (paths relative to
+ * the ASTR source tree).  Arrays that carry halos follow the Fortran layout of
+ * the reference: f(-hm:n+hm) for a line, and a(-hm:im+hm,-hm:jm+hm,-hm:km+hm)
+ * with i fastest for fields, hm = 5 (src/commvar.F90:190).
+ */
+#ifndef ASTR_ORACLE_H
+#define ASTR_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_HM 5
+
+/* ---------------- banded solver (src/commfunc.F90) ---------------- */
+/* tridiagonal_thomas_proprocess, src/commfunc.F90:752-774 */
+void orc_thomas_prep(const double *a, const double *c, int n, double *ac);
+/* tridiagonal_thomas_solver, src/commfunc.F90:790-813; d is modified */
+void orc_thomas_solve(const double *ac, double *d, int n, double *x);
+
+/* ---------------- line operators ---------------- */
+/* fd_scheme_initiate (643c), src/derivative.F90:63-158.
+ * Returns the row count; first/last receive the node range.  a,c,ac have
+ * room for dim+3 rows (ac is 3 per row, row-major [row][3]). */
+int orc_fd_init(int ntype, int dim, int *first, int *last,
+                double *a, double *c, double *ac);
+/* df_compact + compact_fd_rhs, src/derivative.F90:171-306.
+ * f points at node -hm of f(-hm:dim+hm); df receives df(0:dim). */
+void orc_df_compact(int ntype, int dim, const double *f, double *df);
+/* diff6ec, src/derivative.F90:350-413 */
+void orc_diff6ec(int ntype, int dim, const double *f, double *df);
+
+/* filter_coefficient_cal, src/filter.F90:299-432.
+ * Layout of coef (out, 6+45+55 = 106 doubles):
+ *   [0..5]    coef10i(0:5)
+ *   [6..50]   coefb(0:4,0:8)  as coefb[k*9+j]
+ *   [51..105] coefh(0:4,0:10) as coefh[k*11+j]
+ * plus coef6i/coef8i in the trailing 9 (total 115 doubles):
+ *   [106..109] coef6i(0:3), [110..114] coef8i(0:4) */
+#define ORC_NCOEF 115
+void orc_filter_coef(double alfa, double beta_halo, double beta_bound,
+                     double *coef);
+/* compact_filter_initiate, src/filter.F90:31-100 (note absent). */
+int orc_filter_init(int ntype, int dim, double alfa, int *first, int *last,
+                    double *a, double *c, double *ac);
+/* compact_filter + compact_filter_rhs, src/filter.F90:112-285 */
+void orc_compact_filter(int ntype, int dim, double alfa, double beta_halo,
+                        double beta_bound, const double *f, double *ff);
+
+/* ---------------- periodic-box solver ---------------- */
+typedef struct orc_box orc_box;
+
+/* variant 0: main path (src/mainloop.F90 time_integration_rk with rk3,
+ *            src/solver.F90 rhscal with convrsdcal6/diffrsdcal6,
+ *            src/comsolver.F90 filterq, src/parallel.F90 qswap).
+ * variant 1: mini-app (miniapps/tgv_solver_3d/tgvsolver.F90 rk3). */
+orc_box *orc_box_new(int im, int jm, int km, int variant,
+                     double reynolds, double mach, double prandtl,
+                     double gamma, double ref_tem, double alfa_filter,
+                     double lx, double ly, double lz);
+void orc_box_free(orc_box *b);
+/* Sutherland constant numerator (110.3 in src/solver.F90:121, 110.4 in
+ * tgvsolver.F90:133) and the filter's boundary beta (0.98 main, 1.09 mini)
+ * are set by the variant; they can be overridden. */
+void orc_box_set_sutherland(orc_box *b, double s_over_tref_numerator);
+
+/* Set the Taylor-Green initial field: tgvini (src/initialisation.F90:1079-1149)
+ * followed by updateq (src/initialisation.F90:30-162 / src/fludyna.F90:254)
+ * for variant 0, flowfield_init (tgvsolver.F90:176-234) for variant 1. */
+void orc_box_init_tgv(orc_box *b);
+/* Load / store the interior (0:im,0:jm,0:km) of the state arrays.
+ * Each buffer holds (im+1)*(jm+1)*(km+1) values, i fastest.
+ * which: 0..4 q(1..5), 5 rho, 6..8 vel(1..3), 9 prs, 10 tmp.          */
+void orc_box_get(const orc_box *b, int which, double *out);
+void orc_box_set(orc_box *b, int which, const double *in);
+/* The whole haloed array (im+11)*(jm+11)*(km+11) of the same fields. */
+void orc_box_get_haloed(const orc_box *b, int which, double *out);
+/* dvel(m,n) for which = 3*(m-1)+(n-1), dtmp(n) for which = 9+(n-1),
+ * qrhs(m) for which = 12+(m-1). */
+void orc_box_get_deriv(const orc_box *b, int which, double *out);
+
+/* One RK3 step; stats (ke, enstrophy, dissipation) are computed at
+ * stage 1 after the right-hand side, where rkfirst (src/mainloop.F90:608)
+ * or stacal (tgvsolver.F90:893) evaluates them.  stats may be NULL. */
+void orc_box_step(orc_box *b, double dt, double *stats);
+/* The individual stage pieces of the main variant, for operator tests. */
+void orc_box_filterq(orc_box *b);
+void orc_box_qswap(orc_box *b);
+void orc_box_rhscal(orc_box *b);
+void orc_box_updatefvar(orc_box *b);
+void orc_box_stats(orc_box *b, double *stats);
+
+int orc_omp_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

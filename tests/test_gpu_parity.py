@@ -1,0 +1,227 @@
+"""Parity of the HIP solver (through the C ABI) with the checker.
+
+Tolerances (written per test):
+  * single derivative / filter sweeps: max |gpu - ref| <= 1e-12 * max |ref|
+    over the field (the chunked-scan Thomas solve and FMA contraction change
+    only the rounding);
+  * stage pieces (filterq, qswap, rhscal): the same 1e-12 normwise bound;
+  * whole steps: KE and enstrophy within 1e-10 relative, primitive fields
+    within 1e-9 absolute (the bar the reference sets for 100 RK3 steps).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import astr_gpu_amd as ag
+import pyoracle
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+HM = 5
+
+
+def nerr(a, b):
+    """max |a-b| / max |b| (normwise relative error of a field)."""
+    den = float(np.max(np.abs(b)))
+    return float(np.max(np.abs(a - b))) / (den if den > 0 else 1.0)
+
+
+def lines_ref(fn, f, direction):
+    """Apply a 1-D line operator of the checker to every line of a haloed
+    field along `direction` (1..3); f has shape (ni+11, nj+11, nk+11)."""
+    ax = direction - 1
+    g = np.moveaxis(f, ax, 0)
+    other = g.shape[1:]
+    out = np.zeros((g.shape[0] - 2 * HM,) + tuple(o - 2 * HM for o in other))
+    for a in range(HM, other[0] - HM):
+        for b in range(HM, other[1] - HM):
+            out[:, a - HM, b - HM] = fn(np.ascontiguousarray(g[:, a, b]))
+    return np.moveaxis(out, 0, ax)
+
+
+@pytest.mark.parametrize("ntype", [1, 2, 3, 4])
+def test_field_line_operators(ntype):
+    dims = (40, 36, 33)
+    p = ag.default_params(dims, npdci=ntype, npdcj=ntype, npdck=ntype)
+    s = ag.AstrSolver(p)
+    rng = np.random.default_rng(100 + ntype)
+    f = np.asfortranarray(rng.standard_normal(ag.haloed_shape(p)))
+    for d in (1, 2, 3):
+        df = s.central(d, f)
+        ref = lines_ref(lambda ln: pyoracle.df_compact(ntype, ln), f, d)
+        e = nerr(df, ref)
+        print(f"deriv ntype {ntype} dir {d}: {e:.2e}")
+        assert e < 1e-12
+        ff = s.compact_filter(d, f)
+        ref = lines_ref(lambda ln: pyoracle.compact_filter(ntype, ln), f, d)
+        e = nerr(ff, ref)
+        print(f"filter ntype {ntype} dir {d}: {e:.2e}")
+        assert e < 1e-12
+    s.close()
+
+
+def test_field_line_operators_long_lines():
+    # 256-node lines (5 rows per lane) against the golden vectors of the
+    # reference's own modules
+    G = np.load(os.path.join(GOLDEN, "lines_ref.npz"))
+    p = ag.default_params((256, 16, 16))
+    s = ag.AstrSolver(p)
+    f = np.zeros(ag.haloed_shape(p), order="F")
+    f[:, :, :] = G["f_smooth_3_256"][:, None, None]
+    df = s.central(1, f)
+    ff = s.compact_filter(1, f)
+    assert nerr(df[:, 3, 4], G["df_smooth_3_256"]) < 1e-12
+    assert nerr(ff[:, 3, 4], G["ff098_smooth_3_256"]) < 1e-12
+    s.close()
+
+
+def make_pair(dims):
+    ob = pyoracle.OracleBox(dims, variant=0)
+    ob.init_tgv()
+    st = ob.state_haloed()
+    p = ag.default_params(dims)
+    s = ag.AstrSolver(p)
+    s.upload_geom(*ag.box_metrics(p))
+    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+    return ob, s, p
+
+
+PRIMS = [("rho", None), ("u", 0), ("v", 1), ("w", 2), ("prs", None), ("tmp", None)]
+
+
+def compare_state(ob, s, p, tol, halo_faces=False):
+    out = s.download_state()
+    worst = 0.0
+    for m in range(5):
+        e = nerr(ag.interior(out["q"][..., m], p), ob.get(f"q{m + 1}"))
+        worst = max(worst, e)
+        assert e < tol, (f"q{m + 1}", e)
+    for nm, comp in PRIMS:
+        g = out["vel"][..., comp] if comp is not None else out[nm]
+        e = nerr(ag.interior(g, p), ob.get(nm))
+        worst = max(worst, e)
+        assert e < tol, (nm, e)
+    if halo_faces:
+        # the halo cells qswap writes: i-halos over j,k in 0..n etc.
+        n0, n1, n2 = p.im, p.jm, p.km
+        I0, I1, I2 = slice(HM, HM + n0 + 1), slice(HM, HM + n1 + 1), slice(HM, HM + n2 + 1)
+        lo, hi = slice(0, HM), slice(-HM, None)
+        faces = [(lo, I1, I2), (hi, I1, I2), (I0, lo, I2), (I0, hi, I2), (I0, I1, lo),
+                 (I0, I1, hi)]
+        for m in range(5):
+            ref = ob.get_haloed(f"q{m + 1}")
+            for fc in faces:
+                e = nerr(out["q"][..., m][fc], ref[fc])
+                assert e < tol, ("q halo", m, e)
+        for nm, comp in PRIMS:
+            ref = ob.get_haloed(nm)
+            g = out["vel"][..., comp] if comp is not None else out[nm]
+            for fc in faces:
+                e = nerr(g[fc], ref[fc])
+                assert e < tol, (nm + " halo", e)
+    return worst
+
+
+def test_stage_pieces_match_oracle():
+    dims = (40, 36, 32)
+    ob, s, p = make_pair(dims)
+    ob.filterq()
+    s.filterq()
+    print("filterq", compare_state(ob, s, p, 1e-12))
+    ob.qswap()
+    s.qswap()
+    print("qswap", compare_state(ob, s, p, 1e-12, halo_faces=True))
+    ob.rhscal()
+    s.rhscal()
+    r = s.download_rhs()
+    for m in range(5):
+        e = nerr(r["qrhs"][..., m], ob.deriv(12 + m))
+        print(f"qrhs{m + 1}: {e:.2e}")
+        assert e < 1e-12
+    for m in range(3):
+        for d in range(3):
+            e = nerr(r["dvel"][..., m, d], ob.deriv(3 * m + d))
+            assert e < 1e-12, ("dvel", m, d, e)
+    for d in range(3):
+        assert nerr(r["dtmp"][..., d], ob.deriv(9 + d)) < 1e-12
+    st_o = ob.stats()
+    st_g = s.statcal()
+    for a, b in zip(st_g, st_o):
+        assert abs(a - b) <= 1e-12 * abs(b)
+    s.close()
+    ob.close()
+
+
+def test_steps_match_oracle():
+    dims = (32, 32, 32)
+    ob, s, p = make_pair(dims)
+    for k in range(5):
+        so = ob.step(0.004, stats=True)
+        sg = s.time_integration_rk(0.004, stats=True)
+        for a, b in zip(sg, so):
+            assert abs(a - b) <= 1e-10 * abs(b), (k, sg, so)
+    print("5 steps", compare_state(ob, s, p, 1e-10))
+    s.close()
+    ob.close()
+
+
+def test_non_cubic_steps_match_oracle():
+    dims = (48, 40, 36)
+    ob, s, p = make_pair(dims)
+    for k in range(3):
+        ob.step(0.004)
+    s.steploop(3, 0.004)
+    print("non-cubic 3 steps", compare_state(ob, s, p, 1e-10))
+    s.close()
+    ob.close()
+
+
+def test_tgv64_100_steps_against_checker_series():
+    """64^3 Taylor-Green box, 100 RK3 steps of dt 0.004 (the small-size
+    workload): KE and enstrophy within 1e-10 relative at every step,
+    primitive fields within 1e-9 absolute at the probes."""
+    G = np.load(os.path.join(GOLDEN, "tgv64_main.npz"))
+    p = ag.default_params(64)
+    s = ag.AstrSolver(p)
+    s.upload_geom(*ag.box_metrics(p))
+    ob = pyoracle.OracleBox(64, variant=0)
+    ob.init_tgv()
+    st = ob.state_haloed()
+    ob.close()
+    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+    stats = s.steploop(100, 0.004, stats=True)
+    ref = G["stats"]
+    ke_err = np.max(np.abs(stats[:, 0] - ref[:, 0]) / np.abs(ref[:, 0]))
+    en_err = np.max(np.abs(stats[:, 1] - ref[:, 1]) / np.abs(ref[:, 1]))
+    print(f"KE rel err {ke_err:.2e}, enstrophy rel err {en_err:.2e}")
+    assert ke_err < 1e-10 and en_err < 1e-10
+    out = s.download_state()
+    for nm, comp in PRIMS:
+        g = out["vel"][..., comp] if comp is not None else out[nm]
+        g = ag.interior(g, p)[::8, ::8, ::8]
+        e = float(np.max(np.abs(g - G["probe_" + nm])))
+        print(f"{nm} probe abs err {e:.2e}")
+        assert e < 1e-9
+    s.close()
+
+
+def test_periodic_only_for_steps():
+    p = ag.default_params(32, npdci=1)
+    s = ag.AstrSolver(p)
+    s.upload_geom(*ag.box_metrics(p))
+    st = ag.tgv_state(p)
+    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+    with pytest.raises(ag.AstrError, match="UNSUPPORTED"):
+        s.time_integration_rk(0.004)
+    s.close()
+
+
+def test_curvilinear_metrics_rejected():
+    p = ag.default_params(32)
+    s = ag.AstrSolver(p)
+    jac, dxi = ag.box_metrics(p)
+    dxi[10, 10, 10, 0, 1] = 1e-3
+    with pytest.raises(ag.AstrError, match="UNSUPPORTED"):
+        s.upload_geom(jac, dxi)
+    s.close()
