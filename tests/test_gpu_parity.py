@@ -5,6 +5,9 @@ Tolerances (written per test):
     over the field (the chunked-scan Thomas solve and FMA contraction change
     only the rounding);
   * stage pieces (filterq, qswap, rhscal): the same 1e-12 normwise bound;
+    qrhs, a small difference of large pressure fluxes, within 1e-13 of the
+    pressure-flux derivative scale max|p| * max d(xi)/dx, and the gradients
+    within 1e-13 of max|f| * max d(xi)/dx of the differentiated field;
   * whole steps: KE and enstrophy within 1e-10 relative, primitive fields
     within 1e-9 absolute (the bar the reference sets for 100 RK3 steps).
 """
@@ -135,16 +138,27 @@ def test_stage_pieces_match_oracle():
     ob.rhscal()
     s.rhscal()
     r = s.download_rhs()
+    # The residual is a small difference of fluxes of size p * d(xi)/dx
+    # (p ~ 1/(gamma Ma^2) = 71, the Taylor-Green residual is ~1e-3 and its
+    # mass component ~1e-10), so flux rounding sets an absolute floor: the
+    # bound is 1e-13 of the pressure-flux derivative scale.
+    flux_scale = float(np.max(np.abs(ob.get("prs")))) * max(p.im, p.jm, p.km) / (2 * np.pi)
     for m in range(5):
-        e = nerr(r["qrhs"][..., m], ob.deriv(12 + m))
+        e = float(np.max(np.abs(r["qrhs"][..., m] - ob.deriv(12 + m)))) / flux_scale
         print(f"qrhs{m + 1}: {e:.2e}")
-        assert e < 1e-12
+        assert e < 1e-13
+    # gradients against the derivative scale of the differentiated field,
+    # max|f| * max d(xi)/dx (T ~ 1 varies by 1e-3 in this field)
+    xi = max(p.im, p.jm, p.km) / (2 * np.pi)
+    fs = max(float(np.max(np.abs(ob.get(nm)))) for nm in "uvw") * xi
     for m in range(3):
         for d in range(3):
-            e = nerr(r["dvel"][..., m, d], ob.deriv(3 * m + d))
-            assert e < 1e-12, ("dvel", m, d, e)
+            e = float(np.max(np.abs(r["dvel"][..., m, d] - ob.deriv(3 * m + d)))) / fs
+            assert e < 1e-13, ("dvel", m, d, e)
+    fs = float(np.max(np.abs(ob.get("tmp")))) * xi
     for d in range(3):
-        assert nerr(r["dtmp"][..., d], ob.deriv(9 + d)) < 1e-12
+        e = float(np.max(np.abs(r["dtmp"][..., d] - ob.deriv(9 + d)))) / fs
+        assert e < 1e-13, ("dtmp", d, e)
     st_o = ob.stats()
     st_g = s.statcal()
     for a, b in zip(st_g, st_o):
