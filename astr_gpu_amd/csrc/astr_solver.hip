@@ -57,7 +57,7 @@ constexpr int TX = 16;     // lines per tile
 constexpr int WGT = 512;   // threads per sweep workgroup
 constexpr int NWAVE = WGT / 64;
 constexpr int OI = 16;     // element offset of node i=0 in a row
-constexpr int LMAX = 8;    // rows per lane supported (line length <= 64*LMAX)
+constexpr int LMAX = 16;   // rows per lane supported (line length <= 32*LMAX)
 
 struct Geo {
     int n[3];    // cells per axis (nodes 0..n)
@@ -74,10 +74,11 @@ __device__ __forceinline__ long gidx(const Geo& g, int i, int j, int k)
 
 enum Pass : int {
     PASS_FILTER = 0,       // filterq along one axis, q(1:5) in place
-    PASS_CONV = 1,         // gradcal + convrsdcal6 along one axis
-    PASS_VISC = 2,         // viscous fluxes along one axis
-    PASS_DERIV_FIELD = 3,  // fds%central of one field (halos as given)
-    PASS_FILTER_FIELD = 4  // compact_filter of one field (halos as given)
+    PASS_GRAD = 1,         // gradcal along one axis: u,v,w,T
+    PASS_FLUX = 2,         // convrsdcal6 along one axis: the 5 inviscid fluxes
+    PASS_VISC = 3,         // viscous fluxes along one axis
+    PASS_DERIV_FIELD = 4,  // fds%central of one field (halos as given)
+    PASS_FILTER_FIELD = 5  // compact_filter of one field (halos as given)
 };
 
 struct SweepArgs {
@@ -110,166 +111,171 @@ __device__ __forceinline__ int wrap_node(int n, int dim)
     return n < 0 ? n + dim : (n > dim ? n - dim : n);
 }
 
+// Order of the pass's quantities; the flux pass loads the momentum flux of
+// the sweep axis first (it reads u_d, p and J and keeps J*U in registers).
+template <int PASS>
+__device__ __forceinline__ int quantity(int qi, int dir)
+{
+    if constexpr (PASS == PASS_FLUX) return qi == 0 ? dir + 1 : (qi <= dir + 1 ? qi - 1 : qi);
+    return qi;
+}
+
 template <int L, int PASS>
-__global__ __launch_bounds__(WGT) void k_sweep(SweepArgs a)
+__global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
 {
     // One LDS tile: line l, staged node n (node n-HM) at tile[l*rs_in + n].
     // The solution of a line overwrites that line's slots in place, which is
-    // safe because only the wavefront solving a line reads it.
+    // safe because only the half-wavefront solving a line reads it.
     extern __shared__ double tile[];
+    constexpr int OP = (PASS == PASS_FILTER || PASS == PASS_FILTER_FIELD) ? OP_FILTER : OP_DERIV;
+    constexpr int EMAX = L + 1;  // staged nodes per thread: 32 per step, dim+11 <= 32*L+4
+    constexpr int NKEEP = PASS == PASS_FLUX || PASS == PASS_VISC ? 1 : 0;
+    constexpr int EK = NKEEP ? EMAX : 1;
+    constexpr int NQ = PASS == PASS_FILTER ? 5 : PASS == PASS_GRAD ? 4 : PASS == PASS_FLUX ? 5
+                     : PASS == PASS_VISC ? 4 : 1;
 
     const int ta = blockIdx.x % a.ntile_a;
     const int b = blockIdx.x / a.ntile_a;
     const int a0 = ta * TX;
     const int dir = a.dir;
     const int dim = a.g.n[dir];
-    const int nn = dim + 1 + 2 * HM;   // nodes per staged line
-    const int no = dim + 1;            // output nodes per line
+    const int nn = dim + 1 + 2 * HM;   // staged nodes per line
     const long sd = a.g.s[dir];
-
-    // element offset of node 0 of line l of this tile
-    auto line_off = [&](int l) -> long {
-        const long av = a0 + l;
-        if (dir == 0) return a.g.org + av * a.g.s[1] + (long)b * a.g.s[2];
-        if (dir == 1) return a.g.org + av + (long)b * a.g.s[2];
-        return a.g.org + av + (long)b * a.g.s[1];
-    };
     const int nvalid = min(TX, a.na - a0);
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int myline = tid >> 5;  // line solved by this half-wavefront
+    const bool xdir = dir == 0;
 
-    // loader element mapping: x-lines node-fastest, y/z lines line-fastest
-    const int nel_in = TX * nn;
-    constexpr int EMAX = (TX * (64 * L + 2 * HM + 1) + WGT - 1) / WGT;
-    constexpr int NKEEP = PASS == PASS_CONV ? 3 : PASS == PASS_VISC ? 2 : 0;
-    constexpr int EK = NKEEP ? EMAX : 1;
-    constexpr int NQ = PASS == PASS_FILTER ? 5 : PASS == PASS_CONV ? 9 : PASS == PASS_VISC ? 4 : 1;
-
-    // per-element values kept in registers across the quantities of a pass:
-    // conv: U = dxi(d,d)*u_d, p, J;  visc: dxi(d,d), J
-    double k0[EK], k1[EK], k2[EK];
-    if constexpr (NKEEP > 0) {
-#pragma unroll
-        for (int e = 0; e < EMAX; ++e) {
-            k0[e] = 0.0; k1[e] = 0.0; k2[e] = 0.0;
-            const int el = tid + e * WGT;
-            if (el < nel_in) {
-                int l, n;
-                if (dir == 0) { l = el / nn; n = el - l * nn; }
-                else { n = el / TX; l = el - n * TX; }
-                if (l < nvalid) {
-                    const long gi = line_off(l) + (long)(n - HM) * sd;
-                    k2[e] = a.jac[gi];
-                    if constexpr (PASS == PASS_CONV) {
-                        // uu = dxi(d,1)*u + dxi(d,2)*v + dxi(d,3)*w with the
-                        // off-diagonal metrics zero (src/solver.F90:2201)
-                        k0[e] = a.xi[n] * a.prim[1 + dir][gi];
-                        k1[e] = a.prim[4][gi];
-                    } else {
-                        k0[e] = a.xi[n];
-                    }
-                }
-            }
-        }
+    // Per staged node, J*U (flux pass) or J*dxi(d,d) (viscous pass) stays in
+    // registers across the pass's quantities.
+    // Staging map.  x-lines are contiguous: the half-wavefront that solves a
+    // line also stages and stores it (32 consecutive nodes per access).  y/z
+    // tiles are 16 consecutive i: a wavefront moves 4 nodes x 16 lines.
+    const int ll = xdir ? myline : (tid & 15);
+    const int n0 = xdir ? (tid & 31) : (tid >> 4);
+    const bool lvalid = ll < nvalid;
+    long lo;  // element offset of node 0 of line ll
+    {
+        const long av = a0 + ll;
+        if (dir == 0) lo = a.g.org + av * a.g.s[1] + (long)b * a.g.s[2];
+        else if (dir == 1) lo = a.g.org + av + (long)b * a.g.s[2];
+        else lo = a.g.org + av + (long)b * a.g.s[1];
     }
+    double* const trow = tile + ll * a.rs_in;
+    auto sync = [&]() {
+        if (xdir) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            __syncthreads();
+        }
+    };
+
+    double k0[EK];
+#pragma unroll
+    for (int e = 0; e < EK; ++e) k0[e] = 0.0;
 
     for (int qi = 0; qi < NQ; ++qi) {
-        // ---------------- stage the 16 lines of this quantity in LDS
+        const int m = quantity<PASS>(qi, dir);
+        // ---------------- stage the lines of this quantity in LDS
 #pragma unroll
         for (int e = 0; e < EMAX; ++e) {
-            const int el = tid + e * WGT;
-            if (el < nel_in) {
-                int l, n;
-                if (dir == 0) { l = el / nn; n = el - l * nn; }
-                else { n = el / TX; l = el - n * TX; }
+            const int n = n0 + 32 * e;
+            if (n < nn) {
+                const int node = n - HM;
                 double v = 0.0;
-                if (l < nvalid) {
-                    const int node = n - HM;
-                    const long lo = line_off(l);
+                if (lvalid) {
                     if constexpr (PASS == PASS_FILTER) {
-                        v = a.q[qi][lo + (long)wrap_node(node, dim) * sd];
-                    } else if constexpr (PASS == PASS_CONV) {
+                        v = a.q[m][lo + (long)wrap_node(node, dim) * sd];
+                    } else if constexpr (PASS == PASS_GRAD) {
+                        v = a.prim[m < 3 ? 1 + m : 5][lo + (long)node * sd];
+                    } else if constexpr (PASS == PASS_FLUX) {
+                        // fcs(:,1..5) of convrsdcal6, src/solver.F90:2201-2207,
+                        // with the off-diagonal metrics zero:
+                        // F = J*(q_m*U + [m = d+1] dxi(d,d)*p), F5 = J*(q5+p)*U
                         const long gi = lo + (long)node * sd;
-                        if (qi < 4) {
-                            v = a.prim[qi < 3 ? 1 + qi : 5][gi];
+                        const double qm = a.q[m][gi];
+                        if (qi == 0) {
+                            const double jc = a.jac[gi], pr = a.prim[4][gi];
+                            const double uu = a.xi[n] * a.prim[1 + dir][gi];
+                            k0[e] = jc * uu;
+                            v = jc * (qm * uu + a.xi[n] * pr);
+                        } else if (m == 4) {
+                            // p was read for the first flux moments ago (cache)
+                            v = (qm + a.prim[4][gi]) * k0[e];
                         } else {
-                            const int m = qi - 4;
-                            const double qm = a.q[m][gi];
-                            const double uu = k0[e], pr = k1[e], jc = k2[e];
-                            // fcs(:,1..5), src/solver.F90:2203-2207
-                            if (m == 0) v = jc * qm * uu;
-                            else if (m == 4) v = jc * (qm + pr) * uu;
-                            else if (m == dir + 1) v = jc * (qm * uu + a.xi[n] * pr);
-                            else v = jc * (qm * uu);
+                            v = qm * k0[e];
                         }
                     } else if constexpr (PASS == PASS_VISC) {
                         // ff(:,2..5), src/solver.F90:2628-2639: the diagonal
                         // metric picks sigma(d,c) and qflux(d)
+                        if (qi == 0) k0[e] = a.jac[lo + (long)node * sd] * a.xi[n];
                         const long gw = lo + (long)wrap_node(node, dim) * sd;
-                        const double s = qi < 3 ? a.sig[qi][gw] : a.qfl[0][gw];
-                        v = (s * k0[e]) * k2[e];
+                        const double s = m < 3 ? a.sig[m][gw] : a.qfl[0][gw];
+                        v = s * k0[e];
                     } else {
                         v = a.fin[lo + (long)node * sd];
                     }
                 }
-                tile[l * a.rs_in + n] = v;
+                trow[n] = v;
             }
         }
-        __syncthreads();
+        sync();
 
-        // ---------------- solve: wavefront w takes lines w and w+8
-        for (int l = wave; l < nvalid; l += NWAVE) {
+        // ---------------- solve: each half-wavefront takes one line
+        if (2 * wave < nvalid) {
             double x[L];
-            double* ln = tile + l * a.rs_in;
-            solve_line<L>(ln, a.tab, a.fc, x);
-            const int lane = tid & 63;
-            double keep0 = ln[HM], keepd = ln[HM + dim];
+            double* ln = tile + myline * a.rs_in;
+            const double keep0 = ln[HM], keepd = ln[HM + dim];
+            solve_line<L, OP>(ln, a.tab, a.fc, x);
+            if (myline < nvalid) {
+                const int sub = lane & 31;
 #pragma unroll
-            for (int t = 0; t < L; ++t) {
-                const int node = a.tab.first + lane * L + t;
-                if (lane * L + t < a.tab.R && node >= 0 && node <= dim) {
-                    double v = x[t];
-                    if (a.tab.op == OP_FILTER) {
-                        // compact_filter keeps the boundary node
-                        // (src/filter.F90:141-142)
-                        if (node == 0 && lo_boundary(a.tab.ntype)) v = keep0;
-                        if (node == dim && hi_boundary(a.tab.ntype)) v = keepd;
+                for (int t = 0; t < L; ++t) {
+                    const int node = a.tab.first + sub * L + t;
+                    if (sub * L + t < a.tab.R && node >= 0 && node <= dim) {
+                        double v = x[t];
+                        if constexpr (OP == OP_FILTER) {
+                            // compact_filter keeps the boundary node
+                            // (src/filter.F90:141-142)
+                            if (node == 0 && lo_boundary(a.tab.ntype)) v = keep0;
+                            if (node == dim && hi_boundary(a.tab.ntype)) v = keepd;
+                        }
+                        ln[node + HM] = v;
                     }
-                    ln[node + HM] = v;
                 }
             }
         }
-        __syncthreads();
+        sync();
 
-        // ---------------- write back
-        const int nel_out = TX * no;
-        for (int el = tid; el < nel_out; el += WGT) {
-            int l, n;
-            if (dir == 0) { l = el / no; n = el - l * no; }
-            else { n = el / TX; l = el - n * TX; }
-            if (l >= nvalid) continue;
-            const double o = tile[l * a.rs_in + n + HM];
-            const long gi = line_off(l) + (long)n * sd;
-            if constexpr (PASS == PASS_FILTER) {
-                a.q[qi][gi] = o;
-            } else if constexpr (PASS == PASS_CONV) {
-                if (qi < 3) {
-                    a.dvel[qi * 3 + dir][gi] = o * a.xi[n + HM];
-                } else if (qi == 3) {
-                    a.dtmp[dir][gi] = o * a.xi[n + HM];
-                } else {
-                    const int m = qi - 4;
-                    if (dir == 0) a.qrhs[m][gi] = o;
-                    else if (dir == 1) a.qrhs[m][gi] = a.qrhs[m][gi] + o;
-                    else a.qrhs[m][gi] = -(a.qrhs[m][gi] + o);  // qrhs=-qrhs
+        // ---------------- write back nodes 0..dim
+        if (lvalid) {
+#pragma unroll
+            for (int e = 0; e < EMAX; ++e) {
+                const int n = n0 + 32 * e;
+                if (n <= dim) {
+                    const double o = trow[n + HM];
+                    const long gi = lo + (long)n * sd;
+                    if constexpr (PASS == PASS_FILTER) {
+                        a.q[m][gi] = o;
+                    } else if constexpr (PASS == PASS_GRAD) {
+                        if (m < 3) a.dvel[m * 3 + dir][gi] = o * a.xi[n + HM];
+                        else a.dtmp[dir][gi] = o * a.xi[n + HM];
+                    } else if constexpr (PASS == PASS_FLUX) {
+                        if (dir == 0) a.qrhs[m][gi] = o;
+                        else if (dir == 1) a.qrhs[m][gi] = a.qrhs[m][gi] + o;
+                        else a.qrhs[m][gi] = -(a.qrhs[m][gi] + o);  // qrhs=-qrhs
+                    } else if constexpr (PASS == PASS_VISC) {
+                        a.qrhs[m + 1][gi] = a.qrhs[m + 1][gi] + o;
+                    } else {
+                        a.fout[gi] = o;
+                    }
                 }
-            } else if constexpr (PASS == PASS_VISC) {
-                a.qrhs[qi + 1][gi] = a.qrhs[qi + 1][gi] + o;
-            } else {
-                a.fout[gi] = o;
             }
         }
-        if (qi + 1 < NQ) __syncthreads();  // the next staging overwrites the tile
+        if (qi + 1 < NQ) sync();  // the next staging overwrites the tile
     }
 }
 
@@ -554,7 +560,7 @@ struct astr_ctx {
     double *q[5], *prim[6], *qsave[5], *qrhs[5], *dvel[9], *dtmp[3], *sig[6], *qfl[3], *jac;
     double* scratch = nullptr;  // field in of the field-level operators
     double* scratch2 = nullptr; // field out of the field-level operators
-    double* xi = nullptr;       // 3 metric lines, each 64*LMAX+2*HM+1
+    double* xi = nullptr;       // 3 metric lines, each LPL*LMAX+2*HM+1
     long xi_stride = 0;
     double* tabs = nullptr;
     LineTab tab[3][2];
@@ -576,7 +582,7 @@ namespace {
 // tridiagonal_thomas_proprocess restated, then the chunk products.
 struct HostTab {
     int R, first, last;
-    std::vector<double> t;  // 5 * 64 * L
+    std::vector<double> t;  // 3 * LPL * L
 };
 
 void node_range(int ntype, int dim, int ext, int& first, int& last)
@@ -607,13 +613,11 @@ HostTab build_tab(int op, int ntype, int dim, double alfa, int L)
         a[0] = c[0] = b0;
         a[R - 1] = c[R - 1] = bm;
     }
-    const int S = 64 * L;
-    h.t.assign(5 * S, 0.0);
+    const int S = LPL * L;
+    h.t.assign(3 * S, 0.0);
     double* ac1 = h.t.data();
     double* ac2 = ac1 + S;
     double* ac3 = ac1 + 2 * S;
-    double* gp = ac1 + 3 * S;
-    double* hp = ac1 + 4 * S;
     // src/commfunc.F90:760-770
     ac1[0] = c[0];
     ac2[0] = 1.0;
@@ -625,19 +629,6 @@ HostTab build_tab(int op, int ntype, int dim, double alfa, int L)
         ac3[r] = a[r] / den;
     }
     ac1[R - 1] = 0.0;  // the solver starts the back substitution at x(n)=d(n)
-    for (int l = 0; l < 64; ++l) {
-        const int r0 = l * L;
-        double pr = 1.0;
-        for (int t = 0; t < L; ++t) {
-            pr *= -ac3[r0 + t];
-            gp[r0 + t] = pr;
-        }
-        pr = 1.0;
-        for (int t = L - 1; t >= 0; --t) {
-            pr *= -ac1[r0 + t];
-            hp[r0 + t] = pr;
-        }
-    }
     return h;
 }
 
@@ -744,6 +735,7 @@ int launch_sweep_p(astr_ctx* c, const SweepArgs& a)
         hipLaunchKernelGGL((k_sweep<LL, PASS>), dim3(nblk), dim3(WGT), lds, c->stream, a); \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
     default:
         return fail(ASTR_E_UNSUPPORTED, "line too long");
@@ -756,7 +748,8 @@ int launch_sweep(astr_ctx* c, const SweepArgs& a)
 {
     switch (a.pass) {
     case PASS_FILTER: return launch_sweep_p<PASS_FILTER>(c, a);
-    case PASS_CONV: return launch_sweep_p<PASS_CONV>(c, a);
+    case PASS_GRAD: return launch_sweep_p<PASS_GRAD>(c, a);
+    case PASS_FLUX: return launch_sweep_p<PASS_FLUX>(c, a);
     case PASS_VISC: return launch_sweep_p<PASS_VISC>(c, a);
     case PASS_DERIV_FIELD: return launch_sweep_p<PASS_DERIV_FIELD>(c, a);
     case PASS_FILTER_FIELD: return launch_sweep_p<PASS_FILTER_FIELD>(c, a);
@@ -810,7 +803,10 @@ int do_qswap(astr_ctx* c)
 
 int do_rhscal(astr_ctx* c)
 {
-    for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_CONV)));
+    for (int d = 0; d < 3; ++d) {
+        TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
+        TRY(launch_sweep(c, sweep_args(c, d, PASS_FLUX)));
+    }
     TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
     for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_VISC)));
     return ASTR_OK;
@@ -901,7 +897,10 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
         if (timed) TRY(time_mark(c, 1));
         TRY(do_qswap(c));
         if (timed) TRY(time_mark(c, 2));
-        for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_CONV)));
+        for (int d = 0; d < 3; ++d) {
+            TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
+            TRY(launch_sweep(c, sweep_args(c, d, PASS_FLUX)));
+        }
         if (timed) TRY(time_mark(c, 3));
         TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
         if (timed) TRY(time_mark(c, 4));
@@ -962,7 +961,7 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     for (int d = 0; d < 3; ++d) {
         if (dims[d] < 16) return fail(ASTR_E_UNSUPPORTED, "each axis needs at least 16 cells");
         if (nts[d] < 1 || nts[d] > 4) return fail(ASTR_E_ARG, "npdc must be 1..4");
-        if ((dims[d] + 7 + 63) / 64 > LMAX || TX * (dims[d] + 12) * 8 > 65536)
+        if ((dims[d] + 7 + LPL - 1) / LPL > LMAX || TX * (dims[d] + 12) * 8 > 65536)
             return fail(ASTR_E_UNSUPPORTED, "axis longer than 500 cells");
     }
     auto* c = new astr_ctx();
@@ -1019,7 +1018,7 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     std::vector<double> tabs;
     std::vector<long> offs;
     for (int d = 0; d < 3; ++d) {
-        c->L[d] = (dims[d] + 7 + 63) / 64;
+        c->L[d] = (dims[d] + 7 + LPL - 1) / LPL;
         for (int op = 0; op < 2; ++op) {
             HostTab h = build_tab(op, nts[d], dims[d], p->alfa_filter, c->L[d]);
             offs.push_back((long)tabs.size());
@@ -1046,7 +1045,7 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
         hipMemcpy(c->fc, &fc, sizeof fc, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(ASTR_E_HIP, "coefficient upload failed"));
 
-    c->xi_stride = 64 * LMAX + 2 * HM + 1;
+    c->xi_stride = LPL * LMAX + 2 * HM + 1;
     if (hipMalloc(&c->xi, sizeof(double) * 3 * c->xi_stride) != hipSuccess)
         return cleanup(fail(ASTR_E_NOMEM, "metric allocation failed"));
     (void)hipMemset(c->xi, 0, sizeof(double) * 3 * c->xi_stride);

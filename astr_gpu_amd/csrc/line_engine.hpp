@@ -1,33 +1,36 @@
-// Device-side line engine: one 64-lane wavefront solves one compact line.
+// Device-side line engine: 32 lanes of a wavefront solve one compact line.
 //
 // A line of R tridiagonal rows (the pre-factored Thomas system of
-// tridiagonal_thomas_solver, src/commfunc.F90:790-813) is spread over the 64
-// lanes in chunks of L consecutive rows.  Both first-order recurrences of the
+// tridiagonal_thomas_solver, src/commfunc.F90:790-813) is spread over the 32
+// lanes of a half-wavefront in chunks of L consecutive rows, so one
+// wavefront solves two lines at once.  Both first-order recurrences of the
 // Thomas solve,
 //     forward   y_r = rhs_r * ac2_r - ac3_r * y_{r-1}
 //     backward  x_r = y_r - ac1_r * x_{r+1},
 // are evaluated as a chunked scan: each lane runs its chunk sequentially from
-// a zero carry, the chunk-end values are combined across lanes with a
-// Kogge-Stone scan of affine maps (6 shuffle steps), and every row is then
-// corrected by (product of the chunk's multipliers) x (carry).  The products
-// g_r = prod(-ac3) and h_r = prod(-ac1) over the chunk depend only on the row,
-// so they are precomputed on the host with the factors.  The result equals
-// the sequential solve up to rounding (a few ulp), with no per-line scratch.
+// a zero carry, the chunk-end values are combined across the 32 lanes with a
+// Kogge-Stone scan of affine maps (four DPP row shifts inside each 16-lane
+// row, then one step across the two rows), and every row is corrected by
+// (product of the chunk's multipliers) x (carry).  The result equals the
+// sequential solve up to rounding (a few ulp), with no per-line scratch.
 //
 // The right-hand sides restate compact_fd_rhs (src/derivative.F90:210-306)
-// and compact_filter_rhs (src/filter.F90:156-285); they read the line from
-// LDS, where node n of the line sits at ln[n + HM].
+// and compact_filter_rhs (src/filter.F90:156-285).  A lane loads its rows'
+// stencil window from the LDS line (node n at ln[n + HM]) into registers
+// once; the few closure rows at the line ends are evaluated wave-uniformly
+// (scalar coefficient loads) and handed to the lane that owns them.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace astr {
 
-constexpr int HM = 5;  // halo width, src/commvar.F90:190
+constexpr int HM = 5;    // halo width, src/commvar.F90:190
+constexpr int LPL = 32;  // lanes per line
 
 enum LineOp : int { OP_DERIV = 0, OP_FILTER = 1 };
 
-// One pre-factored line system on the device.  t points at 5 arrays of
-// 64*L doubles: ac1, ac2, ac3, g, h (rows >= R are zero).
+// One pre-factored line system on the device.  t points at 3 arrays of
+// LPL*L doubles: ac1, ac2, ac3 (rows >= R are zero).
 struct LineTab {
     int R;      // rows
     int first;  // node index of row 0
@@ -52,34 +55,95 @@ struct FiltCoef {
 __device__ __forceinline__ bool lo_boundary(int nt) { return nt == 1 || nt == 4; }
 __device__ __forceinline__ bool hi_boundary(int nt) { return nt == 2 || nt == 4; }
 
-// compact_fd_rhs for row r (node tb.first + r), 643c scheme.
-__device__ __forceinline__ double rhs_deriv(const double* ln, int r, const LineTab& tb)
+// ---------------------------------------------------------------- DPP helpers
+// update_dpp on the two halves of a double; lanes whose source is outside
+// the pattern keep `ident` (bound_ctrl off).
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ double dpp(double v, double ident)
 {
-    const double* f = ln + HM + tb.first + r;  // f[s] = node (first+r+s)
-    const bool lob = lo_boundary(tb.ntype), hib = hi_boundary(tb.ntype);
-    if (r == 0) {
-        if (lob) return -2.5 * f[0] + 2.0 * f[1] + 0.5 * f[2];
-        return 0.75 * (f[1] - f[-1]) - 0.15 * (f[2] - f[-2]) + (1.0 / 60.0) * (f[3] - f[-3]);
-    }
-    if (r == tb.R - 1) {
-        if (hib) return 2.5 * f[0] - 2.0 * f[-1] - 0.5 * f[-2];
-        return 0.75 * (f[1] - f[-1]) - 0.15 * (f[2] - f[-2]) + (1.0 / 60.0) * (f[3] - f[-3]);
-    }
-    if ((r == 1 && lob) || (r == tb.R - 2 && hib)) return 0.75 * (f[1] - f[-1]);
-    return (7.0 / 9.0) * (f[1] - f[-1]) + (1.0 / 36.0) * (f[2] - f[-2]);
+    const long long b = __double_as_longlong(v), ib = __double_as_longlong(ident);
+    const int lo = __builtin_amdgcn_update_dpp((int)ib, (int)b, CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(ib >> 32), (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// compact_filter_rhs for row r.
-__device__ __forceinline__ double rhs_filter(const double* ln, int r, const LineTab& tb,
-                                             const FiltCoef* __restrict__ fc)
+__device__ __forceinline__ double readlane(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+constexpr int DPP_ROW_SR1 = 0x111, DPP_ROW_SR2 = 0x112, DPP_ROW_SR4 = 0x114, DPP_ROW_SR8 = 0x118;
+constexpr int DPP_ROW_SL1 = 0x101, DPP_ROW_SL2 = 0x102, DPP_ROW_SL4 = 0x104, DPP_ROW_SL8 = 0x108;
+constexpr int DPP_WAVE_SR1 = 0x138, DPP_WAVE_SL1 = 0x130, DPP_ROW_BCAST15 = 0x142;
+
+// one Kogge-Stone step: (A,B) <- (A,B) o (Ap,Bp), i.e. B = A*Bp + B, A = A*Ap
+__device__ __forceinline__ void ks(double& A, double& B, double Ap, double Bp)
+{
+    B = fma(A, Bp, B);
+    A = A * Ap;
+}
+
+// Inclusive scan over each 32-lane group of Y_l = A_l Y_{l-1} + B_l
+// (forward: lane order); returns the exclusive carry Y_{l-1} (0 at lane 0).
+__device__ __forceinline__ double scan_fwd(double A, double B, int lane)
+{
+    ks(A, B, dpp<DPP_ROW_SR1>(A, 1.0), dpp<DPP_ROW_SR1>(B, 0.0));
+    ks(A, B, dpp<DPP_ROW_SR2>(A, 1.0), dpp<DPP_ROW_SR2>(B, 0.0));
+    ks(A, B, dpp<DPP_ROW_SR4>(A, 1.0), dpp<DPP_ROW_SR4>(B, 0.0));
+    ks(A, B, dpp<DPP_ROW_SR8>(A, 1.0), dpp<DPP_ROW_SR8>(B, 0.0));
+    // rows 1 and 3 take the totals of rows 0 and 2 (lanes 15, 47)
+    ks(A, B, dpp<DPP_ROW_BCAST15, 0xA>(A, 1.0), dpp<DPP_ROW_BCAST15, 0xA>(B, 0.0));
+    double c = dpp<DPP_WAVE_SR1>(B, 0.0);
+    return (lane & 31) == 0 ? 0.0 : c;
+}
+
+// Same in reverse lane order: X_l = A_l X_{l+1} + B_l; returns X_{l+1}.
+__device__ __forceinline__ double scan_bwd(double A, double B, int lane)
+{
+    ks(A, B, dpp<DPP_ROW_SL1>(A, 1.0), dpp<DPP_ROW_SL1>(B, 0.0));
+    ks(A, B, dpp<DPP_ROW_SL2>(A, 1.0), dpp<DPP_ROW_SL2>(B, 0.0));
+    ks(A, B, dpp<DPP_ROW_SL4>(A, 1.0), dpp<DPP_ROW_SL4>(B, 0.0));
+    ks(A, B, dpp<DPP_ROW_SL8>(A, 1.0), dpp<DPP_ROW_SL8>(B, 0.0));
+    // rows 0 and 2 take the totals of rows 1 and 3 (lanes 16, 48)
+    const double a16 = readlane(A, 16), b16 = readlane(B, 16);
+    const double a48 = readlane(A, 48), b48 = readlane(B, 48);
+    if ((lane & 16) == 0) {
+        const bool hi = lane >= 32;
+        ks(A, B, hi ? a48 : a16, hi ? b48 : b16);
+    }
+    double c = dpp<DPP_WAVE_SL1>(B, 0.0);
+    return (lane & 31) == 31 ? 0.0 : c;
+}
+
+// ------------------------------------------------------------ right-hand sides
+
+// Special rows of compact_fd_rhs (src/derivative.F90:230-294) for row r
+// (node n = first + r); f points at node n in the LDS line.
+__device__ __forceinline__ double rhs_deriv_edge(const double* f, int r, const LineTab& tb)
+{
+    const bool lob = lo_boundary(tb.ntype), hib = hi_boundary(tb.ntype);
+    if (r == 0 && lob) return -2.5 * f[0] + 2.0 * f[1] + 0.5 * f[2];
+    if (r == tb.R - 1 && hib) return 2.5 * f[0] - 2.0 * f[-1] - 0.5 * f[-2];
+    if ((r == 1 && lob) || (r == tb.R - 2 && hib)) return 0.75 * (f[1] - f[-1]);
+    // explicit 6th-order interface row
+    return 0.75 * (f[1] - f[-1]) - 0.15 * (f[2] - f[-2]) + (1.0 / 60.0) * (f[3] - f[-3]);
+}
+
+// Special rows of compact_filter_rhs (src/filter.F90:176-264) for row r;
+// ln is the LDS line (node n at ln[n + HM]).  Called with r uniform across
+// the wavefront, so the coefficient reads are scalar.
+__device__ __forceinline__ double rhs_filter_edge(const double* ln, int r, const LineTab& tb,
+                                                  const FiltCoef* __restrict__ fc)
 {
     const double* f = ln + HM;  // f[n] = node n
-    const int n = tb.first + r;
     const int R = tb.R;
     const bool lob = lo_boundary(tb.ntype), hib = hi_boundary(tb.ntype);
+    double v = 0.0;
     if (r >= R - 3) {
         const int k = R - 1 - r;
-        double v = 0.0;
         if (hib) {
             for (int j = 0; j <= 6; ++j) v = v + fc->cb[k][j] * f[tb.last - j];
         } else {
@@ -88,7 +152,6 @@ __device__ __forceinline__ double rhs_filter(const double* ln, int r, const Line
         return v;
     }
     if (r <= 2) {
-        double v = 0.0;
         if (lob) {
             for (int j = 0; j <= 6; ++j) v = v + fc->cb[r][j] * f[tb.first + j];
         } else {
@@ -96,87 +159,124 @@ __device__ __forceinline__ double rhs_filter(const double* ln, int r, const Line
         }
         return v;
     }
-    const double* g = f + n;
-    if ((lob && r == 3) || (hib && r == R - 4)) {
+    const double* g = f + tb.first + r;
+    if (r == 3 || r == R - 4)  // 6th-order row next to a boundary
         return fc->c6[0] * (g[0] + g[0]) + fc->c6[1] * (g[1] + g[-1]) +
                fc->c6[2] * (g[2] + g[-2]) + fc->c6[3] * (g[3] + g[-3]);
-    }
-    if ((lob && r == 4) || (hib && r == R - 5)) {
-        return fc->c8[0] * (g[0] + g[0]) + fc->c8[1] * (g[1] + g[-1]) +
-               fc->c8[2] * (g[2] + g[-2]) + fc->c8[3] * (g[3] + g[-3]) +
-               fc->c8[4] * (g[4] + g[-4]);
-    }
+    // 8th-order row
+    return fc->c8[0] * (g[0] + g[0]) + fc->c8[1] * (g[1] + g[-1]) +
+           fc->c8[2] * (g[2] + g[-2]) + fc->c8[3] * (g[3] + g[-3]) + fc->c8[4] * (g[4] + g[-4]);
+}
+
+// Interior rows: 6th-order compact derivative / 10th-order filter.
+__device__ __forceinline__ double rhs_deriv_in(const double* f)
+{
+    return (7.0 / 9.0) * (f[1] - f[-1]) + (1.0 / 36.0) * (f[2] - f[-2]);
+}
+
+__device__ __forceinline__ double rhs_filter_in(const double* g, const FiltCoef* __restrict__ fc)
+{
     return fc->c10[0] * (g[0] + g[0]) + fc->c10[1] * (g[1] + g[-1]) +
            fc->c10[2] * (g[2] + g[-2]) + fc->c10[3] * (g[3] + g[-3]) +
            fc->c10[4] * (g[4] + g[-4]) + fc->c10[5] * (g[5] + g[-5]);
 }
 
-// Solve the line held in LDS at ln (nodes -HM..dim+HM); on return lane l
-// holds the solution of rows l*L .. l*L+L-1 in x[].  Must be called by all
-// 64 lanes of a wavefront together.
+// Put v into b[r - sub*L] on the lane that owns row r (r uniform).
 template <int L>
+__device__ __forceinline__ void set_row(double (&b)[L], int r, int sub, double v)
+{
+    const int owner = r / L, t = r - owner * L;
+    if (sub == owner) {
+#pragma unroll
+        for (int tt = 0; tt < L; ++tt)
+            if (tt == t) b[tt] = v;
+    }
+}
+
+// Solve the line held in LDS at ln (nodes -HM..dim+HM).  Called by a whole
+// wavefront; lanes 0-31 and 32-63 each solve their own line (same system),
+// lane sub of a half holds rows sub*L .. sub*L+L-1 in x[] on return.
+template <int L, int OP>
 __device__ __forceinline__ void solve_line(const double* ln, const LineTab& tb,
                                            const FiltCoef* __restrict__ fc, double (&x)[L])
 {
+    constexpr int HW = OP == OP_DERIV ? 2 : 5;  // interior stencil half-width
+    constexpr int W = L + 2 * HW;
     const int lane = threadIdx.x & 63;
-    const int stride = 64 * L;
+    const int sub = lane & 31;
+    const int S = LPL * L;
     const double* __restrict__ ac1 = tb.t;
-    const double* __restrict__ ac2 = tb.t + stride;
-    const double* __restrict__ ac3 = tb.t + 2 * stride;
-    const double* __restrict__ gp = tb.t + 3 * stride;
-    const double* __restrict__ hp = tb.t + 4 * stride;
-    const int r0 = lane * L;
+    const double* __restrict__ ac2 = tb.t + S;
+    const double* __restrict__ ac3 = tb.t + 2 * S;
+    const int r0 = sub * L;
+    const int R = tb.R;
 
-    double y[L];
-    double prev = 0.0;
+    // interior right-hand side of every row from a register window
+    {
+        double w[W];
+        const int nb = tb.first + r0 - HW;  // node of w[0]
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            const int nd = nb + k;
+            w[k] = (nd >= -HM && nd <= tb.dim + HM) ? ln[nd + HM] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < L; ++t) {
+            if constexpr (OP == OP_DERIV) x[t] = rhs_deriv_in(w + t + HW);
+            else x[t] = rhs_filter_in(w + t + HW, fc);
+        }
+    }
+    // the closure rows at both ends, computed wave-uniformly
+    if constexpr (OP == OP_DERIV) {
+        const int ne = (lo_boundary(tb.ntype) ? 2 : 1);
+        for (int j = 0; j < ne; ++j) set_row(x, j, sub, rhs_deriv_edge(ln + HM + tb.first + j, j, tb));
+        const int me = (hi_boundary(tb.ntype) ? 2 : 1);
+        for (int j = 0; j < me; ++j) {
+            const int r = R - 1 - j;
+            set_row(x, r, sub, rhs_deriv_edge(ln + HM + tb.first + r, r, tb));
+        }
+    } else {
+        const int ne = lo_boundary(tb.ntype) ? 5 : 3;
+        for (int r = 0; r < ne; ++r) set_row(x, r, sub, rhs_filter_edge(ln, r, tb, fc));
+        const int me = hi_boundary(tb.ntype) ? 5 : 3;
+        for (int j = 0; j < me; ++j) {
+            const int r = R - 1 - j;
+            set_row(x, r, sub, rhs_filter_edge(ln, r, tb, fc));
+        }
+    }
+
+    // forward elimination, chunk-local, with the chunk product g
+    double prev = 0.0, g = 1.0;
+    double gr[L];
 #pragma unroll
     for (int t = 0; t < L; ++t) {
         const int r = r0 + t;
-        double b = 0.0;
-        if (r < tb.R) b = (tb.op == OP_DERIV) ? rhs_deriv(ln, r, tb) : rhs_filter(ln, r, tb, fc);
-        const double v = b * ac2[r] - ac3[r] * prev;
-        y[t] = v;
+        const double b = r < R ? x[t] : 0.0;
+        const double c3 = ac3[r];
+        const double v = b * ac2[r] - c3 * prev;
+        g = -c3 * g;
+        gr[t] = g;
+        x[t] = v;
         prev = v;
     }
-    // inclusive scan over lanes of Y_l = A_l * Y_{l-1} + B_l
-    double A = gp[r0 + L - 1], B = prev;
+    double cin = scan_fwd(g, prev, lane);
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double Ap = __shfl_up(A, off, 64);
-        const double Bp = __shfl_up(B, off, 64);
-        if (lane >= off) {
-            B = fma(A, Bp, B);
-            A = A * Ap;
-        }
-    }
-    double cin = __shfl_up(B, 1, 64);
-    if (lane == 0) cin = 0.0;
-#pragma unroll
-    for (int t = 0; t < L; ++t) y[t] = fma(gp[r0 + t], cin, y[t]);
+    for (int t = 0; t < L; ++t) x[t] = fma(gr[t], cin, x[t]);
 
-    double nxt = 0.0;
+    // back substitution, chunk-local, with the chunk product h
+    double nxt = 0.0, h = 1.0;
 #pragma unroll
     for (int t = L - 1; t >= 0; --t) {
-        const double v = y[t] - ac1[r0 + t] * nxt;
+        const double c1 = ac1[r0 + t];
+        const double v = x[t] - c1 * nxt;
+        h = -c1 * h;
+        gr[t] = h;
         x[t] = v;
         nxt = v;
     }
-    // reverse inclusive scan of X_l = H_l * X_{l+1} + x_loc(first row of l)
-    A = hp[r0];
-    B = nxt;
+    cin = scan_bwd(h, nxt, lane);
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double Ap = __shfl_down(A, off, 64);
-        const double Bp = __shfl_down(B, off, 64);
-        if (lane + off < 64) {
-            B = fma(A, Bp, B);
-            A = A * Ap;
-        }
-    }
-    cin = __shfl_down(B, 1, 64);
-    if (lane == 63) cin = 0.0;
-#pragma unroll
-    for (int t = 0; t < L; ++t) x[t] = fma(hp[r0 + t], cin, x[t]);
+    for (int t = 0; t < L; ++t) x[t] = fma(gr[t], cin, x[t]);
 }
 
 }  // namespace astr
