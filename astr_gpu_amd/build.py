@@ -43,5 +43,27 @@ def build_library(verbose: bool = False) -> str:
     return TARGET
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_EXE = os.path.join(FORTRAN_DIR, "tgv_hip")
+
+
+def build_fortran_example() -> str | None:
+    """Compile the iso_c_binding module and the Fortran Taylor-Green driver
+    against libastr_hip.so (needs amdflang; returns None without it)."""
+    fc = os.environ.get("FC_ASTR", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(fc):
+        return None
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        mod = os.path.join(FORTRAN_DIR, "astr_hip_mod.f90")
+        obj = os.path.join(tmp, "astr_hip_mod.o")
+        subprocess.run([fc, "-O2", "-J", tmp, "-c", mod, "-o", obj], check=True)
+        subprocess.run([fc, "-O2", "-J", tmp, "-o", FORTRAN_EXE,
+                        os.path.join(FORTRAN_DIR, "tgv_hip.f90"), obj, "-L" + HERE, "-lastr_hip",
+                        "-Wl,-rpath,$ORIGIN/.."], check=True)
+    return FORTRAN_EXE
+
+
 if __name__ == "__main__":
     print(build_library(verbose="-v" in sys.argv))
+    print(build_fortran_example())

@@ -53,9 +53,10 @@ int fail(int code, const std::string& msg)
             return fail(ASTR_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int TX = 16;     // lines per tile
-constexpr int WGT = 512;   // threads per sweep workgroup
-constexpr int NWAVE = WGT / 64;
+constexpr int TXY = 16;    // lines per y/z tile: 16 consecutive i = 128 bytes
+constexpr int TXX = 8;     // lines per x tile: one per half-wavefront
+constexpr int TX = TXY;    // largest tile
+constexpr int WGT = 256;   // threads per sweep workgroup
 constexpr int OI = 16;     // element offset of node i=0 in a row
 constexpr int LMAX = 16;   // rows per lane supported (line length <= 32*LMAX)
 
@@ -111,13 +112,33 @@ __device__ __forceinline__ int wrap_node(int n, int dim)
     return n < 0 ? n + dim : (n > dim ? n - dim : n);
 }
 
-// Order of the pass's quantities; the flux pass loads the momentum flux of
-// the sweep axis first (it reads u_d, p and J and keeps J*U in registers).
+// Quantity of a workgroup: identity (the flux order is the variable order).
 template <int PASS>
 __device__ __forceinline__ int quantity(int qi, int dir)
 {
-    if constexpr (PASS == PASS_FLUX) return qi == 0 ? dir + 1 : (qi <= dir + 1 ? qi - 1 : qi);
+    (void)dir;
     return qi;
+}
+
+// Every workgroup solves one quantity of one tile.  Block ids are laid out
+// so that the NQ quantity-workgroups of a tile are dispatched within a few
+// dozen blocks of each other onto the same XCD (workgroups go round-robin
+// over the 8 XCDs): the inputs they share (u_d, p, J of the flux pass, J of
+// the viscous pass) are then re-read from that XCD's L2, not from HBM.
+template <int PASS>
+struct PassCfg {
+    static constexpr int NQ = PASS == PASS_FILTER ? 5 : PASS == PASS_GRAD ? 4 : PASS == PASS_FLUX ? 5
+                            : PASS == PASS_VISC ? 4 : 1;
+};
+
+// block id -> (tile, quantity)
+__device__ __forceinline__ void block_tile(int bid, int nq, int ntiles, int& tile, int& q)
+{
+    const int grp = 8 * nq;
+    const int g = bid / grp, r = bid - g * grp;
+    q = r >> 3;
+    tile = g * 8 + (r & 7);
+    (void)ntiles;
 }
 
 template <int L, int PASS>
@@ -127,34 +148,35 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     // The solution of a line overwrites that line's slots in place, which is
     // safe because only the half-wavefront solving a line reads it.
     extern __shared__ double tile[];
+    using C = PassCfg<PASS>;
     constexpr int OP = (PASS == PASS_FILTER || PASS == PASS_FILTER_FIELD) ? OP_FILTER : OP_DERIV;
-    constexpr int EMAX = L + 1;  // staged nodes per thread: 32 per step, dim+11 <= 32*L+4
-    constexpr int NKEEP = PASS == PASS_FLUX || PASS == PASS_VISC ? 1 : 0;
-    constexpr int EK = NKEEP ? EMAX : 1;
-    constexpr int NQ = PASS == PASS_FILTER ? 5 : PASS == PASS_GRAD ? 4 : PASS == PASS_FLUX ? 5
-                     : PASS == PASS_VISC ? 4 : 1;
-
-    const int ta = blockIdx.x % a.ntile_a;
-    const int b = blockIdx.x / a.ntile_a;
-    const int a0 = ta * TX;
     const int dir = a.dir;
+    const bool xdir = dir == 0;
+    const int tx = xdir ? TXX : TXY;
+    // staged nodes per thread: x 32 per step, y/z 16 per step; dim+11 <= 32*L+4
+    constexpr int EMAX = 2 * L + 1;
+
+    int tile_id, m;
+    block_tile(blockIdx.x, C::NQ, a.ntile_a * a.nb, tile_id, m);
+    if (tile_id >= a.ntile_a * a.nb) return;  // padding blocks of the last group
+    if constexpr (PASS == PASS_FLUX) m = quantity<PASS>(m, dir);
+    const int ta = tile_id % a.ntile_a;
+    const int b = tile_id / a.ntile_a;
+    const int a0 = ta * tx;
     const int dim = a.g.n[dir];
     const int nn = dim + 1 + 2 * HM;   // staged nodes per line
     const long sd = a.g.s[dir];
-    const int nvalid = min(TX, a.na - a0);
+    const int nvalid = min(tx, a.na - a0);
     const int tid = threadIdx.x;
-    const int wave = tid >> 6;
     const int lane = tid & 63;
-    const int myline = tid >> 5;  // line solved by this half-wavefront
-    const bool xdir = dir == 0;
+    const int half = tid >> 5;  // half-wavefront: solves lines half (+8 for y/z)
 
-    // Per staged node, J*U (flux pass) or J*dxi(d,d) (viscous pass) stays in
-    // registers across the pass's quantities.
     // Staging map.  x-lines are contiguous: the half-wavefront that solves a
     // line also stages and stores it (32 consecutive nodes per access).  y/z
     // tiles are 16 consecutive i: a wavefront moves 4 nodes x 16 lines.
-    const int ll = xdir ? myline : (tid & 15);
+    const int ll = xdir ? half : (tid & 15);
     const int n0 = xdir ? (tid & 31) : (tid >> 4);
+    const int nstep = xdir ? 32 : 16;
     const bool lvalid = ll < nvalid;
     long lo;  // element offset of node 0 of line ll
     {
@@ -173,17 +195,12 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
         }
     };
 
-    double k0[EK];
-#pragma unroll
-    for (int e = 0; e < EK; ++e) k0[e] = 0.0;
-
-    for (int qi = 0; qi < NQ; ++qi) {
-        const int m = quantity<PASS>(qi, dir);
+    {
         // ---------------- stage the lines of this quantity in LDS
 #pragma unroll
         for (int e = 0; e < EMAX; ++e) {
-            const int n = n0 + 32 * e;
-            if (n < nn) {
+            const int n = n0 + nstep * e;
+            if (n < nn && (xdir ? e <= L : true)) {
                 const int node = n - HM;
                 double v = 0.0;
                 if (lvalid) {
@@ -196,25 +213,18 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
                         // with the off-diagonal metrics zero:
                         // F = J*(q_m*U + [m = d+1] dxi(d,d)*p), F5 = J*(q5+p)*U
                         const long gi = lo + (long)node * sd;
-                        const double qm = a.q[m][gi];
-                        if (qi == 0) {
-                            const double jc = a.jac[gi], pr = a.prim[4][gi];
-                            const double uu = a.xi[n] * a.prim[1 + dir][gi];
-                            k0[e] = jc * uu;
-                            v = jc * (qm * uu + a.xi[n] * pr);
-                        } else if (m == 4) {
-                            // p was read for the first flux moments ago (cache)
-                            v = (qm + a.prim[4][gi]) * k0[e];
-                        } else {
-                            v = qm * k0[e];
-                        }
+                        const double qm = a.q[m][gi], jc = a.jac[gi];
+                        const double uu = a.xi[n] * a.prim[1 + dir][gi];
+                        if (m == 0) v = jc * qm * uu;
+                        else if (m == 4) v = jc * (qm + a.prim[4][gi]) * uu;
+                        else if (m == dir + 1) v = jc * (qm * uu + a.xi[n] * a.prim[4][gi]);
+                        else v = jc * (qm * uu);
                     } else if constexpr (PASS == PASS_VISC) {
                         // ff(:,2..5), src/solver.F90:2628-2639: the diagonal
                         // metric picks sigma(d,c) and qflux(d)
-                        if (qi == 0) k0[e] = a.jac[lo + (long)node * sd] * a.xi[n];
                         const long gw = lo + (long)wrap_node(node, dim) * sd;
                         const double s = m < 3 ? a.sig[m][gw] : a.qfl[0][gw];
-                        v = s * k0[e];
+                        v = (s * a.xi[n]) * a.jac[lo + (long)node * sd];
                     } else {
                         v = a.fin[lo + (long)node * sd];
                     }
@@ -224,13 +234,14 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
         }
         sync();
 
-        // ---------------- solve: each half-wavefront takes one line
-        if (2 * wave < nvalid) {
+        // ---------------- solve: each half-wavefront takes its lines
+        for (int l = half; l < tx; l += 8) {
+            if (l - (lane >> 5) >= nvalid) break;  // wave-uniform: both halves invalid
             double x[L];
-            double* ln = tile + myline * a.rs_in;
+            double* ln = tile + l * a.rs_in;
             const double keep0 = ln[HM], keepd = ln[HM + dim];
             solve_line<L, OP>(ln, a.tab, a.fc, x);
-            if (myline < nvalid) {
+            if (l < nvalid) {
                 const int sub = lane & 31;
 #pragma unroll
                 for (int t = 0; t < L; ++t) {
@@ -254,7 +265,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
         if (lvalid) {
 #pragma unroll
             for (int e = 0; e < EMAX; ++e) {
-                const int n = n0 + 32 * e;
+                const int n = n0 + nstep * e;
                 if (n <= dim) {
                     const double o = trow[n + HM];
                     const long gi = lo + (long)n * sd;
@@ -275,7 +286,6 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
                 }
             }
         }
-        if (qi + 1 < NQ) sync();  // the next staging overwrites the tile
     }
 }
 
@@ -708,7 +718,8 @@ SweepArgs sweep_args(astr_ctx* c, int dir, int pass)
     if (dir == 0) { a.na = n1 + 1; a.nb = n2 + 1; }
     else if (dir == 1) { a.na = n0 + 1; a.nb = n2 + 1; }
     else { a.na = n0 + 1; a.nb = n1 + 1; }
-    a.ntile_a = (a.na + TX - 1) / TX;
+    const int tx = dir == 0 ? TXX : TXY;
+    a.ntile_a = (a.na + tx - 1) / tx;
     const int dim = c->g.n[dir];
     a.rs_in = (dim + 1 + 2 * HM) | 1;
     for (int m = 0; m < 5; ++m) { a.q[m] = c->q[m]; a.qrhs[m] = c->qrhs[m]; }
@@ -727,12 +738,14 @@ template <int PASS>
 int launch_sweep_p(astr_ctx* c, const SweepArgs& a)
 {
     const int L = c->L[a.dir];
-    const size_t lds = sizeof(double) * TX * (size_t)a.rs_in;
-    const int nblk = a.ntile_a * a.nb;
+    const int tx = a.dir == 0 ? TXX : TXY;
+    const size_t lds = sizeof(double) * tx * (size_t)a.rs_in;
+    const int nq = PassCfg<PASS>::NQ, ntiles = a.ntile_a * a.nb;
+    const dim3 grid((ntiles + 7) / 8 * 8 * nq);
     switch (L) {
-#define CASE(LL)                                                                          \
-    case LL:                                                                              \
-        hipLaunchKernelGGL((k_sweep<LL, PASS>), dim3(nblk), dim3(WGT), lds, c->stream, a); \
+#define CASE(LL)                                                                        \
+    case LL:                                                                            \
+        hipLaunchKernelGGL((k_sweep<LL, PASS>), grid, dim3(WGT), lds, c->stream, a);    \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)

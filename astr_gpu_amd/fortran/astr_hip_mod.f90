@@ -1,0 +1,175 @@
+! iso_c_binding interface of the HIP solver (include/astr_hip.h).
+!
+! The Fortran host keeps its module state (commarray, commvar) and replaces
+! the calls of time_integration_rk (src/mainloop.F90:392-476) by the entry
+! points below.  Host arrays are passed as they are declared in the
+! reference, e.g. q(-hm:im+hm,-hm:jm+hm,-hm:km+hm,1:numq).  A nonzero
+! return code is turned into a stop by astr_check, which is what mpistop
+! does in the reference.
+module astr_hip
+
+  use iso_c_binding
+
+  implicit none
+
+  ! struct astr_params
+  type, bind(c) :: astr_params
+    integer(c_int32_t) :: im, jm, km, hm, numq
+    integer(c_int32_t) :: npdci, npdcj, npdck
+    integer(c_int32_t) :: variant, reserved
+    real(c_double) :: alfa_filter, reynolds, mach, prandtl, gamma
+    real(c_double) :: ref_tem, sutherland, beta_halo, beta_bound
+  end type astr_params
+
+  interface
+
+    subroutine astr_default_params(p, n) bind(c, name='astr_default_params')
+      import :: astr_params, c_int32_t
+      type(astr_params), intent(out) :: p
+      integer(c_int32_t), value :: n
+    end subroutine astr_default_params
+
+    integer(c_int) function astr_create(p, device, ctx) bind(c, name='astr_create')
+      import :: astr_params, c_int, c_int32_t, c_ptr
+      type(astr_params), intent(in) :: p
+      integer(c_int32_t), value :: device
+      type(c_ptr), intent(out) :: ctx
+    end function astr_create
+
+    integer(c_int) function astr_destroy(ctx) bind(c, name='astr_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function astr_destroy
+
+    integer(c_int) function astr_upload_geom(ctx, jacob, dxi) bind(c, name='astr_upload_geom')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: jacob(*), dxi(*)
+    end function astr_upload_geom
+
+    integer(c_int) function astr_upload_state(ctx, q, rho, vel, prs, tmp) &
+        bind(c, name='astr_upload_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: q(*), rho(*), vel(*), prs(*), tmp(*)
+    end function astr_upload_state
+
+    integer(c_int) function astr_download_state(ctx, q, rho, vel, prs, tmp) &
+        bind(c, name='astr_download_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: q(*), rho(*), vel(*), prs(*), tmp(*)
+    end function astr_download_state
+
+    ! call filterq (src/mainloop.F90:395)
+    integer(c_int) function astr_filterq(ctx) bind(c, name='astr_filterq')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function astr_filterq
+
+    ! call qswap (src/mainloop.F90:415)
+    integer(c_int) function astr_qswap(ctx) bind(c, name='astr_qswap')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function astr_qswap
+
+    ! qrhs=0; call rhscal (src/mainloop.F90:404,417)
+    integer(c_int) function astr_rhscal(ctx) bind(c, name='astr_rhscal')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function astr_rhscal
+
+    ! RK3 combine and updatefvar (src/mainloop.F90:423-476)
+    integer(c_int) function astr_rk3_update(ctx, stage, dt) bind(c, name='astr_rk3_update')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: stage
+      real(c_double), value :: dt
+    end function astr_rk3_update
+
+    ! kenergycal, enstophycal, diss_rate_cal (src/statistic.F90:870-1042)
+    integer(c_int) function astr_stats(ctx, out3) bind(c, name='astr_stats')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out3(3)
+    end function astr_stats
+
+    ! call time_integration_rk (src/mainloop.F90:104); stats as rkfirst
+    integer(c_int) function astr_rk3_step(ctx, dt, stats) bind(c, name='astr_rk3_step')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt
+      real(c_double), intent(out) :: stats(3)
+    end function astr_rk3_step
+
+    integer(c_int) function astr_rk3_steps(ctx, nsteps, dt, stats) bind(c, name='astr_rk3_steps')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsteps
+      real(c_double), value :: dt
+      real(c_double), intent(out) :: stats(*)
+    end function astr_rk3_steps
+
+    integer(c_int) function astr_download_rhs(ctx, qrhs, dvel, dtmp) &
+        bind(c, name='astr_download_rhs')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: qrhs(*), dvel(*), dtmp(*)
+    end function astr_download_rhs
+
+    ! fds%central / compact_filter of a whole haloed field along dir
+    integer(c_int) function astr_deriv_field(ctx, dir, f, df) bind(c, name='astr_deriv_field')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: dir
+      real(c_double), intent(in) :: f(*)
+      real(c_double), intent(out) :: df(*)
+    end function astr_deriv_field
+
+    integer(c_int) function astr_filter_field(ctx, dir, f, ff) bind(c, name='astr_filter_field')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: dir
+      real(c_double), intent(in) :: f(*)
+      real(c_double), intent(out) :: ff(*)
+    end function astr_filter_field
+
+    integer(c_int) function astr_set_timing(ctx, on) bind(c, name='astr_set_timing')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function astr_set_timing
+
+    integer(c_int) function astr_get_timing(ctx, ms7) bind(c, name='astr_get_timing')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ms7(7)
+    end function astr_get_timing
+
+    type(c_ptr) function astr_last_error() bind(c, name='astr_last_error')
+      import :: c_ptr
+    end function astr_last_error
+
+  end interface
+
+contains
+
+  ! Stop with the library's message on a nonzero code (the reference stops
+  ! through mpistop on any error).
+  subroutine astr_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (rc == 0) return
+    call c_f_pointer(astr_last_error(), msg, [1024])
+    n = 0
+    do while (n < 1024)
+      if (msg(n + 1) == c_null_char) exit
+      n = n + 1
+    enddo
+    print '(a,a,i0,a,1024a)', what, ' failed, code ', rc, ': ', msg(1:n)
+    stop 1
+  end subroutine astr_check
+
+end module astr_hip
