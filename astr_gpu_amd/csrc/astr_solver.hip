@@ -57,6 +57,7 @@ constexpr int TXY = 16;    // lines per y/z tile: 16 consecutive i = 128 bytes
 constexpr int TXX = 8;     // lines per x tile: one per half-wavefront
 constexpr int TX = TXY;    // largest tile
 constexpr int WGT = 256;   // threads per sweep workgroup
+constexpr int TG = 3;      // LDS guard slots before each staged line
 constexpr int OI = 16;     // element offset of node i=0 in a row
 constexpr int LMAX = 16;   // rows per lane supported (line length <= 32*LMAX)
 
@@ -141,19 +142,34 @@ __device__ __forceinline__ void block_tile(int bid, int nq, int ntiles, int& til
     (void)ntiles;
 }
 
-template <int L, int PASS>
+// load / store through a 32-bit byte offset from a wave-uniform base (every
+// field is below 4 GB), so the address is one VGPR added to an SGPR pair
+__device__ __forceinline__ double ldb(const double* base, unsigned int boff)
+{
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + boff);
+}
+__device__ __forceinline__ void stb(double* base, unsigned int boff, double v)
+{
+    *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
+template <int L, int PASS, bool XD>
 __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
 {
-    // One LDS tile: line l, staged node n (node n-HM) at tile[l*rs_in + n].
+    // One LDS tile; line l stages node n-HM at tile[l*rs_in + TG + n].  The TG
+    // guard slots and the tail padding keep every window read and every
+    // (clamped) staging write inside the tile, so neither needs a branch.
     // The solution of a line overwrites that line's slots in place, which is
     // safe because only the half-wavefront solving a line reads it.
     extern __shared__ double tile[];
     using C = PassCfg<PASS>;
     constexpr int OP = (PASS == PASS_FILTER || PASS == PASS_FILTER_FIELD) ? OP_FILTER : OP_DERIV;
-    const int dir = a.dir;
-    const bool xdir = dir == 0;
-    const int tx = xdir ? TXX : TXY;
-    // staged nodes per thread: x 32 per step, y/z 16 per step; dim+11 <= 32*L+4
+    // XD: x-lines (dir 0), else y- or z-lines
+    const int dir = XD ? 0 : a.dir;
+    constexpr bool xdir = XD;
+    constexpr int tx = XD ? TXX : TXY;
+    // staged nodes per thread: x 32 per step (L+1 steps), y/z 16 per step
+    // (2L+1 steps); dim+11 <= 32*L+4
     constexpr int EMAX = 2 * L + 1;
 
     int tile_id, m;
@@ -164,8 +180,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     const int b = tile_id / a.ntile_a;
     const int a0 = ta * tx;
     const int dim = a.g.n[dir];
-    const int nn = dim + 1 + 2 * HM;   // staged nodes per line
-    const long sd = a.g.s[dir];
+    const int sd = (int)a.g.s[dir];
     const int nvalid = min(tx, a.na - a0);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -176,18 +191,21 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     // tiles are 16 consecutive i: a wavefront moves 4 nodes x 16 lines.
     const int ll = xdir ? half : (tid & 15);
     const int n0 = xdir ? (tid & 31) : (tid >> 4);
-    const int nstep = xdir ? 32 : 16;
+    constexpr int nstep = XD ? 32 : 16;
+    constexpr int esteps = XD ? L + 1 : EMAX;
     const bool lvalid = ll < nvalid;
-    long lo;  // element offset of node 0 of line ll
+    const int lc = min(ll, nvalid - 1);  // staged line (a valid one for padding lanes)
+    int lo;  // element offset of node 0 of line lc
     {
-        const long av = a0 + ll;
-        if (dir == 0) lo = a.g.org + av * a.g.s[1] + (long)b * a.g.s[2];
-        else if (dir == 1) lo = a.g.org + av + (long)b * a.g.s[2];
-        else lo = a.g.org + av + (long)b * a.g.s[1];
+        const int av = a0 + lc;
+        if (XD) lo = (int)a.g.org + av * (int)a.g.s[1] + b * (int)a.g.s[2];
+        else if (dir == 1) lo = (int)a.g.org + av + b * (int)a.g.s[2];
+        else lo = (int)a.g.org + av + b * (int)a.g.s[1];
     }
-    double* const trow = tile + ll * a.rs_in;
+    const unsigned int lob = (unsigned int)lo * 8u, sdb = (unsigned int)sd * 8u;
+    double* const trow = tile + ll * a.rs_in + TG;
     auto sync = [&]() {
-        if (xdir) {
+        if constexpr (XD) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
         } else {
@@ -195,94 +213,91 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
         }
     };
 
-    {
-        // ---------------- stage the lines of this quantity in LDS
+    // ---------------- stage the lines of this quantity in LDS
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) {
+        if (e < esteps) {  // compile-time
+            const int n = n0 + nstep * e;
+            const int node = min(n - HM, dim + HM);  // clamp the tail padding
+            const unsigned int od = lob + (unsigned int)node * sdb;
+            double v;
+            if constexpr (PASS == PASS_FILTER) {
+                v = ldb(a.q[m], lob + (unsigned int)wrap_node(node, dim) * sdb);
+            } else if constexpr (PASS == PASS_GRAD) {
+                v = ldb(a.prim[m < 3 ? 1 + m : 5], od);
+            } else if constexpr (PASS == PASS_FLUX) {
+                // fcs(:,1..5) of convrsdcal6, src/solver.F90:2201-2207,
+                // with the off-diagonal metrics zero:
+                // F = J*(q_m*U + [m = d+1] dxi(d,d)*p), F5 = J*(q5+p)*U
+                const double xn = a.xi[node + HM];
+                const double qm = ldb(a.q[m], od), jc = ldb(a.jac, od);
+                const double uu = xn * ldb(a.prim[1 + dir], od);
+                if (m == 0) v = jc * qm * uu;
+                else if (m == 4) v = jc * (qm + ldb(a.prim[4], od)) * uu;
+                else if (m == dir + 1) v = jc * (qm * uu + xn * ldb(a.prim[4], od));
+                else v = jc * (qm * uu);
+            } else if constexpr (PASS == PASS_VISC) {
+                // ff(:,2..5), src/solver.F90:2628-2639: the diagonal metric
+                // picks sigma(d,c) and qflux(d)
+                const unsigned int ow = lob + (unsigned int)wrap_node(node, dim) * sdb;
+                const double sv = m < 3 ? ldb(a.sig[m], ow) : ldb(a.qfl[0], ow);
+                v = (sv * a.xi[node + HM]) * ldb(a.jac, od);
+            } else {
+                v = ldb(a.fin, od);
+            }
+            trow[n] = v;
+        }
+    }
+    sync();
+
+    // ---------------- solve: each half-wavefront takes its lines
+    for (int l = half; l < tx; l += 8) {
+        if (l - (lane >> 5) >= nvalid) break;  // wave-uniform: both halves invalid
+        double x[L];
+        double* ln = tile + l * a.rs_in + TG;
+        const double keep0 = ln[HM], keepd = ln[HM + dim];
+        solve_line<L, OP>(ln, a.tab, a.fc, x);
+        if (l < nvalid) {
+            const int sub = lane & 31;
+#pragma unroll
+            for (int t = 0; t < L; ++t) {
+                const int node = a.tab.first + sub * L + t;
+                if (sub * L + t < a.tab.R && node >= 0 && node <= dim) {
+                    double v = x[t];
+                    if constexpr (OP == OP_FILTER) {
+                        // compact_filter keeps the boundary node
+                        // (src/filter.F90:141-142)
+                        if (node == 0 && lo_boundary(a.tab.ntype)) v = keep0;
+                        if (node == dim && hi_boundary(a.tab.ntype)) v = keepd;
+                    }
+                    ln[node + HM] = v;
+                }
+            }
+        }
+    }
+    sync();
+
+    // ---------------- write back nodes 0..dim
+    if (lvalid) {
 #pragma unroll
         for (int e = 0; e < EMAX; ++e) {
             const int n = n0 + nstep * e;
-            if (n < nn && (xdir ? e <= L : true)) {
-                const int node = n - HM;
-                double v = 0.0;
-                if (lvalid) {
-                    if constexpr (PASS == PASS_FILTER) {
-                        v = a.q[m][lo + (long)wrap_node(node, dim) * sd];
-                    } else if constexpr (PASS == PASS_GRAD) {
-                        v = a.prim[m < 3 ? 1 + m : 5][lo + (long)node * sd];
-                    } else if constexpr (PASS == PASS_FLUX) {
-                        // fcs(:,1..5) of convrsdcal6, src/solver.F90:2201-2207,
-                        // with the off-diagonal metrics zero:
-                        // F = J*(q_m*U + [m = d+1] dxi(d,d)*p), F5 = J*(q5+p)*U
-                        const long gi = lo + (long)node * sd;
-                        const double qm = a.q[m][gi], jc = a.jac[gi];
-                        const double uu = a.xi[n] * a.prim[1 + dir][gi];
-                        if (m == 0) v = jc * qm * uu;
-                        else if (m == 4) v = jc * (qm + a.prim[4][gi]) * uu;
-                        else if (m == dir + 1) v = jc * (qm * uu + a.xi[n] * a.prim[4][gi]);
-                        else v = jc * (qm * uu);
-                    } else if constexpr (PASS == PASS_VISC) {
-                        // ff(:,2..5), src/solver.F90:2628-2639: the diagonal
-                        // metric picks sigma(d,c) and qflux(d)
-                        const long gw = lo + (long)wrap_node(node, dim) * sd;
-                        const double s = m < 3 ? a.sig[m][gw] : a.qfl[0][gw];
-                        v = (s * a.xi[n]) * a.jac[lo + (long)node * sd];
-                    } else {
-                        v = a.fin[lo + (long)node * sd];
-                    }
-                }
-                trow[n] = v;
-            }
-        }
-        sync();
-
-        // ---------------- solve: each half-wavefront takes its lines
-        for (int l = half; l < tx; l += 8) {
-            if (l - (lane >> 5) >= nvalid) break;  // wave-uniform: both halves invalid
-            double x[L];
-            double* ln = tile + l * a.rs_in;
-            const double keep0 = ln[HM], keepd = ln[HM + dim];
-            solve_line<L, OP>(ln, a.tab, a.fc, x);
-            if (l < nvalid) {
-                const int sub = lane & 31;
-#pragma unroll
-                for (int t = 0; t < L; ++t) {
-                    const int node = a.tab.first + sub * L + t;
-                    if (sub * L + t < a.tab.R && node >= 0 && node <= dim) {
-                        double v = x[t];
-                        if constexpr (OP == OP_FILTER) {
-                            // compact_filter keeps the boundary node
-                            // (src/filter.F90:141-142)
-                            if (node == 0 && lo_boundary(a.tab.ntype)) v = keep0;
-                            if (node == dim && hi_boundary(a.tab.ntype)) v = keepd;
-                        }
-                        ln[node + HM] = v;
-                    }
-                }
-            }
-        }
-        sync();
-
-        // ---------------- write back nodes 0..dim
-        if (lvalid) {
-#pragma unroll
-            for (int e = 0; e < EMAX; ++e) {
-                const int n = n0 + nstep * e;
-                if (n <= dim) {
-                    const double o = trow[n + HM];
-                    const long gi = lo + (long)n * sd;
-                    if constexpr (PASS == PASS_FILTER) {
-                        a.q[m][gi] = o;
-                    } else if constexpr (PASS == PASS_GRAD) {
-                        if (m < 3) a.dvel[m * 3 + dir][gi] = o * a.xi[n + HM];
-                        else a.dtmp[dir][gi] = o * a.xi[n + HM];
-                    } else if constexpr (PASS == PASS_FLUX) {
-                        if (dir == 0) a.qrhs[m][gi] = o;
-                        else if (dir == 1) a.qrhs[m][gi] = a.qrhs[m][gi] + o;
-                        else a.qrhs[m][gi] = -(a.qrhs[m][gi] + o);  // qrhs=-qrhs
-                    } else if constexpr (PASS == PASS_VISC) {
-                        a.qrhs[m + 1][gi] = a.qrhs[m + 1][gi] + o;
-                    } else {
-                        a.fout[gi] = o;
-                    }
+            if (e < esteps && n <= dim) {
+                const double o = trow[n + HM];
+                const unsigned int od = lob + (unsigned int)n * sdb;
+                if constexpr (PASS == PASS_FILTER) {
+                    stb(a.q[m], od, o);
+                } else if constexpr (PASS == PASS_GRAD) {
+                    if (m < 3) stb(a.dvel[m * 3 + dir], od, o * a.xi[n + HM]);
+                    else stb(a.dtmp[dir], od, o * a.xi[n + HM]);
+                } else if constexpr (PASS == PASS_FLUX) {
+                    if (dir == 0) stb(a.qrhs[m], od, o);
+                    else if (dir == 1) stb(a.qrhs[m], od, ldb(a.qrhs[m], od) + o);
+                    else stb(a.qrhs[m], od, -(ldb(a.qrhs[m], od) + o));  // qrhs=-qrhs
+                } else if constexpr (PASS == PASS_VISC) {
+                    stb(a.qrhs[m + 1], od, ldb(a.qrhs[m + 1], od) + o);
+                } else {
+                    stb(a.fout, od, o);
                 }
             }
         }
@@ -556,6 +571,16 @@ __global__ __launch_bounds__(256) void k_stats(PtArgs a)
         for (int c = 0; c < 3; ++c) a.part[3 * blockIdx.x + c] = sh[c][0];
 }
 
+// Plain copy of n doubles (16-byte accesses): the bandwidth reference for the
+// roofline and the calibration of the HBM traffic counters.
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src,
+                                              double2* __restrict__ dst, long n2)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n2;
+         i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
 // ====================================================================== host
@@ -583,7 +608,7 @@ struct astr_ctx {
     bool have_geom = false, have_state = false;
     bool timing = false;
     hipEvent_t ev[16] = {};
-    double tms[7] = {0, 0, 0, 0, 0, 0, 0};
+    double tms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -705,6 +730,19 @@ PtArgs pt_args(astr_ctx* c)
     return a;
 }
 
+// LDS row of a staged line: guard + staged nodes including the clamped tail
+// (x: 32(L+1) slots, y/z: 16(2L+1)), which also covers the solver's window
+// reads up to slot 32L+15; odd stride against bank conflicts
+int lds_row(int dir, int L)
+{
+    return (TG + (dir == 0 ? 32 * (L + 1) : 16 * (2 * L + 1))) | 1;
+}
+
+size_t lds_bytes(int dir, int L)
+{
+    return sizeof(double) * (dir == 0 ? TXX : TXY) * (size_t)lds_row(dir, L);
+}
+
 SweepArgs sweep_args(astr_ctx* c, int dir, int pass)
 {
     SweepArgs a{};
@@ -721,7 +759,7 @@ SweepArgs sweep_args(astr_ctx* c, int dir, int pass)
     const int tx = dir == 0 ? TXX : TXY;
     a.ntile_a = (a.na + tx - 1) / tx;
     const int dim = c->g.n[dir];
-    a.rs_in = (dim + 1 + 2 * HM) | 1;
+    a.rs_in = lds_row(dir, c->L[dir]);
     for (int m = 0; m < 5; ++m) { a.q[m] = c->q[m]; a.qrhs[m] = c->qrhs[m]; }
     for (int m = 0; m < 6; ++m) a.prim[m] = c->prim[m];
     for (int m = 0; m < 9; ++m) a.dvel[m] = c->dvel[m];
@@ -738,14 +776,16 @@ template <int PASS>
 int launch_sweep_p(astr_ctx* c, const SweepArgs& a)
 {
     const int L = c->L[a.dir];
-    const int tx = a.dir == 0 ? TXX : TXY;
-    const size_t lds = sizeof(double) * tx * (size_t)a.rs_in;
+    const size_t lds = lds_bytes(a.dir, L);
     const int nq = PassCfg<PASS>::NQ, ntiles = a.ntile_a * a.nb;
     const dim3 grid((ntiles + 7) / 8 * 8 * nq);
     switch (L) {
 #define CASE(LL)                                                                        \
     case LL:                                                                            \
-        hipLaunchKernelGGL((k_sweep<LL, PASS>), grid, dim3(WGT), lds, c->stream, a);    \
+        if (a.dir == 0)                                                                 \
+            hipLaunchKernelGGL((k_sweep<LL, PASS, true>), grid, dim3(WGT), lds, c->stream, a); \
+        else                                                                            \
+            hipLaunchKernelGGL((k_sweep<LL, PASS, false>), grid, dim3(WGT), lds, c->stream, a); \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
@@ -904,32 +944,38 @@ int time_mark(astr_ctx* c, int slot)
 
 int do_step(astr_ctx* c, double dt, double* stats, bool timed)
 {
+    // timing families: 0 filter, 1 qswap, 2 grad, 3 flux, 4 sigma, 5 visc, 6 rk
     for (int s = 1; s <= 3; ++s) {
         if (timed) TRY(time_mark(c, 0));
         TRY(do_filterq(c));
         if (timed) TRY(time_mark(c, 1));
         TRY(do_qswap(c));
         if (timed) TRY(time_mark(c, 2));
-        for (int d = 0; d < 3; ++d) {
-            TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
-            TRY(launch_sweep(c, sweep_args(c, d, PASS_FLUX)));
-        }
+        for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
         if (timed) TRY(time_mark(c, 3));
-        TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
+        for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_FLUX)));
         if (timed) TRY(time_mark(c, 4));
-        for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_VISC)));
+        TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
         if (timed) TRY(time_mark(c, 5));
+        for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_VISC)));
+        if (timed) TRY(time_mark(c, 6));
         if (s == 1 && stats) TRY(do_stats(c, stats));
+        if (timed) TRY(time_mark(c, 7));
         TRY(do_rk(c, s, dt));
         if (timed) {
-            TRY(time_mark(c, 6));
-            HIPCHK(hipEventSynchronize(c->ev[6]));
+            TRY(time_mark(c, 8));
+            HIPCHK(hipEventSynchronize(c->ev[8]));
+            float ms = 0.f;
             for (int f = 0; f < 6; ++f) {
-                float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, c->ev[f], c->ev[f + 1]));
                 c->tms[f] += ms;
-                c->tms[6] += ms;
+                c->tms[8] += ms;
             }
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[7], c->ev[8]));
+            c->tms[6] += ms;
+            c->tms[8] += ms;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
+            c->tms[7] += ms;
         }
     }
     return ASTR_OK;
@@ -974,8 +1020,9 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     for (int d = 0; d < 3; ++d) {
         if (dims[d] < 16) return fail(ASTR_E_UNSUPPORTED, "each axis needs at least 16 cells");
         if (nts[d] < 1 || nts[d] > 4) return fail(ASTR_E_ARG, "npdc must be 1..4");
-        if ((dims[d] + 7 + LPL - 1) / LPL > LMAX || TX * (dims[d] + 12) * 8 > 65536)
-            return fail(ASTR_E_UNSUPPORTED, "axis longer than 500 cells");
+        const int L = (dims[d] + 7 + LPL - 1) / LPL;
+        if (L > LMAX || lds_bytes(0, L) > 65536 || lds_bytes(1, L) > 65536)
+            return fail(ASTR_E_UNSUPPORTED, "axis longer than 473 cells");
     }
     auto* c = new astr_ctx();
     c->p = *p;
@@ -1232,9 +1279,31 @@ int astr_rk3_steps(astr_ctx* c, int32_t nsteps, double dt, double* stats)
 {
     TRY(ready(c));
     if (nsteps < 0) return fail(ASTR_E_ARG, "nsteps < 0");
-    for (int f = 0; f < 7; ++f) c->tms[f] = 0.0;
+    for (int f = 0; f < 9; ++f) c->tms[f] = 0.0;
     for (int s = 0; s < nsteps; ++s) TRY(do_step(c, dt, stats ? stats + 3 * s : nullptr, c->timing));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return ASTR_OK;
+}
+
+int astr_bench_copy(astr_ctx* c, int32_t reps, double* ms_per_copy, int64_t* bytes_moved)
+{
+    if (!c || !ms_per_copy || !bytes_moved || reps < 1) return fail(ASTR_E_ARG, "bad argument");
+    TRY(set_dev(c));
+    const long n2 = c->nfield / 2;
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream,
+                       (const double2*)c->scratch, (double2*)c->scratch2, n2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[14], c->stream));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream,
+                           (const double2*)c->scratch, (double2*)c->scratch2, n2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[15], c->stream));
+    HIPCHK(hipEventSynchronize(c->ev[15]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[14], c->ev[15]));
+    *ms_per_copy = ms / reps;
+    *bytes_moved = (int64_t)(2 * n2 * 16);
     return ASTR_OK;
 }
 
@@ -1245,10 +1314,10 @@ int astr_set_timing(astr_ctx* c, int32_t on)
     return ASTR_OK;
 }
 
-int astr_get_timing(astr_ctx* c, double* ms7)
+int astr_get_timing(astr_ctx* c, double* ms9)
 {
-    if (!c || !ms7) return fail(ASTR_E_ARG, "null argument");
-    for (int f = 0; f < 7; ++f) ms7[f] = c->tms[f];
+    if (!c || !ms9) return fail(ASTR_E_ARG, "null argument");
+    for (int f = 0; f < 9; ++f) ms9[f] = c->tms[f];
     return ASTR_OK;
 }
 

@@ -214,12 +214,12 @@ __device__ __forceinline__ void solve_line(const double* ln, const LineTab& tb,
     // interior right-hand side of every row from a register window
     {
         double w[W];
-        const int nb = tb.first + r0 - HW;  // node of w[0]
+        // the window of the first and last lanes reaches into guard and
+        // padding slots of the LDS row; only closure rows, which are
+        // replaced below, or rows >= R read them
+        const double* wp = ln + HM + tb.first + r0 - HW;  // node of w[0]
 #pragma unroll
-        for (int k = 0; k < W; ++k) {
-            const int nd = nb + k;
-            w[k] = (nd >= -HM && nd <= tb.dim + HM) ? ln[nd + HM] : 0.0;
-        }
+        for (int k = 0; k < W; ++k) w[k] = wp[k];
 #pragma unroll
         for (int t = 0; t < L; ++t) {
             if constexpr (OP == OP_DERIV) x[t] = rhs_deriv_in(w + t + HW);

@@ -140,11 +140,20 @@ module astr_hip
       integer(c_int32_t), value :: on
     end function astr_set_timing
 
-    integer(c_int) function astr_get_timing(ctx, ms7) bind(c, name='astr_get_timing')
+    integer(c_int) function astr_get_timing(ctx, ms9) bind(c, name='astr_get_timing')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(out) :: ms7(7)
+      real(c_double), intent(out) :: ms9(9)
     end function astr_get_timing
+
+    ! measurement utility: field-sized copy, ms per copy and bytes moved
+    integer(c_int) function astr_bench_copy(ctx, reps, ms, nbytes) bind(c, name='astr_bench_copy')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: reps
+      real(c_double), intent(out) :: ms
+      integer(c_int64_t), intent(out) :: nbytes
+    end function astr_bench_copy
 
     type(c_ptr) function astr_last_error() bind(c, name='astr_last_error')
       import :: c_ptr

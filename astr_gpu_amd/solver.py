@@ -131,9 +131,18 @@ class AstrSolver:
         check(self.lib.astr_set_timing(self.ctx, 1 if on else 0), "astr_set_timing")
 
     def timing(self) -> np.ndarray:
-        out = np.zeros(7)
+        """ms per family: filter, qswap, grad, flux, sigma, visc, rk, stats, total."""
+        out = np.zeros(9)
         check(self.lib.astr_get_timing(self.ctx, dptr(out)), "astr_get_timing")
         return out
+
+    def bench_copy(self, reps: int = 10) -> tuple[float, int]:
+        """(ms per copy, bytes read+written per copy) of a field-sized copy."""
+        ms = ctypes.c_double()
+        nb = ctypes.c_int64()
+        check(self.lib.astr_bench_copy(self.ctx, int(reps), ctypes.byref(ms), ctypes.byref(nb)),
+              "astr_bench_copy")
+        return ms.value, nb.value
 
     # ------------------------------------------------- field operators
     def central(self, direction: int, f) -> np.ndarray:

@@ -27,18 +27,23 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 ALGO_BYTES_PER_PT_STEP = 4224.0  # 176 doubles per point per stage, 3 stages
-# algorithmic doubles per interior node per launch of each kernel family
-# (fields each launch must read and write once)
+# algorithmic doubles per interior node per launch of each kernel (the
+# fields a launch must read and write once), averaged over its x, y, z
+# launches where they differ
 FAMILY_DOUBLES = {
     "filter": 10.0,        # q(1:5) in and out, per direction
     "qswap": None,         # boundary planes only
-    "conv": (20.0 + 25.0 + 25.0) / 3.0,  # x: 11 in 9 out; y,z: +qrhs in
+    "grad": 8.0,           # u, v, w, T in; dvel(.,d), dtmp(d) out
+    "flux": (13.0 + 18.0 + 18.0) / 3.0,  # q5, u_d, p, J in, qrhs out; y,z: +qrhs in
     "sigma": 25.0,         # dvel 9, dtmp 3, T, u, v, w in; sigma 6, qflux 3 out
     "visc": 13.0,          # sigma 3, qflux 1, J, qrhs 4 in; qrhs 4 out
     "rk": 27.0,            # q, qrhs, qsave, J in; q, rho,u,v,w,p,T out
 }
-FAMILY_LAUNCHES = {"filter": 3, "qswap": 2, "conv": 3, "sigma": 1, "visc": 3, "rk": 1}
-FAMILY_NAMES = ["filter", "qswap", "conv", "sigma", "visc", "rk"]
+FAMILY_KERNEL = {"filter": "k_sweep<L,0> (filter)", "grad": "k_sweep<L,1> (gradient)",
+                 "flux": "k_sweep<L,2> (inviscid flux)", "sigma": "k_sigma",
+                 "visc": "k_sweep<L,3> (viscous flux)", "rk": "k_rk"}
+FAMILY_LAUNCHES = {"filter": 3, "qswap": 2, "grad": 3, "flux": 3, "sigma": 1, "visc": 3, "rk": 1}
+FAMILY_NAMES = ["filter", "qswap", "grad", "flux", "sigma", "visc", "rk"]
 
 
 def cpu_baseline(n: int, dt: float):
@@ -96,7 +101,9 @@ def main():
     s.steploop(args.steps, dt)
     el_noev = time.perf_counter() - t0
     el = min(el, el_noev)
+    copy_ms, copy_bytes = s.bench_copy(20)
     s.close()
+    copy_gbs = copy_bytes / (copy_ms * 1e-3) / 1e9
 
     pts = float(n) ** 3
     value = pts * args.steps / el
@@ -127,9 +134,11 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_sweep {dom} family, {FAMILY_LAUNCHES[dom]} launches/stage",
+                     "kernel": f"{FAMILY_KERNEL[dom]}, {FAMILY_LAUNCHES[dom]} launches/stage",
                      "algo_bytes_per_launch": algo, "avg_launch_ms": avg_s * 1e3},
         "step_hbm_frac": value * ALGO_BYTES_PER_PT_STEP / (HBM_PEAK_GBS * 1e9),
+        "copy_gbs": copy_gbs,
+        "step_copy_frac": value * ALGO_BYTES_PER_PT_STEP / (copy_gbs * 1e9),
         "family_ms_per_step": {k: v / args.steps for k, v in fam_ms.items()},
     }
     if not args.no_cpu:

@@ -122,11 +122,19 @@ int astr_deriv_field(astr_ctx *ctx, int32_t dir, const double *f, double *df);
 int astr_filter_field(astr_ctx *ctx, int32_t dir, const double *f, double *ff);
 
 /* Kernel time of the last astr_rk3_steps call, in milliseconds, per kernel
- * family: [0] filter, [1] qswap, [2] convective+gradient sweeps,
- * [3] sigma, [4] viscous sweeps, [5] rk update, [6] total; only filled when
- * timing was enabled with astr_set_timing(ctx, 1). */
+ * family (HIP events on the solver's stream): [0] filter sweeps, [1] qswap,
+ * [2] gradient sweeps, [3] inviscid-flux sweeps, [4] sigma/qflux,
+ * [5] viscous sweeps, [6] RK update, [7] statistics, [8] total of 0-6; only
+ * filled when timing was enabled with astr_set_timing(ctx, 1). */
 int astr_set_timing(astr_ctx *ctx, int32_t on);
-int astr_get_timing(astr_ctx *ctx, double *ms7);
+int astr_get_timing(astr_ctx *ctx, double *ms9);
+
+/* Measurement utility with no counterpart in the reference: copies one
+ * field-sized device buffer (16-byte accesses) `reps` times and reports the
+ * time per copy and the bytes read plus written per copy.  It is the stream
+ * bandwidth reference of the roofline and calibrates the HBM traffic
+ * counters. */
+int astr_bench_copy(astr_ctx *ctx, int32_t reps, double *ms_per_copy, int64_t *bytes_moved);
 
 /* Human-readable text of the last error on this thread. */
 const char *astr_last_error(void);
