@@ -77,8 +77,8 @@ __device__ __forceinline__ long gidx(const Geo& g, int i, int j, int k)
 enum Pass : int {
     PASS_FILTER = 0,       // filterq along one axis, q(1:5) in place
     PASS_GRAD = 1,         // gradcal along one axis: u,v,w,T
-    PASS_FLUX = 2,         // convrsdcal6 along one axis: the 5 inviscid fluxes
-    PASS_VISC = 3,         // viscous fluxes along one axis
+    PASS_RHS = 2,          // convrsdcal6 + diffrsdcal6 along one axis: the
+                           // derivative of (viscous - inviscid) flux of q(1:5)
     PASS_DERIV_FIELD = 4,  // fds%central of one field (halos as given)
     PASS_FILTER_FIELD = 5  // compact_filter of one field (halos as given)
 };
@@ -104,6 +104,11 @@ struct SweepArgs {
     const double* xi;       // metric along dir, xi[n+HM]
     const double* fin;      // field in (PASS_*_FIELD)
     double* fout;           // field out (PASS_*_FIELD)
+    // PASS_RHS along z only: RK3 stage (1..3) applied to q in the epilogue
+    // instead of storing qrhs (0: store qrhs)
+    int rk_stage;
+    double rka, rkb, rkc, dt;
+    double* qsave[5];
 };
 
 // node -> element offset along the line, with periodic wrap (dataswap:
@@ -113,23 +118,14 @@ __device__ __forceinline__ int wrap_node(int n, int dim)
     return n < 0 ? n + dim : (n > dim ? n - dim : n);
 }
 
-// Quantity of a workgroup: identity (the flux order is the variable order).
-template <int PASS>
-__device__ __forceinline__ int quantity(int qi, int dir)
-{
-    (void)dir;
-    return qi;
-}
-
 // Every workgroup solves one quantity of one tile.  Block ids are laid out
 // so that the NQ quantity-workgroups of a tile are dispatched within a few
 // dozen blocks of each other onto the same XCD (workgroups go round-robin
-// over the 8 XCDs): the inputs they share (u_d, p, J of the flux pass, J of
-// the viscous pass) are then re-read from that XCD's L2, not from HBM.
+// over the 8 XCDs): the inputs they share (u_d, p, J of the flux pass) are
+// then re-read from that XCD's L2, not from HBM.
 template <int PASS>
 struct PassCfg {
-    static constexpr int NQ = PASS == PASS_FILTER ? 5 : PASS == PASS_GRAD ? 4 : PASS == PASS_FLUX ? 5
-                            : PASS == PASS_VISC ? 4 : 1;
+    static constexpr int NQ = PASS == PASS_FILTER ? 5 : PASS == PASS_GRAD ? 4 : PASS == PASS_RHS ? 5 : 1;
 };
 
 // block id -> (tile, quantity)
@@ -175,7 +171,6 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     int tile_id, m;
     block_tile(blockIdx.x, C::NQ, a.ntile_a * a.nb, tile_id, m);
     if (tile_id >= a.ntile_a * a.nb) return;  // padding blocks of the last group
-    if constexpr (PASS == PASS_FLUX) m = quantity<PASS>(m, dir);
     const int ta = tile_id % a.ntile_a;
     const int b = tile_id / a.ntile_a;
     const int a0 = ta * tx;
@@ -225,23 +220,32 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
                 v = ldb(a.q[m], lob + (unsigned int)wrap_node(node, dim) * sdb);
             } else if constexpr (PASS == PASS_GRAD) {
                 v = ldb(a.prim[m < 3 ? 1 + m : 5], od);
-            } else if constexpr (PASS == PASS_FLUX) {
-                // fcs(:,1..5) of convrsdcal6, src/solver.F90:2201-2207,
-                // with the off-diagonal metrics zero:
-                // F = J*(q_m*U + [m = d+1] dxi(d,d)*p), F5 = J*(q5+p)*U
+            } else if constexpr (PASS == PASS_RHS) {
+                // Inviscid flux fcs(:,1..5) of convrsdcal6 (src/solver.F90:
+                // 2201-2207) with the off-diagonal metrics zero:
+                // F = J*(q_m*U + [m = d+1] dxi(d,d)*p), F5 = J*(q5+p)*U.
+                // Viscous flux ff(:,2..5) of diffrsdcal6 (src/solver.F90:
+                // 2628-2639): the diagonal metric picks sigma(d,c) and
+                // qflux(d), whose halos are plain periodic copies.
+                // The derivative is linear, so the two passes of the
+                // reference (qrhs = -sum D F, then qrhs += sum D Fv) are one
+                // derivative of Fv - F per axis; only the summation order
+                // differs.
                 const double xn = a.xi[node + HM];
                 const double qm = ldb(a.q[m], od), jc = ldb(a.jac, od);
                 const double uu = xn * ldb(a.prim[1 + dir], od);
-                if (m == 0) v = jc * qm * uu;
-                else if (m == 4) v = jc * (qm + ldb(a.prim[4], od)) * uu;
-                else if (m == dir + 1) v = jc * (qm * uu + xn * ldb(a.prim[4], od));
-                else v = jc * (qm * uu);
-            } else if constexpr (PASS == PASS_VISC) {
-                // ff(:,2..5), src/solver.F90:2628-2639: the diagonal metric
-                // picks sigma(d,c) and qflux(d)
-                const unsigned int ow = lob + (unsigned int)wrap_node(node, dim) * sdb;
-                const double sv = m < 3 ? ldb(a.sig[m], ow) : ldb(a.qfl[0], ow);
-                v = (sv * a.xi[node + HM]) * ldb(a.jac, od);
+                double fi;
+                if (m == 0) fi = jc * qm * uu;
+                else if (m == 4) fi = jc * (qm + ldb(a.prim[4], od)) * uu;
+                else if (m == dir + 1) fi = jc * (qm * uu + xn * ldb(a.prim[4], od));
+                else fi = jc * (qm * uu);
+                if (m == 0) {
+                    v = -fi;
+                } else {
+                    const unsigned int ow = lob + (unsigned int)wrap_node(node, dim) * sdb;
+                    const double sv = m < 4 ? ldb(a.sig[m - 1], ow) : ldb(a.qfl[0], ow);
+                    v = (sv * xn) * jc - fi;
+                }
             } else {
                 v = ldb(a.fin, od);
             }
@@ -290,12 +294,26 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
                 } else if constexpr (PASS == PASS_GRAD) {
                     if (m < 3) stb(a.dvel[m * 3 + dir], od, o * a.xi[n + HM]);
                     else stb(a.dtmp[dir], od, o * a.xi[n + HM]);
-                } else if constexpr (PASS == PASS_FLUX) {
-                    if (dir == 0) stb(a.qrhs[m], od, o);
-                    else if (dir == 1) stb(a.qrhs[m], od, ldb(a.qrhs[m], od) + o);
-                    else stb(a.qrhs[m], od, -(ldb(a.qrhs[m], od) + o));  // qrhs=-qrhs
-                } else if constexpr (PASS == PASS_VISC) {
-                    stb(a.qrhs[m + 1], od, ldb(a.qrhs[m + 1], od) + o);
+                } else if constexpr (PASS == PASS_RHS) {
+                    if (XD) {
+                        stb(a.qrhs[m], od, o);
+                    } else if (a.rk_stage == 0) {
+                        stb(a.qrhs[m], od, ldb(a.qrhs[m], od) + o);
+                    } else {
+                        // last axis of the step: the RK3 combine of
+                        // src/mainloop.F90:423-445 for this quantity; the
+                        // workgroup owns these nodes of q_m
+                        const double r = ldb(a.qrhs[m], od) + o;
+                        const double jc = ldb(a.jac, od), qo = ldb(a.q[m], od);
+                        double qs;
+                        if (a.rk_stage == 1) {
+                            qs = qo * jc;
+                            stb(a.qsave[m], od, qs);
+                        } else {
+                            qs = ldb(a.qsave[m], od);
+                        }
+                        stb(a.q[m], od, (a.rka * qs + a.rkb * qo * jc + a.rkc * r * a.dt) / jc);
+                    }
                 } else {
                     stb(a.fout, od, o);
                 }
@@ -526,6 +544,22 @@ __global__ void k_rk(PtArgs a)
         qv[m] = vn / jc;
         a.q[m][gi] = qv[m];
     }
+    q2fvar(a, qv, gi);
+}
+
+// updatefvar (src/fludyna.F90:191-242) on the interior, once the last flux
+// sweep of the stage has applied the RK3 combine to q.
+__global__ void k_prim(PtArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ntot = (long)(a.g.n[0] + 1) * (a.g.n[1] + 1) * (a.g.n[2] + 1);
+    if (p >= ntot) return;
+    int i, j, k;
+    interior_ijk(a.g, p, i, j, k);
+    const long gi = gidx(a.g, i, j, k);
+    double qv[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) qv[m] = a.q[m][gi];
     q2fvar(a, qv, gi);
 }
 
@@ -769,6 +803,7 @@ SweepArgs sweep_args(astr_ctx* c, int dir, int pass)
     a.qfl[0] = c->qfl[dir];
     a.jac = c->jac;
     a.xi = c->xi + dir * c->xi_stride;
+    for (int m = 0; m < 5; ++m) a.qsave[m] = c->qsave[m];
     return a;
 }
 
@@ -802,8 +837,7 @@ int launch_sweep(astr_ctx* c, const SweepArgs& a)
     switch (a.pass) {
     case PASS_FILTER: return launch_sweep_p<PASS_FILTER>(c, a);
     case PASS_GRAD: return launch_sweep_p<PASS_GRAD>(c, a);
-    case PASS_FLUX: return launch_sweep_p<PASS_FLUX>(c, a);
-    case PASS_VISC: return launch_sweep_p<PASS_VISC>(c, a);
+    case PASS_RHS: return launch_sweep_p<PASS_RHS>(c, a);
     case PASS_DERIV_FIELD: return launch_sweep_p<PASS_DERIV_FIELD>(c, a);
     case PASS_FILTER_FIELD: return launch_sweep_p<PASS_FILTER_FIELD>(c, a);
     }
@@ -854,18 +888,39 @@ int do_qswap(astr_ctx* c)
     return ASTR_OK;
 }
 
-int do_rhscal(astr_ctx* c)
+// gradcal, then sigma/qflux, then the flux derivatives along x, y, z; the
+// z sweep either finishes qrhs (rk_stage 0) or applies the RK3 combine of
+// that stage to q (rk_stage 1..3), after which k_prim refreshes the
+// primitives.
+int do_grad_sigma(astr_ctx* c)
 {
-    for (int d = 0; d < 3; ++d) {
-        TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
-        TRY(launch_sweep(c, sweep_args(c, d, PASS_FLUX)));
-    }
-    TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
-    for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_VISC)));
-    return ASTR_OK;
+    for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
+    return launch_pt(c, k_sigma, n_interior(c), pt_args(c));
 }
 
 const double RKC[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+
+int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt)
+{
+    for (int d = 0; d < 3; ++d) {
+        SweepArgs a = sweep_args(c, d, PASS_RHS);
+        if (d == 2 && rk_stage > 0) {
+            a.rk_stage = rk_stage;
+            a.rka = RKC[rk_stage - 1][0];
+            a.rkb = RKC[rk_stage - 1][1];
+            a.rkc = RKC[rk_stage - 1][2];
+            a.dt = dt;
+        }
+        TRY(launch_sweep(c, a));
+    }
+    return ASTR_OK;
+}
+
+int do_rhscal(astr_ctx* c)
+{
+    TRY(do_grad_sigma(c));
+    return do_flux_sweeps(c, 0, 0.0);
+}
 
 int do_rk(astr_ctx* c, int stage, double dt)
 {
@@ -944,7 +999,8 @@ int time_mark(astr_ctx* c, int slot)
 
 int do_step(astr_ctx* c, double dt, double* stats, bool timed)
 {
-    // timing families: 0 filter, 1 qswap, 2 grad, 3 flux, 4 sigma, 5 visc, 6 rk
+    // timing families: 0 filter, 1 qswap, 2 grad, 3 sigma, 4 stats,
+    // 5 flux sweeps (with the RK3 combine), 6 primitive update
     for (int s = 1; s <= 3; ++s) {
         if (timed) TRY(time_mark(c, 0));
         TRY(do_filterq(c));
@@ -953,29 +1009,23 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
         if (timed) TRY(time_mark(c, 2));
         for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
         if (timed) TRY(time_mark(c, 3));
-        for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_FLUX)));
-        if (timed) TRY(time_mark(c, 4));
         TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
-        if (timed) TRY(time_mark(c, 5));
-        for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_VISC)));
-        if (timed) TRY(time_mark(c, 6));
+        if (timed) TRY(time_mark(c, 4));
+        // rkfirst's statcal sees the stage-1 state and its gradients
         if (s == 1 && stats) TRY(do_stats(c, stats));
-        if (timed) TRY(time_mark(c, 7));
-        TRY(do_rk(c, s, dt));
+        if (timed) TRY(time_mark(c, 5));
+        TRY(do_flux_sweeps(c, s, dt));
+        if (timed) TRY(time_mark(c, 6));
+        TRY(launch_pt(c, k_prim, n_interior(c), pt_args(c)));
         if (timed) {
-            TRY(time_mark(c, 8));
-            HIPCHK(hipEventSynchronize(c->ev[8]));
+            TRY(time_mark(c, 7));
+            HIPCHK(hipEventSynchronize(c->ev[7]));
             float ms = 0.f;
-            for (int f = 0; f < 6; ++f) {
+            for (int f = 0; f < 7; ++f) {
                 HIPCHK(hipEventElapsedTime(&ms, c->ev[f], c->ev[f + 1]));
                 c->tms[f] += ms;
-                c->tms[8] += ms;
+                c->tms[7] += ms;
             }
-            HIPCHK(hipEventElapsedTime(&ms, c->ev[7], c->ev[8]));
-            c->tms[6] += ms;
-            c->tms[8] += ms;
-            HIPCHK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
-            c->tms[7] += ms;
         }
     }
     return ASTR_OK;

@@ -131,7 +131,7 @@ class AstrSolver:
         check(self.lib.astr_set_timing(self.ctx, 1 if on else 0), "astr_set_timing")
 
     def timing(self) -> np.ndarray:
-        """ms per family: filter, qswap, grad, flux, sigma, visc, rk, stats, total."""
+        """ms per family: filter, qswap, grad, sigma, stats, flux, prim, total, 0."""
         out = np.zeros(9)
         check(self.lib.astr_get_timing(self.ctx, dptr(out)), "astr_get_timing")
         return out

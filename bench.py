@@ -26,24 +26,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-ALGO_BYTES_PER_PT_STEP = 4224.0  # 176 doubles per point per stage, 3 stages
-# algorithmic doubles per interior node per launch of each kernel (the
-# fields a launch must read and write once), averaged over its x, y, z
-# launches where they differ
+# Algorithmic doubles per interior node per launch of each kernel family:
+# the fields a launch must read and write once (averaged over its x, y, z
+# launches where they differ).
 FAMILY_DOUBLES = {
-    "filter": 10.0,        # q(1:5) in and out, per direction
-    "qswap": None,         # boundary planes only
-    "grad": 8.0,           # u, v, w, T in; dvel(.,d), dtmp(d) out
-    "flux": (13.0 + 18.0 + 18.0) / 3.0,  # q5, u_d, p, J in, qrhs out; y,z: +qrhs in
-    "sigma": 25.0,         # dvel 9, dtmp 3, T, u, v, w in; sigma 6, qflux 3 out
-    "visc": 13.0,          # sigma 3, qflux 1, J, qrhs 4 in; qrhs 4 out
-    "rk": 27.0,            # q, qrhs, qsave, J in; q, rho,u,v,w,p,T out
+    "filter": 10.0,   # q(1:5) in and out, per axis
+    "qswap": None,    # boundary planes only
+    "grad": 8.0,      # u, v, w, T in; dvel(.,d), dtmp(d) out
+    "sigma": 25.0,    # dvel 9, dtmp 3, T, u, v, w in; sigma 6, qflux 3 out
+    "stats": None,    # once per step (14 doubles per node), not per stage
+    # q 5, J, u_d, p, sigma(d,.) 3, qflux(d) in; x: qrhs out (17); y: qrhs
+    # in and out (22); z: qrhs, qsave in, q out (27; stage 1: qsave out)
+    "flux": (17.0 + 22.0 + 27.0) / 3.0,
+    "prim": 11.0,     # q 5 in; rho, u, v, w, p, T out
 }
 FAMILY_KERNEL = {"filter": "k_sweep<L,0> (filter)", "grad": "k_sweep<L,1> (gradient)",
-                 "flux": "k_sweep<L,2> (inviscid flux)", "sigma": "k_sigma",
-                 "visc": "k_sweep<L,3> (viscous flux)", "rk": "k_rk"}
-FAMILY_LAUNCHES = {"filter": 3, "qswap": 2, "grad": 3, "flux": 3, "sigma": 1, "visc": 3, "rk": 1}
-FAMILY_NAMES = ["filter", "qswap", "grad", "flux", "sigma", "visc", "rk"]
+                 "sigma": "k_sigma", "flux": "k_sweep<L,2> (flux derivatives, RK3 in z)",
+                 "prim": "k_prim"}
+FAMILY_LAUNCHES = {"filter": 3, "qswap": 2, "grad": 3, "sigma": 1, "stats": 0, "flux": 3,
+                   "prim": 1}
+FAMILY_NAMES = ["filter", "qswap", "grad", "sigma", "stats", "flux", "prim"]
+STATS_DOUBLES = 14.0
+# whole-step algorithmic bytes per node: three stages of the kernels above
+# plus the statistics once
+ALGO_BYTES_PER_PT_STEP = 8.0 * (3.0 * sum(FAMILY_DOUBLES[f] * FAMILY_LAUNCHES[f]
+                                          for f in FAMILY_NAMES if FAMILY_DOUBLES[f])
+                                + STATS_DOUBLES)
 
 
 def cpu_baseline(n: int, dt: float):
@@ -111,7 +119,8 @@ def main():
     nodes = float(n + 1) ** 3
     stages = 3 * args.steps
     fam_ms = {nm: float(fam[i]) for i, nm in enumerate(FAMILY_NAMES)}
-    dom = max((k for k in fam_ms if FAMILY_DOUBLES[k]), key=lambda k: fam_ms[k])
+    fam_ms["total"] = float(fam[7])
+    dom = max((k for k in FAMILY_NAMES if FAMILY_DOUBLES[k]), key=lambda k: fam_ms[k])
     launches = FAMILY_LAUNCHES[dom] * stages
     avg_s = fam_ms[dom] / 1e3 / launches
     algo = FAMILY_DOUBLES[dom] * 8.0 * nodes

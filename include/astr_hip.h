@@ -87,7 +87,9 @@ int astr_filterq(astr_ctx *ctx);
 /* call qswap, src/mainloop.F90:415 -> src/parallel.F90:4849-5319 */
 int astr_qswap(astr_ctx *ctx);
 /* qrhs=0 (src/mainloop.F90:404) then call rhscal (src/solver.F90:184-283:
- * gradcal, convrsdcal6, qrhs=-qrhs, diffrsdcal6) */
+ * gradcal, convrsdcal6, qrhs=-qrhs, diffrsdcal6).  The inviscid and viscous
+ * flux derivatives of one axis are taken as one derivative of their
+ * difference (the operator is linear; only the summation order differs). */
 int astr_rhscal(astr_ctx *ctx);
 /* RK3 combine of stage s (1..3) and updatefvar, src/mainloop.F90:423-476;
  * stage 1 also saves qsave = q*jacob. */
@@ -99,8 +101,10 @@ int astr_stats(astr_ctx *ctx, double *out3);
 /* --- whole steps: time_integration_rk, src/mainloop.F90:298-595 ------- */
 
 /* One RK3 step.  If stats is non-null it receives (ke, enstrophy,
- * dissipation) evaluated at stage 1 after rhscal, where rkfirst
- * (src/mainloop.F90:608-688) evaluates them. */
+ * dissipation) of the stage-1 state and velocity gradients, which rkfirst
+ * (src/mainloop.F90:608-688) evaluates after rhscal.  The step applies the
+ * RK3 combine inside the last flux sweep of each stage, so qrhs is left
+ * holding the x+y part; astr_rhscal gives the whole qrhs. */
 int astr_rk3_step(astr_ctx *ctx, double dt, double *stats);
 /* nsteps RK3 steps back to back; stats (nullable) receives 3 per step. */
 int astr_rk3_steps(astr_ctx *ctx, int32_t nsteps, double dt, double *stats);
@@ -121,11 +125,12 @@ int astr_deriv_field(astr_ctx *ctx, int32_t dir, const double *f, double *df);
  * f(-hm:...) haloed in, ff(0:im,0:jm,0:km) out.  Halos used as given. */
 int astr_filter_field(astr_ctx *ctx, int32_t dir, const double *f, double *ff);
 
-/* Kernel time of the last astr_rk3_steps call, in milliseconds, per kernel
- * family (HIP events on the solver's stream): [0] filter sweeps, [1] qswap,
- * [2] gradient sweeps, [3] inviscid-flux sweeps, [4] sigma/qflux,
- * [5] viscous sweeps, [6] RK update, [7] statistics, [8] total of 0-6; only
- * filled when timing was enabled with astr_set_timing(ctx, 1). */
+/* Kernel time of the astr_rk3_step(s) calls since timing was enabled with
+ * astr_set_timing(ctx, 1), in milliseconds, per kernel family (HIP events
+ * on the solver's stream): [0] filter sweeps, [1] qswap, [2] gradient
+ * sweeps, [3] sigma/qflux, [4] statistics, [5] flux sweeps (inviscid and
+ * viscous flux derivatives; the z sweep applies the RK3 combine),
+ * [6] primitive update, [7] total of 0-6, [8] zero. */
 int astr_set_timing(astr_ctx *ctx, int32_t on);
 int astr_get_timing(astr_ctx *ctx, double *ms9);
 

@@ -180,6 +180,36 @@ def test_steps_match_oracle():
     ob.close()
 
 
+def test_stage_pieces_reproduce_the_fused_step():
+    """The whole step applies the RK3 combine inside the last flux sweep;
+    driving the same stage through the separate entry points (filterq,
+    qswap, rhscal, rk3_update) does the same arithmetic in the same order.
+    The compiler may contract a multiply-add differently in the two
+    kernels, so the bound is 1e-14 of each field's magnitude rather than
+    bit equality (a wrong combine or a missed update is off by far more)."""
+    dims = (36, 34, 32)
+    _, s1, p = make_pair(dims)
+    _, s2, _ = make_pair(dims)
+    for k in range(2):
+        st1 = s1.time_integration_rk(0.004, stats=True)
+        for stage in (1, 2, 3):
+            s2.filterq()
+            s2.qswap()
+            s2.rhscal()
+            if stage == 1:
+                st2 = s2.statcal()
+            s2.rk3_update(stage, 0.004)
+        for x, y in zip(st1, st2):
+            assert abs(x - y) <= 1e-14 * abs(y), (k, st1, st2)
+        a, b = s1.download_state(), s2.download_state()
+        for nm in a:
+            e = nerr(a[nm], b[nm])
+            print(f"step {k} {nm}: {'bit-equal' if np.array_equal(a[nm], b[nm]) else e}")
+            assert e <= 1e-14, (k, nm, e)
+    s1.close()
+    s2.close()
+
+
 def test_non_cubic_steps_match_oracle():
     dims = (48, 40, 36)
     ob, s, p = make_pair(dims)
