@@ -99,6 +99,56 @@ void orc_box_stats(orc_box *b, double *stats);
 
 int orc_omp_threads(void);
 
+/* ---------------- explicit upwind convection path (upwind.c) ---------- */
+
+typedef struct {
+    int im, jm, km;      /* km = 0: two-dimensional box */
+    int npdc[3];         /* closure type of each axis, 1..4 */
+    int recon_schem;     /* -1, 0 SUW, 1 WENO, 2 WENO-Z, 3 MP, 5 MP-LD, 6 ROUND */
+    int lchardecomp;     /* characteristic projection where the shock flag is set */
+    double bfacmpld;     /* MP-LD dissipation weight */
+    double shkcrt;       /* Ducros sensor threshold */
+    double gamma, mach;
+} orc_uwd;
+
+double orc_minmod2(double a, double b);
+double orc_minmod4(double a, double b, double c, double d);
+double orc_suw3(const double *u);
+double orc_suw5(const double *u);
+double orc_suw7(const double *u);
+double orc_mp5(const double *u, const double *ul, int discont);
+double orc_mp7(const double *u);
+double orc_mp5ld(const double *u, double weightbw);
+double orc_mp7ld(const double *u, double weightbw, int lskt);
+double orc_weno5(const double *u);
+double orc_weno5z(const double *u);
+double orc_weno7(const double *u);
+double orc_weno7z(const double *u);
+double orc_round(const double *u);
+double orc_recons_exp(const double *f, int inode, int dim, int ntype, int reschem, int shock,
+                      double bfacmpld);
+void orc_sw_split(double gamma, double mach, double rho, const double *vel, double prs,
+                  double tmp, const double *q, const double *dxi, double jacob, double *fp,
+                  double *fm);
+int orc_chardecomp(double gamma, double tmp_l, double ro_l, double p_l, double e_l,
+                   const double *vel_l, const double *ddi_l, double tmp_r, double ro_r,
+                   double p_r, double e_r, const double *vel_r, const double *ddi_r, double *R,
+                   double *Lv);
+int orc_iwind8(int i, int n, int imin, int imax, int plus);
+void orc_uwd_range(int ntype, int n, int *lo, int *hi);
+int orc_ndims(const orc_uwd *u);
+void orc_gradcal(const orc_uwd *u, int explicit_fd, const double *vel, const double *tmp,
+                 const double *dxi, double *dvel, double *dtmp);
+void orc_ducros(const orc_uwd *u, const double *dvel, const double *prs, double *ssf,
+                unsigned char *lshock);
+int orc_convrsduwd(const orc_uwd *u, const double *q, const double *rho, const double *vel,
+                   const double *prs, const double *tmp, const double *dxi, const double *jacob,
+                   const unsigned char *lshock, double *qrhs);
+int orc_uwd_rhscal(const orc_uwd *u, int explicit_fd, const double *q, const double *rho,
+                   const double *vel, const double *prs, const double *tmp, const double *dxi,
+                   const double *jacob, double *qrhs, double *dvel, double *dtmp, double *ssf,
+                   unsigned char *lshock);
+
 #ifdef __cplusplus
 }
 #endif

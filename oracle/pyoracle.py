@@ -247,3 +247,186 @@ class OracleBox:
         out["prs"][I] = self.get("prs")
         out["tmp"][I] = self.get("tmp")
         return out
+
+
+# ----------------------------------------------------- explicit upwind path
+
+class UwdCfg(ctypes.Structure):
+    """Mirror of orc_uwd (oracle.h)."""
+    _fields_ = [("im", ctypes.c_int), ("jm", ctypes.c_int), ("km", ctypes.c_int),
+                ("npdc", ctypes.c_int * 3), ("recon_schem", ctypes.c_int),
+                ("lchardecomp", ctypes.c_int), ("bfacmpld", ctypes.c_double),
+                ("shkcrt", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("mach", ctypes.c_double)]
+
+
+_uwd_declared = False
+
+
+def uwd_lib():
+    global _uwd_declared
+    lib = oracle_lib()
+    if not _uwd_declared:
+        for n in ("orc_suw3", "orc_suw5", "orc_suw7", "orc_mp7", "orc_weno5", "orc_weno5z",
+                  "orc_weno7", "orc_weno7z", "orc_round"):
+            getattr(lib, n).restype = _F
+            getattr(lib, n).argtypes = [_D]
+        lib.orc_mp5.restype = _F
+        lib.orc_mp5.argtypes = [_D, _D, _I]
+        lib.orc_mp5ld.restype = _F
+        lib.orc_mp5ld.argtypes = [_D, _F]
+        lib.orc_mp7ld.restype = _F
+        lib.orc_mp7ld.argtypes = [_D, _F, _I]
+        lib.orc_minmod2.restype = _F
+        lib.orc_minmod2.argtypes = [_F, _F]
+        lib.orc_minmod4.restype = _F
+        lib.orc_minmod4.argtypes = [_F, _F, _F, _F]
+        lib.orc_recons_exp.restype = _F
+        lib.orc_recons_exp.argtypes = [_D, _I, _I, _I, _I, _I, _F]
+        lib.orc_sw_split.argtypes = [_F, _F, _F, _D, _F, _F, _D, _D, _F, _D, _D]
+        lib.orc_chardecomp.restype = _I
+        lib.orc_chardecomp.argtypes = [_F, _F, _F, _F, _F, _D, _D, _F, _F, _F, _F, _D, _D, _D,
+                                       _D]
+        lib.orc_iwind8.restype = _I
+        lib.orc_iwind8.argtypes = [_I, _I, _I, _I, _I]
+        P = ctypes.POINTER(UwdCfg)
+        U8 = ctypes.POINTER(ctypes.c_ubyte)
+        lib.orc_gradcal.argtypes = [P, _I, _D, _D, _D, _D, _D]
+        lib.orc_ducros.argtypes = [P, _D, _D, _D, U8]
+        lib.orc_convrsduwd.restype = _I
+        lib.orc_convrsduwd.argtypes = [P, _D, _D, _D, _D, _D, _D, _D, U8, _D]
+        lib.orc_uwd_rhscal.restype = _I
+        lib.orc_uwd_rhscal.argtypes = [P, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, U8]
+        _uwd_declared = True
+    return lib
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def recon(name: str, u, *extra) -> float:
+    """One scalar reconstruction of src/flux.F90 (suw3/5/7, mp5, mp7, mp5ld,
+    mp7ld, weno5/5z/7/7z, round)."""
+    lib = uwd_lib()
+    u = _c(u)
+    fn = getattr(lib, "orc_" + name)
+    if name == "mp5":
+        return fn(_p(u), None, -1)
+    return fn(_p(u), *extra)
+
+
+def recons_exp(f, inode, dim, ntype, reschem, shock=True, bfacmpld=0.7) -> float:
+    f = _c(f)
+    return uwd_lib().orc_recons_exp(_p(f), inode, dim, ntype, reschem, int(shock), bfacmpld)
+
+
+def sw_split(rho, vel, prs, tmp, q, dxi, jacob, gamma=1.4, mach=0.1):
+    """(F+, F-) of flux_steger_warming at one node."""
+    fp = np.zeros(5)
+    fm = np.zeros(5)
+    uwd_lib().orc_sw_split(gamma, mach, rho, _p(_c(vel)), prs, tmp, _p(_c(q)), _p(_c(dxi)),
+                           jacob, _p(fp), _p(fm))
+    return fp, fm
+
+
+def chardecomp(left, right, gamma=1.4):
+    """REV, LEV (5x5, REV[m-1, n-1] = REV(m, n)) of chardecomp from two
+    states given as dicts with tmp, rho, prs, E, vel(3), ddi(3)."""
+    R = np.zeros(25)
+    L = np.zeros(25)
+    rc = uwd_lib().orc_chardecomp(gamma, left["tmp"], left["rho"], left["prs"], left["E"],
+                                  _p(_c(left["vel"])), _p(_c(left["ddi"])), right["tmp"],
+                                  right["rho"], right["prs"], right["E"], _p(_c(right["vel"])),
+                                  _p(_c(right["ddi"])), _p(R), _p(L))
+    if rc:
+        raise ValueError("zero metric normal")
+    return R.reshape(5, 5), L.reshape(5, 5)
+
+
+class UwdBox:
+    """The explicit upwind right-hand side on one haloed box, host layout
+    (Fortran order, halos hm = 5 on every axis; km = 0 is a 2-D box)."""
+
+    def __init__(self, dims, npdc=(3, 3, 3), recon_schem=3, lchardecomp=True, bfacmpld=0.7,
+                 shkcrt=0.01, gamma=1.4, mach=1.0):
+        im, jm, km = dims
+        self.cfg = UwdCfg(im, jm, km, (ctypes.c_int * 3)(*npdc), recon_schem, int(lchardecomp),
+                          bfacmpld, shkcrt, gamma, mach)
+        self.dims = (im, jm, km)
+        self.hshape = (im + 1 + 2 * HM, jm + 1 + 2 * HM, km + 1 + 2 * HM)
+        self.ishape = (im + 1, jm + 1, km + 1)
+
+    def _h(self, a, n=None):
+        shape = self.hshape + ((n,) if n else ())
+        a = np.asarray(a, dtype=np.float64)
+        assert a.shape == shape, (a.shape, shape)
+        return np.asfortranarray(a)
+
+    def rhscal(self, st, dxi, jacob, explicit_fd=True):
+        """qrhs, dvel, dtmp, ssf, lshock of the upwind rhscal for state st
+        (dict q, rho, vel, prs, tmp; haloed)."""
+        lib = uwd_lib()
+        q = self._h(st["q"], 5)
+        vel = self._h(st["vel"], 3)
+        rho, prs, tmp = (self._h(st[k]) for k in ("rho", "prs", "tmp"))
+        dx = self._h(dxi, 9) if np.asarray(dxi).ndim == 4 else np.asfortranarray(
+            np.asarray(dxi).reshape(self.hshape + (9,), order="F"))
+        jac = self._h(jacob)
+        qrhs = np.zeros(self.ishape + (5,), order="F")
+        dvel = np.zeros(self.ishape + (9,), order="F")
+        dtmp = np.zeros(self.ishape + (3,), order="F")
+        ssf = np.zeros(self.hshape, order="F")
+        lsh = np.zeros(self.ishape, dtype=np.uint8, order="F")
+        rc = lib.orc_uwd_rhscal(ctypes.byref(self.cfg), int(explicit_fd), _p(q), _p(rho),
+                                _p(vel), _p(prs), _p(tmp), _p(dx), _p(jac), _p(qrhs),
+                                _p(dvel), _p(dtmp), _p(ssf),
+                                lsh.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+        if rc:
+            raise ValueError(f"orc_uwd_rhscal: unsupported configuration ({rc})")
+        return {"qrhs": qrhs, "dvel": dvel.reshape(self.ishape + (3, 3), order="F"),
+                "dtmp": dtmp, "ssf": ssf, "lshock": lsh.astype(bool)}
+
+    def convrsduwd(self, st, dxi, jacob, lshock=None):
+        """qrhs accumulated by convrsduwd alone (from zero, not negated)."""
+        lib = uwd_lib()
+        q = self._h(st["q"], 5)
+        vel = self._h(st["vel"], 3)
+        rho, prs, tmp = (self._h(st[k]) for k in ("rho", "prs", "tmp"))
+        dx = np.asfortranarray(np.asarray(dxi, dtype=np.float64).reshape(self.hshape + (9,),
+                                                                         order="F"))
+        jac = self._h(jacob)
+        qrhs = np.zeros(self.ishape + (5,), order="F")
+        ls = None
+        if lshock is not None:
+            lsa = np.asfortranarray(np.asarray(lshock, dtype=np.uint8))
+            ls = lsa.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
+        rc = lib.orc_convrsduwd(ctypes.byref(self.cfg), _p(q), _p(rho), _p(vel), _p(prs),
+                                _p(tmp), _p(dx), _p(jac), ls, _p(qrhs))
+        if rc:
+            raise ValueError(f"orc_convrsduwd: unsupported configuration ({rc})")
+        return qrhs
+
+    def ducros(self, dvel, prs):
+        lib = uwd_lib()
+        dv = np.asfortranarray(np.asarray(dvel, dtype=np.float64).reshape(self.ishape + (9,),
+                                                                          order="F"))
+        p = self._h(prs)
+        ssf = np.zeros(self.hshape, order="F")
+        lsh = np.zeros(self.ishape, dtype=np.uint8, order="F")
+        lib.orc_ducros(ctypes.byref(self.cfg), _p(dv), _p(p), _p(ssf),
+                       lsh.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+        return ssf, lsh.astype(bool)
+
+
+def prim_to_state(rho, u, v, w, p, gamma=1.4, mach=1.0):
+    """Conservative q and T from primitives (nondimensional EoS of
+    src/fludyna.F90:449-533: const2 = gamma Ma^2, const6 = 1/(gamma-1))."""
+    const2 = gamma * mach * mach
+    const6 = 1.0 / (gamma - 1.0)
+    q = np.stack([rho, rho * u, rho * v, rho * w,
+                  p * const6 + 0.5 * rho * (u * u + v * v + w * w)], axis=-1)
+    tmp = p / rho * const2
+    return {"q": np.asfortranarray(q), "rho": np.asfortranarray(rho),
+            "vel": np.asfortranarray(np.stack([u, v, w], axis=-1)),
+            "prs": np.asfortranarray(p), "tmp": np.asfortranarray(tmp)}

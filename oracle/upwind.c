@@ -1,0 +1,821 @@
+/* Checker for the explicit upwind convection path of ASTR: a plain C
+ * restatement of the reference routines, cited file:line, used only by the
+ * tests and the CPU baseline.  Arithmetic follows the Fortran expression
+ * order (left to right, parentheses as written), compiled without FMA
+ * contraction, so the HIP kernels (built with contraction off for this
+ * path) can be compared bit for bit.
+ *
+ * Layouts (host, the reference's own):
+ *   haloed field f(-hm:im+hm,-hm:jm+hm,-hm:km+hm), i fastest;
+ *   q(...,1:5), vel(...,1:3): 5 / 3 haloed fields back to back;
+ *   dxi(...,1:3,1:3): 9 haloed fields, field (d-1)+3*(n-1) = dxi(.,d,n);
+ *   qrhs(0:im,0:jm,0:km,1:5), dvel(0:im,..,1:3,1:3), dtmp(..,1:3);
+ *   lshock(0:im,0:jm,0:km) as bytes.
+ * km = 0 is a two-dimensional box (ndims = 2): no k sweeps, k halos of
+ * hm planes on each side as the reference allocates them.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+#define HMU 5
+
+/* ---------------------------------------------------------------- scalars */
+
+/* minmod2, minmod4: src/commfunc.F90:699-738 */
+double orc_minmod2(double a, double b)
+{
+    if (a > 0.0 && b > 0.0) return fmin(fabs(a), fabs(b));
+    if (a < 0.0 && b < 0.0) return -1.0 * fmin(fabs(a), fabs(b));
+    return 0.0;
+}
+
+static double min2(double a, double b) { return a < b ? a : b; }
+static double max2(double a, double b) { return a > b ? a : b; }
+
+double orc_minmod4(double a, double b, double c, double d)
+{
+    if (a > 0.0 && b > 0.0 && c > 0.0 && d > 0.0)
+        return min2(min2(min2(fabs(a), fabs(b)), fabs(c)), fabs(d));
+    if (a < 0.0 && b < 0.0 && c < 0.0 && d < 0.0)
+        return -1.0 * min2(min2(min2(fabs(a), fabs(b)), fabs(c)), fabs(d));
+    return 0.0;
+}
+
+/* suw3/suw5/suw7, src/flux.F90:388-422 (constdef num1d6, num5d6, num1d3) */
+double orc_suw3(const double *u)
+{
+    return -(1.0 / 6.0) * u[0] + (5.0 / 6.0) * u[1] + (1.0 / 3.0) * u[2];
+}
+
+double orc_suw5(const double *u)
+{
+    return (2.0 * u[0] - 13.0 * u[1] + 47.0 * u[2] + 27.0 * u[3] - 3.0 * u[4]) / 60.0;
+}
+
+double orc_suw7(const double *u)
+{
+    return (-3.0 * u[0] + 25.0 * u[1] - 101.0 * u[2] + 319.0 * u[3] + 214.0 * u[4] -
+            38.0 * u[5] + 4.0 * u[6]) / 420.0;
+}
+
+/* The monotonicity-preserving limiter shared by MP5/MP7/MP5LD/MP7LD
+ * (Suresh & Huynh 1997), src/flux.F90:456-490: u[0..4] are the points
+ * u(1..5) of MP5 (u(i-2..i+2)), ul the linear value. */
+static double mp_limit(const double *u, double ulinear)
+{
+    const double dm1 = u[0] - 2.0 * u[1] + u[2];
+    const double d0 = u[1] - 2.0 * u[2] + u[3];
+    const double d1 = u[2] - 2.0 * u[3] + u[4];
+    const double dhm1 = orc_minmod4(4.0 * dm1 - d0, 4.0 * d0 - dm1, dm1, d0);
+    const double dh0 = orc_minmod4(4.0 * d0 - d1, 4.0 * d1 - d0, d0, d1);
+    const double uUL = u[2] + 4.0 * (u[2] - u[1]);
+    const double uAV = 0.5 * (u[2] + u[3]);
+    const double uMD = uAV - 0.5 * dh0;
+    const double uLC = u[2] + 0.5 * (u[2] - u[1]) + 1.333333333333333 * dhm1;
+    double v1 = min2(min2(u[2], u[3]), uMD);
+    double v2 = min2(min2(u[2], uUL), uLC);
+    const double uMIN = max2(v1, v2);
+    v1 = max2(max2(u[2], u[3]), uMD);
+    v2 = max2(max2(u[2], uUL), uLC);
+    const double uMAX = min2(v1, v2);
+    return ulinear + orc_minmod2(uMIN - ulinear, uMAX - ulinear);
+}
+
+/* MP5, src/flux.F90:434-496.  ul < 0 pointer: linear value from suw5;
+ * discont < 0: argument absent (limiter allowed). */
+double orc_mp5(const double *u, const double *ul, int discont)
+{
+    const double ulinear = ul ? *ul : orc_suw5(u);
+    const int lskt = discont < 0 ? 1 : discont;
+    const double uMP = u[2] + orc_minmod2(u[3] - u[2], 4.0 * (u[2] - u[1]));
+    const double v = (ulinear - u[2]) * (ulinear - uMP);
+    if (lskt && v >= 1.e-10) return mp_limit(u, ulinear);
+    return ulinear;
+}
+
+/* MP7, src/flux.F90:498-551: u(0:6), the limiter on u(1:5) */
+double orc_mp7(const double *u)
+{
+    const double ulinear = (-3.0 * u[0] + 25.0 * u[1] - 101.0 * u[2] + 319.0 * u[3] +
+                            214.0 * u[4] - 38.0 * u[5] + 4.0 * u[6]) / 420.0;
+    const double uMP = u[3] + orc_minmod2(u[4] - u[3], 4.0 * (u[3] - u[2]));
+    const double v = (ulinear - u[3]) * (ulinear - uMP);
+    if (v >= 1.e-10) return mp_limit(u + 1, ulinear);
+    return ulinear;
+}
+
+/* MP5LD, src/flux.F90:556-633: u(1:6) */
+double orc_mp5ld(const double *u, double weightbw)
+{
+    const double vadp = 1.666666666666667e-2 * weightbw;
+    const double b1 = -vadp + 3.333333333333333e-2;
+    const double b2 = 5.0 * vadp - 2.166666666666667e-1;
+    const double b3 = -10.0 * vadp + 7.833333333333333e-1;
+    const double b4 = 10.0 * vadp + 0.45;
+    const double b5 = -5.0 * vadp - 5.e-2;
+    const double b6 = vadp;
+    const double ulinear = b1 * u[0] + b2 * u[1] + b3 * u[2] + b4 * u[3] + b5 * u[4] + b6 * u[5];
+    const double uMP = u[2] + orc_minmod2(u[3] - u[2], 4.0 * (u[2] - u[1]));
+    const double v = (ulinear - u[2]) * (ulinear - uMP);
+    if (v >= 1.e-10) return mp_limit(u, ulinear);
+    return ulinear;
+}
+
+/* MP7LD, src/flux.F90:635-706: u(0:7); the limiter is switched by the
+ * shock flag alone */
+double orc_mp7ld(const double *u, double weightbw, int lskt)
+{
+    const double vadp = 3.571428571428571e-3 * weightbw;
+    double b[8];
+    b[0] = 1.0 * vadp - 7.142857142857143e-3;
+    b[1] = -7.0 * vadp + 5.952380952380952e-2;
+    b[2] = 21.0 * vadp - 0.240476190476190;
+    b[3] = -35.0 * vadp + 0.759523809523809;
+    b[4] = 35.0 * vadp + 0.509523809523809;
+    b[5] = -21.0 * vadp - 9.047619047619047e-2;
+    b[6] = 7.0 * vadp + 9.523809523809525e-3;
+    b[7] = -1.0 * vadp;
+    const double ulinear = b[0] * u[0] + b[1] * u[1] + b[2] * u[2] + b[3] * u[3] +
+                           b[4] * u[4] + b[5] * u[5] + b[6] * u[6] + b[7] * u[7];
+    if (lskt) return mp_limit(u + 1, ulinear);
+    return ulinear;
+}
+
+/* WENO5, src/flux.F90:761-821 */
+static void weno5_parts(const double *u, double *uh, double *bt)
+{
+    uh[0] = 0.333333333333333 * u[0] - 1.16666666666667 * u[1] + 1.83333333333333 * u[2];
+    uh[1] = -0.166666666666667 * u[1] + 0.833333333333333 * u[2] + 0.333333333333333 * u[3];
+    uh[2] = 0.333333333333333 * u[2] + 0.833333333333333 * u[3] - 0.166666666666667 * u[4];
+    double a = u[0] - 2.0 * u[1] + u[2], b = u[0] - 4.0 * u[1] + 3.0 * u[2];
+    bt[0] = 1.08333333333333 * (a * a) + 0.25 * (b * b);
+    a = u[1] - 2.0 * u[2] + u[3];
+    b = u[1] - u[3];
+    bt[1] = 1.08333333333333 * (a * a) + 0.25 * (b * b);
+    a = u[2] - 2.0 * u[3] + u[4];
+    b = 3.0 * u[2] - 4.0 * u[3] + u[4];
+    bt[2] = 1.08333333333333 * (a * a) + 0.25 * (b * b);
+}
+
+double orc_weno5(const double *u)
+{
+    double uh[3], bt[3];
+    weno5_parts(u, uh, bt);
+    const double eps = 1.e-6;
+    double e;
+    e = bt[0] + eps; const double a1 = 0.1 / (e * e);
+    e = bt[1] + eps; const double a2 = 0.6 / (e * e);
+    e = bt[2] + eps; const double a3 = 0.3 / (e * e);
+    const double as = a1 + a2 + a3;
+    return a1 / as * uh[0] + a2 / as * uh[1] + a3 / as * uh[2];
+}
+
+/* WENO5Z, src/flux.F90:936-994 */
+double orc_weno5z(const double *u)
+{
+    double uh[3], bt[3];
+    weno5_parts(u, uh, bt);
+    const double eps = 1.e-6;
+    const double tau5 = fabs(bt[2] - bt[0]);
+    double e;
+    e = bt[0] + eps; const double a1 = 0.1 + 0.1 * (tau5 / (e * e));
+    e = bt[1] + eps; const double a2 = 0.6 + 0.6 * (tau5 / (e * e));
+    e = bt[2] + eps; const double a3 = 0.3 + 0.3 * (tau5 / (e * e));
+    const double as = a1 + a2 + a3;
+    return a1 / as * uh[0] + a2 / as * uh[1] + a3 / as * uh[2];
+}
+
+/* WENO7, src/flux.F90:826-925 */
+static void weno7_parts(const double *u, double *uh, double *bt)
+{
+    uh[0] = -0.25 * u[0] + 1.083333333333333 * u[1] - 1.916666666666667 * u[2] +
+            2.083333333333333 * u[3];
+    uh[1] = 8.333333333333333e-2 * u[1] - 4.166666666666667e-1 * u[2] +
+            1.083333333333333 * u[3] + 0.25 * u[4];
+    uh[2] = -8.333333333333333e-2 * u[2] + 5.833333333333333e-1 * u[3] +
+            5.833333333333333e-1 * u[4] - 8.333333333333333e-2 * u[5];
+    uh[3] = 0.25 * u[3] + 1.083333333333333 * u[4] - 4.166666666666667e-1 * u[5] +
+            8.333333333333333e-2 * u[6];
+    double s1, s2, s3;
+    s1 = -2.0 * u[0] + 9.0 * u[1] - 18.0 * u[2] + 11.0 * u[3];
+    s2 = -1.0 * u[0] + 4.0 * u[1] - 5.0 * u[2] + 2.0 * u[3];
+    s3 = -1.0 * u[0] + 3.0 * u[1] - 3.0 * u[2] + 1.0 * u[3];
+    bt[0] = 2.777777777777778e-2 * (s1 * s1) + 1.083333333333333 * (s2 * s2) +
+            1.084722222222222 * (s3 * s3);
+    s1 = u[1] - 6.0 * u[2] + 3.0 * u[3] + 2.0 * u[4];
+    s2 = u[2] - 2.0 * u[3] + u[4];
+    s3 = -1.0 * u[1] + 3.0 * u[2] - 3.0 * u[3] + 1.0 * u[4];
+    bt[1] = 2.777777777777778e-2 * (s1 * s1) + 1.083333333333333 * (s2 * s2) +
+            1.084722222222222 * (s3 * s3);
+    s1 = -2.0 * u[2] - 3.0 * u[3] + 6.0 * u[4] - 1.0 * u[5];
+    s2 = u[2] - 2.0 * u[3] + u[4];
+    s3 = -1.0 * u[2] + 3.0 * u[3] - 3.0 * u[4] + 1.0 * u[5];
+    bt[2] = 2.777777777777778e-2 * (s1 * s1) + 1.083333333333333 * (s2 * s2) +
+            1.084722222222222 * (s3 * s3);
+    s1 = -11.0 * u[3] + 18.0 * u[4] - 9.0 * u[5] + 2.0 * u[6];
+    s2 = 2.0 * u[3] - 5.0 * u[4] + 4.0 * u[5] - u[6];
+    s3 = -1.0 * u[3] + 3.0 * u[4] - 3.0 * u[5] + 1.0 * u[6];
+    bt[3] = 2.777777777777778e-2 * (s1 * s1) + 1.083333333333333 * (s2 * s2) +
+            1.084722222222222 * (s3 * s3);
+}
+
+static const double C7[4] = {2.857142857142857e-2, 3.428571428571429e-1, 5.142857142857143e-1,
+                             1.142857142857143e-1};
+
+double orc_weno7(const double *u)
+{
+    double uh[4], bt[4], a[4];
+    weno7_parts(u, uh, bt);
+    for (int k = 0; k < 4; ++k) {
+        const double e = bt[k] + 1.e-6;
+        a[k] = C7[k] / (e * e);
+    }
+    const double as = a[0] + a[1] + a[2] + a[3];
+    return a[0] / as * uh[0] + a[1] / as * uh[1] + a[2] / as * uh[2] + a[3] / as * uh[3];
+}
+
+/* WENO7Z, src/flux.F90:999-1050 */
+double orc_weno7z(const double *u)
+{
+    double uh[4], bt[4], a[4];
+    weno7_parts(u, uh, bt);
+    const double tau7 = fabs(bt[3] - bt[0]);
+    for (int k = 0; k < 4; ++k) {
+        const double e = bt[k] + 1.e-6;
+        a[k] = C7[k] + C7[k] * (tau7 / (e * e));
+    }
+    const double as = a[0] + a[1] + a[2] + a[3];
+    return a[0] / as * uh[0] + a[1] / as * uh[1] + a[2] / as * uh[2] + a[3] / as * uh[3];
+}
+
+/* ROUND, src/flux.F90:717-752 */
+double orc_round(const double *u)
+{
+    const double eps = 1.e-16;
+    const double z0c = (u[1] - u[0] + eps) / (u[2] - u[0] + eps);
+    const double a1c = 1.0 + 12.0 * z0c * z0c;
+    const double a2c = 1.0 + 5.0 * (z0c - 1.0) * (z0c - 1.0);
+    const double zl = z0c - 0.05, zr = 0.47 - z0c;
+    const double plc = 1100.0 * zl * zl * zl * zr * zr * zr;
+    const double zp = z0c - 0.55, zq = 0.97 - z0c;
+    const double prc = 18000.0 * zp * zp * zp * zq * zq * zq * zq * zq;
+    const double p1c = 0.833333333333333 * z0c + 0.333333333333333 + max2(plc, 0.0) + max2(prc, 0.0);
+    const double p2c = 1.5 * z0c;
+    const double p3c = 0.5 * z0c + 0.5;
+    const double wc1c = 1.0 / a1c / a1c / a1c / a1c;
+    const double wc2c = 1.0 / a2c / a2c / a2c / a2c / a2c / a2c / a2c / a2c;
+    const double gc = (p1c * (1.0 - wc1c) + p2c * wc1c) * (1.0 - wc2c) + p3c * wc2c;
+    return gc * (u[2] - u[0]) + u[0];
+}
+
+/* recons_exp, src/flux.F90:269-350: f(1:8) of the interface inode+1/2,
+ * degraded near ntype 1/2 boundaries.  Returns NAN for an undefined
+ * scheme (the reference stops). */
+double orc_recons_exp(const double *f, int inode, int dim, int ntype, int reschem, int shock,
+                      double bfacmpld)
+{
+    if ((ntype == 1 && inode == 0) || (ntype == 2 && inode == dim - 1))
+        return reschem == -1 ? f[3] : 0.5 * (f[3] + f[4]);
+    if ((ntype == 1 && inode == 1) || (ntype == 2 && inode == dim - 2))
+        return reschem == -1 ? f[3] : orc_suw3(f + 2);
+    if ((ntype == 1 && inode == 2) || (ntype == 2 && inode == dim - 3)) {
+        switch (reschem) {
+        case -1: return f[3];
+        case 0: return orc_suw5(f + 1);
+        case 1: return orc_weno5(f + 1);
+        case 2: return orc_weno5z(f + 1);
+        case 3: return orc_mp5(f + 1, NULL, -1);
+        case 5: return orc_mp5ld(f + 1, bfacmpld);
+        case 6: return orc_round(f + 2);
+        }
+        return NAN;
+    }
+    switch (reschem) {
+    case -1: return f[3];
+    case 0: return orc_suw7(f);
+    case 1: return orc_weno7(f);
+    case 2: return orc_weno7z(f);
+    case 3: return orc_mp7(f);
+    case 5: return orc_mp7ld(f, bfacmpld, shock);
+    case 6: return orc_round(f + 2);
+    }
+    return NAN;
+}
+
+/* ------------------------------------------------------- flux splitting */
+
+/* flux_steger_warming at one node, src/riemann.F90:23-161 (no species),
+ * sound speed sos = sqrt(T)/Ma (src/fludyna.F90:832-859, nondimen). */
+void orc_sw_split(double gamma, double mach, double rho, const double *vel, double prs,
+                  double tmp, const double *q, const double *dxi, double jacob, double *fp,
+                  double *fm)
+{
+    const double eps = 0.04;
+    const double uu = dxi[0] * vel[0] + dxi[1] * vel[1] + dxi[2] * vel[2];
+    const double var0 = 1.0 / sqrt(dxi[0] * dxi[0] + dxi[1] * dxi[1] + dxi[2] * dxi[2]);
+    const double gpd[3] = {dxi[0] * var0, dxi[1] * var0, dxi[2] * var0};
+    const double gm2 = 0.5 / gamma;
+    const double css = sqrt(tmp) / mach;
+    const double csa = css / var0;
+    const double lmach = uu / csa;
+    const double lmda[5] = {uu, uu, uu, uu + csa, uu - csa};
+    double lp[5], lm[5];
+    for (int k = 0; k < 5; ++k) {
+        lp[k] = 0.5 * (lmda[k] + sqrt(lmda[k] * lmda[k] + eps * eps));
+        lm[k] = lmda[k] - lp[k];
+    }
+    if (lmach >= 1.0 || lmach <= -1.0) {
+        double *f = lmach >= 1.0 ? fp : fm, *z = lmach >= 1.0 ? fm : fp;
+        f[0] = jacob * q[0] * uu;
+        f[1] = jacob * (q[1] * uu + dxi[0] * prs);
+        f[2] = jacob * (q[2] * uu + dxi[1] * prs);
+        f[3] = jacob * (q[3] * uu + dxi[2] * prs);
+        f[4] = jacob * (q[4] + prs) * uu;
+        for (int k = 0; k < 5; ++k) z[k] = 0.0;
+        return;
+    }
+    const double fhi = 0.5 * (gamma - 1.0) * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+    const double jro = jacob * rho;
+    for (int s = 0; s < 2; ++s) {
+        const double *l = s == 0 ? lp : lm;
+        double *f = s == 0 ? fp : fm;
+        const double var1 = l[0];
+        const double var2 = l[3] - l[4];
+        const double var3 = 2.0 * l[0] - l[3] - l[4];
+        const double var4 = var1 - var3 * gm2;
+        f[0] = jro * var4;
+        f[1] = jro * (var4 * vel[0] + var2 * css * gpd[0] * gm2);
+        f[2] = jro * (var4 * vel[1] + var2 * css * gpd[1] * gm2);
+        f[3] = jro * (var4 * vel[2] + var2 * css * gpd[2] * gm2);
+        f[4] = jacob * (var1 * q[4] + rho * (var2 * uu * var0 * css * gm2 -
+                                             var3 * (fhi + css * css) * gm2 / (gamma - 1.0)));
+    }
+}
+
+/* chardecomp, src/solver.F90:1964-2163 (no species): Roe-averaged left
+ * and right eigenvectors along the mean metric normal.  LEV, REV are
+ * row-major 5x5, L[m][n] = LEV(m+1,n+1).  Returns 1 when the normal is
+ * zero (the reference stops). */
+int orc_chardecomp(double gamma, double tmp_l, double ro_l, double p_l, double e_l,
+                   const double *vel_l, const double *ddi_l, double tmp_r, double ro_r,
+                   double p_r, double e_r, const double *vel_r, const double *ddi_r, double *R,
+                   double *Lv)
+{
+    (void)tmp_l;
+    (void)tmp_r;
+    const double rero = 1.e-12;
+#define LEV(a, b) Lv[((a)-1) * 5 + (b)-1]
+#define REV(a, b) R[((a)-1) * 5 + (b)-1]
+    const double sl = sqrt(ro_l), sr = sqrt(ro_r);
+    const double wroe = sl / (sl + sr);
+    const double wroe1 = 1.0 - wroe;
+    const double u1 = wroe * vel_l[0] + wroe1 * vel_r[0];
+    const double u2 = wroe * vel_l[1] + wroe1 * vel_r[1];
+    const double u3 = wroe * vel_l[2] + wroe1 * vel_r[2];
+    const double kroe = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3);
+    const double hl = (e_l + p_l) / ro_l;
+    const double hr = (e_r + p_r) / ro_r;
+    const double hroe = wroe * hl + wroe1 * hr;
+    const double cs = sqrt((gamma - 1.0) * (hroe - kroe));
+    const double rcs = 1.0 / cs;
+    const double var1 = 0.5 * (ddi_l[0] + ddi_r[0]);
+    const double var2 = 0.5 * (ddi_l[1] + ddi_r[1]);
+    const double var3 = 0.5 * (ddi_l[2] + ddi_r[2]);
+    const double var4 = 1.0 / sqrt(var1 * var1 + var2 * var2 + var3 * var3);
+    const double g1 = var1 * var4, g2 = var2 * var4, g3 = var3 * var4;
+    const double ugp = u1 * g1 + u2 * g2 + u3 * g3;
+    const double b1 = (gamma - 1.0) / (cs * cs);
+    const double b2 = 1.0 + 2.0 * b1 * kroe - b1 * hroe;
+
+    LEV(1, 1) = 0.5 * (b2 + ugp * rcs);
+    LEV(1, 2) = -0.5 * (b1 * u1 + g1 * rcs);
+    LEV(1, 3) = -0.5 * (b1 * u2 + g2 * rcs);
+    LEV(1, 4) = -0.5 * (b1 * u3 + g3 * rcs);
+    LEV(1, 5) = 0.5 * b1;
+    REV(1, 1) = 1.0;
+    REV(2, 1) = u1 - cs * g1;
+    REV(3, 1) = u2 - cs * g2;
+    REV(4, 1) = u3 - cs * g3;
+    REV(5, 1) = hroe - ugp * cs;
+
+    LEV(2, 1) = 1.0 - b2;
+    LEV(2, 2) = b1 * u1;
+    LEV(2, 3) = b1 * u2;
+    LEV(2, 4) = b1 * u3;
+    LEV(2, 5) = -b1;
+    REV(1, 2) = 1.0;
+    REV(2, 2) = u1;
+    REV(3, 2) = u2;
+    REV(4, 2) = u3;
+    REV(5, 2) = hroe - 1.0 / b1;
+
+    if (fabs(var1) > rero) {
+        const double rgp = 1.0 / g1;
+        LEV(3, 1) = (ugp * g2 - u2) * rgp;
+        LEV(3, 2) = -g2;
+        LEV(3, 3) = (1.0 - g2 * g2) * rgp;
+        LEV(3, 4) = -g2 * g3 * rgp;
+        LEV(3, 5) = 0.0;
+        LEV(4, 1) = (ugp * g3 - u3) * rgp;
+        LEV(4, 2) = -g3;
+        LEV(4, 3) = -g2 * g3 * rgp;
+        LEV(4, 4) = (1.0 - g3 * g3) * rgp;
+        LEV(4, 5) = 0.0;
+        REV(1, 3) = 0.0;
+        REV(2, 3) = -g2;
+        REV(3, 3) = g1;
+        REV(4, 3) = 0.0;
+        REV(5, 3) = u2 * g1 - u1 * g2;
+        REV(1, 4) = 0.0;
+        REV(2, 4) = -g3;
+        REV(3, 4) = 0.0;
+        REV(4, 4) = g1;
+        REV(5, 4) = u3 * g1 - u1 * g3;
+    } else if (fabs(var2) > rero) {
+        const double rgp = 1.0 / g2;
+        LEV(3, 1) = (ugp * g1 - u1) * rgp;
+        LEV(3, 2) = (1.0 - g1 * g1) * rgp;
+        LEV(3, 3) = -g1;
+        LEV(3, 4) = -g1 * g3 * rgp;
+        LEV(3, 5) = 0.0;
+        LEV(4, 1) = (ugp * g3 - u3) * rgp;
+        LEV(4, 2) = -g1 * g3 * rgp;
+        LEV(4, 3) = -g3;
+        LEV(4, 4) = (1.0 - g3 * g3) * rgp;
+        LEV(4, 5) = 0.0;
+        REV(1, 3) = 0.0;
+        REV(2, 3) = g2;
+        REV(3, 3) = -g1;
+        REV(4, 3) = 0.0;
+        REV(5, 3) = u1 * g2 - u2 * g1;
+        REV(1, 4) = 0.0;
+        REV(2, 4) = 0.0;
+        REV(3, 4) = -g3;
+        REV(4, 4) = g2;
+        REV(5, 4) = u3 * g2 - u2 * g3;
+    } else if (fabs(var3) > rero) {
+        const double rgp = 1.0 / g3;
+        LEV(3, 1) = (ugp * g1 - u1) * rgp;
+        LEV(3, 2) = (1.0 - g1 * g1) * rgp;
+        LEV(3, 3) = -g1 * g2 * rgp;
+        LEV(3, 4) = -g1;
+        LEV(3, 5) = 0.0;
+        LEV(4, 1) = (ugp * g2 - u2) * rgp;
+        LEV(4, 2) = -g1 * g2 * rgp;
+        LEV(4, 3) = (1.0 - g2 * g2) * rgp;
+        LEV(4, 4) = -g2;
+        LEV(4, 5) = 0.0;
+        REV(1, 3) = 0.0;
+        REV(2, 3) = g3;
+        REV(3, 3) = 0.0;
+        REV(4, 3) = -g1;
+        REV(5, 3) = u1 * g3 - u3 * g1;
+        REV(1, 4) = 0.0;
+        REV(2, 4) = 0.0;
+        REV(3, 4) = g3;
+        REV(4, 4) = -g2;
+        REV(5, 4) = u2 * g3 - u3 * g2;
+    } else {
+        return 1;
+    }
+
+    LEV(5, 1) = 0.5 * (b2 - ugp * rcs);
+    LEV(5, 2) = -0.5 * (b1 * u1 - g1 * rcs);
+    LEV(5, 3) = -0.5 * (b1 * u2 - g2 * rcs);
+    LEV(5, 4) = -0.5 * (b1 * u3 - g3 * rcs);
+    LEV(5, 5) = 0.5 * b1;
+    REV(1, 5) = 1.0;
+    REV(2, 5) = u1 + cs * g1;
+    REV(3, 5) = u2 + cs * g2;
+    REV(4, 5) = u3 + cs * g3;
+    REV(5, 5) = hroe + ugp * cs;
+#undef LEV
+#undef REV
+    return 0;
+}
+
+/* iwind8, src/solver.F90:1214-1235: stencil node n (1..8) of interface
+ * i+1/2 for the + or - flux, clamped to [imin, imax] */
+int orc_iwind8(int i, int n, int imin, int imax, int plus)
+{
+    int w = plus ? i + n - 4 : i + 5 - n;
+    if (w < imin) w = imin;
+    if (w > imax) w = imax;
+    return w;
+}
+
+/* ------------------------------------------------------------- box level */
+
+typedef struct {
+    long ni, nj, nk, nf;  /* haloed extents, elements per haloed field */
+    long ii, ij, ik, nint;
+} shape_t;
+
+static shape_t shape_of(const orc_uwd *u)
+{
+    shape_t s;
+    s.ni = u->im + 1 + 2 * HMU;
+    s.nj = u->jm + 1 + 2 * HMU;
+    s.nk = u->km + 1 + 2 * HMU;
+    s.nf = s.ni * s.nj * s.nk;
+    s.ii = u->im + 1;
+    s.ij = u->jm + 1;
+    s.ik = u->km + 1;
+    s.nint = s.ii * s.ij * s.ik;
+    return s;
+}
+
+static long hidx(const shape_t *s, int i, int j, int k)
+{
+    return (i + HMU) + s->ni * ((j + HMU) + s->nj * (long)(k + HMU));
+}
+
+static long iidx(const shape_t *s, int i, int j, int k)
+{
+    return i + s->ii * (j + s->ij * (long)k);
+}
+
+/* is..ie of an axis: single-rank ranges of src/parallel.F90:1043-1160
+ * for ntype 1 (is=1), 2 (ie=n-1), 3 (0..n); ntype 4 (both ends bounded,
+ * one rank) leaves is/ie unset in the reference and is taken as 1..n-1. */
+void orc_uwd_range(int ntype, int n, int *lo, int *hi)
+{
+    *lo = (ntype == 1 || ntype == 4) ? 1 : 0;
+    *hi = (ntype == 2 || ntype == 4) ? n - 1 : n;
+}
+
+int orc_ndims(const orc_uwd *u) { return u->km == 0 ? 2 : 3; }
+
+/* gradcal, src/comsolver.F90:244-497 (no species, turbmode none):
+ * dvel(m,n) = sum_d d/dxi_d(u_m) * dxi(d,n), accumulated from zero in the
+ * order d = 1, 2, 3; the derivative is df_compact (explicit_fd = 0) or
+ * diff6ec (explicit_fd = 1) with the axis' ntype. */
+void orc_gradcal(const orc_uwd *u, int explicit_fd, const double *vel, const double *tmp,
+                 const double *dxi, double *dvel, double *dtmp)
+{
+    const shape_t s = shape_of(u);
+    const int nd = orc_ndims(u);
+    const int dims[3] = {u->im, u->jm, u->km};
+    memset(dvel, 0, sizeof(double) * 9 * s.nint);
+    memset(dtmp, 0, sizeof(double) * 3 * s.nint);
+    for (int d = 0; d < nd; ++d) {
+        const int n = dims[d];
+        double *f = malloc(sizeof(double) * (n + 1 + 2 * HMU));
+        double *df = malloc(sizeof(double) * (n + 1) * 4);
+        const int na = d == 0 ? u->jm : u->im, nb = d == 2 ? u->jm : u->km;
+        for (int b = 0; b <= nb; ++b)
+            for (int a = 0; a <= na; ++a) {
+                for (int c = 0; c < 4; ++c) {
+                    const double *src = c < 3 ? vel + c * s.nf : tmp;
+                    for (int t = -HMU; t <= n + HMU; ++t) {
+                        const int i = d == 0 ? t : a, j = d == 1 ? t : (d == 0 ? a : b);
+                        const int k = d == 2 ? t : b;
+                        f[t + HMU] = src[hidx(&s, i, j, k)];
+                    }
+                    if (explicit_fd) orc_diff6ec(u->npdc[d], n, f, df + c * (n + 1));
+                    else orc_df_compact(u->npdc[d], n, f, df + c * (n + 1));
+                }
+                for (int t = 0; t <= n; ++t) {
+                    const int i = d == 0 ? t : a, j = d == 1 ? t : (d == 0 ? a : b);
+                    const int k = d == 2 ? t : b;
+                    const long p = iidx(&s, i, j, k), h = hidx(&s, i, j, k);
+                    for (int c = 0; c < 4; ++c) {
+                        const double g = df[c * (n + 1) + t];
+                        for (int nn = 0; nn < 3; ++nn) {
+                            const double x = dxi[(d + 3 * nn) * s.nf + h];
+                            if (c < 3) dvel[(c + 3 * nn) * s.nint + p] += g * x;
+                            else dtmp[nn * s.nint + p] += g * x;
+                        }
+                    }
+                }
+            }
+        free(f);
+        free(df);
+    }
+}
+
+/* ducrossensor, src/commcal.F90:196-353: ssf on 0..n, its halo swap
+ * (periodic axes: plain copy, src/parallel.F90:4170-4175), then lshock =
+ * (max of ssf over -hm+1..hm along each axis) > shkcrt.  The pressure
+ * stencil and the window are clamped at ntype 1 (low end) and ntype 2
+ * (high end) only, as in the reference; elsewhere they read halos: the
+ * pressure halos as given, ssf halos of non-periodic ends zero (no
+ * neighbour to swap with on one domain). */
+void orc_ducros(const orc_uwd *u, const double *dvel, const double *prs, double *ssf,
+                unsigned char *lshock)
+{
+    const shape_t s = shape_of(u);
+    const int n[3] = {u->im, u->jm, u->km};
+    const int nd = orc_ndims(u);
+    for (long e = 0; e < s.nf; ++e) ssf[e] = 0.0;
+    for (int k = 0; k <= u->km; ++k)
+        for (int j = 0; j <= u->jm; ++j)
+            for (int i = 0; i <= u->im; ++i) {
+                const long p = iidx(&s, i, j, k);
+#define DV(m, nn) dvel[((m)-1 + 3 * ((nn)-1)) * s.nint + p]
+                double div = DV(1, 1) + DV(2, 2) + DV(3, 3);
+                const double div2 = div * div;
+                const double vx = DV(3, 2) - DV(2, 3);
+                const double vy = DV(1, 3) - DV(3, 1);
+                const double vz = DV(2, 1) - DV(1, 2);
+#undef DV
+                const double vort = vx * vx + vy * vy + vz * vz;
+                int ip1 = i + 1, im1 = i - 1, jp1 = j + 1, jm1 = j - 1, kp1 = k + 1, km1 = k - 1;
+                if (u->npdc[0] == 1 && im1 < 0) im1 = 0;
+                if (u->npdc[1] == 1 && jm1 < 0) jm1 = 0;
+                if (u->npdc[2] == 1 && km1 < 0) km1 = 0;
+                if (u->npdc[0] == 2 && ip1 > u->im) ip1 = u->im;
+                if (u->npdc[1] == 2 && jp1 > u->jm) jp1 = u->jm;
+                if (u->npdc[2] == 2 && kp1 > u->km) kp1 = u->km;
+                const double pc = prs[hidx(&s, i, j, k)];
+#define P(a, b, c) prs[hidx(&s, a, b, c)]
+                const double dpdi = fabs(P(ip1, j, k) - 2.0 * pc + P(im1, j, k)) /
+                                    (P(ip1, j, k) + 2.0 * pc + P(im1, j, k));
+                const double dpdj = fabs(P(i, jp1, k) - 2.0 * pc + P(i, jm1, k)) /
+                                    (P(i, jp1, k) + 2.0 * pc + P(i, jm1, k));
+                const double dpdk = fabs(P(i, j, kp1) - 2.0 * pc + P(i, j, km1)) /
+                                    (P(i, j, kp1) + 2.0 * pc + P(i, j, km1));
+#undef P
+                ssf[hidx(&s, i, j, k)] =
+                    div2 / (div2 + vort + 1.e-30) * max2(max2(dpdi, dpdj), dpdk);
+            }
+    /* halo swap of ssf along the periodic axes, in the order i, j, k, each
+     * over the full extent of the other axes */
+    for (int d = 0; d < nd; ++d) {
+        if (u->npdc[d] != 3) continue;
+        for (long e = 0; e < s.nf; ++e) {
+            int c[3];
+            c[0] = (int)(e % s.ni) - HMU;
+            c[1] = (int)((e / s.ni) % s.nj) - HMU;
+            c[2] = (int)(e / (s.ni * s.nj)) - HMU;
+            int t = c[d];
+            if (t >= 0 && t <= n[d]) continue;
+            c[d] = t < 0 ? t + n[d] : t - n[d];
+            ssf[e] = ssf[hidx(&s, c[0], c[1], c[2])];
+        }
+    }
+    for (int k = 0; k <= u->km; ++k)
+        for (int j = 0; j <= u->jm; ++j)
+            for (int i = 0; i <= u->im; ++i) {
+                double m = 0.0;
+                const int c0[3] = {i, j, k};
+                for (int d = 0; d < 3; ++d) {
+                    for (int o = -HMU + 1; o <= HMU; ++o) {
+                        int c[3] = {c0[0], c0[1], c0[2]};
+                        int t = c[d] + o;
+                        if (u->npdc[d] == 1 && t < 0) t = 0;
+                        if (u->npdc[d] == 2 && t > n[d]) t = n[d];
+                        c[d] = t;
+                        m = max2(m, ssf[hidx(&s, c[0], c[1], c[2])]);
+                    }
+                }
+                lshock[iidx(&s, i, j, k)] = m > u->shkcrt;
+            }
+}
+
+/* convrsduwd, src/solver.F90:549-1202 (no species, crinod false, no
+ * solids): per axis, Steger-Warming split fluxes on iss..iee, the
+ * interface flux Fh(i+1/2) for i = is-1..ie (characteristic projection
+ * when lchardecomp and the shock flag), qrhs += Fh(i) - Fh(i-1) on is..ie.
+ * lshock NULL: the sensor is off (sson false, lsh true).  Returns nonzero
+ * on an unsupported ntype (the reference stops: ntype 4 on j and k). */
+int orc_convrsduwd(const orc_uwd *u, const double *q, const double *rho, const double *vel,
+                   const double *prs, const double *tmp, const double *dxi, const double *jacob,
+                   const unsigned char *lshock, double *qrhs)
+{
+    const shape_t s = shape_of(u);
+    const int nd = orc_ndims(u);
+    const int n[3] = {u->im, u->jm, u->km};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) orc_uwd_range(u->npdc[d], n[d], &lo[d], &hi[d]);
+    if (nd == 2) lo[2] = hi[2] = 0;
+    for (int d = 0; d < nd; ++d) {
+        const int nt = u->npdc[d];
+        if (nt < 1 || nt > 4 || (d > 0 && nt == 4)) return 1;
+        const int iss = (nt == 1 || nt == 4) ? 0 : -HMU;
+        const int iee = (nt == 2 || nt == 4) ? n[d] : n[d] + HMU;
+        const int len = n[d] + 1 + 2 * HMU;
+        double *fp = malloc(sizeof(double) * 5 * len), *fm = malloc(sizeof(double) * 5 * len);
+        double *fh = malloc(sizeof(double) * 5 * (n[d] + 2));  /* Fh(-1:n) */
+        /* lines: d=0 over (j,k) in js..je x ks..ke, d=1 over (i,k), d=2 (i,j) */
+        const int a0 = d == 0 ? lo[1] : lo[0], a1 = d == 0 ? hi[1] : hi[0];
+        const int b0 = d == 2 ? lo[1] : lo[2], b1 = d == 2 ? hi[1] : hi[2];
+        for (int b = b0; b <= b1; ++b)
+            for (int a = a0; a <= a1; ++a) {
+#define NODE(t, I, J, K)                                   \
+    const int I = d == 0 ? (t) : a;                        \
+    const int J = d == 1 ? (t) : (d == 0 ? a : b);         \
+    const int K = d == 2 ? (t) : b;
+                for (int t = iss; t <= iee; ++t) {
+                    NODE(t, i, j, k)
+                    const long h = hidx(&s, i, j, k);
+                    double v[3], qq[5], dd[3];
+                    for (int c = 0; c < 3; ++c) {
+                        v[c] = vel[c * s.nf + h];
+                        dd[c] = dxi[(d + 3 * c) * s.nf + h];
+                    }
+                    for (int m = 0; m < 5; ++m) qq[m] = q[m * s.nf + h];
+                    orc_sw_split(u->gamma, u->mach, rho[h], v, prs[h], tmp[h], qq, dd, jacob[h],
+                                 fp + 5 * (t + HMU), fm + 5 * (t + HMU));
+                }
+                for (int i = lo[d] - 1; i <= hi[d]; ++i) {
+                    int lsh = 1;
+                    if (lshock) {
+                        NODE(i < 0 ? i + 1 : i, i1, j1, k1)
+                        NODE(i + 1 > n[d] ? i : i + 1, i2, j2, k2)
+                        lsh = lshock[iidx(&s, i1, j1, k1)] || lshock[iidx(&s, i2, j2, k2)];
+                    }
+                    double Lv[25], Rv[25], fcp[5][8], fcm[5][8], fhc[5];
+                    const int chr = u->lchardecomp && lsh;
+                    if (chr) {
+                        NODE(i, il, jl, kl)
+                        NODE(i + 1, ir, jr, kr)
+                        const long hl = hidx(&s, il, jl, kl), hr = hidx(&s, ir, jr, kr);
+                        double vl[3], vr[3], dl[3], dr[3];
+                        for (int c = 0; c < 3; ++c) {
+                            vl[c] = vel[c * s.nf + hl];
+                            vr[c] = vel[c * s.nf + hr];
+                            dl[c] = dxi[(d + 3 * c) * s.nf + hl];
+                            dr[c] = dxi[(d + 3 * c) * s.nf + hr];
+                        }
+                        if (orc_chardecomp(u->gamma, tmp[hl], rho[hl], prs[hl], q[4 * s.nf + hl],
+                                           vl, dl, tmp[hr], rho[hr], prs[hr], q[4 * s.nf + hr],
+                                           vr, dr, Rv, Lv))
+                            return 2;
+                        for (int m = 0; m < 5; ++m)
+                            for (int w = 1; w <= 8; ++w) {
+                                const double *P = fp + 5 * (orc_iwind8(i, w, iss, iee, 1) + HMU);
+                                const double *M = fm + 5 * (orc_iwind8(i, w, iss, iee, 0) + HMU);
+                                const double *l = Lv + 5 * m;
+                                fcp[m][w - 1] = l[0] * P[0] + l[1] * P[1] + l[2] * P[2] +
+                                                l[3] * P[3] + l[4] * P[4];
+                                fcm[m][w - 1] = l[0] * M[0] + l[1] * M[1] + l[2] * M[2] +
+                                                l[3] * M[3] + l[4] * M[4];
+                            }
+                    } else {
+                        for (int w = 1; w <= 8; ++w) {
+                            const double *P = fp + 5 * (orc_iwind8(i, w, iss, iee, 1) + HMU);
+                            const double *M = fm + 5 * (orc_iwind8(i, w, iss, iee, 0) + HMU);
+                            for (int m = 0; m < 5; ++m) {
+                                fcp[m][w - 1] = P[m];
+                                fcm[m][w - 1] = M[m];
+                            }
+                        }
+                    }
+                    for (int m = 0; m < 5; ++m) {
+                        const double v1 = orc_recons_exp(fcp[m], i, n[d], nt, u->recon_schem, lsh,
+                                                         u->bfacmpld);
+                        const double v2 = orc_recons_exp(fcm[m], i, n[d], nt, u->recon_schem, lsh,
+                                                         u->bfacmpld);
+                        fhc[m] = v1 + v2;
+                    }
+                    double *F = fh + 5 * (i + 1);
+                    if (chr) {
+                        for (int m = 0; m < 5; ++m) {
+                            const double *r = Rv + 5 * m;
+                            F[m] = r[0] * fhc[0] + r[1] * fhc[1] + r[2] * fhc[2] + r[3] * fhc[3] +
+                                   r[4] * fhc[4];
+                        }
+                    } else {
+                        for (int m = 0; m < 5; ++m) F[m] = fhc[m];
+                    }
+                }
+                for (int i = lo[d]; i <= hi[d]; ++i) {
+                    NODE(i, ii, jj, kk)
+                    const long p = iidx(&s, ii, jj, kk);
+                    for (int m = 0; m < 5; ++m)
+                        qrhs[m * s.nint + p] = qrhs[m * s.nint + p] + fh[5 * (i + 1) + m] -
+                                               fh[5 * i + m];
+                }
+#undef NODE
+            }
+        free(fp);
+        free(fm);
+        free(fh);
+    }
+    return 0;
+}
+
+/* rhscal for an odd-order explicit convection scheme (conschm(4:4)='e'),
+ * src/solver.F90:184-283 without viscous terms: qrhs = 0, gradcal,
+ * ducrossensor when recon_schem = 5 or lchardecomp, convrsduwd, then
+ * qrhs = -qrhs.  ssf/lshock may be NULL when the sensor is not run. */
+int orc_uwd_rhscal(const orc_uwd *u, int explicit_fd, const double *q, const double *rho,
+                   const double *vel, const double *prs, const double *tmp, const double *dxi,
+                   const double *jacob, double *qrhs, double *dvel, double *dtmp, double *ssf,
+                   unsigned char *lshock)
+{
+    const shape_t s = shape_of(u);
+    memset(qrhs, 0, sizeof(double) * 5 * s.nint);
+    orc_gradcal(u, explicit_fd, vel, tmp, dxi, dvel, dtmp);
+    const int sensor = u->recon_schem == 5 || u->lchardecomp;
+    if (sensor) orc_ducros(u, dvel, prs, ssf, lshock);
+    const int rc = orc_convrsduwd(u, q, rho, vel, prs, tmp, dxi, jacob, sensor ? lshock : NULL,
+                                  qrhs);
+    if (rc) return rc;
+    for (long e = 0; e < 5 * s.nint; ++e) qrhs[e] = -qrhs[e];
+    return 0;
+}
