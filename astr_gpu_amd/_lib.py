@@ -34,6 +34,10 @@ class AstrParams(ctypes.Structure):
         ("gamma", ctypes.c_double), ("ref_tem", ctypes.c_double),
         ("sutherland", ctypes.c_double), ("beta_halo", ctypes.c_double),
         ("beta_bound", ctypes.c_double),
+        ("convection", ctypes.c_int32), ("deriv_explicit", ctypes.c_int32),
+        ("recon_schem", ctypes.c_int32), ("lchardecomp", ctypes.c_int32),
+        ("lfilter", ctypes.c_int32), ("diffterm", ctypes.c_int32),
+        ("shkcrt", ctypes.c_double), ("bfacmpld", ctypes.c_double),
     ]
 
 
@@ -52,6 +56,7 @@ SIGNATURES = {
     "astr_rk3_step": (ctypes.c_int, [_P, ctypes.c_double, _D]),
     "astr_rk3_steps": (ctypes.c_int, [_P, _I, ctypes.c_double, _D]),
     "astr_download_rhs": (ctypes.c_int, [_P, _D, _D, _D]),
+    "astr_download_shock": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint8)]),
     "astr_deriv_field": (ctypes.c_int, [_P, _I, _D, _D]),
     "astr_filter_field": (ctypes.c_int, [_P, _I, _D, _D]),
     "astr_set_timing": (ctypes.c_int, [_P, _I]),

@@ -33,6 +33,7 @@
 
 #include "../../include/astr_hip.h"
 #include "line_engine.hpp"
+#include "upwind.hpp"
 
 using namespace astr;
 
@@ -67,7 +68,7 @@ struct Geo {
     long org;    // element offset of node (0,0,0)
 };
 
-__device__ __forceinline__ long gidx(const Geo& g, int i, int j, int k)
+__host__ __device__ __forceinline__ long gidx(const Geo& g, int i, int j, int k)
 {
     return g.org + i + j * g.s[1] + k * g.s[2];
 }
@@ -109,6 +110,7 @@ struct SweepArgs {
     int rk_stage;
     double rka, rkb, rkc, dt;
     double* qsave[5];
+    int visc;  // PASS_RHS: include the viscous fluxes (diffterm)
 };
 
 // node -> element offset along the line, with periodic wrap (dataswap:
@@ -239,7 +241,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
                 else if (m == 4) fi = jc * (qm + ldb(a.prim[4], od)) * uu;
                 else if (m == dir + 1) fi = jc * (qm * uu + xn * ldb(a.prim[4], od));
                 else fi = jc * (qm * uu);
-                if (m == 0) {
+                if (m == 0 || !a.visc) {
                     v = -fi;
                 } else {
                     const unsigned int ow = lob + (unsigned int)wrap_node(node, dim) * sdb;
@@ -605,6 +607,337 @@ __global__ __launch_bounds__(256) void k_stats(PtArgs a)
         for (int c = 0; c < 3; ++c) a.part[3 * blockIdx.x + c] = sh[c][0];
 }
 
+// ------------------------------------------------------- upwind convection
+//
+// Contraction is off here as in upwind.hpp: the explicit gradient, the
+// sensor and the interface fluxes keep the reference's expression order.
+#pragma clang fp contract(off)
+
+struct UwdArgs {
+    Geo g;
+    int dir, ntype, ndims;
+    int lo, hi;          // qrhs nodes along dir (is..ie of the axis)
+    int iss, iee;        // nodes the split fluxes are taken on
+    int a0, a1, b0, b1;  // line ranges over the two other axes
+    int ntile_line, ntile_a;
+    int recon, lchar, sensor, last;
+    double bfac, gamma, mach;
+    const double* q[5];
+    const double* prim[6];  // rho,u,v,w,p,T
+    const double* jac;
+    const double* xi;       // metric along dir, xi[n+HM]
+    const unsigned char* lshock;
+    double* qrhs[5];
+};
+
+struct SensorArgs {
+    Geo g;
+    int ntype[3], ndims;
+    double shkcrt;
+    const double* dvel[9];
+    const double* prs;
+    double* ssf;
+    unsigned char* lshock;
+};
+
+// ducrossensor, part 1 (src/commcal.F90:236-272): ssf on 0..n; the
+// pressure stencil is clamped at ntype 1 (low end) and 2 (high end) only
+__global__ void k_ducros_ssf(SensorArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
+    if (p >= ni * nj * (a.g.n[2] + 1)) return;
+    const int i = (int)(p % ni), j = (int)((p / ni) % nj), k = (int)(p / (ni * nj));
+    const long gi = gidx(a.g, i, j, k);
+    const double div = a.dvel[0][gi] + a.dvel[4][gi] + a.dvel[8][gi];
+    const double div2 = div * div;
+    // dvel(m,n) is field 3(m-1)+(n-1)
+    const double vx = a.dvel[7][gi] - a.dvel[5][gi];  // dvel(3,2) - dvel(2,3)
+    const double vy = a.dvel[2][gi] - a.dvel[6][gi];  // dvel(1,3) - dvel(3,1)
+    const double vz = a.dvel[3][gi] - a.dvel[1][gi];  // dvel(2,1) - dvel(1,2)
+    const double vort = vx * vx + vy * vy + vz * vz;
+    int ip1 = i + 1, im1 = i - 1, jp1 = j + 1, jm1 = j - 1, kp1 = k + 1, km1 = k - 1;
+    if (a.ntype[0] == 1 && im1 < 0) im1 = 0;
+    if (a.ntype[1] == 1 && jm1 < 0) jm1 = 0;
+    if (a.ntype[2] == 1 && km1 < 0) km1 = 0;
+    if (a.ntype[0] == 2 && ip1 > a.g.n[0]) ip1 = a.g.n[0];
+    if (a.ntype[1] == 2 && jp1 > a.g.n[1]) jp1 = a.g.n[1];
+    if (a.ntype[2] == 2 && kp1 > a.g.n[2]) kp1 = a.g.n[2];
+    const double pc = a.prs[gi];
+    const double* P = a.prs;
+    const double pi1 = P[gidx(a.g, ip1, j, k)], pi0 = P[gidx(a.g, im1, j, k)];
+    const double pj1 = P[gidx(a.g, i, jp1, k)], pj0 = P[gidx(a.g, i, jm1, k)];
+    const double pk1 = P[gidx(a.g, i, j, kp1)], pk0 = P[gidx(a.g, i, j, km1)];
+    const double dpdi = fabs(pi1 - 2.0 * pc + pi0) / (pi1 + 2.0 * pc + pi0);
+    const double dpdj = fabs(pj1 - 2.0 * pc + pj0) / (pj1 + 2.0 * pc + pj0);
+    const double dpdk = fabs(pk1 - 2.0 * pc + pk0) / (pk1 + 2.0 * pc + pk0);
+    a.ssf[gi] = div2 / (div2 + vort + 1.e-30) * uwd::max2(uwd::max2(dpdi, dpdj), dpdk);
+}
+
+// dataswap(ssf) (src/commcal.F90:276; src/parallel.F90:4170-4175): a halo
+// cell takes the interior value at its periodically wrapped coordinates;
+// halo cells of non-periodic axes are zero (no neighbour on one domain).
+// Reads the interior, writes halo cells only.
+__global__ void k_ssf_halo(SensorArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ni = a.g.n[0] + 1 + 2 * HM, nj = a.g.n[1] + 1 + 2 * HM;
+    if (p >= ni * nj * (a.g.n[2] + 1 + 2 * HM)) return;
+    int c[3] = {(int)(p % ni) - HM, (int)((p / ni) % nj) - HM, (int)(p / (ni * nj)) - HM};
+    bool halo = false, keep = true;
+    int s[3];
+    for (int d = 0; d < 3; ++d) {
+        s[d] = c[d];
+        if (c[d] >= 0 && c[d] <= a.g.n[d]) continue;
+        halo = true;
+        if (d < a.ndims && a.ntype[d] == 3) s[d] = c[d] < 0 ? c[d] + a.g.n[d] : c[d] - a.g.n[d];
+        else keep = false;
+    }
+    if (!halo) return;
+    a.ssf[gidx(a.g, c[0], c[1], c[2])] = keep ? a.ssf[gidx(a.g, s[0], s[1], s[2])] : 0.0;
+}
+
+// ducrossensor, part 2 (src/commcal.F90:278-316): lshock = max of ssf over
+// offsets -hm+1..hm along each axis > shkcrt
+__global__ void k_ducros_flag(SensorArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
+    if (p >= ni * nj * (a.g.n[2] + 1)) return;
+    const int c0[3] = {(int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj))};
+    double m = 0.0;
+    for (int d = 0; d < 3; ++d)
+        for (int o = -HM + 1; o <= HM; ++o) {
+            int c[3] = {c0[0], c0[1], c0[2]};
+            int t = c[d] + o;
+            if (a.ntype[d] == 1 && t < 0) t = 0;
+            if (a.ntype[d] == 2 && t > a.g.n[d]) t = a.g.n[d];
+            c[d] = t;
+            m = uwd::max2(m, a.ssf[gidx(a.g, c[0], c[1], c[2])]);
+        }
+    a.lshock[gidx(a.g, c0[0], c0[1], c0[2])] = m > a.shkcrt ? 1 : 0;
+}
+
+struct GradXArgs {
+    Geo g;
+    int ntype[3], ndims;
+    const double* prim[6];
+    const double* xi[3];
+    double* dvel[9];
+    double* dtmp[3];
+};
+
+// diff6ec at node t of a line v(-hm..dim+hm), src/derivative.F90:350-413
+__device__ __forceinline__ double diff6ec_at(const double* base, long stride, int t, int dim,
+                                             int ntype)
+{
+    auto V = [&](int n) { return base[(long)n * stride]; };
+    const bool lob = ntype == 1 || ntype == 4, hib = ntype == 2 || ntype == 4;
+    if (lob && t == 0) return -0.5 * V(2) + 2.0 * V(1) - 1.5 * V(0);
+    if (lob && t == 1) return 0.5 * (V(2) - V(0));
+    if (lob && t == 2) return (2.0 / 3.0) * (V(3) - V(1)) - (1.0 / 12.0) * (V(4) - V(0));
+    if (hib && t == dim) return 0.5 * V(dim - 2) - 2.0 * V(dim - 1) + 1.5 * V(dim);
+    if (hib && t == dim - 1) return 0.5 * (V(dim) - V(dim - 2));
+    if (hib && t == dim - 2)
+        return (2.0 / 3.0) * (V(dim - 1) - V(dim - 3)) - (1.0 / 12.0) * (V(dim) - V(dim - 4));
+    return 0.75 * (V(t + 1) - V(t - 1)) - 0.15 * (V(t + 2) - V(t - 2)) +
+           (1.0 / 60.0) * (V(t + 3) - V(t - 3));
+}
+
+// gradcal with the explicit 6th-order derivative (difschm '643e'),
+// src/comsolver.F90:244-497: dvel(m,d) = d/dxi_d(u_m) * dxi(d,d)
+__global__ void k_grad_explicit(GradXArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
+    if (p >= ni * nj * (a.g.n[2] + 1)) return;
+    const int c[3] = {(int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj))};
+    const long gi = gidx(a.g, c[0], c[1], c[2]);
+    for (int d = 0; d < 3; ++d) {
+        if (d >= a.ndims) {
+            for (int m = 0; m < 3; ++m) a.dvel[3 * m + d][gi] = 0.0;
+            a.dtmp[d][gi] = 0.0;
+            continue;
+        }
+        const long st = a.g.s[d];
+        const long line0 = gi - (long)c[d] * st;  // node 0 of this line
+        const double x = a.xi[d][c[d] + HM];
+        for (int m = 0; m < 4; ++m) {
+            const double* f = a.prim[m < 3 ? 1 + m : 5] + line0;
+            const double df = diff6ec_at(f, st, c[d], a.g.n[d], a.ntype[d]);
+            if (m < 3) a.dvel[3 * m + d][gi] = df * x;
+            else a.dtmp[d][gi] = df * x;
+        }
+    }
+}
+
+struct FieldXArgs {
+    Geo g;
+    int dir, ntype;
+    const double* fin;
+    double* fout;
+};
+
+// df_explicit (fds%central with difschm '643e') of every line of a field,
+// src/derivative.F90:318-337
+__global__ void k_deriv_explicit_field(FieldXArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
+    if (p >= ni * nj * (a.g.n[2] + 1)) return;
+    const int c[3] = {(int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj))};
+    const long gi = gidx(a.g, c[0], c[1], c[2]);
+    const long st = a.g.s[a.dir];
+    a.fout[gi] = diff6ec_at(a.fin + (gi - (long)c[a.dir] * st), st, c[a.dir], a.g.n[a.dir], a.ntype);
+}
+
+// Zero qrhs on the interior (qrhs=0 before rhscal, src/mainloop.F90:404)
+__global__ void k_zero_qrhs(UwdArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
+    if (p >= ni * nj * (a.g.n[2] + 1)) return;
+    const long gi = gidx(a.g, (int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj)));
+    for (int m = 0; m < 5; ++m) a.qrhs[m][gi] = 0.0;
+}
+
+// convrsduwd along one axis, src/solver.F90:549-1202 (no species, crinod
+// false, no solids).  A tile is TL lines x S interfaces i0-1 .. i0+S-2:
+//  1. the Steger-Warming split fluxes of nodes i0-4 .. i0+S+2 (their index
+//     clamped to iss..iee, which is iwind8's clamp) go to LDS;
+//  2. each thread forms the interface flux Fh(i) from the 8-point stencils
+//     of F+ and F-, projected on the Roe eigenvectors where the scheme and
+//     the shock flag ask for it;
+//  3. qrhs(i) += Fh(i) - Fh(i-1) for the nodes i0 .. i0+S-2 within is..ie
+//     (negated on the last axis: qrhs=-qrhs, src/solver.F90:243).
+// x tiles are 1 line x 256 interfaces; y/z tiles 16 lines (consecutive i,
+// 128-byte rows) x 16 interfaces.
+template <bool XD>
+__global__ __launch_bounds__(256) void k_uwd(UwdArgs a)
+{
+    constexpr int TL = XD ? 1 : 16;
+    constexpr int S = XD ? 256 : 16;
+    constexpr int NN = S + 7;
+    __shared__ double sw[TL * NN * 10];
+    __shared__ double fh[TL * S * 5];
+    const int tid = threadIdx.x;
+    int bid = blockIdx.x;
+    const int seg = bid % a.ntile_line;
+    bid /= a.ntile_line;
+    const int ta = bid % a.ntile_a;
+    const int b = a.b0 + bid / a.ntile_a;
+    const int i0 = a.lo + seg * (S - 1);
+    const int dir = a.dir;
+    const int dim = a.g.n[dir];
+    auto line_node = [&](int l, int t, int& i, int& j, int& k) {
+        const int av = min(a.a0 + ta * TL + l, a.a1);
+        if (dir == 0) { i = t; j = av; k = b; }
+        else if (dir == 1) { i = av; j = t; k = b; }
+        else { i = av; j = b; k = t; }
+    };
+    // 1. split fluxes
+    for (int e = tid; e < TL * NN; e += 256) {
+        const int l = e % TL, t = e / TL;
+        const int node = min(max(i0 - 4 + t, a.iss), a.iee);
+        int i, j, k;
+        line_node(l, node, i, j, k);
+        const long gi = gidx(a.g, i, j, k);
+        double vel[3] = {a.prim[1][gi], a.prim[2][gi], a.prim[3][gi]};
+        double q[5];
+        for (int m = 0; m < 5; ++m) q[m] = a.q[m][gi];
+        double dxi[3] = {0.0, 0.0, 0.0};
+        dxi[dir] = a.xi[node + HM];
+        double* o = sw + (l * NN + t) * 10;
+        uwd::sw_split(a.gamma, a.mach, a.prim[0][gi], vel, a.prim[4][gi], a.prim[5][gi], q, dxi,
+                      a.jac[gi], o, o + 5);
+    }
+    __syncthreads();
+    // 2. interface fluxes
+    const int l = XD ? 0 : (tid & 15);
+    const int s = XD ? tid : (tid >> 4);
+    const int ii = i0 - 1 + s;  // interface ii+1/2
+    if (ii <= a.hi) {
+        bool lsh = true;
+        if (a.sensor) {
+            int i, j, k;
+            line_node(l, ii < 0 ? ii + 1 : ii, i, j, k);
+            const bool s1 = a.lshock[gidx(a.g, i, j, k)] != 0;
+            line_node(l, ii + 1 > dim ? ii : ii + 1, i, j, k);
+            lsh = s1 || a.lshock[gidx(a.g, i, j, k)] != 0;
+        }
+        const bool chr = a.lchar && lsh;
+        uwd::Roe r;
+        if (chr) {
+            int i, j, k;
+            line_node(l, ii, i, j, k);
+            const long gl = gidx(a.g, i, j, k);
+            line_node(l, ii + 1, i, j, k);
+            const long gr = gidx(a.g, i, j, k);
+            const double vl[3] = {a.prim[1][gl], a.prim[2][gl], a.prim[3][gl]};
+            const double vr[3] = {a.prim[1][gr], a.prim[2][gr], a.prim[3][gr]};
+            double dl[3] = {0.0, 0.0, 0.0}, dr[3] = {0.0, 0.0, 0.0};
+            dl[dir] = a.xi[ii + HM];
+            dr[dir] = a.xi[ii + 1 + HM];
+            r = uwd::roe_state(a.gamma, a.prim[0][gl], a.prim[4][gl], a.q[4][gl], vl, dl,
+                               a.prim[0][gr], a.prim[4][gr], a.q[4][gr], vr, dr);
+        }
+        const double* base = sw + l * NN * 10;
+        double fhc[5];
+        for (int m = 0; m < 5; ++m) {
+            double fp[8], fm[8];
+            if (chr) {
+                double lv[5];
+                uwd::lev_row(r, m, lv);
+#pragma unroll
+                for (int w = 0; w < 8; ++w) {
+                    const double* P = base + (s + w) * 10;      // node ii+w-3
+                    const double* M = base + (s + 7 - w) * 10 + 5;  // node ii+4-w
+                    fp[w] = lv[0] * P[0] + lv[1] * P[1] + lv[2] * P[2] + lv[3] * P[3] + lv[4] * P[4];
+                    fm[w] = lv[0] * M[0] + lv[1] * M[1] + lv[2] * M[2] + lv[3] * M[3] + lv[4] * M[4];
+                }
+            } else {
+#pragma unroll
+                for (int w = 0; w < 8; ++w) {
+                    fp[w] = base[(s + w) * 10 + m];
+                    fm[w] = base[(s + 7 - w) * 10 + 5 + m];
+                }
+            }
+            const double v1 = uwd::recons_exp(fp, ii, dim, a.ntype, a.recon, lsh, a.bfac);
+            const double v2 = uwd::recons_exp(fm, ii, dim, a.ntype, a.recon, lsh, a.bfac);
+            fhc[m] = v1 + v2;
+        }
+        double* F = fh + (l * S + s) * 5;
+        if (chr) {
+            for (int m = 0; m < 5; ++m) {
+                double rv[5];
+                uwd::rev_row(r, m, rv);
+                F[m] = rv[0] * fhc[0] + rv[1] * fhc[1] + rv[2] * fhc[2] + rv[3] * fhc[3] +
+                       rv[4] * fhc[4];
+            }
+        } else {
+            for (int m = 0; m < 5; ++m) F[m] = fhc[m];
+        }
+    }
+    __syncthreads();
+    // 3. flux difference
+    const int node = i0 - 1 + s;
+    const bool lvalid = XD || (a.a0 + ta * TL + l <= a.a1);
+    if (s >= 1 && node <= a.hi && lvalid) {
+        int i, j, k;
+        line_node(l, node, i, j, k);
+        const long gi = gidx(a.g, i, j, k);
+        const double* F1 = fh + (l * S + s) * 5;
+        const double* F0 = F1 - 5;
+        for (int m = 0; m < 5; ++m) {
+            const double v = a.qrhs[m][gi] + F1[m] - F0[m];
+            a.qrhs[m][gi] = a.last ? -v : v;
+        }
+    }
+}
+
+// back to the default contraction of the rest of the library
+#pragma clang fp contract(fast)
+
 // Plain copy of n doubles (16-byte accesses): the bandwidth reference for the
 // roofline and the calibration of the HBM traffic counters.
 __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src,
@@ -627,6 +960,8 @@ struct astr_ctx {
     long pi = 0, pj = 0, pk = 0, nfield = 0;  // padded dims, elements per field
     double* base = nullptr;
     double *q[5], *prim[6], *qsave[5], *qrhs[5], *dvel[9], *dtmp[3], *sig[6], *qfl[3], *jac;
+    double* ssf = nullptr;          // Ducros sensor (upwind path), haloed
+    unsigned char* lshock = nullptr; // shock flags, one byte per padded cell
     double* scratch = nullptr;  // field in of the field-level operators
     double* scratch2 = nullptr; // field out of the field-level operators
     double* xi = nullptr;       // 3 metric lines, each LPL*LMAX+2*HM+1
@@ -804,6 +1139,7 @@ SweepArgs sweep_args(astr_ctx* c, int dir, int pass)
     a.jac = c->jac;
     a.xi = c->xi + dir * c->xi_stride;
     for (int m = 0; m < 5; ++m) a.qsave[m] = c->qsave[m];
+    a.visc = c->p.diffterm;
     return a;
 }
 
@@ -892,12 +1228,6 @@ int do_qswap(astr_ctx* c)
 // z sweep either finishes qrhs (rk_stage 0) or applies the RK3 combine of
 // that stage to q (rk_stage 1..3), after which k_prim refreshes the
 // primitives.
-int do_grad_sigma(astr_ctx* c)
-{
-    for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
-    return launch_pt(c, k_sigma, n_interior(c), pt_args(c));
-}
-
 const double RKC[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
 
 int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt)
@@ -916,9 +1246,132 @@ int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt)
     return ASTR_OK;
 }
 
+int ndims_of(const astr_ctx* c) { return c->g.n[2] == 0 ? 2 : 3; }
+
+long n_haloed(const astr_ctx* c)
+{
+    return (long)(c->g.n[0] + 1 + 2 * HM) * (c->g.n[1] + 1 + 2 * HM) * (c->g.n[2] + 1 + 2 * HM);
+}
+
+// gradcal with the explicit derivative (difschm '643e')
+int do_grad_explicit(astr_ctx* c)
+{
+    GradXArgs a{};
+    a.g = c->g;
+    a.ntype[0] = c->p.npdci;
+    a.ntype[1] = c->p.npdcj;
+    a.ntype[2] = c->p.npdck;
+    a.ndims = ndims_of(c);
+    for (int m = 0; m < 6; ++m) a.prim[m] = c->prim[m];
+    for (int d = 0; d < 3; ++d) a.xi[d] = c->xi + d * c->xi_stride;
+    for (int m = 0; m < 9; ++m) a.dvel[m] = c->dvel[m];
+    for (int d = 0; d < 3; ++d) a.dtmp[d] = c->dtmp[d];
+    const long n = n_interior(c);
+    hipLaunchKernelGGL(k_grad_explicit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       c->stream, a);
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
+}
+
+// ducrossensor: ssf, its halo swap, the windowed flag
+int do_sensor(astr_ctx* c)
+{
+    SensorArgs a{};
+    a.g = c->g;
+    a.ntype[0] = c->p.npdci;
+    a.ntype[1] = c->p.npdcj;
+    a.ntype[2] = c->p.npdck;
+    a.ndims = ndims_of(c);
+    a.shkcrt = c->p.shkcrt;
+    for (int m = 0; m < 9; ++m) a.dvel[m] = c->dvel[m];
+    a.prs = c->prim[4];
+    a.ssf = c->ssf;
+    a.lshock = c->lshock;
+    const long n = n_interior(c), nh = n_haloed(c);
+    hipLaunchKernelGGL(k_ducros_ssf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_ssf_halo, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_ducros_flag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
+}
+
+// is..ie of an axis on one domain (src/parallel.F90:1043-1160): ntype 1
+// starts at 1, ntype 2 ends at n-1; ntype 4 (bounded at both ends) is left
+// unset by the reference on one rank and taken as 1..n-1
+void uwd_range(int ntype, int n, int& lo, int& hi)
+{
+    lo = (ntype == 1 || ntype == 4) ? 1 : 0;
+    hi = (ntype == 2 || ntype == 4) ? n - 1 : n;
+}
+
+// qrhs=0, then convrsduwd along each axis, the last one negating
+int do_upwind(astr_ctx* c)
+{
+    const int nd = ndims_of(c);
+    const int nts[3] = {c->p.npdci, c->p.npdcj, c->p.npdck};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) uwd_range(nts[d], c->g.n[d], lo[d], hi[d]);
+    if (nd == 2) lo[2] = hi[2] = 0;
+    UwdArgs a{};
+    a.g = c->g;
+    a.ndims = nd;
+    a.recon = c->p.recon_schem;
+    a.lchar = c->p.lchardecomp;
+    a.sensor = (c->p.recon_schem == 5 || c->p.lchardecomp) ? 1 : 0;
+    a.bfac = c->p.bfacmpld;
+    a.gamma = c->p.gamma;
+    a.mach = c->p.mach;
+    for (int m = 0; m < 5; ++m) { a.q[m] = c->q[m]; a.qrhs[m] = c->qrhs[m]; }
+    for (int m = 0; m < 6; ++m) a.prim[m] = c->prim[m];
+    a.jac = c->jac;
+    a.lshock = c->lshock;
+    const long n = n_interior(c);
+    hipLaunchKernelGGL(k_zero_qrhs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    for (int d = 0; d < nd; ++d) {
+        a.dir = d;
+        a.ntype = nts[d];
+        a.lo = lo[d];
+        a.hi = hi[d];
+        a.iss = (nts[d] == 1 || nts[d] == 4) ? 0 : -HM;
+        a.iee = (nts[d] == 2 || nts[d] == 4) ? c->g.n[d] : c->g.n[d] + HM;
+        const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
+        a.a0 = lo[oa];
+        a.a1 = hi[oa];
+        a.b0 = lo[ob];
+        a.b1 = hi[ob];
+        a.xi = c->xi + d * c->xi_stride;
+        a.last = d == nd - 1;
+        const int S = d == 0 ? 256 : 16, TL = d == 0 ? 1 : 16;
+        a.ntile_line = (hi[d] - lo[d] + 1 + S - 2) / (S - 1);
+        a.ntile_a = (a.a1 - a.a0 + 1 + TL - 1) / TL;
+        const long nblk = (long)a.ntile_line * a.ntile_a * (a.b1 - a.b0 + 1);
+        if (nblk <= 0) continue;
+        if (d == 0)
+            hipLaunchKernelGGL(k_uwd<true>, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL(k_uwd<false>, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+        HIPCHK(hipGetLastError());
+    }
+    return ASTR_OK;
+}
+
+int do_grad(astr_ctx* c)
+{
+    if (c->p.deriv_explicit) return do_grad_explicit(c);
+    for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
+    return ASTR_OK;
+}
+
 int do_rhscal(astr_ctx* c)
 {
-    TRY(do_grad_sigma(c));
+    if (c->p.convection == 1) {
+        TRY(do_grad(c));
+        if (c->p.recon_schem == 5 || c->p.lchardecomp) TRY(do_sensor(c));
+        return do_upwind(c);
+    }
+    TRY(do_grad(c));
+    if (c->p.diffterm) TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
     return do_flux_sweeps(c, 0, 0.0);
 }
 
@@ -1000,23 +1453,32 @@ int time_mark(astr_ctx* c, int slot)
 int do_step(astr_ctx* c, double dt, double* stats, bool timed)
 {
     // timing families: 0 filter, 1 qswap, 2 grad, 3 sigma, 4 stats,
-    // 5 flux sweeps (with the RK3 combine), 6 primitive update
+    // 5 flux sweeps (central: with the RK3 combine; upwind: sensor and
+    // interface fluxes), 6 update (central: primitives; upwind: RK3 combine
+    // and primitives)
+    const bool upwind = c->p.convection == 1;
     for (int s = 1; s <= 3; ++s) {
         if (timed) TRY(time_mark(c, 0));
-        TRY(do_filterq(c));
+        if (c->p.lfilter) TRY(do_filterq(c));
         if (timed) TRY(time_mark(c, 1));
         TRY(do_qswap(c));
         if (timed) TRY(time_mark(c, 2));
-        for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
+        TRY(do_grad(c));
         if (timed) TRY(time_mark(c, 3));
-        TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
+        if (!upwind && c->p.diffterm) TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
         if (timed) TRY(time_mark(c, 4));
         // rkfirst's statcal sees the stage-1 state and its gradients
         if (s == 1 && stats) TRY(do_stats(c, stats));
         if (timed) TRY(time_mark(c, 5));
-        TRY(do_flux_sweeps(c, s, dt));
+        if (upwind) {
+            if (c->p.recon_schem == 5 || c->p.lchardecomp) TRY(do_sensor(c));
+            TRY(do_upwind(c));
+        } else {
+            TRY(do_flux_sweeps(c, s, dt));
+        }
         if (timed) TRY(time_mark(c, 6));
-        TRY(launch_pt(c, k_prim, n_interior(c), pt_args(c)));
+        if (upwind) TRY(do_rk(c, s, dt));
+        else TRY(launch_pt(c, k_prim, n_interior(c), pt_args(c)));
         if (timed) {
             TRY(time_mark(c, 7));
             HIPCHK(hipEventSynchronize(c->ev[7]));
@@ -1056,6 +1518,14 @@ void astr_default_params(astr_params* p, int32_t n)
     p->sutherland = 110.3;
     p->beta_halo = 1.11;
     p->beta_bound = 0.98;
+    p->convection = 0;
+    p->deriv_explicit = 0;
+    p->recon_schem = 3;
+    p->lchardecomp = 1;
+    p->lfilter = 1;
+    p->diffterm = 1;
+    p->shkcrt = 0.01;
+    p->bfacmpld = 0.7;
 }
 
 int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
@@ -1067,7 +1537,25 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     if (p->variant != 0) return fail(ASTR_E_UNSUPPORTED, "only variant 0 is implemented");
     const int dims[3] = {p->im, p->jm, p->km};
     const int nts[3] = {p->npdci, p->npdcj, p->npdck};
+    if (p->convection != 0 && p->convection != 1)
+        return fail(ASTR_E_ARG, "convection must be 0 (central) or 1 (upwind)");
+    if (p->deriv_explicit != 0 && p->deriv_explicit != 1)
+        return fail(ASTR_E_ARG, "deriv_explicit must be 0 or 1");
+    const bool two_d = p->km == 0;
+    if (p->convection == 1) {
+        const int r = p->recon_schem;
+        if (!(r == -1 || r == 0 || r == 1 || r == 2 || r == 3 || r == 5 || r == 6))
+            return fail(ASTR_E_ARG, "recon_schem must be -1, 0, 1, 2, 3, 5 or 6");
+        if (p->diffterm)
+            return fail(ASTR_E_UNSUPPORTED, "the upwind path is built for inviscid runs (diffterm 0)");
+        if (nts[1] == 4 || (!two_d && nts[2] == 4))
+            return fail(ASTR_E_UNSUPPORTED, "convrsduwd takes npdc 1..3 on the j and k axes");
+    }
+    if (two_d && (p->convection != 1 || p->deriv_explicit != 1 || nts[2] != 3))
+        return fail(ASTR_E_UNSUPPORTED,
+                    "two-dimensional boxes need convection 1, deriv_explicit 1 and npdck 3");
     for (int d = 0; d < 3; ++d) {
+        if (d == 2 && two_d) continue;
         if (dims[d] < 16) return fail(ASTR_E_UNSUPPORTED, "each axis needs at least 16 cells");
         if (nts[d] < 1 || nts[d] > 4) return fail(ASTR_E_ARG, "npdc must be 1..4");
         const int L = (dims[d] + 7 + LPL - 1) / LPL;
@@ -1093,7 +1581,7 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     c->g.s[1] = c->pi;
     c->g.s[2] = c->pi * c->pj;
     c->g.org = OI + HM * c->g.s[1] + HM * c->g.s[2];
-    const int nfields = 5 + 6 + 5 + 5 + 9 + 3 + 6 + 3 + 1 + 2;
+    const int nfields = 5 + 6 + 5 + 5 + 9 + 3 + 6 + 3 + 1 + 2 + (p->convection == 1 ? 1 : 0);
     if (hipMalloc(&c->base, sizeof(double) * c->nfield * nfields) != hipSuccess)
         return cleanup(fail(ASTR_E_NOMEM, "device allocation of the fields failed"));
     double* f = c->base;
@@ -1109,8 +1597,12 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     c->jac = take();
     c->scratch = take();
     c->scratch2 = take();
+    if (p->convection == 1) c->ssf = take();
     if (hipMemset(c->base, 0, sizeof(double) * c->nfield * nfields) != hipSuccess)
         return cleanup(fail(ASTR_E_HIP, "hipMemset failed"));
+    if (hipMalloc(&c->lshock, c->nfield) != hipSuccess ||
+        hipMemset(c->lshock, 0, c->nfield) != hipSuccess)
+        return cleanup(fail(ASTR_E_NOMEM, "device allocation of the shock flags failed"));
 
     // constants of refcal, src/solver.F90:105-122
     const double g = p->gamma, ma = p->mach;
@@ -1128,9 +1620,12 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     std::vector<double> tabs;
     std::vector<long> offs;
     for (int d = 0; d < 3; ++d) {
-        c->L[d] = (dims[d] + 7 + LPL - 1) / LPL;
+        // a two-dimensional box keeps the line systems of a 16-cell k axis,
+        // which no entry point uses
+        const int nd = (d == 2 && two_d) ? 16 : dims[d];
+        c->L[d] = (nd + 7 + LPL - 1) / LPL;
         for (int op = 0; op < 2; ++op) {
-            HostTab h = build_tab(op, nts[d], dims[d], p->alfa_filter, c->L[d]);
+            HostTab h = build_tab(op, nts[d], nd, p->alfa_filter, c->L[d]);
             offs.push_back((long)tabs.size());
             tabs.insert(tabs.end(), h.t.begin(), h.t.end());
             LineTab& t = c->tab[d][op];
@@ -1180,6 +1675,7 @@ int astr_destroy(astr_ctx* c)
     (void)hipFree(c->fc);
     (void)hipFree(c->xi);
     (void)hipFree(c->part);
+    (void)hipFree(c->lshock);
     for (int e = 0; e < 16; ++e)
         if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1274,15 +1770,23 @@ int astr_download_rhs(astr_ctx* c, double* qrhs, double* dvel, double* dtmp)
     return ASTR_OK;
 }
 
-static int ready(astr_ctx* c)
+static int ready_state(astr_ctx* c)
 {
     if (!c) return fail(ASTR_E_ARG, "null context");
     if (!c->have_geom || !c->have_state) return fail(ASTR_E_STATE, "upload geometry and state first");
+    return set_dev(c);
+}
+
+// stage pieces and steps: three-dimensional periodic boxes
+static int ready(astr_ctx* c)
+{
+    TRY(ready_state(c));
+    if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "two-dimensional boxes support astr_rhscal only");
     for (int d = 0; d < 3; ++d) {
         const int nt = d == 0 ? c->p.npdci : d == 1 ? c->p.npdcj : c->p.npdck;
         if (nt != 3) return fail(ASTR_E_UNSUPPORTED, "the solver steps periodic boxes only");
     }
-    return set_dev(c);
+    return ASTR_OK;
 }
 
 int astr_filterq(astr_ctx* c)
@@ -1299,8 +1803,25 @@ int astr_qswap(astr_ctx* c)
 
 int astr_rhscal(astr_ctx* c)
 {
-    TRY(ready(c));
+    // the upwind right-hand side takes any closure type with halos as given
+    if (c && c->p.convection == 1) TRY(ready_state(c));
+    else TRY(ready(c));
     return do_rhscal(c);
+}
+
+int astr_download_shock(astr_ctx* c, uint8_t* lshock)
+{
+    if (!c || !lshock) return fail(ASTR_E_ARG, "null argument");
+    TRY(set_dev(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<unsigned char> all(c->nfield);
+    HIPCHK(hipMemcpy(all.data(), c->lshock, c->nfield, hipMemcpyDeviceToHost));
+    const int ni = c->g.n[0] + 1, nj = c->g.n[1] + 1, nk = c->g.n[2] + 1;
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i)
+                lshock[i + (long)ni * (j + (long)nj * k)] = all[gidx(c->g, i, j, k)];
+    return ASTR_OK;
 }
 
 int astr_rk3_update(astr_ctx* c, int32_t stage, double dt)
@@ -1374,14 +1895,28 @@ int astr_get_timing(astr_ctx* c, double* ms9)
 static int field_op(astr_ctx* c, int dir, const double* f, double* df, int pass)
 {
     if (!c || !f || !df) return fail(ASTR_E_ARG, "null argument");
-    if (dir < 1 || dir > 3) return fail(ASTR_E_ARG, "dir must be 1..3");
+    if (dir < 1 || dir > ndims_of(c)) return fail(ASTR_E_ARG, "dir must be 1..3 (1..2 in 2-D)");
     TRY(set_dev(c));
     std::vector<double> stage;
     TRY(upload_field(c, c->scratch, f, stage));
-    SweepArgs a = sweep_args(c, dir - 1, pass);
-    a.fin = c->scratch;
-    a.fout = c->scratch2;
-    TRY(launch_sweep(c, a));
+    if (pass == PASS_DERIV_FIELD && c->p.deriv_explicit) {
+        FieldXArgs a{};
+        a.g = c->g;
+        a.dir = dir - 1;
+        a.ntype = dir == 1 ? c->p.npdci : dir == 2 ? c->p.npdcj : c->p.npdck;
+        a.fin = c->scratch;
+        a.fout = c->scratch2;
+        const long n = n_interior(c);
+        hipLaunchKernelGGL(k_deriv_explicit_field, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           c->stream, a);
+        HIPCHK(hipGetLastError());
+    } else {
+        if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "compact operators need a 3-D box");
+        SweepArgs a = sweep_args(c, dir - 1, pass);
+        a.fin = c->scratch;
+        a.fout = c->scratch2;
+        TRY(launch_sweep(c, a));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     TRY(download_field(c, c->scratch2, df, stage, true));
     return ASTR_OK;

@@ -19,6 +19,9 @@ module astr_hip
     integer(c_int32_t) :: variant, reserved
     real(c_double) :: alfa_filter, reynolds, mach, prandtl, gamma
     real(c_double) :: ref_tem, sutherland, beta_halo, beta_bound
+    integer(c_int32_t) :: convection, deriv_explicit, recon_schem, lchardecomp
+    integer(c_int32_t) :: lfilter, diffterm
+    real(c_double) :: shkcrt, bfacmpld
   end type astr_params
 
   interface
@@ -116,6 +119,13 @@ module astr_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: qrhs(*), dvel(*), dtmp(*)
     end function astr_download_rhs
+
+    ! lshock(0:im,0:jm,0:km) of the last ducrossensor, as bytes
+    integer(c_int) function astr_download_shock(ctx, lshock) bind(c, name='astr_download_shock')
+      import :: c_int, c_ptr, c_int8_t
+      type(c_ptr), value :: ctx
+      integer(c_int8_t), intent(out) :: lshock(*)
+    end function astr_download_shock
 
     ! fds%central / compact_filter of a whole haloed field along dir
     integer(c_int) function astr_deriv_field(ctx, dir, f, df) bind(c, name='astr_deriv_field')

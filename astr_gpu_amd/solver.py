@@ -97,6 +97,13 @@ class AstrSolver:
                                          dptr(out["dtmp"])), "astr_download_rhs")
         return out
 
+    def download_shock(self) -> np.ndarray:
+        """lshock(0:im,0:jm,0:km) of the last shock sensor, as booleans."""
+        out = np.zeros((self.p.im + 1, self.p.jm + 1, self.p.km + 1), dtype=np.uint8, order="F")
+        check(self.lib.astr_download_shock(
+            self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), "astr_download_shock")
+        return out.astype(bool)
+
     # ------------------------------------------------------- stage pieces
     def filterq(self) -> None:
         check(self.lib.astr_filterq(self.ctx), "astr_filterq")
@@ -167,8 +174,10 @@ class AstrSolver:
 def box_metrics(p: AstrParams, lengths=(2 * math.pi,) * 3):
     """Metrics of the uniform box of gridcube (src/gridgeneration.F90:233-267):
     dxi = diag(1/dx, 1/dy, 1/dz), jacob = dx*dy*dz, halos included (the
-    single-rank grid exchange extends the grid linearly)."""
-    dx = [lengths[0] / p.im, lengths[1] / p.jm, lengths[2] / p.km]
+    single-rank grid exchange extends the grid linearly).  An axis of 0
+    cells (a two-dimensional box) has unit spacing."""
+    n = (p.im, p.jm, p.km)
+    dx = [lengths[d] / n[d] if n[d] else 1.0 for d in range(3)]
     s = haloed_shape(p)
     jacob = np.full(s, dx[0] * dx[1] * dx[2], order="F")
     dxi = np.zeros(s + (3, 3), order="F")

@@ -51,18 +51,37 @@ typedef struct astr_params {
     double sutherland;           /* 110.3 (src/solver.F90:121)            */
     double beta_halo;            /* 1.11 (src/comsolver.F90:121)          */
     double beta_bound;           /* 0.98 (src/comsolver.F90:121)          */
+    /* scheme switches of the input deck (src/readwrite.F90:466-771)       */
+    int32_t convection;          /* 0: central compact, conschm '643c'
+                                    (convrsdcal6); 1: explicit upwind,
+                                    conschm odd order with 'e' (convrsduwd) */
+    int32_t deriv_explicit;      /* difschm(4:4): 0 'c' compact (df_compact),
+                                    1 'e' explicit (diff6ec)               */
+    int32_t recon_schem;         /* -1, 0 SUW, 1 WENO, 2 WENO-Z, 3 MP,
+                                    5 MP-LD, 6 ROUND (src/flux.F90:269)    */
+    int32_t lchardecomp;         /* characteristic projection (1/0)       */
+    int32_t lfilter;             /* filter q at every stage (1/0)         */
+    int32_t diffterm;            /* viscous terms (1/0)                   */
+    double shkcrt;               /* Ducros sensor threshold               */
+    double bfacmpld;             /* MP-LD dissipation weight              */
 } astr_params;
 
 typedef struct astr_ctx astr_ctx;
 
 /* Fill *p with the Taylor-Green deck values (examples/Taylor_Green_Vortex/
- * datin/input.tgv, Re 1600, Ma 0.1) for an n x n x n periodic box. */
+ * datin/input.tgv, Re 1600, Ma 0.1, '643c' central scheme, filter and
+ * viscous terms on) for an n x n x n periodic box.  The upwind fields take
+ * the Riemann2D deck's values (examples/Riemann2D/datin/input.riemann2d:
+ * recon_schem 3, lchardecomp on, bfacmpld 0.7, shkcrt 0.01). */
 void astr_default_params(astr_params *p, int32_t n);
 
 /* Replaces solvrinit (src/comsolver.F90:47-146) and allocommarray: builds
  * the pre-factored line systems (fd_scheme_initiate, src/derivative.F90:
  * 63-158; compact_filter_initiate + filter_coefficient_cal, src/filter.F90:
- * 31-100,299-432) and allocates the device fields on HIP device `device`. */
+ * 31-100,299-432) and allocates the device fields on HIP device `device`.
+ * km = 0 is a two-dimensional box (convection 1 with deriv_explicit 1 and
+ * npdck 3; astr_rhscal only).  With convection 1 the j and k axes take
+ * npdc 1..3: convrsduwd stops on 4 there (src/solver.F90:808-822). */
 int astr_create(const astr_params *p, int32_t device, astr_ctx **out);
 int astr_destroy(astr_ctx *ctx);
 
@@ -89,7 +108,11 @@ int astr_qswap(astr_ctx *ctx);
 /* qrhs=0 (src/mainloop.F90:404) then call rhscal (src/solver.F90:184-283:
  * gradcal, convrsdcal6, qrhs=-qrhs, diffrsdcal6).  The inviscid and viscous
  * flux derivatives of one axis are taken as one derivative of their
- * difference (the operator is linear; only the summation order differs). */
+ * difference (the operator is linear; only the summation order differs).
+ * With convection 1: gradcal, ducrossensor when recon_schem is 5 or
+ * lchardecomp is set, convrsduwd, qrhs=-qrhs; any closure type, halos used
+ * as given; qrhs is set on is..ie of each axis (ntype 1: 1..n, 2: 0..n-1,
+ * 3: 0..n, 4: 1..n-1) and zero elsewhere. */
 int astr_rhscal(astr_ctx *ctx);
 /* RK3 combine of stage s (1..3) and updatefvar, src/mainloop.F90:423-476;
  * stage 1 also saves qsave = q*jacob. */
@@ -110,6 +133,9 @@ int astr_rk3_step(astr_ctx *ctx, double dt, double *stats);
 int astr_rk3_steps(astr_ctx *ctx, int32_t nsteps, double dt, double *stats);
 
 /* --- results of the last rhscal, for parity checks --------------------- */
+/* lshock(0:im,0:jm,0:km) of the last ducrossensor (src/commcal.F90:196-353)
+ * as bytes 0/1; all zero when the upwind path has not run the sensor. */
+int astr_download_shock(astr_ctx *ctx, uint8_t *lshock);
 /* qrhs(0:im,0:jm,0:km,1:5), dvel(0:im,0:jm,0:km,1:3,1:3),
  * dtmp(0:im,0:jm,0:km,1:3) */
 int astr_download_rhs(astr_ctx *ctx, double *qrhs, double *dvel, double *dtmp);
@@ -118,8 +144,9 @@ int astr_download_rhs(astr_ctx *ctx, double *qrhs, double *dvel, double *dtmp);
 
 /* fds%central along axis dir (1,2,3) for every line of a haloed field:
  * f(-hm:im+hm,-hm:jm+hm,-hm:km+hm) -> df(0:im,0:jm,0:km), index-space
- * derivative (src/derivative.F90:16-46,171-306), closure npdc{i,j,k} of
- * the context.  Halos are used as given. */
+ * derivative (src/derivative.F90:16-46,171-306; diff6ec, :350-413, when
+ * deriv_explicit is set), closure npdc{i,j,k} of the context.  Halos are
+ * used as given. */
 int astr_deriv_field(astr_ctx *ctx, int32_t dir, const double *f, double *df);
 /* compact_filter along axis dir for every line (src/filter.F90:112-285):
  * f(-hm:...) haloed in, ff(0:im,0:jm,0:km) out.  Halos used as given. */

@@ -37,12 +37,15 @@ def test_binding_covers_every_declared_symbol():
 
 
 def test_params_struct_layout():
-    # 10 int32 followed by 9 doubles, no padding
-    assert ctypes.sizeof(_lib.AstrParams) == 10 * 4 + 9 * 8
+    # 10 int32, 9 doubles, 6 int32, 2 doubles: no padding
+    assert ctypes.sizeof(_lib.AstrParams) == 10 * 4 + 9 * 8 + 6 * 4 + 2 * 8
+    assert _lib.AstrParams.shkcrt.offset == 10 * 4 + 9 * 8 + 6 * 4
     p = ag.default_params(64)
     assert (p.im, p.jm, p.km, p.hm, p.numq) == (64, 64, 64, 5, 5)
     assert (p.npdci, p.npdcj, p.npdck) == (3, 3, 3)
     assert p.alfa_filter == 0.49 and p.reynolds == 1600.0 and p.mach == 0.1
+    assert (p.convection, p.deriv_explicit, p.recon_schem, p.lchardecomp) == (0, 0, 3, 1)
+    assert (p.lfilter, p.diffterm, p.shkcrt, p.bfacmpld) == (1, 1, 0.01, 0.7)
     assert p.sutherland == 110.3 and p.ref_tem == 273.15
 
 
@@ -59,6 +62,19 @@ def test_create_rejects_unsupported_without_touching_the_gpu():
     p.npdci = 7
     with pytest.raises(ag.AstrError, match="ASTR_E_ARG"):
         ag.AstrSolver(p)
+    # upwind path: inviscid only; convrsduwd stops on ntype 4 along j and k
+    # (src/solver.F90:808-822,1003-1014); 2-D boxes only for it
+    bad = [dict(convection=1, diffterm=1), dict(convection=1, diffterm=0, npdcj=4),
+           dict(convection=1, diffterm=0, npdck=4), dict(convection=0, km=0),
+           dict(convection=1, diffterm=0, deriv_explicit=0, km=0)]
+    for over in bad:
+        p = ag.default_params(32, **over)
+        with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
+            ag.AstrSolver(p)
+    for over in [dict(convection=2), dict(convection=1, diffterm=0, recon_schem=4)]:
+        p = ag.default_params(32, **over)
+        with pytest.raises(ag.AstrError, match="ASTR_E_ARG"):
+            ag.AstrSolver(p)
 
 
 def test_missing_library_fails_loudly(tmp_path, monkeypatch):
