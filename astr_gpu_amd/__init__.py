@@ -5,7 +5,7 @@ C ABI of include/astr_hip.h; this package is its host-side binding.
 """
 from ._lib import AstrError, AstrParams, load  # noqa: F401
 from .solver import (AstrSolver, box_metrics, default_params, haloed_shape,  # noqa: F401
-                     interior, tgv_state)
+                     interior, riemann2d_state, tgv_state)
 
 __all__ = ["AstrError", "AstrParams", "AstrSolver", "box_metrics", "default_params",
-           "haloed_shape", "interior", "load", "tgv_state"]
+           "haloed_shape", "interior", "load", "riemann2d_state", "tgv_state"]

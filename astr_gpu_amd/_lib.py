@@ -60,6 +60,7 @@ SIGNATURES = {
     "astr_deriv_field": (ctypes.c_int, [_P, _I, _D, _D]),
     "astr_filter_field": (ctypes.c_int, [_P, _I, _D, _D]),
     "astr_set_timing": (ctypes.c_int, [_P, _I]),
+    "astr_synchronize": (ctypes.c_int, [_P]),
     "astr_bench_copy": (ctypes.c_int, [_P, _I, _D, ctypes.POINTER(ctypes.c_int64)]),
     "astr_get_timing": (ctypes.c_int, [_P, _D]),
     "astr_last_error": (ctypes.c_char_p, []),

@@ -628,6 +628,9 @@ struct UwdArgs {
     const double* xi;       // metric along dir, xi[n+HM]
     const unsigned char* lshock;
     double* qrhs[5];
+    // compact upwind: interface fluxes of F+ / F- at node index i (i+1/2)
+    const double* fhp[5];
+    const double* fhm[5];
 };
 
 struct SensorArgs {
@@ -791,6 +794,131 @@ __global__ void k_deriv_explicit_field(FieldXArgs a)
     a.fout[gi] = diff6ec_at(a.fin + (gi - (long)c[a.dir] * st), st, c[a.dir], a.g.n[a.dir], a.ntype);
 }
 
+struct CmpArgs {
+    Geo g;
+    int dir, ntype, ndims;
+    int iss, iee;
+    int a0, a1, b0, b1, na;
+    int first, last;           // rows of the compact flux system
+    double bfac, gamma, mach;
+    const double* q[5];
+    const double* prim[6];
+    const double* jac;
+    const double* xi;          // metric along dir, xi[n+HM]
+    double* sw[10];            // F+(1:5), F-(1:5) on iss..iee
+    double* fh[10];            // interface fluxes of F+ / F-, rows first..last
+    const double* ac[2];       // Thomas factors (3 per row) of the + / - system
+};
+
+// flux_steger_warming on the nodes iss..iee of every line of one axis,
+// src/solver.F90:1326-1336, into the 10 split-flux fields
+__global__ void k_sw_field(CmpArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nt = a.iee - a.iss + 1;
+    const long nl = (long)a.na * (a.b1 - a.b0 + 1);
+    if (p >= nt * nl) return;
+    // consecutive threads walk the fastest-varying memory axis
+    int t, av, b;
+    if (a.dir == 0) {
+        t = a.iss + (int)(p % nt);
+        const long r = p / nt;
+        av = a.a0 + (int)(r % a.na);
+        b = a.b0 + (int)(r / a.na);
+    } else {
+        av = a.a0 + (int)(p % a.na);
+        const long r = p / a.na;
+        t = a.iss + (int)(r % nt);
+        b = a.b0 + (int)(r / nt);
+    }
+    int c[3];
+    c[a.dir] = t;
+    c[a.dir == 0 ? 1 : 0] = av;
+    c[a.dir == 2 ? 1 : 2] = b;
+    const long gi = gidx(a.g, c[0], c[1], c[2]);
+    const double vel[3] = {a.prim[1][gi], a.prim[2][gi], a.prim[3][gi]};
+    double q[5];
+    for (int m = 0; m < 5; ++m) q[m] = a.q[m][gi];
+    double dxi[3] = {0.0, 0.0, 0.0};
+    dxi[a.dir] = a.xi[t + HM];
+    double fp[5], fm[5];
+    uwd::sw_split(a.gamma, a.mach, a.prim[0][gi], vel, a.prim[4][gi], a.prim[5][gi], q, dxi,
+                  a.jac[gi], fp, fm);
+    for (int m = 0; m < 5; ++m) {
+        a.sw[m][gi] = fp[m];
+        a.sw[5 + m][gi] = fm[m];
+    }
+}
+
+// flux_compact (src/flux.F90:125-256) of one split-flux component along one
+// line, per thread: the right-hand side of compact_flux_rhs, then the
+// sequential tridiagonal_thomas_solver (src/commfunc.F90:790-813) in place
+// in the output field (row r at node index r).
+__global__ void k_flux_compact(CmpArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nl = (long)a.na * (a.b1 - a.b0 + 1);
+    if (p >= nl * 10) return;
+    const int av = a.a0 + (int)(p % a.na);
+    const long r = p / a.na;
+    const int b = a.b0 + (int)(r % (a.b1 - a.b0 + 1));
+    const int comp = (int)(r / (a.b1 - a.b0 + 1));  // 0..4 F+, 5..9 F-
+    const bool plus = comp < 5;
+    int c[3];
+    c[a.dir] = 0;
+    c[a.dir == 0 ? 1 : 0] = av;
+    c[a.dir == 2 ? 1 : 2] = b;
+    const long st = a.g.s[a.dir];
+    const double* f = a.sw[comp] + gidx(a.g, c[0], c[1], c[2]);
+    double* d = a.fh[comp] + gidx(a.g, c[0], c[1], c[2]);
+    const double* ac = a.ac[plus ? 0 : 1];
+    auto F = [&](int n) { return f[(long)n * st]; };
+    const int i0 = a.first, im_ = a.last, nt = a.ntype;
+    const double bf = a.bfac;
+    int is = i0 + 1, ie = im_ - 1;
+    // compact_flux_rhs, src/flux.F90:187-254
+    if (nt == 1 || nt == 4) {
+        d[(long)i0 * st] = 2.5 * F(i0 + 1) + 0.5 * F(i0 + 2);
+        d[(long)(i0 + 1) * st] = 0.75 * F(i0 + 1) + 0.75 * F(i0 + 2);
+        is = i0 + 2;
+    } else {
+        const int j = i0;
+        const double v1 = F(j) + F(j + 1), v2 = F(j - 1) + F(j + 2), v3 = F(j - 2) + F(j + 3);
+        d[(long)j * st] = (37.0 / 60.0) * v1 - (2.0 / 15.0) * v2 + (1.0 / 60.0) * v3;
+    }
+    if (nt == 2 || nt == 4) {
+        d[(long)(im_ - 1) * st] = 0.75 * F(im_) + 0.75 * F(im_ - 1);
+        d[(long)im_ * st] = 2.5 * F(im_) + 0.5 * F(im_ - 1);
+        ie = im_ - 2;
+    } else {
+        const int j = im_;
+        const double v1 = F(j) + F(j + 1), v2 = F(j - 1) + F(j + 2), v3 = F(j - 2) + F(j + 3);
+        d[(long)j * st] = (37.0 / 60.0) * v1 - (2.0 / 15.0) * v2 + (1.0 / 60.0) * v3;
+    }
+    const double c0 = (1.0 / 18.0) - (1.0 / 36.0) * bf, c1 = (19.0 / 18.0) - (9.0 / 36.0) * bf;
+    const double c2 = (5.0 / 9.0) + (9.0 / 36.0) * bf, c3 = (1.0 / 36.0) * bf;
+    for (int j = is; j <= ie; ++j) {
+        double v;
+        if (plus) v = c0 * F(j - 1) + c1 * F(j) + c2 * F(j + 1) + c3 * F(j + 2);
+        else v = c0 * F(j + 2) + c1 * F(j + 1) + c2 * F(j) + c3 * F(j - 1);
+        d[(long)j * st] = v;
+    }
+    // forward elimination and back substitution (rows 0..n-1 = i0..im_)
+    const int n = im_ - i0 + 1;
+    double prev = d[(long)i0 * st];
+    for (int k = 1; k < n; ++k) {
+        double* dk = d + (long)(i0 + k) * st;
+        prev = *dk * ac[3 * k + 1] - prev * ac[3 * k + 2];
+        *dk = prev;
+    }
+    double nxt = prev;
+    for (int k = n - 2; k >= 0; --k) {
+        double* dk = d + (long)(i0 + k) * st;
+        nxt = *dk - ac[3 * k + 0] * nxt;
+        *dk = nxt;
+    }
+}
+
 // Zero qrhs on the interior (qrhs=0 before rhscal, src/mainloop.F90:404)
 __global__ void k_zero_qrhs(UwdArgs a)
 {
@@ -812,7 +940,7 @@ __global__ void k_zero_qrhs(UwdArgs a)
 //     (negated on the last axis: qrhs=-qrhs, src/solver.F90:243).
 // x tiles are 1 line x 256 interfaces; y/z tiles 16 lines (consecutive i,
 // 128-byte rows) x 16 interfaces.
-template <bool XD>
+template <bool XD, bool CMP>
 __global__ __launch_bounds__(256) void k_uwd(UwdArgs a)
 {
     constexpr int TL = XD ? 1 : 16;
@@ -865,7 +993,8 @@ __global__ __launch_bounds__(256) void k_uwd(UwdArgs a)
             line_node(l, ii + 1 > dim ? ii : ii + 1, i, j, k);
             lsh = s1 || a.lshock[gidx(a.g, i, j, k)] != 0;
         }
-        const bool chr = a.lchar && lsh;
+        // convrsdcmp projects every interface, convrsduwd only flagged ones
+        const bool chr = a.lchar && (CMP || lsh);
         uwd::Roe r;
         if (chr) {
             int i, j, k;
@@ -883,6 +1012,46 @@ __global__ __launch_bounds__(256) void k_uwd(UwdArgs a)
         }
         const double* base = sw + l * NN * 10;
         double fhc[5];
+        if constexpr (CMP) {
+            // convrsdcmp (src/solver.F90:1340-1460): 5-point stencils of
+            // iwind6 about the compact interface values, then mplimiter
+            int i, j, k;
+            line_node(l, ii, i, j, k);
+            const long gf = gidx(a.g, i, j, k);  // interface ii+1/2 sits at node ii
+            double cp5[5], cm5[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                cp5[v] = a.fhp[v][gf];
+                cm5[v] = a.fhm[v][gf];
+            }
+            for (int m = 0; m < 5; ++m) {
+                double fp[5], fm[5], cpl, cml;
+                if (chr) {
+                    double lv[5];
+                    uwd::lev_row(r, m, lv);
+#pragma unroll
+                    for (int w = 0; w < 5; ++w) {
+                        const double* P = base + (s + 1 + w) * 10;      // node ii+w-2
+                        const double* M = base + (s + 6 - w) * 10 + 5;  // node ii+3-w
+                        fp[w] = lv[0] * P[0] + lv[1] * P[1] + lv[2] * P[2] + lv[3] * P[3] + lv[4] * P[4];
+                        fm[w] = lv[0] * M[0] + lv[1] * M[1] + lv[2] * M[2] + lv[3] * M[3] + lv[4] * M[4];
+                    }
+                    cpl = lv[0] * cp5[0] + lv[1] * cp5[1] + lv[2] * cp5[2] + lv[3] * cp5[3] + lv[4] * cp5[4];
+                    cml = lv[0] * cm5[0] + lv[1] * cm5[1] + lv[2] * cm5[2] + lv[3] * cm5[3] + lv[4] * cm5[4];
+                } else {
+#pragma unroll
+                    for (int w = 0; w < 5; ++w) {
+                        fp[w] = base[(s + 1 + w) * 10 + m];
+                        fm[w] = base[(s + 6 - w) * 10 + 5 + m];
+                    }
+                    cpl = cp5[m];
+                    cml = cm5[m];
+                }
+                const double v1 = uwd::mplimiter(fp, cpl, lsh, ii, dim, a.ntype);
+                const double v2 = uwd::mplimiter(fm, cml, lsh, ii, dim, a.ntype);
+                fhc[m] = v1 + v2;
+            }
+        } else
         for (int m = 0; m < 5; ++m) {
             double fp[8], fm[8];
             if (chr) {
@@ -960,6 +1129,10 @@ struct astr_ctx {
     long pi = 0, pj = 0, pk = 0, nfield = 0;  // padded dims, elements per field
     double* base = nullptr;
     double *q[5], *prim[6], *qsave[5], *qrhs[5], *dvel[9], *dtmp[3], *sig[6], *qfl[3], *jac;
+    double* swf[10] = {};           // compact upwind: split fluxes F+, F-
+    double* fhf[10] = {};           // compact upwind: their interface fluxes
+    double* cmp_ac[3][2] = {};      // compact upwind Thomas factors per axis, wind
+    int cmp_first[3] = {}, cmp_last[3] = {};
     double* ssf = nullptr;          // Ducros sensor (upwind path), haloed
     unsigned char* lshock = nullptr; // shock flags, one byte per padded cell
     double* scratch = nullptr;  // field in of the field-level operators
@@ -1034,6 +1207,34 @@ HostTab build_tab(int op, int ntype, int dim, double alfa, int L)
     }
     ac1[R - 1] = 0.0;  // the solver starts the back substitution at x(n)=d(n)
     return h;
+}
+
+// compact_flux_initiate (scheme 543), src/flux.F90:32-108, and its Thomas
+// factors (tridiagonal_thomas_proprocess, src/commfunc.F90:752-774), three
+// per row as the sequential solver reads them
+std::vector<double> build_flux_tab(int ntype, int dim, bool plus, double bf, int& first, int& last)
+{
+    first = (ntype == 1 || ntype == 4) ? -1 : -2;
+    last = (ntype == 2 || ntype == 4) ? dim : dim + 1;
+    const int n = last - first + 1;
+    std::vector<double> a(n), c(n), ac(3 * n);
+    for (int r = 0; r < n; ++r) {
+        a[r] = plus ? 0.5 - (1.0 / 6.0) * bf : (1.0 / 6.0) + (1.0 / 6.0) * bf;
+        c[r] = plus ? (1.0 / 6.0) + (1.0 / 6.0) * bf : 0.5 - (1.0 / 6.0) * bf;
+    }
+    a[0] = c[0] = 0.0;
+    a[n - 1] = c[n - 1] = 0.0;
+    if (ntype == 1 || ntype == 4) { a[0] = c[0] = 2.0; a[1] = c[1] = 0.25; }
+    if (ntype == 2 || ntype == 4) { a[n - 1] = c[n - 1] = 2.0; a[n - 2] = c[n - 2] = 0.25; }
+    ac[0] = c[0];
+    ac[1] = 1.0;
+    ac[2] = 0.0;
+    for (int r = 1; r < n; ++r) {
+        ac[3 * r + 0] = c[r] / (1.0 - a[r] * ac[3 * (r - 1)]);
+        ac[3 * r + 1] = 1.0 / (1.0 - a[r] * ac[3 * (r - 1)]);
+        ac[3 * r + 2] = a[r] / (1.0 - a[r] * ac[3 * (r - 1)]);
+    }
+    return ac;
 }
 
 // filter_coefficient_cal closed forms, src/filter.F90:303-431
@@ -1253,6 +1454,16 @@ long n_haloed(const astr_ctx* c)
     return (long)(c->g.n[0] + 1 + 2 * HM) * (c->g.n[1] + 1 + 2 * HM) * (c->g.n[2] + 1 + 2 * HM);
 }
 
+// ducrossensor runs for recon_schem 5 or lchardecomp with the explicit
+// upwind scheme (src/solver.F90:227), for lchardecomp with the compact one
+// (src/solver.F90:233)
+bool sensor_on(const astr_ctx* c)
+{
+    if (c->p.convection == 1) return c->p.recon_schem == 5 || c->p.lchardecomp;
+    if (c->p.convection == 2) return c->p.lchardecomp != 0;
+    return false;
+}
+
 // gradcal with the explicit derivative (difschm '643e')
 int do_grad_explicit(astr_ctx* c)
 {
@@ -1304,9 +1515,11 @@ void uwd_range(int ntype, int n, int& lo, int& hi)
     hi = (ntype == 2 || ntype == 4) ? n - 1 : n;
 }
 
-// qrhs=0, then convrsduwd along each axis, the last one negating
+// qrhs=0, then convrsduwd (convection 1) or convrsdcmp (convection 2)
+// along each axis, the last one negating
 int do_upwind(astr_ctx* c)
 {
+    const bool cmp = c->p.convection == 2;
     const int nd = ndims_of(c);
     const int nts[3] = {c->p.npdci, c->p.npdcj, c->p.npdck};
     int lo[3], hi[3];
@@ -1317,7 +1530,7 @@ int do_upwind(astr_ctx* c)
     a.ndims = nd;
     a.recon = c->p.recon_schem;
     a.lchar = c->p.lchardecomp;
-    a.sensor = (c->p.recon_schem == 5 || c->p.lchardecomp) ? 1 : 0;
+    a.sensor = sensor_on(c) ? 1 : 0;
     a.bfac = c->p.bfacmpld;
     a.gamma = c->p.gamma;
     a.mach = c->p.mach;
@@ -1347,10 +1560,56 @@ int do_upwind(astr_ctx* c)
         a.ntile_a = (a.a1 - a.a0 + 1 + TL - 1) / TL;
         const long nblk = (long)a.ntile_line * a.ntile_a * (a.b1 - a.b0 + 1);
         if (nblk <= 0) continue;
-        if (d == 0)
-            hipLaunchKernelGGL(k_uwd<true>, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
-        else
-            hipLaunchKernelGGL(k_uwd<false>, dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+        if (cmp) {
+            // split fluxes and their compact interface fluxes first
+            CmpArgs ca{};
+            ca.g = c->g;
+            ca.dir = d;
+            ca.ntype = nts[d];
+            ca.ndims = nd;
+            ca.iss = a.iss;
+            ca.iee = a.iee;
+            ca.a0 = a.a0;
+            ca.a1 = a.a1;
+            ca.b0 = a.b0;
+            ca.b1 = a.b1;
+            ca.na = a.a1 - a.a0 + 1;
+            ca.first = c->cmp_first[d];
+            ca.last = c->cmp_last[d];
+            ca.bfac = c->p.bfacmpld;
+            ca.gamma = c->p.gamma;
+            ca.mach = c->p.mach;
+            for (int m = 0; m < 5; ++m) ca.q[m] = c->q[m];
+            for (int m = 0; m < 6; ++m) ca.prim[m] = c->prim[m];
+            ca.jac = c->jac;
+            ca.xi = a.xi;
+            for (int m = 0; m < 10; ++m) {
+                ca.sw[m] = c->swf[m];
+                ca.fh[m] = c->fhf[m];
+            }
+            ca.ac[0] = c->cmp_ac[d][0];
+            ca.ac[1] = c->cmp_ac[d][1];
+            const long nl = (long)ca.na * (ca.b1 - ca.b0 + 1);
+            const long nsw = nl * (ca.iee - ca.iss + 1);
+            hipLaunchKernelGGL(k_sw_field, dim3((unsigned)((nsw + 255) / 256)), dim3(256), 0,
+                               c->stream, ca);
+            hipLaunchKernelGGL(k_flux_compact, dim3((unsigned)((10 * nl + 127) / 128)), dim3(128),
+                               0, c->stream, ca);
+            HIPCHK(hipGetLastError());
+            for (int m = 0; m < 5; ++m) {
+                a.fhp[m] = c->fhf[m];
+                a.fhm[m] = c->fhf[5 + m];
+            }
+            if (d == 0)
+                hipLaunchKernelGGL((k_uwd<true, true>), dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL((k_uwd<false, true>), dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+        } else {
+            if (d == 0)
+                hipLaunchKernelGGL((k_uwd<true, false>), dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL((k_uwd<false, false>), dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+        }
         HIPCHK(hipGetLastError());
     }
     return ASTR_OK;
@@ -1363,12 +1622,31 @@ int do_grad(astr_ctx* c)
     return ASTR_OK;
 }
 
-int do_rhscal(astr_ctx* c)
+int time_mark(astr_ctx* c, int slot);
+
+int do_rhscal(astr_ctx* c, bool timed = false)
 {
-    if (c->p.convection == 1) {
+    if (c->p.convection != 0) {
+        // timing families of an rhscal call: 2 gradient, 8 shock sensor,
+        // 5 interface-flux sweeps, 7 total
+        if (timed) TRY(time_mark(c, 0));
         TRY(do_grad(c));
-        if (c->p.recon_schem == 5 || c->p.lchardecomp) TRY(do_sensor(c));
-        return do_upwind(c);
+        if (timed) TRY(time_mark(c, 1));
+        if (sensor_on(c)) TRY(do_sensor(c));
+        if (timed) TRY(time_mark(c, 2));
+        TRY(do_upwind(c));
+        if (timed) {
+            TRY(time_mark(c, 3));
+            HIPCHK(hipEventSynchronize(c->ev[3]));
+            const int slot[3] = {2, 8, 5};
+            for (int f = 0; f < 3; ++f) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, c->ev[f], c->ev[f + 1]));
+                c->tms[slot[f]] += ms;
+                c->tms[7] += ms;
+            }
+        }
+        return ASTR_OK;
     }
     TRY(do_grad(c));
     if (c->p.diffterm) TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
@@ -1456,7 +1734,7 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
     // 5 flux sweeps (central: with the RK3 combine; upwind: sensor and
     // interface fluxes), 6 update (central: primitives; upwind: RK3 combine
     // and primitives)
-    const bool upwind = c->p.convection == 1;
+    const bool upwind = c->p.convection != 0;
     for (int s = 1; s <= 3; ++s) {
         if (timed) TRY(time_mark(c, 0));
         if (c->p.lfilter) TRY(do_filterq(c));
@@ -1471,7 +1749,7 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
         if (s == 1 && stats) TRY(do_stats(c, stats));
         if (timed) TRY(time_mark(c, 5));
         if (upwind) {
-            if (c->p.recon_schem == 5 || c->p.lchardecomp) TRY(do_sensor(c));
+            if (sensor_on(c)) TRY(do_sensor(c));
             TRY(do_upwind(c));
         } else {
             TRY(do_flux_sweeps(c, s, dt));
@@ -1537,11 +1815,13 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     if (p->variant != 0) return fail(ASTR_E_UNSUPPORTED, "only variant 0 is implemented");
     const int dims[3] = {p->im, p->jm, p->km};
     const int nts[3] = {p->npdci, p->npdcj, p->npdck};
-    if (p->convection != 0 && p->convection != 1)
-        return fail(ASTR_E_ARG, "convection must be 0 (central) or 1 (upwind)");
+    if (p->convection < 0 || p->convection > 2)
+        return fail(ASTR_E_ARG, "convection must be 0 (central), 1 (upwind) or 2 (compact upwind)");
     if (p->deriv_explicit != 0 && p->deriv_explicit != 1)
         return fail(ASTR_E_ARG, "deriv_explicit must be 0 or 1");
     const bool two_d = p->km == 0;
+    if (p->convection == 2 && p->diffterm)
+        return fail(ASTR_E_UNSUPPORTED, "the upwind path is built for inviscid runs (diffterm 0)");
     if (p->convection == 1) {
         const int r = p->recon_schem;
         if (!(r == -1 || r == 0 || r == 1 || r == 2 || r == 3 || r == 5 || r == 6))
@@ -1551,9 +1831,9 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
         if (nts[1] == 4 || (!two_d && nts[2] == 4))
             return fail(ASTR_E_UNSUPPORTED, "convrsduwd takes npdc 1..3 on the j and k axes");
     }
-    if (two_d && (p->convection != 1 || p->deriv_explicit != 1 || nts[2] != 3))
+    if (two_d && (p->convection == 0 || p->deriv_explicit != 1 || nts[2] != 3))
         return fail(ASTR_E_UNSUPPORTED,
-                    "two-dimensional boxes need convection 1, deriv_explicit 1 and npdck 3");
+                    "two-dimensional boxes need an upwind convection, deriv_explicit 1 and npdck 3");
     for (int d = 0; d < 3; ++d) {
         if (d == 2 && two_d) continue;
         if (dims[d] < 16) return fail(ASTR_E_UNSUPPORTED, "each axis needs at least 16 cells");
@@ -1581,7 +1861,8 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     c->g.s[1] = c->pi;
     c->g.s[2] = c->pi * c->pj;
     c->g.org = OI + HM * c->g.s[1] + HM * c->g.s[2];
-    const int nfields = 5 + 6 + 5 + 5 + 9 + 3 + 6 + 3 + 1 + 2 + (p->convection == 1 ? 1 : 0);
+    const int nfields = 5 + 6 + 5 + 5 + 9 + 3 + 6 + 3 + 1 + 2 + (p->convection != 0 ? 1 : 0) +
+                        (p->convection == 2 ? 20 : 0);
     if (hipMalloc(&c->base, sizeof(double) * c->nfield * nfields) != hipSuccess)
         return cleanup(fail(ASTR_E_NOMEM, "device allocation of the fields failed"));
     double* f = c->base;
@@ -1597,7 +1878,12 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     c->jac = take();
     c->scratch = take();
     c->scratch2 = take();
-    if (p->convection == 1) c->ssf = take();
+    if (p->convection != 0) c->ssf = take();
+    if (p->convection == 2)
+        for (int m = 0; m < 10; ++m) {
+            c->swf[m] = take();
+            c->fhf[m] = take();
+        }
     if (hipMemset(c->base, 0, sizeof(double) * c->nfield * nfields) != hipSuccess)
         return cleanup(fail(ASTR_E_HIP, "hipMemset failed"));
     if (hipMalloc(&c->lshock, c->nfield) != hipSuccess ||
@@ -1645,6 +1931,16 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     for (int d = 0, s = 0; d < 3; ++d)
         for (int op = 0; op < 2; ++op, ++s) c->tab[d][op].t = c->tabs + offs[s];
 
+    if (p->convection == 2)
+        for (int d = 0; d < (two_d ? 2 : 3); ++d)
+            for (int w = 0; w < 2; ++w) {
+                std::vector<double> ac = build_flux_tab(nts[d], dims[d], w == 0, p->bfacmpld,
+                                                        c->cmp_first[d], c->cmp_last[d]);
+                if (hipMalloc(&c->cmp_ac[d][w], sizeof(double) * ac.size()) != hipSuccess ||
+                    hipMemcpy(c->cmp_ac[d][w], ac.data(), sizeof(double) * ac.size(),
+                              hipMemcpyHostToDevice) != hipSuccess)
+                    return cleanup(fail(ASTR_E_HIP, "compact flux table upload failed"));
+            }
     FiltCoef fc = build_coef(p->alfa_filter, p->beta_halo, p->beta_bound);
     if (hipMalloc(&c->fc, sizeof(FiltCoef)) != hipSuccess ||
         hipMemcpy(c->fc, &fc, sizeof fc, hipMemcpyHostToDevice) != hipSuccess)
@@ -1676,6 +1972,8 @@ int astr_destroy(astr_ctx* c)
     (void)hipFree(c->xi);
     (void)hipFree(c->part);
     (void)hipFree(c->lshock);
+    for (int d = 0; d < 3; ++d)
+        for (int w = 0; w < 2; ++w) (void)hipFree(c->cmp_ac[d][w]);
     for (int e = 0; e < 16; ++e)
         if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1804,9 +2102,9 @@ int astr_qswap(astr_ctx* c)
 int astr_rhscal(astr_ctx* c)
 {
     // the upwind right-hand side takes any closure type with halos as given
-    if (c && c->p.convection == 1) TRY(ready_state(c));
+    if (c && c->p.convection != 0) TRY(ready_state(c));
     else TRY(ready(c));
-    return do_rhscal(c);
+    return do_rhscal(c, c->timing);
 }
 
 int astr_download_shock(astr_ctx* c, uint8_t* lshock)
@@ -1878,10 +2176,19 @@ int astr_bench_copy(astr_ctx* c, int32_t reps, double* ms_per_copy, int64_t* byt
     return ASTR_OK;
 }
 
+int astr_synchronize(astr_ctx* c)
+{
+    if (!c) return fail(ASTR_E_ARG, "null context");
+    TRY(set_dev(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ASTR_OK;
+}
+
 int astr_set_timing(astr_ctx* c, int32_t on)
 {
     if (!c) return fail(ASTR_E_ARG, "null context");
     c->timing = on != 0;
+    for (int f = 0; f < 9; ++f) c->tms[f] = 0.0;
     return ASTR_OK;
 }
 

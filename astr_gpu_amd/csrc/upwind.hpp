@@ -78,6 +78,18 @@ __device__ double mp5(const double* u)
     return ul;
 }
 
+// MP5 with a supplied linear value and shock switch, and mplimiter,
+// src/flux.F90:363-379,434-496
+__device__ double mplimiter(const double* u, double ul, bool lskt, int inode, int dim, int ntype)
+{
+    if ((ntype == 1 && (inode == 0 || inode == 1)) ||
+        (ntype == 2 && (inode == dim - 1 || inode == dim - 2)))
+        return ul;
+    const double uMP = u[2] + minmod2(u[3] - u[2], 4.0 * (u[2] - u[1]));
+    if (lskt && (ul - u[2]) * (ul - uMP) >= 1.e-10) return mp_limit(u, ul);
+    return ul;
+}
+
 // MP7, src/flux.F90:498-551
 __device__ double mp7(const double* u)
 {

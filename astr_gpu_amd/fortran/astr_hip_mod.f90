@@ -144,6 +144,11 @@ module astr_hip
       real(c_double), intent(out) :: ff(*)
     end function astr_filter_field
 
+    integer(c_int) function astr_synchronize(ctx) bind(c, name='astr_synchronize')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function astr_synchronize
+
     integer(c_int) function astr_set_timing(ctx, on) bind(c, name='astr_set_timing')
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx

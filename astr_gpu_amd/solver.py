@@ -134,11 +134,14 @@ class AstrSolver:
               "astr_rk3_steps")
         return out
 
+    def synchronize(self) -> None:
+        check(self.lib.astr_synchronize(self.ctx), "astr_synchronize")
+
     def set_timing(self, on: bool) -> None:
         check(self.lib.astr_set_timing(self.ctx, 1 if on else 0), "astr_set_timing")
 
     def timing(self) -> np.ndarray:
-        """ms per family: filter, qswap, grad, sigma, stats, flux, prim, total, 0."""
+        """ms per family: filter, qswap, grad, sigma, stats, flux, update, total, sensor."""
         out = np.zeros(9)
         check(self.lib.astr_get_timing(self.ctx, dptr(out)), "astr_get_timing")
         return out
@@ -221,6 +224,51 @@ def tgv_state(p: AstrParams, lengths=(2 * math.pi,) * 3):
     out["q"][I + (3,)] = rho * w
     out["q"][I + (4,)] = rho * (tmp * const1 + 0.5 * (u * u + v * v + w * w))
     return out
+
+
+def riemann2d_state(p: AstrParams, lengths=(1.0, 1.0), periodic_halos=False):
+    """Four-quadrant 2-D Riemann initial field of riem2dini
+    (src/initialisation.F90:870-921) on the unit square, split at 0.8:
+    (rho, u, v, p) NE (1.5, 0, 0, 1.5), NW (0.5323, 1.206, 0, 0.30),
+    SW (0.138, 1.206, 1.206, 0.029), SE (0.5323, 0, 1.206, 0.3).  The NW
+    pressure literal is single precision in the reference (0.30, not
+    0.30d0) and is kept as such.  T from thermal (p/rho*const2), q from
+    fvar2q's temperature form (src/fludyna.F90:495-505).  With
+    periodic_halos the halos (all axes) are periodic copies of nodes 0..n-1,
+    otherwise the halos repeat the nearest boundary node."""
+    gamma, mach = p.gamma, p.mach
+    const1 = 1.0 / (gamma * (gamma - 1.0) * (mach * mach))
+    const2 = gamma * (mach * mach)
+    s = haloed_shape(p)
+    n = (p.im, p.jm, p.km)
+    idx = []
+    for d in range(3):
+        t = np.arange(-HM, n[d] + HM + 1)
+        if n[d] == 0:
+            t = np.zeros_like(t)
+        elif periodic_halos:
+            t = np.mod(t, n[d])
+        else:
+            t = np.clip(t, 0, n[d])
+        idx.append(t)
+    x = lengths[0] / p.im * idx[0]
+    y = lengths[1] / p.jm * idx[1]
+    X, Y, _ = np.meshgrid(x, y, idx[2], indexing="ij")
+    ne = (X >= 0.8) & (Y >= 0.8)
+    nw = (X < 0.8) & (Y >= 0.8)
+    sw = (X < 0.8) & (Y < 0.8)
+    se = (X >= 0.8) & (Y < 0.8)
+    rho = np.select([ne, nw, sw, se], [1.5, 0.5323, 0.138, 0.5323])
+    u = np.select([ne, nw, sw, se], [0.0, 1.206, 1.206, 0.0])
+    v = np.select([ne, nw, sw, se], [0.0, 0.0, 1.206, 1.206])
+    prs = np.select([ne, nw, sw, se], [1.5, float(np.float32(0.30)), 0.029, 0.3])
+    w = np.zeros_like(rho)
+    tmp = prs / rho * const2
+    q = np.stack([rho, rho * u, rho * v, rho * w,
+                  rho * (tmp * const1 + 0.5 * (u ** 2 + v ** 2 + w ** 2))], axis=-1)
+    f = np.asfortranarray
+    return {"q": f(q), "rho": f(rho), "vel": f(np.stack([u, v, w], axis=-1)), "prs": f(prs),
+            "tmp": f(tmp)}
 
 
 def interior(a, p: AstrParams):

@@ -54,6 +54,16 @@ ALGO_BYTES_PER_PT_STEP = 8.0 * (3.0 * sum(FAMILY_DOUBLES[f] * FAMILY_LAUNCHES[f]
                                 + STATS_DOUBLES)
 
 
+def measured_traffic(family: str, n: int):
+    """HBM bytes per launch of a kernel family from the committed rocprof
+    PMC summary (profiles/tgv256_traffic.json), for the 256^3 workload."""
+    path = os.path.join(ROOT, "profiles", "tgv256_traffic.json")
+    if n != 256 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["bytes_per_launch"].get(family)
+
+
 def cpu_baseline(n: int, dt: float):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -71,6 +81,97 @@ def cpu_baseline(n: int, dt: float):
                       f"(OpenMP, {thr} threads), {el:.1f} s"}
 
 
+FP64_PEAK_TFLOPS = 78.6
+
+
+def measured_flops(n: int):
+    """FP64 flops per upwind interface-flux launch from the committed rocprof
+    PMC summary (profiles/riemann2d_upwind.json), for the 2048^2 workload."""
+    path = os.path.join(ROOT, "profiles", "riemann2d_upwind.json")
+    if n != 2048 or not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("fp64_flops_per_launch"), d.get("traffic_bytes_per_launch")
+
+
+def cpu_baseline_upwind(n_small: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    import astr_gpu_amd as ag
+
+    p = ag.default_params((n_small, n_small, 0), convection=1, deriv_explicit=1, recon_schem=3,
+                          lchardecomp=1, lfilter=0, diffterm=0, mach=1.0)
+    st = ag.riemann2d_state(p, periodic_halos=True)
+    jac, dxi = ag.box_metrics(p, lengths=(1.0, 1.0, 1.0))
+    box = pyoracle.UwdBox((n_small, n_small, 0), recon_schem=3, lchardecomp=True, mach=1.0)
+    t0 = time.perf_counter()
+    box.rhscal(st, dxi.reshape(dxi.shape[:3] + (9,), order="F"), jac)
+    el = time.perf_counter() - t0
+    return {"value": n_small * n_small / el, "unit": "grid-point-RHS/s", "cores": 1,
+            "kind": "port",
+            "sample": f"one upwind right-hand side of the {n_small}^2 four-quadrant box by the "
+                      f"C restatement (one thread), {el:.1f} s"}
+
+
+def bench_riemann(args):
+    """Explicit upwind right-hand side (MP7, characteristic projection, Ducros
+    sensor, explicit 6th-order gradient) of the 2-D four-quadrant Riemann
+    state on an n x n box: grid points per second of astr_rhscal."""
+    import astr_gpu_amd as ag
+
+    n = args.n
+    p = ag.default_params((n, n, 0), convection=1, deriv_explicit=1, recon_schem=3,
+                          lchardecomp=1, lfilter=0, diffterm=0, mach=1.0)
+    s = ag.AstrSolver(p, device=0)
+    jac, dxi = ag.box_metrics(p, lengths=(1.0, 1.0, 1.0))
+    s.upload_geom(jac, dxi)
+    st = ag.riemann2d_state(p, periodic_halos=True)
+    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+    del st
+    for _ in range(args.warmup):
+        s.rhscal()
+    s.synchronize()
+    s.set_timing(True)
+    for _ in range(args.steps):
+        s.rhscal()
+    s.synchronize()
+    fam = s.timing()
+    s.set_timing(False)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.rhscal()
+    s.synchronize()
+    el = time.perf_counter() - t0
+    s.close()
+    pts = float(n) * n
+    value = pts * args.steps / el
+    avg_ms = fam[5] / (2 * args.steps)  # two interface-flux sweeps per call
+    flops, traffic = measured_flops(n)
+    out = {
+        "metric": "FP64 grid-point-RHS/s, explicit upwind 2-D Riemann",
+        "value": value, "unit": "grid-point-RHS/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (four-quadrant Riemann initial field)",
+        "config": {"workload": f"2-D four-quadrant Riemann {n}^2, recon MP7, characteristic "
+                               f"projection, Ducros sensor, explicit 643e gradient", "n": n,
+                   "parallelism": "single GPU"},
+        "family_ms_per_call": {"grad": fam[2] / args.steps, "sensor": fam[8] / args.steps,
+                               "interface_flux": fam[5] / args.steps},
+        "roofline": None,
+    }
+    if flops:
+        achieved = flops / (avg_ms * 1e-3) / 1e12
+        out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                           "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                           "traffic": traffic, "kernel": "k_uwd (interface fluxes), 2 launches/call",
+                           "flops_per_launch": flops, "avg_launch_ms": avg_ms}
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline_upwind(1024)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,13 +179,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["tgv", "riemann2d"], default="tgv",
+                    help="tgv: the headline RK3 step; riemann2d: the upwind right-hand side")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the solver keeps the whole box on one device "
               "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
+    if args.workload == "riemann2d":
+        if args.n == 256:
+            args.n = 2048
+        bench_riemann(args)
+        return
 
     import astr_gpu_amd as ag
 
@@ -142,7 +249,7 @@ def main():
                                f"filter alpha 0.49, RK3 dt 0.004", "n": n,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, n),
                      "kernel": f"{FAMILY_KERNEL[dom]}, {FAMILY_LAUNCHES[dom]} launches/stage",
                      "algo_bytes_per_launch": algo, "avg_launch_ms": avg_s * 1e3},
         "step_hbm_frac": value * ALGO_BYTES_PER_PT_STEP / (HBM_PEAK_GBS * 1e9),

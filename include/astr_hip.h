@@ -152,14 +152,22 @@ int astr_deriv_field(astr_ctx *ctx, int32_t dir, const double *f, double *df);
  * f(-hm:...) haloed in, ff(0:im,0:jm,0:km) out.  Halos used as given. */
 int astr_filter_field(astr_ctx *ctx, int32_t dir, const double *f, double *ff);
 
-/* Kernel time of the astr_rk3_step(s) calls since timing was enabled with
- * astr_set_timing(ctx, 1), in milliseconds, per kernel family (HIP events
- * on the solver's stream): [0] filter sweeps, [1] qswap, [2] gradient
- * sweeps, [3] sigma/qflux, [4] statistics, [5] flux sweeps (inviscid and
- * viscous flux derivatives; the z sweep applies the RK3 combine),
- * [6] primitive update, [7] total of 0-6, [8] zero. */
+/* Kernel time since timing was (re)enabled with astr_set_timing(ctx, 1),
+ * in milliseconds, per kernel family (HIP events on the solver's stream):
+ * [0] filter sweeps, [1] qswap, [2] gradient, [3] sigma/qflux,
+ * [4] statistics, [5] flux sweeps (central: inviscid and viscous flux
+ * derivatives, the z sweep applying the RK3 combine; upwind: interface
+ * fluxes), [6] update (central: primitives; upwind: RK3 combine and
+ * primitives), [7] total, [8] shock sensor.  astr_rk3_step(s) fill 0-7;
+ * astr_rhscal of an upwind context fills 2, 5, 7 and 8.  astr_rk3_steps
+ * restarts the count at each call. */
 int astr_set_timing(astr_ctx *ctx, int32_t on);
 int astr_get_timing(astr_ctx *ctx, double *ms9);
+
+/* Wait for the work queued on the context's stream (entry points that
+ * only queue kernels, such as astr_rhscal, return before it ends).  No
+ * counterpart in the reference, whose calls are synchronous. */
+int astr_synchronize(astr_ctx *ctx);
 
 /* Measurement utility with no counterpart in the reference: copies one
  * field-sized device buffer (16-byte accesses) `reps` times and reports the

@@ -144,6 +144,17 @@ void orc_ducros(const orc_uwd *u, const double *dvel, const double *prs, double 
 int orc_convrsduwd(const orc_uwd *u, const double *q, const double *rho, const double *vel,
                    const double *prs, const double *tmp, const double *dxi, const double *jacob,
                    const unsigned char *lshock, double *qrhs);
+int orc_flux_compact_init(int ntype, int dim, int plus, double bfac, int *first, int *last,
+                          double *ac);
+void orc_flux_compact(int ntype, int dim, int plus, double bfac, const double *f, double *fh);
+double orc_mplimiter(const double *f, double fl, int shock, int inode, int dim, int ntype);
+int orc_convrsdcmp(const orc_uwd *u, const double *q, const double *rho, const double *vel,
+                   const double *prs, const double *tmp, const double *dxi, const double *jacob,
+                   const unsigned char *lshock, double *qrhs);
+int orc_cmp_rhscal(const orc_uwd *u, int explicit_fd, const double *q, const double *rho,
+                   const double *vel, const double *prs, const double *tmp, const double *dxi,
+                   const double *jacob, double *qrhs, double *dvel, double *dtmp, double *ssf,
+                   unsigned char *lshock);
 int orc_uwd_rhscal(const orc_uwd *u, int explicit_fd, const double *q, const double *rho,
                    const double *vel, const double *prs, const double *tmp, const double *dxi,
                    const double *jacob, double *qrhs, double *dvel, double *dtmp, double *ssf,

@@ -297,6 +297,14 @@ def uwd_lib():
         lib.orc_convrsduwd.argtypes = [P, _D, _D, _D, _D, _D, _D, _D, U8, _D]
         lib.orc_uwd_rhscal.restype = _I
         lib.orc_uwd_rhscal.argtypes = [P, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, U8]
+        lib.orc_cmp_rhscal.restype = _I
+        lib.orc_cmp_rhscal.argtypes = [P, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, U8]
+        lib.orc_flux_compact.argtypes = [_I, _I, _I, _F, _D, _D]
+        lib.orc_flux_compact_init.restype = _I
+        lib.orc_flux_compact_init.argtypes = [_I, _I, _I, _F, ctypes.POINTER(_I),
+                                              ctypes.POINTER(_I), _D]
+        lib.orc_mplimiter.restype = _F
+        lib.orc_mplimiter.argtypes = [_D, _F, _I, _I, _I, _I]
         _uwd_declared = True
     return lib
 
@@ -319,6 +327,19 @@ def recon(name: str, u, *extra) -> float:
 def recons_exp(f, inode, dim, ntype, reschem, shock=True, bfacmpld=0.7) -> float:
     f = _c(f)
     return uwd_lib().orc_recons_exp(_p(f), inode, dim, ntype, reschem, int(shock), bfacmpld)
+
+
+def flux_compact(ntype, f, plus=True, bfac=0.7) -> np.ndarray:
+    """fh(-1:dim) of flux_compact for a line f(-hm:dim+hm)."""
+    f = _c(f)
+    dim = len(f) - 1 - 2 * HM
+    fh = np.zeros(dim + 2)
+    uwd_lib().orc_flux_compact(ntype, dim, int(plus), bfac, _p(f), _p(fh))
+    return fh
+
+
+def mplimiter(f5, fl, shock, inode, dim, ntype) -> float:
+    return uwd_lib().orc_mplimiter(_p(_c(f5)), fl, int(shock), inode, dim, ntype)
 
 
 def sw_split(rho, vel, prs, tmp, q, dxi, jacob, gamma=1.4, mach=0.1):
@@ -363,9 +384,10 @@ class UwdBox:
         assert a.shape == shape, (a.shape, shape)
         return np.asfortranarray(a)
 
-    def rhscal(self, st, dxi, jacob, explicit_fd=True):
+    def rhscal(self, st, dxi, jacob, explicit_fd=True, compact=False):
         """qrhs, dvel, dtmp, ssf, lshock of the upwind rhscal for state st
-        (dict q, rho, vel, prs, tmp; haloed)."""
+        (dict q, rho, vel, prs, tmp; haloed): convrsduwd, or convrsdcmp
+        with compact=True."""
         lib = uwd_lib()
         q = self._h(st["q"], 5)
         vel = self._h(st["vel"], 3)
@@ -378,7 +400,8 @@ class UwdBox:
         dtmp = np.zeros(self.ishape + (3,), order="F")
         ssf = np.zeros(self.hshape, order="F")
         lsh = np.zeros(self.ishape, dtype=np.uint8, order="F")
-        rc = lib.orc_uwd_rhscal(ctypes.byref(self.cfg), int(explicit_fd), _p(q), _p(rho),
+        fn = lib.orc_cmp_rhscal if compact else lib.orc_uwd_rhscal
+        rc = fn(ctypes.byref(self.cfg), int(explicit_fd), _p(q), _p(rho),
                                 _p(vel), _p(prs), _p(tmp), _p(dx), _p(jac), _p(qrhs),
                                 _p(dvel), _p(dtmp), _p(ssf),
                                 lsh.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))

@@ -819,3 +819,278 @@ int orc_uwd_rhscal(const orc_uwd *u, int explicit_fd, const double *q, const dou
     for (long e = 0; e < 5 * s.nint; ++e) qrhs[e] = -qrhs[e];
     return 0;
 }
+
+/* ------------------------------------------------ compact upwind ('543c') */
+
+static const double n1d6 = 1.0 / 6.0, n37d60 = 37.0 / 60.0, n2d15 = 2.0 / 15.0,
+                    n1d60 = 1.0 / 60.0, n1d18 = 1.0 / 18.0, n1d36 = 1.0 / 36.0,
+                    n19d18 = 19.0 / 18.0, n9d36 = 9.0 / 36.0, n5d9 = 5.0 / 9.0;
+
+/* compact_flux_initiate, src/flux.F90:32-108 (scheme 543): rows first..last
+ * (ntype 1: -1..dim+1, 2: -2..dim, 3: -2..dim+1, 4: -1..dim), the LHS of
+ * the 5th-order compact upwind interface flux for the + or - wind, and its
+ * Thomas factors ac[3*row].  Returns the row count. */
+int orc_flux_compact_init(int ntype, int dim, int plus, double bfac, int *first, int *last,
+                          double *ac)
+{
+    const int i0 = (ntype == 1 || ntype == 4) ? -1 : -2;
+    const int im_ = (ntype == 2 || ntype == 4) ? dim : dim + 1;
+    const int n = im_ - i0 + 1;
+    double *a = malloc(sizeof(double) * n), *c = malloc(sizeof(double) * n);
+    for (int r = 0; r < n; ++r) {
+        a[r] = plus ? 0.5 - n1d6 * bfac : n1d6 + n1d6 * bfac;
+        c[r] = plus ? n1d6 + n1d6 * bfac : 0.5 - n1d6 * bfac;
+    }
+    a[0] = c[0] = 0.0;
+    a[n - 1] = c[n - 1] = 0.0;
+    if (ntype == 1 || ntype == 4) {
+        a[0] = c[0] = 2.0;
+        a[1] = c[1] = 0.25;
+    }
+    if (ntype == 2 || ntype == 4) {
+        a[n - 1] = c[n - 1] = 2.0;
+        a[n - 2] = c[n - 2] = 0.25;
+    }
+    if (ac) orc_thomas_prep(a, c, n, ac);
+    free(a);
+    free(c);
+    *first = i0;
+    *last = im_;
+    return n;
+}
+
+/* compact_flux_rhs, src/flux.F90:158-256, rows first..last; f(node) =
+ * f[node + HMU], d indexed by row - first */
+static void flux_rhs(int ntype, int i0, int im_, int plus, double bfac, const double *f,
+                     double *d)
+{
+#define F(nd) f[(nd) + HMU]
+#define D(nd) d[(nd) - i0]
+    int is = i0 + 1, ie = im_ - 1, j;
+    if (ntype == 1 || ntype == 4) {
+        j = i0;
+        D(j) = 2.5 * F(j + 1) + 0.5 * F(j + 2);
+        j = i0 + 1;
+        D(j) = 0.75 * F(j) + 0.75 * F(j + 1);
+        is = i0 + 2;
+    } else {
+        j = i0;
+        const double v1 = F(j) + F(j + 1), v2 = F(j - 1) + F(j + 2), v3 = F(j - 2) + F(j + 3);
+        D(j) = n37d60 * v1 - n2d15 * v2 + n1d60 * v3;
+    }
+    if (ntype == 2 || ntype == 4) {
+        j = im_ - 1;
+        D(j) = 0.75 * F(j + 1) + 0.75 * F(j);
+        j = im_;
+        D(j) = 2.5 * F(j) + 0.5 * F(j - 1);
+        ie = im_ - 2;
+    } else {
+        j = im_;
+        const double v1 = F(j) + F(j + 1), v2 = F(j - 1) + F(j + 2), v3 = F(j - 2) + F(j + 3);
+        D(j) = n37d60 * v1 - n2d15 * v2 + n1d60 * v3;
+    }
+    for (j = is; j <= ie; ++j) {
+        if (plus)
+            D(j) = (n1d18 - n1d36 * bfac) * F(j - 1) + (n19d18 - n9d36 * bfac) * F(j) +
+                   (n5d9 + n9d36 * bfac) * F(j + 1) + n1d36 * bfac * F(j + 2);
+        else
+            D(j) = (n1d18 - n1d36 * bfac) * F(j + 2) + (n19d18 - n9d36 * bfac) * F(j + 1) +
+                   (n5d9 + n9d36 * bfac) * F(j) + n1d36 * bfac * F(j - 1);
+    }
+#undef F
+#undef D
+}
+
+/* flux_compact, src/flux.F90:125-147: fh(-1:dim) of f(-hm:dim+hm) */
+void orc_flux_compact(int ntype, int dim, int plus, double bfac, const double *f, double *fh)
+{
+    int i0, im_;
+    const int n = orc_flux_compact_init(ntype, dim, plus, bfac, &i0, &im_, NULL);
+    double *ac = malloc(sizeof(double) * 3 * n), *d = malloc(sizeof(double) * n),
+           *x = malloc(sizeof(double) * n);
+    orc_flux_compact_init(ntype, dim, plus, bfac, &i0, &im_, ac);
+    flux_rhs(ntype, i0, im_, plus, bfac, f, d);
+    orc_thomas_solve(ac, d, n, x);
+    for (int i = -1; i <= dim; ++i) fh[i + 1] = x[i - i0];
+    free(ac);
+    free(d);
+    free(x);
+}
+
+/* mplimiter, src/flux.F90:363-379 */
+double orc_mplimiter(const double *f, double fl, int shock, int inode, int dim, int ntype)
+{
+    if ((ntype == 1 && inode == 0) || (ntype == 2 && inode == dim - 1) ||
+        (ntype == 1 && inode == 1) || (ntype == 2 && inode == dim - 2))
+        return fl;
+    return orc_mp5(f, &fl, shock);
+}
+
+/* iwind6, src/solver.F90:1237-1258 */
+static int iwind6(int i, int n, int imin, int imax, int plus)
+{
+    int w = plus ? i + n - 3 : i + 4 - n;
+    if (w < imin) w = imin;
+    if (w > imax) w = imax;
+    return w;
+}
+
+/* convrsdcmp, src/solver.F90:1272-1938 (no species, crinod false): per
+ * axis, Steger-Warming split, the compact interface fluxes of F+ and F-,
+ * the MP limiter about them on 5-point stencils (characteristic variables
+ * when lchardecomp, for every interface), qrhs += Fh(i) - Fh(i-1). */
+int orc_convrsdcmp(const orc_uwd *u, const double *q, const double *rho, const double *vel,
+                   const double *prs, const double *tmp, const double *dxi, const double *jacob,
+                   const unsigned char *lshock, double *qrhs)
+{
+    const shape_t s = shape_of(u);
+    const int nd = orc_ndims(u);
+    const int n[3] = {u->im, u->jm, u->km};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) orc_uwd_range(u->npdc[d], n[d], &lo[d], &hi[d]);
+    if (nd == 2) lo[2] = hi[2] = 0;
+    for (int d = 0; d < nd; ++d) {
+        const int nt = u->npdc[d];
+        if (nt < 1 || nt > 4) return 1;
+        const int iss = (nt == 1 || nt == 4) ? 0 : -HMU;
+        const int iee = (nt == 2 || nt == 4) ? n[d] : n[d] + HMU;
+        const int len = n[d] + 1 + 2 * HMU;
+        double *fp = calloc(5 * len, sizeof(double)), *fm = calloc(5 * len, sizeof(double));
+        double *line = malloc(sizeof(double) * len);
+        double *hp = malloc(sizeof(double) * 5 * (n[d] + 2)), *hmn = malloc(sizeof(double) * 5 * (n[d] + 2));
+        double *fh = malloc(sizeof(double) * 5 * (n[d] + 2));
+        const int a0 = d == 0 ? lo[1] : lo[0], a1 = d == 0 ? hi[1] : hi[0];
+        const int b0 = d == 2 ? lo[1] : lo[2], b1 = d == 2 ? hi[1] : hi[2];
+        for (int b = b0; b <= b1; ++b)
+            for (int a = a0; a <= a1; ++a) {
+#define NODE(t, I, J, K)                                   \
+    const int I = d == 0 ? (t) : a;                        \
+    const int J = d == 1 ? (t) : (d == 0 ? a : b);         \
+    const int K = d == 2 ? (t) : b;
+                for (int t = iss; t <= iee; ++t) {
+                    NODE(t, i, j, k)
+                    const long h = hidx(&s, i, j, k);
+                    double v[3], qq[5], dd[3];
+                    for (int c = 0; c < 3; ++c) {
+                        v[c] = vel[c * s.nf + h];
+                        dd[c] = dxi[(d + 3 * c) * s.nf + h];
+                    }
+                    for (int m = 0; m < 5; ++m) qq[m] = q[m * s.nf + h];
+                    double P[5], M[5];
+                    orc_sw_split(u->gamma, u->mach, rho[h], v, prs[h], tmp[h], qq, dd, jacob[h], P,
+                                 M);
+                    for (int m = 0; m < 5; ++m) {
+                        fp[m * len + t + HMU] = P[m];
+                        fm[m * len + t + HMU] = M[m];
+                    }
+                }
+                for (int m = 0; m < 5; ++m) {
+                    orc_flux_compact(nt, n[d], 1, u->bfacmpld, fp + m * len, hp + m * (n[d] + 2));
+                    orc_flux_compact(nt, n[d], 0, u->bfacmpld, fm + m * len, hmn + m * (n[d] + 2));
+                }
+                (void)line;
+                for (int i = lo[d] - 1; i <= hi[d]; ++i) {
+                    double Lv[25], Rv[25], fcp[5][5], fcm[5][5], cp[5], cm[5], fhc[5];
+                    if (u->lchardecomp) {
+                        NODE(i, il, jl, kl)
+                        NODE(i + 1, ir, jr, kr)
+                        const long hl = hidx(&s, il, jl, kl), hr = hidx(&s, ir, jr, kr);
+                        double vl[3], vr[3], dl[3], dr[3];
+                        for (int c = 0; c < 3; ++c) {
+                            vl[c] = vel[c * s.nf + hl];
+                            vr[c] = vel[c * s.nf + hr];
+                            dl[c] = dxi[(d + 3 * c) * s.nf + hl];
+                            dr[c] = dxi[(d + 3 * c) * s.nf + hr];
+                        }
+                        if (orc_chardecomp(u->gamma, tmp[hl], rho[hl], prs[hl], q[4 * s.nf + hl],
+                                           vl, dl, tmp[hr], rho[hr], prs[hr], q[4 * s.nf + hr],
+                                           vr, dr, Rv, Lv))
+                            return 2;
+                        for (int m = 0; m < 5; ++m) {
+                            const double *l = Lv + 5 * m;
+                            for (int w = 1; w <= 5; ++w) {
+                                const int tp = iwind6(i, w, iss, iee, 1) + HMU;
+                                const int tm = iwind6(i, w, iss, iee, 0) + HMU;
+                                fcp[m][w - 1] = l[0] * fp[tp] + l[1] * fp[len + tp] +
+                                                l[2] * fp[2 * len + tp] + l[3] * fp[3 * len + tp] +
+                                                l[4] * fp[4 * len + tp];
+                                fcm[m][w - 1] = l[0] * fm[tm] + l[1] * fm[len + tm] +
+                                                l[2] * fm[2 * len + tm] + l[3] * fm[3 * len + tm] +
+                                                l[4] * fm[4 * len + tm];
+                            }
+                            const int ix = i + 1;
+                            const int st = n[d] + 2;
+                            cp[m] = l[0] * hp[ix] + l[1] * hp[st + ix] + l[2] * hp[2 * st + ix] +
+                                    l[3] * hp[3 * st + ix] + l[4] * hp[4 * st + ix];
+                            cm[m] = l[0] * hmn[ix] + l[1] * hmn[st + ix] + l[2] * hmn[2 * st + ix] +
+                                    l[3] * hmn[3 * st + ix] + l[4] * hmn[4 * st + ix];
+                        }
+                    } else {
+                        for (int m = 0; m < 5; ++m) {
+                            for (int w = 1; w <= 5; ++w) {
+                                fcp[m][w - 1] = fp[m * len + iwind6(i, w, iss, iee, 1) + HMU];
+                                fcm[m][w - 1] = fm[m * len + iwind6(i, w, iss, iee, 0) + HMU];
+                            }
+                            cp[m] = hp[m * (n[d] + 2) + i + 1];
+                            cm[m] = hmn[m * (n[d] + 2) + i + 1];
+                        }
+                    }
+                    int lsh = 1;
+                    if (lshock) {
+                        NODE(i < 0 ? i + 1 : i, i1, j1, k1)
+                        NODE(i + 1 > n[d] ? i : i + 1, i2, j2, k2)
+                        lsh = lshock[iidx(&s, i1, j1, k1)] || lshock[iidx(&s, i2, j2, k2)];
+                    }
+                    for (int m = 0; m < 5; ++m) {
+                        const double v1 = orc_mplimiter(fcp[m], cp[m], lsh, i, n[d], nt);
+                        const double v2 = orc_mplimiter(fcm[m], cm[m], lsh, i, n[d], nt);
+                        fhc[m] = v1 + v2;
+                    }
+                    double *F = fh + 5 * (i + 1);
+                    if (u->lchardecomp) {
+                        for (int m = 0; m < 5; ++m) {
+                            const double *r = Rv + 5 * m;
+                            F[m] = r[0] * fhc[0] + r[1] * fhc[1] + r[2] * fhc[2] + r[3] * fhc[3] +
+                                   r[4] * fhc[4];
+                        }
+                    } else {
+                        for (int m = 0; m < 5; ++m) F[m] = fhc[m];
+                    }
+                }
+                for (int i = lo[d]; i <= hi[d]; ++i) {
+                    NODE(i, ii, jj, kk)
+                    const long p = iidx(&s, ii, jj, kk);
+                    for (int m = 0; m < 5; ++m)
+                        qrhs[m * s.nint + p] = qrhs[m * s.nint + p] + fh[5 * (i + 1) + m] -
+                                               fh[5 * i + m];
+                }
+#undef NODE
+            }
+        free(fp);
+        free(fm);
+        free(line);
+        free(hp);
+        free(hmn);
+        free(fh);
+    }
+    return 0;
+}
+
+/* rhscal for the compact upwind scheme (conschm(4:4)='c', odd order),
+ * src/solver.F90:226-236: qrhs = 0, gradcal, ducrossensor when
+ * lchardecomp, convrsdcmp, qrhs = -qrhs. */
+int orc_cmp_rhscal(const orc_uwd *u, int explicit_fd, const double *q, const double *rho,
+                   const double *vel, const double *prs, const double *tmp, const double *dxi,
+                   const double *jacob, double *qrhs, double *dvel, double *dtmp, double *ssf,
+                   unsigned char *lshock)
+{
+    const shape_t s = shape_of(u);
+    memset(qrhs, 0, sizeof(double) * 5 * s.nint);
+    orc_gradcal(u, explicit_fd, vel, tmp, dxi, dvel, dtmp);
+    if (u->lchardecomp) orc_ducros(u, dvel, prs, ssf, lshock);
+    const int rc = orc_convrsdcmp(u, q, rho, vel, prs, tmp, dxi, jacob,
+                                  u->lchardecomp ? lshock : NULL, qrhs);
+    if (rc) return rc;
+    for (long e = 0; e < 5 * s.nint; ++e) qrhs[e] = -qrhs[e];
+    return 0;
+}

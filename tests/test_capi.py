@@ -71,7 +71,7 @@ def test_create_rejects_unsupported_without_touching_the_gpu():
         p = ag.default_params(32, **over)
         with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
             ag.AstrSolver(p)
-    for over in [dict(convection=2), dict(convection=1, diffterm=0, recon_schem=4)]:
+    for over in [dict(convection=3), dict(convection=1, diffterm=0, recon_schem=4)]:
         p = ag.default_params(32, **over)
         with pytest.raises(ag.AstrError, match="ASTR_E_ARG"):
             ag.AstrSolver(p)

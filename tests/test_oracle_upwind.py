@@ -379,3 +379,102 @@ def test_two_dimensional_box():
     assert np.all(q[..., 3] == 0.0)  # no z momentum flux in 2-D
     tot = q[:-1, :-1, 0].sum(axis=(0, 1))
     assert np.all(np.abs(tot) < 1e-12 * np.abs(q).max() * 500)
+
+
+# ------------------------------------------------ compact upwind ('543c')
+
+def line_of_cell_avgs(H, dim, lo=-HM):
+    """f(-hm:dim+hm) = unit-cell averages of H' (H an antiderivative)."""
+    t = np.arange(lo, dim + HM + 1)
+    return H(t + 0.5) - H(t - 0.5)
+
+
+@pytest.mark.parametrize("ntype", [1, 2, 3, 4])
+@pytest.mark.parametrize("plus", [True, False])
+def test_flux_compact_preserves_constants(ntype, plus):
+    dim = 40
+    fh = po.flux_compact(ntype, np.full(dim + 1 + 2 * HM, 1.7), plus)
+    assert np.allclose(fh, 1.7, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("plus", [True, False])
+def test_flux_compact_exact_on_polynomials(plus):
+    """Every row of the inner (ntype 3) system is exact for polynomials of
+    degree <= 4 (5th-order compact upwind interior, 6th-order explicit
+    interface rows), so the solve returns the interface values h(j+1/2)
+    exactly; the 3rd-order boundary rows (ntype 1, 4) keep degree <= 2.
+    One degree higher is not reproduced."""
+    dim = 30
+    rng = np.random.default_rng(9)
+    for ntype, deg in ((3, 4), (1, 2), (4, 2), (2, 2)):
+        for d in range(deg + 2):
+            c = np.zeros(d + 1)
+            c[d] = 1.0
+            c[:d] = rng.standard_normal(d)
+            h = np.polynomial.Polynomial(c) * (1.0 / dim ** d)  # keep values O(1)
+            H = h.integ()
+            fh = po.flux_compact(ntype, line_of_cell_avgs(H, dim), plus, bfac=0.7)
+            exact = h(np.arange(-1, dim + 1) + 0.5)
+            err = np.abs(fh - exact).max()
+            if d <= deg:
+                assert err < 1e-11, (ntype, d, err)
+            elif ntype == 3:
+                assert err > 1e-9, (ntype, d, err)
+
+
+def test_flux_compact_converges_at_fifth_order():
+    errs = []
+    for dim in (32, 64):
+        h = 1.0 / dim
+        t = np.arange(-HM, dim + HM + 1)
+        k = 2 * np.pi
+        f = (np.cos(k * (t - 0.5) * h) - np.cos(k * (t + 0.5) * h)) / (k * h)
+        fh = po.flux_compact(3, f, True, bfac=0.7)
+        exact = np.sin(k * (np.arange(-1, dim + 1) + 0.5) * h)
+        errs.append(np.abs(fh - exact)[5:-5].max())
+    assert math.log2(errs[0] / errs[1]) > 4.5, errs
+
+
+def test_mplimiter_boundary_and_interior():
+    f = np.array([1.0, 1.0, 1.0, 0.0, 0.0])
+    fl = 1.3  # an overshooting linear value
+    assert po.mplimiter(f, fl, True, 0, 40, 1) == fl
+    assert po.mplimiter(f, fl, True, 38, 40, 2) == fl
+    v = po.mplimiter(f, fl, True, 10, 40, 3)
+    assert 0.0 <= v <= 1.0
+    assert po.mplimiter(f, fl, False, 10, 40, 3) == fl  # shock flag off
+
+
+@pytest.mark.parametrize("lchar", [True, False])
+def test_compact_upwind_free_stream(lchar):
+    # (no telescoping identity here: the non-cyclic ntype-3 compact systems
+    # of periodic lines make Fh(-1) and Fh(n-1) differ at truncation level)
+    dims = (24, 20, 16)
+    box = po.UwdBox(dims, recon_schem=3, lchardecomp=lchar, shkcrt=-1.0)
+    X = coords(dims, (1, 1, 1))
+    one = np.ones_like(X[0])
+    st = po.prim_to_state(1.3 * one, 0.4 * one, -0.2 * one, 0.1 * one, 0.9 * one)
+    dxi, jac, _ = box_metrics(dims, (1, 1, 1))
+    r = box.rhscal(st, dxi, jac, compact=True)
+    assert np.abs(r["qrhs"]).max() < 1e-13
+
+
+def test_compact_upwind_converges_at_design_order():
+    errs = []
+    for n in (24, 48):
+        dims = (n, 16, 16)
+        X = coords(dims, (1, 1, 1))
+        kx = 2 * np.pi
+        rho = 1.0 + 0.2 * np.sin(kx * X[0])
+        u = 0.5 + 0.1 * np.cos(kx * X[0])
+        p = 1.0 / 1.4 + 0.05 * np.sin(kx * X[0])
+        st = po.prim_to_state(rho, u, 0 * u, 0 * u, p)
+        dxi, jac, h = box_metrics(dims, (1, 1, 1))
+        box = po.UwdBox(dims, recon_schem=3, lchardecomp=False)
+        q = box.rhscal(st, dxi, jac, compact=True)["qrhs"]
+        xs = np.arange(0, n + 1) * h[0]
+        J = h[0] * h[1] * h[2]
+        exact = -J * (0.2 * kx * np.cos(kx * xs) * (0.5 + 0.1 * np.cos(kx * xs)) +
+                      (1.0 + 0.2 * np.sin(kx * xs)) * (-0.1 * kx * np.sin(kx * xs)))
+        errs.append(np.abs(q[:, 5, 5, 0] - exact).max() / np.abs(exact).max())
+    assert math.log2(errs[0] / errs[1]) > 4.5, errs
