@@ -1133,6 +1133,7 @@ struct astr_ctx {
     double* fhf[10] = {};           // compact upwind: their interface fluxes
     double* cmp_ac[3][2] = {};      // compact upwind Thomas factors per axis, wind
     int cmp_first[3] = {}, cmp_last[3] = {};
+    bool lines_ok[3] = {};          // axis fits the compact line sweeps
     double* ssf = nullptr;          // Ducros sensor (upwind path), haloed
     unsigned char* lshock = nullptr; // shock flags, one byte per padded cell
     double* scratch = nullptr;  // field in of the field-level operators
@@ -1371,6 +1372,8 @@ int launch_sweep_p(astr_ctx* c, const SweepArgs& a)
 
 int launch_sweep(astr_ctx* c, const SweepArgs& a)
 {
+    if (!c->lines_ok[a.dir])
+        return fail(ASTR_E_UNSUPPORTED, "axis too long (or absent) for the compact line operators");
     switch (a.pass) {
     case PASS_FILTER: return launch_sweep_p<PASS_FILTER>(c, a);
     case PASS_GRAD: return launch_sweep_p<PASS_GRAD>(c, a);
@@ -1834,17 +1837,25 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     if (two_d && (p->convection == 0 || p->deriv_explicit != 1 || nts[2] != 3))
         return fail(ASTR_E_UNSUPPORTED,
                     "two-dimensional boxes need an upwind convection, deriv_explicit 1 and npdck 3");
+    // the compact line sweeps (filter, compact derivatives, central fluxes)
+    // take axes of up to 473 cells; an upwind context without the filter
+    // and with the explicit gradient never runs them
+    const bool need_lines = p->convection == 0 || p->lfilter || !p->deriv_explicit;
+    bool lines_ok[3] = {false, false, false};
     for (int d = 0; d < 3; ++d) {
         if (d == 2 && two_d) continue;
         if (dims[d] < 16) return fail(ASTR_E_UNSUPPORTED, "each axis needs at least 16 cells");
         if (nts[d] < 1 || nts[d] > 4) return fail(ASTR_E_ARG, "npdc must be 1..4");
+        if (dims[d] > 1000000) return fail(ASTR_E_UNSUPPORTED, "axis longer than 10^6 cells");
         const int L = (dims[d] + 7 + LPL - 1) / LPL;
-        if (L > LMAX || lds_bytes(0, L) > 65536 || lds_bytes(1, L) > 65536)
-            return fail(ASTR_E_UNSUPPORTED, "axis longer than 473 cells");
+        lines_ok[d] = L <= LMAX && lds_bytes(0, L) <= 65536 && lds_bytes(1, L) <= 65536;
+        if (!lines_ok[d] && need_lines)
+            return fail(ASTR_E_UNSUPPORTED, "axis longer than 473 cells for the compact line operators");
     }
     auto* c = new astr_ctx();
     c->p = *p;
     c->device = device;
+    for (int d = 0; d < 3; ++d) c->lines_ok[d] = lines_ok[d];
     auto cleanup = [&](int rc) {
         astr_destroy(c);
         return rc;
@@ -1906,9 +1917,10 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     std::vector<double> tabs;
     std::vector<long> offs;
     for (int d = 0; d < 3; ++d) {
-        // a two-dimensional box keeps the line systems of a 16-cell k axis,
-        // which no entry point uses
-        const int nd = (d == 2 && two_d) ? 16 : dims[d];
+        // an axis the compact sweeps cannot take (the k axis of a 2-D box, a
+        // long axis of an upwind context) keeps the line systems of a
+        // 16-cell axis, which no entry point then uses
+        const int nd = lines_ok[d] ? dims[d] : 16;
         c->L[d] = (nd + 7 + LPL - 1) / LPL;
         for (int op = 0; op < 2; ++op) {
             HostTab h = build_tab(op, nts[d], nd, p->alfa_filter, c->L[d]);
@@ -1946,7 +1958,8 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
         hipMemcpy(c->fc, &fc, sizeof fc, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(ASTR_E_HIP, "coefficient upload failed"));
 
-    c->xi_stride = LPL * LMAX + 2 * HM + 1;
+    c->xi_stride = std::max(LPL * LMAX, std::max(dims[0], std::max(dims[1], dims[2])) + 1 + LPL) +
+                   2 * HM + 1;
     if (hipMalloc(&c->xi, sizeof(double) * 3 * c->xi_stride) != hipSuccess)
         return cleanup(fail(ASTR_E_NOMEM, "metric allocation failed"));
     (void)hipMemset(c->xi, 0, sizeof(double) * 3 * c->xi_stride);
@@ -2176,6 +2189,117 @@ int astr_bench_copy(astr_ctx* c, int32_t reps, double* ms_per_copy, int64_t* byt
     return ASTR_OK;
 }
 
+struct astr_batch {
+    std::vector<astr_ctx*> ctx;
+    int device = 0;
+    hipStream_t root = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double dt = 0.0;
+    bool built = false;
+};
+
+int astr_batch_destroy(astr_batch* b)
+{
+    if (!b) return ASTR_OK;
+    (void)hipSetDevice(b->device);
+    if (b->root) (void)hipStreamSynchronize(b->root);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->root) (void)hipStreamDestroy(b->root);
+    delete b;
+    return ASTR_OK;
+}
+
+int astr_batch_create(astr_ctx* const* ctxs, int32_t n, astr_batch** out)
+{
+    if (!ctxs || !out || n < 1) return fail(ASTR_E_ARG, "bad argument");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        TRY(ready(ctxs[i]));
+        if (ctxs[i]->device != ctxs[0]->device)
+            return fail(ASTR_E_ARG, "batch members must share one device");
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) return fail(ASTR_E_ARG, "a context appears twice in the batch");
+    }
+    auto* b = new astr_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    b->device = ctxs[0]->device;
+    auto cleanup = [&](int rc) {
+        astr_batch_destroy(b);
+        return rc;
+    };
+    if (hipSetDevice(b->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->root, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(ASTR_E_HIP, "batch stream set-up failed"));
+    b->join.assign(n, nullptr);
+    for (int i = 0; i < n; ++i)
+        if (hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(ASTR_E_HIP, "batch event set-up failed"));
+    *out = b;
+    return ASTR_OK;
+}
+
+// one RK3 step of every member, each on its own stream, forked from and
+// joined back into the root stream under capture
+static int batch_capture(astr_batch* b, double dt)
+{
+    if (b->exec) {
+        HIPCHK(hipGraphExecDestroy(b->exec));
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        HIPCHK(hipGraphDestroy(b->graph));
+        b->graph = nullptr;
+    }
+    // members' own queued work must be done before the graph reads their fields
+    for (astr_ctx* c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamBeginCapture(b->root, hipStreamCaptureModeThreadLocal));
+    int rc = ASTR_OK;
+    if (hipEventRecord(b->fork, b->root) != hipSuccess) rc = fail(ASTR_E_HIP, "fork record failed");
+    for (size_t i = 0; rc == ASTR_OK && i < b->ctx.size(); ++i) {
+        astr_ctx* c = b->ctx[i];
+        if (hipStreamWaitEvent(c->stream, b->fork, 0) != hipSuccess) {
+            rc = fail(ASTR_E_HIP, "fork wait failed");
+            break;
+        }
+        rc = do_step(c, dt, nullptr, false);
+        if (rc == ASTR_OK && (hipEventRecord(b->join[i], c->stream) != hipSuccess ||
+                              hipStreamWaitEvent(b->root, b->join[i], 0) != hipSuccess))
+            rc = fail(ASTR_E_HIP, "join failed");
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(b->root, &g);
+    if (rc != ASTR_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(ASTR_E_HIP, "graph capture failed");
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->dt = dt;
+    b->built = true;
+    return ASTR_OK;
+}
+
+int astr_batch_rk3_steps(astr_batch* b, int32_t nsteps, double dt)
+{
+    if (!b || nsteps < 0) return fail(ASTR_E_ARG, "bad argument");
+    HIPCHK(hipSetDevice(b->device));
+    if (!b->built || b->dt != dt) TRY(batch_capture(b, dt));
+    else
+        for (astr_ctx* c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(b->exec, b->root));
+    HIPCHK(hipStreamSynchronize(b->root));
+    return ASTR_OK;
+}
+
 int astr_synchronize(astr_ctx* c)
 {
     if (!c) return fail(ASTR_E_ARG, "null context");
@@ -2218,7 +2342,6 @@ static int field_op(astr_ctx* c, int dir, const double* f, double* df, int pass)
                            c->stream, a);
         HIPCHK(hipGetLastError());
     } else {
-        if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "compact operators need a 3-D box");
         SweepArgs a = sweep_args(c, dir - 1, pass);
         a.fin = c->scratch;
         a.fout = c->scratch2;

@@ -144,6 +144,26 @@ module astr_hip
       real(c_double), intent(out) :: ff(*)
     end function astr_filter_field
 
+    ! batched ensemble: ctxs(1:n) advance together (one HIP graph per step)
+    integer(c_int) function astr_batch_create(ctxs, n, batch) bind(c, name='astr_batch_create')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), intent(in) :: ctxs(*)
+      integer(c_int32_t), value :: n
+      type(c_ptr), intent(out) :: batch
+    end function astr_batch_create
+
+    integer(c_int) function astr_batch_destroy(batch) bind(c, name='astr_batch_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function astr_batch_destroy
+
+    integer(c_int) function astr_batch_rk3_steps(batch, nsteps, dt) bind(c, name='astr_batch_rk3_steps')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: nsteps
+      real(c_double), value :: dt
+    end function astr_batch_rk3_steps
+
     integer(c_int) function astr_synchronize(ctx) bind(c, name='astr_synchronize')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

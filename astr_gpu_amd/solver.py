@@ -174,6 +174,115 @@ class AstrSolver:
 
 # -------------------------------------------------------------- set-up data
 
+class AstrBatch:
+    """Contexts that advance together: one HIP graph per RK3 step covering
+    every member (astr_batch_*).  The members stay usable on their own."""
+
+    def __init__(self, solvers):
+        self.lib = _lib.load()
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s.ctx for s in self.solvers])
+        b = ctypes.c_void_p()
+        check(self.lib.astr_batch_create(arr, len(self.solvers), ctypes.byref(b)),
+              "astr_batch_create")
+        self.b = b
+
+    def rk3_steps(self, nsteps: int, dt: float) -> None:
+        check(self.lib.astr_batch_rk3_steps(self.b, int(nsteps), float(dt)),
+              "astr_batch_rk3_steps")
+
+    def close(self) -> None:
+        if self.b:
+            self.lib.astr_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def hit_state(p: AstrParams, seed: int, mt: float = 0.3, k0: float = 4.0,
+              lengths=(2 * math.pi,) * 3):
+    """Decaying isotropic turbulence initial field: a solenoidal random
+    velocity with spectrum E(k) = A k^4 exp(-2 (k/k0)^2) built as
+    div_free_gen does (src/pp.F90:2589-2837: Rogallo's construction from two
+    random complex amplitudes per wavevector, Hermitian completion, inverse
+    FFT; IniEnergDis, :2849-2859), from a seeded generator instead of the
+    reference's unseeded random_number.  The velocity is scaled so that
+    sqrt(<u.u>) / c = mt with uniform rho = 1 and T = 1 (c = 1/Ma in the
+    nondimensional EoS, so Ma is taken equal to mt); p = 1/(gamma Ma^2).
+    Interior nodes 0..n on each axis, node n a copy of node 0; halos zero
+    (qswap fills them)."""
+    rng = np.random.default_rng(seed)
+    n = (p.im, p.jm, p.km)
+    kx = np.fft.fftfreq(n[0], d=1.0 / n[0])
+    ky = np.fft.fftfreq(n[1], d=1.0 / n[1])
+    kz = np.fft.rfftfreq(n[2], d=1.0 / n[2])
+    K1, K2, K3 = np.meshgrid(kx, ky, kz, indexing="ij")
+    kk = np.sqrt(K1 ** 2 + K2 ** 2 + K3 ** 2)
+    k12 = np.sqrt(K1 ** 2 + K2 ** 2)
+    rn = rng.random((3,) + kk.shape) * 2 * np.pi
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ek = 2.0 / 224.7699 * kk ** 4 * np.exp(-2.0 * (kk / k0) ** 2)
+        amp = np.where(kk > 0, np.sqrt(ek / (4 * np.pi * kk ** 2)), 0.0)
+        a1 = amp * np.exp(1j * rn[0]) * np.cos(rn[2])
+        a2 = amp * np.exp(1j * rn[1]) * np.sin(rn[2])
+        on_axis = k12 == 0
+        u1 = np.where(on_axis, a1, (a1 * kk * K2 + a2 * K1 * K3) / (kk * k12))
+        u2 = np.where(on_axis, a2, (a2 * K2 * K3 - a1 * kk * K1) / (kk * k12))
+        u3 = np.where(on_axis, 0.0, -a2 * k12 / kk)
+    u1[kk == 0] = u2[kk == 0] = u3[kk == 0] = 0.0
+    vel = [np.fft.irfftn(u, s=n, axes=(0, 1, 2)) for u in (u1, u2, u3)]
+    # solenoidal projection on the full spectrum (the real transform's
+    # Hermitian completion of the k3 = 0 and Nyquist planes breaks the
+    # construction's divergence-free property there); Nyquist modes removed
+    fk = [np.fft.fftn(v) for v in vel]
+    k = [np.fft.fftfreq(n[d], d=1.0 / n[d]) for d in range(3)]
+    K = np.meshgrid(*k, indexing="ij")
+    k2 = K[0] ** 2 + K[1] ** 2 + K[2] ** 2
+    k2[0, 0, 0] = 1.0
+    dot = (K[0] * fk[0] + K[1] * fk[1] + K[2] * fk[2]) / k2
+    nyq = np.zeros(k2.shape, dtype=bool)
+    for d in range(3):
+        if n[d] % 2 == 0:
+            nyq |= np.abs(K[d]) == n[d] // 2
+    vel = []
+    for d in range(3):
+        g = fk[d] - K[d] * dot
+        g[nyq] = 0.0
+        vel.append(np.real(np.fft.ifftn(g)))
+    urms = math.sqrt(sum(float(np.mean(v * v)) for v in vel))
+    vel = [v / urms for v in vel]  # sqrt(<u.u>) = 1, so Mt = Ma
+    gamma, mach = p.gamma, p.mach
+    const1 = 1.0 / (gamma * (gamma - 1.0) * (mach * mach))
+    const2 = gamma * (mach * mach)
+    s = haloed_shape(p)
+    idx = [np.mod(np.arange(n[d] + 1), n[d]) for d in range(3)]
+    ix = np.ix_(*idx)
+    u, v, w = (vv[ix] for vv in vel)
+    rho = np.ones_like(u)
+    prs = np.full_like(u, 1.0 / const2)
+    tmp = prs / rho * const2
+    out = {"q": np.zeros(s + (5,), order="F"), "rho": np.zeros(s, order="F"),
+           "vel": np.zeros(s + (3,), order="F"), "prs": np.zeros(s, order="F"),
+           "tmp": np.zeros(s, order="F")}
+    I = (slice(HM, HM + n[0] + 1), slice(HM, HM + n[1] + 1), slice(HM, HM + n[2] + 1))
+    out["rho"][I] = rho
+    out["vel"][I + (0,)] = u
+    out["vel"][I + (1,)] = v
+    out["vel"][I + (2,)] = w
+    out["prs"][I] = prs
+    out["tmp"][I] = tmp
+    out["q"][I + (0,)] = rho
+    out["q"][I + (1,)] = rho * u
+    out["q"][I + (2,)] = rho * v
+    out["q"][I + (3,)] = rho * w
+    out["q"][I + (4,)] = rho * (tmp * const1 + 0.5 * (u * u + v * v + w * w))
+    return out
+
+
 def box_metrics(p: AstrParams, lengths=(2 * math.pi,) * 3):
     """Metrics of the uniform box of gridcube (src/gridgeneration.F90:233-267):
     dxi = diag(1/dx, 1/dy, 1/dz), jacob = dx*dy*dz, halos included (the

@@ -164,6 +164,20 @@ int astr_filter_field(astr_ctx *ctx, int32_t dir, const double *f, double *ff);
 int astr_set_timing(astr_ctx *ctx, int32_t on);
 int astr_get_timing(astr_ctx *ctx, double *ms9);
 
+/* --- batched ensemble -------------------------------------------------- */
+
+/* A group of contexts (same device; each created, with geometry and state
+ * uploaded; three-dimensional periodic boxes) that advance together.
+ * astr_batch_rk3_steps captures one RK3 step of every member into one HIP
+ * graph, each member's kernels on its own stream branching from the batch's
+ * root stream so that boxes of any size run concurrently, and replays the
+ * graph once per step (it is rebuilt when dt changes).  No counterpart in
+ * the reference, which advances one box per run. */
+typedef struct astr_batch astr_batch;
+int astr_batch_create(astr_ctx *const *ctxs, int32_t n, astr_batch **out);
+int astr_batch_destroy(astr_batch *b);
+int astr_batch_rk3_steps(astr_batch *b, int32_t nsteps, double dt);
+
 /* Wait for the work queued on the context's stream (entry points that
  * only queue kernels, such as astr_rhscal, return before it ends).  No
  * counterpart in the reference, whose calls are synchronous. */

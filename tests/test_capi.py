@@ -93,3 +93,26 @@ def test_host_initial_state_matches_closed_form():
     u = ag.interior(st["vel"], p)
     ke = 0.5 * np.mean(np.sum(u[1:, 1:, 1:] ** 2, axis=-1))
     assert abs(ke - 0.125) < 1e-12
+
+
+def test_hit_initial_field_is_solenoidal_with_the_requested_spectrum():
+    import numpy as np
+    p = ag.default_params((32, 40, 48), mach=0.3)
+    st = ag.hit_state(p, 7)
+    u = ag.interior(st["vel"], p)[:-1, :-1, :-1]
+    # the duplicate plane n copies plane 0 on every axis
+    full = ag.interior(st["vel"], p)
+    assert np.array_equal(full[-1], full[0]) and np.array_equal(full[:, -1], full[:, 0])
+    assert abs(np.sqrt((u ** 2).sum(-1).mean()) - 1.0) < 1e-12
+    k = [np.fft.fftfreq(n, 1.0 / n) for n in u.shape[:3]]
+    K = np.meshgrid(*k, indexing="ij")
+    uh = [np.fft.fftn(u[..., c]) for c in range(3)]
+    div = sum(1j * K[c] * uh[c] for c in range(3))
+    assert np.abs(div).max() < 1e-12 * np.abs(uh[0]).max()
+    E = sum(np.abs(h) ** 2 for h in uh)
+    kk = np.sqrt(sum(Kc ** 2 for Kc in K)).round().astype(int)
+    spec = np.bincount(kk.ravel(), E.ravel())
+    assert spec[1:12].argmax() + 1 == 4  # E ~ k^4 exp(-2 (k/4)^2) peaks at k0 = 4
+    # seeded: same seed, same field; another seed, another field
+    assert np.array_equal(ag.hit_state(p, 7)["vel"], st["vel"])
+    assert not np.array_equal(ag.hit_state(p, 8)["vel"], st["vel"])

@@ -153,6 +153,20 @@ def test_upwind_rhs_two_dimensional_quadrants():
             assert e <= 1e-13
 
 
+@pytest.mark.parametrize("compact", [False, True])
+def test_upwind_long_lines_two_dimensional(compact):
+    """x lines longer than one 256-interface tile (several segments per line)
+    and an axis beyond the compact line sweeps' 473-cell limit."""
+    dims = (600, 40, 0)
+    print(f"case long 2-D dims={dims} compact={compact}", flush=True)
+    g, ref, lsh = run_both(dims, (1, 3, 3), 3, True, compact=compact)
+    assert np.array_equal(lsh, ref["lshock"])
+    for m in range(5):
+        e = nerr(g["qrhs"][..., m], ref["qrhs"][..., m])
+        print(f"  qrhs{m + 1}: {e:.2e}")
+        assert e <= 1e-13
+
+
 def test_upwind_with_compact_gradient():
     dims = (36, 34, 32)
     g, ref, lsh = run_both(dims, (3, 3, 3), 3, True, explicit=False)
