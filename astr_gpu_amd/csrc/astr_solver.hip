@@ -99,8 +99,7 @@ struct SweepArgs {
     double* dvel[9];
     double* dtmp[3];
     double* qrhs[5];
-    const double* sig[6];
-    const double* qfl[3];
+    const double* gf[5];    // PASS_RHS: flux of this axis, G = Fv - F, q(1:5)
     const double* jac;
     const double* xi;       // metric along dir, xi[n+HM]
     const double* fin;      // field in (PASS_*_FIELD)
@@ -223,31 +222,8 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
             } else if constexpr (PASS == PASS_GRAD) {
                 v = ldb(a.prim[m < 3 ? 1 + m : 5], od);
             } else if constexpr (PASS == PASS_RHS) {
-                // Inviscid flux fcs(:,1..5) of convrsdcal6 (src/solver.F90:
-                // 2201-2207) with the off-diagonal metrics zero:
-                // F = J*(q_m*U + [m = d+1] dxi(d,d)*p), F5 = J*(q5+p)*U.
-                // Viscous flux ff(:,2..5) of diffrsdcal6 (src/solver.F90:
-                // 2628-2639): the diagonal metric picks sigma(d,c) and
-                // qflux(d), whose halos are plain periodic copies.
-                // The derivative is linear, so the two passes of the
-                // reference (qrhs = -sum D F, then qrhs += sum D Fv) are one
-                // derivative of Fv - F per axis; only the summation order
-                // differs.
-                const double xn = a.xi[node + HM];
-                const double qm = ldb(a.q[m], od), jc = ldb(a.jac, od);
-                const double uu = xn * ldb(a.prim[1 + dir], od);
-                double fi;
-                if (m == 0) fi = jc * qm * uu;
-                else if (m == 4) fi = jc * (qm + ldb(a.prim[4], od)) * uu;
-                else if (m == dir + 1) fi = jc * (qm * uu + xn * ldb(a.prim[4], od));
-                else fi = jc * (qm * uu);
-                if (m == 0 || !a.visc) {
-                    v = -fi;
-                } else {
-                    const unsigned int ow = lob + (unsigned int)wrap_node(node, dim) * sdb;
-                    const double sv = m < 4 ? ldb(a.sig[m - 1], ow) : ldb(a.qfl[0], ow);
-                    v = (sv * xn) * jc - fi;
-                }
+                // the flux of this axis, viscous minus inviscid (k_gflux)
+                v = ldb(a.gf[m], od);
             } else {
                 v = ldb(a.fin, od);
             }
@@ -334,8 +310,9 @@ struct PtArgs {
     const double* qrhs[5];
     const double* dvel[9];
     const double* dtmp[3];
-    double* sig[6];
-    double* qfl[3];
+    double* gfl[15];        // k_gflux: G = Fv - F per axis d and quantity m, 5d+m
+    const double* xi[3];    // metric along each axis, xi[n+HM]
+    int visc;               // include the viscous fluxes (diffterm)
     const double* jac;
     double const1, const2, const5, const6;
     double reynolds, prandtl, tempconst, tempconst1;
@@ -492,15 +469,10 @@ __global__ void k_qswap_seam(PtArgs a)
             }
 }
 
-// sigma and qflux of diffrsdcal6, src/solver.F90:2434-2546 (turbmode none)
-__global__ void k_sigma(PtArgs a)
+// sigma and qflux of diffrsdcal6 (src/solver.F90:2434-2546, turbmode none)
+// at one node: sg = (s11, s12, s13, s22, s23, s33), qf(1:3)
+__device__ __forceinline__ void sigma_at(const PtArgs& a, long gi, double* sg, double* qf)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long ntot = (long)(a.g.n[0] + 1) * (a.g.n[1] + 1) * (a.g.n[2] + 1);
-    if (p >= ntot) return;
-    int i, j, k;
-    interior_ijk(a.g, p, i, j, k);
-    const long gi = gidx(a.g, i, j, k);
     const double miu = miucal(a, a.prim[5][gi]) / a.reynolds;
     const double d11 = a.dvel[0][gi], d12 = a.dvel[1][gi], d13 = a.dvel[2][gi];
     const double d21 = a.dvel[3][gi], d22 = a.dvel[4][gi], d23 = a.dvel[5][gi];
@@ -512,12 +484,92 @@ __global__ void k_sigma(PtArgs a)
     const double hcc = (miu / a.prandtl) / a.const5;
     const double g1 = miu2 * (s11 - skk), g2 = miu2 * s12, g3 = miu2 * s13;
     const double g4 = miu2 * (s22 - skk), g5 = miu2 * s23, g6 = miu2 * (s33 - skk);
-    a.sig[0][gi] = g1; a.sig[1][gi] = g2; a.sig[2][gi] = g3;
-    a.sig[3][gi] = g4; a.sig[4][gi] = g5; a.sig[5][gi] = g6;
+    sg[0] = g1; sg[1] = g2; sg[2] = g3; sg[3] = g4; sg[4] = g5; sg[5] = g6;
     const double u = a.prim[1][gi], v = a.prim[2][gi], w = a.prim[3][gi];
-    a.qfl[0][gi] = hcc * a.dtmp[0][gi] + g1 * u + g2 * v + g3 * w;
-    a.qfl[1][gi] = hcc * a.dtmp[1][gi] + g2 * u + g4 * v + g5 * w;
-    a.qfl[2][gi] = hcc * a.dtmp[2][gi] + g3 * u + g5 * v + g6 * w;
+    qf[0] = hcc * a.dtmp[0][gi] + g1 * u + g2 * v + g3 * w;
+    qf[1] = hcc * a.dtmp[1][gi] + g2 * u + g4 * v + g5 * w;
+    qf[2] = hcc * a.dtmp[2][gi] + g3 * u + g5 * v + g6 * w;
+}
+
+// The flux of axis d that the flux sweep differentiates, G = Fv - F, at one
+// node: inviscid fcs(:,1..5) of convrsdcal6 (src/solver.F90:2201-2207) with
+// the off-diagonal metrics zero, F = J*(q_m*U + [m = d+1] dxi(d,d)*p),
+// F5 = J*(q5+p)*U, and viscous ff(:,2..5) of diffrsdcal6 (src/solver.F90:
+// 2628-2639), where the diagonal metric picks sigma(d,c) and qflux(d).
+// The derivative is linear, so the reference's two passes (qrhs = -sum D F,
+// then qrhs += sum D Fv) become one derivative of Fv - F per axis; only the
+// summation order differs.
+__device__ __forceinline__ void gflux_at(const PtArgs& a, int d, long gi, double xn,
+                                         const double* sg, const double* qf)
+{
+    static constexpr int sidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    const double jc = a.jac[gi];
+    const double uu = xn * a.prim[1 + d][gi];
+    const double p = a.prim[4][gi];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const double qm = a.q[m][gi];
+        double fi;
+        if (m == 0) fi = jc * qm * uu;
+        else if (m == 4) fi = jc * (qm + p) * uu;
+        else if (m == d + 1) fi = jc * (qm * uu + xn * p);
+        else fi = jc * (qm * uu);
+        double v;
+        if (m == 0 || !a.visc) {
+            v = -fi;
+        } else {
+            const double sv = m < 4 ? sg[sidx[d][m - 1]] : qf[d];
+            v = (sv * xn) * jc - fi;
+        }
+        a.gfl[5 * d + m][gi] = v;
+    }
+}
+
+// G of all three axes on the interior, and of axis d on its halo strips
+// (the nodes -hm..-1 and n+1..n+hm of every d-line, where the flux sweep
+// also reads it): sigma and qflux there are those of the periodic image
+// (their halos are plain copies, src/solver.F90:2605-2607), the inviscid
+// part is that of the halo node itself.
+__global__ void k_gflux(PtArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1, nk = a.g.n[2] + 1;
+    const long nin = ni * nj * nk;
+    int c[3];
+    int d = -1;  // -1: interior node (all axes)
+    if (p < nin) {
+        c[0] = (int)(p % ni);
+        c[1] = (int)((p / ni) % nj);
+        c[2] = (int)(p / (ni * nj));
+    } else {
+        long r = p - nin;
+        const long s0 = 2L * HM * nj * nk, s1 = 2L * HM * ni * nk, s2 = 2L * HM * ni * nj;
+        long ext[3] = {ni, nj, nk};
+        if (r < s0) d = 0;
+        else if ((r -= s0) < s1) d = 1;
+        else if ((r -= s1) < s2) d = 2;
+        else return;
+        // r -> (h in 0..2hm-1, the two other axes)
+        const int h = (int)(r % (2 * HM));
+        r /= 2 * HM;
+        const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
+        c[oa] = (int)(r % ext[oa]);
+        c[ob] = (int)(r / ext[oa]);
+        c[d] = h < HM ? -(h + 1) : a.g.n[d] + (h - HM + 1);
+    }
+    const long gi = gidx(a.g, c[0], c[1], c[2]);
+    double sg[6] = {0, 0, 0, 0, 0, 0}, qf[3] = {0, 0, 0};
+    if (d < 0) {
+        if (a.visc) sigma_at(a, gi, sg, qf);
+        for (int e = 0; e < 3; ++e) gflux_at(a, e, gi, a.xi[e][c[e] + HM], sg, qf);
+    } else {
+        if (a.visc) {
+            int w[3] = {c[0], c[1], c[2]};
+            w[d] = wrap_node(c[d], a.g.n[d]);
+            sigma_at(a, gidx(a.g, w[0], w[1], w[2]), sg, qf);
+        }
+        gflux_at(a, d, gi, a.xi[d][c[d] + HM], sg, qf);
+    }
 }
 
 // RK3 combine (src/mainloop.F90:423-445) and updatefvar (src/fludyna.F90:
@@ -1128,7 +1180,7 @@ struct astr_ctx {
     Geo g{};
     long pi = 0, pj = 0, pk = 0, nfield = 0;  // padded dims, elements per field
     double* base = nullptr;
-    double *q[5], *prim[6], *qsave[5], *qrhs[5], *dvel[9], *dtmp[3], *sig[6], *qfl[3], *jac;
+    double *q[5], *prim[6], *qsave[5], *qrhs[5], *dvel[9], *dtmp[3], *gfl[15], *jac;
     double* swf[10] = {};           // compact upwind: split fluxes F+, F-
     double* fhf[10] = {};           // compact upwind: their interface fluxes
     double* cmp_ac[3][2] = {};      // compact upwind Thomas factors per axis, wind
@@ -1290,9 +1342,12 @@ PtArgs pt_args(astr_ctx* c)
     PtArgs a{};
     a.g = c->g;
     for (int m = 0; m < 5; ++m) { a.q[m] = c->q[m]; a.qsave[m] = c->qsave[m]; a.qrhs[m] = c->qrhs[m]; }
-    for (int m = 0; m < 6; ++m) { a.prim[m] = c->prim[m]; a.sig[m] = c->sig[m]; }
+    for (int m = 0; m < 6; ++m) a.prim[m] = c->prim[m];
+    for (int m = 0; m < 15; ++m) a.gfl[m] = c->gfl[m];
+    for (int d = 0; d < 3; ++d) a.xi[d] = c->xi + d * c->xi_stride;
+    a.visc = c->p.diffterm;
     for (int m = 0; m < 9; ++m) a.dvel[m] = c->dvel[m];
-    for (int m = 0; m < 3; ++m) { a.dtmp[m] = c->dtmp[m]; a.qfl[m] = c->qfl[m]; }
+    for (int m = 0; m < 3; ++m) a.dtmp[m] = c->dtmp[m];
     a.jac = c->jac;
     a.const1 = c->const1; a.const2 = c->const2; a.const5 = c->const5; a.const6 = c->const6;
     a.reynolds = c->p.reynolds; a.prandtl = c->p.prandtl;
@@ -1335,9 +1390,7 @@ SweepArgs sweep_args(astr_ctx* c, int dir, int pass)
     for (int m = 0; m < 6; ++m) a.prim[m] = c->prim[m];
     for (int m = 0; m < 9; ++m) a.dvel[m] = c->dvel[m];
     for (int m = 0; m < 3; ++m) a.dtmp[m] = c->dtmp[m];
-    static const int sidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-    for (int cc = 0; cc < 3; ++cc) a.sig[cc] = c->sig[sidx[dir][cc]];
-    a.qfl[0] = c->qfl[dir];
+    for (int m = 0; m < 5; ++m) a.gf[m] = c->gfl[5 * dir + m];
     a.jac = c->jac;
     a.xi = c->xi + dir * c->xi_stride;
     for (int m = 0; m < 5; ++m) a.qsave[m] = c->qsave[m];
@@ -1448,6 +1501,15 @@ int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt)
         TRY(launch_sweep(c, a));
     }
     return ASTR_OK;
+}
+
+// sigma/qflux and the flux G = Fv - F of each axis (interior and the halo
+// strips the flux sweeps read)
+int do_gflux(astr_ctx* c)
+{
+    const long ni = c->g.n[0] + 1, nj = c->g.n[1] + 1, nk = c->g.n[2] + 1;
+    const long n = ni * nj * nk + 2L * HM * (nj * nk + ni * nk + ni * nj);
+    return launch_pt(c, k_gflux, n, pt_args(c));
 }
 
 int ndims_of(const astr_ctx* c) { return c->g.n[2] == 0 ? 2 : 3; }
@@ -1652,7 +1714,7 @@ int do_rhscal(astr_ctx* c, bool timed = false)
         return ASTR_OK;
     }
     TRY(do_grad(c));
-    if (c->p.diffterm) TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
+    TRY(do_gflux(c));
     return do_flux_sweeps(c, 0, 0.0);
 }
 
@@ -1746,7 +1808,7 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
         if (timed) TRY(time_mark(c, 2));
         TRY(do_grad(c));
         if (timed) TRY(time_mark(c, 3));
-        if (!upwind && c->p.diffterm) TRY(launch_pt(c, k_sigma, n_interior(c), pt_args(c)));
+        if (!upwind) TRY(do_gflux(c));
         if (timed) TRY(time_mark(c, 4));
         // rkfirst's statcal sees the stage-1 state and its gradients
         if (s == 1 && stats) TRY(do_stats(c, stats));
@@ -1872,7 +1934,7 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     c->g.s[1] = c->pi;
     c->g.s[2] = c->pi * c->pj;
     c->g.org = OI + HM * c->g.s[1] + HM * c->g.s[2];
-    const int nfields = 5 + 6 + 5 + 5 + 9 + 3 + 6 + 3 + 1 + 2 + (p->convection != 0 ? 1 : 0) +
+    const int nfields = 5 + 6 + 5 + 5 + 9 + 3 + 15 + 1 + 2 + (p->convection != 0 ? 1 : 0) +
                         (p->convection == 2 ? 20 : 0);
     if (hipMalloc(&c->base, sizeof(double) * c->nfield * nfields) != hipSuccess)
         return cleanup(fail(ASTR_E_NOMEM, "device allocation of the fields failed"));
@@ -1884,8 +1946,7 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     for (int m = 0; m < 5; ++m) c->qrhs[m] = take();
     for (int m = 0; m < 9; ++m) c->dvel[m] = take();
     for (int m = 0; m < 3; ++m) c->dtmp[m] = take();
-    for (int m = 0; m < 6; ++m) c->sig[m] = take();
-    for (int m = 0; m < 3; ++m) c->qfl[m] = take();
+    for (int m = 0; m < 15; ++m) c->gfl[m] = take();
     c->jac = take();
     c->scratch = take();
     c->scratch2 = take();

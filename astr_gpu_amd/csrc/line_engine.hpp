@@ -181,15 +181,26 @@ __device__ __forceinline__ double rhs_filter_in(const double* g, const FiltCoef*
            fc->c10[4] * (g[4] + g[-4]) + fc->c10[5] * (g[5] + g[-5]);
 }
 
-// Put v into b[r - sub*L] on the lane that owns row r (r uniform).
+// Put v into b[r - sub*L] on the lane that owns row r (r uniform).  The
+// register index is selected by a branch on the uniform t, so the write is
+// one move on one lane rather than a select over the whole array.
 template <int L>
 __device__ __forceinline__ void set_row(double (&b)[L], int r, int sub, double v)
 {
-    const int owner = r / L, t = r - owner * L;
+    const int owner = r / L, t = __builtin_amdgcn_readfirstlane(r - owner * L);
     if (sub == owner) {
-#pragma unroll
-        for (int tt = 0; tt < L; ++tt)
-            if (tt == t) b[tt] = v;
+        switch (t) {
+#define ASTR_SET_ROW(k) \
+    case k:             \
+        if constexpr (k < L) b[k] = v; \
+        break;
+            ASTR_SET_ROW(0) ASTR_SET_ROW(1) ASTR_SET_ROW(2) ASTR_SET_ROW(3)
+            ASTR_SET_ROW(4) ASTR_SET_ROW(5) ASTR_SET_ROW(6) ASTR_SET_ROW(7)
+            ASTR_SET_ROW(8) ASTR_SET_ROW(9) ASTR_SET_ROW(10) ASTR_SET_ROW(11)
+            ASTR_SET_ROW(12) ASTR_SET_ROW(13) ASTR_SET_ROW(14) ASTR_SET_ROW(15)
+#undef ASTR_SET_ROW
+        default: break;
+        }
     }
 }
 
@@ -229,7 +240,9 @@ __device__ __forceinline__ void solve_line(const double* ln, const LineTab& tb,
     // the closure rows at both ends, computed wave-uniformly
     if constexpr (OP == OP_DERIV) {
         const int ne = (lo_boundary(tb.ntype) ? 2 : 1);
-        for (int j = 0; j < ne; ++j) set_row(x, j, sub, rhs_deriv_edge(ln + HM + tb.first + j, j, tb));
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            if (j < ne) set_row(x, j, sub, rhs_deriv_edge(ln + HM + tb.first + j, j, tb));
         const int me = (hi_boundary(tb.ntype) ? 2 : 1);
         for (int j = 0; j < me; ++j) {
             const int r = R - 1 - j;
@@ -237,7 +250,9 @@ __device__ __forceinline__ void solve_line(const double* ln, const LineTab& tb,
         }
     } else {
         const int ne = lo_boundary(tb.ntype) ? 5 : 3;
-        for (int r = 0; r < ne; ++r) set_row(x, r, sub, rhs_filter_edge(ln, r, tb, fc));
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+            if (r < ne) set_row(x, r, sub, rhs_filter_edge(ln, r, tb, fc));
         const int me = hi_boundary(tb.ntype) ? 5 : 3;
         for (int j = 0; j < me; ++j) {
             const int r = R - 1 - j;

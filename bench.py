@@ -33,19 +33,20 @@ FAMILY_DOUBLES = {
     "filter": 10.0,   # q(1:5) in and out, per axis
     "qswap": None,    # boundary planes only
     "grad": 8.0,      # u, v, w, T in; dvel(.,d), dtmp(d) out
-    "sigma": 25.0,    # dvel 9, dtmp 3, T, u, v, w in; sigma 6, qflux 3 out
+    # dvel 9, dtmp 3, T, u, v, w, q 5, p, J in; the axis fluxes G (3 x 5) out
+    "gflux": 38.0,
     "stats": None,    # once per step (14 doubles per node), not per stage
-    # q 5, J, u_d, p, sigma(d,.) 3, qflux(d) in; x: qrhs out (17); y: qrhs
-    # in and out (22); z: qrhs, qsave in, q out (27; stage 1: qsave out)
-    "flux": (17.0 + 22.0 + 27.0) / 3.0,
+    # G(d,1:5) in; x: qrhs out (10); y: qrhs in and out (15); z: qrhs, J,
+    # q, qsave in, q out (26; stage 1: qsave out instead of in)
+    "flux": (10.0 + 15.0 + 26.0) / 3.0,
     "prim": 11.0,     # q 5 in; rho, u, v, w, p, T out
 }
 FAMILY_KERNEL = {"filter": "k_sweep<L,0> (filter)", "grad": "k_sweep<L,1> (gradient)",
-                 "sigma": "k_sigma", "flux": "k_sweep<L,2> (flux derivatives, RK3 in z)",
+                 "gflux": "k_gflux", "flux": "k_sweep<L,2> (flux derivatives, RK3 in z)",
                  "prim": "k_prim"}
-FAMILY_LAUNCHES = {"filter": 3, "qswap": 2, "grad": 3, "sigma": 1, "stats": 0, "flux": 3,
+FAMILY_LAUNCHES = {"filter": 3, "qswap": 2, "grad": 3, "gflux": 1, "stats": 0, "flux": 3,
                    "prim": 1}
-FAMILY_NAMES = ["filter", "qswap", "grad", "sigma", "stats", "flux", "prim"]
+FAMILY_NAMES = ["filter", "qswap", "grad", "gflux", "stats", "flux", "prim"]
 STATS_DOUBLES = 14.0
 # whole-step algorithmic bytes per node: three stages of the kernels above
 # plus the statistics once

@@ -154,7 +154,8 @@ int astr_filter_field(astr_ctx *ctx, int32_t dir, const double *f, double *ff);
 
 /* Kernel time since timing was (re)enabled with astr_set_timing(ctx, 1),
  * in milliseconds, per kernel family (HIP events on the solver's stream):
- * [0] filter sweeps, [1] qswap, [2] gradient, [3] sigma/qflux,
+ * [0] filter sweeps, [1] qswap, [2] gradient, [3] sigma/qflux and the
+ * flux of each axis,
  * [4] statistics, [5] flux sweeps (central: inviscid and viscous flux
  * derivatives, the z sweep applying the RK3 combine; upwind: interface
  * fluxes), [6] update (central: primitives; upwind: RK3 combine and
