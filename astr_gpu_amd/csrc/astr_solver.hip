@@ -2260,6 +2260,7 @@ struct astr_batch {
     hipGraphExec_t exec = nullptr;
     double dt = 0.0;
     bool built = false;
+    int mode = 0;  // 0: one HIP graph per step; 1: direct launches on the members' streams
 };
 
 int astr_batch_destroy(astr_batch* b)
@@ -2349,10 +2350,25 @@ static int batch_capture(astr_batch* b, double dt)
     return ASTR_OK;
 }
 
+int astr_batch_set_mode(astr_batch* b, int32_t mode)
+{
+    if (!b || mode < 0 || mode > 1) return fail(ASTR_E_ARG, "mode must be 0 (graph) or 1 (streams)");
+    b->mode = mode;
+    return ASTR_OK;
+}
+
 int astr_batch_rk3_steps(astr_batch* b, int32_t nsteps, double dt)
 {
     if (!b || nsteps < 0) return fail(ASTR_E_ARG, "bad argument");
     HIPCHK(hipSetDevice(b->device));
+    if (b->mode == 1) {
+        // every member on its own stream, one step of all members at a time;
+        // the runtime spreads the streams over its hardware queues
+        for (int s = 0; s < nsteps; ++s)
+            for (astr_ctx* c : b->ctx) TRY(do_step(c, dt, nullptr, false));
+        for (astr_ctx* c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));
+        return ASTR_OK;
+    }
     if (!b->built || b->dt != dt) TRY(batch_capture(b, dt));
     else
         for (astr_ctx* c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));

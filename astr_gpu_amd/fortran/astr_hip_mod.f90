@@ -164,6 +164,12 @@ module astr_hip
       real(c_double), value :: dt
     end function astr_batch_rk3_steps
 
+    integer(c_int) function astr_batch_set_mode(batch, mode) bind(c, name='astr_batch_set_mode')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: mode
+    end function astr_batch_set_mode
+
     integer(c_int) function astr_synchronize(ctx) bind(c, name='astr_synchronize')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

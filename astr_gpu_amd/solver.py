@@ -191,6 +191,11 @@ class AstrBatch:
         check(self.lib.astr_batch_rk3_steps(self.b, int(nsteps), float(dt)),
               "astr_batch_rk3_steps")
 
+    def set_mode(self, mode: int) -> None:
+        """0: one HIP graph per step (default); 1: direct launches on the
+        members' own streams."""
+        check(self.lib.astr_batch_set_mode(self.b, int(mode)), "astr_batch_set_mode")
+
     def close(self) -> None:
         if self.b:
             self.lib.astr_batch_destroy(self.b)

@@ -178,6 +178,10 @@ typedef struct astr_batch astr_batch;
 int astr_batch_create(astr_ctx *const *ctxs, int32_t n, astr_batch **out);
 int astr_batch_destroy(astr_batch *b);
 int astr_batch_rk3_steps(astr_batch *b, int32_t nsteps, double dt);
+/* mode 0 (default): the graph above; mode 1: no graph, every member's
+ * kernels launched directly on its own stream, all members one step at a
+ * time (the runtime spreads the streams over its hardware queues). */
+int astr_batch_set_mode(astr_batch *b, int32_t mode);
 
 /* Wait for the work queued on the context's stream (entry points that
  * only queue kernels, such as astr_rhscal, return before it ends).  No

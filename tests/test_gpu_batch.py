@@ -19,15 +19,16 @@ def make_box(dims, seed, **over):
     return s
 
 
-@pytest.mark.parametrize("convection", [0, 1])
-def test_batch_graph_replays_equal_box_by_box_steps(convection):
+@pytest.mark.parametrize("convection,mode", [(0, 0), (1, 0), (0, 1)])
+def test_batch_graph_replays_equal_box_by_box_steps(convection, mode):
     dims = [(32, 32, 32), (40, 36, 32), (48, 32, 40)]
     over = {} if convection == 0 else dict(convection=1, deriv_explicit=1, lfilter=1,
                                            diffterm=0)
-    print(f"case batch convection={convection} dims={dims}", flush=True)
+    print(f"case batch convection={convection} mode={mode} dims={dims}", flush=True)
     eager = [make_box(d, 10 + i, **over) for i, d in enumerate(dims)]
     graph = [make_box(d, 10 + i, **over) for i, d in enumerate(dims)]
     b = ag.AstrBatch(graph)
+    b.set_mode(mode)
     dt = 0.004
     for step in range(3):
         b.rk3_steps(1, dt)
