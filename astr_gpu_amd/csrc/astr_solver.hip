@@ -110,6 +110,10 @@ struct SweepArgs {
     double rka, rkb, rkc, dt;
     double* qsave[5];
     int visc;  // PASS_RHS: include the viscous fluxes (diffterm)
+    // mini-app variant (tgvsolver.F90): derivatives divided by the uniform
+    // spacing h instead of multiplied by the metric; RK3 without J
+    int mini;
+    double h;
 };
 
 // node -> element offset along the line, with periodic wrap (dataswap:
@@ -270,13 +274,29 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
                 if constexpr (PASS == PASS_FILTER) {
                     stb(a.q[m], od, o);
                 } else if constexpr (PASS == PASS_GRAD) {
-                    if (m < 3) stb(a.dvel[m * 3 + dir], od, o * a.xi[n + HM]);
-                    else stb(a.dtmp[dir], od, o * a.xi[n + HM]);
+                    // gradcal: d/dxi times dxi(d,d) (src/comsolver.F90:304-318),
+                    // or df/dx (tgvsolver.F90:420-470)
+                    const double gv = a.mini ? o / a.h : o * a.xi[n + HM];
+                    if (m < 3) stb(a.dvel[m * 3 + dir], od, gv);
+                    else stb(a.dtmp[dir], od, gv);
                 } else if constexpr (PASS == PASS_RHS) {
+                    const double dv = a.mini ? o / a.h : o;
                     if (XD) {
-                        stb(a.qrhs[m], od, o);
+                        stb(a.qrhs[m], od, dv);
                     } else if (a.rk_stage == 0) {
-                        stb(a.qrhs[m], od, ldb(a.qrhs[m], od) + o);
+                        stb(a.qrhs[m], od, ldb(a.qrhs[m], od) + dv);
+                    } else if (a.mini) {
+                        // rk3 of tgvsolver.F90:1040-1060: q = a qsave + b q + c dt qrhs
+                        const double r = ldb(a.qrhs[m], od) + dv;
+                        const double qo = ldb(a.q[m], od);
+                        double qs;
+                        if (a.rk_stage == 1) {
+                            qs = qo;
+                            stb(a.qsave[m], od, qs);
+                        } else {
+                            qs = ldb(a.qsave[m], od);
+                        }
+                        stb(a.q[m], od, a.rka * qs + a.rkb * qo + a.rkc * r * a.dt);
                     } else {
                         // last axis of the step: the RK3 combine of
                         // src/mainloop.F90:423-445 for this quantity; the
@@ -313,6 +333,7 @@ struct PtArgs {
     double* gfl[15];        // k_gflux: G = Fv - F per axis d and quantity m, 5d+m
     const double* xi[3];    // metric along each axis, xi[n+HM]
     int visc;               // include the viscous fluxes (diffterm)
+    int mini;               // mini-app variant (tgvsolver.F90) flux forms
     const double* jac;
     double const1, const2, const5, const6;
     double reynolds, prandtl, tempconst, tempconst1;
@@ -503,9 +524,28 @@ __device__ __forceinline__ void gflux_at(const PtArgs& a, int d, long gi, double
                                          const double* sg, const double* qf)
 {
     static constexpr int sidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    const double p = a.prim[4][gi];
+    if (a.mini) {
+        // tgvsolver.F90:578-631,760-843: fcs = q_m u_d (+ p on the momentum
+        // of the sweep axis), (q5 + p) u_d; ff = sigma(d,c), qflux(d); no
+        // metrics (the derivative is divided by the spacing)
+        const double uu = a.prim[1 + d][gi];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            const double qm = a.q[m][gi];
+            double fc;
+            if (m == 4) fc = (qm + p) * uu;
+            else if (m == d + 1) fc = qm * uu + p;
+            else fc = qm * uu;
+            double v;
+            if (m == 0 || !a.visc) v = -fc;
+            else v = (m < 4 ? sg[sidx[d][m - 1]] : qf[d]) - fc;
+            a.gfl[5 * d + m][gi] = v;
+        }
+        return;
+    }
     const double jc = a.jac[gi];
     const double uu = xn * a.prim[1 + d][gi];
-    const double p = a.prim[4][gi];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         const double qm = a.q[m][gi];
@@ -570,6 +610,45 @@ __global__ void k_gflux(PtArgs a)
         }
         gflux_at(a, d, gi, a.xi[d][c[d] + HM], sg, qf);
     }
+}
+
+// bchomo of the mini-app (tgvsolver.F90:271-369): the primitives of the
+// halo faces copied from their periodic images (x, y and z faces, each over
+// the interior extent of the other axes, so every source is interior) and q
+// rebuilt there with fvar2q from the pressure (tgvsolver.F90:45-66).
+__global__ void k_bchomo(PtArgs a)
+{
+    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1, nk = a.g.n[2] + 1;
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long s0 = 2L * HM * nj * nk, s1 = 2L * HM * ni * nk, s2 = 2L * HM * ni * nj;
+    int d;
+    if (r < s0) d = 0;
+    else if ((r -= s0) < s1) d = 1;
+    else if ((r -= s1) < s2) d = 2;
+    else return;
+    const long ext[3] = {ni, nj, nk};
+    const int h = (int)(r % (2 * HM));
+    r /= 2 * HM;
+    const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
+    int c[3], w[3];
+    c[oa] = w[oa] = (int)(r % ext[oa]);
+    c[ob] = w[ob] = (int)(r / ext[oa]);
+    const int n = a.g.n[d];
+    c[d] = h < HM ? -(h + 1) : n + (h - HM + 1);
+    w[d] = h < HM ? n - (h + 1) : (h - HM + 1);
+    const long gd = gidx(a.g, c[0], c[1], c[2]), gs = gidx(a.g, w[0], w[1], w[2]);
+    double pr[6];
+    for (int k = 0; k < 6; ++k) {
+        pr[k] = a.prim[k][gs];
+        a.prim[k][gd] = pr[k];
+    }
+    const double rho = pr[0], u = pr[1], v = pr[2], ww = pr[3];
+    a.q[0][gd] = rho;
+    a.q[1][gd] = rho * u;
+    a.q[2][gd] = rho * v;
+    a.q[3][gd] = rho * ww;
+    const double var1 = 0.5 * (u * u + v * v + ww * ww);
+    a.q[4][gd] = pr[4] * a.const6 + rho * var1;
 }
 
 // RK3 combine (src/mainloop.F90:423-445) and updatefvar (src/fludyna.F90:
@@ -1186,6 +1265,8 @@ struct astr_ctx {
     double* cmp_ac[3][2] = {};      // compact upwind Thomas factors per axis, wind
     int cmp_first[3] = {}, cmp_last[3] = {};
     bool lines_ok[3] = {};          // axis fits the compact line sweeps
+    double h[3] = {0.0, 0.0, 0.0};  // uniform spacing of the mini-app variant
+    bool have_h = false;
     double* ssf = nullptr;          // Ducros sensor (upwind path), haloed
     unsigned char* lshock = nullptr; // shock flags, one byte per padded cell
     double* scratch = nullptr;  // field in of the field-level operators
@@ -1346,6 +1427,7 @@ PtArgs pt_args(astr_ctx* c)
     for (int m = 0; m < 15; ++m) a.gfl[m] = c->gfl[m];
     for (int d = 0; d < 3; ++d) a.xi[d] = c->xi + d * c->xi_stride;
     a.visc = c->p.diffterm;
+    a.mini = c->p.variant == 1;
     for (int m = 0; m < 9; ++m) a.dvel[m] = c->dvel[m];
     for (int m = 0; m < 3; ++m) a.dtmp[m] = c->dtmp[m];
     a.jac = c->jac;
@@ -1395,6 +1477,8 @@ SweepArgs sweep_args(astr_ctx* c, int dir, int pass)
     a.xi = c->xi + dir * c->xi_stride;
     for (int m = 0; m < 5; ++m) a.qsave[m] = c->qsave[m];
     a.visc = c->p.diffterm;
+    a.mini = c->p.variant == 1;
+    a.h = c->h[dir];
     return a;
 }
 
@@ -1800,6 +1884,40 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
     // interface fluxes), 6 update (central: primitives; upwind: RK3 combine
     // and primitives)
     const bool upwind = c->p.convection != 0;
+    if (c->p.variant == 1) {
+        // rk3 of tgvsolver.F90:994-1082: bchomo, rhscal (gradcal, convection,
+        // diffusion by df/dx), stage-1 statistics and qsave, the RK3 combine
+        // (in the last flux sweep), then the filter and q2fvar
+        for (int s = 1; s <= 3; ++s) {
+            if (timed) TRY(time_mark(c, 1));
+            {
+                const long ni = c->g.n[0] + 1, nj = c->g.n[1] + 1, nk = c->g.n[2] + 1;
+                TRY(launch_pt(c, k_bchomo, 2L * HM * (nj * nk + ni * nk + ni * nj), pt_args(c)));
+            }
+            if (timed) TRY(time_mark(c, 2));
+            TRY(do_grad(c));
+            if (timed) TRY(time_mark(c, 3));
+            TRY(do_gflux(c));
+            if (timed) TRY(time_mark(c, 4));
+            if (s == 1 && stats) TRY(do_stats(c, stats));
+            if (timed) TRY(time_mark(c, 5));
+            TRY(do_flux_sweeps(c, s, dt));
+            if (timed) TRY(time_mark(c, 6));
+            if (c->p.lfilter) TRY(do_filterq(c));
+            TRY(launch_pt(c, k_prim, n_interior(c), pt_args(c)));
+            if (timed) {
+                TRY(time_mark(c, 7));
+                HIPCHK(hipEventSynchronize(c->ev[7]));
+                float ms = 0.f;
+                for (int f = 1; f < 7; ++f) {
+                    HIPCHK(hipEventElapsedTime(&ms, c->ev[f], c->ev[f + 1]));
+                    c->tms[f] += ms;
+                    c->tms[7] += ms;
+                }
+            }
+        }
+        return ASTR_OK;
+    }
     for (int s = 1; s <= 3; ++s) {
         if (timed) TRY(time_mark(c, 0));
         if (c->p.lfilter) TRY(do_filterq(c));
@@ -1877,7 +1995,10 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     *out = nullptr;
     if (p->hm != HM) return fail(ASTR_E_UNSUPPORTED, "hm must be 5");
     if (p->numq != 5) return fail(ASTR_E_UNSUPPORTED, "numq must be 5");
-    if (p->variant != 0) return fail(ASTR_E_UNSUPPORTED, "only variant 0 is implemented");
+    if (p->variant != 0 && p->variant != 1)
+        return fail(ASTR_E_ARG, "variant must be 0 (src/mainloop.F90) or 1 (tgvsolver.F90)");
+    if (p->variant == 1 && (p->convection != 0 || p->km == 0))
+        return fail(ASTR_E_UNSUPPORTED, "the mini-app variant is the central scheme on a 3-D box");
     const int dims[3] = {p->im, p->jm, p->km};
     const int nts[3] = {p->npdci, p->npdcj, p->npdck};
     if (p->convection < 0 || p->convection > 2)
@@ -2153,6 +2274,8 @@ static int ready_state(astr_ctx* c)
 static int ready(astr_ctx* c)
 {
     TRY(ready_state(c));
+    if (c->p.variant == 1 && !c->have_h)
+        return fail(ASTR_E_STATE, "the mini-app variant needs astr_set_spacing first");
     if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "two-dimensional boxes support astr_rhscal only");
     for (int d = 0; d < 3; ++d) {
         const int nt = d == 0 ? c->p.npdci : d == 1 ? c->p.npdcj : c->p.npdck;
@@ -2164,12 +2287,14 @@ static int ready(astr_ctx* c)
 int astr_filterq(astr_ctx* c)
 {
     TRY(ready(c));
+    if (c->p.variant == 1) return fail(ASTR_E_UNSUPPORTED, "the mini-app variant runs whole steps only");
     return do_filterq(c);
 }
 
 int astr_qswap(astr_ctx* c)
 {
     TRY(ready(c));
+    if (c->p.variant == 1) return fail(ASTR_E_UNSUPPORTED, "the mini-app variant runs whole steps only");
     return do_qswap(c);
 }
 
@@ -2178,6 +2303,7 @@ int astr_rhscal(astr_ctx* c)
     // the upwind right-hand side takes any closure type with halos as given
     if (c && c->p.convection != 0) TRY(ready_state(c));
     else TRY(ready(c));
+    if (c->p.variant == 1) return fail(ASTR_E_UNSUPPORTED, "the mini-app variant runs whole steps only");
     return do_rhscal(c, c->timing);
 }
 
@@ -2199,6 +2325,7 @@ int astr_download_shock(astr_ctx* c, uint8_t* lshock)
 int astr_rk3_update(astr_ctx* c, int32_t stage, double dt)
 {
     TRY(ready(c));
+    if (c->p.variant == 1) return fail(ASTR_E_UNSUPPORTED, "the mini-app variant runs whole steps only");
     if (stage < 1 || stage > 3) return fail(ASTR_E_ARG, "stage must be 1..3");
     return do_rk(c, stage, dt);
 }
@@ -2374,6 +2501,17 @@ int astr_batch_rk3_steps(astr_batch* b, int32_t nsteps, double dt)
         for (astr_ctx* c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));
     for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(b->exec, b->root));
     HIPCHK(hipStreamSynchronize(b->root));
+    return ASTR_OK;
+}
+
+int astr_set_spacing(astr_ctx* c, double dx, double dy, double dz)
+{
+    if (!c) return fail(ASTR_E_ARG, "null context");
+    if (!(dx > 0.0 && dy > 0.0 && dz > 0.0)) return fail(ASTR_E_ARG, "spacings must be positive");
+    c->h[0] = dx;
+    c->h[1] = dy;
+    c->h[2] = dz;
+    c->have_h = true;
     return ASTR_OK;
 }
 

@@ -170,6 +170,13 @@ module astr_hip
       integer(c_int32_t), value :: mode
     end function astr_batch_set_mode
 
+    ! uniform spacing of the mini-app variant (variant 1)
+    integer(c_int) function astr_set_spacing(ctx, dx, dy, dz) bind(c, name='astr_set_spacing')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dx, dy, dz
+    end function astr_set_spacing
+
     integer(c_int) function astr_synchronize(ctx) bind(c, name='astr_synchronize')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -134,6 +134,10 @@ class AstrSolver:
               "astr_rk3_steps")
         return out
 
+    def set_spacing(self, dx: float, dy: float, dz: float) -> None:
+        check(self.lib.astr_set_spacing(self.ctx, float(dx), float(dy), float(dz)),
+              "astr_set_spacing")
+
     def synchronize(self) -> None:
         check(self.lib.astr_synchronize(self.ctx), "astr_synchronize")
 

@@ -43,7 +43,9 @@ typedef struct astr_params {
     int32_t numq;                /* must be 5 (no species, no k-omega)    */
     int32_t npdci, npdcj, npdck; /* closure type per axis; the solver
                                     steps periodic boxes (3 on all axes)  */
-    int32_t variant;             /* 0: src/mainloop.F90 stage order       */
+    int32_t variant;             /* 0: src/mainloop.F90 stage order; 1: the
+                                    mini-app tgvsolver.F90 (bchomo halos,
+                                    df/dx, filter after the update)       */
     int32_t reserved;
     double alfa_filter;          /* 0.49 in every deck                    */
     double reynolds, mach, prandtl, gamma;
@@ -182,6 +184,12 @@ int astr_batch_rk3_steps(astr_batch *b, int32_t nsteps, double dt);
  * kernels launched directly on its own stream, all members one step at a
  * time (the runtime spreads the streams over its hardware queues). */
 int astr_batch_set_mode(astr_batch *b, int32_t mode);
+
+/* Uniform spacing of the mini-app variant (variant 1,
+ * miniapps/tgv_solver_3d/tgvsolver.F90), which divides every derivative
+ * by dx, dy, dz (tgvsolver.F90:420-470) instead of using metrics; its
+ * steps need it.  Variant 1 runs whole steps (astr_rk3_step(s)) only. */
+int astr_set_spacing(astr_ctx *ctx, double dx, double dy, double dz);
 
 /* Wait for the work queued on the context's stream (entry points that
  * only queue kernels, such as astr_rhscal, return before it ends).  No
