@@ -4,9 +4,10 @@ The compute path is the HIP library astr_gpu_amd/libastr_hip.so behind the
 C ABI of include/astr_hip.h; this package is its host-side binding.
 """
 from ._lib import AstrError, AstrParams, load  # noqa: F401
-from .solver import (AstrBatch, AstrSolver, box_metrics, default_params,  # noqa: F401
-                     haloed_shape, hit_state, interior, riemann2d_state, tgv_state)
+from .solver import (UDF_RIEMANN2D, AstrBatch, AstrSolver, box_metrics, cfl_dt,  # noqa: F401
+                     default_params, haloed_shape, hit_state, interior, riemann2d_state,
+                     tgv_state)
 
-__all__ = ["AstrBatch", "AstrError", "AstrParams", "AstrSolver", "box_metrics",
-           "default_params", "haloed_shape", "hit_state", "interior", "load", "riemann2d_state",
+__all__ = ["UDF_RIEMANN2D", "AstrBatch", "AstrError", "AstrParams", "AstrSolver", "box_metrics",
+           "cfl_dt", "default_params", "haloed_shape", "hit_state", "interior", "load", "riemann2d_state",
            "tgv_state"]

@@ -62,6 +62,10 @@ SIGNATURES = {
     "astr_set_timing": (ctypes.c_int, [_P, _I]),
     "astr_synchronize": (ctypes.c_int, [_P]),
     "astr_set_spacing": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "astr_set_bc": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
+    "astr_boucon": (ctypes.c_int, [_P]),
+    "astr_set_time": (ctypes.c_int, [_P, ctypes.c_double]),
+    "astr_get_time": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
     "astr_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P)]),
     "astr_batch_destroy": (ctypes.c_int, [_P]),
     "astr_batch_rk3_steps": (ctypes.c_int, [_P, _I, ctypes.c_double]),

@@ -7,20 +7,26 @@
 // with i fastest; node i=0 sits at a 128-byte boundary and the row pitch s1
 // is a multiple of 16 doubles, so 16 consecutive i form one 128-byte line.
 //
-// Line sweeps.  A workgroup of 512 threads (8 wavefronts) owns a tile of 16
-// lines.  For each quantity it stages the 16 lines (nodes -HM..n+HM) in LDS
-// with coalesced loads, each wavefront solves 2 lines with the chunked-scan
-// Thomas solver of line_engine.hpp, and the results go back through an LDS
-// output tile with coalesced stores.  x-lines are contiguous; y/z tiles are
-// 16 consecutive i, i.e. one 128-byte segment per node.
+// Line sweeps.  A workgroup of 256 threads (4 wavefronts) owns a tile of
+// lines and one quantity: 8 x-lines (one per half-wavefront) or 16 y/z
+// lines (16 consecutive i, one 128-byte segment per node).  It stages the
+// lines (nodes -HM..n+HM) in LDS with coalesced loads, each half-wavefront
+// solves one line with the chunked-scan Thomas solver of line_engine.hpp,
+// and the results overwrite the staged line and leave with coalesced stores.
 //
-// Per RK3 stage (src/mainloop.F90:392-476) the device runs:
+// Per RK3 stage of the central path (src/mainloop.F90:392-476) the device
+// runs:
 //   filter x,y,z (in place)           filterq      src/comsolver.F90:514
 //   qswap halo + seam average          qswap        src/parallel.F90:4849
-//   conv+grad sweeps x,y,z             gradcal + convrsdcal6 (+ qrhs=-qrhs)
-//   sigma/qflux pointwise              diffrsdcal6 first half
-//   viscous sweeps x,y,z               diffrsdcal6 second half
-//   RK3 combine + q2fvar               src/mainloop.F90:435-476
+//   gradient sweeps x,y,z              gradcal      src/comsolver.F90:244
+//   k_gflux (pointwise)                sigma, qflux and the axis fluxes
+//                                      G = Fv - F of convrsdcal6/diffrsdcal6
+//   flux sweeps x,y,z                  one derivative of G per axis into qrhs
+//                                      (qrhs=-qrhs folded in); the z sweep
+//                                      applies the RK3 combine
+//   k_prim                             updatefvar   src/mainloop.F90:476
+// The upwind paths (convrsduwd, convrsdcmp) run k_uwd per axis with the
+// Ducros sensor and the explicit gradient; their RK3 combine is k_rk.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -116,6 +122,23 @@ struct SweepArgs {
     double h;
 };
 
+// The RK3 combine in the reference's operation order, without FMA
+// contraction (as k_rk and the checker evaluate it): q = (a qsave + b q J +
+// c qrhs dt)/J (src/mainloop.F90:435-445), and the mini-app's form without
+// J (tgvsolver.F90:1040-1060).
+#pragma clang fp contract(off)
+__device__ __forceinline__ double rk_combine(double ra, double rb, double rc, double dt, double qs,
+                                             double qo, double jc, double r)
+{
+    return (ra * qs + rb * qo * jc + rc * r * dt) / jc;
+}
+__device__ __forceinline__ double rk_combine_mini(double ra, double rb, double rc, double dt,
+                                                  double qs, double qo, double r)
+{
+    return ra * qs + rb * qo + rc * r * dt;
+}
+#pragma clang fp contract(fast)
+
 // node -> element offset along the line, with periodic wrap (dataswap:
 // plain copy of hm planes, src/parallel.F90:4170-4175) or direct.
 __device__ __forceinline__ int wrap_node(int n, int dim)
@@ -152,6 +175,88 @@ __device__ __forceinline__ double ldb(const double* base, unsigned int boff)
 __device__ __forceinline__ void stb(double* base, unsigned int boff, double v)
 {
     *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
+// Write-back of the y/z flux sweeps for the nodes n0 + nstep*e (e <
+// EMAX) of one lane, given the derivative o of G in the LDS row trow:
+//   MODE 0  qrhs += o (y axis, or z when no RK3 combine follows)
+//   MODE 1  z axis, RK3 stage 1: qsave = q J, then the combine
+//   MODE 2  z axis, RK3 stages 2-3: the combine with the saved qsave
+//   MODE 3/4 the mini-app's combine without J (stage 1 / stages 2-3)
+// The operands of a batch of RB nodes are loaded before the stores of the
+// previous batch (a store to q or qrhs could alias a later load, so the
+// compiler keeps program order); nodes past dim load node dim and store
+// nothing.
+template <int MODE, int EMAX>
+__device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const double* trow,
+                                             int n0, int nstep, int dim, unsigned int lob,
+                                             unsigned int sdb)
+{
+    constexpr int RB = MODE == 0 ? 8 : 4;
+    constexpr int NB = (EMAX + RB - 1) / RB;
+    double cr[RB] = {}, cq[RB] = {}, cs[RB] = {}, cj[RB] = {};  // batch being combined
+    double nr[RB] = {}, nq[RB] = {}, ns[RB] = {}, nj[RB] = {};  // next batch, loading
+    auto load = [&](int b, double* r, double* q, double* sv, double* jv) {
+#pragma unroll
+        for (int g = 0; g < RB; ++g) {
+            const int e = b * RB + g;
+            if (e < EMAX) {
+                const int n = min(n0 + nstep * e, dim);
+                const unsigned int od = lob + (unsigned int)n * sdb;
+                r[g] = ldb(a.qrhs[m], od);
+                if constexpr (MODE != 0) q[g] = ldb(a.q[m], od);
+                if constexpr (MODE == 1 || MODE == 2) jv[g] = ldb(a.jac, od);
+                if constexpr (MODE == 2 || MODE == 4) sv[g] = ldb(a.qsave[m], od);
+            }
+        }
+    };
+    load(0, cr, cq, cs, cj);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if (b + 1 < NB) load(b + 1, nr, nq, ns, nj);
+#pragma unroll
+        for (int g = 0; g < RB; ++g) {
+            const int e = b * RB + g;
+            const int n = n0 + nstep * e;
+            if (e < EMAX && n <= dim) {
+                const double o = trow[n + HM];
+                const unsigned int od = lob + (unsigned int)n * sdb;
+                if constexpr (MODE == 0) {
+                    stb(a.qrhs[m], od, cr[g] + (a.mini ? o / a.h : o));
+                } else if constexpr (MODE == 1 || MODE == 2) {
+                    // the RK3 combine of src/mainloop.F90:423-445 for this
+                    // quantity; the workgroup owns these nodes of q_m
+                    const double r = cr[g] + o;
+                    double qs;
+                    if constexpr (MODE == 1) {
+                        qs = cq[g] * cj[g];
+                        stb(a.qsave[m], od, qs);
+                    } else {
+                        qs = cs[g];
+                    }
+                    stb(a.q[m], od, rk_combine(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], cj[g], r));
+                } else {
+                    // rk3 of tgvsolver.F90:1040-1060: q = a qsave + b q + c dt qrhs
+                    const double r = cr[g] + o / a.h;
+                    double qs;
+                    if constexpr (MODE == 3) {
+                        qs = cq[g];
+                        stb(a.qsave[m], od, qs);
+                    } else {
+                        qs = cs[g];
+                    }
+                    stb(a.q[m], od, rk_combine_mini(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], r));
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < RB; ++g) {
+            cr[g] = nr[g];
+            cq[g] = nq[g];
+            cs[g] = ns[g];
+            cj[g] = nj[g];
+        }
+    }
 }
 
 template <int L, int PASS, bool XD>
@@ -264,6 +369,21 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     sync();
 
     // ---------------- write back nodes 0..dim
+    if constexpr (PASS == PASS_RHS && !XD) {
+        // read-modify-write of qrhs (y) or the RK3 combine (z), specialised
+        // per mode so that each is straight-line code
+        if (lvalid) {
+            const int mode = a.rk_stage == 0 ? 0 : (a.mini ? 3 : 1) + (a.rk_stage > 1);
+            switch (mode) {
+            case 0: rhs_epilogue<0, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
+            case 1: rhs_epilogue<1, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
+            case 2: rhs_epilogue<2, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
+            case 3: rhs_epilogue<3, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
+            default: rhs_epilogue<4, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
+            }
+        }
+        return;
+    }
     if (lvalid) {
 #pragma unroll
         for (int e = 0; e < EMAX; ++e) {
@@ -280,38 +400,8 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
                     if (m < 3) stb(a.dvel[m * 3 + dir], od, gv);
                     else stb(a.dtmp[dir], od, gv);
                 } else if constexpr (PASS == PASS_RHS) {
-                    const double dv = a.mini ? o / a.h : o;
-                    if (XD) {
-                        stb(a.qrhs[m], od, dv);
-                    } else if (a.rk_stage == 0) {
-                        stb(a.qrhs[m], od, ldb(a.qrhs[m], od) + dv);
-                    } else if (a.mini) {
-                        // rk3 of tgvsolver.F90:1040-1060: q = a qsave + b q + c dt qrhs
-                        const double r = ldb(a.qrhs[m], od) + dv;
-                        const double qo = ldb(a.q[m], od);
-                        double qs;
-                        if (a.rk_stage == 1) {
-                            qs = qo;
-                            stb(a.qsave[m], od, qs);
-                        } else {
-                            qs = ldb(a.qsave[m], od);
-                        }
-                        stb(a.q[m], od, a.rka * qs + a.rkb * qo + a.rkc * r * a.dt);
-                    } else {
-                        // last axis of the step: the RK3 combine of
-                        // src/mainloop.F90:423-445 for this quantity; the
-                        // workgroup owns these nodes of q_m
-                        const double r = ldb(a.qrhs[m], od) + o;
-                        const double jc = ldb(a.jac, od), qo = ldb(a.q[m], od);
-                        double qs;
-                        if (a.rk_stage == 1) {
-                            qs = qo * jc;
-                            stb(a.qsave[m], od, qs);
-                        } else {
-                            qs = ldb(a.qsave[m], od);
-                        }
-                        stb(a.q[m], od, (a.rka * qs + a.rkb * qo * jc + a.rkc * r * a.dt) / jc);
-                    }
+                    // x: the first axis sets qrhs
+                    stb(a.qrhs[m], od, a.mini ? o / a.h : o);
                 } else {
                     stb(a.fout, od, o);
                 }
@@ -339,6 +429,7 @@ struct PtArgs {
     double reynolds, prandtl, tempconst, tempconst1;
     double rka, rkb, rkc, dt;
     int stage;
+    int axis;               // k_qswap2d_line: the periodic in-plane axis
     double* part;
 };
 
@@ -351,7 +442,10 @@ __device__ __forceinline__ void interior_ijk(const Geo& g, long p, int& i, int& 
     k = (int)(r / nj);
 }
 
-// q2fvar_3da, src/fludyna.F90:545-634 (nondimensional, numq = 5)
+// q2fvar_3da, src/fludyna.F90:545-634 (nondimensional, numq = 5), evaluated
+// without FMA contraction (in the reference's operation order, as the
+// upwind checker does)
+#pragma clang fp contract(off)
 __device__ __forceinline__ void q2fvar(const PtArgs& a, const double qv[5], long gi)
 {
     const double rho = qv[0];
@@ -364,6 +458,7 @@ __device__ __forceinline__ void q2fvar(const PtArgs& a, const double qv[5], long
     a.prim[4][gi] = p;
     a.prim[5][gi] = p / rho * a.const2;
 }
+#pragma clang fp contract(fast)
 
 // miucal, src/fludyna.F90:791-820
 __device__ __forceinline__ double miucal(const PtArgs& a, double t)
@@ -380,35 +475,41 @@ __global__ void k_qswap_halo(PtArgs a)
 {
     const Geo& g = a.g;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    // face sizes: i-halo 2*HM*(n1+1)*(n2+1), j-halo 2*HM*(n0+1)*(n2+1), ...
+    // thread order: the k-halo and j-halo cells as rows of i (i fastest,
+    // so a wavefront reads and writes consecutive nodes), then the i-halo
+    // cells, 2*HM per (j,k)
+    const long ri = n0 + 1;
+    const long fk = 2L * HM * (n1 + 1) * ri;
+    const long fj = 2L * HM * (n2 + 1) * ri;
     const long fi = 2L * HM * (n1 + 1) * (n2 + 1);
-    const long fj = 2L * HM * (n0 + 1) * (n2 + 1);
-    const long fk = 2L * HM * (n0 + 1) * (n1 + 1);
     long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= fi + fj + fk) return;
-    int axis;
-    if (p < fi) axis = 0;
-    else if (p < fi + fj) { axis = 1; p -= fi; }
-    else { axis = 2; p -= fi + fj; }
-    // p -> (h in 0..2HM-1, u, v) with u fastest
-    const int nu = axis == 0 ? n1 + 1 : n0 + 1;
-    const int nv = axis == 2 ? n1 + 1 : n2 + 1;
-    const int h = (int)(p % (2 * HM));
-    const long r = p / (2 * HM);
-    const int uu = (int)(r % nu);
-    const int vv = (int)(r / nu);
-    (void)nv;
+    int axis, h, ci[3];
+    if (p < fk) {
+        axis = 2;
+        const long r = p / ri;
+        ci[0] = (int)(p - r * ri);
+        ci[1] = (int)(r % (n1 + 1));
+        h = (int)(r / (n1 + 1));
+    } else if ((p -= fk) < fj) {
+        axis = 1;
+        const long r = p / ri;
+        ci[0] = (int)(p - r * ri);
+        h = (int)(r % (2 * HM));
+        ci[2] = (int)(r / (2 * HM));
+    } else {
+        p -= fj;
+        axis = 0;
+        h = (int)(p % (2 * HM));
+        const long r = p / (2 * HM);
+        ci[1] = (int)(r % (n1 + 1));
+        ci[2] = (int)(r / (n1 + 1));
+    }
     const int dimh = g.n[axis];
     // halo node and its periodic source
-    const int hn = h < HM ? -(h + 1) : dimh + (h - HM + 1);
-    const int sn = h < HM ? dimh - (h + 1) : (h - HM + 1);
-    int ci[3];  // halo cell
-    int si[3];  // source cell
-    if (axis == 0) { ci[0] = hn; ci[1] = uu; ci[2] = vv; }
-    else if (axis == 1) { ci[0] = uu; ci[1] = hn; ci[2] = vv; }
-    else { ci[0] = uu; ci[1] = vv; ci[2] = hn; }
-    si[0] = ci[0]; si[1] = ci[1]; si[2] = ci[2];
-    si[axis] = sn;
+    ci[axis] = h < HM ? -(h + 1) : dimh + (h - HM + 1);
+    int si[3] = {ci[0], ci[1], ci[2]};  // source cell
+    si[axis] = h < HM ? dimh - (h + 1) : (h - HM + 1);
     double qv[5];
     // the source value after the averages of the axes before `axis`
     for (int m = 0; m < 5; ++m) {
@@ -490,14 +591,89 @@ __global__ void k_qswap_seam(PtArgs a)
             }
 }
 
-// sigma and qflux of diffrsdcal6 (src/solver.F90:2434-2546, turbmode none)
-// at one node: sg = (s11, s12, s13, s22, s23, s33), qf(1:3)
-__device__ __forceinline__ void sigma_at(const PtArgs& a, long gi, double* sg, double* qf)
+// qswap of a two-dimensional box (km = 0), single rank: along a periodic
+// in-plane axis (src/parallel.F90:4877-4915 for i, 5019-5065 for j) the
+// halo copy, the duplicate average q(0) = q(n) = (q(0)+q(n))/2 and q2fvar
+// on the strips -hm..0 and n..n+hm, one thread per line of the plane k = 0.
+__global__ void k_qswap2d_line(PtArgs a)
 {
+    const int d = a.axis, o = 1 - d;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > a.g.n[o]) return;
+    const int n = a.g.n[d];
+    const long h0 = gidx(a.g, d == 0 ? 0 : t, d == 0 ? t : 0, 0);
+    const long sd = a.g.s[d];
+    for (int m = 0; m < 5; ++m) {
+        double* f = a.q[m] + h0;
+        for (int r = 1; r <= HM; ++r) f[-r * sd] = f[(n - r) * sd];
+        for (int r = 1; r <= HM; ++r) f[(n + r) * sd] = f[r * sd];
+        const double avg = 0.5 * (f[0] + f[n * sd]);
+        f[0] = avg;
+        f[n * sd] = avg;
+    }
+    // the strips n..n+hm and -hm..0
+    for (int r = 0; r <= HM; ++r)
+        for (int e = 0; e < 2; ++e) {
+            const long gi = h0 + (e ? -r : n + r) * sd;
+            double qv[5];
+            for (int m = 0; m < 5; ++m) qv[m] = a.q[m][gi];
+            q2fvar(a, qv, gi);
+        }
+}
+
+// The k part of qswap on a two-dimensional box, the ka = 0 branch
+// (src/parallel.F90:5161-5200): q of the plane k = 0 is copied into every k
+// plane -hm..hm and q2fvar refreshes all of them.  Of the copies only the
+// pressure of the planes k = -1 and k = +1 is written: it is the one k-halo
+// value a two-dimensional kernel reads (the pressure stencil of the Ducros
+// sensor); the plane k = 0 gets q2fvar in full.
+__global__ void k_qswap2d_plane(PtArgs a)
+{
+    const long ni = a.g.n[0] + 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * (a.g.n[1] + 1)) return;
+    const long gi = gidx(a.g, (int)(t % ni), (int)(t / ni), 0);
+    double qv[5];
+    for (int m = 0; m < 5; ++m) qv[m] = a.q[m][gi];
+    q2fvar(a, qv, gi);
+    const double p = a.prim[4][gi];
+    a.prim[4][gi - a.g.s[2]] = p;
+    a.prim[4][gi + a.g.s[2]] = p;
+}
+
+// The inputs of G at one node, loaded once: every output store could alias
+// an input array, so loads interleaved with the stores would be repeated.
+struct NodeIn {
+    double q[5], u[3], p, jc;
+};
+
+__device__ __forceinline__ NodeIn load_node(const PtArgs& a, long gi)
+{
+    NodeIn v;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) v.q[m] = a.q[m][gi];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v.u[c] = a.prim[1 + c][gi];
+    v.p = a.prim[4][gi];
+    v.jc = a.mini ? 1.0 : a.jac[gi];
+    return v;
+}
+
+// sigma and qflux of diffrsdcal6 (src/solver.F90:2434-2546, turbmode none)
+// at one node from its gradients, temperature and velocity:
+// sg = (s11, s12, s13, s22, s23, s33), qf(1:3)
+__device__ __forceinline__ void sigma_of(const PtArgs& a, long gi, const double* uvw, double* sg,
+                                         double* qf)
+{
+    double dv[9], dt[3];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) dv[c] = a.dvel[c][gi];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dt[c] = a.dtmp[c][gi];
     const double miu = miucal(a, a.prim[5][gi]) / a.reynolds;
-    const double d11 = a.dvel[0][gi], d12 = a.dvel[1][gi], d13 = a.dvel[2][gi];
-    const double d21 = a.dvel[3][gi], d22 = a.dvel[4][gi], d23 = a.dvel[5][gi];
-    const double d31 = a.dvel[6][gi], d32 = a.dvel[7][gi], d33 = a.dvel[8][gi];
+    const double d11 = dv[0], d12 = dv[1], d13 = dv[2];
+    const double d21 = dv[3], d22 = dv[4], d23 = dv[5];
+    const double d31 = dv[6], d32 = dv[7], d33 = dv[8];
     const double s11 = d11, s12 = 0.5 * (d12 + d21), s13 = 0.5 * (d13 + d31);
     const double s22 = d22, s23 = 0.5 * (d23 + d32), s33 = d33;
     const double skk = (1.0 / 3.0) * (s11 + s22 + s33);
@@ -506,10 +682,10 @@ __device__ __forceinline__ void sigma_at(const PtArgs& a, long gi, double* sg, d
     const double g1 = miu2 * (s11 - skk), g2 = miu2 * s12, g3 = miu2 * s13;
     const double g4 = miu2 * (s22 - skk), g5 = miu2 * s23, g6 = miu2 * (s33 - skk);
     sg[0] = g1; sg[1] = g2; sg[2] = g3; sg[3] = g4; sg[4] = g5; sg[5] = g6;
-    const double u = a.prim[1][gi], v = a.prim[2][gi], w = a.prim[3][gi];
-    qf[0] = hcc * a.dtmp[0][gi] + g1 * u + g2 * v + g3 * w;
-    qf[1] = hcc * a.dtmp[1][gi] + g2 * u + g4 * v + g5 * w;
-    qf[2] = hcc * a.dtmp[2][gi] + g3 * u + g5 * v + g6 * w;
+    const double u = uvw[0], v = uvw[1], w = uvw[2];
+    qf[0] = hcc * dt[0] + g1 * u + g2 * v + g3 * w;
+    qf[1] = hcc * dt[1] + g2 * u + g4 * v + g5 * w;
+    qf[2] = hcc * dt[2] + g3 * u + g5 * v + g6 * w;
 }
 
 // The flux of axis d that the flux sweep differentiates, G = Fv - F, at one
@@ -520,48 +696,44 @@ __device__ __forceinline__ void sigma_at(const PtArgs& a, long gi, double* sg, d
 // The derivative is linear, so the reference's two passes (qrhs = -sum D F,
 // then qrhs += sum D Fv) become one derivative of Fv - F per axis; only the
 // summation order differs.
-__device__ __forceinline__ void gflux_at(const PtArgs& a, int d, long gi, double xn,
-                                         const double* sg, const double* qf)
+__device__ __forceinline__ void gflux_of(const PtArgs& a, int d, double xn, const NodeIn& n,
+                                         const double* sg, const double* qf, double* out)
 {
     static constexpr int sidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-    const double p = a.prim[4][gi];
+    const double p = n.p;
     if (a.mini) {
         // tgvsolver.F90:578-631,760-843: fcs = q_m u_d (+ p on the momentum
         // of the sweep axis), (q5 + p) u_d; ff = sigma(d,c), qflux(d); no
         // metrics (the derivative is divided by the spacing)
-        const double uu = a.prim[1 + d][gi];
+        const double uu = n.u[d];
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
-            const double qm = a.q[m][gi];
+            const double qm = n.q[m];
             double fc;
             if (m == 4) fc = (qm + p) * uu;
             else if (m == d + 1) fc = qm * uu + p;
             else fc = qm * uu;
-            double v;
-            if (m == 0 || !a.visc) v = -fc;
-            else v = (m < 4 ? sg[sidx[d][m - 1]] : qf[d]) - fc;
-            a.gfl[5 * d + m][gi] = v;
+            if (m == 0 || !a.visc) out[m] = -fc;
+            else out[m] = (m < 4 ? sg[sidx[d][m - 1]] : qf[d]) - fc;
         }
         return;
     }
-    const double jc = a.jac[gi];
-    const double uu = xn * a.prim[1 + d][gi];
+    const double jc = n.jc;
+    const double uu = xn * n.u[d];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        const double qm = a.q[m][gi];
+        const double qm = n.q[m];
         double fi;
         if (m == 0) fi = jc * qm * uu;
         else if (m == 4) fi = jc * (qm + p) * uu;
         else if (m == d + 1) fi = jc * (qm * uu + xn * p);
         else fi = jc * (qm * uu);
-        double v;
         if (m == 0 || !a.visc) {
-            v = -fi;
+            out[m] = -fi;
         } else {
             const double sv = m < 4 ? sg[sidx[d][m - 1]] : qf[d];
-            v = (sv * xn) * jc - fi;
+            out[m] = (sv * xn) * jc - fi;
         }
-        a.gfl[5 * d + m][gi] = v;
     }
 }
 
@@ -569,46 +741,53 @@ __device__ __forceinline__ void gflux_at(const PtArgs& a, int d, long gi, double
 // (the nodes -hm..-1 and n+1..n+hm of every d-line, where the flux sweep
 // also reads it): sigma and qflux there are those of the periodic image
 // (their halos are plain copies, src/solver.F90:2605-2607), the inviscid
-// part is that of the halo node itself.
-__global__ void k_gflux(PtArgs a)
+// part is that of the halo node itself.  One thread per node of the padded
+// box, i fastest (a 2-D grid: x over the (i,j) plane, y over k), so every
+// load and store of a wavefront is a run of consecutive i; nodes outside
+// the interior along two or more axes have nothing to do.  All inputs are
+// loaded before the first store.
+__global__ __launch_bounds__(256) void k_gflux(PtArgs a)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1, nk = a.g.n[2] + 1;
-    const long nin = ni * nj * nk;
-    int c[3];
-    int d = -1;  // -1: interior node (all axes)
-    if (p < nin) {
-        c[0] = (int)(p % ni);
-        c[1] = (int)((p / ni) % nj);
-        c[2] = (int)(p / (ni * nj));
-    } else {
-        long r = p - nin;
-        const long s0 = 2L * HM * nj * nk, s1 = 2L * HM * ni * nk, s2 = 2L * HM * ni * nj;
-        long ext[3] = {ni, nj, nk};
-        if (r < s0) d = 0;
-        else if ((r -= s0) < s1) d = 1;
-        else if ((r -= s1) < s2) d = 2;
-        else return;
-        // r -> (h in 0..2hm-1, the two other axes)
-        const int h = (int)(r % (2 * HM));
-        r /= 2 * HM;
-        const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
-        c[oa] = (int)(r % ext[oa]);
-        c[ob] = (int)(r / ext[oa]);
-        c[d] = h < HM ? -(h + 1) : a.g.n[d] + (h - HM + 1);
-    }
+    const unsigned pw = (unsigned)(a.g.n[0] + 1 + 2 * HM);
+    const unsigned ph = (unsigned)(a.g.n[1] + 1 + 2 * HM);
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= pw * ph) return;
+    const unsigned jj = t / pw;
+    int c[3] = {(int)(t - jj * pw) - HM, (int)jj - HM, (int)blockIdx.y - HM};
+    int d = -1, nout = 0;  // d = -1: interior node (all axes)
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+        if (c[e] < 0 || c[e] > a.g.n[e]) {
+            d = e;
+            ++nout;
+        }
+    if (nout > 1) return;
     const long gi = gidx(a.g, c[0], c[1], c[2]);
+    const NodeIn nd = load_node(a, gi);
     double sg[6] = {0, 0, 0, 0, 0, 0}, qf[3] = {0, 0, 0};
     if (d < 0) {
-        if (a.visc) sigma_at(a, gi, sg, qf);
-        for (int e = 0; e < 3; ++e) gflux_at(a, e, gi, a.xi[e][c[e] + HM], sg, qf);
+        if (a.visc) sigma_of(a, gi, nd.u, sg, qf);
+        double out[15];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) gflux_of(a, e, a.xi[e][c[e] + HM], nd, sg, qf, out + 5 * e);
+#pragma unroll
+        for (int k = 0; k < 15; ++k) a.gfl[k][gi] = out[k];
     } else {
         if (a.visc) {
             int w[3] = {c[0], c[1], c[2]};
             w[d] = wrap_node(c[d], a.g.n[d]);
-            sigma_at(a, gidx(a.g, w[0], w[1], w[2]), sg, qf);
+            const long gw = gidx(a.g, w[0], w[1], w[2]);
+            const double uw[3] = {a.prim[1][gw], a.prim[2][gw], a.prim[3][gw]};
+            sigma_of(a, gw, uw, sg, qf);
         }
-        gflux_at(a, d, gi, a.xi[d][c[d] + HM], sg, qf);
+        // a compile-time axis keeps sg and qf in registers
+        double out[5];
+        const double xn = a.xi[d][c[d] + HM];
+        if (d == 0) gflux_of(a, 0, xn, nd, sg, qf, out);
+        else if (d == 1) gflux_of(a, 1, xn, nd, sg, qf, out);
+        else gflux_of(a, 2, xn, nd, sg, qf, out);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) a.gfl[5 * d + m][gi] = out[m];
     }
 }
 
@@ -652,7 +831,9 @@ __global__ void k_bchomo(PtArgs a)
 }
 
 // RK3 combine (src/mainloop.F90:423-445) and updatefvar (src/fludyna.F90:
-// 191-242) on the interior.
+// 191-242) on the interior, in the reference's operation order (no FMA
+// contraction).
+#pragma clang fp contract(off)
 __global__ void k_rk(PtArgs a)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -662,23 +843,24 @@ __global__ void k_rk(PtArgs a)
     interior_ijk(a.g, p, i, j, k);
     const long gi = gidx(a.g, i, j, k);
     const double jc = a.jac[gi];
-    double qv[5];
+    // every operand is read before the first store (which could alias it)
+    double qo[5], qs[5], qr[5], qv[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        const double qo = a.q[m][gi];
-        double qs;
-        if (a.stage == 1) {
-            qs = qo * jc;
-            a.qsave[m][gi] = qs;
-        } else {
-            qs = a.qsave[m][gi];
-        }
-        const double vn = a.rka * qs + a.rkb * qo * jc + a.rkc * a.qrhs[m][gi] * a.dt;
+        qo[m] = a.q[m][gi];
+        qr[m] = a.qrhs[m][gi];
+        qs[m] = a.stage == 1 ? qo[m] * jc : a.qsave[m][gi];
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        if (a.stage == 1) a.qsave[m][gi] = qs[m];
+        const double vn = a.rka * qs[m] + a.rkb * qo[m] * jc + a.rkc * qr[m] * a.dt;
         qv[m] = vn / jc;
         a.q[m][gi] = qv[m];
     }
     q2fvar(a, qv, gi);
 }
+#pragma clang fp contract(fast)
 
 // updatefvar (src/fludyna.F90:191-242) on the interior, once the last flux
 // sweep of the stage has applied the RK3 combine to q.
@@ -774,8 +956,17 @@ struct SensorArgs {
     unsigned char* lshock;
 };
 
+// A bounded low / high end of an axis: ntype 1 / 2, and both ends of ntype
+// 4.  The reference assigns ntype 4 only to a single rank along a
+// non-periodic axis, where it leaves is/ie unset (src/parallel.F90:
+// 1070-1078); on several ranks, the decomposition it runs bounded axes on,
+// the physical ends have ntype 1 and 2.  ntype 4 here is their union.
+__host__ __device__ __forceinline__ bool lo_end(int nt) { return nt == 1 || nt == 4; }
+__host__ __device__ __forceinline__ bool hi_end(int nt) { return nt == 2 || nt == 4; }
+
 // ducrossensor, part 1 (src/commcal.F90:236-272): ssf on 0..n; the
-// pressure stencil is clamped at ntype 1 (low end) and 2 (high end) only
+// pressure stencil is clamped at a bounded end (ntype 1 and 2 in the
+// reference, both ends of ntype 4)
 __global__ void k_ducros_ssf(SensorArgs a)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -791,12 +982,12 @@ __global__ void k_ducros_ssf(SensorArgs a)
     const double vz = a.dvel[3][gi] - a.dvel[1][gi];  // dvel(2,1) - dvel(1,2)
     const double vort = vx * vx + vy * vy + vz * vz;
     int ip1 = i + 1, im1 = i - 1, jp1 = j + 1, jm1 = j - 1, kp1 = k + 1, km1 = k - 1;
-    if (a.ntype[0] == 1 && im1 < 0) im1 = 0;
-    if (a.ntype[1] == 1 && jm1 < 0) jm1 = 0;
-    if (a.ntype[2] == 1 && km1 < 0) km1 = 0;
-    if (a.ntype[0] == 2 && ip1 > a.g.n[0]) ip1 = a.g.n[0];
-    if (a.ntype[1] == 2 && jp1 > a.g.n[1]) jp1 = a.g.n[1];
-    if (a.ntype[2] == 2 && kp1 > a.g.n[2]) kp1 = a.g.n[2];
+    if (lo_end(a.ntype[0]) && im1 < 0) im1 = 0;
+    if (lo_end(a.ntype[1]) && jm1 < 0) jm1 = 0;
+    if (lo_end(a.ntype[2]) && km1 < 0) km1 = 0;
+    if (hi_end(a.ntype[0]) && ip1 > a.g.n[0]) ip1 = a.g.n[0];
+    if (hi_end(a.ntype[1]) && jp1 > a.g.n[1]) jp1 = a.g.n[1];
+    if (hi_end(a.ntype[2]) && kp1 > a.g.n[2]) kp1 = a.g.n[2];
     const double pc = a.prs[gi];
     const double* P = a.prs;
     const double pi1 = P[gidx(a.g, ip1, j, k)], pi0 = P[gidx(a.g, im1, j, k)];
@@ -811,13 +1002,17 @@ __global__ void k_ducros_ssf(SensorArgs a)
 // dataswap(ssf) (src/commcal.F90:276; src/parallel.F90:4170-4175): a halo
 // cell takes the interior value at its periodically wrapped coordinates;
 // halo cells of non-periodic axes are zero (no neighbour on one domain).
-// Reads the interior, writes halo cells only.
+// Reads the interior, writes halo cells only.  A two-dimensional box needs
+// the plane k = 0 only: the window max skips the k axis there (the k halo
+// planes would repeat the plane itself, which the window already holds).
 __global__ void k_ssf_halo(SensorArgs a)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ni = a.g.n[0] + 1 + 2 * HM, nj = a.g.n[1] + 1 + 2 * HM;
-    if (p >= ni * nj * (a.g.n[2] + 1 + 2 * HM)) return;
-    int c[3] = {(int)(p % ni) - HM, (int)((p / ni) % nj) - HM, (int)(p / (ni * nj)) - HM};
+    const long nk = a.ndims == 2 ? 1 : a.g.n[2] + 1 + 2 * HM;
+    if (p >= ni * nj * nk) return;
+    int c[3] = {(int)(p % ni) - HM, (int)((p / ni) % nj) - HM,
+                a.ndims == 2 ? 0 : (int)(p / (ni * nj)) - HM};
     bool halo = false, keep = true;
     int s[3];
     for (int d = 0; d < 3; ++d) {
@@ -832,7 +1027,7 @@ __global__ void k_ssf_halo(SensorArgs a)
 }
 
 // ducrossensor, part 2 (src/commcal.F90:278-316): lshock = max of ssf over
-// offsets -hm+1..hm along each axis > shkcrt
+// offsets -hm+1..hm along each axis > shkcrt, clamped at bounded ends
 __global__ void k_ducros_flag(SensorArgs a)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -840,12 +1035,12 @@ __global__ void k_ducros_flag(SensorArgs a)
     if (p >= ni * nj * (a.g.n[2] + 1)) return;
     const int c0[3] = {(int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj))};
     double m = 0.0;
-    for (int d = 0; d < 3; ++d)
+    for (int d = 0; d < a.ndims; ++d)
         for (int o = -HM + 1; o <= HM; ++o) {
             int c[3] = {c0[0], c0[1], c0[2]};
             int t = c[d] + o;
-            if (a.ntype[d] == 1 && t < 0) t = 0;
-            if (a.ntype[d] == 2 && t > a.g.n[d]) t = a.g.n[d];
+            if (lo_end(a.ntype[d]) && t < 0) t = 0;
+            if (hi_end(a.ntype[d]) && t > a.g.n[d]) t = a.g.n[d];
             c[d] = t;
             m = uwd::max2(m, a.ssf[gidx(a.g, c[0], c[1], c[2])]);
         }
@@ -1182,7 +1377,12 @@ __global__ __launch_bounds__(256) void k_uwd(UwdArgs a)
                 const double v2 = uwd::mplimiter(fm, cml, lsh, ii, dim, a.ntype);
                 fhc[m] = v1 + v2;
             }
-        } else
+        } else {
+        // recons_exp sees ntype 4 as ntype 1 on the low half of the line
+        // and ntype 2 on the high half: what the reference computes with
+        // the axis split over several ranks (the explicit flux has no
+        // closure at a rank interface)
+        const int nte = a.ntype == 4 ? (ii < dim / 2 ? 1 : 2) : a.ntype;
         for (int m = 0; m < 5; ++m) {
             double fp[8], fm[8];
             if (chr) {
@@ -1202,9 +1402,10 @@ __global__ __launch_bounds__(256) void k_uwd(UwdArgs a)
                     fm[w] = base[(s + 7 - w) * 10 + 5 + m];
                 }
             }
-            const double v1 = uwd::recons_exp(fp, ii, dim, a.ntype, a.recon, lsh, a.bfac);
-            const double v2 = uwd::recons_exp(fm, ii, dim, a.ntype, a.recon, lsh, a.bfac);
+            const double v1 = uwd::recons_exp(fp, ii, dim, nte, a.recon, lsh, a.bfac);
+            const double v2 = uwd::recons_exp(fm, ii, dim, nte, a.recon, lsh, a.bfac);
             fhc[m] = v1 + v2;
+        }
         }
         double* F = fh + (l * S + s) * 5;
         if (chr) {
@@ -1228,12 +1429,138 @@ __global__ __launch_bounds__(256) void k_uwd(UwdArgs a)
         const long gi = gidx(a.g, i, j, k);
         const double* F1 = fh + (l * S + s) * 5;
         const double* F0 = F1 - 5;
+        double qr[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) qr[m] = a.qrhs[m][gi];
+#pragma unroll
         for (int m = 0; m < 5; ++m) {
-            const double v = a.qrhs[m][gi] + F1[m] - F0[m];
+            const double v = qr[m] + F1[m] - F0[m];
             a.qrhs[m][gi] = a.last ? -v : v;
         }
     }
 }
+
+// ---- user boundary condition of the 2-D Riemann example
+
+struct BcArgs {
+    Geo g;
+    double* q[5];
+    double* prim[6];
+    int face[4];    // faces 1..4 (i=0, i=im, j=0, j=jm) under the user condition
+    double time;    // the step's start time
+    double gamma, mach, const2, const6;
+};
+
+// One boundary node as udf_bc sets it: the primitives, tmp = thermal(
+// pressure, density) = p/rho*const2 and q from fvar2q_sca's pressure form,
+// q5 = p*const6 + rho*(0.5*sum(u*u)) (src/fludyna.F90:45-70, 310-362).
+__device__ __forceinline__ void bc_node(const BcArgs& a, long gi, double r, double u, double v,
+                                        double w, double p)
+{
+    a.prim[0][gi] = r;
+    a.prim[1][gi] = u;
+    a.prim[2][gi] = v;
+    a.prim[3][gi] = w;
+    a.prim[4][gi] = p;
+    a.prim[5][gi] = p / r * a.const2;
+    a.q[0][gi] = r;
+    a.q[1][gi] = r * u;
+    a.q[2][gi] = r * v;
+    a.q[3][gi] = r * w;
+    const double var1 = 0.5 * (u * u + v * v + w * w);
+    a.q[4][gi] = p * a.const6 + r * var1;
+}
+
+// shockspeed, examples/Riemann2D/user_define_module/userdefine_riemann2d.F90:
+// 670-686: the Rankine-Hugoniot speed of the shock p1 -> p2 into a gas at
+// temperature t1, sqrt(1 + (p2/p1-1)(g+1)/(2g)) sos(t1), sos = sqrt(T)/Ma.
+__device__ __forceinline__ double shockspeed(const BcArgs& a, double p1, double t1, double p2)
+{
+    const double mx2 = (p2 / p1 - 1.0) * (a.gamma + 1.0) / (2.0 * a.gamma) + 1.0;
+    return sqrt(mx2) * (sqrt(t1) / a.mach);
+}
+
+// the quadrant states of the example (rho, u, v, p): NE 11, NW 01, SW 00,
+// SE 10; the NW pressure literal is single precision (0.30) in the source
+#define R2D_STATES                                                    \
+    const double r01 = 0.5323, r11 = 1.5, u01 = 1.206, v01 = 0.0;     \
+    const double v11 = 0.0, p01 = (double)0.30f, p11 = 1.5;           \
+    const double r00 = 0.138, r10 = 0.5323, u00 = 1.206, v00 = 1.206; \
+    const double u10 = 0.0, v10 = 1.206, p00 = 0.029, p10 = 0.3;
+
+// udf_bc faces 1 and 2 (userdefine_riemann2d.F90:446-541), one thread per
+// (j,k): face 1 (i=0) takes the states behind and ahead of the shock that
+// moves down from y = 0.8 at vss, with u = 1.206; face 2 (i=im) extrapolates
+// rho, p and v as (4f(im-1) - f(im-2))/3 with u = w = 0, and the corner
+// (im,jm) is the north-east state at rest.  The grid is gridcube(1,1,0)'s:
+// x = i/im, y = j/jm (src/gridgeneration.F90:233-267).
+__global__ void k_bc_riem2d_i(BcArgs a)
+{
+    const long nj = a.g.n[1] + 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nj * (a.g.n[2] + 1)) return;
+    const int j = (int)(t % nj), k = (int)(t / nj);
+    const int im = a.g.n[0], jm = a.g.n[1];
+    R2D_STATES
+    (void)u10; (void)v10; (void)p10; (void)r10;
+    if (a.face[0]) {
+        const double t1 = p01 / r01 * a.const2;
+        const double vss = shockspeed(a, p01, t1, p00);
+        const double ys = 0.8 - vss * a.time;
+        const bool up = (1.0 / (double)jm) * (double)j > ys;
+        bc_node(a, gidx(a.g, 0, j, k), up ? r01 : r00, u01, up ? v01 : v00, 0.0, up ? p01 : p00);
+    }
+    if (a.face[1]) {
+        const long gd = gidx(a.g, im, j, k);
+        if (j == jm) {
+            bc_node(a, gd, r11, 0.0, v11, 0.0, p11);
+        } else {
+            const long g1 = gidx(a.g, im - 1, j, k), g2 = gidx(a.g, im - 2, j, k);
+            const double n1d3 = 1.0 / 3.0;
+            const double r = n1d3 * (4.0 * a.prim[0][g1] - a.prim[0][g2]);
+            const double p = n1d3 * (4.0 * a.prim[4][g1] - a.prim[4][g2]);
+            const double v = n1d3 * (4.0 * a.prim[2][g1] - a.prim[2][g2]);
+            bc_node(a, gd, r, 0.0, v, 0.0, p);
+        }
+    }
+}
+
+// udf_bc faces 3 and 4 (userdefine_riemann2d.F90:543-666), one thread per
+// (i,k), after faces 1 and 2: face 3 (j=0) takes the states either side of
+// the shock that moves left from x = 0.8, with v = 1.206; face 4 (j=jm)
+// extrapolates rho, p and u along j with v = w = 0 (its nodes i = 0 and
+// i = im read what faces 1 and 2 wrote), and the corner again.
+__global__ void k_bc_riem2d_j(BcArgs a)
+{
+    const long ni = a.g.n[0] + 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * (a.g.n[2] + 1)) return;
+    const int i = (int)(t % ni), k = (int)(t / ni);
+    const int im = a.g.n[0], jm = a.g.n[1];
+    R2D_STATES
+    (void)u01; (void)v01; (void)p01; (void)r01;
+    if (a.face[2]) {
+        const double t1 = p10 / r10 * a.const2;
+        const double vss = shockspeed(a, p10, t1, p00);
+        const double xs = 0.8 - vss * a.time;
+        const bool up = (1.0 / (double)im) * (double)i > xs;
+        bc_node(a, gidx(a.g, i, 0, k), up ? r10 : r00, up ? u10 : u00, v10, 0.0, up ? p10 : p00);
+    }
+    if (a.face[3]) {
+        const long gd = gidx(a.g, i, jm, k);
+        if (i == im) {
+            bc_node(a, gd, r11, 0.0, 0.0, 0.0, p11);
+        } else {
+            const long g1 = gidx(a.g, i, jm - 1, k), g2 = gidx(a.g, i, jm - 2, k);
+            const double n1d3 = 1.0 / 3.0;
+            const double r = n1d3 * (4.0 * a.prim[0][g1] - a.prim[0][g2]);
+            const double p = n1d3 * (4.0 * a.prim[4][g1] - a.prim[4][g2]);
+            const double u = n1d3 * (4.0 * a.prim[1][g1] - a.prim[1][g2]);
+            bc_node(a, gd, r, u, 0.0, 0.0, p);
+        }
+    }
+}
+#undef R2D_STATES
 
 // back to the default contraction of the rest of the library
 #pragma clang fp contract(fast)
@@ -1282,6 +1609,9 @@ struct astr_ctx {
     std::vector<double> host_part;
     double const1, const2, const3, const4, const5, const6, const7, tempconst, tempconst1;
     bool have_geom = false, have_state = false;
+    int bctype[6] = {1, 1, 1, 1, 1, 1};  // boucon per face: 1 nothing done, 0 user
+    int udf = 0;                          // the user condition (ASTR_UDF_*), 0 none
+    double time = 0.0;                    // simulation time (commvar time)
     bool timing = false;
     hipEvent_t ev[16] = {};
     double tms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -1557,11 +1887,52 @@ int do_qswap(astr_ctx* c)
 {
     PtArgs a = pt_args(c);
     const int n0 = c->g.n[0], n1 = c->g.n[1], n2 = c->g.n[2];
+    if (n2 == 0) {
+        // two-dimensional box: periodic in-plane axes, then the k part
+        const int nts[2] = {c->p.npdci, c->p.npdcj};
+        for (int d = 0; d < 2; ++d) {
+            if (nts[d] != 3) continue;
+            a.axis = d;
+            TRY(launch_pt(c, k_qswap2d_line, (long)c->g.n[1 - d] + 1, a));
+        }
+        return launch_pt(c, k_qswap2d_plane, (long)(n0 + 1) * (n1 + 1), a);
+    }
     const long nh = 2L * HM * ((long)(n1 + 1) * (n2 + 1) + (long)(n0 + 1) * (n2 + 1) +
                                (long)(n0 + 1) * (n1 + 1));
     TRY(launch_pt(c, k_qswap_halo, nh, a));
     const long ns = (long)n0 * n1 * n2 - (long)(n0 - 1) * (n1 - 1) * (n2 - 1);
     TRY(launch_pt(c, k_qswap_seam, ns, a));
+    return ASTR_OK;
+}
+
+// boucon (src/bc.F90:327-411) for the faces under the user condition of
+// the 2-D Riemann example, at the context's time
+int do_boucon(astr_ctx* c)
+{
+    if (c->udf != ASTR_UDF_RIEMANN2D) return ASTR_OK;
+    BcArgs a{};
+    a.g = c->g;
+    for (int m = 0; m < 5; ++m) a.q[m] = c->q[m];
+    for (int m = 0; m < 6; ++m) a.prim[m] = c->prim[m];
+    for (int f = 0; f < 4; ++f) a.face[f] = c->bctype[f] == 0;
+    a.time = c->time;
+    a.gamma = c->p.gamma;
+    a.mach = c->p.mach;
+    a.const2 = c->const2;
+    a.const6 = c->const6;
+    const long nk = c->g.n[2] + 1;
+    if (a.face[0] || a.face[1]) {
+        const long n = (long)(c->g.n[1] + 1) * nk;
+        hipLaunchKernelGGL(k_bc_riem2d_i, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           c->stream, a);
+        HIPCHK(hipGetLastError());
+    }
+    if (a.face[2] || a.face[3]) {
+        const long n = (long)(c->g.n[0] + 1) * nk;
+        hipLaunchKernelGGL(k_bc_riem2d_j, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           c->stream, a);
+        HIPCHK(hipGetLastError());
+    }
     return ASTR_OK;
 }
 
@@ -1591,9 +1962,11 @@ int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt)
 // strips the flux sweeps read)
 int do_gflux(astr_ctx* c)
 {
-    const long ni = c->g.n[0] + 1, nj = c->g.n[1] + 1, nk = c->g.n[2] + 1;
-    const long n = ni * nj * nk + 2L * HM * (nj * nk + ni * nk + ni * nj);
-    return launch_pt(c, k_gflux, n, pt_args(c));
+    const long plane = (long)(c->g.n[0] + 1 + 2 * HM) * (c->g.n[1] + 1 + 2 * HM);
+    const dim3 grid((unsigned)((plane + 255) / 256), (unsigned)(c->g.n[2] + 1 + 2 * HM));
+    hipLaunchKernelGGL(k_gflux, grid, dim3(256), 0, c->stream, pt_args(c));
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
 }
 
 int ndims_of(const astr_ctx* c) { return c->g.n[2] == 0 ? 2 : 3; }
@@ -1647,7 +2020,9 @@ int do_sensor(astr_ctx* c)
     a.prs = c->prim[4];
     a.ssf = c->ssf;
     a.lshock = c->lshock;
-    const long n = n_interior(c), nh = n_haloed(c);
+    const long n = n_interior(c);
+    const long nh = a.ndims == 2 ? (long)(c->g.n[0] + 1 + 2 * HM) * (c->g.n[1] + 1 + 2 * HM)
+                                 : n_haloed(c);
     hipLaunchKernelGGL(k_ducros_ssf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a);
     hipLaunchKernelGGL(k_ssf_halo, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, c->stream, a);
     hipLaunchKernelGGL(k_ducros_flag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a);
@@ -1879,8 +2254,9 @@ int time_mark(astr_ctx* c, int slot)
 
 int do_step(astr_ctx* c, double dt, double* stats, bool timed)
 {
-    // timing families: 0 filter, 1 qswap, 2 grad, 3 sigma, 4 stats,
-    // 5 flux sweeps (central: with the RK3 combine; upwind: sensor and
+    // timing families: 0 filter, 1 boucon and qswap, 2 grad, 3 central:
+    // sigma and the axis fluxes (k_gflux), upwind: the shock sensor,
+    // 4 stats, 5 flux sweeps (central: with the RK3 combine; upwind: the
     // interface fluxes), 6 update (central: primitives; upwind: RK3 combine
     // and primitives)
     const bool upwind = c->p.convection != 0;
@@ -1916,23 +2292,27 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
                 }
             }
         }
+        c->time = c->time + dt;
         return ASTR_OK;
     }
     for (int s = 1; s <= 3; ++s) {
         if (timed) TRY(time_mark(c, 0));
         if (c->p.lfilter) TRY(do_filterq(c));
         if (timed) TRY(time_mark(c, 1));
+        // boucon before qswap (src/mainloop.F90:413-415), at the step's
+        // start time in every stage
+        TRY(do_boucon(c));
         TRY(do_qswap(c));
         if (timed) TRY(time_mark(c, 2));
         TRY(do_grad(c));
         if (timed) TRY(time_mark(c, 3));
         if (!upwind) TRY(do_gflux(c));
+        else if (sensor_on(c)) TRY(do_sensor(c));
         if (timed) TRY(time_mark(c, 4));
         // rkfirst's statcal sees the stage-1 state and its gradients
         if (s == 1 && stats) TRY(do_stats(c, stats));
         if (timed) TRY(time_mark(c, 5));
         if (upwind) {
-            if (sensor_on(c)) TRY(do_sensor(c));
             TRY(do_upwind(c));
         } else {
             TRY(do_flux_sweeps(c, s, dt));
@@ -1951,6 +2331,7 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
             }
         }
     }
+    c->time = c->time + dt;  // time=time+deltat, src/mainloop.F90:200
     return ASTR_OK;
 }
 
@@ -2014,8 +2395,6 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
             return fail(ASTR_E_ARG, "recon_schem must be -1, 0, 1, 2, 3, 5 or 6");
         if (p->diffterm)
             return fail(ASTR_E_UNSUPPORTED, "the upwind path is built for inviscid runs (diffterm 0)");
-        if (nts[1] == 4 || (!two_d && nts[2] == 4))
-            return fail(ASTR_E_UNSUPPORTED, "convrsduwd takes npdc 1..3 on the j and k axes");
     }
     if (two_d && (p->convection == 0 || p->deriv_explicit != 1 || nts[2] != 3))
         return fail(ASTR_E_UNSUPPORTED,
@@ -2276,17 +2655,35 @@ static int ready(astr_ctx* c)
     TRY(ready_state(c));
     if (c->p.variant == 1 && !c->have_h)
         return fail(ASTR_E_STATE, "the mini-app variant needs astr_set_spacing first");
-    if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "two-dimensional boxes support astr_rhscal only");
+    if (c->g.n[2] == 0) {
+        // two-dimensional boxes (upwind convection, explicit gradient):
+        // any closure on i and j, boundary conditions by astr_set_bc
+        if (c->p.lfilter)
+            return fail(ASTR_E_UNSUPPORTED, "two-dimensional boxes step without the filter (lfilter 0)");
+        return ASTR_OK;
+    }
     for (int d = 0; d < 3; ++d) {
         const int nt = d == 0 ? c->p.npdci : d == 1 ? c->p.npdcj : c->p.npdck;
-        if (nt != 3) return fail(ASTR_E_UNSUPPORTED, "the solver steps periodic boxes only");
+        if (nt != 3)
+            return fail(ASTR_E_UNSUPPORTED, "three-dimensional boxes step with periodic axes only");
     }
+    return ASTR_OK;
+}
+
+// batch members: three-dimensional periodic boxes without a time-dependent
+// boundary condition (a captured step replays the same launches)
+static int ready_batch(astr_ctx* c)
+{
+    TRY(ready(c));
+    if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "batched members are three-dimensional boxes");
+    if (c->udf) return fail(ASTR_E_UNSUPPORTED, "batched members take no user boundary condition");
     return ASTR_OK;
 }
 
 int astr_filterq(astr_ctx* c)
 {
     TRY(ready(c));
+    if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "the filter runs on three-dimensional boxes");
     if (c->p.variant == 1) return fail(ASTR_E_UNSUPPORTED, "the mini-app variant runs whole steps only");
     return do_filterq(c);
 }
@@ -2334,12 +2731,15 @@ int astr_stats(astr_ctx* c, double* out3)
 {
     TRY(ready(c));
     if (!out3) return fail(ASTR_E_ARG, "null argument");
+    if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "statistics of three-dimensional boxes only");
     return do_stats(c, out3);
 }
 
 int astr_rk3_step(astr_ctx* c, double dt, double* stats)
 {
     TRY(ready(c));
+    if (stats && c->g.n[2] == 0)
+        return fail(ASTR_E_UNSUPPORTED, "statistics of three-dimensional boxes only");
     TRY(do_step(c, dt, stats, c->timing));
     HIPCHK(hipStreamSynchronize(c->stream));
     return ASTR_OK;
@@ -2349,6 +2749,8 @@ int astr_rk3_steps(astr_ctx* c, int32_t nsteps, double dt, double* stats)
 {
     TRY(ready(c));
     if (nsteps < 0) return fail(ASTR_E_ARG, "nsteps < 0");
+    if (stats && c->g.n[2] == 0)
+        return fail(ASTR_E_UNSUPPORTED, "statistics of three-dimensional boxes only");
     for (int f = 0; f < 9; ++f) c->tms[f] = 0.0;
     for (int s = 0; s < nsteps; ++s) TRY(do_step(c, dt, stats ? stats + 3 * s : nullptr, c->timing));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -2410,7 +2812,7 @@ int astr_batch_create(astr_ctx* const* ctxs, int32_t n, astr_batch** out)
     if (!ctxs || !out || n < 1) return fail(ASTR_E_ARG, "bad argument");
     *out = nullptr;
     for (int i = 0; i < n; ++i) {
-        TRY(ready(ctxs[i]));
+        TRY(ready_batch(ctxs[i]));
         if (ctxs[i]->device != ctxs[0]->device)
             return fail(ASTR_E_ARG, "batch members must share one device");
         for (int j = 0; j < i; ++j)
@@ -2458,7 +2860,9 @@ static int batch_capture(astr_batch* b, double dt)
             rc = fail(ASTR_E_HIP, "fork wait failed");
             break;
         }
+        const double t0 = c->time;  // the replays advance it, not the capture
         rc = do_step(c, dt, nullptr, false);
+        c->time = t0;
         if (rc == ASTR_OK && (hipEventRecord(b->join[i], c->stream) != hipSuccess ||
                               hipStreamWaitEvent(b->root, b->join[i], 0) != hipSuccess))
             rc = fail(ASTR_E_HIP, "join failed");
@@ -2501,6 +2905,8 @@ int astr_batch_rk3_steps(astr_batch* b, int32_t nsteps, double dt)
         for (astr_ctx* c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));
     for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(b->exec, b->root));
     HIPCHK(hipStreamSynchronize(b->root));
+    for (astr_ctx* c : b->ctx)
+        for (int s = 0; s < nsteps; ++s) c->time = c->time + dt;
     return ASTR_OK;
 }
 
@@ -2513,6 +2919,47 @@ int astr_set_spacing(astr_ctx* c, double dx, double dy, double dz)
     c->h[2] = dz;
     c->have_h = true;
     return ASTR_OK;
+}
+
+int astr_set_bc(astr_ctx* c, const int32_t* bctype, int32_t udf)
+{
+    if (!c || !bctype) return fail(ASTR_E_ARG, "null argument");
+    bool any = false;
+    for (int f = 0; f < 6; ++f) {
+        if (bctype[f] != 0 && bctype[f] != 1)
+            return fail(ASTR_E_UNSUPPORTED,
+                        "bctype 1 (nothing done) and 0 (user condition) are built; the inflow, "
+                        "outflow, wall and NSCBC conditions of src/bc.F90 are not");
+        any = any || bctype[f] == 0;
+    }
+    if (udf != 0 && udf != ASTR_UDF_RIEMANN2D)
+        return fail(ASTR_E_ARG, "udf must be 0 (none) or ASTR_UDF_RIEMANN2D");
+    if (any && udf == 0) return fail(ASTR_E_ARG, "bctype 0 needs a user condition (udf)");
+    if (c->p.variant == 1) return fail(ASTR_E_UNSUPPORTED, "the mini-app variant has its own halos");
+    for (int f = 0; f < 6; ++f) c->bctype[f] = bctype[f];
+    c->udf = any ? udf : 0;
+    return ASTR_OK;
+}
+
+int astr_set_time(astr_ctx* c, double time)
+{
+    if (!c) return fail(ASTR_E_ARG, "null context");
+    c->time = time;
+    return ASTR_OK;
+}
+
+int astr_get_time(astr_ctx* c, double* time)
+{
+    if (!c || !time) return fail(ASTR_E_ARG, "null argument");
+    *time = c->time;
+    return ASTR_OK;
+}
+
+int astr_boucon(astr_ctx* c)
+{
+    TRY(ready(c));
+    if (c->p.variant == 1) return fail(ASTR_E_UNSUPPORTED, "the mini-app variant runs whole steps only");
+    return do_boucon(c);
 }
 
 int astr_synchronize(astr_ctx* c)

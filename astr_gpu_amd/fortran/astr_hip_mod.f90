@@ -24,6 +24,9 @@ module astr_hip
     real(c_double) :: shkcrt, bfacmpld
   end type astr_params
 
+  ! astr_set_bc: udf_bc of the 2-D Riemann example
+  integer(c_int32_t), parameter :: ASTR_UDF_RIEMANN2D = 1
+
   interface
 
     subroutine astr_default_params(p, n) bind(c, name='astr_default_params')
@@ -176,6 +179,30 @@ module astr_hip
       type(c_ptr), value :: ctx
       real(c_double), value :: dx, dy, dz
     end function astr_set_spacing
+
+    integer(c_int) function astr_set_bc(ctx, bctype, udf) bind(c, name='astr_set_bc')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(in) :: bctype(6)
+      integer(c_int32_t), value :: udf
+    end function astr_set_bc
+
+    integer(c_int) function astr_boucon(ctx) bind(c, name='astr_boucon')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function astr_boucon
+
+    integer(c_int) function astr_set_time(ctx, time) bind(c, name='astr_set_time')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: time
+    end function astr_set_time
+
+    integer(c_int) function astr_get_time(ctx, time) bind(c, name='astr_get_time')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: time
+    end function astr_get_time
 
     integer(c_int) function astr_synchronize(ctx) bind(c, name='astr_synchronize')
       import :: c_int, c_ptr

@@ -19,6 +19,7 @@ from . import _lib
 from ._lib import AstrParams, check, dptr
 
 HM = 5
+UDF_RIEMANN2D = 1  # ASTR_UDF_RIEMANN2D: udf_bc of the 2-D Riemann example
 
 
 def default_params(n: int | tuple[int, int, int], **over) -> AstrParams:
@@ -138,6 +139,26 @@ class AstrSolver:
         check(self.lib.astr_set_spacing(self.ctx, float(dx), float(dy), float(dz)),
               "astr_set_spacing")
 
+    def set_bc(self, bctype=(1, 1, 1, 1, 1, 1), udf: int = 0) -> None:
+        """boucon's faces (i=0, i=im, j=0, j=jm, k=0, k=km): 1 nothing done,
+        0 the user condition udf (UDF_RIEMANN2D: udf_bc of the 2-D Riemann
+        example)."""
+        arr = (ctypes.c_int32 * 6)(*[int(b) for b in bctype])
+        check(self.lib.astr_set_bc(self.ctx, arr, int(udf)), "astr_set_bc")
+
+    def boucon(self) -> None:
+        check(self.lib.astr_boucon(self.ctx), "astr_boucon")
+
+    @property
+    def time(self) -> float:
+        out = ctypes.c_double(0.0)
+        check(self.lib.astr_get_time(self.ctx, ctypes.byref(out)), "astr_get_time")
+        return out.value
+
+    @time.setter
+    def time(self, t: float) -> None:
+        check(self.lib.astr_set_time(self.ctx, float(t)), "astr_set_time")
+
     def synchronize(self) -> None:
         check(self.lib.astr_synchronize(self.ctx), "astr_synchronize")
 
@@ -145,7 +166,8 @@ class AstrSolver:
         check(self.lib.astr_set_timing(self.ctx, 1 if on else 0), "astr_set_timing")
 
     def timing(self) -> np.ndarray:
-        """ms per family: filter, qswap, grad, sigma, stats, flux, update, total, sensor."""
+        """ms per family: filter, boucon+qswap, grad, gflux (central) or sensor
+        (upwind steps), stats, flux, update, total, sensor (upwind rhscal)."""
         out = np.zeros(9)
         check(self.lib.astr_get_timing(self.ctx, dptr(out)), "astr_get_timing")
         return out
@@ -387,6 +409,24 @@ def riemann2d_state(p: AstrParams, lengths=(1.0, 1.0), periodic_halos=False):
     f = np.asfortranarray
     return {"q": f(q), "rho": f(rho), "vel": f(np.stack([u, v, w], axis=-1)), "prs": f(prs),
             "tmp": f(tmp)}
+
+
+def cfl_dt(p: AstrParams, st: dict, dxi, cfl: float = 0.5) -> float:
+    """The time step with the given CFL number as cflcal measures it
+    (src/commcal.F90:27-87): cfl = dt * sum over the axes of the largest
+    characteristic speed max(U, U - c|xi|, U + c|xi|), U = xi . u, c =
+    sqrt(T)/Ma, over the nodes 0..n (the reference only reports it; its
+    decks fix deltat)."""
+    sl = tuple(slice(HM, HM + n + 1) for n in (p.im, p.jm, p.km))
+    vel = np.asarray(st["vel"])[sl]
+    css = np.sqrt(np.asarray(st["tmp"])[sl]) / p.mach
+    dx = np.asarray(dxi)[sl]
+    total = 0.0
+    for d in range(3):
+        ub = sum(dx[..., d, c] * vel[..., c] for c in range(3))
+        cs = css * np.sqrt(sum(dx[..., d, c] ** 2 for c in range(3)))
+        total += max(0.0, float(np.max(np.maximum(ub, np.maximum(ub - cs, ub + cs)))))
+    return cfl / total
 
 
 def interior(a, p: AstrParams):

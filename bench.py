@@ -8,6 +8,11 @@ whole domain resident in one GPU's HBM.  A step is one full RK3 step (three
 stages of filter, qswap, right-hand side, update).  The metric counts unique
 periodic grid points (n^3) advanced one RK3 step per second.
 
+Other workloads (--workload): riemann2d, RK3 steps of the 2-D four-quadrant
+Riemann problem at 2048^2 with the explicit upwind scheme and the example's
+boundary conditions; ensemble, 48 periodic turbulence boxes of mixed sizes
+advanced together against looping the plain solver over them.
+
 Prints one JSON line.  roofline: the dominant kernel family, its algorithmic
 HBM bytes per launch over its average launch time (HIP events on the
 solver's stream, over the timed region).  cpu_baseline: the checker's C
@@ -21,6 +26,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -86,8 +93,9 @@ FP64_PEAK_TFLOPS = 78.6
 
 
 def measured_flops(n: int):
-    """FP64 flops per upwind interface-flux launch from the committed rocprof
-    PMC summary (profiles/riemann2d_upwind.json), for the 2048^2 workload."""
+    """FP64 flops and HBM bytes per upwind interface-flux launch from the
+    committed rocprof PMC summary (profiles/riemann2d_upwind.json), for the
+    2048^2 workload."""
     path = os.path.join(ROOT, "profiles", "riemann2d_upwind.json")
     if n != 2048 or not os.path.exists(path):
         return None, None
@@ -96,80 +104,124 @@ def measured_flops(n: int):
     return d.get("fp64_flops_per_launch"), d.get("traffic_bytes_per_launch")
 
 
-def cpu_baseline_upwind(n_small: int):
+RIEMANN_FACES = (0, 0, 0, 0, 1, 1)  # the example's input.riemann2d bctype
+
+
+def riemann_params(n: int):
+    import astr_gpu_amd as ag
+
+    # one domain bounded at both ends on i and j (ntype 4), k absent
+    return ag.default_params((n, n, 0), npdci=4, npdcj=4, npdck=3, convection=1,
+                             deriv_explicit=1, recon_schem=3, lchardecomp=1, lfilter=0,
+                             diffterm=0, mach=1.0)
+
+
+def cpu_baseline_riemann(n_small: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     import astr_gpu_amd as ag
 
-    p = ag.default_params((n_small, n_small, 0), convection=1, deriv_explicit=1, recon_schem=3,
-                          lchardecomp=1, lfilter=0, diffterm=0, mach=1.0)
-    st = ag.riemann2d_state(p, periodic_halos=True)
+    p = riemann_params(n_small)
+    st = ag.riemann2d_state(p)
     jac, dxi = ag.box_metrics(p, lengths=(1.0, 1.0, 1.0))
-    box = pyoracle.UwdBox((n_small, n_small, 0), recon_schem=3, lchardecomp=True, mach=1.0)
+    dt = ag.cfl_dt(p, st, dxi, 0.5)
+    box = pyoracle.UwdBox((n_small, n_small, 0), npdc=(4, 4, 3), recon_schem=3,
+                          lchardecomp=True, mach=1.0)
     t0 = time.perf_counter()
-    box.rhscal(st, dxi.reshape(dxi.shape[:3] + (9,), order="F"), jac)
+    box.rk3_steps(st, dxi.reshape(dxi.shape[:3] + (9,), order="F"), jac, dt, nsteps=1,
+                  bctype=RIEMANN_FACES, udf=1)
     el = time.perf_counter() - t0
-    return {"value": n_small * n_small / el, "unit": "grid-point-RHS/s", "cores": 1,
+    return {"value": (n_small + 1) ** 2 / el, "unit": "grid-point-RK3-steps/s", "cores": 1,
             "kind": "port",
-            "sample": f"one upwind right-hand side of the {n_small}^2 four-quadrant box by the "
-                      f"C restatement (one thread), {el:.1f} s"}
+            "sample": f"one RK3 step of the {n_small}^2 Riemann box by the C restatement "
+                      f"(one thread), {el:.1f} s"}
 
 
 def bench_riemann(args):
-    """Explicit upwind right-hand side (MP7, characteristic projection, Ducros
-    sensor, explicit 6th-order gradient) of the 2-D four-quadrant Riemann
-    state on an n x n box: grid points per second of astr_rhscal."""
+    """RK3 steps of the 2-D four-quadrant Riemann problem on an n x n box of
+    [0,1]^2 (explicit upwind MP7 with characteristic projection where the
+    Ducros sensor flags a shock, explicit 6th-order gradient, the example's
+    boundary condition on the four faces, dt from CFL 0.5): grid points
+    ((n+1)^2, every node is unique on a bounded box) advanced per second.
+    --t-end also runs the problem from t = 0 to that time and reports the
+    wall time."""
     import astr_gpu_amd as ag
 
     n = args.n
-    p = ag.default_params((n, n, 0), convection=1, deriv_explicit=1, recon_schem=3,
-                          lchardecomp=1, lfilter=0, diffterm=0, mach=1.0)
+    p = riemann_params(n)
     s = ag.AstrSolver(p, device=0)
     jac, dxi = ag.box_metrics(p, lengths=(1.0, 1.0, 1.0))
     s.upload_geom(jac, dxi)
-    st = ag.riemann2d_state(p, periodic_halos=True)
-    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
-    del st
-    for _ in range(args.warmup):
-        s.rhscal()
+    st = ag.riemann2d_state(p)
+    dt = ag.cfl_dt(p, st, dxi, 0.5)
+    s.set_bc(RIEMANN_FACES, ag.UDF_RIEMANN2D)
+
+    def restart():
+        s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+        s.time = 0.0
+
+    restart()
+    s.steploop(args.warmup, dt)
     s.synchronize()
     s.set_timing(True)
-    for _ in range(args.steps):
-        s.rhscal()
-    s.synchronize()
+    s.steploop(args.steps, dt)
     fam = s.timing()
     s.set_timing(False)
+    restart()
+    s.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s.rhscal()
+    s.steploop(args.steps, dt)
     s.synchronize()
     el = time.perf_counter() - t0
-    s.close()
-    pts = float(n) * n
+    pts = float(n + 1) ** 2
     value = pts * args.steps / el
-    avg_ms = fam[5] / (2 * args.steps)  # two interface-flux sweeps per call
+    avg_ms = fam[5] / (6 * args.steps)  # two interface-flux sweeps per stage
     flops, traffic = measured_flops(n)
     out = {
-        "metric": "FP64 grid-point-RHS/s, explicit upwind 2-D Riemann",
-        "value": value, "unit": "grid-point-RHS/s", "n_gpus": 1, "steps": args.steps,
+        "metric": "FP64 grid-point-RK3-steps/s, 2-D Riemann (explicit upwind)",
+        "value": value, "unit": "grid-point-RK3-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (four-quadrant Riemann initial field)",
-        "config": {"workload": f"2-D four-quadrant Riemann {n}^2, recon MP7, characteristic "
-                               f"projection, Ducros sensor, explicit 643e gradient", "n": n,
-                   "parallelism": "single GPU"},
-        "family_ms_per_call": {"grad": fam[2] / args.steps, "sensor": fam[8] / args.steps,
-                               "interface_flux": fam[5] / args.steps},
+        "data": "synthetic (four-quadrant Riemann initial field of riem2dini)",
+        "config": {"workload": f"2-D four-quadrant Riemann {n}^2 on [0,1]^2, recon MP7, "
+                               f"characteristic projection, Ducros sensor, explicit 643e "
+                               f"gradient, udf_bc on four faces, dt {dt:.4e} (CFL 0.5)",
+                   "n": n, "parallelism": "single GPU"},
+        "family_ms_per_step": {"boucon+qswap": fam[1] / args.steps, "grad": fam[2] / args.steps,
+                               "sensor": fam[3] / args.steps,
+                               "interface_flux": fam[5] / args.steps,
+                               "rk_update": fam[6] / args.steps, "total": fam[7] / args.steps},
         "roofline": None,
     }
     if flops:
         achieved = flops / (avg_ms * 1e-3) / 1e12
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                           "traffic": traffic, "kernel": "k_uwd (interface fluxes), 2 launches/call",
+                           "traffic": traffic,
+                           "kernel": "k_uwd (interface fluxes), 2 launches/stage",
                            "flops_per_launch": flops, "avg_launch_ms": avg_ms}
-    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline_upwind(1024)
+    if args.t_end:
+        restart()
+        nsteps = 0
+        t = 0.0
+        while t < args.t_end - 1e-12:
+            t = t + dt
+            nsteps += 1
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.steploop(nsteps, dt)
+        s.synchronize()
+        run = time.perf_counter() - t0
+        fin = s.download_state()
+        rho = ag.interior(fin["rho"], p)
+        prs = ag.interior(fin["prs"], p)
+        out["run"] = {"t_end": s.time, "steps": nsteps, "wall_s": run,
+                      "rho_min": float(rho.min()), "rho_max": float(rho.max()),
+                      "p_min": float(prs.min()), "p_max": float(prs.max()),
+                      "finite": bool(np.isfinite(fin["q"]).all())}
+    s.close()
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline_riemann(1024)
     print(json.dumps(out))
 
 
@@ -255,8 +307,12 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["tgv", "riemann2d", "ensemble"], default="tgv",
-                    help="tgv: the headline RK3 step; riemann2d: the upwind right-hand side; "
-                         "ensemble: batched boxes against looping the plain solver")
+                    help="tgv: the headline RK3 step; riemann2d: RK3 steps of the 2-D "
+                         "Riemann problem (upwind, boundary conditions); ensemble: batched "
+                         "boxes against looping the plain solver")
+    ap.add_argument("--t-end", type=float, default=0.0,
+                    help="riemann2d: also run the problem from t = 0 to this time (0.3 in "
+                         "the benchmark) and report its wall time")
     ap.add_argument("--nbox", type=int, default=48)
     ap.add_argument("--seed", type=int, default=1)
     args = ap.parse_args()

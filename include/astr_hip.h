@@ -81,9 +81,17 @@ void astr_default_params(astr_params *p, int32_t n);
  * the pre-factored line systems (fd_scheme_initiate, src/derivative.F90:
  * 63-158; compact_filter_initiate + filter_coefficient_cal, src/filter.F90:
  * 31-100,299-432) and allocates the device fields on HIP device `device`.
- * km = 0 is a two-dimensional box (convection 1 with deriv_explicit 1 and
- * npdck 3; astr_rhscal only).  With convection 1 the j and k axes take
- * npdc 1..3: convrsduwd stops on 4 there (src/solver.F90:808-822). */
+ * km = 0 is a two-dimensional box (an upwind convection with
+ * deriv_explicit 1 and npdck 3; right-hand sides and whole steps without
+ * the filter).  npdc 4 (bounded at both ends on one domain) is the union of
+ * ntype 1 at the low end and ntype 2 at the high end in the upwind path:
+ * recons_exp (convrsduwd), the Ducros sensor's clamps, diff6ec and the
+ * ranges is..ie = 1..n-1.  The reference gives ntype 4 only to a single rank
+ * along a non-periodic axis, where convrsduwd stops on j and k
+ * (src/solver.F90:808-822) and is/ie stay unset (src/parallel.F90:
+ * 1070-1078); it runs bounded axes on several ranks, whose result the
+ * union reproduces (the explicit upwind flux has no closure at a rank
+ * interface). */
 int astr_create(const astr_params *p, int32_t device, astr_ctx **out);
 int astr_destroy(astr_ctx *ctx);
 
@@ -125,11 +133,14 @@ int astr_stats(astr_ctx *ctx, double *out3);
 
 /* --- whole steps: time_integration_rk, src/mainloop.F90:298-595 ------- */
 
-/* One RK3 step.  If stats is non-null it receives (ke, enstrophy,
- * dissipation) of the stage-1 state and velocity gradients, which rkfirst
- * (src/mainloop.F90:608-688) evaluates after rhscal.  The step applies the
- * RK3 combine inside the last flux sweep of each stage, so qrhs is left
- * holding the x+y part; astr_rhscal gives the whole qrhs. */
+/* One RK3 step: three-dimensional periodic boxes (any scheme) and
+ * two-dimensional upwind boxes (any closures, boundary conditions by
+ * astr_set_bc, no filter).  If stats is non-null (three-dimensional boxes)
+ * it receives (ke, enstrophy, dissipation) of the stage-1 state and
+ * velocity gradients, which rkfirst (src/mainloop.F90:608-688) evaluates
+ * after rhscal.  The central path applies the RK3 combine inside the last
+ * flux sweep of each stage, so qrhs is left holding the x+y part;
+ * astr_rhscal gives the whole qrhs.  The step advances the time by dt. */
 int astr_rk3_step(astr_ctx *ctx, double dt, double *stats);
 /* nsteps RK3 steps back to back; stats (nullable) receives 3 per step. */
 int astr_rk3_steps(astr_ctx *ctx, int32_t nsteps, double dt, double *stats);
@@ -156,21 +167,22 @@ int astr_filter_field(astr_ctx *ctx, int32_t dir, const double *f, double *ff);
 
 /* Kernel time since timing was (re)enabled with astr_set_timing(ctx, 1),
  * in milliseconds, per kernel family (HIP events on the solver's stream):
- * [0] filter sweeps, [1] qswap, [2] gradient, [3] sigma/qflux and the
- * flux of each axis,
+ * [0] filter sweeps, [1] boucon and qswap, [2] gradient, [3] central:
+ * sigma/qflux and the flux of each axis, upwind steps: the shock sensor,
  * [4] statistics, [5] flux sweeps (central: inviscid and viscous flux
  * derivatives, the z sweep applying the RK3 combine; upwind: interface
  * fluxes), [6] update (central: primitives; upwind: RK3 combine and
- * primitives), [7] total, [8] shock sensor.  astr_rk3_step(s) fill 0-7;
- * astr_rhscal of an upwind context fills 2, 5, 7 and 8.  astr_rk3_steps
- * restarts the count at each call. */
+ * primitives), [7] total, [8] shock sensor of astr_rhscal.
+ * astr_rk3_step(s) fill 0-7; astr_rhscal of an upwind context fills 2, 5,
+ * 7 and 8.  astr_rk3_steps restarts the count at each call. */
 int astr_set_timing(astr_ctx *ctx, int32_t on);
 int astr_get_timing(astr_ctx *ctx, double *ms9);
 
 /* --- batched ensemble -------------------------------------------------- */
 
 /* A group of contexts (same device; each created, with geometry and state
- * uploaded; three-dimensional periodic boxes) that advance together.
+ * uploaded; three-dimensional periodic boxes without a user boundary
+ * condition) that advance together.
  * astr_batch_rk3_steps captures one RK3 step of every member into one HIP
  * graph, each member's kernels on its own stream branching from the batch's
  * root stream so that boxes of any size run concurrently, and replays the
@@ -190,6 +202,26 @@ int astr_batch_set_mode(astr_batch *b, int32_t mode);
  * by dx, dy, dz (tgvsolver.F90:420-470) instead of using metrics; its
  * steps need it.  Variant 1 runs whole steps (astr_rk3_step(s)) only. */
 int astr_set_spacing(astr_ctx *ctx, double dx, double dy, double dz);
+
+/* Boundary conditions, boucon (src/bc.F90:327-411), applied in every stage
+ * before qswap (src/mainloop.F90:413-415) at the step's start time.
+ * bctype[0..5] per face (i=0, i=im, j=0, j=jm, k=0, k=km): 1 nothing done
+ * (periodic, or a bounded face left as it is), 0 the user condition udf.
+ * udf ASTR_UDF_RIEMANN2D is udf_bc of the 2-D Riemann example
+ * (examples/Riemann2D/user_define_module/userdefine_riemann2d.F90:421-668)
+ * on the grid of gridcube(1,1,0): shock-moving quadrant states on faces
+ * 1 and 3, zero-slope extrapolation on faces 2 and 4, the north-east state
+ * in the corner (im,jm).  The other bctypes of bc.F90 (inflow, outflow,
+ * walls, NSCBC) return ASTR_E_UNSUPPORTED.  Default: all faces 1. */
+#define ASTR_UDF_RIEMANN2D 1
+int astr_set_bc(astr_ctx *ctx, const int32_t *bctype, int32_t udf);
+/* call boucon alone (src/mainloop.F90:413), at the context's time */
+int astr_boucon(astr_ctx *ctx);
+/* The simulation time (commvar time) that boucon sees; each whole step
+ * adds dt after its three stages (time=time+deltat, src/mainloop.F90:200),
+ * batched steps included.  Starts at 0. */
+int astr_set_time(astr_ctx *ctx, double time);
+int astr_get_time(astr_ctx *ctx, double *time);
 
 /* Wait for the work queued on the context's stream (entry points that
  * only queue kernels, such as astr_rhscal, return before it ends).  No

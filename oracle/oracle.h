@@ -159,6 +159,14 @@ int orc_uwd_rhscal(const orc_uwd *u, int explicit_fd, const double *q, const dou
                    const double *vel, const double *prs, const double *tmp, const double *dxi,
                    const double *jacob, double *qrhs, double *dvel, double *dtmp, double *ssf,
                    unsigned char *lshock);
+double orc_riem2d_shockspeed(double gamma, double mach, double p1, double t1, double p2);
+void orc_riem2d_bc(const orc_uwd *u, const int *bctype, double time, double *q, double *rho,
+                   double *vel, double *prs, double *tmp);
+void orc_uwd_qswap(const orc_uwd *u, double *q, double *rho, double *vel, double *prs,
+                   double *tmp);
+int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *bctype, int udf,
+                     double time, double dt, const double *dxi, const double *jacob, double *q,
+                     double *rho, double *vel, double *prs, double *tmp);
 
 #ifdef __cplusplus
 }

@@ -305,6 +305,13 @@ def uwd_lib():
                                               ctypes.POINTER(_I), _D]
         lib.orc_mplimiter.restype = _F
         lib.orc_mplimiter.argtypes = [_D, _F, _I, _I, _I, _I]
+        I6 = ctypes.POINTER(ctypes.c_int)
+        lib.orc_riem2d_shockspeed.restype = _F
+        lib.orc_riem2d_shockspeed.argtypes = [_F, _F, _F, _F, _F]
+        lib.orc_riem2d_bc.argtypes = [P, I6, _F, _D, _D, _D, _D, _D]
+        lib.orc_uwd_qswap.argtypes = [P, _D, _D, _D, _D, _D]
+        lib.orc_uwd_rk3_step.restype = _I
+        lib.orc_uwd_rk3_step.argtypes = [P, _I, _I, I6, _I, _F, _F, _D, _D, _D, _D, _D, _D, _D]
         _uwd_declared = True
     return lib
 
@@ -429,6 +436,53 @@ class UwdBox:
         if rc:
             raise ValueError(f"orc_convrsduwd: unsupported configuration ({rc})")
         return qrhs
+
+    def _state(self, st):
+        """Fortran-ordered float64 copies of a state dict (q, rho, vel, prs,
+        tmp), which the in-place routines below update."""
+        return {"q": self._h(st["q"], 5).copy(order="F"), "vel": self._h(st["vel"], 3).copy(order="F"),
+                **{k: self._h(st[k]).copy(order="F") for k in ("rho", "prs", "tmp")}}
+
+    @staticmethod
+    def _faces(bctype):
+        return (ctypes.c_int * 6)(*[int(b) for b in bctype])
+
+    def riem2d_bc(self, st, time, bctype=(0, 0, 0, 0, 1, 1)):
+        """A copy of st with the 2-D Riemann example's udf_bc applied on the
+        faces whose bctype is 0, at the given time."""
+        lib = uwd_lib()
+        o = self._state(st)
+        lib.orc_riem2d_bc(ctypes.byref(self.cfg), self._faces(bctype), float(time), _p(o["q"]),
+                          _p(o["rho"]), _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
+        return o
+
+    def qswap(self, st):
+        """A copy of st after the single-rank qswap."""
+        lib = uwd_lib()
+        o = self._state(st)
+        lib.orc_uwd_qswap(ctypes.byref(self.cfg), _p(o["q"]), _p(o["rho"]), _p(o["vel"]),
+                          _p(o["prs"]), _p(o["tmp"]))
+        return o
+
+    def rk3_steps(self, st, dxi, jacob, dt, nsteps=1, time=0.0, bctype=(1, 1, 1, 1, 1, 1), udf=0,
+                  compact=False, explicit_fd=True):
+        """A copy of st advanced nsteps RK3 steps of the upwind path from the
+        given start time (boundary conditions by bctype/udf)."""
+        lib = uwd_lib()
+        o = self._state(st)
+        dx = np.asfortranarray(np.asarray(dxi, dtype=np.float64).reshape(self.hshape + (9,),
+                                                                         order="F"))
+        jac = self._h(jacob)
+        t = float(time)
+        for _ in range(nsteps):
+            rc = lib.orc_uwd_rk3_step(ctypes.byref(self.cfg), int(compact), int(explicit_fd),
+                                      self._faces(bctype), int(udf), t, float(dt), _p(dx), _p(jac),
+                                      _p(o["q"]), _p(o["rho"]), _p(o["vel"]), _p(o["prs"]),
+                                      _p(o["tmp"]))
+            if rc:
+                raise ValueError(f"orc_uwd_rk3_step: unsupported configuration ({rc})")
+            t = t + float(dt)
+        return o
 
     def ducros(self, dvel, prs):
         lib = uwd_lib()

@@ -22,6 +22,21 @@
 
 #define HMU 5
 
+/* a bounded low / high end: ntype 1 / 2, and both ends of ntype 4 */
+#define LO_END(nt) ((nt) == 1 || (nt) == 4)
+#define HI_END(nt) ((nt) == 2 || (nt) == 4)
+
+/* The closure type recons_exp sees at interface inode+1/2 of an axis of
+ * ntype nt: ntype 4 acts as ntype 1 on the low half and ntype 2 on the
+ * high half, which is what a run on several ranks along the axis computes
+ * (the explicit upwind flux has no closure at rank interfaces: its stencils
+ * read the neighbour's nodes through the halos). */
+static int end_type(int nt, int inode, int dim)
+{
+    if (nt != 4) return nt;
+    return inode < dim / 2 ? 1 : 2;
+}
+
 /* ---------------------------------------------------------------- scalars */
 
 /* minmod2, minmod4: src/commfunc.F90:699-738 */
@@ -600,10 +615,15 @@ void orc_gradcal(const orc_uwd *u, int explicit_fd, const double *vel, const dou
 /* ducrossensor, src/commcal.F90:196-353: ssf on 0..n, its halo swap
  * (periodic axes: plain copy, src/parallel.F90:4170-4175), then lshock =
  * (max of ssf over -hm+1..hm along each axis) > shkcrt.  The pressure
- * stencil and the window are clamped at ntype 1 (low end) and ntype 2
- * (high end) only, as in the reference; elsewhere they read halos: the
- * pressure halos as given, ssf halos of non-periodic ends zero (no
- * neighbour to swap with on one domain). */
+ * stencil and the window are clamped at a bounded end: the low end of
+ * ntype 1 and the high end of ntype 2 (as in the reference), and both ends
+ * of ntype 4.  ntype 4 (one domain bounded at both ends) is the union of
+ * the two: the reference assigns it only to a single rank along a
+ * non-periodic axis, where it leaves is/ie unset (src/parallel.F90:
+ * 1070-1078), so no reference run defines these reads; a run on several
+ * ranks, which the reference can complete, has ntype 1 and 2 at the
+ * physical ends and clamps there.  Elsewhere the stencils read halos: the
+ * pressure halos as given, ssf halos of non-periodic ends zero. */
 void orc_ducros(const orc_uwd *u, const double *dvel, const double *prs, double *ssf,
                 unsigned char *lshock)
 {
@@ -624,12 +644,12 @@ void orc_ducros(const orc_uwd *u, const double *dvel, const double *prs, double 
 #undef DV
                 const double vort = vx * vx + vy * vy + vz * vz;
                 int ip1 = i + 1, im1 = i - 1, jp1 = j + 1, jm1 = j - 1, kp1 = k + 1, km1 = k - 1;
-                if (u->npdc[0] == 1 && im1 < 0) im1 = 0;
-                if (u->npdc[1] == 1 && jm1 < 0) jm1 = 0;
-                if (u->npdc[2] == 1 && km1 < 0) km1 = 0;
-                if (u->npdc[0] == 2 && ip1 > u->im) ip1 = u->im;
-                if (u->npdc[1] == 2 && jp1 > u->jm) jp1 = u->jm;
-                if (u->npdc[2] == 2 && kp1 > u->km) kp1 = u->km;
+                if (LO_END(u->npdc[0]) && im1 < 0) im1 = 0;
+                if (LO_END(u->npdc[1]) && jm1 < 0) jm1 = 0;
+                if (LO_END(u->npdc[2]) && km1 < 0) km1 = 0;
+                if (HI_END(u->npdc[0]) && ip1 > u->im) ip1 = u->im;
+                if (HI_END(u->npdc[1]) && jp1 > u->jm) jp1 = u->jm;
+                if (HI_END(u->npdc[2]) && kp1 > u->km) kp1 = u->km;
                 const double pc = prs[hidx(&s, i, j, k)];
 #define P(a, b, c) prs[hidx(&s, a, b, c)]
                 const double dpdi = fabs(P(ip1, j, k) - 2.0 * pc + P(im1, j, k)) /
@@ -666,8 +686,8 @@ void orc_ducros(const orc_uwd *u, const double *dvel, const double *prs, double 
                     for (int o = -HMU + 1; o <= HMU; ++o) {
                         int c[3] = {c0[0], c0[1], c0[2]};
                         int t = c[d] + o;
-                        if (u->npdc[d] == 1 && t < 0) t = 0;
-                        if (u->npdc[d] == 2 && t > n[d]) t = n[d];
+                        if (LO_END(u->npdc[d]) && t < 0) t = 0;
+                        if (HI_END(u->npdc[d]) && t > n[d]) t = n[d];
                         c[d] = t;
                         m = max2(m, ssf[hidx(&s, c[0], c[1], c[2])]);
                     }
@@ -694,7 +714,7 @@ int orc_convrsduwd(const orc_uwd *u, const double *q, const double *rho, const d
     if (nd == 2) lo[2] = hi[2] = 0;
     for (int d = 0; d < nd; ++d) {
         const int nt = u->npdc[d];
-        if (nt < 1 || nt > 4 || (d > 0 && nt == 4)) return 1;
+        if (nt < 1 || nt > 4) return 1;
         const int iss = (nt == 1 || nt == 4) ? 0 : -HMU;
         const int iee = (nt == 2 || nt == 4) ? n[d] : n[d] + HMU;
         const int len = n[d] + 1 + 2 * HMU;
@@ -765,10 +785,11 @@ int orc_convrsduwd(const orc_uwd *u, const double *q, const double *rho, const d
                             }
                         }
                     }
+                    const int nte = end_type(nt, i, n[d]);
                     for (int m = 0; m < 5; ++m) {
-                        const double v1 = orc_recons_exp(fcp[m], i, n[d], nt, u->recon_schem, lsh,
+                        const double v1 = orc_recons_exp(fcp[m], i, n[d], nte, u->recon_schem, lsh,
                                                          u->bfacmpld);
-                        const double v2 = orc_recons_exp(fcm[m], i, n[d], nt, u->recon_schem, lsh,
+                        const double v2 = orc_recons_exp(fcm[m], i, n[d], nte, u->recon_schem, lsh,
                                                          u->bfacmpld);
                         fhc[m] = v1 + v2;
                     }
@@ -1093,4 +1114,250 @@ int orc_cmp_rhscal(const orc_uwd *u, int explicit_fd, const double *q, const dou
     if (rc) return rc;
     for (long e = 0; e < 5 * s.nint; ++e) qrhs[e] = -qrhs[e];
     return 0;
+}
+
+/* ------------------------------------ boundary conditions and RK3 steps */
+
+/* q2fvar_3da at one node, src/fludyna.F90:545-634 (nondimensional, no
+ * species): rho, u, v, w from q; p = (q5 - rho/2 |u|^2)/const6;
+ * T = thermal(pressure, density) = p/rho*const2. */
+static void q2fvar_node(const shape_t *s, long h, double const2, double const6, const double *q,
+                        double *rho, double *vel, double *prs, double *tmp)
+{
+    const double r = q[h];
+    const double u = q[s->nf + h] / r, v = q[2 * s->nf + h] / r, w = q[3 * s->nf + h] / r;
+    const double p = (q[4 * s->nf + h] - 0.5 * r * (u * u + v * v + w * w)) / const6;
+    rho[h] = r;
+    vel[h] = u;
+    vel[s->nf + h] = v;
+    vel[2 * s->nf + h] = w;
+    prs[h] = p;
+    tmp[h] = p / r * const2;
+}
+
+/* The primitives of one boundary node as udf_bc sets them, then
+ * tmp = thermal(pressure, density) and q = fvar2q_sca's pressure form,
+ * q5 = p*const6 + rho*(0.5*sum(u*u)) (src/fludyna.F90:45-70, 310-362). */
+static void bc_node(const shape_t *s, long h, double const2, double const6, double r, double u,
+                    double v, double w, double p, double *q, double *rho, double *vel, double *prs,
+                    double *tmp)
+{
+    rho[h] = r;
+    vel[h] = u;
+    vel[s->nf + h] = v;
+    vel[2 * s->nf + h] = w;
+    prs[h] = p;
+    tmp[h] = p / r * const2;
+    q[h] = r;
+    q[s->nf + h] = r * u;
+    q[2 * s->nf + h] = r * v;
+    q[3 * s->nf + h] = r * w;
+    const double var1 = 0.5 * (u * u + v * v + w * w);
+    q[4 * s->nf + h] = p * const6 + r * var1;
+}
+
+/* shockspeed, examples/Riemann2D/user_define_module/userdefine_riemann2d.F90:
+ * 670-686: the Rankine-Hugoniot speed of the shock that raises the pressure
+ * p1 -> p2 in a gas at temperature t1, sqrt(1 + (p2/p1-1)(g+1)/(2g)) times
+ * the sound speed sos(t1) = sqrt(t1)/Ma (src/fludyna.F90:832-859). */
+double orc_riem2d_shockspeed(double gamma, double mach, double p1, double t1, double p2)
+{
+    const double mx2 = (p2 / p1 - 1.0) * (gamma + 1.0) / (2.0 * gamma) + 1.0;
+    const double ss = sqrt(t1) / mach;
+    return sqrt(mx2) * ss;
+}
+
+/* udf_bc of the 2-D Riemann example (userdefine_riemann2d.F90:421-668), as
+ * boucon calls it (src/bc.F90:327-411: faces n = 1..6 in order, udf_bc
+ * where bctype(n) = 0), on the grid of gridcube(1,1,0) (x = i/im,
+ * y = j/jm; src/gridgeneration.F90:233-267, single rank):
+ *   face 1 (i=0):  the quadrant states behind/ahead of the moving shock
+ *                  y > 0.8 - vss*time, u = 1.206;
+ *   face 2 (i=im): rho, p, v extrapolated, (4f(im-1) - f(im-2))/3, u = 0;
+ *                  then the corner (im,jm) fixed at the NE state;
+ *   face 3 (j=0):  the states by x > 0.8 - vss*time, v = 1.206;
+ *   face 4 (j=jm): rho, p, u extrapolated along j, v = 0; corner again.
+ * The NW pressure literal is single precision (0.30) as in the source.
+ * Faces 5 and 6 have no user condition. */
+void orc_riem2d_bc(const orc_uwd *u, const int *bctype, double time, double *q, double *rho,
+                   double *vel, double *prs, double *tmp)
+{
+    const shape_t s = shape_of(u);
+    const double gamma = u->gamma, mach = u->mach;
+    const double const2 = gamma * (mach * mach), const6 = 1.0 / (gamma - 1.0);
+    const double r01 = 0.5323, r11 = 1.5, u01 = 1.206, v01 = 0.0, v11 = 0.0;
+    const double p01 = (double)0.30f, p11 = 1.5;
+    const double r00 = 0.138, r10 = 0.5323, u00 = 1.206, v00 = 1.206, u10 = 0.0, v10 = 1.206;
+    const double p00 = 0.029, p10 = 0.3;
+    const double n1d3 = 1.0 / 3.0;
+    const int im = u->im, jm = u->jm, km = u->km;
+    const double dx = 1.0 / (double)im, dy = 1.0 / (double)jm;
+#define H(i, j, k) hidx(&s, i, j, k)
+#define V(c, h) vel[(c) * s.nf + (h)]
+    if (bctype[0] == 0) {
+        const double t1 = p01 / r01 * const2;
+        const double vss = orc_riem2d_shockspeed(gamma, mach, p01, t1, p00);
+        for (int k = 0; k <= km; ++k)
+            for (int j = 0; j <= jm; ++j) {
+                const double ys = 0.8 - vss * time;
+                const double y = dy * (double)j;
+                const int up = y > ys;
+                bc_node(&s, H(0, j, k), const2, const6, up ? r01 : r00, u01, up ? v01 : v00, 0.0,
+                        up ? p01 : p00, q, rho, vel, prs, tmp);
+            }
+    }
+    if (bctype[1] == 0) {
+        for (int k = 0; k <= km; ++k)
+            for (int j = 0; j <= jm; ++j) {
+                const long a = H(im - 1, j, k), b = H(im - 2, j, k);
+                const double r = n1d3 * (4.0 * rho[a] - rho[b]);
+                const double p = n1d3 * (4.0 * prs[a] - prs[b]);
+                const double v = n1d3 * (4.0 * V(1, a) - V(1, b));
+                bc_node(&s, H(im, j, k), const2, const6, r, 0.0, v, 0.0, p, q, rho, vel, prs,
+                        tmp);
+            }
+        for (int k = 0; k <= km; ++k)
+            bc_node(&s, H(im, jm, k), const2, const6, r11, 0.0, v11, 0.0, p11, q, rho, vel, prs,
+                    tmp);
+    }
+    if (bctype[2] == 0) {
+        const double t1 = p10 / r10 * const2;
+        const double vss = orc_riem2d_shockspeed(gamma, mach, p10, t1, p00);
+        for (int k = 0; k <= km; ++k)
+            for (int i = 0; i <= im; ++i) {
+                const double xs = 0.8 - vss * time;
+                const double x = dx * (double)i;
+                const int up = x > xs;
+                bc_node(&s, H(i, 0, k), const2, const6, up ? r10 : r00, up ? u10 : u00, v10, 0.0,
+                        up ? p10 : p00, q, rho, vel, prs, tmp);
+            }
+    }
+    if (bctype[3] == 0) {
+        for (int k = 0; k <= km; ++k)
+            for (int i = 0; i <= im; ++i) {
+                const long a = H(i, jm - 1, k), b = H(i, jm - 2, k);
+                const double r = n1d3 * (4.0 * rho[a] - rho[b]);
+                const double p = n1d3 * (4.0 * prs[a] - prs[b]);
+                const double uu = n1d3 * (4.0 * V(0, a) - V(0, b));
+                bc_node(&s, H(i, jm, k), const2, const6, r, uu, 0.0, 0.0, p, q, rho, vel, prs,
+                        tmp);
+            }
+        for (int k = 0; k <= km; ++k)
+            bc_node(&s, H(im, jm, k), const2, const6, r11, 0.0, 0.0, 0.0, p11, q, rho, vel, prs,
+                    tmp);
+    }
+#undef V
+#undef H
+}
+
+/* qswap, single rank (src/parallel.F90:4849-5319, turbmode none): along
+ * each periodic axis in the order i, j, k the halo copy, the duplicate
+ * average q(0) = q(n) = (q(0)+q(n))/2 and q2fvar on the strips -hm..0 and
+ * n..n+hm, over 0..n of the other axes; a two-dimensional box (km = 0)
+ * copies the plane k = 0 into every k plane -hm..hm and refreshes the
+ * primitives of all of them (the ka = 0 branch).  Non-periodic axes are
+ * left alone (no neighbour on one rank). */
+void orc_uwd_qswap(const orc_uwd *u, double *q, double *rho, double *vel, double *prs,
+                   double *tmp)
+{
+    const shape_t s = shape_of(u);
+    const double const2 = u->gamma * (u->mach * u->mach), const6 = 1.0 / (u->gamma - 1.0);
+    const int n[3] = {u->im, u->jm, u->km};
+    for (int d = 0; d < 3; ++d) {
+        const int two_d = d == 2 && n[2] == 0;
+        if (!two_d && u->npdc[d] != 3) continue;
+        const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
+        for (int b = 0; b <= n[ob]; ++b)
+            for (int a = 0; a <= n[oa]; ++a) {
+                int c[3];
+                c[oa] = a;
+                c[ob] = b;
+                c[d] = 0;
+                const long h0 = hidx(&s, c[0], c[1], c[2]);
+                const long sd = d == 0 ? 1 : d == 1 ? s.ni : s.ni * s.nj;
+#define AT(t) (h0 + (long)(t) * sd)
+                for (int m = 0; m < 5; ++m) {
+                    double *f = q + m * s.nf;
+                    if (two_d) {
+                        const double v0 = f[AT(0)];
+                        for (int t = -HMU; t <= HMU; ++t)
+                            if (t) f[AT(t)] = v0;
+                        continue;
+                    }
+                    for (int t = 1; t <= HMU; ++t) {
+                        const double lo = f[AT(n[d] - t)];
+                        f[AT(-t)] = lo;
+                    }
+                    for (int t = 1; t <= HMU; ++t) {
+                        const double hi = f[AT(t)];
+                        f[AT(n[d] + t)] = hi;
+                    }
+                    const double avg = 0.5 * (f[AT(0)] + f[AT(n[d])]);
+                    f[AT(0)] = avg;
+                    f[AT(n[d])] = avg;
+                }
+                for (int t = n[d]; t <= n[d] + HMU; ++t)
+                    q2fvar_node(&s, AT(t), const2, const6, q, rho, vel, prs, tmp);
+                for (int t = -HMU; t <= 0; ++t)
+                    q2fvar_node(&s, AT(t), const2, const6, q, rho, vel, prs, tmp);
+#undef AT
+            }
+    }
+}
+
+/* One RK3 step of time_integration_rk (src/mainloop.F90:392-476, rk3, no
+ * filter) on the upwind path: per stage qrhs = 0, boucon (the 2-D Riemann
+ * udf_bc on faces with bctype 0 when udf = 1) at the step's start time,
+ * qswap, rhscal (convrsduwd, or convrsdcmp with compact; explicit gradient
+ * with explicit_fd), qsave = q*J at stage 1, the combine
+ * q = (a qsave + b q J + c qrhs dt)/J on 0..n with (a,b,c) = (1,0,1),
+ * (3/4,1/4,1/4), (1/3,2/3,2/3), and updatefvar on 0..n.  Returns nonzero
+ * on an unsupported configuration. */
+int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *bctype, int udf,
+                     double time, double dt, const double *dxi, const double *jacob, double *q,
+                     double *rho, double *vel, double *prs, double *tmp)
+{
+    const shape_t s = shape_of(u);
+    const double const2 = u->gamma * (u->mach * u->mach), const6 = 1.0 / (u->gamma - 1.0);
+    const double rk[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25},
+                             {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    int anyudf = 0;
+    for (int f = 0; f < 6; ++f) {
+        if (bctype[f] != 0 && bctype[f] != 1) return 3;
+        anyudf |= bctype[f] == 0;
+    }
+    if (anyudf && udf != 1) return 3;
+    double *qrhs = malloc(sizeof(double) * 5 * s.nint), *qsave = malloc(sizeof(double) * 5 * s.nint);
+    double *dvel = malloc(sizeof(double) * 9 * s.nint), *dtmp = malloc(sizeof(double) * 3 * s.nint);
+    double *ssf = malloc(sizeof(double) * s.nf);
+    unsigned char *lsh = malloc(s.nint);
+    int rc = 0;
+    for (int st = 0; st < 3 && !rc; ++st) {
+        if (anyudf) orc_riem2d_bc(u, bctype, time, q, rho, vel, prs, tmp);
+        orc_uwd_qswap(u, q, rho, vel, prs, tmp);
+        rc = (compact ? orc_cmp_rhscal : orc_uwd_rhscal)(u, explicit_fd, q, rho, vel, prs, tmp,
+                                                         dxi, jacob, qrhs, dvel, dtmp, ssf, lsh);
+        if (rc) break;
+        for (int k = 0; k <= u->km; ++k)
+            for (int j = 0; j <= u->jm; ++j)
+                for (int i = 0; i <= u->im; ++i) {
+                    const long h = hidx(&s, i, j, k), p = iidx(&s, i, j, k);
+                    const double jc = jacob[h];
+                    for (int m = 0; m < 5; ++m) {
+                        const double qo = q[m * s.nf + h];
+                        if (st == 0) qsave[m * s.nint + p] = qo * jc;
+                        const double v = rk[st][0] * qsave[m * s.nint + p] + rk[st][1] * qo * jc +
+                                         rk[st][2] * qrhs[m * s.nint + p] * dt;
+                        q[m * s.nf + h] = v / jc;
+                    }
+                    q2fvar_node(&s, h, const2, const6, q, rho, vel, prs, tmp);
+                }
+    }
+    free(qrhs);
+    free(qsave);
+    free(dvel);
+    free(dtmp);
+    free(ssf);
+    free(lsh);
+    return rc;
 }

@@ -62,10 +62,9 @@ def test_create_rejects_unsupported_without_touching_the_gpu():
     p.npdci = 7
     with pytest.raises(ag.AstrError, match="ASTR_E_ARG"):
         ag.AstrSolver(p)
-    # upwind path: inviscid only; convrsduwd stops on ntype 4 along j and k
-    # (src/solver.F90:808-822,1003-1014); 2-D boxes only for it
-    bad = [dict(convection=1, diffterm=1), dict(convection=1, diffterm=0, npdcj=4),
-           dict(convection=1, diffterm=0, npdck=4), dict(convection=0, km=0),
+    # upwind path: inviscid only; 2-D boxes only for it, with the explicit
+    # gradient
+    bad = [dict(convection=1, diffterm=1), dict(convection=0, km=0),
            dict(convection=1, diffterm=0, deriv_explicit=0, km=0)]
     for over in bad:
         p = ag.default_params(32, **over)
