@@ -410,6 +410,199 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     }
 }
 
+// ------------------------------------------------------ transposed sweeps
+//
+// The adjoint of a compact line operator.  The forward maps the staged line
+// f (nodes -HM..dim+HM) to x = A^-1 B f on the rows first..last and keeps
+// x on the nodes 0..dim; its transpose maps the adjoint g of those nodes to
+//     f = B^T A^-T (g on the rows 0..dim, zero on the others).
+// A^T is tridiagonal too: its Thomas factors are built on the host and
+// solved by the same chunked scan.  B^T is the transposed interior stencil
+// (antisymmetric for the derivative, symmetric for the filter) plus, for
+// the closure rows at each end, the difference between their own stencil
+// and the interior one, held as dense blocks of TC columns.
+
+constexpr int TC = 16;  // columns of a closure-correction block
+
+struct TransTab {
+    int nlo, nhi;       // closure rows: 0..nlo-1 and R-nhi..R-1
+    int lo0, hi0;       // first column node of each block (first-5, last-10)
+    const double* dlo;  // nlo x TC: B - B_interior of row r at column lo0 + c
+    const double* dhi;  // nhi x TC: the same for row R-1-k at column hi0 + c
+};
+
+struct TSweepArgs {
+    Geo g;
+    int dir;
+    LineTab tab;        // the transposed system A^T of this axis and operator
+    TransTab tt;
+    const FiltCoef* fc;
+    int na, nb, ntile_a, rs_in;
+    const double* fin;  // adjoint of the forward output, nodes 0..dim of a line
+    double* fout;       // adjoint of the forward input
+    const double* xi;   // when set, fin is scaled by xi[n+HM] first (gradcal's metric)
+    int wrap;           // 1: the forward read its halos as periodic copies of the
+                        // nodes (dataswap): fold their adjoint into 0..dim;
+                        // 0: the halos are outputs too (nodes -HM..dim+HM)
+    int accumulate;     // add into fout instead of overwriting it
+};
+
+// (B^T w) at column node j of one line; w holds the solved rows at
+// ln[n + HM] (zero outside first..last), read as zero beyond -HM..dim+HM
+template <int OP>
+__device__ __forceinline__ double bt_column(const double* ln, int j, int dim, const LineTab& tb,
+                                            const TransTab& tt, const FiltCoef* __restrict__ fc)
+{
+    auto W = [&](int n) { return (n >= -HM && n <= dim + HM) ? ln[n + HM] : 0.0; };
+    double v;
+    if constexpr (OP == OP_DERIV) {
+        v = (7.0 / 9.0) * (W(j - 1) - W(j + 1)) + (1.0 / 36.0) * (W(j - 2) - W(j + 2));
+    } else {
+        v = fc->c10[0] * (W(j) + W(j));
+#pragma unroll
+        for (int k = 1; k <= 5; ++k) v += fc->c10[k] * (W(j - k) + W(j + k));
+    }
+    int c = j - tt.lo0;
+    if (c >= 0 && c < TC)
+        for (int r = 0; r < tt.nlo; ++r) v += tt.dlo[r * TC + c] * W(tb.first + r);
+    c = j - tt.hi0;
+    if (c >= 0 && c < TC)
+        for (int k = 0; k < tt.nhi; ++k) v += tt.dhi[k * TC + c] * W(tb.last - k);
+    return v;
+}
+
+// One workgroup per tile of lines (the forward sweep's tiles and staging
+// map): stage g, solve A^T per half-wavefront, form B^T w in place, write
+// the adjoint of the line's input.
+template <int L, int OP, bool XD>
+__global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
+{
+    extern __shared__ double tile[];
+    const int dir = XD ? 0 : a.dir;
+    constexpr int tx = XD ? TXX : TXY;
+    constexpr int EMAX = 2 * L + 1;
+    const int tile_id = blockIdx.x;
+    if (tile_id >= a.ntile_a * a.nb) return;
+    const int ta = tile_id % a.ntile_a;
+    const int b = tile_id / a.ntile_a;
+    const int a0 = ta * tx;
+    const int dim = a.g.n[dir];
+    const int sd = (int)a.g.s[dir];
+    const int nvalid = min(tx, a.na - a0);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int half = tid >> 5;
+    const int ll = XD ? half : (tid & 15);
+    const int n0 = XD ? (tid & 31) : (tid >> 4);
+    constexpr int nstep = XD ? 32 : 16;
+    constexpr int esteps = XD ? L + 1 : EMAX;
+    const bool lvalid = ll < nvalid;
+    const int lc = min(ll, nvalid - 1);
+    int lo;
+    {
+        const int av = a0 + lc;
+        if (XD) lo = (int)a.g.org + av * (int)a.g.s[1] + b * (int)a.g.s[2];
+        else if (dir == 1) lo = (int)a.g.org + av + b * (int)a.g.s[2];
+        else lo = (int)a.g.org + av + b * (int)a.g.s[1];
+    }
+    const unsigned int lob = (unsigned int)lo * 8u, sdb = (unsigned int)sd * 8u;
+    double* const trow = tile + ll * a.rs_in + TG;
+    auto wsync = []() {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    };
+    auto sync = [&]() {
+        if constexpr (XD) wsync();
+        else __syncthreads();
+    };
+
+    // ---------------- stage g on the nodes 0..dim, zero on the rest
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) {
+        if (e < esteps) {
+            const int n = n0 + nstep * e;
+            const int node = n - HM;
+            double v = 0.0;
+            if (node >= 0 && node <= dim) {
+                v = ldb(a.fin, lob + (unsigned int)node * sdb);
+                if (a.xi) v *= a.xi[node + HM];
+            }
+            trow[n] = v;
+        }
+    }
+    sync();
+
+    // ---------------- A^T w = g, then B^T w, one line per half-wavefront
+    for (int l = half; l < tx; l += 8) {
+        if (l - (lane >> 5) >= nvalid) break;  // wave-uniform: both halves invalid
+        double* ln = tile + l * a.rs_in + TG;
+        const int sub = lane & 31;
+        const int r0 = sub * L;
+        const int R = a.tab.R;
+        // compact_filter keeps the input at a bounded end node (src/filter.F90:
+        // 141-142): the adjoint there bypasses the solve
+        const bool k0 = OP == OP_FILTER && lo_boundary(a.tab.ntype);
+        const bool kd = OP == OP_FILTER && hi_boundary(a.tab.ntype);
+        const double g0 = ln[HM], gd = ln[HM + dim];
+        double x[L];
+#pragma unroll
+        for (int t = 0; t < L; ++t) {
+            const int node = a.tab.first + r0 + t;
+            const bool kept = (k0 && node == 0) || (kd && node == dim);
+            x[t] = r0 + t < R && !kept ? ln[HM + node] : 0.0;
+        }
+        thomas_rows<L>(a.tab, x);
+        wsync();
+        if (l < nvalid) {
+#pragma unroll
+            for (int t = 0; t < L; ++t)
+                if (r0 + t < R) ln[HM + a.tab.first + r0 + t] = x[t];
+        }
+        wsync();
+        double f[L + 1];
+#pragma unroll
+        for (int t = 0; t <= L; ++t) {
+            const int j = -HM + sub + 32 * t;
+            f[t] = j <= dim + HM ? bt_column<OP>(ln, j, dim, a.tab, a.tt, a.fc) : 0.0;
+            if (k0 && j == 0) f[t] += g0;
+            if (kd && j == dim) f[t] += gd;
+        }
+        wsync();
+        if (l < nvalid) {
+#pragma unroll
+            for (int t = 0; t <= L; ++t) {
+                const int j = -HM + sub + 32 * t;
+                if (j <= dim + HM) ln[HM + j] = f[t];
+            }
+        }
+    }
+    sync();
+
+    // ---------------- write the adjoint of the line's input
+    if (lvalid) {
+#pragma unroll
+        for (int e = 0; e < EMAX; ++e) {
+            if (e >= esteps) continue;
+            const int n = n0 + nstep * e;
+            const int node = n - HM;
+            double v;
+            if (a.wrap) {
+                if (node < 0 || node > dim) continue;
+                // the forward's halos were f(-k) = f(dim-k), f(dim+k) = f(k)
+                v = trow[n];
+                if (node >= dim - HM && node < dim) v += trow[node - dim + HM];
+                if (node >= 1 && node <= HM) v += trow[dim + node + HM];
+            } else {
+                if (node > dim + HM) continue;
+                v = trow[n];
+            }
+            const unsigned int od = lob + (unsigned int)node * sdb;
+            if (a.accumulate) v += ldb(a.fout, od);
+            stb(a.fout, od, v);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- pointwise
 
 struct PtArgs {
@@ -1602,6 +1795,9 @@ struct astr_ctx {
     long xi_stride = 0;
     double* tabs = nullptr;
     LineTab tab[3][2];
+    double* tabs_t = nullptr;   // transposed systems and closure corrections
+    LineTab tab_t[3][2];
+    TransTab tt[3][2];
     int L[3];
     FiltCoef* fc = nullptr;
     double* part = nullptr;
@@ -1632,13 +1828,12 @@ void node_range(int ntype, int dim, int ext, int& first, int& last)
     last = (ntype == 2 || ntype == 4) ? dim : dim + ext;
 }
 
-HostTab build_tab(int op, int ntype, int dim, double alfa, int L)
+// The LHS rows of a line system, row r: a[r] x(r-1) + x(r) + c[r] x(r+1)
+// (a[0] and c[R-1] are outside the matrix)
+void lhs_rows(int op, int ntype, int R, double alfa, std::vector<double>& a, std::vector<double>& c)
 {
-    HostTab h;
-    node_range(ntype, dim, op == OP_DERIV ? 1 : 3, h.first, h.last);
-    const int R = h.last - h.first + 1;
-    h.R = R;
-    std::vector<double> a(R), c(R);
+    a.assign(R, 0.0);
+    c.assign(R, 0.0);
     if (op == OP_DERIV) {
         // src/derivative.F90:104-139 (643c)
         for (int r = 0; r < R; ++r) a[r] = c[r] = 1.0 / 3.0;
@@ -1654,12 +1849,18 @@ HostTab build_tab(int op, int ntype, int dim, double alfa, int L)
         a[0] = c[0] = b0;
         a[R - 1] = c[R - 1] = bm;
     }
+}
+
+// tridiagonal_thomas_proprocess, src/commfunc.F90:760-770, in the device
+// table layout (ac1, ac2, ac3, LPL*L each)
+void factor_rows(const std::vector<double>& a, const std::vector<double>& c, int L, HostTab& h)
+{
+    const int R = h.R;
     const int S = LPL * L;
     h.t.assign(3 * S, 0.0);
     double* ac1 = h.t.data();
     double* ac2 = ac1 + S;
     double* ac3 = ac1 + 2 * S;
-    // src/commfunc.F90:760-770
     ac1[0] = c[0];
     ac2[0] = 1.0;
     ac3[0] = 0.0;
@@ -1670,7 +1871,113 @@ HostTab build_tab(int op, int ntype, int dim, double alfa, int L)
         ac3[r] = a[r] / den;
     }
     ac1[R - 1] = 0.0;  // the solver starts the back substitution at x(n)=d(n)
+}
+
+HostTab build_tab(int op, int ntype, int dim, double alfa, int L)
+{
+    HostTab h;
+    node_range(ntype, dim, op == OP_DERIV ? 1 : 3, h.first, h.last);
+    h.R = h.last - h.first + 1;
+    std::vector<double> a, c;
+    lhs_rows(op, ntype, h.R, alfa, a, c);
+    factor_rows(a, c, L, h);
     return h;
+}
+
+// The transposed system A^T: sub-diagonal c[r-1], super-diagonal a[r+1]
+HostTab build_tab_t(int op, int ntype, int dim, double alfa, int L)
+{
+    HostTab h;
+    node_range(ntype, dim, op == OP_DERIV ? 1 : 3, h.first, h.last);
+    const int R = h.last - h.first + 1;
+    h.R = R;
+    std::vector<double> a, c, at(R, 0.0), ct(R, 0.0);
+    lhs_rows(op, ntype, R, alfa, a, c);
+    for (int r = 1; r < R; ++r) at[r] = c[r - 1];
+    for (int r = 0; r + 1 < R; ++r) ct[r] = a[r + 1];
+    factor_rows(at, ct, L, h);
+    return h;
+}
+
+// Coefficient of the interior right-hand-side stencil at offset off from
+// the row's node: 643c (derivative) or the 10th-order filter
+double b_interior(int op, const FiltCoef& f, int off)
+{
+    if (op == OP_DERIV) {
+        switch (off) {
+        case 1: return 7.0 / 9.0;
+        case -1: return -7.0 / 9.0;
+        case 2: return 1.0 / 36.0;
+        case -2: return -1.0 / 36.0;
+        default: return 0.0;
+        }
+    }
+    const int k = off < 0 ? -off : off;
+    if (k > 5) return 0.0;
+    return k == 0 ? 2.0 * f.c10[0] : f.c10[k];
+}
+
+// The (node, coefficient) pairs of closure row r of the right-hand side, as
+// rhs_deriv_edge and rhs_filter_edge evaluate it
+std::vector<std::pair<int, double>> b_closure_row(int op, int ntype, int first, int last, int R,
+                                                  int r, const FiltCoef& f)
+{
+    std::vector<std::pair<int, double>> v;
+    const bool lob = ntype == 1 || ntype == 4, hib = ntype == 2 || ntype == 4;
+    const int m = first + r;
+    if (op == OP_DERIV) {
+        if (r == 0 && lob) return {{m, -2.5}, {m + 1, 2.0}, {m + 2, 0.5}};
+        if (r == R - 1 && hib) return {{m, 2.5}, {m - 1, -2.0}, {m - 2, -0.5}};
+        if ((r == 1 && lob) || (r == R - 2 && hib)) return {{m + 1, 0.75}, {m - 1, -0.75}};
+        return {{m + 1, 0.75}, {m - 1, -0.75}, {m + 2, -0.15}, {m - 2, 0.15},
+                {m + 3, 1.0 / 60.0}, {m - 3, -1.0 / 60.0}};
+    }
+    if (r >= R - 3) {
+        const int k = R - 1 - r;
+        if (hib)
+            for (int j = 0; j <= 6; ++j) v.push_back({last - j, f.cb[k][j]});
+        else
+            for (int j = 0; j <= 10; ++j) v.push_back({last + 2 - j, f.ch[k][j]});
+        return v;
+    }
+    if (r <= 2) {
+        if (lob)
+            for (int j = 0; j <= 6; ++j) v.push_back({first + j, f.cb[r][j]});
+        else
+            for (int j = 0; j <= 10; ++j) v.push_back({first - 2 + j, f.ch[r][j]});
+        return v;
+    }
+    if (r == 3 || r == R - 4) {
+        v.push_back({m, 2.0 * f.c6[0]});
+        for (int k = 1; k <= 3; ++k) { v.push_back({m + k, f.c6[k]}); v.push_back({m - k, f.c6[k]}); }
+        return v;
+    }
+    v.push_back({m, 2.0 * f.c8[0]});
+    for (int k = 1; k <= 4; ++k) { v.push_back({m + k, f.c8[k]}); v.push_back({m - k, f.c8[k]}); }
+    return v;
+}
+
+// The closure-correction blocks of B^T for one line system: nlo rows at the
+// low end (columns first-5 ..), nhi at the high end (columns last-10 ..)
+void build_trans_blocks(int op, int ntype, const HostTab& h, const FiltCoef& f, int& nlo, int& nhi,
+                        std::vector<double>& dlo, std::vector<double>& dhi)
+{
+    const bool lob = ntype == 1 || ntype == 4, hib = ntype == 2 || ntype == 4;
+    if (op == OP_DERIV) { nlo = lob ? 2 : 1; nhi = hib ? 2 : 1; }
+    else { nlo = lob ? 5 : 3; nhi = hib ? 5 : 3; }
+    const int lo0 = h.first - 5, hi0 = h.last - 10;
+    dlo.assign(nlo * TC, 0.0);
+    dhi.assign(nhi * TC, 0.0);
+    auto fill = [&](int r, int c0, double* row) {
+        const int m = h.first + r;
+        for (int c = 0; c < TC; ++c) row[c] = -b_interior(op, f, c0 + c - m);
+        for (const auto& e : b_closure_row(op, ntype, h.first, h.last, h.R, r, f)) {
+            const int c = e.first - c0;
+            if (c >= 0 && c < TC) row[c] += e.second;
+        }
+    };
+    for (int r = 0; r < nlo; ++r) fill(r, lo0, dlo.data() + r * TC);
+    for (int k = 0; k < nhi; ++k) fill(h.R - 1 - k, hi0, dhi.data() + k * TC);
 }
 
 // compact_flux_initiate (scheme 543), src/flux.F90:32-108, and its Thomas
@@ -1849,6 +2156,66 @@ int launch_sweep(astr_ctx* c, const SweepArgs& a)
     case PASS_FILTER_FIELD: return launch_sweep_p<PASS_FILTER_FIELD>(c, a);
     }
     return fail(ASTR_E_ARG, "unknown pass");
+}
+
+// ---- transposed sweeps
+
+TSweepArgs tsweep_args(astr_ctx* c, int dir, int op)
+{
+    TSweepArgs a{};
+    a.g = c->g;
+    a.dir = dir;
+    a.tab = c->tab_t[dir][op];
+    a.tt = c->tt[dir][op];
+    a.fc = c->fc;
+    const int n0 = c->g.n[0], n1 = c->g.n[1], n2 = c->g.n[2];
+    if (dir == 0) { a.na = n1 + 1; a.nb = n2 + 1; }
+    else if (dir == 1) { a.na = n0 + 1; a.nb = n2 + 1; }
+    else { a.na = n0 + 1; a.nb = n1 + 1; }
+    const int tx = dir == 0 ? TXX : TXY;
+    a.ntile_a = (a.na + tx - 1) / tx;
+    a.rs_in = lds_row(dir, c->L[dir]);
+    return a;
+}
+
+template <int OP>
+int launch_sweep_t_op(astr_ctx* c, const TSweepArgs& a)
+{
+    const int L = c->L[a.dir];
+    const size_t lds = lds_bytes(a.dir, L);
+    const dim3 grid(a.ntile_a * a.nb);
+    switch (L) {
+#define CASE(LL)                                                                              \
+    case LL:                                                                                  \
+        if (a.dir == 0)                                                                       \
+            hipLaunchKernelGGL((k_sweep_t<LL, OP, true>), grid, dim3(WGT), lds, c->stream, a); \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_sweep_t<LL, OP, false>), grid, dim3(WGT), lds, c->stream, a); \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+    default:
+        return fail(ASTR_E_UNSUPPORTED, "line too long");
+    }
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
+}
+
+// fout (+)= the transpose of the compact derivative (op OP_DERIV) or filter
+// (OP_FILTER) along dir applied to fin; see TSweepArgs for wrap and xi
+int launch_sweep_t(astr_ctx* c, int dir, int op, const double* fin, double* fout, const double* xi,
+                   bool wrap, bool accumulate)
+{
+    if (!c->lines_ok[dir])
+        return fail(ASTR_E_UNSUPPORTED, "axis too long (or absent) for the compact line operators");
+    TSweepArgs a = tsweep_args(c, dir, op);
+    a.fin = fin;
+    a.fout = fout;
+    a.xi = xi;
+    a.wrap = wrap;
+    a.accumulate = accumulate;
+    return op == OP_DERIV ? launch_sweep_t_op<OP_DERIV>(c, a) : launch_sweep_t_op<OP_FILTER>(c, a);
 }
 
 long n_interior(const astr_ctx* c)
@@ -2519,6 +2886,46 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
         hipMemcpy(c->fc, &fc, sizeof fc, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(ASTR_E_HIP, "coefficient upload failed"));
 
+    // transposed line systems and the closure blocks of B^T (the adjoint
+    // sweeps)
+    {
+        std::vector<double> tt;
+        std::vector<long> toff;
+        int nl[3][2], nh[3][2];
+        for (int d = 0; d < 3; ++d) {
+            const int nd = lines_ok[d] ? dims[d] : 16;
+            for (int op = 0; op < 2; ++op) {
+                HostTab h = build_tab_t(op, nts[d], nd, p->alfa_filter, c->L[d]);
+                std::vector<double> dlo, dhi;
+                build_trans_blocks(op, nts[d], h, fc, nl[d][op], nh[d][op], dlo, dhi);
+                toff.push_back((long)tt.size());
+                tt.insert(tt.end(), h.t.begin(), h.t.end());
+                toff.push_back((long)tt.size());
+                tt.insert(tt.end(), dlo.begin(), dlo.end());
+                toff.push_back((long)tt.size());
+                tt.insert(tt.end(), dhi.begin(), dhi.end());
+                LineTab& t = c->tab_t[d][op];
+                t = c->tab[d][op];
+                TransTab& b = c->tt[d][op];
+                b.nlo = nl[d][op];
+                b.nhi = nh[d][op];
+                b.lo0 = h.first - 5;
+                b.hi0 = h.last - 10;
+            }
+        }
+        if (hipMalloc(&c->tabs_t, sizeof(double) * tt.size()) != hipSuccess)
+            return cleanup(fail(ASTR_E_NOMEM, "table allocation failed"));
+        if (hipMemcpy(c->tabs_t, tt.data(), sizeof(double) * tt.size(), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return cleanup(fail(ASTR_E_HIP, "table copy failed"));
+        for (int d = 0, s = 0; d < 3; ++d)
+            for (int op = 0; op < 2; ++op, s += 3) {
+                c->tab_t[d][op].t = c->tabs_t + toff[s];
+                c->tt[d][op].dlo = c->tabs_t + toff[s + 1];
+                c->tt[d][op].dhi = c->tabs_t + toff[s + 2];
+            }
+    }
+
     c->xi_stride = std::max(LPL * LMAX, std::max(dims[0], std::max(dims[1], dims[2])) + 1 + LPL) +
                    2 * HM + 1;
     if (hipMalloc(&c->xi, sizeof(double) * 3 * c->xi_stride) != hipSuccess)
@@ -2543,6 +2950,7 @@ int astr_destroy(astr_ctx* c)
     (void)hipFree(c->base);
     (void)hipFree(c->tabs);
     (void)hipFree(c->fc);
+    (void)hipFree(c->tabs_t);
     (void)hipFree(c->xi);
     (void)hipFree(c->part);
     (void)hipFree(c->lshock);
@@ -3022,6 +3430,24 @@ int astr_deriv_field(astr_ctx* c, int32_t dir, const double* f, double* df)
 int astr_filter_field(astr_ctx* c, int32_t dir, const double* f, double* ff)
 {
     return field_op(c, dir, f, ff, PASS_FILTER_FIELD);
+}
+
+int astr_line_adjoint(astr_ctx* c, int32_t op, int32_t dir, int32_t wrap, const double* gbar,
+                      double* fbar)
+{
+    if (!c || !gbar || !fbar) return fail(ASTR_E_ARG, "null argument");
+    if (op != 0 && op != 1) return fail(ASTR_E_ARG, "op must be 0 (derivative) or 1 (filter)");
+    if (dir < 1 || dir > ndims_of(c)) return fail(ASTR_E_ARG, "dir must be 1..3 (1..2 in 2-D)");
+    if (op == 0 && c->p.deriv_explicit)
+        return fail(ASTR_E_UNSUPPORTED, "the transposed derivative is the compact one");
+    TRY(set_dev(c));
+    std::vector<double> stage;
+    TRY(upload_field(c, c->scratch, gbar, stage));
+    HIPCHK(hipMemsetAsync(c->scratch2, 0, sizeof(double) * c->nfield, c->stream));
+    TRY(launch_sweep_t(c, dir - 1, op == 0 ? OP_DERIV : OP_FILTER, c->scratch, c->scratch2, nullptr,
+                       wrap != 0, false));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return download_field(c, c->scratch2, fbar, stage, false);
 }
 
 }  // extern "C"

@@ -204,6 +204,9 @@ __device__ __forceinline__ void set_row(double (&b)[L], int r, int sub, double v
     }
 }
 
+template <int L>
+__device__ __forceinline__ void thomas_rows(const LineTab& tb, double (&x)[L]);
+
 // Solve the line held in LDS at ln (nodes -HM..dim+HM).  Called by a whole
 // wavefront; lanes 0-31 and 32-63 each solve their own line (same system),
 // lane sub of a half holds rows sub*L .. sub*L+L-1 in x[] on return.
@@ -215,10 +218,6 @@ __device__ __forceinline__ void solve_line(const double* ln, const LineTab& tb,
     constexpr int W = L + 2 * HW;
     const int lane = threadIdx.x & 63;
     const int sub = lane & 31;
-    const int S = LPL * L;
-    const double* __restrict__ ac1 = tb.t;
-    const double* __restrict__ ac2 = tb.t + S;
-    const double* __restrict__ ac3 = tb.t + 2 * S;
     const int r0 = sub * L;
     const int R = tb.R;
 
@@ -260,6 +259,22 @@ __device__ __forceinline__ void solve_line(const double* ln, const LineTab& tb,
         }
     }
 
+    thomas_rows<L>(tb, x);
+}
+
+// The pre-factored Thomas recurrences of tb on the rows a half-wavefront
+// holds in x (lane sub: rows sub*L .. sub*L+L-1), as a chunked scan.
+template <int L>
+__device__ __forceinline__ void thomas_rows(const LineTab& tb, double (&x)[L])
+{
+    const int lane = threadIdx.x & 63;
+    const int sub = lane & 31;
+    const int S = LPL * L;
+    const double* __restrict__ ac1 = tb.t;
+    const double* __restrict__ ac2 = tb.t + S;
+    const double* __restrict__ ac3 = tb.t + 2 * S;
+    const int r0 = sub * L;
+    const int R = tb.R;
     // forward elimination, chunk-local, with the chunk product g
     double prev = 0.0, g = 1.0;
     double gr[L];

@@ -139,6 +139,16 @@ class AstrSolver:
         check(self.lib.astr_set_spacing(self.ctx, float(dx), float(dy), float(dz)),
               "astr_set_spacing")
 
+    def line_adjoint(self, op: int, direction: int, gbar, wrap: bool = False) -> np.ndarray:
+        """The transpose of the compact derivative (op 0) or filter (op 1)
+        along direction (1..3) applied to gbar (haloed; nodes 0..n used):
+        the haloed adjoint of the operator's input (astr_line_adjoint)."""
+        g = self._f(gbar, haloed_shape(self.p))
+        out = np.zeros(haloed_shape(self.p), order="F")
+        check(self.lib.astr_line_adjoint(self.ctx, int(op), int(direction), int(bool(wrap)),
+                                         dptr(g), dptr(out)), "astr_line_adjoint")
+        return out
+
     def set_bc(self, bctype=(1, 1, 1, 1, 1, 1), udf: int = 0) -> None:
         """boucon's faces (i=0, i=im, j=0, j=jm, k=0, k=km): 1 nothing done,
         0 the user condition udf (UDF_RIEMANN2D: udf_bc of the 2-D Riemann

@@ -164,6 +164,16 @@ int astr_deriv_field(astr_ctx *ctx, int32_t dir, const double *f, double *df);
 /* compact_filter along axis dir for every line (src/filter.F90:112-285):
  * f(-hm:...) haloed in, ff(0:im,0:jm,0:km) out.  Halos used as given. */
 int astr_filter_field(astr_ctx *ctx, int32_t dir, const double *f, double *ff);
+/* The transpose of one of the two operators above (op 0: the compact
+ * derivative, 1: the filter) along dir: gbar(-hm:..) holds the adjoint of
+ * the output on 0..im etc. (halo entries are ignored), fbar(-hm:..)
+ * receives the adjoint of the input: f = B^T A^-T g per line.  wrap 1 folds
+ * the adjoint of the halos into the nodes they copy periodically (f(-k) =
+ * f(n-k), f(n+k) = f(k), as filterq's dataswap fills them) and leaves the
+ * halo entries zero.  No counterpart in the reference: the building block
+ * of the discrete adjoint. */
+int astr_line_adjoint(astr_ctx *ctx, int32_t op, int32_t dir, int32_t wrap, const double *gbar,
+                      double *fbar);
 
 /* Kernel time since timing was (re)enabled with astr_set_timing(ctx, 1),
  * in milliseconds, per kernel family (HIP events on the solver's stream):
