@@ -1113,6 +1113,326 @@ __global__ __launch_bounds__(256) void k_stats(PtArgs a)
         for (int c = 0; c < 3; ++c) a.part[3 * blockIdx.x + c] = sh[c][0];
 }
 
+// ------------------------------------------------------ discrete adjoint
+//
+// Reverse-mode kernels of one stage of the central path (filterq, qswap,
+// gradcal, the axis fluxes G, their derivatives, the RK3 combine and
+// updatefvar).  x-bar is the derivative of the objective with respect to
+// x.  The forward values each kernel needs (the state after qswap, the RK3
+// output, the gradients) come from the stage's stored or recomputed fields.
+
+struct AdjArgs {
+    Geo g;
+    const double* q[5];     // forward q after qswap (or of the state)
+    const double* prim[6];  // forward primitives after qswap
+    const double* qn[5];    // the stage's RK3 output
+    const double* dvel[9];
+    const double* dtmp[3];
+    const double* jac;
+    const double* xi[3];
+    double* qb[5];          // q-bar
+    double* pb[6];          // primitive-bar (rho, u, v, w, p, T)
+    double* db[9];          // dvel-bar
+    double* tb[3];          // dtmp-bar
+    double* gb[15];         // G-bar, 5d+m
+    double* rb[5];          // qrhs-bar
+    double* sb[5];          // qsave-bar
+    double const2, const5, const6, reynolds, prandtl, tempconst, tempconst1;
+    double rka, rkb, rkc, dt;
+    int stage, axis, visc, mode;
+    double scale;           // 1/(n0 n1 n2) of the statistic
+};
+
+// q-bar += (d q2fvar / d q)^T primitive-bar at one node, primitive-bar there
+// zeroed (the forward overwrote the primitives)
+__device__ __forceinline__ void q2fvar_adj(const AdjArgs& a, const double* q, long gi)
+{
+    const double rho = q[0];
+    const double u = q[1] / rho, v = q[2] / rho, w = q[3] / rho;
+    const double uu = u * u + v * v + w * w;
+    const double p = (q[4] - 0.5 * rho * uu) / a.const6;
+    double rb = a.pb[0][gi], ub = a.pb[1][gi], vb = a.pb[2][gi], wb = a.pb[3][gi];
+    double pbv = a.pb[4][gi];
+    const double tbv = a.pb[5][gi];
+    // T = p/rho*const2
+    pbv += tbv * a.const2 / rho;
+    rb -= tbv * p * a.const2 / (rho * rho);
+    // p = (q5 - rho |u|^2/2)/const6
+    const double q4b = pbv / a.const6;
+    rb -= pbv * 0.5 * uu / a.const6;
+    ub -= pbv * rho * u / a.const6;
+    vb -= pbv * rho * v / a.const6;
+    wb -= pbv * rho * w / a.const6;
+    // u = q2/rho, ...
+    a.qb[1][gi] += ub / rho;
+    a.qb[2][gi] += vb / rho;
+    a.qb[3][gi] += wb / rho;
+    rb -= (ub * u + vb * v + wb * w) / rho;
+    a.qb[0][gi] += rb;
+    a.qb[4][gi] += q4b;
+    for (int k = 0; k < 6; ++k) a.pb[k][gi] = 0.0;
+}
+
+// mode 0: updatefvar on the interior, at the RK3 output qn; mode 1: qswap's
+// refresh of the halo cells (one axis outside 0..n) and the seam nodes
+// (interior, some coordinate 0 or n), at q after qswap.  One thread per
+// node of the padded box (x over the (i,j) plane, y over k).
+__global__ __launch_bounds__(256) void k_q2f_adj(AdjArgs a)
+{
+    const unsigned pw = (unsigned)(a.g.n[0] + 1 + 2 * HM);
+    const unsigned ph = (unsigned)(a.g.n[1] + 1 + 2 * HM);
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= pw * ph) return;
+    const unsigned jj = t / pw;
+    const int c[3] = {(int)(t - jj * pw) - HM, (int)jj - HM, (int)blockIdx.y - HM};
+    int nout = 0;
+    bool seam = false;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        if (c[e] < 0 || c[e] > a.g.n[e]) ++nout;
+        else if (c[e] == 0 || c[e] == a.g.n[e]) seam = true;
+    }
+    if (a.mode == 0 ? nout != 0 : (nout > 1 || (nout == 0 && !seam))) return;
+    const long gi = gidx(a.g, c[0], c[1], c[2]);
+    double q[5];
+    const double* const* src = a.mode == 0 ? a.qn : a.q;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) q[m] = src[m][gi];
+    q2fvar_adj(a, q, gi);
+}
+
+// The RK3 combine reversed on the interior: q-bar holds the adjoint of the
+// stage output q' = (a qsave + b q J + c qrhs dt)/J; it becomes the adjoint
+// of q after qswap, qrhs-bar is set, and qsave-bar collects a/J q'-bar over
+// stages 2-3 and is spent at stage 1, where qsave = q J.
+__global__ void k_rk_adj(AdjArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
+    if (p >= ni * nj * (a.g.n[2] + 1)) return;
+    const long gi = gidx(a.g, (int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj)));
+    const double jc = a.jac[gi];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const double ob = a.qb[m][gi];
+        a.rb[m][gi] = a.rkc * a.dt * ob / jc;
+        double qv = a.rkb * ob;
+        if (a.stage == 1) {
+            qv += a.rka * ob + jc * a.sb[m][gi];
+            a.sb[m][gi] = 0.0;
+        } else {
+            a.sb[m][gi] += a.rka * ob / jc;
+        }
+        a.qb[m][gi] = qv;
+    }
+}
+
+// The inviscid part of G_d (convrsdcal6's fcs) reversed at one node: q-bar
+// and primitive-bar (u_d, p) collect -dF/d(.) G-bar
+__device__ __forceinline__ void inviscid_adj(const AdjArgs& a, int d, long gi, double xn,
+                                             const double* gbv)
+{
+    const double jc = a.jac[gi];
+    const double p = a.prim[4][gi];
+    const double ud = a.prim[1 + d][gi];
+    const double uu = xn * ud;
+    double ubar = 0.0, pbar = 0.0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const double g = -gbv[m];  // G = -F + (viscous)
+        const double qm = a.q[m][gi];
+        a.qb[m][gi] += g * jc * uu;
+        if (m == 4) {
+            ubar += g * jc * (qm + p);
+            pbar += g * jc * uu;
+        } else {
+            ubar += g * jc * qm;
+            if (m == d + 1) pbar += g * jc * xn;
+        }
+    }
+    a.pb[1 + d][gi] += ubar * xn;
+    a.pb[4][gi] += pbar;
+}
+
+// G on the halo strips of axis d: the inviscid part at the halo node itself
+// (its viscous part, taken at the periodic image, is reversed by
+// k_gflux_adj_int).  One thread per halo node, i fastest within a row.
+__global__ void k_gflux_adj_halo(AdjArgs a)
+{
+    const int d = a.axis;
+    const long ext[3] = {a.g.n[0] + 1, a.g.n[1] + 1, a.g.n[2] + 1};
+    const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= 2L * HM * ext[oa] * ext[ob]) return;
+    int c[3];
+    c[oa] = (int)(r % ext[oa]);
+    r /= ext[oa];
+    c[ob] = (int)(r % ext[ob]);
+    const int h = (int)(r / ext[ob]);
+    c[d] = h < HM ? -(h + 1) : a.g.n[d] + (h - HM + 1);
+    const long gi = gidx(a.g, c[0], c[1], c[2]);
+    double gbv[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) gbv[m] = a.gb[5 * d + m][gi];
+    inviscid_adj(a, d, gi, a.xi[d][c[d] + HM], gbv);
+}
+
+// G on the interior reversed at one node: the inviscid parts of its three
+// axis fluxes, and sigma and the heat flux, whose adjoint gathers the
+// node's own viscous terms and those of the halo nodes that took it as
+// their periodic image; then sigma -> dvel, T (through miucal), u and the
+// heat flux -> dtmp.
+__global__ void k_gflux_adj_int(AdjArgs a)
+{
+    static constexpr int sidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
+    if (p >= ni * nj * (a.g.n[2] + 1)) return;
+    const int c[3] = {(int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj))};
+    const long gi = gidx(a.g, c[0], c[1], c[2]);
+    const double jc = a.jac[gi];
+    double sgb[6] = {0, 0, 0, 0, 0, 0}, qfb[3] = {0, 0, 0};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const double xn = a.xi[e][c[e] + HM];
+        double gbv[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) gbv[m] = a.gb[5 * e + m][gi];
+        inviscid_adj(a, e, gi, xn, gbv);
+        if (a.visc) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) sgb[sidx[e][k]] += gbv[1 + k] * xn * jc;
+            qfb[e] += gbv[4] * xn * jc;
+            // the halo node of this axis whose image is this node
+            const int n = a.g.n[e];
+            int hc = 1 << 30;
+            if (c[e] >= 1 && c[e] <= HM) hc = n + c[e];
+            else if (c[e] >= n - HM && c[e] <= n - 1) hc = c[e] - n;
+            if (hc != (1 << 30)) {
+                int w[3] = {c[0], c[1], c[2]};
+                w[e] = hc;
+                const long gh = gidx(a.g, w[0], w[1], w[2]);
+                const double xh = a.xi[e][hc + HM], jh = a.jac[gh];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) sgb[sidx[e][k]] += a.gb[5 * e + 1 + k][gh] * xh * jh;
+                qfb[e] += a.gb[5 * e + 4][gh] * xh * jh;
+            }
+        }
+    }
+    if (!a.visc) return;
+    // forward sigma and heat flux at this node
+    double dv[9], dt[3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dv[k] = a.dvel[k][gi];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dt[k] = a.dtmp[k][gi];
+    const double T = a.prim[5][gi];
+    const double st = sqrt(T);
+    const double mu0 = T * st * a.tempconst1 / (T + a.tempconst);
+    const double miu = mu0 / a.reynolds;
+    const double s11 = dv[0], s22 = dv[4], s33 = dv[8];
+    const double s12 = 0.5 * (dv[1] + dv[3]), s13 = 0.5 * (dv[2] + dv[6]), s23 = 0.5 * (dv[5] + dv[7]);
+    const double skk = (1.0 / 3.0) * (s11 + s22 + s33);
+    const double miu2 = 2.0 * miu;
+    const double hcc = (miu / a.prandtl) / a.const5;
+    const double g[6] = {miu2 * (s11 - skk), miu2 * s12, miu2 * s13, miu2 * (s22 - skk), miu2 * s23,
+                         miu2 * (s33 - skk)};
+    const double uv[3] = {a.prim[1][gi], a.prim[2][gi], a.prim[3][gi]};
+    // qf_d = hcc dtmp_d + sum_c sigma(d,c) u_c
+    double hccb = 0.0, ubv[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        a.tb[d][gi] += hcc * qfb[d];
+        hccb += qfb[d] * dt[d];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            sgb[sidx[d][k]] += qfb[d] * uv[k];
+            ubv[k] += qfb[d] * g[sidx[d][k]];
+        }
+    }
+    for (int k = 0; k < 3; ++k) a.pb[1 + k][gi] += ubv[k];
+    // sigma = 2 mu (S - skk I)
+    const double miu2b = sgb[0] * (s11 - skk) + sgb[1] * s12 + sgb[2] * s13 +
+                         sgb[3] * (s22 - skk) + sgb[4] * s23 + sgb[5] * (s33 - skk);
+    const double skkb = -miu2 * (sgb[0] + sgb[3] + sgb[5]);
+    const double s11b = miu2 * sgb[0] + skkb / 3.0;
+    const double s22b = miu2 * sgb[3] + skkb / 3.0;
+    const double s33b = miu2 * sgb[5] + skkb / 3.0;
+    const double s12b = miu2 * sgb[1], s13b = miu2 * sgb[2], s23b = miu2 * sgb[4];
+    a.db[0][gi] += s11b;
+    a.db[4][gi] += s22b;
+    a.db[8][gi] += s33b;
+    a.db[1][gi] += 0.5 * s12b;
+    a.db[3][gi] += 0.5 * s12b;
+    a.db[2][gi] += 0.5 * s13b;
+    a.db[6][gi] += 0.5 * s13b;
+    a.db[5][gi] += 0.5 * s23b;
+    a.db[7][gi] += 0.5 * s23b;
+    // miu = miucal(T)/Re, hcc = miu/Pr/const5
+    const double miub = 2.0 * miu2b + hccb / a.prandtl / a.const5;
+    const double dmu = a.tempconst1 * st * (0.5 * T + 1.5 * a.tempconst) /
+                       ((T + a.tempconst) * (T + a.tempconst));
+    a.pb[5][gi] += miub * dmu / a.reynolds;
+}
+
+// qswap's copies and averages along a.axis reversed, for the lines of that
+// axis over 0..n of the other two: the average q(0) = q(n) = (q(0)+q(n))/2,
+// then the halo copies q(-k) = q(n-k), q(n+k) = q(k).  Run for the axes in
+// the order z, y, x.
+__global__ void k_qswap_adj_line(AdjArgs a)
+{
+    const int d = a.axis;
+    const long ext[3] = {a.g.n[0] + 1, a.g.n[1] + 1, a.g.n[2] + 1};
+    const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ext[oa] * ext[ob]) return;
+    int c[3];
+    c[oa] = (int)(r % ext[oa]);
+    c[ob] = (int)(r / ext[oa]);
+    c[d] = 0;
+    const long h0 = gidx(a.g, c[0], c[1], c[2]);
+    const long sd = a.g.s[d];
+    const int n = a.g.n[d];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        double* f = a.qb[m] + h0;
+        const double t = f[0] + f[n * sd];
+        f[0] = 0.5 * t;
+        f[n * sd] = 0.5 * t;
+        for (int k = 1; k <= HM; ++k) {
+            f[(n - k) * sd] += f[-k * sd];
+            f[-k * sd] = 0.0;
+            f[k * sd] += f[(n + k) * sd];
+            f[(n + k) * sd] = 0.0;
+        }
+    }
+}
+
+// The enstrophy statistic (k_stats, 1/2 sum over nodes 1..n of rho |omega|^2
+// times scale) reversed: rho-bar and dvel-bar at the nodes 1..n
+__global__ void k_ens_adj(AdjArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n0 = a.g.n[0], n1 = a.g.n[1];
+    if (p >= n0 * n1 * a.g.n[2]) return;
+    const int i = 1 + (int)(p % n0);
+    const long r = p / n0;
+    const int j = 1 + (int)(r % n1), k = 1 + (int)(r / n1);
+    const long gi = gidx(a.g, i, j, k);
+    const double rho = a.prim[0][gi];
+    const double d12 = a.dvel[1][gi], d13 = a.dvel[2][gi], d21 = a.dvel[3][gi];
+    const double d23 = a.dvel[5][gi], d31 = a.dvel[6][gi], d32 = a.dvel[7][gi];
+    const double o1 = d32 - d23, o2 = d13 - d31, o3 = d21 - d12;
+    a.pb[0][gi] += 0.5 * a.scale * (o1 * o1 + o2 * o2 + o3 * o3);
+    const double c = rho * a.scale;
+    a.db[7][gi] += c * o1;
+    a.db[5][gi] -= c * o1;
+    a.db[2][gi] += c * o2;
+    a.db[6][gi] -= c * o2;
+    a.db[3][gi] += c * o3;
+    a.db[1][gi] -= c * o3;
+}
+
 // ------------------------------------------------------- upwind convection
 //
 // Contraction is off here as in upwind.hpp: the explicit gradient, the
@@ -1795,6 +2115,8 @@ struct astr_ctx {
     long xi_stride = 0;
     double* tabs = nullptr;
     LineTab tab[3][2];
+    double* adj_base = nullptr; // discrete adjoint workspace (fields)
+    long adj_nf = 0;            // fields it holds
     double* tabs_t = nullptr;   // transposed systems and closure corrections
     LineTab tab_t[3][2];
     TransTab tt[3][2];
@@ -2702,6 +3024,237 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
     return ASTR_OK;
 }
 
+// ---- discrete adjoint of the central path (host)
+
+// adjoint fields: q 5, primitives 6, dvel 9, dtmp 3, G 15, qrhs 5, qsave 5
+constexpr int ADJ_FIELDS = 48;
+// what a stage stores for its reversal: q and the primitives after qswap,
+// the RK3 output
+constexpr int STAGE_FIELDS = 16;
+constexpr int STATE_FIELDS = 11;  // q and the primitives
+
+struct AdjFields {
+    double *qb[5], *pb[6], *db[9], *tb[3], *gb[15], *rb[5], *sb[5];
+};
+struct StageStore {
+    double *q[5], *prim[6], *qn[5];
+};
+
+double* adj_field(astr_ctx* c, long k) { return c->adj_base + k * c->nfield; }
+
+AdjFields adj_fields(astr_ctx* c)
+{
+    AdjFields f;
+    long k = 0;
+    for (int m = 0; m < 5; ++m) f.qb[m] = adj_field(c, k++);
+    for (int m = 0; m < 6; ++m) f.pb[m] = adj_field(c, k++);
+    for (int m = 0; m < 9; ++m) f.db[m] = adj_field(c, k++);
+    for (int m = 0; m < 3; ++m) f.tb[m] = adj_field(c, k++);
+    for (int m = 0; m < 15; ++m) f.gb[m] = adj_field(c, k++);
+    for (int m = 0; m < 5; ++m) f.rb[m] = adj_field(c, k++);
+    for (int m = 0; m < 5; ++m) f.sb[m] = adj_field(c, k++);
+    return f;
+}
+
+// a saved state (q, primitives) k of the checkpoint area
+void state_slot(astr_ctx* c, long k, double** q, double** prim)
+{
+    const long b = ADJ_FIELDS + k * STATE_FIELDS;
+    for (int m = 0; m < 5; ++m) q[m] = adj_field(c, b + m);
+    for (int m = 0; m < 6; ++m) prim[m] = adj_field(c, b + 5 + m);
+}
+
+StageStore stage_slot(astr_ctx* c, long nstates, long k)
+{
+    StageStore st;
+    const long b = ADJ_FIELDS + nstates * STATE_FIELDS + k * STAGE_FIELDS;
+    for (int m = 0; m < 5; ++m) st.q[m] = adj_field(c, b + m);
+    for (int m = 0; m < 6; ++m) st.prim[m] = adj_field(c, b + 5 + m);
+    for (int m = 0; m < 5; ++m) st.qn[m] = adj_field(c, b + 11 + m);
+    return st;
+}
+
+int copy_fields(astr_ctx* c, double* const* dst, double* const* src, int n)
+{
+    for (int m = 0; m < n; ++m)
+        HIPCHK(hipMemcpyAsync(dst[m], src[m], sizeof(double) * c->nfield, hipMemcpyDeviceToDevice,
+                              c->stream));
+    return ASTR_OK;
+}
+
+int save_state(astr_ctx* c, long k)
+{
+    double *q[5], *pr[6];
+    state_slot(c, k, q, pr);
+    TRY(copy_fields(c, q, c->q, 5));
+    return copy_fields(c, pr, c->prim, 6);
+}
+
+int load_state(astr_ctx* c, long k)
+{
+    double *q[5], *pr[6];
+    state_slot(c, k, q, pr);
+    TRY(copy_fields(c, c->q, q, 5));
+    return copy_fields(c, c->prim, pr, 6);
+}
+
+AdjArgs adj_args(astr_ctx* c, double* const* q, double* const* prim, double* const* qn)
+{
+    AdjArgs a{};
+    a.g = c->g;
+    const AdjFields f = adj_fields(c);
+    for (int m = 0; m < 5; ++m) {
+        a.q[m] = q[m];
+        a.qn[m] = qn ? qn[m] : nullptr;
+        a.qb[m] = f.qb[m];
+        a.rb[m] = f.rb[m];
+        a.sb[m] = f.sb[m];
+    }
+    for (int m = 0; m < 6; ++m) { a.prim[m] = prim[m]; a.pb[m] = f.pb[m]; }
+    for (int m = 0; m < 9; ++m) { a.dvel[m] = c->dvel[m]; a.db[m] = f.db[m]; }
+    for (int m = 0; m < 3; ++m) { a.dtmp[m] = c->dtmp[m]; a.tb[m] = f.tb[m]; }
+    for (int m = 0; m < 15; ++m) a.gb[m] = f.gb[m];
+    a.jac = c->jac;
+    for (int d = 0; d < 3; ++d) a.xi[d] = c->xi + d * c->xi_stride;
+    a.const2 = c->const2; a.const5 = c->const5; a.const6 = c->const6;
+    a.reynolds = c->p.reynolds; a.prandtl = c->p.prandtl;
+    a.tempconst = c->tempconst; a.tempconst1 = c->tempconst1;
+    a.visc = c->p.diffterm;
+    a.scale = 1.0 / ((double)c->g.n[0] * (double)c->g.n[1] * (double)c->g.n[2]);
+    return a;
+}
+
+dim3 padded_grid(const astr_ctx* c)
+{
+    const long plane = (long)(c->g.n[0] + 1 + 2 * HM) * (c->g.n[1] + 1 + 2 * HM);
+    return dim3((unsigned)((plane + 255) / 256), (unsigned)(c->g.n[2] + 1 + 2 * HM));
+}
+
+template <class K>
+int launch_adj(astr_ctx* c, K kernel, long n, const AdjArgs& a)
+{
+    if (n <= 0) return ASTR_OK;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
+}
+
+// gradcal of the given primitives into dvel and dtmp
+int grad_of(astr_ctx* c, double* const* prim)
+{
+    for (int d = 0; d < 3; ++d) {
+        SweepArgs a = sweep_args(c, d, PASS_GRAD);
+        for (int m = 0; m < 6; ++m) a.prim[m] = prim[m];
+        TRY(launch_sweep(c, a));
+    }
+    return ASTR_OK;
+}
+
+// one stage of the central path, as do_step runs it, storing what its
+// reversal needs
+int stage_fwd(astr_ctx* c, int s, double dt, const StageStore* st)
+{
+    if (c->p.lfilter) TRY(do_filterq(c));
+    TRY(do_qswap(c));
+    if (st) {
+        TRY(copy_fields(c, st->q, c->q, 5));
+        TRY(copy_fields(c, st->prim, c->prim, 6));
+    }
+    TRY(do_grad(c));
+    TRY(do_gflux(c));
+    TRY(do_flux_sweeps(c, s, dt));
+    TRY(launch_pt(c, k_prim, n_interior(c), pt_args(c)));
+    if (st) TRY(copy_fields(c, st->qn, c->q, 5));
+    return ASTR_OK;
+}
+
+// gradcal reversed: primitive-bar (u, v, w, T; halos too) += D_d^T (xi_d
+// dvel-bar(.,d)); dvel-bar and dtmp-bar are spent
+int grad_adj(astr_ctx* c)
+{
+    const AdjFields f = adj_fields(c);
+    for (int d = 0; d < 3; ++d) {
+        const double* xi = c->xi + d * c->xi_stride;
+        for (int m = 0; m < 3; ++m)
+            TRY(launch_sweep_t(c, d, OP_DERIV, f.db[3 * m + d], f.pb[1 + m], xi, false, true));
+        TRY(launch_sweep_t(c, d, OP_DERIV, f.tb[d], f.pb[5], xi, false, true));
+    }
+    HIPCHK(hipMemsetAsync(f.db[0], 0, sizeof(double) * c->nfield * 12, c->stream));
+    return ASTR_OK;
+}
+
+// qswap reversed at the state q after it: the primitive refresh of the
+// halo and seam nodes, then the copies and averages in the order z, y, x
+int qswap_adj(astr_ctx* c, AdjArgs a)
+{
+    a.mode = 1;
+    hipLaunchKernelGGL(k_q2f_adj, padded_grid(c), dim3(256), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    const long ext[3] = {c->g.n[0] + 1L, c->g.n[1] + 1L, c->g.n[2] + 1L};
+    for (int d = 2; d >= 0; --d) {
+        a.axis = d;
+        const long nl = d == 0 ? ext[1] * ext[2] : d == 1 ? ext[0] * ext[2] : ext[0] * ext[1];
+        TRY(launch_adj(c, k_qswap_adj_line, nl, a));
+    }
+    return ASTR_OK;
+}
+
+// one stage reversed: q-bar and primitive-bar hold the adjoint of the
+// stage's outputs on entry and of its inputs on return
+int stage_adj(astr_ctx* c, const StageStore& st, int s, double dt)
+{
+    AdjArgs a = adj_args(c, st.q, st.prim, st.qn);
+    a.stage = s;
+    a.rka = RKC[s - 1][0];
+    a.rkb = RKC[s - 1][1];
+    a.rkc = RKC[s - 1][2];
+    a.dt = dt;
+    const AdjFields f = adj_fields(c);
+    // updatefvar, the RK3 combine
+    a.mode = 0;
+    hipLaunchKernelGGL(k_q2f_adj, padded_grid(c), dim3(256), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    TRY(launch_adj(c, k_rk_adj, n_interior(c), a));
+    // the flux derivatives: G-bar_d = D_d^T qrhs-bar
+    for (int d = 0; d < 3; ++d)
+        for (int m = 0; m < 5; ++m)
+            TRY(launch_sweep_t(c, d, OP_DERIV, f.rb[m], f.gb[5 * d + m], nullptr, false, false));
+    // the axis fluxes, at the gradients of the stored primitives
+    TRY(grad_of(c, st.prim));
+    TRY(launch_adj(c, k_gflux_adj_int, n_interior(c), a));
+    const long ext[3] = {c->g.n[0] + 1L, c->g.n[1] + 1L, c->g.n[2] + 1L};
+    for (int d = 0; d < 3; ++d) {
+        a.axis = d;
+        const long nh = 2L * HM * (d == 0 ? ext[1] * ext[2] : d == 1 ? ext[0] * ext[2] : ext[0] * ext[1]);
+        TRY(launch_adj(c, k_gflux_adj_halo, nh, a));
+    }
+    TRY(grad_adj(c));
+    TRY(qswap_adj(c, a));
+    // filterq: the filters of z, y, x transposed, in place
+    if (c->p.lfilter)
+        for (int d = 2; d >= 0; --d)
+            for (int m = 0; m < 5; ++m)
+                TRY(launch_sweep_t(c, d, OP_FILTER, f.qb[m], f.qb[m], nullptr, true, false));
+    return ASTR_OK;
+}
+
+// the enstrophy of the state after a qswap, and its adjoint in q-bar and
+// primitive-bar (the state is left after the qswap)
+int objective_adj(astr_ctx* c, double* J)
+{
+    TRY(do_qswap(c));
+    TRY(do_grad(c));
+    double st[3];
+    TRY(do_stats(c, st));
+    *J = st[1];
+    const AdjFields f = adj_fields(c);
+    HIPCHK(hipMemsetAsync(f.qb[0], 0, sizeof(double) * c->nfield * ADJ_FIELDS, c->stream));
+    AdjArgs a = adj_args(c, c->q, c->prim, nullptr);
+    TRY(launch_adj(c, k_ens_adj, (long)c->g.n[0] * c->g.n[1] * c->g.n[2], a));
+    TRY(grad_adj(c));
+    return qswap_adj(c, a);
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -2951,6 +3504,7 @@ int astr_destroy(astr_ctx* c)
     (void)hipFree(c->tabs);
     (void)hipFree(c->fc);
     (void)hipFree(c->tabs_t);
+    (void)hipFree(c->adj_base);
     (void)hipFree(c->xi);
     (void)hipFree(c->part);
     (void)hipFree(c->lshock);
@@ -3430,6 +3984,77 @@ int astr_deriv_field(astr_ctx* c, int32_t dir, const double* f, double* df)
 int astr_filter_field(astr_ctx* c, int32_t dir, const double* f, double* ff)
 {
     return field_op(c, dir, f, ff, PASS_FILTER_FIELD);
+}
+
+int astr_adjoint_enstrophy(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt, double* J,
+                           double* qbar, double* pbar, double* ms2)
+{
+    TRY(ready(c));
+    if (!J || !qbar || !pbar) return fail(ASTR_E_ARG, "null argument");
+    if (nsteps < 0 || ckpt < 1) return fail(ASTR_E_ARG, "nsteps >= 0 and ckpt >= 1 required");
+    if (c->p.convection != 0 || c->p.variant != 0 || c->g.n[2] == 0)
+        return fail(ASTR_E_UNSUPPORTED, "the adjoint is built for the central path on periodic 3-D boxes");
+    if (c->udf) return fail(ASTR_E_UNSUPPORTED, "the adjoint takes no user boundary condition");
+    const long nseg = nsteps > 0 ? (nsteps + ckpt - 1) / ckpt : 0;
+    const long nstates = nseg + 1;  // the segment starts and the final state
+    const long seglen = std::min<long>(ckpt, std::max(nsteps, 1));
+    const long need = ADJ_FIELDS + nstates * STATE_FIELDS + 3 * seglen * STAGE_FIELDS;
+    if (need > c->adj_nf) {
+        (void)hipFree(c->adj_base);
+        c->adj_base = nullptr;
+        c->adj_nf = 0;
+        if (hipMalloc(&c->adj_base, sizeof(double) * c->nfield * need) != hipSuccess)
+            return fail(ASTR_E_NOMEM, "device allocation of the adjoint workspace failed");
+        c->adj_nf = need;
+    }
+    HIPCHK(hipEventRecord(c->ev[12], c->stream));
+    // forward, keeping the start of every segment
+    for (long t = 0; t < nsteps; ++t) {
+        if (t % ckpt == 0) TRY(save_state(c, t / ckpt));
+        TRY(do_step(c, dt, nullptr, false));
+    }
+    TRY(save_state(c, nseg));
+    TRY(objective_adj(c, J));
+    HIPCHK(hipEventRecord(c->ev[13], c->stream));
+    // reverse, segment by segment: recompute the segment's stages, storing
+    // what their reversal needs, then reverse them
+    for (long seg = nseg - 1; seg >= 0; --seg) {
+        const long t0 = seg * ckpt, t1 = std::min<long>(nsteps, t0 + ckpt);
+        TRY(load_state(c, seg));
+        for (long t = t0; t < t1; ++t)
+            for (int s = 1; s <= 3; ++s) {
+                const StageStore st = stage_slot(c, nstates, 3 * (t - t0) + (s - 1));
+                TRY(stage_fwd(c, s, dt, &st));
+            }
+        for (long t = t1 - 1; t >= t0; --t)
+            for (int s = 3; s >= 1; --s)
+                TRY(stage_adj(c, stage_slot(c, nstates, 3 * (t - t0) + (s - 1)), s, dt));
+    }
+    HIPCHK(hipEventRecord(c->ev[14], c->stream));
+    HIPCHK(hipEventSynchronize(c->ev[14]));
+    if (ms2) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, c->ev[12], c->ev[13]));
+        HIPCHK(hipEventElapsedTime(&b, c->ev[13], c->ev[14]));
+        ms2[0] = a;
+        ms2[1] = b;
+    }
+    // the adjoint of the initial state, haloed host layout
+    const AdjFields f = adj_fields(c);
+    std::vector<double> stage, one((size_t)n_haloed(c));
+    const long nh = n_haloed(c);
+    for (int m = 0; m < 5; ++m) {
+        TRY(download_field(c, f.qb[m], one.data(), stage, false));
+        memcpy(qbar + m * nh, one.data(), sizeof(double) * nh);
+    }
+    for (int m = 0; m < 6; ++m) {
+        TRY(download_field(c, f.pb[m], one.data(), stage, false));
+        memcpy(pbar + m * nh, one.data(), sizeof(double) * nh);
+    }
+    // leave the device state at the end of the forward steps
+    TRY(load_state(c, nseg));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ASTR_OK;
 }
 
 int astr_line_adjoint(astr_ctx* c, int32_t op, int32_t dir, int32_t wrap, const double* gbar,

@@ -149,6 +149,19 @@ class AstrSolver:
                                          dptr(g), dptr(out)), "astr_line_adjoint")
         return out
 
+    def adjoint_enstrophy(self, nsteps: int, dt: float, ckpt: int = 8):
+        """(J, qbar, pbar, ms): the enstrophy after nsteps RK3 steps from the
+        current state and its gradient with respect to that state
+        (astr_adjoint_enstrophy); ms = (forward, reversal) milliseconds."""
+        J = np.zeros(1)
+        qb = np.zeros(haloed_shape(self.p, (5,)), order="F")
+        pb = np.zeros(haloed_shape(self.p, (6,)), order="F")
+        ms = np.zeros(2)
+        check(self.lib.astr_adjoint_enstrophy(self.ctx, int(nsteps), float(dt), int(ckpt),
+                                              dptr(J), dptr(qb), dptr(pb), dptr(ms)),
+              "astr_adjoint_enstrophy")
+        return float(J[0]), qb, pb, (float(ms[0]), float(ms[1]))
+
     def set_bc(self, bctype=(1, 1, 1, 1, 1, 1), udf: int = 0) -> None:
         """boucon's faces (i=0, i=im, j=0, j=jm, k=0, k=km): 1 nothing done,
         0 the user condition udf (UDF_RIEMANN2D: udf_bc of the 2-D Riemann

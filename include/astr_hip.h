@@ -175,6 +175,31 @@ int astr_filter_field(astr_ctx *ctx, int32_t dir, const double *f, double *ff);
 int astr_line_adjoint(astr_ctx *ctx, int32_t op, int32_t dir, int32_t wrap, const double *gbar,
                       double *fbar);
 
+/* --- discrete adjoint ---------------------------------------------------- */
+
+/* Gradient of the enstrophy after nsteps RK3 steps with respect to the
+ * current (uploaded) state.  The steps are the plain forward (the central
+ * path on a periodic box, filter and viscous terms as configured).  J is the
+ * enstrophy statistic of the final state after a qswap (enstophycal,
+ * src/statistic.F90:870-925: 1/2 mean over the nodes 1..n of rho |omega|^2).
+ * qbar(-hm:..,1:5) and pbar(-hm:..,1:6) receive dJ/dq and dJ/d(rho, u, v,
+ * w, p, T) of the initial state.  The interior primitives are their own
+ * inputs, because the filter leaves them stale.  Hand-written reverse
+ * kernels of every stage piece:
+ *   - the transposed compact solves of the filter, gradient and flux
+ *     sweeps;
+ *   - the Jacobian-transposes of the axis fluxes and of q2fvar;
+ *   - qswap's copies and averages reversed;
+ *   - the RK3 combine reversed.
+ * The steps are reversed segment by segment: the state is kept every ckpt
+ * steps, and a segment's stages are recomputed from it before they are
+ * reversed.  ms2 (nullable) receives the milliseconds of the forward
+ * (with the objective) and of the reversal.  The device state is left at
+ * the end of the forward steps, with the time advanced.  No counterpart in
+ * the reference. */
+int astr_adjoint_enstrophy(astr_ctx *ctx, int32_t nsteps, double dt, int32_t ckpt, double *J,
+                           double *qbar, double *pbar, double *ms2);
+
 /* Kernel time since timing was (re)enabled with astr_set_timing(ctx, 1),
  * in milliseconds, per kernel family (HIP events on the solver's stream):
  * [0] filter sweeps, [1] boucon and qswap, [2] gradient, [3] central:

@@ -227,8 +227,21 @@ class Box:
         n3 = float(self.n[0] * self.n[1] * self.n[2])
         return 0.5 * torch.sum(rho * (o1 * o1 + o2 * o2 + o3 * o3)) / n3
 
-    def objective(self, u0, nsteps, dt):
-        Q, P = self.initial(u0)
+    def objective_state(self, Q, P, nsteps, dt):
         for _ in range(nsteps):
             Q, P = self.step(Q, P, dt)
         return self.enstrophy(Q, P)
+
+    def objective(self, u0, nsteps, dt):
+        Q, P = self.initial(u0)
+        return self.objective_state(Q, P, nsteps, dt)
+
+
+def to_host(Q, P):
+    """Torch (field, i, j, k) state -> the haloed host arrays of
+    AstrSolver.upload_state."""
+    q = np.asfortranarray(np.moveaxis(Q.detach().numpy(), 0, -1))
+    pr = P.detach().numpy()
+    return {"q": q, "rho": np.asfortranarray(pr[0]),
+            "vel": np.asfortranarray(np.moveaxis(pr[1:4], 0, -1)),
+            "prs": np.asfortranarray(pr[4]), "tmp": np.asfortranarray(pr[5])}
