@@ -438,8 +438,9 @@ struct TSweepArgs {
     TransTab tt;
     const FiltCoef* fc;
     int na, nb, ntile_a, rs_in;
-    const double* fin;  // adjoint of the forward output, nodes 0..dim of a line
-    double* fout;       // adjoint of the forward input
+    int nq;              // quantities of this launch
+    const double* fin[5];  // adjoint of the forward output, nodes 0..dim of a line
+    double* fout[5];       // adjoint of the forward input
     const double* xi;   // when set, fin is scaled by xi[n+HM] first (gradcal's metric)
     int wrap;           // 1: the forward read its halos as periodic copies of the
                         // nodes (dataswap): fold their adjoint into 0..dim;
@@ -453,6 +454,19 @@ template <int OP>
 __device__ __forceinline__ double bt_column(const double* ln, int j, int dim, const LineTab& tb,
                                             const TransTab& tt, const FiltCoef* __restrict__ fc)
 {
+    constexpr int HW = OP == OP_DERIV ? 2 : 5;
+    if (j - HW >= -HM && j + HW <= dim + HM && j >= tt.lo0 + TC && j < tt.hi0) {
+        // away from both ends: the interior stencil, no closure rows
+        const double* w = ln + j + HM;
+        if constexpr (OP == OP_DERIV) {
+            return (7.0 / 9.0) * (w[-1] - w[1]) + (1.0 / 36.0) * (w[-2] - w[2]);
+        } else {
+            double v = fc->c10[0] * (w[0] + w[0]);
+#pragma unroll
+            for (int k = 1; k <= 5; ++k) v += fc->c10[k] * (w[-k] + w[k]);
+            return v;
+        }
+    }
     auto W = [&](int n) { return (n >= -HM && n <= dim + HM) ? ln[n + HM] : 0.0; };
     double v;
     if constexpr (OP == OP_DERIV) {
@@ -471,9 +485,10 @@ __device__ __forceinline__ double bt_column(const double* ln, int j, int dim, co
     return v;
 }
 
-// One workgroup per tile of lines (the forward sweep's tiles and staging
-// map): stage g, solve A^T per half-wavefront, form B^T w in place, write
-// the adjoint of the line's input.
+// One workgroup per tile of lines and quantity (the forward sweep's tiles,
+// staging map and XCD-grouped block ids): stage g, solve A^T per
+// half-wavefront, form B^T w in place, write the adjoint of the line's
+// input.
 template <int L, int OP, bool XD>
 __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
 {
@@ -481,8 +496,11 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
     const int dir = XD ? 0 : a.dir;
     constexpr int tx = XD ? TXX : TXY;
     constexpr int EMAX = 2 * L + 1;
-    const int tile_id = blockIdx.x;
-    if (tile_id >= a.ntile_a * a.nb) return;
+    int tile_id, m;
+    block_tile(blockIdx.x, a.nq, a.ntile_a * a.nb, tile_id, m);
+    if (tile_id >= a.ntile_a * a.nb) return;  // padding blocks of the last group
+    const double* const fin = a.fin[m];
+    double* const fout = a.fout[m];
     const int ta = tile_id % a.ntile_a;
     const int b = tile_id / a.ntile_a;
     const int a0 = ta * tx;
@@ -524,7 +542,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
             const int node = n - HM;
             double v = 0.0;
             if (node >= 0 && node <= dim) {
-                v = ldb(a.fin, lob + (unsigned int)node * sdb);
+                v = ldb(fin, lob + (unsigned int)node * sdb);
                 if (a.xi) v *= a.xi[node + HM];
             }
             trow[n] = v;
@@ -597,8 +615,8 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
                 v = trow[n];
             }
             const unsigned int od = lob + (unsigned int)node * sdb;
-            if (a.accumulate) v += ldb(a.fout, od);
-            stb(a.fout, od, v);
+            if (a.accumulate) v += ldb(fout, od);
+            stb(fout, od, v);
         }
     }
 }
@@ -1264,11 +1282,20 @@ __global__ void k_gflux_adj_halo(AdjArgs a)
     const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= 2L * HM * ext[oa] * ext[ob]) return;
-    int c[3];
-    c[oa] = (int)(r % ext[oa]);
-    r /= ext[oa];
-    c[ob] = (int)(r % ext[ob]);
-    const int h = (int)(r / ext[ob]);
+    int c[3], h;
+    if (d == 0) {
+        // x strips: the 2 HM halo nodes of a row are the contiguous ones
+        h = (int)(r % (2 * HM));
+        r /= 2 * HM;
+        c[oa] = (int)(r % ext[oa]);
+        c[ob] = (int)(r / ext[oa]);
+    } else {
+        // y/z strips: consecutive i
+        c[oa] = (int)(r % ext[oa]);
+        r /= ext[oa];
+        c[ob] = (int)(r % ext[ob]);
+        h = (int)(r / ext[ob]);
+    }
     c[d] = h < HM ? -(h + 1) : a.g.n[d] + (h - HM + 1);
     const long gi = gidx(a.g, c[0], c[1], c[2]);
     double gbv[5];
@@ -1342,7 +1369,7 @@ __global__ void k_gflux_adj_int(AdjArgs a)
     double hccb = 0.0, ubv[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        a.tb[d][gi] += hcc * qfb[d];
+        a.tb[d][gi] = hcc * qfb[d];
         hccb += qfb[d] * dt[d];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -1350,7 +1377,7 @@ __global__ void k_gflux_adj_int(AdjArgs a)
             ubv[k] += qfb[d] * g[sidx[d][k]];
         }
     }
-    for (int k = 0; k < 3; ++k) a.pb[1 + k][gi] += ubv[k];
+    for (int k = 0; k < 3; ++k) a.pb[1 + k][gi] += ubv[k];  // + inviscid_adj's share
     // sigma = 2 mu (S - skk I)
     const double miu2b = sgb[0] * (s11 - skk) + sgb[1] * s12 + sgb[2] * s13 +
                          sgb[3] * (s22 - skk) + sgb[4] * s23 + sgb[5] * (s33 - skk);
@@ -1359,15 +1386,17 @@ __global__ void k_gflux_adj_int(AdjArgs a)
     const double s22b = miu2 * sgb[3] + skkb / 3.0;
     const double s33b = miu2 * sgb[5] + skkb / 3.0;
     const double s12b = miu2 * sgb[1], s13b = miu2 * sgb[2], s23b = miu2 * sgb[4];
-    a.db[0][gi] += s11b;
-    a.db[4][gi] += s22b;
-    a.db[8][gi] += s33b;
-    a.db[1][gi] += 0.5 * s12b;
-    a.db[3][gi] += 0.5 * s12b;
-    a.db[2][gi] += 0.5 * s13b;
-    a.db[6][gi] += 0.5 * s13b;
-    a.db[5][gi] += 0.5 * s23b;
-    a.db[7][gi] += 0.5 * s23b;
+    // dvel-bar and dtmp-bar are zero on entry (spent by the previous
+    // gradient reversal): stored, not accumulated
+    a.db[0][gi] = s11b;
+    a.db[4][gi] = s22b;
+    a.db[8][gi] = s33b;
+    a.db[1][gi] = 0.5 * s12b;
+    a.db[3][gi] = 0.5 * s12b;
+    a.db[2][gi] = 0.5 * s13b;
+    a.db[6][gi] = 0.5 * s13b;
+    a.db[5][gi] = 0.5 * s23b;
+    a.db[7][gi] = 0.5 * s23b;
     // miu = miucal(T)/Re, hcc = miu/Pr/const5
     const double miub = 2.0 * miu2b + hccb / a.prandtl / a.const5;
     const double dmu = a.tempconst1 * st * (0.5 * T + 1.5 * a.tempconst) /
@@ -1384,8 +1413,10 @@ __global__ void k_qswap_adj_line(AdjArgs a)
     const int d = a.axis;
     const long ext[3] = {a.g.n[0] + 1, a.g.n[1] + 1, a.g.n[2] + 1};
     const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
-    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= ext[oa] * ext[ob]) return;
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= 5 * ext[oa] * ext[ob]) return;
+    const int m = (int)(r / (ext[oa] * ext[ob]));
+    r -= m * ext[oa] * ext[ob];
     int c[3];
     c[oa] = (int)(r % ext[oa]);
     c[ob] = (int)(r / ext[oa]);
@@ -1393,8 +1424,7 @@ __global__ void k_qswap_adj_line(AdjArgs a)
     const long h0 = gidx(a.g, c[0], c[1], c[2]);
     const long sd = a.g.s[d];
     const int n = a.g.n[d];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) {
+    {
         double* f = a.qb[m] + h0;
         const double t = f[0] + f[n * sd];
         f[0] = 0.5 * t;
@@ -2505,7 +2535,8 @@ int launch_sweep_t_op(astr_ctx* c, const TSweepArgs& a)
 {
     const int L = c->L[a.dir];
     const size_t lds = lds_bytes(a.dir, L);
-    const dim3 grid(a.ntile_a * a.nb);
+    const int ntiles = a.ntile_a * a.nb;
+    const dim3 grid((ntiles + 7) / 8 * 8 * a.nq);
     switch (L) {
 #define CASE(LL)                                                                              \
     case LL:                                                                                  \
@@ -2524,16 +2555,20 @@ int launch_sweep_t_op(astr_ctx* c, const TSweepArgs& a)
     return ASTR_OK;
 }
 
-// fout (+)= the transpose of the compact derivative (op OP_DERIV) or filter
-// (OP_FILTER) along dir applied to fin; see TSweepArgs for wrap and xi
-int launch_sweep_t(astr_ctx* c, int dir, int op, const double* fin, double* fout, const double* xi,
-                   bool wrap, bool accumulate)
+// fout[m] (+)= the transpose of the compact derivative (op OP_DERIV) or
+// filter (OP_FILTER) along dir applied to fin[m], m < nq, in one launch;
+// see TSweepArgs for wrap and xi
+int launch_sweep_t(astr_ctx* c, int dir, int op, int nq, const double* const* fin,
+                   double* const* fout, const double* xi, bool wrap, bool accumulate)
 {
     if (!c->lines_ok[dir])
         return fail(ASTR_E_UNSUPPORTED, "axis too long (or absent) for the compact line operators");
     TSweepArgs a = tsweep_args(c, dir, op);
-    a.fin = fin;
-    a.fout = fout;
+    a.nq = nq;
+    for (int m = 0; m < nq; ++m) {
+        a.fin[m] = fin[m];
+        a.fout[m] = fout[m];
+    }
     a.xi = xi;
     a.wrap = wrap;
     a.accumulate = accumulate;
@@ -3175,9 +3210,9 @@ int grad_adj(astr_ctx* c)
     const AdjFields f = adj_fields(c);
     for (int d = 0; d < 3; ++d) {
         const double* xi = c->xi + d * c->xi_stride;
-        for (int m = 0; m < 3; ++m)
-            TRY(launch_sweep_t(c, d, OP_DERIV, f.db[3 * m + d], f.pb[1 + m], xi, false, true));
-        TRY(launch_sweep_t(c, d, OP_DERIV, f.tb[d], f.pb[5], xi, false, true));
+        const double* fin[4] = {f.db[d], f.db[3 + d], f.db[6 + d], f.tb[d]};
+        double* fout[4] = {f.pb[1], f.pb[2], f.pb[3], f.pb[5]};
+        TRY(launch_sweep_t(c, d, OP_DERIV, 4, fin, fout, xi, false, true));
     }
     HIPCHK(hipMemsetAsync(f.db[0], 0, sizeof(double) * c->nfield * 12, c->stream));
     return ASTR_OK;
@@ -3194,7 +3229,7 @@ int qswap_adj(astr_ctx* c, AdjArgs a)
     for (int d = 2; d >= 0; --d) {
         a.axis = d;
         const long nl = d == 0 ? ext[1] * ext[2] : d == 1 ? ext[0] * ext[2] : ext[0] * ext[1];
-        TRY(launch_adj(c, k_qswap_adj_line, nl, a));
+        TRY(launch_adj(c, k_qswap_adj_line, 5 * nl, a));
     }
     return ASTR_OK;
 }
@@ -3217,8 +3252,7 @@ int stage_adj(astr_ctx* c, const StageStore& st, int s, double dt)
     TRY(launch_adj(c, k_rk_adj, n_interior(c), a));
     // the flux derivatives: G-bar_d = D_d^T qrhs-bar
     for (int d = 0; d < 3; ++d)
-        for (int m = 0; m < 5; ++m)
-            TRY(launch_sweep_t(c, d, OP_DERIV, f.rb[m], f.gb[5 * d + m], nullptr, false, false));
+        TRY(launch_sweep_t(c, d, OP_DERIV, 5, f.rb, f.gb + 5 * d, nullptr, false, false));
     // the axis fluxes, at the gradients of the stored primitives
     TRY(grad_of(c, st.prim));
     TRY(launch_adj(c, k_gflux_adj_int, n_interior(c), a));
@@ -3233,8 +3267,7 @@ int stage_adj(astr_ctx* c, const StageStore& st, int s, double dt)
     // filterq: the filters of z, y, x transposed, in place
     if (c->p.lfilter)
         for (int d = 2; d >= 0; --d)
-            for (int m = 0; m < 5; ++m)
-                TRY(launch_sweep_t(c, d, OP_FILTER, f.qb[m], f.qb[m], nullptr, true, false));
+            TRY(launch_sweep_t(c, d, OP_FILTER, 5, f.qb, f.qb, nullptr, true, false));
     return ASTR_OK;
 }
 
@@ -4008,24 +4041,36 @@ int astr_adjoint_enstrophy(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt,
         c->adj_nf = need;
     }
     HIPCHK(hipEventRecord(c->ev[12], c->stream));
-    // forward, keeping the start of every segment
+    // forward, keeping the start of every segment; the last segment's
+    // stages are stored as they run
     for (long t = 0; t < nsteps; ++t) {
         if (t % ckpt == 0) TRY(save_state(c, t / ckpt));
-        TRY(do_step(c, dt, nullptr, false));
-    }
-    TRY(save_state(c, nseg));
-    TRY(objective_adj(c, J));
-    HIPCHK(hipEventRecord(c->ev[13], c->stream));
-    // reverse, segment by segment: recompute the segment's stages, storing
-    // what their reversal needs, then reverse them
-    for (long seg = nseg - 1; seg >= 0; --seg) {
-        const long t0 = seg * ckpt, t1 = std::min<long>(nsteps, t0 + ckpt);
-        TRY(load_state(c, seg));
-        for (long t = t0; t < t1; ++t)
+        if (t / ckpt == nseg - 1) {
+            const long t0 = (nseg - 1) * ckpt;
             for (int s = 1; s <= 3; ++s) {
                 const StageStore st = stage_slot(c, nstates, 3 * (t - t0) + (s - 1));
                 TRY(stage_fwd(c, s, dt, &st));
             }
+            c->time = c->time + dt;
+        } else {
+            TRY(do_step(c, dt, nullptr, false));
+        }
+    }
+    TRY(save_state(c, nseg));
+    TRY(objective_adj(c, J));
+    HIPCHK(hipEventRecord(c->ev[13], c->stream));
+    // reverse, segment by segment: recompute the segment's stages from its
+    // kept state (the last one is stored already), then reverse them
+    for (long seg = nseg - 1; seg >= 0; --seg) {
+        const long t0 = seg * ckpt, t1 = std::min<long>(nsteps, t0 + ckpt);
+        if (seg != nseg - 1) {
+            TRY(load_state(c, seg));
+            for (long t = t0; t < t1; ++t)
+                for (int s = 1; s <= 3; ++s) {
+                    const StageStore st = stage_slot(c, nstates, 3 * (t - t0) + (s - 1));
+                    TRY(stage_fwd(c, s, dt, &st));
+                }
+        }
         for (long t = t1 - 1; t >= t0; --t)
             for (int s = 3; s >= 1; --s)
                 TRY(stage_adj(c, stage_slot(c, nstates, 3 * (t - t0) + (s - 1)), s, dt));
@@ -4069,8 +4114,10 @@ int astr_line_adjoint(astr_ctx* c, int32_t op, int32_t dir, int32_t wrap, const 
     std::vector<double> stage;
     TRY(upload_field(c, c->scratch, gbar, stage));
     HIPCHK(hipMemsetAsync(c->scratch2, 0, sizeof(double) * c->nfield, c->stream));
-    TRY(launch_sweep_t(c, dir - 1, op == 0 ? OP_DERIV : OP_FILTER, c->scratch, c->scratch2, nullptr,
-                       wrap != 0, false));
+    const double* fin[1] = {c->scratch};
+    double* fout[1] = {c->scratch2};
+    TRY(launch_sweep_t(c, dir - 1, op == 0 ? OP_DERIV : OP_FILTER, 1, fin, fout, nullptr, wrap != 0,
+                       false));
     HIPCHK(hipStreamSynchronize(c->stream));
     return download_field(c, c->scratch2, fbar, stage, false);
 }

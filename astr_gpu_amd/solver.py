@@ -309,6 +309,16 @@ def hit_state(p: AstrParams, seed: int, mt: float = 0.3, k0: float = 4.0,
         vel.append(np.real(np.fft.ifftn(g)))
     urms = math.sqrt(sum(float(np.mean(v * v)) for v in vel))
     vel = [v / urms for v in vel]  # sqrt(<u.u>) = 1, so Mt = Ma
+    return state_from_velocity(p, np.stack(vel))
+
+
+def state_from_velocity(p: AstrParams, u0) -> dict:
+    """The decaying-turbulence state of a velocity field u0 (3, n0, n1, n2)
+    given at the unique periodic nodes: rho = 1, p = 1/(gamma Ma^2) (T = 1),
+    q from fvar2q's temperature form (src/fludyna.F90:449-533); node n
+    copies node 0, halos zero (qswap fills them)."""
+    n = (p.im, p.jm, p.km)
+    vel = [np.asarray(u0[c], dtype=np.float64) for c in range(3)]
     gamma, mach = p.gamma, p.mach
     const1 = 1.0 / (gamma * (gamma - 1.0) * (mach * mach))
     const2 = gamma * (mach * mach)

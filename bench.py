@@ -11,7 +11,10 @@ periodic grid points (n^3) advanced one RK3 step per second.
 Other workloads (--workload): riemann2d, RK3 steps of the 2-D four-quadrant
 Riemann problem at 2048^2 with the explicit upwind scheme and the example's
 boundary conditions; ensemble, 48 periodic turbulence boxes of mixed sizes
-advanced together against looping the plain solver over them.
+advanced together against looping the plain solver over them; adjoint, the
+gradient of the enstrophy after 40 RK3 steps of a 128^3 turbulence box by
+the discrete adjoint, its cost over the forward's and its agreement with
+central differences.
 
 Prints one JSON line.  roofline: the dominant kernel family, its algorithmic
 HBM bytes per launch over its average launch time (HIP events on the
@@ -225,6 +228,98 @@ def bench_riemann(args):
     print(json.dumps(out))
 
 
+def bench_adjoint(args):
+    """Discrete adjoint on the decaying-turbulence box (n^3, HIT spectrum,
+    seeded; Mt 0.3, Re 173): J = enstrophy after nsteps RK3 steps, dJ/du0
+    by the GPU adjoint (checkpoints every ckpt steps), checked against
+    central differences (eps 1e-6) along 8 seeded Gaussian directions.  The
+    metric is the cost of the gradient over the cost of the plain forward
+    steps (lower is better)."""
+    import torch
+
+    import astr_gpu_amd as ag
+    from astr_gpu_amd.autograd import velocity_state
+
+    n, nsteps, ckpt, dt = args.n, args.nsteps, args.ckpt, 0.002
+    p = ag.default_params(n, mach=0.3, reynolds=173.0)
+    s = ag.AstrSolver(p, device=0)
+    s.upload_geom(*ag.box_metrics(p))
+    st = ag.hit_state(p, args.seed)
+    u0 = np.moveaxis(ag.interior(st["vel"], p)[:-1, :-1, :-1], -1, 0).copy()
+
+    def upload(state):
+        s.upload_state(state["q"], state["rho"], state["vel"], state["prs"], state["tmp"])
+
+    def objective(u):
+        upload(ag.state_from_velocity(p, u))
+        s.steploop(nsteps, dt)
+        s.qswap()
+        s.rhscal()
+        return s.statcal()[1]
+
+    # warm-up of every kernel involved
+    upload(st)
+    s.adjoint_enstrophy(min(nsteps, 2), dt, ckpt)
+    # plain forward
+    reps = max(1, args.steps // 10)
+    fwd = []
+    for _ in range(reps):
+        upload(st)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.steploop(nsteps, dt)
+        s.synchronize()
+        fwd.append(time.perf_counter() - t0)
+    t_fwd = min(fwd)
+    # gradient: checkpoints every ckpt steps, and every stage kept (no
+    # recomputation; 16 fields per stage in HBM)
+    def gradient(k):
+        best, res = None, None
+        for _ in range(reps):
+            upload(st)
+            s.synchronize()
+            t0 = time.perf_counter()
+            res = s.adjoint_enstrophy(nsteps, dt, k)
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        return best, res
+
+    t_all, _ = gradient(nsteps)
+    t_adj, (J, qb, pb, ms) = gradient(ckpt)
+    # dJ/du0 through the state construction
+    x = torch.tensor(u0, requires_grad=True)
+    q0, pr0 = velocity_state(p, x)
+    torch.autograd.backward([q0, pr0], [torch.from_numpy(qb), torch.from_numpy(pb)])
+    g = x.grad.numpy()
+    rng = np.random.default_rng(args.seed)
+    errs = {1e-6: [], 1e-4: []}
+    for _ in range(8):
+        v = rng.standard_normal(u0.shape)
+        ad = float(np.sum(g * v))
+        for eps in errs:
+            fd = (objective(u0 + eps * v) - objective(u0 - eps * v)) / (2 * eps)
+            errs[eps].append(abs(fd - ad) / abs(ad))
+    s.close()
+    out = {
+        "metric": "adjoint/forward cost ratio (gradient of the enstrophy after the steps)",
+        "value": t_adj / t_fwd, "unit": "ratio", "n_gpus": 1, "steps": nsteps,
+        "warmup": 1, "ms_per_step": 1e3 * t_fwd / nsteps, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded decaying isotropic turbulence)",
+        "config": {"workload": f"discrete adjoint, HIT {n}^3 Mt 0.3 Re 173 seed {args.seed}, "
+                               f"{nsteps} RK3 steps dt {dt}, checkpoints every {ckpt} steps, "
+                               f"J = enstrophy of the final state",
+                   "n": n, "parallelism": "single GPU"},
+        "forward_s": t_fwd, "gradient_s": t_adj,
+        "ratio_all_stages_kept": t_all / t_fwd,
+        "gradient_split_ms": {"forward_with_checkpoints": ms[0], "reversal": ms[1]},
+        "J": J, "fd_max_rel_err": max(errs[1e-6]), "fd_eps": 1e-6, "fd_directions": 8,
+        "fd_max_rel_err_eps_1e-4": max(errs[1e-4]),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 ENSEMBLE_SIZES = (64, 96, 128, 160, 192)
 
 
@@ -306,10 +401,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["tgv", "riemann2d", "ensemble"], default="tgv",
+    ap.add_argument("--workload", choices=["tgv", "riemann2d", "ensemble", "adjoint"],
+                    default="tgv",
                     help="tgv: the headline RK3 step; riemann2d: RK3 steps of the 2-D "
                          "Riemann problem (upwind, boundary conditions); ensemble: batched "
                          "boxes against looping the plain solver")
+    ap.add_argument("--nsteps", type=int, default=40, help="adjoint: RK3 steps of the objective")
+    ap.add_argument("--ckpt", type=int, default=8, help="adjoint: steps between kept states")
     ap.add_argument("--t-end", type=float, default=0.0,
                     help="riemann2d: also run the problem from t = 0 to this time (0.3 in "
                          "the benchmark) and report its wall time")
@@ -327,6 +425,11 @@ def main():
         if args.warmup == 3:
             args.warmup = 1
         bench_ensemble(args)
+        return
+    if args.workload == "adjoint":
+        if args.n == 256:
+            args.n = 128
+        bench_adjoint(args)
         return
     if args.workload == "riemann2d":
         if args.n == 256:

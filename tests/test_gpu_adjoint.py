@@ -136,7 +136,7 @@ def test_adjoint_matches_torch_autograd(ckpt):
     h = to_host(Q0, P0)
     s.upload_state(h["q"], h["rho"], h["vel"], h["prs"], h["tmp"])
     Jg, qb, pb, ms = s.adjoint_enstrophy(nsteps, dt, ckpt)
-    print(f"  J gpu {Jg:.15e} torch {float(J):.15e}; ms {ms}")
+    print(f"  J gpu {Jg:.15e} torch {float(J.detach()):.15e}; ms {ms}")
     assert abs(Jg - float(J)) <= 1e-11 * abs(float(J))
     gq = np.moveaxis(Q0.grad.numpy(), 0, -1)
     gp = np.moveaxis(P0.grad.numpy(), 0, -1)
@@ -185,4 +185,32 @@ def test_adjoint_against_central_differences():
         ad = float(np.sum(g * v))
         print(f"  direction {k}: fd {fd:.12e} adjoint {ad:.12e} rel {abs(fd - ad) / abs(ad):.2e}")
         assert abs(fd - ad) <= 1e-6 * abs(ad)
+    s.close()
+
+
+def test_autograd_function_gives_the_velocity_gradient():
+    """EnstrophyAfterSteps (forward: the plain GPU steps, backward: the GPU
+    adjoint) composed with velocity_state: dJ/du0 against PyTorch-CPU
+    autograd of the restated forward."""
+    import torch
+
+    from astr_gpu_amd.autograd import EnstrophyAfterSteps, velocity_state
+
+    dims = (16, 18, 20)
+    p, box, u0, _ = adjoint_case(dims, seed=9)
+    dt, nsteps = 0.004, 2
+    x = u0.clone().requires_grad_(True)
+    Jref = box.objective(x, nsteps, dt)
+    Jref.backward()
+    s = ag.AstrSolver(p)
+    s.upload_geom(*ag.box_metrics(p))
+    y = u0.clone().requires_grad_(True)
+    q0, pr0 = velocity_state(p, y)
+    J = EnstrophyAfterSteps.apply(q0, pr0, s, nsteps, dt, 1)
+    J.backward()
+    print(f"  J {float(J):.15e} ref {float(Jref.detach()):.15e}")
+    assert abs(float(J) - float(Jref.detach())) <= 1e-11 * abs(float(Jref.detach()))
+    e = float((y.grad - x.grad).abs().max() / x.grad.abs().max())
+    print(f"  dJ/du0 rel err {e:.2e}")
+    assert e <= 1e-9
     s.close()
