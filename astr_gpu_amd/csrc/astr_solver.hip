@@ -577,6 +577,8 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
                 if (r0 + t < R) ln[HM + a.tab.first + r0 + t] = x[t];
         }
         wsync();
+        // B^T w, columns j = -HM + sub + 32 t (a half-wavefront reads 32
+        // consecutive LDS slots at a time)
         double f[L + 1];
 #pragma unroll
         for (int t = 0; t <= L; ++t) {
@@ -598,6 +600,15 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
 
     // ---------------- write the adjoint of the line's input
     if (lvalid) {
+        // accumulate: the old values of every node of the lane are loaded
+        // before the first store (a store could alias a later load)
+        double old[EMAX];
+#pragma unroll
+        for (int e = 0; e < EMAX; ++e) {
+            old[e] = 0.0;
+            const int node = min(n0 + nstep * e - HM, dim + HM);
+            if (a.accumulate && e < esteps) old[e] = ldb(fout, lob + (unsigned int)node * sdb);
+        }
 #pragma unroll
         for (int e = 0; e < EMAX; ++e) {
             if (e >= esteps) continue;
@@ -614,9 +625,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
                 if (node > dim + HM) continue;
                 v = trow[n];
             }
-            const unsigned int od = lob + (unsigned int)node * sdb;
-            if (a.accumulate) v += ldb(fout, od);
-            stb(fout, od, v);
+            stb(fout, lob + (unsigned int)node * sdb, v + old[e]);
         }
     }
 }
@@ -1162,32 +1171,41 @@ struct AdjArgs {
 };
 
 // q-bar += (d q2fvar / d q)^T primitive-bar at one node, primitive-bar there
-// zeroed (the forward overwrote the primitives)
+// zeroed (the forward overwrote the primitives).  Every operand is loaded
+// before the first store (a store could alias a later load).
 __device__ __forceinline__ void q2fvar_adj(const AdjArgs& a, const double* q, long gi)
 {
+    double pbv[6], qbv[5];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pbv[k] = a.pb[k][gi];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) qbv[m] = a.qb[m][gi];
     const double rho = q[0];
     const double u = q[1] / rho, v = q[2] / rho, w = q[3] / rho;
     const double uu = u * u + v * v + w * w;
     const double p = (q[4] - 0.5 * rho * uu) / a.const6;
-    double rb = a.pb[0][gi], ub = a.pb[1][gi], vb = a.pb[2][gi], wb = a.pb[3][gi];
-    double pbv = a.pb[4][gi];
-    const double tbv = a.pb[5][gi];
+    double rb = pbv[0], ub = pbv[1], vb = pbv[2], wb = pbv[3];
+    double pbar = pbv[4];
+    const double tbv = pbv[5];
     // T = p/rho*const2
-    pbv += tbv * a.const2 / rho;
+    pbar += tbv * a.const2 / rho;
     rb -= tbv * p * a.const2 / (rho * rho);
     // p = (q5 - rho |u|^2/2)/const6
-    const double q4b = pbv / a.const6;
-    rb -= pbv * 0.5 * uu / a.const6;
-    ub -= pbv * rho * u / a.const6;
-    vb -= pbv * rho * v / a.const6;
-    wb -= pbv * rho * w / a.const6;
+    const double q4b = pbar / a.const6;
+    rb -= pbar * 0.5 * uu / a.const6;
+    ub -= pbar * rho * u / a.const6;
+    vb -= pbar * rho * v / a.const6;
+    wb -= pbar * rho * w / a.const6;
     // u = q2/rho, ...
-    a.qb[1][gi] += ub / rho;
-    a.qb[2][gi] += vb / rho;
-    a.qb[3][gi] += wb / rho;
+    qbv[1] += ub / rho;
+    qbv[2] += vb / rho;
+    qbv[3] += wb / rho;
     rb -= (ub * u + vb * v + wb * w) / rho;
-    a.qb[0][gi] += rb;
-    a.qb[4][gi] += q4b;
+    qbv[0] += rb;
+    qbv[4] += q4b;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) a.qb[m][gi] = qbv[m];
+#pragma unroll
     for (int k = 0; k < 6; ++k) a.pb[k][gi] = 0.0;
 }
 
@@ -1230,36 +1248,40 @@ __global__ void k_rk_adj(AdjArgs a)
     if (p >= ni * nj * (a.g.n[2] + 1)) return;
     const long gi = gidx(a.g, (int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj)));
     const double jc = a.jac[gi];
+    double ob[5], sv[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        const double ob = a.qb[m][gi];
-        a.rb[m][gi] = a.rkc * a.dt * ob / jc;
-        double qv = a.rkb * ob;
+        ob[m] = a.qb[m][gi];
+        sv[m] = a.sb[m][gi];
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        a.rb[m][gi] = a.rkc * a.dt * ob[m] / jc;
+        double qv = a.rkb * ob[m];
         if (a.stage == 1) {
-            qv += a.rka * ob + jc * a.sb[m][gi];
+            qv += a.rka * ob[m] + jc * sv[m];
             a.sb[m][gi] = 0.0;
         } else {
-            a.sb[m][gi] += a.rka * ob / jc;
+            a.sb[m][gi] = sv[m] + a.rka * ob[m] / jc;
         }
         a.qb[m][gi] = qv;
     }
 }
 
-// The inviscid part of G_d (convrsdcal6's fcs) reversed at one node: q-bar
-// and primitive-bar (u_d, p) collect -dF/d(.) G-bar
-__device__ __forceinline__ void inviscid_adj(const AdjArgs& a, int d, long gi, double xn,
-                                             const double* gbv)
+// The inviscid part of G_d (convrsdcal6's fcs) reversed at one node, on
+// register copies: qbv and pbv (u_d, p) collect -dF/d(.) G-bar; q, p, u_d
+// are the forward values at the node
+__device__ __forceinline__ void inviscid_adj(int d, double xn, double jc, const double* qv,
+                                             double p, double ud, const double* gbv, double* qbv,
+                                             double* pbv)
 {
-    const double jc = a.jac[gi];
-    const double p = a.prim[4][gi];
-    const double ud = a.prim[1 + d][gi];
     const double uu = xn * ud;
     double ubar = 0.0, pbar = 0.0;
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         const double g = -gbv[m];  // G = -F + (viscous)
-        const double qm = a.q[m][gi];
-        a.qb[m][gi] += g * jc * uu;
+        const double qm = qv[m];
+        qbv[m] += g * jc * uu;
         if (m == 4) {
             ubar += g * jc * (qm + p);
             pbar += g * jc * uu;
@@ -1268,8 +1290,8 @@ __device__ __forceinline__ void inviscid_adj(const AdjArgs& a, int d, long gi, d
             if (m == d + 1) pbar += g * jc * xn;
         }
     }
-    a.pb[1 + d][gi] += ubar * xn;
-    a.pb[4][gi] += pbar;
+    pbv[1 + d] += ubar * xn;
+    pbv[4] += pbar;
 }
 
 // G on the halo strips of axis d: the inviscid part at the halo node itself
@@ -1298,10 +1320,25 @@ __global__ void k_gflux_adj_halo(AdjArgs a)
     }
     c[d] = h < HM ? -(h + 1) : a.g.n[d] + (h - HM + 1);
     const long gi = gidx(a.g, c[0], c[1], c[2]);
-    double gbv[5];
+    double gbv[5], qv[5], qbv[5], pbv[6];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) gbv[m] = a.gb[5 * d + m][gi];
-    inviscid_adj(a, d, gi, a.xi[d][c[d] + HM], gbv);
+    for (int m = 0; m < 5; ++m) {
+        gbv[m] = a.gb[5 * d + m][gi];
+        qv[m] = a.q[m][gi];
+        qbv[m] = a.qb[m][gi];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pbv[k] = a.pb[k][gi];
+    // a compile-time axis keeps the arrays in registers
+    const double xn = a.xi[d][c[d] + HM], jc = a.jac[gi], pf = a.prim[4][gi];
+    const double ud = a.prim[1 + d][gi];
+    if (d == 0) inviscid_adj(0, xn, jc, qv, pf, ud, gbv, qbv, pbv);
+    else if (d == 1) inviscid_adj(1, xn, jc, qv, pf, ud, gbv, qbv, pbv);
+    else inviscid_adj(2, xn, jc, qv, pf, ud, gbv, qbv, pbv);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) a.qb[m][gi] = qbv[m];
+    a.pb[1 + d][gi] = pbv[1 + d];
+    a.pb[4][gi] = pbv[4];
 }
 
 // G on the interior reversed at one node: the inviscid parts of its three
@@ -1318,14 +1355,24 @@ __global__ void k_gflux_adj_int(AdjArgs a)
     const int c[3] = {(int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj))};
     const long gi = gidx(a.g, c[0], c[1], c[2]);
     const double jc = a.jac[gi];
+    // every operand first (a store could alias a later load); primitive-bar
+    // is zero on entry (spent by updatefvar's reversal)
+    double qv[5], qbv[5], pbv[6] = {0, 0, 0, 0, 0, 0}, gball[15];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        qv[m] = a.q[m][gi];
+        qbv[m] = a.qb[m][gi];
+    }
+#pragma unroll
+    for (int k = 0; k < 15; ++k) gball[k] = a.gb[k][gi];
+    const double pf = a.prim[4][gi];
+    const double uvw[3] = {a.prim[1][gi], a.prim[2][gi], a.prim[3][gi]};
     double sgb[6] = {0, 0, 0, 0, 0, 0}, qfb[3] = {0, 0, 0};
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
         const double xn = a.xi[e][c[e] + HM];
-        double gbv[5];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) gbv[m] = a.gb[5 * e + m][gi];
-        inviscid_adj(a, e, gi, xn, gbv);
+        const double* gbv = gball + 5 * e;
+        inviscid_adj(e, xn, jc, qv, pf, uvw[e], gbv, qbv, pbv);
         if (a.visc) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) sgb[sidx[e][k]] += gbv[1 + k] * xn * jc;
@@ -1346,7 +1393,13 @@ __global__ void k_gflux_adj_int(AdjArgs a)
             }
         }
     }
-    if (!a.visc) return;
+    if (!a.visc) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) a.qb[m][gi] = qbv[m];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a.pb[k][gi] = pbv[k];
+        return;
+    }
     // forward sigma and heat flux at this node
     double dv[9], dt[3];
 #pragma unroll
@@ -1364,7 +1417,7 @@ __global__ void k_gflux_adj_int(AdjArgs a)
     const double hcc = (miu / a.prandtl) / a.const5;
     const double g[6] = {miu2 * (s11 - skk), miu2 * s12, miu2 * s13, miu2 * (s22 - skk), miu2 * s23,
                          miu2 * (s33 - skk)};
-    const double uv[3] = {a.prim[1][gi], a.prim[2][gi], a.prim[3][gi]};
+    const double* uv = uvw;
     // qf_d = hcc dtmp_d + sum_c sigma(d,c) u_c
     double hccb = 0.0, ubv[3] = {0, 0, 0};
 #pragma unroll
@@ -1377,7 +1430,7 @@ __global__ void k_gflux_adj_int(AdjArgs a)
             ubv[k] += qfb[d] * g[sidx[d][k]];
         }
     }
-    for (int k = 0; k < 3; ++k) a.pb[1 + k][gi] += ubv[k];  // + inviscid_adj's share
+    for (int k = 0; k < 3; ++k) pbv[1 + k] += ubv[k];
     // sigma = 2 mu (S - skk I)
     const double miu2b = sgb[0] * (s11 - skk) + sgb[1] * s12 + sgb[2] * s13 +
                          sgb[3] * (s22 - skk) + sgb[4] * s23 + sgb[5] * (s33 - skk);
@@ -1401,40 +1454,50 @@ __global__ void k_gflux_adj_int(AdjArgs a)
     const double miub = 2.0 * miu2b + hccb / a.prandtl / a.const5;
     const double dmu = a.tempconst1 * st * (0.5 * T + 1.5 * a.tempconst) /
                        ((T + a.tempconst) * (T + a.tempconst));
-    a.pb[5][gi] += miub * dmu / a.reynolds;
+    pbv[5] += miub * dmu / a.reynolds;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) a.qb[m][gi] = qbv[m];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.pb[k][gi] = pbv[k];
 }
 
 // qswap's copies and averages along a.axis reversed, for the lines of that
 // axis over 0..n of the other two: the average q(0) = q(n) = (q(0)+q(n))/2,
-// then the halo copies q(-k) = q(n-k), q(n+k) = q(k).  Run for the axes in
-// the order z, y, x.
+// and the halo copies q(-k) = q(n-k), q(n+k) = q(k).  The eleven pieces
+// touch different nodes (the copies' sources exclude 0 and n), so each is
+// its own thread: op 0 the average, op 1..HM the low halos, HM+1..2HM the
+// high ones.  Run for the axes in the order z, y, x.
 __global__ void k_qswap_adj_line(AdjArgs a)
 {
     const int d = a.axis;
     const long ext[3] = {a.g.n[0] + 1, a.g.n[1] + 1, a.g.n[2] + 1};
     const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
+    const long nl = ext[oa] * ext[ob];
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= 5 * ext[oa] * ext[ob]) return;
-    const int m = (int)(r / (ext[oa] * ext[ob]));
-    r -= m * ext[oa] * ext[ob];
+    if (r >= (2 * HM + 1) * 5 * nl) return;
+    const int op = (int)(r / (5 * nl));
+    r -= op * 5 * nl;
+    const int m = (int)(r / nl);
+    r -= m * nl;
     int c[3];
     c[oa] = (int)(r % ext[oa]);
     c[ob] = (int)(r / ext[oa]);
     c[d] = 0;
-    const long h0 = gidx(a.g, c[0], c[1], c[2]);
     const long sd = a.g.s[d];
     const int n = a.g.n[d];
-    {
-        double* f = a.qb[m] + h0;
+    double* f = a.qb[m] + gidx(a.g, c[0], c[1], c[2]);
+    if (op == 0) {
         const double t = f[0] + f[n * sd];
         f[0] = 0.5 * t;
         f[n * sd] = 0.5 * t;
-        for (int k = 1; k <= HM; ++k) {
-            f[(n - k) * sd] += f[-k * sd];
-            f[-k * sd] = 0.0;
-            f[k * sd] += f[(n + k) * sd];
-            f[(n + k) * sd] = 0.0;
-        }
+    } else if (op <= HM) {
+        const int k = op;
+        f[(n - k) * sd] += f[-k * sd];
+        f[-k * sd] = 0.0;
+    } else {
+        const int k = op - HM;
+        f[k * sd] += f[(n + k) * sd];
+        f[(n + k) * sd] = 0.0;
     }
 }
 
@@ -3214,7 +3277,9 @@ int grad_adj(astr_ctx* c)
         double* fout[4] = {f.pb[1], f.pb[2], f.pb[3], f.pb[5]};
         TRY(launch_sweep_t(c, d, OP_DERIV, 4, fin, fout, xi, false, true));
     }
-    HIPCHK(hipMemsetAsync(f.db[0], 0, sizeof(double) * c->nfield * 12, c->stream));
+    // with viscous terms k_gflux_adj_int overwrites them in the next stage
+    if (!c->p.diffterm)
+        HIPCHK(hipMemsetAsync(f.db[0], 0, sizeof(double) * c->nfield * 12, c->stream));
     return ASTR_OK;
 }
 
@@ -3229,7 +3294,7 @@ int qswap_adj(astr_ctx* c, AdjArgs a)
     for (int d = 2; d >= 0; --d) {
         a.axis = d;
         const long nl = d == 0 ? ext[1] * ext[2] : d == 1 ? ext[0] * ext[2] : ext[0] * ext[1];
-        TRY(launch_adj(c, k_qswap_adj_line, 5 * nl, a));
+        TRY(launch_adj(c, k_qswap_adj_line, (2 * HM + 1) * 5 * nl, a));
     }
     return ASTR_OK;
 }
