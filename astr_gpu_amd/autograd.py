@@ -1,4 +1,5 @@
-"""torch.autograd binding of the discrete adjoint (astr_adjoint_enstrophy).
+"""torch.autograd binding of the discrete adjoint (astr_adjoint_record and
+astr_adjoint_reverse).
 
 EnstrophyAfterSteps.apply(q0, prim0, solver, nsteps, dt, ckpt) returns the
 enstrophy J after nsteps RK3 steps of the plain solver from the state
@@ -31,27 +32,28 @@ def _upload(solver: AstrSolver, q0: torch.Tensor, prim0: torch.Tensor) -> None:
 
 
 class EnstrophyAfterSteps(torch.autograd.Function):
-    """Forward: nsteps plain RK3 steps on the GPU, then J = the enstrophy
-    statistic of the final state after a qswap.  Backward: astr_adjoint_
-    enstrophy from the same initial state (segments of ckpt steps are
-    recomputed from kept states)."""
+    """Forward: nsteps plain RK3 steps on the GPU (astr_adjoint_record: the
+    state is kept every ckpt steps and the last segment's stages are stored
+    on the device as they run), then J = the enstrophy statistic of the
+    final state after a qswap.  Backward: astr_adjoint_reverse of that
+    recorded trajectory (the segments before the last are recomputed from
+    their kept states).  The solver keeps one recorded trajectory: a second
+    forward on the same solver before this backward replaces it, and the
+    backward then raises."""
 
     @staticmethod
     def forward(ctx, q0, prim0, solver, nsteps, dt, ckpt):
         _upload(solver, q0, prim0)
-        solver.steploop(int(nsteps), float(dt))
-        solver.qswap()
-        solver.rhscal()  # the gradients of the state after qswap
-        J = solver.statcal()[1]
-        ctx.save_for_backward(q0.detach().clone(), prim0.detach().clone())
-        ctx.solver, ctx.nsteps, ctx.dt, ctx.ckpt = solver, int(nsteps), float(dt), int(ckpt)
+        J = solver.adjoint_record(int(nsteps), float(dt), int(ckpt))
+        solver._adjoint_token = token = object()
+        ctx.solver, ctx.token = solver, token
         return torch.tensor(J, dtype=torch.float64)
 
     @staticmethod
     def backward(ctx, gJ):
-        q0, prim0 = ctx.saved_tensors
-        _upload(ctx.solver, q0, prim0)
-        _, qb, pb, _ = ctx.solver.adjoint_enstrophy(ctx.nsteps, ctx.dt, ctx.ckpt)
+        if getattr(ctx.solver, "_adjoint_token", None) is not ctx.token:
+            raise RuntimeError("the solver recorded another trajectory after this forward")
+        qb, pb, _ = ctx.solver.adjoint_reverse()
         g = gJ.to(torch.float64)
         return (g * torch.from_numpy(np.ascontiguousarray(qb)),
                 g * torch.from_numpy(np.ascontiguousarray(pb)), None, None, None, None)

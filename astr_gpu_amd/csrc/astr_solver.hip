@@ -115,6 +115,9 @@ struct SweepArgs {
     int rk_stage;
     double rka, rkb, rkc, dt;
     double* qsave[5];
+    // where PASS_FILTER and the RK3 combine write q (q itself unless a
+    // caller sends the result to another buffer)
+    double* qo[5];
     int visc;  // PASS_RHS: include the viscous fluxes (diffterm)
     // mini-app variant (tgvsolver.F90): derivatives divided by the uniform
     // spacing h instead of multiplied by the metric; RK3 without J
@@ -234,7 +237,7 @@ __device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const do
                     } else {
                         qs = cs[g];
                     }
-                    stb(a.q[m], od, rk_combine(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], cj[g], r));
+                    stb(a.qo[m], od, rk_combine(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], cj[g], r));
                 } else {
                     // rk3 of tgvsolver.F90:1040-1060: q = a qsave + b q + c dt qrhs
                     const double r = cr[g] + o / a.h;
@@ -245,7 +248,7 @@ __device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const do
                     } else {
                         qs = cs[g];
                     }
-                    stb(a.q[m], od, rk_combine_mini(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], r));
+                    stb(a.qo[m], od, rk_combine_mini(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], r));
                 }
             }
         }
@@ -392,7 +395,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
                 const double o = trow[n + HM];
                 const unsigned int od = lob + (unsigned int)n * sdb;
                 if constexpr (PASS == PASS_FILTER) {
-                    stb(a.q[m], od, o);
+                    stb(a.qo[m], od, o);
                 } else if constexpr (PASS == PASS_GRAD) {
                     // gradcal: d/dxi times dxi(d,d) (src/comsolver.F90:304-318),
                     // or df/dx (tgvsolver.F90:420-470)
@@ -1098,11 +1101,33 @@ __global__ void k_prim(PtArgs a)
     q2fvar(a, qv, gi);
 }
 
+// s + e = a + b exactly (Knuth's TwoSum; plain IEEE additions)
+__host__ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
+{
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// (ah, al) += (bh, bl) in double-double arithmetic
+__host__ __device__ __forceinline__ void dd_add(double& ah, double& al, double bh, double bl)
+{
+    double s, e;
+    two_sum(ah, bh, s, e);
+    e += al + bl;
+    ah = s + e;
+    al = e - (ah - s);
+}
+
 // Partial sums of kenergycal / enstophycal / diss_rate_cal over 1..n
-// (src/statistic.F90:870-1042).
+// (src/statistic.F90:870-1042), one double-double (hi, lo) pair per
+// statistic and workgroup: the sums are then exact to about 1e-30 relative
+// of each term, so two statistics of nearby states differ by what the
+// states do, not by summation order (central differences of the enstrophy
+// at eps 1e-6 rely on that).
 __global__ __launch_bounds__(256) void k_stats(PtArgs a)
 {
-    __shared__ double sh[3][256];
+    __shared__ double sh[3][256], sl[3][256];
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ntot = (long)a.g.n[0] * a.g.n[1] * a.g.n[2];
     double ke = 0.0, en = 0.0, di = 0.0;
@@ -1130,14 +1155,23 @@ __global__ __launch_bounds__(256) void k_stats(PtArgs a)
     sh[0][threadIdx.x] = ke;
     sh[1][threadIdx.x] = en;
     sh[2][threadIdx.x] = di;
+    for (int c = 0; c < 3; ++c) sl[c][threadIdx.x] = 0.0;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s)
-            for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + s];
+            for (int c = 0; c < 3; ++c) {
+                double h = sh[c][threadIdx.x], l = sl[c][threadIdx.x];
+                dd_add(h, l, sh[c][threadIdx.x + s], sl[c][threadIdx.x + s]);
+                sh[c][threadIdx.x] = h;
+                sl[c][threadIdx.x] = l;
+            }
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int c = 0; c < 3; ++c) a.part[3 * blockIdx.x + c] = sh[c][0];
+        for (int c = 0; c < 3; ++c) {
+            a.part[6 * blockIdx.x + 2 * c] = sh[c][0];
+            a.part[6 * blockIdx.x + 2 * c + 1] = sl[c][0];
+        }
 }
 
 // ------------------------------------------------------ discrete adjoint
@@ -1166,20 +1200,15 @@ struct AdjArgs {
     double* sb[5];          // qsave-bar
     double const2, const5, const6, reynolds, prandtl, tempconst, tempconst1;
     double rka, rkb, rkc, dt;
-    int stage, axis, visc, mode;
+    int stage, axis, visc;
     double scale;           // 1/(n0 n1 n2) of the statistic
 };
 
-// q-bar += (d q2fvar / d q)^T primitive-bar at one node, primitive-bar there
-// zeroed (the forward overwrote the primitives).  Every operand is loaded
-// before the first store (a store could alias a later load).
-__device__ __forceinline__ void q2fvar_adj(const AdjArgs& a, const double* q, long gi)
+// q-bar += (d q2fvar / d q)^T primitive-bar at one node, on registers: q is
+// the state the primitives were computed from, pbv their adjoint
+__device__ __forceinline__ void q2fvar_adj(const AdjArgs& a, const double* q, const double* pbv,
+                                           double* qbv)
 {
-    double pbv[6], qbv[5];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) pbv[k] = a.pb[k][gi];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) qbv[m] = a.qb[m][gi];
     const double rho = q[0];
     const double u = q[1] / rho, v = q[2] / rho, w = q[3] / rho;
     const double uu = u * u + v * v + w * w;
@@ -1203,16 +1232,13 @@ __device__ __forceinline__ void q2fvar_adj(const AdjArgs& a, const double* q, lo
     rb -= (ub * u + vb * v + wb * w) / rho;
     qbv[0] += rb;
     qbv[4] += q4b;
-#pragma unroll
-    for (int m = 0; m < 5; ++m) a.qb[m][gi] = qbv[m];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) a.pb[k][gi] = 0.0;
 }
 
-// mode 0: updatefvar on the interior, at the RK3 output qn; mode 1: qswap's
-// refresh of the halo cells (one axis outside 0..n) and the seam nodes
-// (interior, some coordinate 0 or n), at q after qswap.  One thread per
-// node of the padded box (x over the (i,j) plane, y over k).
+// qswap's refresh of the primitives reversed at the halo cells (one axis
+// outside 0..n) and the seam nodes (interior, some coordinate 0 or n), at
+// q after qswap: q-bar collects the primitive-bar there, which is zeroed
+// (the forward overwrote those primitives).  One thread per node of the
+// padded box (x over the (i,j) plane, y over k).
 __global__ __launch_bounds__(256) void k_q2f_adj(AdjArgs a)
 {
     const unsigned pw = (unsigned)(a.g.n[0] + 1 + 2 * HM);
@@ -1228,42 +1254,58 @@ __global__ __launch_bounds__(256) void k_q2f_adj(AdjArgs a)
         if (c[e] < 0 || c[e] > a.g.n[e]) ++nout;
         else if (c[e] == 0 || c[e] == a.g.n[e]) seam = true;
     }
-    if (a.mode == 0 ? nout != 0 : (nout > 1 || (nout == 0 && !seam))) return;
+    if (nout > 1 || (nout == 0 && !seam)) return;
     const long gi = gidx(a.g, c[0], c[1], c[2]);
-    double q[5];
-    const double* const* src = a.mode == 0 ? a.qn : a.q;
+    // every operand before the first store (a store could alias a later load)
+    double q[5], pbv[6], qbv[5];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) q[m] = src[m][gi];
-    q2fvar_adj(a, q, gi);
+    for (int m = 0; m < 5; ++m) {
+        q[m] = a.q[m][gi];
+        qbv[m] = a.qb[m][gi];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pbv[k] = a.pb[k][gi];
+    q2fvar_adj(a, q, pbv, qbv);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) a.qb[m][gi] = qbv[m];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.pb[k][gi] = 0.0;
 }
 
-// The RK3 combine reversed on the interior: q-bar holds the adjoint of the
-// stage output q' = (a qsave + b q J + c qrhs dt)/J; it becomes the adjoint
-// of q after qswap, qrhs-bar is set, and qsave-bar collects a/J q'-bar over
-// stages 2-3 and is spent at stage 1, where qsave = q J.
-__global__ void k_rk_adj(AdjArgs a)
+// updatefvar and the RK3 combine reversed on the interior, one node per
+// thread.  The primitive-bar of the stage output qn is spent into q-bar
+// (it is not zeroed: k_gflux_adj_int overwrites the interior primitive-bar
+// next); q-bar, then the adjoint of q' = (a qsave + b q J + c qrhs dt)/J,
+// becomes the adjoint of q after qswap and qrhs-bar is set.  qsave-bar
+// collects a/J q'-bar over stages 3 and 2 and is spent at stage 1, where
+// qsave = q J; stage 3 (the first reversed stage of a step) neither reads
+// it nor needs it cleared.
+__global__ __launch_bounds__(256) void k_update_adj(AdjArgs a)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
     if (p >= ni * nj * (a.g.n[2] + 1)) return;
     const long gi = gidx(a.g, (int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj)));
     const double jc = a.jac[gi];
-    double ob[5], sv[5];
+    double qn[5], pbv[6], ob[5], sv[5] = {0, 0, 0, 0, 0};
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
+        qn[m] = a.qn[m][gi];
         ob[m] = a.qb[m][gi];
-        sv[m] = a.sb[m][gi];
     }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pbv[k] = a.pb[k][gi];
+    if (a.stage != 3) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) sv[m] = a.sb[m][gi];
+    }
+    q2fvar_adj(a, qn, pbv, ob);
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         a.rb[m][gi] = a.rkc * a.dt * ob[m] / jc;
         double qv = a.rkb * ob[m];
-        if (a.stage == 1) {
-            qv += a.rka * ob[m] + jc * sv[m];
-            a.sb[m][gi] = 0.0;
-        } else {
-            a.sb[m][gi] = sv[m] + a.rka * ob[m] / jc;
-        }
+        if (a.stage == 1) qv += a.rka * ob[m] + jc * sv[m];
+        else a.sb[m][gi] = sv[m] + a.rka * ob[m] / jc;
         a.qb[m][gi] = qv;
     }
 }
@@ -2210,6 +2252,9 @@ struct astr_ctx {
     LineTab tab[3][2];
     double* adj_base = nullptr; // discrete adjoint workspace (fields)
     long adj_nf = 0;            // fields it holds
+    // the trajectory astr_adjoint_record kept there (rec_nsteps < 0: none)
+    long rec_nsteps = -1, rec_ckpt = 1;
+    double rec_dt = 0.0, rec_t_end = 0.0;
     double* tabs_t = nullptr;   // transposed systems and closure corrections
     LineTab tab_t[3][2];
     TransTab tt[3][2];
@@ -2520,7 +2565,7 @@ SweepArgs sweep_args(astr_ctx* c, int dir, int pass)
     a.ntile_a = (a.na + tx - 1) / tx;
     const int dim = c->g.n[dir];
     a.rs_in = lds_row(dir, c->L[dir]);
-    for (int m = 0; m < 5; ++m) { a.q[m] = c->q[m]; a.qrhs[m] = c->qrhs[m]; }
+    for (int m = 0; m < 5; ++m) { a.q[m] = c->q[m]; a.qo[m] = c->q[m]; a.qrhs[m] = c->qrhs[m]; }
     for (int m = 0; m < 6; ++m) a.prim[m] = c->prim[m];
     for (int m = 0; m < 9; ++m) a.dvel[m] = c->dvel[m];
     for (int m = 0; m < 3; ++m) a.dtmp[m] = c->dtmp[m];
@@ -2729,10 +2774,12 @@ int do_boucon(astr_ctx* c)
 // primitives.
 const double RKC[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
 
-int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt)
+int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt, double* const* qo = nullptr)
 {
     for (int d = 0; d < 3; ++d) {
         SweepArgs a = sweep_args(c, d, PASS_RHS);
+        if (qo)
+            for (int m = 0; m < 5; ++m) a.qo[m] = qo[m];
         if (d == 2 && rk_stage > 0) {
             a.rk_stage = rk_stage;
             a.rka = RKC[rk_stage - 1][0];
@@ -2981,12 +3028,13 @@ int do_stats(astr_ctx* c, double* out)
     const long n = (long)c->g.n[0] * c->g.n[1] * c->g.n[2];
     TRY(launch_pt(c, k_stats, n, a, 256));
     const long nb = (n + 255) / 256;
-    HIPCHK(hipMemcpyAsync(c->host_part.data(), c->part, sizeof(double) * 3 * nb,
+    HIPCHK(hipMemcpyAsync(c->host_part.data(), c->part, sizeof(double) * 6 * nb,
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    double s[3] = {0, 0, 0};
+    double s[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
     for (long b = 0; b < nb; ++b)
-        for (int k = 0; k < 3; ++k) s[k] += c->host_part[3 * b + k];
+        for (int k = 0; k < 3; ++k)
+            dd_add(s[k], sl[k], c->host_part[6 * b + 2 * k], c->host_part[6 * b + 2 * k + 1]);
     const double n3 = (double)c->g.n[0] * (double)c->g.n[1] * (double)c->g.n[2];
     // roinf = uinf = 1 and l_0 = 1 in the nondimensional box
     out[0] = 0.5 * s[0] / n3;
@@ -3127,15 +3175,15 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
 // adjoint fields: q 5, primitives 6, dvel 9, dtmp 3, G 15, qrhs 5, qsave 5
 constexpr int ADJ_FIELDS = 48;
 // what a stage stores for its reversal: q and the primitives after qswap,
-// the RK3 output
-constexpr int STAGE_FIELDS = 16;
+// their gradients (dvel, dtmp), the RK3 output
+constexpr int STAGE_FIELDS = 28;
 constexpr int STATE_FIELDS = 11;  // q and the primitives
 
 struct AdjFields {
     double *qb[5], *pb[6], *db[9], *tb[3], *gb[15], *rb[5], *sb[5];
 };
 struct StageStore {
-    double *q[5], *prim[6], *qn[5];
+    double *q[5], *prim[6], *qn[5], *dvel[9], *dtmp[3];
 };
 
 double* adj_field(astr_ctx* c, long k) { return c->adj_base + k * c->nfield; }
@@ -3169,6 +3217,8 @@ StageStore stage_slot(astr_ctx* c, long nstates, long k)
     for (int m = 0; m < 5; ++m) st.q[m] = adj_field(c, b + m);
     for (int m = 0; m < 6; ++m) st.prim[m] = adj_field(c, b + 5 + m);
     for (int m = 0; m < 5; ++m) st.qn[m] = adj_field(c, b + 11 + m);
+    for (int m = 0; m < 9; ++m) st.dvel[m] = adj_field(c, b + 16 + m);
+    for (int m = 0; m < 3; ++m) st.dtmp[m] = adj_field(c, b + 25 + m);
     return st;
 }
 
@@ -3196,7 +3246,10 @@ int load_state(astr_ctx* c, long k)
     return copy_fields(c, c->prim, pr, 6);
 }
 
-AdjArgs adj_args(astr_ctx* c, double* const* q, double* const* prim, double* const* qn)
+// the adjoint kernels' operands at the state (q, prim) with its gradients
+// (dvel, dtmp) and the stage output qn
+AdjArgs adj_args(astr_ctx* c, double* const* q, double* const* prim, double* const* qn,
+                 double* const* dvel, double* const* dtmp)
 {
     AdjArgs a{};
     a.g = c->g;
@@ -3209,8 +3262,8 @@ AdjArgs adj_args(astr_ctx* c, double* const* q, double* const* prim, double* con
         a.sb[m] = f.sb[m];
     }
     for (int m = 0; m < 6; ++m) { a.prim[m] = prim[m]; a.pb[m] = f.pb[m]; }
-    for (int m = 0; m < 9; ++m) { a.dvel[m] = c->dvel[m]; a.db[m] = f.db[m]; }
-    for (int m = 0; m < 3; ++m) { a.dtmp[m] = c->dtmp[m]; a.tb[m] = f.tb[m]; }
+    for (int m = 0; m < 9; ++m) { a.dvel[m] = dvel[m]; a.db[m] = f.db[m]; }
+    for (int m = 0; m < 3; ++m) { a.dtmp[m] = dtmp[m]; a.tb[m] = f.tb[m]; }
     for (int m = 0; m < 15; ++m) a.gb[m] = f.gb[m];
     a.jac = c->jac;
     for (int d = 0; d < 3; ++d) a.xi[d] = c->xi + d * c->xi_stride;
@@ -3237,33 +3290,72 @@ int launch_adj(astr_ctx* c, K kernel, long n, const AdjArgs& a)
     return ASTR_OK;
 }
 
-// gradcal of the given primitives into dvel and dtmp
-int grad_of(astr_ctx* c, double* const* prim)
+// Stages k0..k0+nst-1 (stage s = k % 3 + 1) as do_step runs them, from the
+// state in c->q and c->prim, each writing what its reversal reads straight
+// into its store slot instead of copying it there: the filter's x sweep
+// reads the incoming state and writes the slot's q (the y and z sweeps then
+// work in place), qswap completes q and the primitives in place, gradcal
+// writes the slot's dvel and dtmp, the RK3 combine writes the slot's qn,
+// and the primitive refresh writes the next slot's primitives.  The last
+// stage's output is copied back into c->q and its primitives go to c->prim;
+// the context's own buffers are restored.
+int segment_fwd(astr_ctx* c, long nstates, long nst, double dt)
 {
-    for (int d = 0; d < 3; ++d) {
-        SweepArgs a = sweep_args(c, d, PASS_GRAD);
-        for (int m = 0; m < 6; ++m) a.prim[m] = prim[m];
-        TRY(launch_sweep(c, a));
-    }
-    return ASTR_OK;
-}
-
-// one stage of the central path, as do_step runs it, storing what its
-// reversal needs
-int stage_fwd(astr_ctx* c, int s, double dt, const StageStore* st)
-{
-    if (c->p.lfilter) TRY(do_filterq(c));
-    TRY(do_qswap(c));
-    if (st) {
-        TRY(copy_fields(c, st->q, c->q, 5));
-        TRY(copy_fields(c, st->prim, c->prim, 6));
-    }
-    TRY(do_grad(c));
-    TRY(do_gflux(c));
-    TRY(do_flux_sweeps(c, s, dt));
-    TRY(launch_pt(c, k_prim, n_interior(c), pt_args(c)));
-    if (st) TRY(copy_fields(c, st->qn, c->q, 5));
-    return ASTR_OK;
+    if (nst <= 0) return ASTR_OK;
+    double *own_q[5], *own_prim[6], *own_dvel[9], *own_dtmp[3];
+    for (int m = 0; m < 5; ++m) own_q[m] = c->q[m];
+    for (int m = 0; m < 6; ++m) own_prim[m] = c->prim[m];
+    for (int m = 0; m < 9; ++m) own_dvel[m] = c->dvel[m];
+    for (int m = 0; m < 3; ++m) own_dtmp[m] = c->dtmp[m];
+    auto restore = [&] {
+        for (int m = 0; m < 5; ++m) c->q[m] = own_q[m];
+        for (int m = 0; m < 6; ++m) c->prim[m] = own_prim[m];
+        for (int m = 0; m < 9; ++m) c->dvel[m] = own_dvel[m];
+        for (int m = 0; m < 3; ++m) c->dtmp[m] = own_dtmp[m];
+    };
+    auto run = [&]() -> int {
+        TRY(copy_fields(c, stage_slot(c, nstates, 0).prim, own_prim, 6));
+        double* qin[5];
+        for (int m = 0; m < 5; ++m) qin[m] = own_q[m];
+        for (long k = 0; k < nst; ++k) {
+            const int s = (int)(k % 3) + 1;
+            const StageStore st = stage_slot(c, nstates, k);
+            if (c->p.lfilter) {
+                for (int d = 0; d < 3; ++d) {
+                    SweepArgs a = sweep_args(c, d, PASS_FILTER);
+                    for (int m = 0; m < 5; ++m) {
+                        a.q[m] = d == 0 ? qin[m] : st.q[m];
+                        a.qo[m] = st.q[m];
+                    }
+                    TRY(launch_sweep(c, a));
+                }
+            } else {
+                TRY(copy_fields(c, st.q, qin, 5));
+            }
+            for (int m = 0; m < 5; ++m) c->q[m] = st.q[m];
+            for (int m = 0; m < 6; ++m) c->prim[m] = st.prim[m];
+            for (int m = 0; m < 9; ++m) c->dvel[m] = st.dvel[m];
+            for (int m = 0; m < 3; ++m) c->dtmp[m] = st.dtmp[m];
+            TRY(do_qswap(c));
+            TRY(do_grad(c));
+            TRY(do_gflux(c));
+            TRY(do_flux_sweeps(c, s, dt, st.qn));
+            for (int m = 0; m < 5; ++m) c->q[m] = st.qn[m];
+            if (k + 1 < nst) {
+                const StageStore nx = stage_slot(c, nstates, k + 1);
+                for (int m = 0; m < 6; ++m) c->prim[m] = nx.prim[m];
+            } else {
+                for (int m = 0; m < 6; ++m) c->prim[m] = own_prim[m];
+            }
+            TRY(launch_pt(c, k_prim, n_interior(c), pt_args(c)));
+            for (int m = 0; m < 5; ++m) qin[m] = st.qn[m];
+            if (s == 3) c->time = c->time + dt;
+        }
+        return copy_fields(c, own_q, qin, 5);
+    };
+    const int rc = run();
+    restore();
+    return rc;
 }
 
 // gradcal reversed: primitive-bar (u, v, w, T; halos too) += D_d^T (xi_d
@@ -3287,7 +3379,6 @@ int grad_adj(astr_ctx* c)
 // halo and seam nodes, then the copies and averages in the order z, y, x
 int qswap_adj(astr_ctx* c, AdjArgs a)
 {
-    a.mode = 1;
     hipLaunchKernelGGL(k_q2f_adj, padded_grid(c), dim3(256), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     const long ext[3] = {c->g.n[0] + 1L, c->g.n[1] + 1L, c->g.n[2] + 1L};
@@ -3303,23 +3394,19 @@ int qswap_adj(astr_ctx* c, AdjArgs a)
 // stage's outputs on entry and of its inputs on return
 int stage_adj(astr_ctx* c, const StageStore& st, int s, double dt)
 {
-    AdjArgs a = adj_args(c, st.q, st.prim, st.qn);
+    AdjArgs a = adj_args(c, st.q, st.prim, st.qn, st.dvel, st.dtmp);
     a.stage = s;
     a.rka = RKC[s - 1][0];
     a.rkb = RKC[s - 1][1];
     a.rkc = RKC[s - 1][2];
     a.dt = dt;
     const AdjFields f = adj_fields(c);
-    // updatefvar, the RK3 combine
-    a.mode = 0;
-    hipLaunchKernelGGL(k_q2f_adj, padded_grid(c), dim3(256), 0, c->stream, a);
-    HIPCHK(hipGetLastError());
-    TRY(launch_adj(c, k_rk_adj, n_interior(c), a));
+    // updatefvar and the RK3 combine
+    TRY(launch_adj(c, k_update_adj, n_interior(c), a));
     // the flux derivatives: G-bar_d = D_d^T qrhs-bar
     for (int d = 0; d < 3; ++d)
         TRY(launch_sweep_t(c, d, OP_DERIV, 5, f.rb, f.gb + 5 * d, nullptr, false, false));
-    // the axis fluxes, at the gradients of the stored primitives
-    TRY(grad_of(c, st.prim));
+    // the axis fluxes, at the stored gradients
     TRY(launch_adj(c, k_gflux_adj_int, n_interior(c), a));
     const long ext[3] = {c->g.n[0] + 1L, c->g.n[1] + 1L, c->g.n[2] + 1L};
     for (int d = 0; d < 3; ++d) {
@@ -3347,7 +3434,7 @@ int objective_adj(astr_ctx* c, double* J)
     *J = st[1];
     const AdjFields f = adj_fields(c);
     HIPCHK(hipMemsetAsync(f.qb[0], 0, sizeof(double) * c->nfield * ADJ_FIELDS, c->stream));
-    AdjArgs a = adj_args(c, c->q, c->prim, nullptr);
+    AdjArgs a = adj_args(c, c->q, c->prim, nullptr, c->dvel, c->dtmp);
     TRY(launch_adj(c, k_ens_adj, (long)c->g.n[0] * c->g.n[1] * c->g.n[2], a));
     TRY(grad_adj(c));
     return qswap_adj(c, a);
@@ -3584,8 +3671,8 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     (void)hipMemset(c->xi, 0, sizeof(double) * 3 * c->xi_stride);
 
     c->npart = ((long)dims[0] * dims[1] * dims[2] + 255) / 256;
-    c->host_part.resize(3 * c->npart);
-    if (hipMalloc(&c->part, sizeof(double) * 3 * c->npart) != hipSuccess)
+    c->host_part.resize(6 * c->npart);
+    if (hipMalloc(&c->part, sizeof(double) * 6 * c->npart) != hipSuccess)
         return cleanup(fail(ASTR_E_NOMEM, "partials allocation failed"));
     for (int e = 0; e < 16; ++e)
         if (hipEventCreate(&c->ev[e]) != hipSuccess) return cleanup(fail(ASTR_E_HIP, "event"));
@@ -4084,15 +4171,15 @@ int astr_filter_field(astr_ctx* c, int32_t dir, const double* f, double* ff)
     return field_op(c, dir, f, ff, PASS_FILTER_FIELD);
 }
 
-int astr_adjoint_enstrophy(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt, double* J,
-                           double* qbar, double* pbar, double* ms2)
+int astr_adjoint_record(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt, double* J)
 {
     TRY(ready(c));
-    if (!J || !qbar || !pbar) return fail(ASTR_E_ARG, "null argument");
+    if (!J) return fail(ASTR_E_ARG, "null argument");
     if (nsteps < 0 || ckpt < 1) return fail(ASTR_E_ARG, "nsteps >= 0 and ckpt >= 1 required");
     if (c->p.convection != 0 || c->p.variant != 0 || c->g.n[2] == 0)
         return fail(ASTR_E_UNSUPPORTED, "the adjoint is built for the central path on periodic 3-D boxes");
     if (c->udf) return fail(ASTR_E_UNSUPPORTED, "the adjoint takes no user boundary condition");
+    c->rec_nsteps = -1;
     const long nseg = nsteps > 0 ? (nsteps + ckpt - 1) / ckpt : 0;
     const long nstates = nseg + 1;  // the segment starts and the final state
     const long seglen = std::min<long>(ckpt, std::max(nsteps, 1));
@@ -4104,37 +4191,56 @@ int astr_adjoint_enstrophy(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt,
         if (hipMalloc(&c->adj_base, sizeof(double) * c->nfield * need) != hipSuccess)
             return fail(ASTR_E_NOMEM, "device allocation of the adjoint workspace failed");
         c->adj_nf = need;
+        // the stage slots' cells that no stage writes (edges and corners of
+        // the halo) hold zeros rather than whatever the allocation held
+        HIPCHK(hipMemsetAsync(c->adj_base, 0, sizeof(double) * c->nfield * need, c->stream));
     }
-    HIPCHK(hipEventRecord(c->ev[12], c->stream));
     // forward, keeping the start of every segment; the last segment's
     // stages are stored as they run
-    for (long t = 0; t < nsteps; ++t) {
-        if (t % ckpt == 0) TRY(save_state(c, t / ckpt));
-        if (t / ckpt == nseg - 1) {
-            const long t0 = (nseg - 1) * ckpt;
-            for (int s = 1; s <= 3; ++s) {
-                const StageStore st = stage_slot(c, nstates, 3 * (t - t0) + (s - 1));
-                TRY(stage_fwd(c, s, dt, &st));
-            }
-            c->time = c->time + dt;
+    for (long seg = 0; seg < nseg; ++seg) {
+        const long t0 = seg * ckpt, t1 = std::min<long>(nsteps, t0 + ckpt);
+        TRY(save_state(c, seg));
+        if (seg == nseg - 1) {
+            TRY(segment_fwd(c, nstates, 3 * (t1 - t0), dt));
         } else {
-            TRY(do_step(c, dt, nullptr, false));
+            for (long t = t0; t < t1; ++t) TRY(do_step(c, dt, nullptr, false));
         }
     }
     TRY(save_state(c, nseg));
-    TRY(objective_adj(c, J));
+    // J at the final state after a qswap (the kept state is the one before)
+    TRY(do_qswap(c));
+    TRY(do_grad(c));
+    double st[3];
+    TRY(do_stats(c, st));
+    *J = st[1];
+    c->rec_nsteps = nsteps;
+    c->rec_ckpt = ckpt;
+    c->rec_dt = dt;
+    c->rec_t_end = c->time;
+    return ASTR_OK;
+}
+
+int astr_adjoint_reverse(astr_ctx* c, double* qbar, double* pbar, double* ms)
+{
+    TRY(ready(c));
+    if (!qbar || !pbar) return fail(ASTR_E_ARG, "null argument");
+    if (c->rec_nsteps < 0) return fail(ASTR_E_ARG, "no trajectory recorded (astr_adjoint_record)");
+    const long nsteps = c->rec_nsteps, ckpt = c->rec_ckpt;
+    const double dt = c->rec_dt;
+    const long nseg = nsteps > 0 ? (nsteps + ckpt - 1) / ckpt : 0;
+    const long nstates = nseg + 1;
     HIPCHK(hipEventRecord(c->ev[13], c->stream));
-    // reverse, segment by segment: recompute the segment's stages from its
-    // kept state (the last one is stored already), then reverse them
+    // the objective at the kept final state, then segment by segment:
+    // recompute the segment's stages from its kept state (the last one is
+    // stored already), then reverse them
+    TRY(load_state(c, nseg));
+    double J = 0.0;
+    TRY(objective_adj(c, &J));
     for (long seg = nseg - 1; seg >= 0; --seg) {
         const long t0 = seg * ckpt, t1 = std::min<long>(nsteps, t0 + ckpt);
         if (seg != nseg - 1) {
             TRY(load_state(c, seg));
-            for (long t = t0; t < t1; ++t)
-                for (int s = 1; s <= 3; ++s) {
-                    const StageStore st = stage_slot(c, nstates, 3 * (t - t0) + (s - 1));
-                    TRY(stage_fwd(c, s, dt, &st));
-                }
+            TRY(segment_fwd(c, nstates, 3 * (t1 - t0), dt));
         }
         for (long t = t1 - 1; t >= t0; --t)
             for (int s = 3; s >= 1; --s)
@@ -4142,12 +4248,10 @@ int astr_adjoint_enstrophy(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt,
     }
     HIPCHK(hipEventRecord(c->ev[14], c->stream));
     HIPCHK(hipEventSynchronize(c->ev[14]));
-    if (ms2) {
-        float a = 0.f, b = 0.f;
-        HIPCHK(hipEventElapsedTime(&a, c->ev[12], c->ev[13]));
+    if (ms) {
+        float b = 0.f;
         HIPCHK(hipEventElapsedTime(&b, c->ev[13], c->ev[14]));
-        ms2[0] = a;
-        ms2[1] = b;
+        *ms = b;
     }
     // the adjoint of the initial state, haloed host layout
     const AdjFields f = adj_fields(c);
@@ -4161,9 +4265,32 @@ int astr_adjoint_enstrophy(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt,
         TRY(download_field(c, f.pb[m], one.data(), stage, false));
         memcpy(pbar + m * nh, one.data(), sizeof(double) * nh);
     }
-    // leave the device state at the end of the forward steps
+    // leave the device state at the end of the forward steps, as the
+    // record left it (after the objective's qswap and gradients)
     TRY(load_state(c, nseg));
+    TRY(do_qswap(c));
+    TRY(do_grad(c));
+    c->time = c->rec_t_end;
     HIPCHK(hipStreamSynchronize(c->stream));
+    return ASTR_OK;
+}
+
+int astr_adjoint_enstrophy(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt, double* J,
+                           double* qbar, double* pbar, double* ms2)
+{
+    if (!c || !J || !qbar || !pbar) return fail(ASTR_E_ARG, "null argument");
+    TRY(set_dev(c));
+    HIPCHK(hipEventRecord(c->ev[12], c->stream));
+    TRY(astr_adjoint_record(c, nsteps, dt, ckpt, J));
+    HIPCHK(hipEventRecord(c->ev[15], c->stream));
+    double rev = 0.0;
+    TRY(astr_adjoint_reverse(c, qbar, pbar, &rev));
+    if (ms2) {
+        float a = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, c->ev[12], c->ev[15]));
+        ms2[0] = a;
+        ms2[1] = rev;
+    }
     return ASTR_OK;
 }
 

@@ -162,6 +162,27 @@ class AstrSolver:
               "astr_adjoint_enstrophy")
         return float(J[0]), qb, pb, (float(ms[0]), float(ms[1]))
 
+    def adjoint_record(self, nsteps: int, dt: float, ckpt: int = 8) -> float:
+        """The forward half of adjoint_enstrophy: nsteps RK3 steps from the
+        current state, kept every ckpt steps (the last segment's stages
+        stored), and J, the enstrophy of the final state
+        (astr_adjoint_record)."""
+        J = np.zeros(1)
+        check(self.lib.astr_adjoint_record(self.ctx, int(nsteps), float(dt), int(ckpt), dptr(J)),
+              "astr_adjoint_record")
+        return float(J[0])
+
+    def adjoint_reverse(self):
+        """(qbar, pbar, ms): the gradient of the recorded J with respect to
+        the recorded initial state, and the milliseconds of the reversal
+        (astr_adjoint_reverse)."""
+        qb = np.zeros(haloed_shape(self.p, (5,)), order="F")
+        pb = np.zeros(haloed_shape(self.p, (6,)), order="F")
+        ms = np.zeros(1)
+        check(self.lib.astr_adjoint_reverse(self.ctx, dptr(qb), dptr(pb), dptr(ms)),
+              "astr_adjoint_reverse")
+        return qb, pb, float(ms[0])
+
     def set_bc(self, bctype=(1, 1, 1, 1, 1, 1), udf: int = 0) -> None:
         """boucon's faces (i=0, i=im, j=0, j=jm, k=0, k=km): 1 nothing done,
         0 the user condition udf (UDF_RIEMANN2D: udf_bc of the 2-D Riemann

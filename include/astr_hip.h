@@ -177,8 +177,27 @@ int astr_line_adjoint(astr_ctx *ctx, int32_t op, int32_t dir, int32_t wrap, cons
 
 /* --- discrete adjoint ---------------------------------------------------- */
 
+/* The forward half of the gradient: nsteps plain RK3 steps from the
+ * current (uploaded) state, keeping the state at the start of every ckpt
+ * steps and every stage of the last segment in the adjoint workspace (device
+ * memory the context owns), then J = the enstrophy of the final state after
+ * a qswap (as astr_adjoint_enstrophy).  The device state is the one the
+ * plain steps reach, bit for bit, followed by that qswap.  A later
+ * astr_adjoint_reverse uses what is kept, whatever runs in between;
+ * another record replaces it.  No counterpart in the reference. */
+int astr_adjoint_record(astr_ctx *ctx, int32_t nsteps, double dt, int32_t ckpt, double *J);
+
+/* The reverse half: qbar and pbar (as astr_adjoint_enstrophy) of the
+ * trajectory astr_adjoint_record kept; the segments before the last are
+ * recomputed from their kept states.  ms (nullable) receives the
+ * milliseconds of the device work.  The device state is left at the
+ * recorded final state.  ASTR_E_ARG if nothing is recorded.  No
+ * counterpart in the reference. */
+int astr_adjoint_reverse(astr_ctx *ctx, double *qbar, double *pbar, double *ms);
+
 /* Gradient of the enstrophy after nsteps RK3 steps with respect to the
- * current (uploaded) state.  The steps are the plain forward (the central
+ * current (uploaded) state: astr_adjoint_record, then
+ * astr_adjoint_reverse.  The steps are the plain forward (the central
  * path on a periodic box, filter and viscous terms as configured).  J is the
  * enstrophy statistic of the final state after a qswap (enstophycal,
  * src/statistic.F90:870-925: 1/2 mean over the nodes 1..n of rho |omega|^2).
@@ -193,8 +212,8 @@ int astr_line_adjoint(astr_ctx *ctx, int32_t op, int32_t dir, int32_t wrap, cons
  *   - the RK3 combine reversed.
  * The steps are reversed segment by segment: the state is kept every ckpt
  * steps, and a segment's stages are recomputed from it before they are
- * reversed.  ms2 (nullable) receives the milliseconds of the forward
- * (with the objective) and of the reversal.  The device state is left at
+ * reversed.  ms2 (nullable) receives the milliseconds of the recorded
+ * forward (with the objective) and of the reversal.  The device state is left at
  * the end of the forward steps, with the time advanced.  No counterpart in
  * the reference. */
 int astr_adjoint_enstrophy(astr_ctx *ctx, int32_t nsteps, double dt, int32_t ckpt, double *J,

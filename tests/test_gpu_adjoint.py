@@ -116,16 +116,17 @@ def adjoint_case(dims, seed=3):
     return p, box, torch.tensor(u0), to_host
 
 
-@pytest.mark.parametrize("ckpt", [1, 8])
-def test_adjoint_matches_torch_autograd(ckpt):
+@pytest.mark.parametrize("nsteps,ckpt", [(2, 1), (2, 8), (3, 2)])
+def test_adjoint_matches_torch_autograd(nsteps, ckpt):
     """dJ/d(state) of the GPU reversal against PyTorch-CPU autograd of the
-    same forward (2 RK3 steps, filter and viscous terms on)."""
+    same forward (RK3 steps, filter and viscous terms on): every stage kept,
+    one segment of stored stages, and a last segment shorter than ckpt."""
     import torch
 
     dims = (16, 18, 20)
-    print(f"case adjoint vs autograd dims={dims} ckpt={ckpt}", flush=True)
+    print(f"case adjoint vs autograd dims={dims} nsteps={nsteps} ckpt={ckpt}", flush=True)
     p, box, u0, to_host = adjoint_case(dims)
-    dt, nsteps = 0.004, 2
+    dt = 0.004
     Q0, P0 = box.initial(u0)
     Q0 = Q0.detach().requires_grad_(True)
     P0 = P0.detach().requires_grad_(True)
@@ -146,6 +147,60 @@ def test_adjoint_matches_torch_autograd(ckpt):
             e = float(np.max(np.abs(a[..., m] - b[..., m])))
             print(f"  {name}[{m}] max|ref| {den:.3e} err {e:.3e}")
             assert e <= 1e-9 * max(den, float(np.max(np.abs(b)))), (name, m, e, den)
+    s.close()
+
+
+def at_most_one_halo(p):
+    """Mask of the haloed nodes outside at most one axis' range 0..n (the
+    edges and corners of the halo belong to no state)."""
+    ext = [k + 1 for k in (p.im, p.jm, p.km)]
+    out = [np.zeros(e + 2 * HM, dtype=int) for e in ext]
+    for o, e in zip(out, ext):
+        o[:HM] = 1
+        o[HM + e:] = 1
+    return (out[0][:, None, None] + out[1][None, :, None] + out[2][None, None, :]) <= 1
+
+
+def test_record_then_reverse():
+    """astr_adjoint_record reaches the state of the plain steps bit for bit
+    (then qswap); astr_adjoint_reverse gives what astr_adjoint_enstrophy
+    gives, with other work on the context in between, and leaves the
+    recorded final state; reversing before a record is an error."""
+    dims = (16, 18, 20)
+    print(f"case record/reverse dims={dims}", flush=True)
+    p = ag.default_params(dims, mach=0.3, reynolds=173.0)
+    st = ag.hit_state(p, 3)
+    s = ag.AstrSolver(p)
+    s.upload_geom(*ag.box_metrics(p))
+
+    def up():
+        s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+
+    with pytest.raises(ag.AstrError):
+        s.adjoint_reverse()
+    nsteps, dt, ckpt = 5, 0.004, 2
+    up()
+    s.steploop(nsteps, dt)
+    s.qswap()
+    ref = s.download_state()
+    up()
+    J = s.adjoint_record(nsteps, dt, ckpt)
+    got = s.download_state()
+    mask = at_most_one_halo(p)
+    for k in ("q", "rho", "vel", "prs", "tmp"):
+        a, b = ref[k], got[k]
+        assert np.array_equal(a[mask], b[mask]), k
+    up()
+    s.steploop(1, dt)
+    qb, pb, ms = s.adjoint_reverse()
+    after = s.download_state()
+    for k in ("q", "rho", "vel", "prs", "tmp"):
+        assert np.array_equal(ref[k][mask], after[k][mask]), k
+    up()
+    J2, qb2, pb2, _ = s.adjoint_enstrophy(nsteps, dt, ckpt)
+    print(f"  J {J:.15e} {J2:.15e}; reversal {ms:.2f} ms")
+    assert J == J2
+    assert np.array_equal(qb, qb2) and np.array_equal(pb, pb2)
     s.close()
 
 
