@@ -204,6 +204,33 @@ module astr_hip
       real(c_double), intent(out) :: time
     end function astr_get_time
 
+    ! the discrete adjoint of nsteps RK3 steps, J = enstrophy of the final
+    ! state; qbar(-hm:..,1:5), pbar(-hm:..,1:6) of the initial state
+    integer(c_int) function astr_adjoint_record(ctx, nsteps, dt, ckpt, j) &
+        bind(c, name='astr_adjoint_record')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsteps, ckpt
+      real(c_double), value :: dt
+      real(c_double), intent(out) :: j
+    end function astr_adjoint_record
+
+    integer(c_int) function astr_adjoint_reverse(ctx, qbar, pbar, ms) &
+        bind(c, name='astr_adjoint_reverse')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: qbar(*), pbar(*), ms
+    end function astr_adjoint_reverse
+
+    integer(c_int) function astr_adjoint_enstrophy(ctx, nsteps, dt, ckpt, j, qbar, pbar, ms2) &
+        bind(c, name='astr_adjoint_enstrophy')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsteps, ckpt
+      real(c_double), value :: dt
+      real(c_double), intent(out) :: j, qbar(*), pbar(*), ms2(2)
+    end function astr_adjoint_enstrophy
+
     integer(c_int) function astr_synchronize(ctx) bind(c, name='astr_synchronize')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
