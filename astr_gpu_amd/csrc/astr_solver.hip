@@ -451,11 +451,17 @@ struct TSweepArgs {
     int accumulate;     // add into fout instead of overwriting it
 };
 
+// the interior filter coefficients c10 (filter_coefficient_cal), held in
+// registers for the whole kernel
+struct C10 {
+    double c[6];
+};
+
 // (B^T w) at column node j of one line; w holds the solved rows at
 // ln[n + HM] (zero outside first..last), read as zero beyond -HM..dim+HM
 template <int OP>
 __device__ __forceinline__ double bt_column(const double* ln, int j, int dim, const LineTab& tb,
-                                            const TransTab& tt, const FiltCoef* __restrict__ fc)
+                                            const TransTab& tt, const C10& cf)
 {
     constexpr int HW = OP == OP_DERIV ? 2 : 5;
     if (j - HW >= -HM && j + HW <= dim + HM && j >= tt.lo0 + TC && j < tt.hi0) {
@@ -464,9 +470,9 @@ __device__ __forceinline__ double bt_column(const double* ln, int j, int dim, co
         if constexpr (OP == OP_DERIV) {
             return (7.0 / 9.0) * (w[-1] - w[1]) + (1.0 / 36.0) * (w[-2] - w[2]);
         } else {
-            double v = fc->c10[0] * (w[0] + w[0]);
+            double v = cf.c[0] * (w[0] + w[0]);
 #pragma unroll
-            for (int k = 1; k <= 5; ++k) v += fc->c10[k] * (w[-k] + w[k]);
+            for (int k = 1; k <= 5; ++k) v += cf.c[k] * (w[-k] + w[k]);
             return v;
         }
     }
@@ -475,9 +481,9 @@ __device__ __forceinline__ double bt_column(const double* ln, int j, int dim, co
     if constexpr (OP == OP_DERIV) {
         v = (7.0 / 9.0) * (W(j - 1) - W(j + 1)) + (1.0 / 36.0) * (W(j - 2) - W(j + 2));
     } else {
-        v = fc->c10[0] * (W(j) + W(j));
+        v = cf.c[0] * (W(j) + W(j));
 #pragma unroll
-        for (int k = 1; k <= 5; ++k) v += fc->c10[k] * (W(j - k) + W(j + k));
+        for (int k = 1; k <= 5; ++k) v += cf.c[k] * (W(j - k) + W(j + k));
     }
     int c = j - tt.lo0;
     if (c >= 0 && c < TC)
@@ -528,6 +534,9 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
     }
     const unsigned int lob = (unsigned int)lo * 8u, sdb = (unsigned int)sd * 8u;
     double* const trow = tile + ll * a.rs_in + TG;
+    C10 cf;  // wave-uniform loads before any divergent branch
+#pragma unroll
+    for (int k = 0; k < 6; ++k) cf.c[k] = OP == OP_FILTER ? a.fc->c10[k] : 0.0;
     auto wsync = []() {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -586,7 +595,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
 #pragma unroll
         for (int t = 0; t <= L; ++t) {
             const int j = -HM + sub + 32 * t;
-            f[t] = j <= dim + HM ? bt_column<OP>(ln, j, dim, a.tab, a.tt, a.fc) : 0.0;
+            f[t] = j <= dim + HM ? bt_column<OP>(ln, j, dim, a.tab, a.tt, cf) : 0.0;
             if (k0 && j == 0) f[t] += g0;
             if (kd && j == dim) f[t] += gd;
         }

@@ -105,27 +105,31 @@ def test_filter_adjoint_with_periodic_halos(d):
 
 # ---------------------------------------------------------------- full adjoint
 
-def adjoint_case(dims, seed=3):
+def adjoint_case(dims, seed=3, lengths=(2 * np.pi,) * 3):
     import torch
     from torch_forward import Box, to_host
 
     p = ag.default_params(dims, mach=0.3, reynolds=173.0)
     st = ag.hit_state(p, seed)
     u0 = np.moveaxis(ag.interior(st["vel"], p)[:-1, :-1, :-1], -1, 0).copy()
-    box = Box(dims, reynolds=p.reynolds, mach=p.mach)
+    box = Box(dims, reynolds=p.reynolds, mach=p.mach, lengths=lengths)
     return p, box, torch.tensor(u0), to_host
 
 
-@pytest.mark.parametrize("nsteps,ckpt", [(2, 1), (2, 8), (3, 2)])
-def test_adjoint_matches_torch_autograd(nsteps, ckpt):
+@pytest.mark.parametrize("nsteps,ckpt,lengths", [
+    (2, 1, None), (2, 8, None), (3, 2, None), (2, 2, (2 * np.pi, 4.0, 9.0))])
+def test_adjoint_matches_torch_autograd(nsteps, ckpt, lengths):
     """dJ/d(state) of the GPU reversal against PyTorch-CPU autograd of the
     same forward (RK3 steps, filter and viscous terms on): every stage kept,
-    one segment of stored stages, and a last segment shorter than ckpt."""
+    one segment of stored stages, a last segment shorter than ckpt, and a
+    box whose three axes have different spacings (metrics and J)."""
     import torch
 
     dims = (16, 18, 20)
-    print(f"case adjoint vs autograd dims={dims} nsteps={nsteps} ckpt={ckpt}", flush=True)
-    p, box, u0, to_host = adjoint_case(dims)
+    lengths = lengths or (2 * np.pi,) * 3
+    print(f"case adjoint vs autograd dims={dims} nsteps={nsteps} ckpt={ckpt} lengths={lengths}",
+          flush=True)
+    p, box, u0, to_host = adjoint_case(dims, lengths=lengths)
     dt = 0.004
     Q0, P0 = box.initial(u0)
     Q0 = Q0.detach().requires_grad_(True)
@@ -133,7 +137,7 @@ def test_adjoint_matches_torch_autograd(nsteps, ckpt):
     J = box.objective_state(Q0, P0, nsteps, dt)
     J.backward()
     s = ag.AstrSolver(p)
-    s.upload_geom(*ag.box_metrics(p))
+    s.upload_geom(*ag.box_metrics(p, lengths))
     h = to_host(Q0, P0)
     s.upload_state(h["q"], h["rho"], h["vel"], h["prs"], h["tmp"])
     Jg, qb, pb, ms = s.adjoint_enstrophy(nsteps, dt, ckpt)
