@@ -457,41 +457,26 @@ struct C10 {
     double c[6];
 };
 
-// (B^T w) at column node j of one line; w holds the solved rows at
-// ln[n + HM] (zero outside first..last), read as zero beyond -HM..dim+HM
+// LDS guard slots before each line of a transposed sweep, zero: the
+// interior stencil of B^T then reads zeros beyond node -HM (the tail slots
+// past dim+HM hold zeros from the staging)
+constexpr int TGT = 5;
+
+// (B^T w) at column node j of one line with the interior stencil of every
+// row, closure rows included (their difference is added separately); w
+// holds the solved rows at ln[n + HM] and zero elsewhere
 template <int OP>
-__device__ __forceinline__ double bt_column(const double* ln, int j, int dim, const LineTab& tb,
-                                            const TransTab& tt, const C10& cf)
+__device__ __forceinline__ double bt_interior(const double* ln, int j, const C10& cf)
 {
-    constexpr int HW = OP == OP_DERIV ? 2 : 5;
-    if (j - HW >= -HM && j + HW <= dim + HM && j >= tt.lo0 + TC && j < tt.hi0) {
-        // away from both ends: the interior stencil, no closure rows
-        const double* w = ln + j + HM;
-        if constexpr (OP == OP_DERIV) {
-            return (7.0 / 9.0) * (w[-1] - w[1]) + (1.0 / 36.0) * (w[-2] - w[2]);
-        } else {
-            double v = cf.c[0] * (w[0] + w[0]);
-#pragma unroll
-            for (int k = 1; k <= 5; ++k) v += cf.c[k] * (w[-k] + w[k]);
-            return v;
-        }
-    }
-    auto W = [&](int n) { return (n >= -HM && n <= dim + HM) ? ln[n + HM] : 0.0; };
-    double v;
+    const double* w = ln + j + HM;
     if constexpr (OP == OP_DERIV) {
-        v = (7.0 / 9.0) * (W(j - 1) - W(j + 1)) + (1.0 / 36.0) * (W(j - 2) - W(j + 2));
+        return (7.0 / 9.0) * (w[-1] - w[1]) + (1.0 / 36.0) * (w[-2] - w[2]);
     } else {
-        v = cf.c[0] * (W(j) + W(j));
+        double v = cf.c[0] * (w[0] + w[0]);
 #pragma unroll
-        for (int k = 1; k <= 5; ++k) v += cf.c[k] * (W(j - k) + W(j + k));
+        for (int k = 1; k <= 5; ++k) v += cf.c[k] * (w[-k] + w[k]);
+        return v;
     }
-    int c = j - tt.lo0;
-    if (c >= 0 && c < TC)
-        for (int r = 0; r < tt.nlo; ++r) v += tt.dlo[r * TC + c] * W(tb.first + r);
-    c = j - tt.hi0;
-    if (c >= 0 && c < TC)
-        for (int k = 0; k < tt.nhi; ++k) v += tt.dhi[k * TC + c] * W(tb.last - k);
-    return v;
 }
 
 // One workgroup per tile of lines and quantity (the forward sweep's tiles,
@@ -533,10 +518,20 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
         else lo = (int)a.g.org + av + b * (int)a.g.s[1];
     }
     const unsigned int lob = (unsigned int)lo * 8u, sdb = (unsigned int)sd * 8u;
-    double* const trow = tile + ll * a.rs_in + TG;
+    double* const trow = tile + ll * a.rs_in + TGT;
     C10 cf;  // wave-uniform loads before any divergent branch
 #pragma unroll
     for (int k = 0; k < 6; ++k) cf.c[k] = OP == OP_FILTER ? a.fc->c10[k] : 0.0;
+    // the closure rows' corrections this lane applies on every line: lanes
+    // 0-15 column lo0 + sub of the low block, lanes 16-31 column hi0 + sub -
+    // 16 of the high block (rows past nlo / nhi contribute zero)
+    const int csub = lane & 31;
+    double dcf[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        if (csub < TC) dcf[r] = r < a.tt.nlo ? a.tt.dlo[r * TC + csub] : 0.0;
+        else dcf[r] = r < a.tt.nhi ? a.tt.dhi[r * TC + csub - TC] : 0.0;
+    }
     auto wsync = []() {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -546,7 +541,13 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
         else __syncthreads();
     };
 
-    // ---------------- stage g on the nodes 0..dim, zero on the rest
+    // ---------------- stage g on the nodes 0..dim, zero on the rest and on
+    // the guard slots (x: each half-wavefront its own line)
+    if (XD) {
+        if (csub < TGT) tile[half * a.rs_in + csub] = 0.0;
+    } else if (tid < tx * TGT) {
+        tile[(tid / TGT) * a.rs_in + tid % TGT] = 0.0;
+    }
 #pragma unroll
     for (int e = 0; e < EMAX; ++e) {
         if (e < esteps) {
@@ -565,7 +566,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
     // ---------------- A^T w = g, then B^T w, one line per half-wavefront
     for (int l = half; l < tx; l += 8) {
         if (l - (lane >> 5) >= nvalid) break;  // wave-uniform: both halves invalid
-        double* ln = tile + l * a.rs_in + TG;
+        double* ln = tile + l * a.rs_in + TGT;
         const int sub = lane & 31;
         const int r0 = sub * L;
         const int R = a.tab.R;
@@ -589,13 +590,20 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
                 if (r0 + t < R) ln[HM + a.tab.first + r0 + t] = x[t];
         }
         wsync();
-        // B^T w, columns j = -HM + sub + 32 t (a half-wavefront reads 32
-        // consecutive LDS slots at a time)
+        // the closure rows' difference from the interior stencil, on this
+        // lane's column of the low or high block (the rows' w are
+        // wave-uniform reads)
+        double corr = 0.0;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+            corr += dcf[r] * ln[HM + (sub < TC ? a.tab.first + r : a.tab.last - r)];
+        // B^T w with the interior stencil, columns j = -HM + sub + 32 t (a
+        // half-wavefront reads 32 consecutive LDS slots at a time)
         double f[L + 1];
 #pragma unroll
         for (int t = 0; t <= L; ++t) {
             const int j = -HM + sub + 32 * t;
-            f[t] = j <= dim + HM ? bt_column<OP>(ln, j, dim, a.tab, a.tt, cf) : 0.0;
+            f[t] = j <= dim + HM ? bt_interior<OP>(ln, j, cf) : 0.0;
             if (k0 && j == 0) f[t] += g0;
             if (kd && j == dim) f[t] += gd;
         }
@@ -607,6 +615,13 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
                 if (j <= dim + HM) ln[HM + j] = f[t];
             }
         }
+        wsync();
+        // the corrections, low block first (the blocks overlap on short lines)
+        const int col = sub < TC ? a.tt.lo0 + sub : a.tt.hi0 + sub - TC;
+        const bool cin = l < nvalid && col >= -HM && col <= dim + HM;
+        if (cin && sub < TC) ln[HM + col] += corr;
+        wsync();
+        if (cin && sub >= TC) ln[HM + col] += corr;
     }
     sync();
 
@@ -2629,6 +2644,12 @@ int launch_sweep(astr_ctx* c, const SweepArgs& a)
 
 // ---- transposed sweeps
 
+// LDS row of a transposed sweep: TGT zero guard slots + the staged nodes
+int lds_row_t(int dir, int L)
+{
+    return (TGT + (dir == 0 ? 32 * (L + 1) : 16 * (2 * L + 1))) | 1;
+}
+
 TSweepArgs tsweep_args(astr_ctx* c, int dir, int op)
 {
     TSweepArgs a{};
@@ -2643,7 +2664,7 @@ TSweepArgs tsweep_args(astr_ctx* c, int dir, int op)
     else { a.na = n0 + 1; a.nb = n1 + 1; }
     const int tx = dir == 0 ? TXX : TXY;
     a.ntile_a = (a.na + tx - 1) / tx;
-    a.rs_in = lds_row(dir, c->L[dir]);
+    a.rs_in = lds_row_t(dir, c->L[dir]);
     return a;
 }
 
@@ -2651,7 +2672,7 @@ template <int OP>
 int launch_sweep_t_op(astr_ctx* c, const TSweepArgs& a)
 {
     const int L = c->L[a.dir];
-    const size_t lds = lds_bytes(a.dir, L);
+    const size_t lds = sizeof(double) * (a.dir == 0 ? TXX : TXY) * (size_t)a.rs_in;
     const int ntiles = a.ntile_a * a.nb;
     const dim3 grid((ntiles + 7) / 8 * 8 * a.nq);
     switch (L) {
