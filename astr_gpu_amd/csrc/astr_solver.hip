@@ -351,22 +351,18 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
         double* ln = tile + l * a.rs_in + TG;
         const double keep0 = ln[HM], keepd = ln[HM + dim];
         solve_line<L, OP>(ln, a.tab, a.fc, x);
-        if (l < nvalid) {
-            const int sub = lane & 31;
+        // every row goes back to its node's slot: rows outside 0..dim (and
+        // the zero padding rows past R) land in slots of the staged input,
+        // which the half-wavefront has finished reading and nothing reads
+        // again
+        const int sub = lane & 31;
 #pragma unroll
-            for (int t = 0; t < L; ++t) {
-                const int node = a.tab.first + sub * L + t;
-                if (sub * L + t < a.tab.R && node >= 0 && node <= dim) {
-                    double v = x[t];
-                    if constexpr (OP == OP_FILTER) {
-                        // compact_filter keeps the boundary node
-                        // (src/filter.F90:141-142)
-                        if (node == 0 && lo_boundary(a.tab.ntype)) v = keep0;
-                        if (node == dim && hi_boundary(a.tab.ntype)) v = keepd;
-                    }
-                    ln[node + HM] = v;
-                }
-            }
+        for (int t = 0; t < L; ++t) ln[HM + a.tab.first + sub * L + t] = x[t];
+        if constexpr (OP == OP_FILTER) {
+            // compact_filter keeps the boundary node (src/filter.F90:141-142);
+            // the lane that wrote the node overwrites it
+            if (lo_boundary(a.tab.ntype) && sub == (0 - a.tab.first) / L) ln[HM] = keep0;
+            if (hi_boundary(a.tab.ntype) && sub == (dim - a.tab.first) / L) ln[HM + dim] = keepd;
         }
     }
     sync();
