@@ -676,13 +676,21 @@ struct PtArgs {
     double* part;
 };
 
+// (i, j, k) of flat index p over an ni x nj x nk box, i fastest, in 32-bit
+// arithmetic (every box here has fewer than 2^32 nodes; a 64-bit division
+// is a long instruction sequence on the GPU)
+__device__ __forceinline__ void split3(long p, long ni, long nj, int& i, int& j, int& k)
+{
+    const unsigned up = (unsigned)p, ui = (unsigned)ni, uj = (unsigned)nj;
+    const unsigned r = up / ui;
+    i = (int)(up - r * ui);
+    k = (int)(r / uj);
+    j = (int)(r - (unsigned)k * uj);
+}
+
 __device__ __forceinline__ void interior_ijk(const Geo& g, long p, int& i, int& j, int& k)
 {
-    const long ni = g.n[0] + 1, nj = g.n[1] + 1;
-    i = (int)(p % ni);
-    const long r = p / ni;
-    j = (int)(r % nj);
-    k = (int)(r / nj);
+    split3(p, g.n[0] + 1, g.n[1] + 1, i, j, k);
 }
 
 // q2fvar_3da, src/fludyna.F90:545-634 (nondimensional, numq = 5), evaluated
@@ -1305,7 +1313,9 @@ __global__ __launch_bounds__(256) void k_update_adj(AdjArgs a)
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
     if (p >= ni * nj * (a.g.n[2] + 1)) return;
-    const long gi = gidx(a.g, (int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj)));
+    int ci, cj, ck;
+    split3(p, ni, nj, ci, cj, ck);
+    const long gi = gidx(a.g, ci, cj, ck);
     const double jc = a.jac[gi];
     double qn[5], pbv[6], ob[5], sv[5] = {0, 0, 0, 0, 0};
 #pragma unroll
@@ -1414,7 +1424,8 @@ __global__ void k_gflux_adj_int(AdjArgs a)
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
     if (p >= ni * nj * (a.g.n[2] + 1)) return;
-    const int c[3] = {(int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj))};
+    int c[3];
+    split3(p, ni, nj, c[0], c[1], c[2]);
     const long gi = gidx(a.g, c[0], c[1], c[2]);
     const double jc = a.jac[gi];
     // every operand first (a store could alias a later load); primitive-bar
@@ -1640,7 +1651,8 @@ __global__ void k_ducros_ssf(SensorArgs a)
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
     if (p >= ni * nj * (a.g.n[2] + 1)) return;
-    const int i = (int)(p % ni), j = (int)((p / ni) % nj), k = (int)(p / (ni * nj));
+    int i, j, k;
+    split3(p, ni, nj, i, j, k);
     const long gi = gidx(a.g, i, j, k);
     const double div = a.dvel[0][gi] + a.dvel[4][gi] + a.dvel[8][gi];
     const double div2 = div * div;
@@ -1679,8 +1691,11 @@ __global__ void k_ssf_halo(SensorArgs a)
     const long ni = a.g.n[0] + 1 + 2 * HM, nj = a.g.n[1] + 1 + 2 * HM;
     const long nk = a.ndims == 2 ? 1 : a.g.n[2] + 1 + 2 * HM;
     if (p >= ni * nj * nk) return;
-    int c[3] = {(int)(p % ni) - HM, (int)((p / ni) % nj) - HM,
-                a.ndims == 2 ? 0 : (int)(p / (ni * nj)) - HM};
+    int c[3];
+    split3(p, ni, nj, c[0], c[1], c[2]);
+    c[0] -= HM;
+    c[1] -= HM;
+    c[2] = a.ndims == 2 ? 0 : c[2] - HM;
     bool halo = false, keep = true;
     int s[3];
     for (int d = 0; d < 3; ++d) {
@@ -1701,7 +1716,8 @@ __global__ void k_ducros_flag(SensorArgs a)
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
     if (p >= ni * nj * (a.g.n[2] + 1)) return;
-    const int c0[3] = {(int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj))};
+    int c0[3];
+    split3(p, ni, nj, c0[0], c0[1], c0[2]);
     double m = 0.0;
     for (int d = 0; d < a.ndims; ++d)
         for (int o = -HM + 1; o <= HM; ++o) {
@@ -1748,7 +1764,8 @@ __global__ void k_grad_explicit(GradXArgs a)
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
     if (p >= ni * nj * (a.g.n[2] + 1)) return;
-    const int c[3] = {(int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj))};
+    int c[3];
+    split3(p, ni, nj, c[0], c[1], c[2]);
     const long gi = gidx(a.g, c[0], c[1], c[2]);
     for (int d = 0; d < 3; ++d) {
         if (d >= a.ndims) {
@@ -1782,7 +1799,8 @@ __global__ void k_deriv_explicit_field(FieldXArgs a)
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
     if (p >= ni * nj * (a.g.n[2] + 1)) return;
-    const int c[3] = {(int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj))};
+    int c[3];
+    split3(p, ni, nj, c[0], c[1], c[2]);
     const long gi = gidx(a.g, c[0], c[1], c[2]);
     const long st = a.g.s[a.dir];
     a.fout[gi] = diff6ec_at(a.fin + (gi - (long)c[a.dir] * st), st, c[a.dir], a.g.n[a.dir], a.ntype);
@@ -1919,7 +1937,9 @@ __global__ void k_zero_qrhs(UwdArgs a)
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
     if (p >= ni * nj * (a.g.n[2] + 1)) return;
-    const long gi = gidx(a.g, (int)(p % ni), (int)((p / ni) % nj), (int)(p / (ni * nj)));
+    int ci, cj, ck;
+    split3(p, ni, nj, ci, cj, ck);
+    const long gi = gidx(a.g, ci, cj, ck);
     for (int m = 0; m < 5; ++m) a.qrhs[m][gi] = 0.0;
 }
 
