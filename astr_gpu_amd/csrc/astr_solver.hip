@@ -66,6 +66,7 @@ constexpr int TX = TXY;    // largest tile
 constexpr int WGT = 256;   // threads per sweep workgroup
 constexpr int TG = 3;      // LDS guard slots before each staged line
 constexpr int OI = 16;     // element offset of node i=0 in a row
+static_assert(OI % 16 == 0, "node i = 0 starts a 128-byte line");
 constexpr int LMAX = 16;   // rows per lane supported (line length <= 32*LMAX)
 
 struct Geo {
@@ -78,6 +79,13 @@ __host__ __device__ __forceinline__ long gidx(const Geo& g, int i, int j, int k)
 {
     return g.org + i + j * g.s[1] + k * g.s[2];
 }
+
+// Node i = 0 of every row starts a 128-byte line (OI = 16 doubles, rows a
+// multiple of 16).  A kernel that writes nodes 0..n of a row also writes,
+// where the field's cells past n are dead or rewritten with their own
+// values, the nodes up to the end of the line holding node n: HBM takes a
+// partly written line at a far higher cost than a whole one.
+__host__ __device__ constexpr int line_end_i(int n) { return (n / 16 + 1) * 16; }
 
 // ---------------------------------------------------------------- sweeps
 
@@ -312,6 +320,21 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     }
     const unsigned int lob = (unsigned int)lo * 8u, sdb = (unsigned int)sd * 8u;
     double* const trow = tile + ll * a.rs_in + TG;
+    // Fields whose cells past node n are dead (dvel, dtmp, qrhs) are written
+    // to the end of the 128-byte line holding node n (line_end_i): x lines
+    // run on past dim, and in y/z tiles the lanes of lines a0 + ll past the
+    // last valid one store too, at their own line (lob_w), whatever their
+    // LDS row holds.
+    const bool dead = PASS == PASS_GRAD || (PASS == PASS_RHS && (XD || a.rk_stage == 0));
+    const bool lw = lvalid || (!XD && dead && a0 + ll < line_end_i(a.g.n[0]));
+    unsigned int lob_w = lob;
+    if (!XD && !lvalid) {
+        const int av = a0 + ll;
+        const int low = dir == 1 ? (int)a.g.org + av + b * (int)a.g.s[2]
+                                 : (int)a.g.org + av + b * (int)a.g.s[1];
+        lob_w = (unsigned int)low * 8u;
+    }
+    const int nlast = XD && dead ? line_end_i(dim) - 1 : dim;  // last node stored
     auto sync = [&]() {
         if constexpr (XD) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -371,10 +394,10 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     if constexpr (PASS == PASS_RHS && !XD) {
         // read-modify-write of qrhs (y) or the RK3 combine (z), specialised
         // per mode so that each is straight-line code
-        if (lvalid) {
+        if (lw) {
             const int mode = a.rk_stage == 0 ? 0 : (a.mini ? 3 : 1) + (a.rk_stage > 1);
             switch (mode) {
-            case 0: rhs_epilogue<0, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
+            case 0: rhs_epilogue<0, EMAX>(a, m, trow, n0, nstep, dim, lob_w, sdb); break;
             case 1: rhs_epilogue<1, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
             case 2: rhs_epilogue<2, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
             case 3: rhs_epilogue<3, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
@@ -383,19 +406,19 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
         }
         return;
     }
-    if (lvalid) {
+    if (lw) {
 #pragma unroll
         for (int e = 0; e < EMAX; ++e) {
             const int n = n0 + nstep * e;
-            if (e < esteps && n <= dim) {
+            if (e < esteps && n <= nlast) {
                 const double o = trow[n + HM];
-                const unsigned int od = lob + (unsigned int)n * sdb;
+                const unsigned int od = lob_w + (unsigned int)n * sdb;
                 if constexpr (PASS == PASS_FILTER) {
                     stb(a.qo[m], od, o);
                 } else if constexpr (PASS == PASS_GRAD) {
                     // gradcal: d/dxi times dxi(d,d) (src/comsolver.F90:304-318),
                     // or df/dx (tgvsolver.F90:420-470)
-                    const double gv = a.mini ? o / a.h : o * a.xi[n + HM];
+                    const double gv = a.mini ? o / a.h : o * a.xi[min(n, dim) + HM];
                     if (m < 3) stb(a.dvel[m * 3 + dir], od, gv);
                     else stb(a.dtmp[dir], od, gv);
                 } else if constexpr (PASS == PASS_RHS) {
@@ -1115,18 +1138,32 @@ __global__ void k_rk(PtArgs a)
 
 // updatefvar (src/fludyna.F90:191-242) on the interior, once the last flux
 // sweep of the stage has applied the RK3 combine to q.
-__global__ void k_prim(PtArgs a)
+// updatefvar on the nodes 0..n (src/fludyna.F90:191-242).  One wavefront
+// per 64 nodes of an i-row, the chunks of a row consecutive; the lanes
+// past node n up to the end of its 128-byte line store the primitives
+// already there, so every line is written whole.
+__global__ __launch_bounds__(64) void k_prim(PtArgs a)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long ntot = (long)(a.g.n[0] + 1) * (a.g.n[1] + 1) * (a.g.n[2] + 1);
-    if (p >= ntot) return;
-    int i, j, k;
-    interior_ijk(a.g, p, i, j, k);
-    const long gi = gidx(a.g, i, j, k);
-    double qv[5];
+    const unsigned nch = (unsigned)(line_end_i(a.g.n[0]) + 63) / 64;
+    const unsigned row = blockIdx.x / nch;
+    const int i = (int)(blockIdx.x - row * nch) * 64 + (int)threadIdx.x;
+    if (i >= line_end_i(a.g.n[0])) return;
+    const unsigned nj = (unsigned)a.g.n[1] + 1;
+    const unsigned k = row / nj, j = row - k * nj;
+    const long gi = gidx(a.g, i, (int)j, (int)k);
+    if (i <= a.g.n[0]) {
+        double qv[5];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) qv[m] = a.q[m][gi];
-    q2fvar(a, qv, gi);
+        for (int m = 0; m < 5; ++m) qv[m] = a.q[m][gi];
+        q2fvar(a, qv, gi);
+    } else {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            double v = a.prim[m][gi];
+            asm volatile("" : "+v"(v));  // the store is not dropped as redundant
+            a.prim[m][gi] = v;
+        }
+    }
 }
 
 // s + e = a + b exactly (Knuth's TwoSum; plain IEEE additions)
@@ -2743,6 +2780,17 @@ int launch_pt(astr_ctx* c, void (*k)(PtArgs), long nthreads, const PtArgs& a, in
     return ASTR_OK;
 }
 
+// k_prim: one wavefront per 64 nodes of each i-row, to the end of the line
+// holding node n
+int launch_prim(astr_ctx* c)
+{
+    const dim3 grid((unsigned)((c->g.n[1] + 1) * (c->g.n[2] + 1) *
+                               ((line_end_i(c->g.n[0]) + 63) / 64)));
+    hipLaunchKernelGGL(k_prim, grid, dim3(64), 0, c->stream, pt_args(c));
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
+}
+
 int set_dev(astr_ctx* c)
 {
     HIPCHK(hipSetDevice(c->device));
@@ -3161,7 +3209,7 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
             TRY(do_flux_sweeps(c, s, dt));
             if (timed) TRY(time_mark(c, 6));
             if (c->p.lfilter) TRY(do_filterq(c));
-            TRY(launch_pt(c, k_prim, n_interior(c), pt_args(c)));
+            TRY(launch_prim(c));
             if (timed) {
                 TRY(time_mark(c, 7));
                 HIPCHK(hipEventSynchronize(c->ev[7]));
@@ -3200,7 +3248,7 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
         }
         if (timed) TRY(time_mark(c, 6));
         if (upwind) TRY(do_rk(c, s, dt));
-        else TRY(launch_pt(c, k_prim, n_interior(c), pt_args(c)));
+        else TRY(launch_prim(c));
         if (timed) {
             TRY(time_mark(c, 7));
             HIPCHK(hipEventSynchronize(c->ev[7]));
@@ -3393,7 +3441,7 @@ int segment_fwd(astr_ctx* c, long nstates, long nst, double dt)
             } else {
                 for (int m = 0; m < 6; ++m) c->prim[m] = own_prim[m];
             }
-            TRY(launch_pt(c, k_prim, n_interior(c), pt_args(c)));
+            TRY(launch_prim(c));
             for (int m = 0; m < 5; ++m) qin[m] = st.qn[m];
             if (s == 3) c->time = c->time + dt;
         }
