@@ -1015,54 +1015,83 @@ __device__ __forceinline__ void gflux_of(const PtArgs& a, int d, double xn, cons
 // (the nodes -hm..-1 and n+1..n+hm of every d-line, where the flux sweep
 // also reads it): sigma and qflux there are those of the periodic image
 // (their halos are plain copies, src/solver.F90:2605-2607), the inviscid
-// part is that of the halo node itself.  One thread per node of the padded
-// box, i fastest (a 2-D grid: x over the (i,j) plane, y over k), so every
-// load and store of a wavefront is a run of consecutive i; nodes outside
-// the interior along two or more axes have nothing to do.  All inputs are
-// loaded before the first store.
-__global__ __launch_bounds__(256) void k_gflux(PtArgs a)
+// part is that of the halo node itself.  All inputs are loaded before the
+// first store.
+// G_d at a node of the halo strip of axis d (outside 0..n along d only):
+// the inviscid part at the node itself, the viscous part (sigma, heat flux)
+// that of its periodic image
+__device__ __forceinline__ void gflux_strip(const PtArgs& a, const int (&c)[3], int d, long gi)
 {
-    const unsigned pw = (unsigned)(a.g.n[0] + 1 + 2 * HM);
-    const unsigned ph = (unsigned)(a.g.n[1] + 1 + 2 * HM);
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= pw * ph) return;
-    const unsigned jj = t / pw;
-    int c[3] = {(int)(t - jj * pw) - HM, (int)jj - HM, (int)blockIdx.y - HM};
-    int d = -1, nout = 0;  // d = -1: interior node (all axes)
-#pragma unroll
-    for (int e = 0; e < 3; ++e)
-        if (c[e] < 0 || c[e] > a.g.n[e]) {
-            d = e;
-            ++nout;
-        }
-    if (nout > 1) return;
-    const long gi = gidx(a.g, c[0], c[1], c[2]);
     const NodeIn nd = load_node(a, gi);
     double sg[6] = {0, 0, 0, 0, 0, 0}, qf[3] = {0, 0, 0};
-    if (d < 0) {
-        if (a.visc) sigma_of(a, gi, nd.u, sg, qf);
-        double out[15];
-#pragma unroll
-        for (int e = 0; e < 3; ++e) gflux_of(a, e, a.xi[e][c[e] + HM], nd, sg, qf, out + 5 * e);
-#pragma unroll
-        for (int k = 0; k < 15; ++k) a.gfl[k][gi] = out[k];
-    } else {
-        if (a.visc) {
-            int w[3] = {c[0], c[1], c[2]};
-            w[d] = wrap_node(c[d], a.g.n[d]);
-            const long gw = gidx(a.g, w[0], w[1], w[2]);
-            const double uw[3] = {a.prim[1][gw], a.prim[2][gw], a.prim[3][gw]};
-            sigma_of(a, gw, uw, sg, qf);
-        }
-        // a compile-time axis keeps sg and qf in registers
-        double out[5];
-        const double xn = a.xi[d][c[d] + HM];
-        if (d == 0) gflux_of(a, 0, xn, nd, sg, qf, out);
-        else if (d == 1) gflux_of(a, 1, xn, nd, sg, qf, out);
-        else gflux_of(a, 2, xn, nd, sg, qf, out);
-#pragma unroll
-        for (int m = 0; m < 5; ++m) a.gfl[5 * d + m][gi] = out[m];
+    if (a.visc) {
+        int w[3] = {c[0], c[1], c[2]};
+        w[d] = wrap_node(c[d], a.g.n[d]);
+        const long gw = gidx(a.g, w[0], w[1], w[2]);
+        const double uw[3] = {a.prim[1][gw], a.prim[2][gw], a.prim[3][gw]};
+        sigma_of(a, gw, uw, sg, qf);
     }
+    // a compile-time axis keeps sg and qf in registers
+    double out[5];
+    const double xn = a.xi[d][c[d] + HM];
+    if (d == 0) gflux_of(a, 0, xn, nd, sg, qf, out);
+    else if (d == 1) gflux_of(a, 1, xn, nd, sg, qf, out);
+    else gflux_of(a, 2, xn, nd, sg, qf, out);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) a.gfl[5 * d + m][gi] = out[m];
+}
+
+__device__ __forceinline__ void gflux_zero(const PtArgs& a, int d, long gi)
+{
+#pragma unroll
+    for (int m = 0; m < 5; ++m) a.gfl[5 * d + m][gi] = 0.0;
+}
+
+// One wavefront per 64 nodes of an i-row of the padded box, the chunks of a
+// row consecutive and line-aligned (i from -16, the start of the line of
+// node -HM, to line_end_i(n)).  Interior rows give G of all axes on the
+// interior and G_x on the x halo strips; rows with j (or k) in a halo give
+// G_y (G_z) on their interior i.  The lanes of cells that no flux sweep
+// reads (row padding past the strips, i past n in halo rows) store zeros,
+// so each field's lines are written whole.
+__global__ __launch_bounds__(64) void k_gflux(PtArgs a)
+{
+    const int n0 = a.g.n[0];
+    const int iend = line_end_i(n0);
+    const unsigned nch = (unsigned)(iend + 16 + 63) / 64;
+    const unsigned row = blockIdx.x / nch;
+    const int i = (int)(blockIdx.x - row * nch) * 64 + (int)threadIdx.x - 16;
+    if (i >= iend) return;
+    const unsigned pj = (unsigned)(a.g.n[1] + 1 + 2 * HM);
+    const unsigned kk = row / pj;
+    const int c[3] = {i, (int)(row - kk * pj) - HM, (int)kk - HM};
+    const bool jin = c[1] >= 0 && c[1] <= a.g.n[1], kin = c[2] >= 0 && c[2] <= a.g.n[2];
+    const bool iin = i >= 0 && i <= n0;
+    const long gi = gidx(a.g, c[0], c[1], c[2]);
+    if (jin && kin) {
+        if (iin) {
+            const NodeIn nd = load_node(a, gi);
+            double sg[6] = {0, 0, 0, 0, 0, 0}, qf[3] = {0, 0, 0};
+            if (a.visc) sigma_of(a, gi, nd.u, sg, qf);
+            double out[15];
+#pragma unroll
+            for (int e = 0; e < 3; ++e) gflux_of(a, e, a.xi[e][c[e] + HM], nd, sg, qf, out + 5 * e);
+#pragma unroll
+            for (int k = 0; k < 15; ++k) a.gfl[k][gi] = out[k];
+            return;
+        }
+        if (i >= -HM && i <= n0 + HM) gflux_strip(a, c, 0, gi);
+        else gflux_zero(a, 0, gi);
+        if (i > n0) {
+            gflux_zero(a, 1, gi);
+            gflux_zero(a, 2, gi);
+        }
+        return;
+    }
+    if (jin == kin || i < 0) return;  // halo edges and corners; the lines before node 0
+    const int d = jin ? 2 : 1;        // the axis whose halo holds this row
+    if (iin) gflux_strip(a, c, d, gi);
+    else gflux_zero(a, d, gi);
 }
 
 // bchomo of the mini-app (tgvsolver.F90:271-369): the primitives of the
@@ -2890,9 +2919,10 @@ int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt, double* const* qo = nul
 // strips the flux sweeps read)
 int do_gflux(astr_ctx* c)
 {
-    const long plane = (long)(c->g.n[0] + 1 + 2 * HM) * (c->g.n[1] + 1 + 2 * HM);
-    const dim3 grid((unsigned)((plane + 255) / 256), (unsigned)(c->g.n[2] + 1 + 2 * HM));
-    hipLaunchKernelGGL(k_gflux, grid, dim3(256), 0, c->stream, pt_args(c));
+    // one wavefront per 64-node chunk of each padded (j, k) row
+    const long rows = (long)(c->g.n[1] + 1 + 2 * HM) * (c->g.n[2] + 1 + 2 * HM);
+    const long nch = (line_end_i(c->g.n[0]) + 16 + 63) / 64;
+    hipLaunchKernelGGL(k_gflux, dim3((unsigned)(rows * nch)), dim3(64), 0, c->stream, pt_args(c));
     HIPCHK(hipGetLastError());
     return ASTR_OK;
 }
