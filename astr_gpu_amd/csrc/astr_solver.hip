@@ -198,10 +198,13 @@ __device__ __forceinline__ void stb(double* base, unsigned int boff, double v)
 // previous batch (a store to q or qrhs could alias a later load, so the
 // compiler keeps program order); nodes past dim load node dim and store
 // nothing.
+// keep: a lane of a line past the tile's valid ones (i past n, up to the
+// end of the 128-byte line) stores q unchanged, so the line of q is
+// written whole and its halo cells keep their values.
 template <int MODE, int EMAX>
 __device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const double* trow,
                                              int n0, int nstep, int dim, unsigned int lob,
-                                             unsigned int sdb)
+                                             unsigned int sdb, bool keep)
 {
     constexpr int RB = MODE == 0 ? 8 : 4;
     constexpr int NB = (EMAX + RB - 1) / RB;
@@ -245,7 +248,8 @@ __device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const do
                     } else {
                         qs = cs[g];
                     }
-                    stb(a.qo[m], od, rk_combine(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], cj[g], r));
+                    stb(a.qo[m], od,
+                        keep ? cq[g] : rk_combine(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], cj[g], r));
                 } else {
                     // rk3 of tgvsolver.F90:1040-1060: q = a qsave + b q + c dt qrhs
                     const double r = cr[g] + o / a.h;
@@ -256,7 +260,7 @@ __device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const do
                     } else {
                         qs = cs[g];
                     }
-                    stb(a.qo[m], od, rk_combine_mini(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], r));
+                    stb(a.qo[m], od, keep ? cq[g] : rk_combine_mini(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], r));
                 }
             }
         }
@@ -326,7 +330,12 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     // last valid one store too, at their own line (lob_w), whatever their
     // LDS row holds.
     const bool dead = PASS == PASS_GRAD || (PASS == PASS_RHS && (XD || a.rk_stage == 0));
-    const bool lw = lvalid || (!XD && dead && a0 + ll < line_end_i(a.g.n[0]));
+    // the RK3 combine of the z sweep stores q unchanged on those lines; the
+    // y/z filter stages those lines themselves, leaves them unsolved and
+    // stores them back as staged
+    const bool keepq = PASS == PASS_RHS && !XD && a.rk_stage != 0;
+    const bool keepf = PASS == PASS_FILTER && !XD;
+    const bool lw = lvalid || (!XD && (dead || keepq || keepf) && a0 + ll < line_end_i(a.g.n[0]));
     unsigned int lob_w = lob;
     if (!XD && !lvalid) {
         const int av = a0 + ll;
@@ -335,6 +344,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
         lob_w = (unsigned int)low * 8u;
     }
     const int nlast = XD && dead ? line_end_i(dim) - 1 : dim;  // last node stored
+    const unsigned int lob_s = keepf && lw ? lob_w : lob;      // staged line
     auto sync = [&]() {
         if constexpr (XD) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -353,7 +363,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
             const unsigned int od = lob + (unsigned int)node * sdb;
             double v;
             if constexpr (PASS == PASS_FILTER) {
-                v = ldb(a.q[m], lob + (unsigned int)wrap_node(node, dim) * sdb);
+                v = ldb(a.q[m], lob_s + (unsigned int)wrap_node(node, dim) * sdb);
             } else if constexpr (PASS == PASS_GRAD) {
                 v = ldb(a.prim[m < 3 ? 1 + m : 5], od);
             } else if constexpr (PASS == PASS_RHS) {
@@ -379,8 +389,10 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
         // which the half-wavefront has finished reading and nothing reads
         // again
         const int sub = lane & 31;
+        if (!keepf || l < nvalid) {
 #pragma unroll
-        for (int t = 0; t < L; ++t) ln[HM + a.tab.first + sub * L + t] = x[t];
+            for (int t = 0; t < L; ++t) ln[HM + a.tab.first + sub * L + t] = x[t];
+        }
         if constexpr (OP == OP_FILTER) {
             // compact_filter keeps the boundary node (src/filter.F90:141-142);
             // the lane that wrote the node overwrites it
@@ -397,11 +409,11 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
         if (lw) {
             const int mode = a.rk_stage == 0 ? 0 : (a.mini ? 3 : 1) + (a.rk_stage > 1);
             switch (mode) {
-            case 0: rhs_epilogue<0, EMAX>(a, m, trow, n0, nstep, dim, lob_w, sdb); break;
-            case 1: rhs_epilogue<1, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
-            case 2: rhs_epilogue<2, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
-            case 3: rhs_epilogue<3, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
-            default: rhs_epilogue<4, EMAX>(a, m, trow, n0, nstep, dim, lob, sdb); break;
+            case 0: rhs_epilogue<0, EMAX>(a, m, trow, n0, nstep, dim, lob_w, sdb, false); break;
+            case 1: rhs_epilogue<1, EMAX>(a, m, trow, n0, nstep, dim, lob_w, sdb, !lvalid); break;
+            case 2: rhs_epilogue<2, EMAX>(a, m, trow, n0, nstep, dim, lob_w, sdb, !lvalid); break;
+            case 3: rhs_epilogue<3, EMAX>(a, m, trow, n0, nstep, dim, lob_w, sdb, !lvalid); break;
+            default: rhs_epilogue<4, EMAX>(a, m, trow, n0, nstep, dim, lob_w, sdb, !lvalid); break;
             }
         }
         return;
