@@ -175,9 +175,37 @@ def test_steps_match_oracle():
         sg = s.time_integration_rk(0.004, stats=True)
         for a, b in zip(sg, so):
             assert abs(a - b) <= 1e-10 * abs(b), (k, sg, so)
-    print("5 steps", compare_state(ob, s, p, 1e-10))
+    print("5 steps", compare_state(ob, s, p, 1e-10, halo_faces=True))
     s.close()
     ob.close()
+
+
+def test_filterq_keeps_every_cell_outside_the_nodes():
+    """filterq writes the nodes 0..n only; the sweeps also store the rest of
+    the 128-byte line that holds node n (so that HBM sees whole lines), with
+    the values already there.  Every cell outside 0..n along some axis (halo
+    faces, edges and corners) must come back bit for bit."""
+    dims = (40, 36, 33)
+    p = ag.default_params(dims)
+    s = ag.AstrSolver(p)
+    s.upload_geom(*ag.box_metrics(p))
+    rng = np.random.default_rng(4)
+    shp = ag.haloed_shape(p)
+    q = np.asfortranarray(1.0 + 0.1 * rng.standard_normal(shp + (5,)))
+    rho = np.asfortranarray(q[..., 0].copy())
+    vel = np.asfortranarray(0.1 * rng.standard_normal(shp + (3,)))
+    prs = np.asfortranarray(1.0 + 0.1 * rng.standard_normal(shp))
+    tmp = np.asfortranarray(1.0 + 0.1 * rng.standard_normal(shp))
+    s.upload_state(q, rho, vel, prs, tmp)
+    s.filterq()
+    out = s.download_state()
+    inside = np.zeros(shp, dtype=bool)
+    inside[HM:HM + dims[0] + 1, HM:HM + dims[1] + 1, HM:HM + dims[2] + 1] = True
+    for m in range(5):
+        assert np.array_equal(out["q"][..., m][~inside], q[..., m][~inside]), m
+        assert not np.array_equal(out["q"][..., m][inside], q[..., m][inside]), m
+    assert np.array_equal(out["vel"], vel) and np.array_equal(out["prs"], prs)
+    s.close()
 
 
 def test_stage_pieces_reproduce_the_fused_step():
