@@ -66,6 +66,7 @@ SIGNATURES = {
     "astr_line_adjoint": (ctypes.c_int, [_P, _I, _I, _I, _D, _D]),
     "astr_adjoint_enstrophy": (ctypes.c_int, [_P, _I, ctypes.c_double, _I, _D, _D, _D, _D]),
     "astr_adjoint_record": (ctypes.c_int, [_P, _I, ctypes.c_double, _I, _D]),
+    "astr_qmax": (ctypes.c_int, [_P, _D]),
     "astr_adjoint_reverse": (ctypes.c_int, [_P, _D, _D, _D]),
     "astr_boucon": (ctypes.c_int, [_P]),
     "astr_set_time": (ctypes.c_int, [_P, ctypes.c_double]),

@@ -1280,6 +1280,36 @@ __global__ __launch_bounds__(256) void k_stats(PtArgs a)
         }
 }
 
+// maxmincal's max_q (src/statistic.F90:737-763): the largest value of each
+// q(m) over the nodes 0..n; each workgroup strides over the nodes and
+// leaves 5 partial maxima (max is exact, so the order does not matter)
+__global__ __launch_bounds__(256) void k_qmax(PtArgs a)
+{
+    __shared__ double sh[5][256];
+    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
+    const long ntot = ni * nj * (a.g.n[2] + 1);
+    double mx[5] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < ntot;
+         p += (long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        split3(p, ni, nj, i, j, k);
+        const long gi = gidx(a.g, i, j, k);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) mx[m] = fmax(mx[m], a.q[m][gi]);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) sh[m][threadIdx.x] = mx[m];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int m = 0; m < 5; ++m)
+                sh[m][threadIdx.x] = fmax(sh[m][threadIdx.x], sh[m][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int m = 0; m < 5; ++m) a.part[5 * blockIdx.x + m] = sh[m][0];
+}
+
 // ------------------------------------------------------ discrete adjoint
 //
 // Reverse-mode kernels of one stage of the central path (filterq, qswap,
@@ -3806,7 +3836,7 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
         return cleanup(fail(ASTR_E_NOMEM, "metric allocation failed"));
     (void)hipMemset(c->xi, 0, sizeof(double) * 3 * c->xi_stride);
 
-    c->npart = ((long)dims[0] * dims[1] * dims[2] + 255) / 256;
+    c->npart = ((long)dims[0] * dims[1] * std::max(dims[2], 1) + 255) / 256;
     c->host_part.resize(6 * c->npart);
     if (hipMalloc(&c->part, sizeof(double) * 6 * c->npart) != hipSuccess)
         return cleanup(fail(ASTR_E_NOMEM, "partials allocation failed"));
@@ -4016,6 +4046,26 @@ int astr_stats(astr_ctx* c, double* out3)
     if (!out3) return fail(ASTR_E_ARG, "null argument");
     if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "statistics of three-dimensional boxes only");
     return do_stats(c, out3);
+}
+
+int astr_qmax(astr_ctx* c, double* qmax5)
+{
+    TRY(ready(c));
+    if (!qmax5) return fail(ASTR_E_ARG, "null argument");
+    // the partial-sum buffer holds 6 doubles per block of npart; 5 per
+    // workgroup of at most npart workgroups fit
+    const long nb = std::min<long>(c->npart, 1024);
+    hipLaunchKernelGGL(k_qmax, dim3((unsigned)nb), dim3(256), 0, c->stream, pt_args(c));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->host_part.data(), c->part, sizeof(double) * 5 * nb,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int m = 0; m < 5; ++m) {
+        double v = -HUGE_VAL;
+        for (long b = 0; b < nb; ++b) v = std::max(v, c->host_part[5 * b + m]);
+        qmax5[m] = v;
+    }
+    return ASTR_OK;
 }
 
 int astr_rk3_step(astr_ctx* c, double dt, double* stats)

@@ -204,6 +204,13 @@ module astr_hip
       real(c_double), intent(out) :: time
     end function astr_get_time
 
+    ! maxmincal's max_q: the largest q(m) over the nodes 0..n
+    integer(c_int) function astr_qmax(ctx, qmax) bind(c, name='astr_qmax')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: qmax(5)
+    end function astr_qmax
+
     ! the discrete adjoint of nsteps RK3 steps, J = enstrophy of the final
     ! state; qbar(-hm:..,1:5), pbar(-hm:..,1:6) of the initial state
     integer(c_int) function astr_adjoint_record(ctx, nsteps, dt, ckpt, j) &

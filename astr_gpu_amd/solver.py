@@ -123,6 +123,13 @@ class AstrSolver:
         check(self.lib.astr_stats(self.ctx, dptr(out)), "astr_stats")
         return float(out[0]), float(out[1]), float(out[2])
 
+    def qmax(self) -> tuple:
+        """max_q of maxmincal: the largest q(m) over the nodes 0..n
+        (astr_qmax)."""
+        out = np.zeros(5)
+        check(self.lib.astr_qmax(self.ctx, dptr(out)), "astr_qmax")
+        return tuple(float(v) for v in out)
+
     # ---------------------------------------------------------- steps
     def time_integration_rk(self, dt: float, stats: bool = False):
         out = np.zeros(3) if stats else None

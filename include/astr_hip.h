@@ -130,6 +130,12 @@ int astr_rk3_update(astr_ctx *ctx, int32_t stage, double dt);
 /* kenergycal, enstophycal, diss_rate_cal (src/statistic.F90:870-1042) of
  * the current device state -> out[0..2]. */
 int astr_stats(astr_ctx *ctx, double *out3);
+/* maxmincal's max_q (src/statistic.F90:737-763): the largest value of each
+ * q(m), m = 1..5, over the nodes 0..n of the current device state ->
+ * qmax[0..4]; the columns statout writes to flowstate.dat for flow types
+ * without their own statistics (src/statistic.F90:835-838).  Two- and
+ * three-dimensional boxes. */
+int astr_qmax(astr_ctx *ctx, double *qmax5);
 
 /* --- whole steps: time_integration_rk, src/mainloop.F90:298-595 ------- */
 
