@@ -11,7 +11,7 @@ src/utility.F90:310-362) and, for nstep = 1..maxstep, one line of the
 statistics rkfirst takes at stage 1 of that step (listwrite,
 src/utility.F90:374-434).
 
-    python -m astr_gpu_amd.deck CASEDIR [--maxstep N] [--out DIR]
+    python -m astr_gpu_amd.deck CASEDIR [--input datin/input.tgv] [--maxstep N] [--out DIR]
 
 Built flow types: ``tgv`` (tgvini), ``hit`` (the seeded turbulence
 generator of this package, same spectrum as the reference's) and
@@ -137,9 +137,12 @@ def read_controller(path):
     return c
 
 
-def find_input(casedir):
-    """The input file of a case: datin/input.dat (readinput's default
-    name), else the single datin/input.* there."""
+def find_input(casedir, name=None):
+    """The input file of a case: name (relative to the case directory, as
+    the reference takes it on its command line), else datin/input.dat
+    (readinput's default), else the single datin/input.* there."""
+    if name:
+        return os.path.join(casedir, name)
     datin = os.path.join(casedir, "datin")
     p = os.path.join(datin, "input.dat")
     if os.path.exists(p):
@@ -247,12 +250,12 @@ def initial_state(d, p, seed=1):
     return ag.hit_state(p, seed)
 
 
-def run_deck(casedir, out=None, maxstep=None, seed=1, device=0, log=sys.stdout):
+def run_deck(casedir, out=None, maxstep=None, seed=1, device=0, log=sys.stdout, input_name=None):
     """Run a case directory; returns the path of the flowstate.dat written
     (in out, default the case directory)."""
     import astr_gpu_amd as ag
 
-    d = read_input(find_input(casedir))
+    d = read_input(find_input(casedir, input_name))
     c = read_controller(os.path.join(casedir, "datin", "controller"))
     if maxstep is not None:
         c["maxstep"] = int(maxstep)
@@ -300,11 +303,15 @@ def run_deck(casedir, out=None, maxstep=None, seed=1, device=0, log=sys.stdout):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("casedir")
+    ap.add_argument("--input", default=None,
+                    help="input file relative to casedir (default datin/input.dat, or the only "
+                         "datin/input.*)")
     ap.add_argument("--out", default=None, help="directory of flowstate.dat (default: casedir)")
     ap.add_argument("--maxstep", type=int, default=None, help="override the controller's maxstep")
     ap.add_argument("--seed", type=int, default=1, help="hit: seed of the turbulence generator")
     args = ap.parse_args(argv)
-    path = run_deck(args.casedir, out=args.out, maxstep=args.maxstep, seed=args.seed)
+    path = run_deck(args.casedir, out=args.out, maxstep=args.maxstep, seed=args.seed,
+                    input_name=args.input)
     print(f" << {path}")
 
 
