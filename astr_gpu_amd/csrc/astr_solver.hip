@@ -354,6 +354,14 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
         }
     };
 
+    // the metric factors of the rows this lane solves (gradient pass)
+    double xr[PASS == PASS_GRAD ? L : 1];
+    if constexpr (PASS == PASS_GRAD) {
+        const int rb = a.tab.first + (lane & 31) * L;
+#pragma unroll
+        for (int t = 0; t < L; ++t) xr[t] = a.xi[min(max(rb + t, -HM), dim + HM) + HM];
+    }
+
     // ---------------- stage the lines of this quantity in LDS
 #pragma unroll
     for (int e = 0; e < EMAX; ++e) {
@@ -384,6 +392,14 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
         double* ln = tile + l * a.rs_in + TG;
         const double keep0 = ln[HM], keepd = ln[HM + dim];
         solve_line<L, OP>(ln, a.tab, a.fc, x);
+        if constexpr (PASS == PASS_GRAD) {
+            // gradcal's metric factor dxi(d,d) (src/comsolver.F90:304-318)
+            // on the lane's rows, loaded with the staging
+            if (!a.mini) {
+#pragma unroll
+                for (int t = 0; t < L; ++t) x[t] *= xr[t];
+            }
+        }
         // every row goes back to its node's slot: rows outside 0..dim (and
         // the zero padding rows past R) land in slots of the staged input,
         // which the half-wavefront has finished reading and nothing reads
@@ -430,7 +446,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
                 } else if constexpr (PASS == PASS_GRAD) {
                     // gradcal: d/dxi times dxi(d,d) (src/comsolver.F90:304-318),
                     // or df/dx (tgvsolver.F90:420-470)
-                    const double gv = a.mini ? o / a.h : o * a.xi[min(n, dim) + HM];
+                    const double gv = a.mini ? o / a.h : o;  // scaled by xi in the solve
                     if (m < 3) stb(a.dvel[m * 3 + dir], od, gv);
                     else stb(a.dtmp[dir], od, gv);
                 } else if constexpr (PASS == PASS_RHS) {
