@@ -87,6 +87,29 @@ __host__ __device__ __forceinline__ long gidx(const Geo& g, int i, int j, int k)
 // partly written line at a far higher cost than a whole one.
 __host__ __device__ constexpr int line_end_i(int n) { return (n / 16 + 1) * 16; }
 
+// One wavefront per 64 nodes of an interior i-row (j, k in 0..n), the
+// chunks of a row consecutive, to line_end_i(n): false for lanes past it.
+// Launched with 64 threads per block, (n1+1)(n2+1) x chunks blocks.
+__device__ __forceinline__ bool row_node(const Geo& g, int& i, int& j, int& k)
+{
+    const unsigned nch = (unsigned)(line_end_i(g.n[0]) + 63) / 64;
+    const unsigned row = blockIdx.x / nch;
+    i = (int)(blockIdx.x - row * nch) * 64 + (int)threadIdx.x;
+    const unsigned nj = (unsigned)g.n[1] + 1;
+    k = (int)(row / nj);
+    j = (int)(row - (unsigned)k * nj);
+    return i < line_end_i(g.n[0]);
+}
+
+// store v back to f[gi] (a cell past node n that keeps its value, written so
+// that its line is written whole)
+__device__ __forceinline__ void keep_cell(double* f, long gi)
+{
+    double v = f[gi];
+    asm volatile("" : "+v"(v));  // the store is not dropped as redundant
+    f[gi] = v;
+}
+
 // ---------------------------------------------------------------- sweeps
 
 enum Pass : int {
@@ -1201,13 +1224,9 @@ __global__ void k_rk(PtArgs a)
 // already there, so every line is written whole.
 __global__ __launch_bounds__(64) void k_prim(PtArgs a)
 {
-    const unsigned nch = (unsigned)(line_end_i(a.g.n[0]) + 63) / 64;
-    const unsigned row = blockIdx.x / nch;
-    const int i = (int)(blockIdx.x - row * nch) * 64 + (int)threadIdx.x;
-    if (i >= line_end_i(a.g.n[0])) return;
-    const unsigned nj = (unsigned)a.g.n[1] + 1;
-    const unsigned k = row / nj, j = row - k * nj;
-    const long gi = gidx(a.g, i, (int)j, (int)k);
+    int i, j, k;
+    if (!row_node(a.g, i, j, k)) return;
+    const long gi = gidx(a.g, i, j, k);
     if (i <= a.g.n[0]) {
         double qv[5];
 #pragma unroll
@@ -1215,11 +1234,7 @@ __global__ __launch_bounds__(64) void k_prim(PtArgs a)
         q2fvar(a, qv, gi);
     } else {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            double v = a.prim[m][gi];
-            asm volatile("" : "+v"(v));  // the store is not dropped as redundant
-            a.prim[m][gi] = v;
-        }
+        for (int m = 0; m < 6; ++m) keep_cell(a.prim[m], gi);
     }
 }
 
@@ -1432,14 +1447,22 @@ __global__ __launch_bounds__(256) void k_q2f_adj(AdjArgs a)
 // collects a/J q'-bar over stages 3 and 2 and is spent at stage 1, where
 // qsave = q J; stage 3 (the first reversed stage of a step) neither reads
 // it nor needs it cleared.
-__global__ __launch_bounds__(256) void k_update_adj(AdjArgs a)
+__global__ __launch_bounds__(64) void k_update_adj(AdjArgs a)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
-    if (p >= ni * nj * (a.g.n[2] + 1)) return;
     int ci, cj, ck;
-    split3(p, ni, nj, ci, cj, ck);
+    if (!row_node(a.g, ci, cj, ck)) return;
     const long gi = gidx(a.g, ci, cj, ck);
+    if (ci > a.g.n[0]) {
+        // the rest of the line holding node n: q-bar keeps its values; the
+        // qrhs-bar and qsave-bar cells there are dead
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            keep_cell(a.qb[m], gi);
+            a.rb[m][gi] = 0.0;
+            if (a.stage != 1) a.sb[m][gi] = 0.0;
+        }
+        return;
+    }
     const double jc = a.jac[gi];
     double qn[5], pbv[6], ob[5], sv[5] = {0, 0, 0, 0, 0};
 #pragma unroll
@@ -1542,15 +1565,27 @@ __global__ void k_gflux_adj_halo(AdjArgs a)
 // node's own viscous terms and those of the halo nodes that took it as
 // their periodic image; then sigma -> dvel, T (through miucal), u and the
 // heat flux -> dtmp.
-__global__ void k_gflux_adj_int(AdjArgs a)
+__global__ __launch_bounds__(64) void k_gflux_adj_int(AdjArgs a)
 {
     static constexpr int sidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long ni = a.g.n[0] + 1, nj = a.g.n[1] + 1;
-    if (p >= ni * nj * (a.g.n[2] + 1)) return;
     int c[3];
-    split3(p, ni, nj, c[0], c[1], c[2]);
+    if (!row_node(a.g, c[0], c[1], c[2])) return;
     const long gi = gidx(a.g, c[0], c[1], c[2]);
+    if (c[0] > a.g.n[0]) {
+        // the rest of the line holding node n: q-bar and primitive-bar keep
+        // their values; the dvel-bar and dtmp-bar cells there are dead
+#pragma unroll
+        for (int m = 0; m < 5; ++m) keep_cell(a.qb[m], gi);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) keep_cell(a.pb[m], gi);
+        if (a.visc) {
+#pragma unroll
+            for (int m = 0; m < 9; ++m) a.db[m][gi] = 0.0;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) a.tb[m][gi] = 0.0;
+        }
+        return;
+    }
     const double jc = a.jac[gi];
     // every operand first (a store could alias a later load); primitive-bar
     // is zero on entry (spent by updatefvar's reversal)
@@ -3463,6 +3498,17 @@ dim3 padded_grid(const astr_ctx* c)
     return dim3((unsigned)((plane + 255) / 256), (unsigned)(c->g.n[2] + 1 + 2 * HM));
 }
 
+// a row_node kernel over the interior rows
+template <class K>
+int launch_rows(astr_ctx* c, K kernel, const AdjArgs& a)
+{
+    const long rows = (long)(c->g.n[1] + 1) * (c->g.n[2] + 1);
+    const long nch = (line_end_i(c->g.n[0]) + 63) / 64;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)(rows * nch)), dim3(64), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
+}
+
 template <class K>
 int launch_adj(astr_ctx* c, K kernel, long n, const AdjArgs& a)
 {
@@ -3584,12 +3630,12 @@ int stage_adj(astr_ctx* c, const StageStore& st, int s, double dt)
     a.dt = dt;
     const AdjFields f = adj_fields(c);
     // updatefvar and the RK3 combine
-    TRY(launch_adj(c, k_update_adj, n_interior(c), a));
+    TRY(launch_rows(c, k_update_adj, a));
     // the flux derivatives: G-bar_d = D_d^T qrhs-bar
     for (int d = 0; d < 3; ++d)
         TRY(launch_sweep_t(c, d, OP_DERIV, 5, f.rb, f.gb + 5 * d, nullptr, false, false));
     // the axis fluxes, at the stored gradients
-    TRY(launch_adj(c, k_gflux_adj_int, n_interior(c), a));
+    TRY(launch_rows(c, k_gflux_adj_int, a));
     const long ext[3] = {c->g.n[0] + 1L, c->g.n[1] + 1L, c->g.n[2] + 1L};
     for (int d = 0; d < 3; ++d) {
         a.axis = d;
