@@ -87,18 +87,21 @@ __host__ __device__ __forceinline__ long gidx(const Geo& g, int i, int j, int k)
 // partly written line at a far higher cost than a whole one.
 __host__ __device__ constexpr int line_end_i(int n) { return (n / 16 + 1) * 16; }
 
-// One wavefront per 64 nodes of an interior i-row (j, k in 0..n), the
-// chunks of a row consecutive, to line_end_i(n): false for lanes past it.
-// Launched with 64 threads per block, (n1+1)(n2+1) x chunks blocks.
+// The interior i-rows (j, k in 0..n) cut at line_end_i(n), laid end to end
+// and taken one node per thread: a row is a whole number of 128-byte lines,
+// so a wavefront covers whole lines only, also where it crosses from one
+// row to the next.  False past the last row.
 __device__ __forceinline__ bool row_node(const Geo& g, int& i, int& j, int& k)
 {
-    const unsigned nch = (unsigned)(line_end_i(g.n[0]) + 63) / 64;
-    const unsigned row = blockIdx.x / nch;
-    i = (int)(blockIdx.x - row * nch) * 64 + (int)threadIdx.x;
+    const unsigned le = (unsigned)line_end_i(g.n[0]);
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned row = t / le;
     const unsigned nj = (unsigned)g.n[1] + 1;
+    if (row >= nj * ((unsigned)g.n[2] + 1)) return false;
+    i = (int)(t - row * le);
     k = (int)(row / nj);
     j = (int)(row - (unsigned)k * nj);
-    return i < line_end_i(g.n[0]);
+    return true;
 }
 
 // store v back to f[gi] (a cell past node n that keeps its value, written so
@@ -1098,22 +1101,24 @@ __device__ __forceinline__ void gflux_zero(const PtArgs& a, int d, long gi)
     for (int m = 0; m < 5; ++m) a.gfl[5 * d + m][gi] = 0.0;
 }
 
-// One wavefront per 64 nodes of an i-row of the padded box, the chunks of a
-// row consecutive and line-aligned (i from -16, the start of the line of
-// node -HM, to line_end_i(n)).  Interior rows give G of all axes on the
-// interior and G_x on the x halo strips; rows with j (or k) in a halo give
-// G_y (G_z) on their interior i.  The lanes of cells that no flux sweep
-// reads (row padding past the strips, i past n in halo rows) store zeros,
-// so each field's lines are written whole.
-__global__ __launch_bounds__(64) void k_gflux(PtArgs a)
+// One thread per cell of the i-rows of the padded box, i from -16 (the
+// start of the line of node -HM) to line_end_i(n), the rows laid end to end
+// (a whole number of 128-byte lines each, so a wavefront covers whole lines
+// only).  Interior rows give G of all axes on the interior and G_x on the x
+// halo strips; rows with j (or k) in a halo give G_y (G_z) on their
+// interior i.  The threads of cells that no flux sweep reads (row padding
+// past the strips, i past n in halo rows) store zeros, so each field's
+// lines are written whole.
+__global__ __launch_bounds__(256) void k_gflux(PtArgs a)
 {
     const int n0 = a.g.n[0];
     const int iend = line_end_i(n0);
-    const unsigned nch = (unsigned)(iend + 16 + 63) / 64;
-    const unsigned row = blockIdx.x / nch;
-    const int i = (int)(blockIdx.x - row * nch) * 64 + (int)threadIdx.x - 16;
-    if (i >= iend) return;
+    const unsigned rl = (unsigned)(iend + 16);  // cells per row
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned row = t / rl;
     const unsigned pj = (unsigned)(a.g.n[1] + 1 + 2 * HM);
+    if (row >= pj * (unsigned)(a.g.n[2] + 1 + 2 * HM)) return;
+    const int i = (int)(t - row * rl) - 16;
     const unsigned kk = row / pj;
     const int c[3] = {i, (int)(row - kk * pj) - HM, (int)kk - HM};
     const bool jin = c[1] >= 0 && c[1] <= a.g.n[1], kin = c[2] >= 0 && c[2] <= a.g.n[2];
@@ -1218,11 +1223,10 @@ __global__ void k_rk(PtArgs a)
 
 // updatefvar (src/fludyna.F90:191-242) on the interior, once the last flux
 // sweep of the stage has applied the RK3 combine to q.
-// updatefvar on the nodes 0..n (src/fludyna.F90:191-242).  One wavefront
-// per 64 nodes of an i-row, the chunks of a row consecutive; the lanes
-// past node n up to the end of its 128-byte line store the primitives
-// already there, so every line is written whole.
-__global__ __launch_bounds__(64) void k_prim(PtArgs a)
+// updatefvar on the nodes 0..n (src/fludyna.F90:191-242), over row_node's
+// rows; the threads past node n up to the end of its 128-byte line store
+// the primitives already there, so every line is written whole.
+__global__ __launch_bounds__(256) void k_prim(PtArgs a)
 {
     int i, j, k;
     if (!row_node(a.g, i, j, k)) return;
@@ -1447,7 +1451,7 @@ __global__ __launch_bounds__(256) void k_q2f_adj(AdjArgs a)
 // collects a/J q'-bar over stages 3 and 2 and is spent at stage 1, where
 // qsave = q J; stage 3 (the first reversed stage of a step) neither reads
 // it nor needs it cleared.
-__global__ __launch_bounds__(64) void k_update_adj(AdjArgs a)
+__global__ __launch_bounds__(256) void k_update_adj(AdjArgs a)
 {
     int ci, cj, ck;
     if (!row_node(a.g, ci, cj, ck)) return;
@@ -1565,7 +1569,7 @@ __global__ void k_gflux_adj_halo(AdjArgs a)
 // node's own viscous terms and those of the halo nodes that took it as
 // their periodic image; then sigma -> dvel, T (through miucal), u and the
 // heat flux -> dtmp.
-__global__ __launch_bounds__(64) void k_gflux_adj_int(AdjArgs a)
+__global__ __launch_bounds__(256) void k_gflux_adj_int(AdjArgs a)
 {
     static constexpr int sidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
     int c[3];
@@ -2902,13 +2906,12 @@ int launch_pt(astr_ctx* c, void (*k)(PtArgs), long nthreads, const PtArgs& a, in
     return ASTR_OK;
 }
 
-// k_prim: one wavefront per 64 nodes of each i-row, to the end of the line
-// holding node n
+// k_prim over the interior rows cut at the end of the line holding node n
 int launch_prim(astr_ctx* c)
 {
-    const dim3 grid((unsigned)((c->g.n[1] + 1) * (c->g.n[2] + 1) *
-                               ((line_end_i(c->g.n[0]) + 63) / 64)));
-    hipLaunchKernelGGL(k_prim, grid, dim3(64), 0, c->stream, pt_args(c));
+    const long n = (long)(c->g.n[1] + 1) * (c->g.n[2] + 1) * line_end_i(c->g.n[0]);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_prim, grid, dim3(256), 0, c->stream, pt_args(c));
     HIPCHK(hipGetLastError());
     return ASTR_OK;
 }
@@ -3012,10 +3015,11 @@ int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt, double* const* qo = nul
 // strips the flux sweeps read)
 int do_gflux(astr_ctx* c)
 {
-    // one wavefront per 64-node chunk of each padded (j, k) row
+    // one thread per cell of the padded (j, k) rows, i from -16 to line_end_i(n)
     const long rows = (long)(c->g.n[1] + 1 + 2 * HM) * (c->g.n[2] + 1 + 2 * HM);
-    const long nch = (line_end_i(c->g.n[0]) + 16 + 63) / 64;
-    hipLaunchKernelGGL(k_gflux, dim3((unsigned)(rows * nch)), dim3(64), 0, c->stream, pt_args(c));
+    const long n = rows * (line_end_i(c->g.n[0]) + 16);
+    hipLaunchKernelGGL(k_gflux, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       pt_args(c));
     HIPCHK(hipGetLastError());
     return ASTR_OK;
 }
@@ -3502,9 +3506,8 @@ dim3 padded_grid(const astr_ctx* c)
 template <class K>
 int launch_rows(astr_ctx* c, K kernel, const AdjArgs& a)
 {
-    const long rows = (long)(c->g.n[1] + 1) * (c->g.n[2] + 1);
-    const long nch = (line_end_i(c->g.n[0]) + 63) / 64;
-    hipLaunchKernelGGL(kernel, dim3((unsigned)(rows * nch)), dim3(64), 0, c->stream, a);
+    const long n = (long)(c->g.n[1] + 1) * (c->g.n[2] + 1) * line_end_i(c->g.n[0]);
+    hipLaunchKernelGGL(kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     return ASTR_OK;
 }
