@@ -1065,15 +1065,11 @@ __device__ __forceinline__ void gflux_of(const PtArgs& a, int d, double xn, cons
     }
 }
 
-// G of all three axes on the interior, and of axis d on its halo strips
-// (the nodes -hm..-1 and n+1..n+hm of every d-line, where the flux sweep
-// also reads it): sigma and qflux there are those of the periodic image
-// (their halos are plain copies, src/solver.F90:2605-2607), the inviscid
-// part is that of the halo node itself.  All inputs are loaded before the
-// first store.
-// G_d at a node of the halo strip of axis d (outside 0..n along d only):
-// the inviscid part at the node itself, the viscous part (sigma, heat flux)
-// that of its periodic image
+// G_d at a node of the halo strip of axis d (the nodes -hm..-1 and
+// n+1..n+hm of a d-line, where the flux sweep also reads G_d): the inviscid
+// part at the node itself, the viscous part (sigma, heat flux) that of its
+// periodic image, as the halos of sigma and qflux are plain copies
+// (src/solver.F90:2605-2607)
 __device__ __forceinline__ void gflux_strip(const PtArgs& a, const int (&c)[3], int d, long gi)
 {
     const NodeIn nd = load_node(a, gi);
@@ -1102,17 +1098,18 @@ __device__ __forceinline__ void gflux_zero(const PtArgs& a, int d, long gi)
 }
 
 // One thread per cell of the i-rows of the padded box, i from -16 (the
-// start of the line of node -HM) to line_end_i(n), the rows laid end to end
-// (a whole number of 128-byte lines each, so a wavefront covers whole lines
-// only).  Interior rows give G of all axes on the interior and G_x on the x
-// halo strips; rows with j (or k) in a halo give G_y (G_z) on their
-// interior i.  The threads of cells that no flux sweep reads (row padding
-// past the strips, i past n in halo rows) store zeros, so each field's
-// lines are written whole.
+// start of the line of node -HM) to line_end_i(n + HM) (the end of the line
+// of node n + HM, which is the row pitch), the rows laid end to end (a whole
+// number of 128-byte lines each, so a wavefront covers whole lines only).
+// Interior rows give G of all axes on the interior and G_x on the x halo
+// strips; rows with j (or k) in a halo give G_y (G_z) on their interior i.
+// The threads of cells that no flux sweep reads (row padding past the
+// strips, i past n in halo rows) store zeros, so each field's lines are
+// written whole.
 __global__ __launch_bounds__(256) void k_gflux(PtArgs a)
 {
     const int n0 = a.g.n[0];
-    const int iend = line_end_i(n0);
+    const int iend = line_end_i(n0 + HM);
     const unsigned rl = (unsigned)(iend + 16);  // cells per row
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned row = t / rl;
@@ -1221,8 +1218,6 @@ __global__ void k_rk(PtArgs a)
 }
 #pragma clang fp contract(fast)
 
-// updatefvar (src/fludyna.F90:191-242) on the interior, once the last flux
-// sweep of the stage has applied the RK3 combine to q.
 // updatefvar on the nodes 0..n (src/fludyna.F90:191-242), over row_node's
 // rows; the threads past node n up to the end of its 128-byte line store
 // the primitives already there, so every line is written whole.
@@ -3015,9 +3010,9 @@ int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt, double* const* qo = nul
 // strips the flux sweeps read)
 int do_gflux(astr_ctx* c)
 {
-    // one thread per cell of the padded (j, k) rows, i from -16 to line_end_i(n)
+    // one thread per cell of the padded (j, k) rows, i from -16 to line_end_i(n + HM)
     const long rows = (long)(c->g.n[1] + 1 + 2 * HM) * (c->g.n[2] + 1 + 2 * HM);
-    const long n = rows * (line_end_i(c->g.n[0]) + 16);
+    const long n = rows * (line_end_i(c->g.n[0] + HM) + 16);
     hipLaunchKernelGGL(k_gflux, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                        pt_args(c));
     HIPCHK(hipGetLastError());

@@ -116,16 +116,17 @@ def adjoint_case(dims, seed=3, lengths=(2 * np.pi,) * 3):
     return p, box, torch.tensor(u0), to_host
 
 
-@pytest.mark.parametrize("nsteps,ckpt,lengths", [
-    (2, 1, None), (2, 8, None), (3, 2, None), (2, 2, (2 * np.pi, 4.0, 9.0))])
-def test_adjoint_matches_torch_autograd(nsteps, ckpt, lengths):
+@pytest.mark.parametrize("nsteps,ckpt,lengths,dims", [
+    (2, 1, None, (16, 18, 20)), (2, 8, None, (16, 18, 20)), (3, 2, None, (16, 18, 20)),
+    (2, 2, (2 * np.pi, 4.0, 9.0), (16, 18, 20)), (2, 2, None, (27, 16, 29))])
+def test_adjoint_matches_torch_autograd(nsteps, ckpt, lengths, dims):
     """dJ/d(state) of the GPU reversal against PyTorch-CPU autograd of the
     same forward (RK3 steps, filter and viscous terms on): every stage kept,
-    one segment of stored stages, a last segment shorter than ckpt, and a
-    box whose three axes have different spacings (metrics and J)."""
+    one segment of stored stages, a last segment shorter than ckpt, a box
+    whose three axes have different spacings (metrics and J), and one whose
+    x halo strip n+1..n+5 runs past the 128-byte line of node n."""
     import torch
 
-    dims = (16, 18, 20)
     lengths = lengths or (2 * np.pi,) * 3
     print(f"case adjoint vs autograd dims={dims} nsteps={nsteps} ckpt={ckpt} lengths={lengths}",
           flush=True)

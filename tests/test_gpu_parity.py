@@ -180,12 +180,13 @@ def test_steps_match_oracle():
     ob.close()
 
 
-def test_filterq_keeps_every_cell_outside_the_nodes():
+@pytest.mark.parametrize("dims", [(40, 36, 33), (45, 43, 27)])
+def test_filterq_keeps_every_cell_outside_the_nodes(dims):
     """filterq writes the nodes 0..n only; the sweeps also store the rest of
     the 128-byte line that holds node n (so that HBM sees whole lines), with
     the values already there.  Every cell outside 0..n along some axis (halo
     faces, edges and corners) must come back bit for bit."""
-    dims = (40, 36, 33)
+    print(f"case filterq keeps dims={dims}", flush=True)
     p = ag.default_params(dims)
     s = ag.AstrSolver(p)
     s.upload_geom(*ag.box_metrics(p))
@@ -238,8 +239,11 @@ def test_stage_pieces_reproduce_the_fused_step():
     s2.close()
 
 
-def test_non_cubic_steps_match_oracle():
-    dims = (48, 40, 36)
+@pytest.mark.parametrize("dims", [(48, 40, 36), (43, 45, 29)])
+def test_non_cubic_steps_match_oracle(dims):
+    """(43, 45, 29): n mod 16 >= 11 on every axis, so the halo strip
+    n+1..n+5 runs past the 128-byte line that holds node n."""
+    print(f"case non-cubic steps dims={dims}", flush=True)
     ob, s, p = make_pair(dims)
     for k in range(3):
         ob.step(0.004)
