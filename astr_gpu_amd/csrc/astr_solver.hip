@@ -1097,54 +1097,80 @@ __device__ __forceinline__ void gflux_zero(const PtArgs& a, int d, long gi)
     for (int m = 0; m < 5; ++m) a.gfl[5 * d + m][gi] = 0.0;
 }
 
-// One thread per cell of the i-rows of the padded box, i from -16 (the
-// start of the line of node -HM) to line_end_i(n + HM) (the end of the line
-// of node n + HM, which is the row pitch), the rows laid end to end (a whole
-// number of 128-byte lines each, so a wavefront covers whole lines only).
-// Interior rows give G of all axes on the interior and G_x on the x halo
-// strips; rows with j (or k) in a halo give G_y (G_z) on their interior i.
-// The threads of cells that no flux sweep reads (row padding past the
-// strips, i past n in halo rows) store zeros, so each field's lines are
-// written whole.
-__global__ __launch_bounds__(256) void k_gflux(PtArgs a)
+// The halo node along an axis of extent n whose periodic image is node c
+// (wrap_node's inverse on the strips): c + n for c in 1..HM, c - n for c in
+// n-HM..n-1, else no halo node (NO_HALO).
+constexpr int NO_HALO = -(1 << 20);
+__device__ __forceinline__ int halo_of_image(int c, int n)
+{
+    return (c >= 1 && c <= HM) ? c + n : ((c >= n - HM && c <= n - 1) ? c - n : NO_HALO);
+}
+
+// k_gflux at cell i of the interior row (j, k); hj, hk are the y and z
+// halo rows of which this row is the periodic image (or NO_HALO).
+__device__ __forceinline__ void gflux_cell(const PtArgs& a, int i, int j, int k, int hj, int hk)
 {
     const int n0 = a.g.n[0];
-    const int iend = line_end_i(n0 + HM);
-    const unsigned rl = (unsigned)(iend + 16);  // cells per row
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned row = t / rl;
-    const unsigned pj = (unsigned)(a.g.n[1] + 1 + 2 * HM);
-    if (row >= pj * (unsigned)(a.g.n[2] + 1 + 2 * HM)) return;
-    const int i = (int)(t - row * rl) - 16;
-    const unsigned kk = row / pj;
-    const int c[3] = {i, (int)(row - kk * pj) - HM, (int)kk - HM};
-    const bool jin = c[1] >= 0 && c[1] <= a.g.n[1], kin = c[2] >= 0 && c[2] <= a.g.n[2];
-    const bool iin = i >= 0 && i <= n0;
+    const int c[3] = {i, j, k};
     const long gi = gidx(a.g, c[0], c[1], c[2]);
-    if (jin && kin) {
-        if (iin) {
-            const NodeIn nd = load_node(a, gi);
-            double sg[6] = {0, 0, 0, 0, 0, 0}, qf[3] = {0, 0, 0};
-            if (a.visc) sigma_of(a, gi, nd.u, sg, qf);
-            double out[15];
+    if (i >= 0 && i <= n0) {
+        const NodeIn nd = load_node(a, gi);
+        double sg[6] = {0, 0, 0, 0, 0, 0}, qf[3] = {0, 0, 0};
+        if (a.visc) sigma_of(a, gi, nd.u, sg, qf);
+        double out[15];
 #pragma unroll
-            for (int e = 0; e < 3; ++e) gflux_of(a, e, a.xi[e][c[e] + HM], nd, sg, qf, out + 5 * e);
+        for (int e = 0; e < 3; ++e) gflux_of(a, e, a.xi[e][c[e] + HM], nd, sg, qf, out + 5 * e);
 #pragma unroll
-            for (int k = 0; k < 15; ++k) a.gfl[k][gi] = out[k];
-            return;
+        for (int k = 0; k < 15; ++k) a.gfl[k][gi] = out[k];
+        if (hj != NO_HALO) {
+            const long gh = gidx(a.g, i, hj, c[2]);
+            const NodeIn nh = load_node(a, gh);
+            gflux_of(a, 1, a.xi[1][hj + HM], nh, sg, qf, out);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) a.gfl[5 + m][gh] = out[m];
         }
-        if (i >= -HM && i <= n0 + HM) gflux_strip(a, c, 0, gi);
-        else gflux_zero(a, 0, gi);
-        if (i > n0) {
-            gflux_zero(a, 1, gi);
-            gflux_zero(a, 2, gi);
+        if (hk != NO_HALO) {
+            const long gh = gidx(a.g, i, c[1], hk);
+            const NodeIn nh = load_node(a, gh);
+            gflux_of(a, 2, a.xi[2][hk + HM], nh, sg, qf, out);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) a.gfl[10 + m][gh] = out[m];
         }
         return;
     }
-    if (jin == kin || i < 0) return;  // halo edges and corners; the lines before node 0
-    const int d = jin ? 2 : 1;        // the axis whose halo holds this row
-    if (iin) gflux_strip(a, c, d, gi);
-    else gflux_zero(a, d, gi);
+    if (i >= -HM && i <= n0 + HM) gflux_strip(a, c, 0, gi);
+    else gflux_zero(a, 0, gi);
+    if (i > n0) {
+        gflux_zero(a, 1, gi);
+        gflux_zero(a, 2, gi);
+        if (hj != NO_HALO) gflux_zero(a, 1, gidx(a.g, i, hj, c[2]));
+        if (hk != NO_HALO) gflux_zero(a, 2, gidx(a.g, i, c[1], hk));
+    }
+}
+
+// One thread per cell of the interior i-rows (j, k in 0..n), i from -16
+// (the start of the line of node -HM) to line_end_i(n + HM) (the end of the
+// line of node n + HM, which is the row pitch), the rows laid end to end (a
+// whole number of 128-byte lines each, so a wavefront covers whole lines
+// only).  A row gives G of all axes on its interior nodes and G_x on its x
+// halo strips.  A row that is the periodic image of a y (z) halo row also
+// gives that row's G_y (G_z): the viscous part is its own sigma and heat
+// flux, already in registers, the inviscid part that of the halo node.  The
+// threads of cells that no flux sweep reads (row padding past the strips,
+// i past n in halo rows) store zeros, so each field's lines are written
+// whole.
+__global__ __launch_bounds__(256) void k_gflux(PtArgs a)
+{
+    const int iend = line_end_i(a.g.n[0] + HM);
+    const unsigned rl = (unsigned)(iend + 16);  // cells per row
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned row = t / rl;
+    const unsigned nj = (unsigned)a.g.n[1] + 1;
+    if (row >= nj * (unsigned)(a.g.n[2] + 1)) return;
+    const unsigned kk = row / nj;
+    const int j = (int)(row - kk * nj), k = (int)kk;
+    gflux_cell(a, (int)(t - row * rl) - 16, j, k, halo_of_image(j, a.g.n[1]),
+               halo_of_image(k, a.g.n[2]));
 }
 
 // bchomo of the mini-app (tgvsolver.F90:271-369): the primitives of the
@@ -3010,8 +3036,8 @@ int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt, double* const* qo = nul
 // strips the flux sweeps read)
 int do_gflux(astr_ctx* c)
 {
-    // one thread per cell of the padded (j, k) rows, i from -16 to line_end_i(n + HM)
-    const long rows = (long)(c->g.n[1] + 1 + 2 * HM) * (c->g.n[2] + 1 + 2 * HM);
+    // one thread per cell of the interior (j, k) rows, i from -16 to line_end_i(n + HM)
+    const long rows = (long)(c->g.n[1] + 1) * (c->g.n[2] + 1);
     const long n = rows * (line_end_i(c->g.n[0] + HM) + 16);
     hipLaunchKernelGGL(k_gflux, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                        pt_args(c));
