@@ -147,6 +147,16 @@ module astr_hip
       real(c_double), intent(out) :: ff(*)
     end function astr_filter_field
 
+    ! transpose of the derivative (op 0) or filter (op 1) along dir
+    integer(c_int) function astr_line_adjoint(ctx, op, dir, wrap, gbar, fbar) &
+        bind(c, name='astr_line_adjoint')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: op, dir, wrap
+      real(c_double), intent(in) :: gbar(*)
+      real(c_double), intent(out) :: fbar(*)
+    end function astr_line_adjoint
+
     ! batched ensemble: ctxs(1:n) advance together (one HIP graph per step)
     integer(c_int) function astr_batch_create(ctxs, n, batch) bind(c, name='astr_batch_create')
       import :: c_int, c_ptr, c_int32_t
