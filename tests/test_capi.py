@@ -36,6 +36,12 @@ def test_binding_covers_every_declared_symbol():
     assert sorted(_lib.SIGNATURES) == declared_symbols()
 
 
+def test_fortran_interface_covers_every_declared_symbol():
+    text = open(os.path.join(ROOT, "astr_gpu_amd", "fortran", "astr_hip_mod.f90")).read()
+    bound = sorted(set(re.findall(r"bind\(c,\s*name='(astr_[a-z0-9_]+)'\)", text)))
+    assert bound == declared_symbols()
+
+
 def test_params_struct_layout():
     # 10 int32, 9 doubles, 6 int32, 2 doubles: no padding
     assert ctypes.sizeof(_lib.AstrParams) == 10 * 4 + 9 * 8 + 6 * 4 + 2 * 8
