@@ -9,8 +9,8 @@
  * a+b+c as (a+b)+c and a*b*c as (a*b)*c).  Paths are relative to the ASTR
  * source tree.
  *
- * Parity status of each part (how it is pinned) is listed in DESIGN.md; in
- * short: the line operators are pinned bit-for-bit against the reference
+ * Parity status of each part (how it is pinned): the line operators are
+ * pinned bit-for-bit against the reference
  * modules compiled from source (oracle/_ref), the mini-app variant against
  * the reference-held series miniapps/tgv_solver_3d/state.ref_128, and the
  * main-path stage order (qswap averaging, stale primitives) by invariants
