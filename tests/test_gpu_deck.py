@@ -62,6 +62,47 @@ def test_taylor_green_deck_against_the_checker(tmp_path):
     ob.close()
 
 
+def test_hit_deck_against_the_checker(tmp_path):
+    """The Taylor-Green deck with flowtype hit: the seeded turbulence field,
+    stepped by the deck runner, against the checker stepped from the same
+    field (its series of KE, enstrophy and dissipation)."""
+    dims, maxstep, dt, seed = (32, 32, 32), 3, 0.002, 5
+    print(f"case hit deck {dims} maxstep {maxstep} seed {seed}", flush=True)
+    case = shrunk_case(tmp_path, "tgv", "input.tgv", dims, maxstep, dt)
+    inp = os.path.join(case, "datin", "input.tgv")
+    text = open(inp).read()
+    assert "\ntgv\n" in text
+    open(inp, "w").write(text.replace("\ntgv\n", "\nhit\n"))
+    path = deck.run_deck(case, log=None, seed=seed)
+    head, rows = read_flowstate(path)
+    assert head == ["nstep", "time", "kenergy", "enstophy", "dissipation"]
+    assert rows.shape == (maxstep, 5)
+    d = deck.read_input(deck.find_input(case))
+    assert d["flowtype"] == "hit"
+    p = deck.params_from_deck(d)
+    st = ag.hit_state(p, seed)
+    ob = pyoracle.OracleBox(dims, variant=0, reynolds=p.reynolds, mach=p.mach,
+                            prandtl=p.prandtl, gamma=p.gamma, ref_tem=p.ref_tem,
+                            alfa=p.alfa_filter)
+    for m in range(5):
+        ob.set(f"q{m + 1}", ag.interior(st["q"][..., m], p))
+    ob.set("rho", ag.interior(st["rho"], p))
+    for c, nm in enumerate("uvw"):
+        ob.set(nm, ag.interior(st["vel"][..., c], p))
+    ob.set("prs", ag.interior(st["prs"], p))
+    ob.set("tmp", ag.interior(st["tmp"], p))
+    t = 0.0
+    for k in range(maxstep + 1):
+        ref = ob.step(dt, stats=True)
+        if k >= 1:
+            row = rows[k - 1]
+            assert row[0] == k and abs(row[1] - t) <= 1e-13 * t
+            for a, b in zip(row[2:], ref):
+                assert abs(a - b) <= 1e-10 * abs(b), (k, row, ref)
+        t += dt
+    ob.close()
+
+
 def test_riemann_deck_writes_max_q_of_each_stage_1_state(tmp_path):
     dims, maxstep, dt = (64, 64, 0), 3, 1e-4
     print(f"case riem2d deck {dims} maxstep {maxstep}", flush=True)
