@@ -78,6 +78,10 @@ def measured_traffic(family: str, n: int):
 
 CPU_SAMPLE_S = 10.0  # the least CPU time a baseline sample runs
 
+# box size, timed steps and warm-up steps of each workload
+WORKLOAD_DEFAULTS = {"tgv": (256, 20, 3), "riemann2d": (2048, 20, 3), "adjoint": (128, 20, 3),
+                     "ensemble": (None, 10, 1)}
+
 
 def cpu_baseline(n: int, dt: float):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -424,9 +428,10 @@ def bench_ensemble(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=None, help="default 20 (ensemble: 10)")
+    ap.add_argument("--warmup", type=int, default=None, help="default 3 (ensemble: 1)")
+    ap.add_argument("--n", type=int, default=None,
+                    help="box size; default 256 (tgv), 2048 (riemann2d), 128 (adjoint)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["tgv", "riemann2d", "ensemble", "adjoint"],
                     default="tgv",
@@ -448,21 +453,18 @@ def main():
               "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
+    # each workload's own sizes where the command line leaves them open
+    n0, steps0, warmup0 = WORKLOAD_DEFAULTS[args.workload]
+    args.n = n0 if args.n is None else args.n
+    args.steps = steps0 if args.steps is None else args.steps
+    args.warmup = warmup0 if args.warmup is None else args.warmup
     if args.workload == "ensemble":
-        if args.steps == 20:
-            args.steps = 10
-        if args.warmup == 3:
-            args.warmup = 1
         bench_ensemble(args)
         return
     if args.workload == "adjoint":
-        if args.n == 256:
-            args.n = 128
         bench_adjoint(args)
         return
     if args.workload == "riemann2d":
-        if args.n == 256:
-            args.n = 2048
         bench_riemann(args)
         return
 
