@@ -5,9 +5,11 @@ C ABI of include/astr_hip.h; this package is its host-side binding.
 """
 from ._lib import AstrError, AstrParams, load  # noqa: F401
 from .solver import (UDF_RIEMANN2D, AstrBatch, AstrSolver, box_metrics, cfl_dt,  # noqa: F401
-                     default_params, haloed_shape, hit_state, interior, riemann2d_state,
+                     default_params, ensemble_dims, ensemble_member_seed,
+                     haloed_shape, hit_state, interior, riemann2d_state,
                      state_from_velocity, tgv_state)
 
 __all__ = ["UDF_RIEMANN2D", "AstrBatch", "AstrError", "AstrParams", "AstrSolver", "box_metrics",
-           "cfl_dt", "default_params", "haloed_shape", "hit_state", "interior", "load", "riemann2d_state",
+           "cfl_dt", "default_params", "ensemble_dims",
+           "ensemble_member_seed", "haloed_shape", "hit_state", "interior", "load", "riemann2d_state",
            "state_from_velocity", "tgv_state"]

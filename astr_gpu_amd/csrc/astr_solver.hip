@@ -68,6 +68,7 @@ constexpr int TG = 3;      // LDS guard slots before each staged line
 constexpr int OI = 16;     // element offset of node i=0 in a row
 static_assert(OI % 16 == 0, "node i = 0 starts a 128-byte line");
 constexpr int LMAX = 16;   // rows per lane supported (line length <= 32*LMAX)
+constexpr int STATS_SLOTS = 1024;  // steps whose statistics stay on the device
 
 struct Geo {
     int n[3];    // cells per axis (nodes 0..n)
@@ -1336,6 +1337,41 @@ __global__ __launch_bounds__(256) void k_stats(PtArgs a)
         }
 }
 
+// The nb workgroup partials of k_stats summed in double-double by one
+// workgroup (each thread a strided share, then a tree over the threads),
+// and the means kenergycal / enstophycal / diss_rate_cal report: ke and
+// enstrophy 1/2 of the mean, dissipation the mean (roinf = uinf = l_0 = 1).
+// Written to out3 on the device, so a step's statistics need no host
+// round trip.
+__global__ __launch_bounds__(256) void k_stats_fin(const double* __restrict__ part, long nb,
+                                                  double n3, double* __restrict__ out3)
+{
+    __shared__ double sh[3][256], sl[3][256];
+    double h[3] = {0.0, 0.0, 0.0}, l[3] = {0.0, 0.0, 0.0};
+    for (long b = threadIdx.x; b < nb; b += 256)
+        for (int c = 0; c < 3; ++c) dd_add(h[c], l[c], part[6 * b + 2 * c], part[6 * b + 2 * c + 1]);
+    for (int c = 0; c < 3; ++c) {
+        sh[c][threadIdx.x] = h[c];
+        sl[c][threadIdx.x] = l[c];
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int c = 0; c < 3; ++c) {
+                double hh = sh[c][threadIdx.x], ll = sl[c][threadIdx.x];
+                dd_add(hh, ll, sh[c][threadIdx.x + s], sl[c][threadIdx.x + s]);
+                sh[c][threadIdx.x] = hh;
+                sl[c][threadIdx.x] = ll;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out3[0] = 0.5 * sh[0][0] / n3;
+        out3[1] = 0.5 * sh[1][0] / n3;
+        out3[2] = sh[2][0] / n3;
+    }
+}
+
 // maxmincal's max_q (src/statistic.F90:737-763): the largest value of each
 // q(m) over the nodes 0..n; each workgroup strides over the nodes and
 // leaves 5 partial maxima (max is exact, so the order does not matter)
@@ -2487,12 +2523,15 @@ struct astr_ctx {
     double* part = nullptr;
     long npart = 0;
     std::vector<double> host_part;
+    // per-step statistics of astr_rk3_steps on the device (3 per step,
+    // STATS_SLOTS steps), copied to the host after the steps
+    double* stats_dev = nullptr;
     double const1, const2, const3, const4, const5, const6, const7, tempconst, tempconst1;
     bool have_geom = false, have_state = false;
     int bctype[6] = {1, 1, 1, 1, 1, 1};  // boucon per face: 1 nothing done, 0 user
     int udf = 0;                          // the user condition (ASTR_UDF_*), 0 none
     double time = 0.0;                    // simulation time (commvar time)
-    bool timing = false;
+    int timing = 0;  // 0 off, 1 per kernel family, 2 whole calls of astr_rk3_steps
     hipEvent_t ev[16] = {};
     double tms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -3264,24 +3303,25 @@ int do_rk(astr_ctx* c, int stage, double dt)
     return launch_pt(c, k_rk, n_interior(c), a);
 }
 
-int do_stats(astr_ctx* c, double* out)
+// k_stats and k_stats_fin: the three statistics of the current state into
+// dst (3 doubles of device memory), queued on the solver's stream
+int launch_stats(astr_ctx* c, double* dst)
 {
     PtArgs a = pt_args(c);
     const long n = (long)c->g.n[0] * c->g.n[1] * c->g.n[2];
     TRY(launch_pt(c, k_stats, n, a, 256));
     const long nb = (n + 255) / 256;
-    HIPCHK(hipMemcpyAsync(c->host_part.data(), c->part, sizeof(double) * 6 * nb,
-                          hipMemcpyDeviceToHost, c->stream));
+    hipLaunchKernelGGL(k_stats_fin, dim3(1), dim3(256), 0, c->stream, (const double*)c->part, nb,
+                       (double)c->g.n[0] * (double)c->g.n[1] * (double)c->g.n[2], dst);
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
+}
+
+int do_stats(astr_ctx* c, double* out)
+{
+    TRY(launch_stats(c, c->stats_dev));
+    HIPCHK(hipMemcpyAsync(out, c->stats_dev, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    double s[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
-    for (long b = 0; b < nb; ++b)
-        for (int k = 0; k < 3; ++k)
-            dd_add(s[k], sl[k], c->host_part[6 * b + 2 * k], c->host_part[6 * b + 2 * k + 1]);
-    const double n3 = (double)c->g.n[0] * (double)c->g.n[1] * (double)c->g.n[2];
-    // roinf = uinf = 1 and l_0 = 1 in the nondimensional box
-    out[0] = 0.5 * s[0] / n3;
-    out[1] = 0.5 * s[1] / n3;
-    out[2] = s[2] / n3;
     return ASTR_OK;
 }
 
@@ -3325,11 +3365,11 @@ int download_field(astr_ctx* c, const double* dev, double* host, std::vector<dou
 
 int time_mark(astr_ctx* c, int slot)
 {
-    if (c->timing) HIPCHK(hipEventRecord(c->ev[slot], c->stream));
+    if (c->timing == 1) HIPCHK(hipEventRecord(c->ev[slot], c->stream));
     return ASTR_OK;
 }
 
-int do_step(astr_ctx* c, double dt, double* stats, bool timed)
+int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed)
 {
     // timing families: 0 filter, 1 boucon and qswap, 2 grad, 3 central:
     // sigma and the axis fluxes (k_gflux), upwind: the shock sensor,
@@ -3352,7 +3392,7 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
             if (timed) TRY(time_mark(c, 3));
             TRY(do_gflux(c));
             if (timed) TRY(time_mark(c, 4));
-            if (s == 1 && stats) TRY(do_stats(c, stats));
+            if (s == 1 && stats_slot) TRY(launch_stats(c, stats_slot));
             if (timed) TRY(time_mark(c, 5));
             TRY(do_flux_sweeps(c, s, dt));
             if (timed) TRY(time_mark(c, 6));
@@ -3387,7 +3427,7 @@ int do_step(astr_ctx* c, double dt, double* stats, bool timed)
         else if (sensor_on(c)) TRY(do_sensor(c));
         if (timed) TRY(time_mark(c, 4));
         // rkfirst's statcal sees the stage-1 state and its gradients
-        if (s == 1 && stats) TRY(do_stats(c, stats));
+        if (s == 1 && stats_slot) TRY(launch_stats(c, stats_slot));
         if (timed) TRY(time_mark(c, 5));
         if (upwind) {
             TRY(do_upwind(c));
@@ -3926,6 +3966,8 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     c->host_part.resize(6 * c->npart);
     if (hipMalloc(&c->part, sizeof(double) * 6 * c->npart) != hipSuccess)
         return cleanup(fail(ASTR_E_NOMEM, "partials allocation failed"));
+    if (hipMalloc(&c->stats_dev, sizeof(double) * 3 * STATS_SLOTS) != hipSuccess)
+        return cleanup(fail(ASTR_E_NOMEM, "statistics allocation failed"));
     for (int e = 0; e < 16; ++e)
         if (hipEventCreate(&c->ev[e]) != hipSuccess) return cleanup(fail(ASTR_E_HIP, "event"));
     *out = c;
@@ -3944,6 +3986,7 @@ int astr_destroy(astr_ctx* c)
     (void)hipFree(c->adj_base);
     (void)hipFree(c->xi);
     (void)hipFree(c->part);
+    (void)hipFree(c->stats_dev);
     (void)hipFree(c->lshock);
     for (int d = 0; d < 3; ++d)
         for (int w = 0; w < 2; ++w) (void)hipFree(c->cmp_ac[d][w]);
@@ -4100,7 +4143,7 @@ int astr_rhscal(astr_ctx* c)
     if (c && c->p.convection != 0) TRY(ready_state(c));
     else TRY(ready(c));
     if (c->p.variant == 1) return fail(ASTR_E_UNSUPPORTED, "the mini-app variant runs whole steps only");
-    return do_rhscal(c, c->timing);
+    return do_rhscal(c, c->timing == 1);
 }
 
 int astr_download_shock(astr_ctx* c, uint8_t* lshock)
@@ -4154,26 +4197,48 @@ int astr_qmax(astr_ctx* c, double* qmax5)
     return ASTR_OK;
 }
 
-int astr_rk3_step(astr_ctx* c, double dt, double* stats)
-{
-    TRY(ready(c));
-    if (stats && c->g.n[2] == 0)
-        return fail(ASTR_E_UNSUPPORTED, "statistics of three-dimensional boxes only");
-    TRY(do_step(c, dt, stats, c->timing));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return ASTR_OK;
-}
-
-int astr_rk3_steps(astr_ctx* c, int32_t nsteps, double dt, double* stats)
+// nsteps whole steps; restart: the per-family kernel times start again at
+// zero (astr_rk3_steps) rather than adding up over calls (astr_rk3_step)
+static int rk3_steps(astr_ctx* c, int32_t nsteps, double dt, double* stats, bool restart)
 {
     TRY(ready(c));
     if (nsteps < 0) return fail(ASTR_E_ARG, "nsteps < 0");
     if (stats && c->g.n[2] == 0)
         return fail(ASTR_E_UNSUPPORTED, "statistics of three-dimensional boxes only");
-    for (int f = 0; f < 9; ++f) c->tms[f] = 0.0;
-    for (int s = 0; s < nsteps; ++s) TRY(do_step(c, dt, stats ? stats + 3 * s : nullptr, c->timing));
+    if (c->timing == 1 && restart)
+        for (int f = 0; f < 9; ++f) c->tms[f] = 0.0;
+    if (c->timing == 2) HIPCHK(hipEventRecord(c->ev[14], c->stream));
+    // each step's statistics go to a device slot; the slots are copied to
+    // the host when they are all used and after the last step
+    int done = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        const int slot = s - done;
+        TRY(do_step(c, dt, stats ? c->stats_dev + 3 * slot : nullptr, c->timing == 1));
+        if (stats && (slot + 1 == STATS_SLOTS || s + 1 == nsteps)) {
+            HIPCHK(hipMemcpyAsync(stats + 3 * (long)done, c->stats_dev,
+                                  sizeof(double) * 3 * (slot + 1), hipMemcpyDeviceToHost,
+                                  c->stream));
+            done = s + 1;
+        }
+    }
+    if (c->timing == 2) HIPCHK(hipEventRecord(c->ev[15], c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->timing == 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[14], c->ev[15]));
+        c->tms[7] += ms;
+    }
     return ASTR_OK;
+}
+
+int astr_rk3_step(astr_ctx* c, double dt, double* stats)
+{
+    return rk3_steps(c, 1, dt, stats, false);
+}
+
+int astr_rk3_steps(astr_ctx* c, int32_t nsteps, double dt, double* stats)
+{
+    return rk3_steps(c, nsteps, dt, stats, true);
 }
 
 int astr_bench_copy(astr_ctx* c, int32_t reps, double* ms_per_copy, int64_t* bytes_moved)
@@ -4392,7 +4457,8 @@ int astr_synchronize(astr_ctx* c)
 int astr_set_timing(astr_ctx* c, int32_t on)
 {
     if (!c) return fail(ASTR_E_ARG, "null context");
-    c->timing = on != 0;
+    if (on < 0 || on > 2) return fail(ASTR_E_ARG, "timing mode must be 0, 1 or 2");
+    c->timing = on;
     for (int f = 0; f < 9; ++f) c->tms[f] = 0.0;
     return ASTR_OK;
 }

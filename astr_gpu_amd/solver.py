@@ -213,8 +213,12 @@ class AstrSolver:
     def synchronize(self) -> None:
         check(self.lib.astr_synchronize(self.ctx), "astr_synchronize")
 
-    def set_timing(self, on: bool) -> None:
-        check(self.lib.astr_set_timing(self.ctx, 1 if on else 0), "astr_set_timing")
+    def set_timing(self, on) -> None:
+        """False/0 off; True/1 per kernel family (events between the
+        families); 2 whole steploop calls only (two events per call, the
+        device time in timing()[7])."""
+        mode = int(on) if not isinstance(on, bool) else (1 if on else 0)
+        check(self.lib.astr_set_timing(self.ctx, mode), "astr_set_timing")
 
     def timing(self) -> np.ndarray:
         """ms per family: filter, boucon+qswap, grad, gflux (central) or sensor
@@ -317,12 +321,13 @@ def hit_state(p: AstrParams, seed: int, mt: float = 0.3, k0: float = 4.0,
         u3 = np.where(on_axis, 0.0, -a2 * k12 / kk)
     u1[kk == 0] = u2[kk == 0] = u3[kk == 0] = 0.0
     vel = [np.fft.irfftn(u, s=n, axes=(0, 1, 2)) for u in (u1, u2, u3)]
-    # solenoidal projection on the full spectrum (the real transform's
+    # solenoidal projection of the real field (the real transform's
     # Hermitian completion of the k3 = 0 and Nyquist planes breaks the
-    # construction's divergence-free property there); Nyquist modes removed
-    fk = [np.fft.fftn(v) for v in vel]
-    k = [np.fft.fftfreq(n[d], d=1.0 / n[d]) for d in range(3)]
-    K = np.meshgrid(*k, indexing="ij")
+    # construction's divergence-free property there); Nyquist modes
+    # removed.  The projector I - k k^T / |k|^2 and the Nyquist mask are
+    # even in k, so the half spectrum of the real transform carries it.
+    fk = [np.fft.rfftn(v) for v in vel]
+    K = (kx[:, None, None], ky[None, :, None], kz[None, None, :])
     k2 = K[0] ** 2 + K[1] ** 2 + K[2] ** 2
     k2[0, 0, 0] = 1.0
     dot = (K[0] * fk[0] + K[1] * fk[1] + K[2] * fk[2]) / k2
@@ -334,10 +339,26 @@ def hit_state(p: AstrParams, seed: int, mt: float = 0.3, k0: float = 4.0,
     for d in range(3):
         g = fk[d] - K[d] * dot
         g[nyq] = 0.0
-        vel.append(np.real(np.fft.ifftn(g)))
+        vel.append(np.fft.irfftn(g, s=n, axes=(0, 1, 2)))
     urms = math.sqrt(sum(float(np.mean(v * v)) for v in vel))
     vel = [v / urms for v in vel]  # sqrt(<u.u>) = 1, so Mt = Ma
     return state_from_velocity(p, np.stack(vel))
+
+
+ENSEMBLE_SIZES = (64, 96, 128, 160, 192)
+
+
+def ensemble_dims(nbox: int = 48, seed: int = 1, sizes=ENSEMBLE_SIZES) -> list:
+    """Box sizes of the decaying-turbulence ensemble: nx, ny, nz of each of
+    nbox boxes drawn uniformly from `sizes` by a generator seeded with
+    `seed` (numpy's default_rng), box after box, x then y then z."""
+    rng = np.random.default_rng(seed)
+    return [tuple(int(v) for v in rng.choice(sizes, 3)) for _ in range(nbox)]
+
+
+def ensemble_member_seed(seed: int, member: int) -> int:
+    """Seed of member `member`'s turbulence field in the ensemble of `seed`."""
+    return seed * 1000 + member
 
 
 def state_from_velocity(p: AstrParams, u0) -> dict:

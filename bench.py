@@ -1,26 +1,33 @@
-"""Throughput of the HIP solver on the Taylor-Green box (FP64, RK3).
+"""Throughput of the HIP solver (FP64, RK3): the three parts of the metric.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+                    [--legs plain,batched,adjoint]
 
-Workload: 256^3 periodic Taylor-Green box (Re 1600, Ma 0.1, 643c compact
-derivatives, alpha 0.49 compact filter every stage, RK3, dt 0.004), the
-whole domain resident in one GPU's HBM.  A step is one full RK3 step (three
-stages of filter, qswap, right-hand side, update).  The metric counts unique
-periodic grid points (n^3) advanced one RK3 step per second.
+Plain (the value): the 256^3 periodic Taylor-Green box (Re 1600, Ma 0.1,
+643c compact derivatives, alpha 0.49 compact filter every stage, RK3 with
+dt 0.001, i.e. the acoustic CFL number of the 64^3 box at dt 0.004), the
+whole domain resident in one GPU's HBM, the statistics of every step
+computed as rkfirst does.  A step is one full RK3 step (three stages of
+filter, qswap, right-hand side, update).  The metric counts unique periodic
+grid points (n^3) advanced one RK3 step per second: K steps between two HIP
+events on the solver's stream, median of 5 runs.
+
+Batched (the "batched" object): 48 periodic turbulence boxes of mixed sizes
+(seed 1) advanced together, against looping the plain solver over them.
+Adjoint (the "adjoint" object): the gradient of the enstrophy after 40 RK3
+steps of a 128^3 turbulence box by the discrete adjoint, its device-time
+cost over the plain forward's and its agreement with central differences.
 
 Other workloads (--workload): riemann2d, RK3 steps of the 2-D four-quadrant
 Riemann problem at 2048^2 with the explicit upwind scheme and the example's
-boundary conditions; ensemble, 48 periodic turbulence boxes of mixed sizes
-advanced together against looping the plain solver over them; adjoint, the
-gradient of the enstrophy after 40 RK3 steps of a 128^3 turbulence box by
-the discrete adjoint, its cost over the forward's and its agreement with
-central differences.
+boundary conditions; ensemble and adjoint, those legs alone.
 
 Prints one JSON line.  roofline: the dominant kernel family, its algorithmic
 HBM bytes per launch over its average launch time (HIP events on the
-solver's stream, over the timed region).  cpu_baseline: the checker's C
-restatement (oracle/, OpenMP over lines) on the host cores, one RK3 step at
-the same size.
+solver's stream).  step_roofline: the whole step against the reference's
+pass structure, 4 224 B per grid-point-RK3-step.  cpu_baseline: the
+checker's C restatement (oracle/, OpenMP over lines) on the host cores,
+whole RK3 steps of the same box for at least 10 s.
 """
 from __future__ import annotations
 
@@ -59,6 +66,8 @@ FAMILY_LAUNCHES = {"filter": 3, "qswap": 2, "grad": 3, "gflux": 1, "stats": 0, "
                    "prim": 1}
 FAMILY_NAMES = ["filter", "qswap", "grad", "gflux", "stats", "flux", "prim"]
 STATS_DOUBLES = 14.0
+# the reference's own pass structure: 176 doubles per point per stage
+REF_BYTES_PER_PT_STEP = 4224.0
 # whole-step algorithmic bytes per node: three stages of the kernels above
 # plus the statistics once
 ALGO_BYTES_PER_PT_STEP = 8.0 * (3.0 * sum(FAMILY_DOUBLES[f] * FAMILY_LAUNCHES[f]
@@ -83,6 +92,18 @@ WORKLOAD_DEFAULTS = {"tgv": (256, 20, 3), "riemann2d": (2048, 20, 3), "adjoint":
                      "ensemble": (None, 10, 1)}
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (what lscpu prints as "Model name")."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n: int, dt: float):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -99,9 +120,9 @@ def cpu_baseline(n: int, dt: float):
     thr = pyoracle.oracle_lib().orc_omp_threads()
     b.close()
     return {"value": steps * n ** 3 / el, "unit": "grid-point-RK3-steps/s", "cores": int(thr),
-            "kind": "port",
-            "sample": f"{steps} RK3 steps of the {n}^3 Taylor-Green box by the C restatement "
-                      f"(OpenMP, {thr} threads), {el:.1f} s"}
+            "kind": "port", "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": f"{steps} RK3 steps (dt {dt:g}) of the {n}^3 Taylor-Green box by the C "
+                      f"restatement (OpenMP, {thr} threads), {el:.1f} s"}
 
 
 FP64_PEAK_TFLOPS = 78.6
@@ -154,7 +175,7 @@ def cpu_baseline_riemann(n_small: int):
     el = time.perf_counter() - t0
     steps = 1 + more
     return {"value": steps * (n_small + 1) ** 2 / el, "unit": "grid-point-RK3-steps/s",
-            "cores": 1, "kind": "port",
+            "cores": 1, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"{steps} RK3 steps of the {n_small}^2 Riemann box by the C restatement "
                       f"(one thread), {el:.1f} s"}
 
@@ -217,7 +238,7 @@ def bench_riemann(args):
     }
     if flops:
         achieved = flops / (avg_ms * 1e-3) / 1e12
-        out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+        out["roofline"] = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                            "traffic": traffic,
                            "kernel": "k_uwd (interface fluxes), 2 launches/stage",
@@ -246,24 +267,26 @@ def bench_riemann(args):
     print(json.dumps(out))
 
 
-def bench_adjoint(args):
+def adjoint_leg(n=128, nsteps=40, ckpt=8, seed=1, fd_dirs=8, fd_eps=1e-6):
     """Discrete adjoint on the decaying-turbulence box (n^3, HIT spectrum,
-    seeded; Mt 0.3, Re 173): J = enstrophy after nsteps RK3 steps, dJ/du0
-    by the GPU adjoint (checkpoints every ckpt steps), checked against
-    central differences (eps 1e-6) along 8 seeded Gaussian directions.  The
-    metric is the device time of the reversal (the backward of the autograd
-    binding) over the time of the plain forward steps (lower is better); the
-    whole gradient with its recorded forward is reported beside it."""
+    seeded; Mt 0.3, Re 173): J = enstrophy after nsteps RK3 steps (dt
+    0.002), dJ/du0 by the GPU adjoint (checkpoints every ckpt steps),
+    checked against central differences (eps fd_eps) along fd_dirs seeded
+    Gaussian directions.  Every time is device time between HIP events on
+    the solver's stream: the plain forward steps (a pair of events around
+    the steps), the recorded forward with the objective and the reversal
+    (events inside astr_adjoint_enstrophy).  The cost ratio is the whole
+    gradient (recorded forward + reversal) over the plain forward."""
     import torch
 
     import astr_gpu_amd as ag
     from astr_gpu_amd.autograd import velocity_state
 
-    n, nsteps, ckpt, dt = args.n, args.nsteps, args.ckpt, 0.002
+    dt = 0.002
     p = ag.default_params(n, mach=0.3, reynolds=173.0)
     s = ag.AstrSolver(p, device=0)
     s.upload_geom(*ag.box_metrics(p))
-    st = ag.hit_state(p, args.seed)
+    st = ag.hit_state(p, seed)
     u0 = np.moveaxis(ag.interior(st["vel"], p)[:-1, :-1, :-1], -1, 0).copy()
 
     def upload(state):
@@ -279,94 +302,82 @@ def bench_adjoint(args):
     # warm-up of every kernel involved
     upload(st)
     s.adjoint_enstrophy(min(nsteps, 2), dt, ckpt)
-    # plain forward
-    reps = max(1, args.steps // 10)
+    runs = 3
     fwd = []
-    for _ in range(reps):
+    for _ in range(runs):
         upload(st)
-        s.synchronize()
-        t0 = time.perf_counter()
+        s.set_timing(2)
         s.steploop(nsteps, dt)
-        s.synchronize()
-        fwd.append(time.perf_counter() - t0)
-    t_fwd = min(fwd)
+        fwd.append(float(s.timing()[7]))
+    s.set_timing(0)
+    t_fwd = float(np.median(fwd))
 
-    # gradient: the recorded forward (astr_adjoint_record: checkpoints every
-    # k steps, the last segment's stages stored), then the reversal
-    # (astr_adjoint_reverse: the earlier segments recomputed, every stage
-    # reversed; device time by HIP events, and the wall time with the
-    # download of the two haloed gradients to host memory)
     def gradient(k):
-        best, res = None, None
-        for _ in range(reps):
+        res = []
+        for _ in range(runs):
             upload(st)
-            s.synchronize()
-            t0 = time.perf_counter()
-            J = s.adjoint_record(nsteps, dt, k)
-            s.synchronize()
-            t1 = time.perf_counter()
-            qb, pb, ms = s.adjoint_reverse()
-            t2 = time.perf_counter()
-            cur = (t1 - t0, 1e-3 * ms, t2 - t1)
-            if best is None or cur[1] < best[1]:
-                best, res = cur, (J, qb, pb)
-        return best, res
+            J, qb, pb, ms = s.adjoint_enstrophy(nsteps, dt, k)
+            res.append((float(ms[0]), float(ms[1]), J, qb, pb))
+        rec = float(np.median([r[0] for r in res]))
+        rev = float(np.median([r[1] for r in res]))
+        return rec, rev, res[-1][2:]
 
-    (rec_all, rev_all, _), _ = gradient(nsteps)
-    (t_rec, t_rev, t_rev_wall), (J, qb, pb) = gradient(ckpt)
+    rec_all, rev_all, _ = gradient(nsteps)
+    t_rec, t_rev, (J, qb, pb) = gradient(ckpt)
     # dJ/du0 through the state construction
     x = torch.tensor(u0, requires_grad=True)
     q0, pr0 = velocity_state(p, x)
     torch.autograd.backward([q0, pr0], [torch.from_numpy(qb), torch.from_numpy(pb)])
     g = x.grad.numpy()
-    rng = np.random.default_rng(args.seed)
-    errs = {1e-6: [], 1e-4: []}
-    for _ in range(args.fd_dirs):
+    rng = np.random.default_rng(seed)
+    errs = []
+    for _ in range(fd_dirs):
         v = rng.standard_normal(u0.shape)
         ad = float(np.sum(g * v))
-        for eps in errs:
-            fd = (objective(u0 + eps * v) - objective(u0 - eps * v)) / (2 * eps)
-            errs[eps].append(abs(fd - ad) / abs(ad))
+        fd = (objective(u0 + fd_eps * v) - objective(u0 - fd_eps * v)) / (2 * fd_eps)
+        errs.append(abs(fd - ad) / abs(ad))
     s.close()
+    return {
+        "workload": f"discrete adjoint, HIT {n}^3 Mt 0.3 Re 173 seed {seed}, {nsteps} RK3 steps "
+                    f"dt {dt}, checkpoints every {ckpt} steps, J = enstrophy of the final state",
+        "gradient_over_forward": (t_rec + t_rev) / t_fwd,
+        "reversal_over_forward": t_rev / t_fwd,
+        "forward_ms": t_fwd, "recorded_forward_ms": t_rec, "reversal_ms": t_rev,
+        "gradient_over_forward_all_stages_kept": (rec_all + rev_all) / t_fwd,
+        "timing": f"device time between HIP events on the solver's stream, median of {runs} runs",
+        "J": J, "fd_max_rel_err": max(errs, default=None), "fd_eps": fd_eps,
+        "fd_directions": fd_dirs,
+    }
+
+
+def bench_adjoint(args):
+    """--workload adjoint: the adjoint leg alone as the JSON line; the value
+    is the whole gradient's cost over the plain forward's (lower is
+    better)."""
+    leg = adjoint_leg(args.n, args.nsteps, args.ckpt, args.seed, args.fd_dirs)
     out = {
-        "metric": "adjoint/forward cost ratio (reversal of the recorded steps over the plain "
-                  "forward steps)",
-        "value": t_rev / t_fwd, "unit": "ratio", "n_gpus": 1, "steps": nsteps,
-        "warmup": 1, "ms_per_step": 1e3 * t_fwd / nsteps, "higher_is_better": False,
+        "metric": "adjoint/forward cost ratio (recorded forward + reversal over the plain "
+                  "forward, device time)",
+        "value": leg["gradient_over_forward"], "unit": "ratio", "n_gpus": 1,
+        "steps": args.nsteps, "warmup": 1,
+        "ms_per_step": leg["forward_ms"] / args.nsteps, "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded decaying isotropic turbulence)",
-        "config": {"workload": f"discrete adjoint, HIT {n}^3 Mt 0.3 Re 173 seed {args.seed}, "
-                               f"{nsteps} RK3 steps dt {dt}, checkpoints every {ckpt} steps, "
-                               f"J = enstrophy of the final state",
-                   "n": n, "parallelism": "single GPU"},
-        "forward_s": t_fwd, "recorded_forward_s": t_rec, "reversal_s": t_rev,
-        "reversal_with_download_s": t_rev_wall,
-        "gradient_total_ratio": (t_rec + t_rev) / t_fwd,
-        "ratio_all_stages_kept": rev_all / t_fwd,
-        "gradient_total_ratio_all_stages_kept": (rec_all + rev_all) / t_fwd,
-        "J": J, "fd_max_rel_err": max(errs[1e-6], default=None), "fd_eps": 1e-6,
-        "fd_directions": args.fd_dirs, "fd_max_rel_err_eps_1e-4": max(errs[1e-4], default=None),
-        "roofline": None, "cpu_baseline": None,
+        "config": {"workload": leg["workload"], "n": args.n, "parallelism": "single GPU"},
+        "adjoint": leg, "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
 
 
-ENSEMBLE_SIZES = (64, 96, 128, 160, 192)
-
-
-def ensemble_dims(nbox: int, seed: int):
-    rng = __import__("numpy").random.default_rng(seed)
-    return [tuple(int(v) for v in rng.choice(ENSEMBLE_SIZES, 3)) for _ in range(nbox)]
-
-
-def bench_ensemble(args):
+def ensemble_leg(nbox=48, seed=1, steps=10, warmup=1, sizes=None):
     """48 independent periodic boxes of decaying isotropic turbulence (sizes
     drawn from {64,...,192} per axis, seeded), the central scheme with
     filter and viscous terms: aggregate grid-point-RK3-steps/s of the
-    batched graph against looping the plain solver over the boxes."""
+    batched graph against looping the plain solver over the boxes, each
+    timed by the wall clock around its steps (both end synchronised)."""
     import astr_gpu_amd as ag
 
-    dims = ensemble_dims(args.nbox, args.seed)
+    dims = ag.ensemble_dims(nbox, seed, sizes or ag.solver.ENSEMBLE_SIZES)
     dt = 0.002
     # Re_lambda ~ 50 for the k^4 exp(-2 (k/4)^2) spectrum at sqrt(<u.u>) = 1:
     # lambda = 0.5, u' = 3^-1/2, so Re = 50 / (0.5 * 3^-1/2) = 173
@@ -375,54 +386,113 @@ def bench_ensemble(args):
         p = ag.default_params(d, mach=0.3, reynolds=173.0)
         s = ag.AstrSolver(p, device=0)
         s.upload_geom(*ag.box_metrics(p))
-        st = ag.hit_state(p, args.seed * 1000 + i)
+        st = ag.hit_state(p, ag.ensemble_member_seed(seed, i))
         s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
         solvers.append(s)
     pts = float(sum(a * b * c for a, b, c in dims))
     # plain solver, box after box
     for s in solvers:
-        s.steploop(args.warmup, dt)
+        s.steploop(warmup, dt)
     t0 = time.perf_counter()
     for s in solvers:
-        s.steploop(args.steps, dt)
+        s.steploop(steps, dt)
     el_loop = time.perf_counter() - t0
     # batched: one graph per step over all boxes, and the same launches made
     # directly on the members' streams
     b = ag.AstrBatch(solvers)
-    b.rk3_steps(args.warmup, dt)
+    b.rk3_steps(warmup, dt)
     t0 = time.perf_counter()
-    b.rk3_steps(args.steps, dt)
+    b.rk3_steps(steps, dt)
     el_graph = time.perf_counter() - t0
     b.set_mode(1)
-    b.rk3_steps(args.warmup, dt)
+    b.rk3_steps(warmup, dt)
     t0 = time.perf_counter()
-    b.rk3_steps(args.steps, dt)
+    b.rk3_steps(steps, dt)
     el_streams = time.perf_counter() - t0
     b.close()
     el = min(el_graph, el_streams)
     for s in solvers:
         s.close()
-    value = pts * args.steps / el
-    loop_value = pts * args.steps / el_loop
+    value = pts * steps / el
+    loop_value = pts * steps / el_loop
+    return {
+        "workload": f"{nbox} periodic boxes, sizes per axis from "
+                    f"{list(sizes or ag.solver.ENSEMBLE_SIZES)} (seed {seed}), HIT Mt 0.3, "
+                    f"Re 173, 643c, filter 0.49, RK3 dt {dt}",
+        "boxes": [list(d) for d in dims], "points": pts, "steps": steps,
+        "value": value, "loop_value": loop_value, "batched_over_loop": value / loop_value,
+        "graph_value": pts * steps / el_graph, "streams_value": pts * steps / el_streams,
+        "ms_per_step": 1e3 * el / steps, "loop_ms_per_step": 1e3 * el_loop / steps,
+    }
+
+
+def bench_ensemble(args):
+    """--workload ensemble: the ensemble leg alone as the JSON line."""
+    leg = ensemble_leg(args.nbox, args.seed, args.steps, args.warmup)
     out = {
         "metric": "FP64 grid-point-RK3-steps/s, batched ensemble",
-        "value": value, "unit": "grid-point-RK3-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "value": leg["value"], "unit": "grid-point-RK3-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": leg["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded decaying isotropic turbulence)",
-        "config": {"workload": f"{args.nbox} periodic boxes, sizes per axis from "
-                               f"{list(ENSEMBLE_SIZES)} (seed {args.seed}), HIT Mt 0.3, "
-                               f"Re 173, 643c, filter 0.49, RK3 dt {dt}",
-                   "boxes": [list(d) for d in dims], "points": pts,
+        "config": {"workload": leg["workload"], "boxes": leg["boxes"], "points": leg["points"],
                    "parallelism": "single GPU"},
-        "loop_value": loop_value,
-        "batched_over_loop": value / loop_value,
-        "graph_value": pts * args.steps / el_graph,
-        "streams_value": pts * args.steps / el_streams,
-        "roofline": None,
-        "cpu_baseline": None,
+        "loop_value": leg["loop_value"], "batched_over_loop": leg["batched_over_loop"],
+        "graph_value": leg["graph_value"], "streams_value": leg["streams_value"],
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
+
+
+def tgv_dt(n: int) -> float:
+    """dt of the n^3 Taylor-Green box: 0.004 at 64^3 (the small-size
+    workload), scaled with the spacing so that the acoustic CFL number
+    (|u| + 1/Ma) dt / dx stays 0.45 (0.001 at 256^3, the reference deck's
+    own dt; 0.004 at 256^3 would be 1.8, past the RK3 stability limit of
+    the compact scheme)."""
+    return 0.004 * 64 / n
+
+
+TIMED_RUNS = 5  # the value is the median of this many timed runs
+
+
+def tgv_leg(n: int, steps: int, warmup: int):
+    """The headline: n^3 Taylor-Green RK3 steps with the statistics of every
+    step (rkfirst's statcal, src/mainloop.F90:626), the box resident in HBM.
+    Each of TIMED_RUNS runs of `steps` steps is bracketed by two HIP events
+    on the solver's stream; the value is the median.  One more run with
+    events between the kernel families gives the roofline's launch times."""
+    import astr_gpu_amd as ag
+
+    dt = tgv_dt(n)
+    p = ag.default_params(n)
+    s = ag.AstrSolver(p, device=0)
+    s.upload_geom(*ag.box_metrics(p))
+    st = ag.tgv_state(p)
+    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+    del st
+    s.steploop(warmup, dt, stats=True)
+    runs_ms, walls = [], []
+    stats = None
+    for _ in range(TIMED_RUNS):
+        s.set_timing(2)
+        t0 = time.perf_counter()
+        stats = s.steploop(steps, dt, stats=True)   # synchronizes before returning
+        walls.append(time.perf_counter() - t0)
+        runs_ms.append(float(s.timing()[7]))
+    # kernel families timed with events between them (not the value: the
+    # events wait at every stage)
+    s.set_timing(1)
+    s.steploop(steps, dt, stats=True)
+    fam = s.timing()
+    s.set_timing(0)
+    copy_ms, copy_bytes = s.bench_copy(20)
+    s.close()
+    ms = float(np.median(runs_ms))
+    return dict(n=n, dt=dt, steps=steps, ms=ms, runs_ms=runs_ms, walls=walls, fam=fam,
+                copy_gbs=copy_bytes / (copy_ms * 1e-3) / 1e9,
+                last_stats=[float(v) for v in np.asarray(stats)[-1]],
+                finite=bool(np.all(np.isfinite(np.asarray(stats)))))
 
 
 def main():
@@ -435,17 +505,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["tgv", "riemann2d", "ensemble", "adjoint"],
                     default="tgv",
-                    help="tgv: the headline RK3 step; riemann2d: RK3 steps of the 2-D "
-                         "Riemann problem (upwind, boundary conditions); ensemble: batched "
-                         "boxes against looping the plain solver")
+                    help="tgv: the headline (plain 256^3 steps, with the batched and the "
+                         "adjoint legs unless --legs says otherwise); riemann2d: RK3 steps of "
+                         "the 2-D Riemann problem (upwind, boundary conditions); ensemble, "
+                         "adjoint: that leg alone")
+    ap.add_argument("--legs", default="plain,batched,adjoint",
+                    help="tgv workload: the parts of the metric to measure")
     ap.add_argument("--nsteps", type=int, default=40, help="adjoint: RK3 steps of the objective")
     ap.add_argument("--ckpt", type=int, default=8, help="adjoint: steps between kept states")
     ap.add_argument("--fd-dirs", type=int, default=8,
                     help="adjoint: directions of the central-difference check (0: none)")
+    ap.add_argument("--adj-n", type=int, default=128, help="tgv workload: adjoint leg box size")
     ap.add_argument("--t-end", type=float, default=0.0,
                     help="riemann2d: also run the problem from t = 0 to this time (0.3 in "
                          "the benchmark) and report its wall time")
     ap.add_argument("--nbox", type=int, default=48)
+    ap.add_argument("--ens-sizes", default=None,
+                    help="tgv workload: axis sizes of the batched leg (default 64,...,192)")
     ap.add_argument("--seed", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -468,36 +544,13 @@ def main():
         bench_riemann(args)
         return
 
-    import astr_gpu_amd as ag
-
-    n, dt = args.n, 0.004
-    p = ag.default_params(n)
-    s = ag.AstrSolver(p, device=0)
-    s.upload_geom(*ag.box_metrics(p))
-    st = ag.tgv_state(p)
-    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
-    del st
-
-    s.steploop(args.warmup, dt)
-    # timed region: K steps, kernel families timed with events on the stream
-    s.set_timing(True)
-    t0 = time.perf_counter()
-    s.steploop(args.steps, dt)   # synchronizes before returning
-    el = time.perf_counter() - t0
-    fam = s.timing()
-    s.set_timing(False)
-    # untimed-by-events pass for the wall clock without event overhead
-    t0 = time.perf_counter()
-    s.steploop(args.steps, dt)
-    el_noev = time.perf_counter() - t0
-    el = min(el, el_noev)
-    copy_ms, copy_bytes = s.bench_copy(20)
-    s.close()
-    copy_gbs = copy_bytes / (copy_ms * 1e-3) / 1e9
-
+    legs = set(args.legs.split(","))
+    n = args.n
+    r = tgv_leg(n, args.steps, args.warmup)
+    fam = r["fam"]
     pts = float(n) ** 3
-    value = pts * args.steps / el
-    ms_per_step = 1e3 * el / args.steps
+    ms_per_step = r["ms"] / args.steps
+    value = pts / (ms_per_step * 1e-3)
     nodes = float(n + 1) ** 3
     stages = 3 * args.steps
     fam_ms = {nm: float(fam[i]) for i, nm in enumerate(FAMILY_NAMES)}
@@ -508,7 +561,7 @@ def main():
     algo = FAMILY_DOUBLES[dom] * 8.0 * nodes
     achieved = algo / avg_s / 1e9
     out = {
-        "metric": "FP64 grid-point-RK3-steps/s at 256^3 (plain)",
+        "metric": "FP64 grid-point-RK3-steps/s at 256^3 (plain vs batched); adjoint/fwd ratio",
         "value": value,
         "unit": "grid-point-RK3-steps/s",
         "n_gpus": 1,
@@ -519,23 +572,43 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (Taylor-Green initial field)",
+        "data": "synthetic (Taylor-Green initial field; seeded turbulence for the batched and "
+                "adjoint legs)",
         "config": {"workload": f"Taylor-Green vortex {n}^3 periodic, Re 1600, Ma 0.1, 643c, "
-                               f"filter alpha 0.49, RK3 dt 0.004", "n": n,
+                               f"filter alpha 0.49, RK3 dt {r['dt']:g} (acoustic CFL 0.45), "
+                               f"statistics every step", "n": n,
                    "parallelism": "single GPU"},
+        "timing": {"method": "two HIP events on the solver's stream around each run of "
+                             f"{args.steps} steps; value from the median of {TIMED_RUNS} runs",
+                   "runs_ms": r["runs_ms"],
+                   "wall_ms": [1e3 * w for w in r["walls"]]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, n),
                      "kernel": f"{FAMILY_KERNEL[dom]}, {FAMILY_LAUNCHES[dom]} launches/stage",
                      "algo_bytes_per_launch": algo, "avg_launch_ms": avg_s * 1e3},
-        "step_hbm_frac": value * ALGO_BYTES_PER_PT_STEP / (HBM_PEAK_GBS * 1e9),
-        "copy_gbs": copy_gbs,
-        "step_copy_frac": value * ALGO_BYTES_PER_PT_STEP / (copy_gbs * 1e9),
+        # the whole step against the reference's 9-pass byte model (4 224 B
+        # per grid-point-RK3-step) and against the bytes of the kernels that
+        # run here (ALGO_BYTES_PER_PT_STEP)
+        "step_roofline": {
+            "bytes_per_pt_step_reference_passes": REF_BYTES_PER_PT_STEP,
+            "achieved_gbs": value * REF_BYTES_PER_PT_STEP / 1e9,
+            "frac": value * REF_BYTES_PER_PT_STEP / (HBM_PEAK_GBS * 1e9),
+            "bytes_per_pt_step_these_kernels": ALGO_BYTES_PER_PT_STEP,
+            "frac_these_kernels": value * ALGO_BYTES_PER_PT_STEP / (HBM_PEAK_GBS * 1e9),
+            "copy_gbs": r["copy_gbs"],
+            "frac_of_copy": value * REF_BYTES_PER_PT_STEP / (r["copy_gbs"] * 1e9)},
         "family_ms_per_step": {k: v / args.steps for k, v in fam_ms.items()},
+        "state": {"finite": r["finite"], "last_step_stats": r["last_stats"]},
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(n, dt)
-    else:
-        out["cpu_baseline"] = None
+    if "batched" in legs:
+        sizes = tuple(int(v) for v in args.ens_sizes.split(",")) if args.ens_sizes else None
+        out["batched"] = ensemble_leg(args.nbox, args.seed, 10, 1, sizes)
+        out["batched_over_loop"] = out["batched"]["batched_over_loop"]
+    if "adjoint" in legs:
+        out["adjoint"] = adjoint_leg(args.adj_n, args.nsteps, args.ckpt, args.seed, args.fd_dirs)
+        out["adjoint_over_forward"] = out["adjoint"]["gradient_over_forward"]
+        out["adjoint_fd_max_rel_err"] = out["adjoint"]["fd_max_rel_err"]
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(n, r["dt"])
     print(json.dumps(out))
 
 

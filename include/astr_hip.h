@@ -144,7 +144,8 @@ int astr_qmax(astr_ctx *ctx, double *qmax5);
  * astr_set_bc, no filter).  If stats is non-null (three-dimensional boxes)
  * it receives (ke, enstrophy, dissipation) of the stage-1 state and
  * velocity gradients, which rkfirst (src/mainloop.F90:608-688) evaluates
- * after rhscal.  The central path applies the RK3 combine inside the last
+ * after rhscal; they are reduced on the device and copied to the host
+ * after the steps, so computing them does not stall the stream.  The central path applies the RK3 combine inside the last
  * flux sweep of each stage, so qrhs is left holding the x+y part;
  * astr_rhscal gives the whole qrhs.  The step advances the time by dt. */
 int astr_rk3_step(astr_ctx *ctx, double dt, double *stats);
@@ -234,7 +235,11 @@ int astr_adjoint_enstrophy(astr_ctx *ctx, int32_t nsteps, double dt, int32_t ckp
  * fluxes), [6] update (central: primitives; upwind: RK3 combine and
  * primitives), [7] total, [8] shock sensor of astr_rhscal.
  * astr_rk3_step(s) fill 0-7; astr_rhscal of an upwind context fills 2, 5,
- * 7 and 8.  astr_rk3_steps restarts the count at each call. */
+ * 7 and 8.  astr_rk3_steps restarts the count at each call.  Mode 1 puts
+ * events between the families and waits for them at every stage.
+ * astr_set_timing(ctx, 2) instead brackets each astr_rk3_step(s) call with
+ * two events and nothing in between: [7] adds up the device time of those
+ * calls (the other entries stay zero).  0 turns timing off. */
 int astr_set_timing(astr_ctx *ctx, int32_t on);
 int astr_get_timing(astr_ctx *ctx, double *ms9);
 

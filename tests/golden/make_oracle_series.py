@@ -2,10 +2,15 @@
 
 The C restatement (oracle/, pinned by tests/test_oracle.py) advances the
 main-path Taylor-Green box (examples/Taylor_Green_Vortex deck: Re 1600,
-Ma 0.1, 643c, alpha 0.49, rk3) with dt = 0.004 for 100 RK3 steps:
+Ma 0.1, 643c, alpha 0.49, rk3) for 100 RK3 steps with dt = 0.004 * 64 / N,
+the same acoustic CFL number at every size, (|u| + 1/Ma) dt / dx = 0.45:
 
-    python tests/golden/make_oracle_series.py          # 64^3  -> tgv64_main.npz
-    python tests/golden/make_oracle_series.py 256      # 256^3 -> tgv256_main.npz
+    python tests/golden/make_oracle_series.py          # 64^3,  dt 0.004 -> tgv64_main.npz
+    python tests/golden/make_oracle_series.py 256      # 256^3, dt 0.001 -> tgv256_main.npz
+
+dt = 0.004 at 256^3 would be an acoustic CFL number of 1.8, past the RK3
+stability limit of the compact scheme: the run grows and turns to NaN
+within 20 steps.
 
 Output (data only), tgv<N>_main.npz with
   stats   (100, 3)  KE, enstrophy, dissipation at stage 1 of every step
@@ -29,7 +34,12 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import pyoracle  # noqa: E402
 
-DT, STEPS = 0.004, 100
+STEPS = 100
+
+
+def series_dt(n):
+    """dt of the n^3 series: 0.004 at 64^3, scaled with the spacing."""
+    return 0.004 * 64 / n
 NAMES = ["q1", "q2", "q3", "q4", "q5", "rho", "u", "v", "w", "prs", "tmp"]
 
 
@@ -40,15 +50,16 @@ def line_node(n):
 
 
 def main(n=64):
+    dt = series_dt(n)
     b = pyoracle.OracleBox(n, variant=0)
     b.init_tgv()
     stats = np.zeros((STEPS, 3))
     t0 = time.time()
     for s in range(STEPS):
-        stats[s] = b.step(DT, stats=True)
+        stats[s] = b.step(dt, stats=True)
         if s % 10 == 0:
             print(s, stats[s], f"{time.time() - t0:.1f}s", flush=True)
-    out = {"stats": stats, "n": n, "dt": DT, "steps": STEPS}
+    out = {"stats": stats, "n": n, "dt": dt, "steps": STEPS}
     st = n // 8
     i0, j0, k0 = line_node(n)
     for nm in NAMES:

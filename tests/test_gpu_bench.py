@@ -7,6 +7,7 @@ import os
 import subprocess
 import sys
 
+import numpy as np
 import pytest
 
 from conftest import ROOT
@@ -29,7 +30,7 @@ def run_bench(*args):
 
 
 def test_taylor_green_line():
-    d = run_bench("--n", "64", "--steps", "3", "--warmup", "1", "--no-cpu")
+    d = run_bench("--n", "64", "--steps", "3", "--warmup", "1", "--no-cpu", "--legs", "plain")
     assert KEYS <= set(d)
     assert d["n_gpus"] == 1 and d["steps"] == 3 and d["warmup"] == 1
     assert d["dtype"] == "f64" and d["higher_is_better"] is True
@@ -43,6 +44,29 @@ def test_taylor_green_line():
         <= 1e-9 * r["achieved"]
     assert 0.0 < r["frac"] < 1.0
     assert d["cpu_baseline"] is None
+    # the value is the median of the event-timed runs
+    runs = d["timing"]["runs_ms"]
+    assert len(runs) == 5
+    assert abs(d["ms_per_step"] - float(np.median(runs)) / 3) <= 1e-9 * d["ms_per_step"]
+    sr = d["step_roofline"]
+    assert abs(sr["frac"] - d["value"] * 4224.0 / 8e12) <= 1e-12
+    assert d["state"]["finite"]
+    assert "batched" not in d and "adjoint" not in d
+
+
+def test_line_carries_every_part_of_the_metric():
+    """Plain, batched and adjoint parts in one line (small sizes)."""
+    d = run_bench("--n", "32", "--steps", "2", "--warmup", "1", "--no-cpu", "--nbox", "3",
+                  "--ens-sizes", "32,48", "--adj-n", "16", "--nsteps", "4", "--ckpt", "2",
+                  "--fd-dirs", "2")
+    assert KEYS <= set(d)
+    b = d["batched"]
+    assert len(b["boxes"]) == 3 and all(v in (32, 48) for bx in b["boxes"] for v in bx)
+    assert abs(d["batched_over_loop"] - b["value"] / b["loop_value"]) <= 1e-12
+    a = d["adjoint"]
+    assert abs(d["adjoint_over_forward"] -
+               (a["recorded_forward_ms"] + a["reversal_ms"]) / a["forward_ms"]) <= 1e-9
+    assert a["fd_directions"] == 2 and a["fd_max_rel_err"] < 1e-5
 
 
 def test_riemann_line():
