@@ -510,7 +510,7 @@ orc_box *orc_box_new(int im, int jm, int km, int variant, double reynolds,
     b->const7 = (gamma - 1.0) * (mach * mach) * reynolds * prandtl;
     b->roinf = 1.0; b->uinf = 1.0; b->tinf = 1.0;
     b->pinf = b->roinf * b->tinf / b->const2;
-    orc_box_set_sutherland(b, variant == 0 ? 110.3 : 110.4);
+    orc_box_set_sutherland(b, variant == 0 ? 110.3 : 110.4);  /* 2: the mini-app's */
     b->lx = lx; b->ly = ly; b->lz = lz;
     b->dx = lx / (double)im; b->dy = ly / (double)jm; b->dz = lz / (double)km;
 
@@ -1291,11 +1291,20 @@ void orc_box_step(orc_box *b, double dt, double *stats)
             orc_box_updatefvar(b);
         }
     } else {
-        /* rk3, tgvsolver.F90:994-1082 */
+        /* rk3, tgvsolver.F90:994-1082.  Variant 2 keeps this stage order
+         * (bchomo halos, right-hand side, qsave, RK3, filter after the
+         * update) but computes the right-hand side and the RK3 combine as
+         * the main path does: gradcal's metric products, convrsdcal6's and
+         * diffrsdcal6's J-weighted fluxes in metric form, qsave = q J and
+         * the combine divided by J (src/comsolver.F90:244-497,
+         * src/solver.F90:2174-2874, src/mainloop.F90:423-445).  On a
+         * uniform box this is the mini-app's arithmetic up to rounding. */
+        const int mainform = b->variant == 2;
         for (int s = 0; s < 3; ++s) {
             bchomo(b);
             zero_qrhs(b);
-            rhscal_mini(b);
+            if (mainform) orc_box_rhscal(b);
+            else rhscal_mini(b);
             if (s == 0) {
                 if (stats) orc_box_stats(b, stats);
                 for (int m = 0; m < 5; ++m)
@@ -1303,7 +1312,7 @@ void orc_box_step(orc_box *b, double dt, double *stats)
                         for (int j = 0; j <= jm; ++j)
                             for (int i = 0; i <= im; ++i) {
                                 size_t p = IX(b, i, j, k);
-                                b->qsave[m][p] = b->q[m][p];
+                                b->qsave[m][p] = mainform ? b->q[m][p] * b->jacob[p] : b->q[m][p];
                             }
             }
 #pragma omp parallel for schedule(static)
@@ -1312,9 +1321,16 @@ void orc_box_step(orc_box *b, double dt, double *stats)
                     for (int j = 0; j <= jm; ++j)
                         for (int i = 0; i <= im; ++i) {
                             size_t p = IX(b, i, j, k);
-                            b->q[m][p] = rkc[s][0] * b->qsave[m][p] +
-                                         rkc[s][1] * b->q[m][p] +
-                                         rkc[s][2] * b->qrhs[m][p] * dt;
+                            if (mainform) {
+                                double v = rkc[s][0] * b->qsave[m][p] +
+                                           rkc[s][1] * b->q[m][p] * b->jacob[p] +
+                                           rkc[s][2] * b->qrhs[m][p] * dt;
+                                b->q[m][p] = v / b->jacob[p];
+                            } else {
+                                b->q[m][p] = rkc[s][0] * b->qsave[m][p] +
+                                             rkc[s][1] * b->q[m][p] +
+                                             rkc[s][2] * b->qrhs[m][p] * dt;
+                            }
                         }
             orc_box_filterq(b);
             orc_box_updatefvar(b);

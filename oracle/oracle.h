@@ -60,7 +60,12 @@ typedef struct orc_box orc_box;
 /* variant 0: main path (src/mainloop.F90 time_integration_rk with rk3,
  *            src/solver.F90 rhscal with convrsdcal6/diffrsdcal6,
  *            src/comsolver.F90 filterq, src/parallel.F90 qswap).
- * variant 1: mini-app (miniapps/tgv_solver_3d/tgvsolver.F90 rk3). */
+ * variant 1: mini-app (miniapps/tgv_solver_3d/tgvsolver.F90 rk3).
+ * variant 2: the mini-app's stage order and constants with the main path's
+ *            right-hand side (metric products, J-weighted fluxes) and RK3
+ *            combine (qsave = q J, divided by J): the bridge that carries
+ *            the reference-held mini-app series over to the main path's
+ *            arithmetic. */
 orc_box *orc_box_new(int im, int jm, int km, int variant,
                      double reynolds, double mach, double prandtl,
                      double gamma, double ref_tem, double alfa_filter,

@@ -162,7 +162,9 @@ FIELDS = {"q1": 0, "q2": 1, "q3": 2, "q4": 3, "q5": 4, "rho": 5, "u": 6, "v": 7,
 
 
 class OracleBox:
-    """Periodic box of the C restatement; variant 0 = main path, 1 = mini-app."""
+    """Periodic box of the C restatement; variant 0 = main path, 1 = mini-app,
+    2 = the mini-app's stage order with the main path's metric form and J
+    factors."""
 
     def __init__(self, n, variant=0, reynolds=1600.0, mach=0.1, prandtl=0.72, gamma=1.4,
                  ref_tem=273.15, alfa=0.49, lengths=(2 * math.pi,) * 3):

@@ -178,6 +178,34 @@ def test_miniapp_variant_reproduces_state_ref_128_all_steps():
         assert ke == ref[s, 2] and en == ref[s, 3], (s, ke, en)
 
 
+def test_bridge_variant_reproduces_state_ref_128_first_steps():
+    """The main path's right-hand side and RK3 combine (gradcal's metric
+    products, the J-weighted convective and viscous fluxes in metric form,
+    qsave = q J and the combine divided by J) driven in the mini-app's
+    stage order: the reference-held series within 1e-12 relative.  A
+    dropped J, a wrong metric component or a missing pressure term in the
+    main path's flux assembly moves the series by far more."""
+    ref = np.loadtxt(os.path.join(GOLDEN, "state.ref_128.txt"))
+    b = pyoracle.OracleBox(128, variant=2)
+    b.init_tgv()
+    for s in range(3):
+        ke, en, _ = b.step(1e-3, stats=True)
+        assert abs(ke - ref[s, 2]) <= 1e-12 * ref[s, 2], (s, ke, ref[s, 2])
+        assert abs(en - ref[s, 3]) <= 1e-12 * ref[s, 3], (s, en, ref[s, 3])
+    b.close()
+
+
+@pytest.mark.slow
+def test_bridge_variant_reproduces_state_ref_128_all_steps():
+    ref = np.loadtxt(os.path.join(GOLDEN, "state.ref_128.txt"))
+    b = pyoracle.OracleBox(128, variant=2)
+    b.init_tgv()
+    for s in range(ref.shape[0]):
+        ke, en, _ = b.step(1e-3, stats=True)
+        assert abs(ke - ref[s, 2]) <= 1e-12 * ref[s, 2], (s, ke, ref[s, 2])
+        assert abs(en - ref[s, 3]) <= 1e-12 * ref[s, 3], (s, en, ref[s, 3])
+
+
 def test_main_variant_initial_statistics_closed_form():
     # KE0 = 1/8, enstrophy0 = 3/8 for the Taylor-Green field (the first row of
     # state.ref_128 holds the same values); dissipation0 = 2 nu * 3/8 up to
@@ -191,6 +219,44 @@ def test_main_variant_initial_statistics_closed_form():
     assert abs(ke - 0.125) < 1e-9
     assert abs(en - 0.375) / 0.375 < 2e-3
     assert abs(di - 2 * 0.375 / 1600) / (2 * 0.375 / 1600) < 2e-2
+
+
+def test_statistics_closed_forms_on_a_compressible_field():
+    """kenergycal, enstophycal and diss_rate_cal (src/statistic.F90:870-1042)
+    on u = a sin x + d sin y, v = b sin y, w = c sin z, rho = T = 1, a field
+    with div u = a cos x + b cos y + c cos z != 0 and one shear component.
+    With mu = 1/Re at T = 1 the means over the periodic nodes are
+      KE = (a^2 + b^2 + c^2 + d^2) / 4,   enstrophy = d^2 / 4,
+      dissipation = 2 mu <S:S - (div u)^2 / 3> = (2/3) mu (a^2+b^2+c^2) + mu d^2 / 2,
+    since <S:S> = (a^2+b^2+c^2)/2 + d^2/4 and <(div u)^2> = (a^2+b^2+c^2)/2.
+    Dropping the dilatation term gives mu (a^2+b^2+c^2) + mu d^2/2, off by
+    a fifth; the gradients are 6th-order compact derivatives of single
+    modes at 64 points (error ~1e-9), so the bar is 1e-8."""
+    n, a, bb, c, d = 64, 0.3, -0.2, 0.25, 0.4
+    box = pyoracle.OracleBox(n, variant=0)
+    x = 2 * np.pi / n * np.arange(n + 1)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    u = a * np.sin(X) + d * np.sin(Y)
+    v = bb * np.sin(Y)
+    w = c * np.sin(Z)
+    gamma, mach, re = 1.4, 0.1, 1600.0
+    const1 = 1.0 / (gamma * (gamma - 1.0) * mach * mach)
+    one = np.ones_like(u)
+    for nm, f in (("q1", one), ("q2", u), ("q3", v), ("q4", w),
+                  ("q5", const1 + 0.5 * (u * u + v * v + w * w)), ("rho", one), ("u", u),
+                  ("v", v), ("w", w), ("prs", one / (gamma * mach * mach)), ("tmp", one)):
+        box.set(nm, f)
+    box.qswap()
+    box.rhscal()
+    ke, en, di = box.stats()
+    box.close()
+    mu = 1.0 / re
+    s2 = a * a + bb * bb + c * c
+    assert abs(ke - (s2 + d * d) / 4) <= 1e-13 * ke
+    assert abs(en - d * d / 4) <= 1e-8 * en, en
+    want = 2.0 / 3.0 * mu * s2 + 0.5 * mu * d * d
+    assert abs(di - want) <= 1e-8 * want, (di, want)
+    assert abs(di - (mu * s2 + 0.5 * mu * d * d)) > 0.1 * want
 
 
 def test_main_variant_free_stream_preservation():

@@ -10,7 +10,9 @@ that a dropped term, a wrong sign or a shifted stencil fails:
     (and failure one degree above), MP = linear scheme on smooth data,
     no new extrema at a step, WENO weights -> linear weights on smooth data;
   * Steger-Warming: F+ + F- = F, pure upwinding for |M| >= 1, the sign of
-    the split eigenvalues;
+    the split eigenvalues, and F+ and F- each against the textbook split;
+  * ROUND: its normalised-variable formula at many points, the NVD corners,
+    the linear-scheme limit and affine invariance;
   * chardecomp: LEV REV = I and REV diagonalises the Roe-averaged flux
     Jacobian along the normal;
   * convrsduwd: free-stream preservation, telescoping conservation on a
@@ -99,10 +101,66 @@ def test_weno_tends_to_linear_scheme_on_smooth_data(name, n, lin):
     assert abs(po.recon(name, np.full(n, 2.5)) - 2.5) < 1e-14
 
 
+def round_nvd(z):
+    """ROUND's face value in the normalised-variable diagram (Deng, "A
+    unified framework for non-linear reconstruction schemes in a compact
+    stencil", J. Comput. Phys. 2023), as a function of the normalised
+    upwind value z = (u_i - u_{i-1}) / (u_{i+1} - u_{i-1}):
+      g(z) = [p1 (1 - w1) + p2 w1] (1 - w2) + p3 w2,
+    p1 the third-order linear scheme 5z/6 + 1/3 with the two
+    boundedness corrections max(0, 1100 (z-0.05)^3 (0.47-z)^3) and
+    max(0, 18000 (z-0.55)^3 (0.97-z)^5), p2 = 3z/2 near z = 0, p3 = (z+1)/2
+    near z = 1, weights w1 = (1 + 12 z^2)^-4, w2 = (1 + 5 (z-1)^2)^-8; the
+    face value is u_{i-1} + g(z) (u_{i+1} - u_{i-1})."""
+    bump = (max(0.0, 1100.0 * (z - 0.05) ** 3 * (0.47 - z) ** 3)
+            + max(0.0, 18000.0 * (z - 0.55) ** 3 * (0.97 - z) ** 5))
+    p1 = 5.0 / 6.0 * z + 1.0 / 3.0 + bump
+    w1 = (1.0 + 12.0 * z * z) ** -4
+    w2 = (1.0 + 5.0 * (z - 1.0) ** 2) ** -8
+    return (p1 * (1.0 - w1) + 1.5 * z * w1) * (1.0 - w2) + (0.5 * z + 0.5) * w2
+
+
 def test_round_preserves_constants_and_linear_data():
-    assert abs(po.recon("round", np.array([1.0, 2.0, 3.0])) - 2.5) < 1e-3
+    # on a linear profile (z = 1/2) ROUND is the third-order linear scheme:
+    # -u0/6 + 5 u1/6 + u2/3
+    for u in ([1.0, 2.0, 3.0], [-4.0, -1.0, 2.0], [7.5, 7.25, 7.0]):
+        u = np.array(u)
+        lin = -u[0] / 6 + 5 * u[1] / 6 + u[2] / 3
+        assert abs(po.recon("round", u) - lin) <= 1e-14 * max(1.0, abs(lin)), u
+    assert po.recon("round", np.full(3, 2.5)) == 2.5
     v = po.recon("round", np.array([0.0, 0.0, 1.0]))
-    assert 0.0 <= v <= 1.0
+    assert 0.0 <= v <= 1e-6
+
+
+def test_round_against_its_normalised_variable_formula():
+    """At 2000 random triples (z from -3 to 4 and across [0, 1] densely):
+    the face value is u_{i-1} + g(z) (u_{i+1} - u_{i-1}) with g the NVD
+    formula; g passes through the NVD corners (0, 0) and (1, 1) and the
+    linear-scheme point (1/2, 3/4); g(z) >= z on [0, 1] (never more
+    diffusive than first-order upwind there); and the face value moves with
+    any affine change u -> a u + b of the data (a > 0)."""
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for k in range(2000):
+        z = rng.uniform(-3.0, 4.0) if k % 2 else rng.uniform(0.0, 1.0)
+        u0 = rng.standard_normal()
+        du = rng.choice([-1.0, 1.0]) * (0.1 + rng.random())
+        u = np.array([u0, u0 + z * du, u0 + du])
+        got = po.recon("round", u)
+        zz = (u[1] - u[0] + 1e-16) / (u[2] - u[0] + 1e-16)
+        want = round_nvd(zz) * (u[2] - u[0]) + u[0]
+        worst = max(worst, abs(got - want) / (abs(du) + abs(u0)))
+        a, b = 0.5 + 3.0 * rng.random(), rng.standard_normal()
+        ga = po.recon("round", a * u + b)
+        assert abs(ga - (a * got + b)) <= 1e-12 * (abs(a * got) + abs(b) + a * abs(du))
+    assert worst <= 1e-14, worst
+    assert round_nvd(1.0) == 1.0 and abs(round_nvd(0.0)) < 1e-6
+    assert abs(round_nvd(0.5) - 0.75) < 1e-15
+    zs = np.linspace(0.0, 1.0, 2001)
+    assert all(round_nvd(z) >= z - 1e-15 for z in zs)
+    # slope of the linear scheme at z = 1/2 (third order there)
+    h = 1e-4
+    assert abs((round_nvd(0.5 + h) - round_nvd(0.5 - h)) / (2 * h) - 5.0 / 6.0) < 5e-3
 
 
 def test_recons_exp_boundary_degradation():
@@ -144,6 +202,66 @@ def test_steger_warming_split_sums_to_the_flux(seed):
             assert np.all(fp == 0.0)
         # split mass fluxes carry the sign of their eigenvalues
         assert fp[0] >= -1e-15 and fm[0] <= 1e-15
+
+
+def textbook_sw(rho, vel, p, c, dxi, J, gamma, eps=0.04):
+    """The Steger-Warming split of the flux along grad(xi) (Steger & Warming,
+    J. Comput. Phys. 40 (1981); Toro, Riemann Solvers, sec. 8.4), written
+    in contravariant form: eigenvalues U, U, U, U + c|grad xi|, U -
+    c|grad xi| with U = grad(xi).u, each split as (l +- sqrt(l^2 + eps^2))/2
+    (the smoothed split with eps = 0.04 that src/riemann.F90:23-161 uses),
+    unit normal n = grad(xi)/|grad xi|, un = n.u, a the speed of sound:
+      F+- = J rho/(2 gamma) [ al,
+                              al u + a n (l3 - l1),
+                              al |u|^2/2 + a un (l3 - l1) + a^2 (l1 + l3)/(gamma - 1) ]
+    with al = 2 (gamma - 1) l2 + l1 + l3, l1 = U - c|grad xi|, l2 = U,
+    l3 = U + c|grad xi| (their split parts)."""
+    g = np.linalg.norm(dxi)
+    n = dxi / g
+    U = dxi @ vel
+    un = n @ vel
+    lam = np.array([U - c * g, U, U + c * g])
+    out = []
+    for sgn in (1.0, -1.0):
+        ls = 0.5 * (lam + sgn * np.sqrt(lam * lam + eps * eps))
+        l1, l2, l3 = ls
+        al = 2.0 * (gamma - 1.0) * l2 + l1 + l3
+        f = np.empty(5)
+        f[0] = al
+        f[1:4] = al * vel + c * n * (l3 - l1)
+        f[4] = al * 0.5 * (vel @ vel) + c * un * (l3 - l1) + c * c * (l1 + l3) / (gamma - 1.0)
+        out.append(J * rho / (2.0 * gamma) * f)
+    return out
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_steger_warming_halves_match_the_textbook_split(seed):
+    """F+ and F- each (not only their sum) against the textbook split of the
+    flux along grad(xi) with the eps-smoothed eigenvalues, at subsonic normal
+    Mach numbers of either sign; above |M| = 1 the whole flux goes upwind."""
+    rng = np.random.default_rng(100 + seed)
+    gamma, mach = 1.4, 1.0
+    rho = 0.5 + rng.random()
+    T = 0.5 + rng.random()
+    p = rho * T / (gamma * mach ** 2)
+    c = math.sqrt(T) / mach
+    dxi = rng.standard_normal(3) * (0.5 + 2.0 * rng.random())
+    nrm = np.linalg.norm(dxi)
+    n = dxi / nrm
+    for mn in (-0.95, -0.5, -0.01, 0.0, 0.02, 0.4, 0.9):
+        vel = mn * c * n + 0.4 * np.cross(n, rng.standard_normal(3))
+        E = p / (gamma - 1) + 0.5 * rho * vel @ vel
+        q = np.array([rho, *(rho * vel), E])
+        J = 0.3 + rng.random()
+        fp, fm = po.sw_split(rho, vel, p, T, q, dxi, J, gamma, mach)
+        tp, tm = textbook_sw(rho, vel, p, c, dxi, J, gamma)
+        scale = max(np.abs(tp).max(), np.abs(tm).max())
+        assert np.allclose(fp, tp, rtol=0, atol=1e-13 * scale), (mn, fp, tp)
+        assert np.allclose(fm, tm, rtol=0, atol=1e-13 * scale), (mn, fm, tm)
+        # a plausible slip (the acoustic eigenvalues swapped, or the
+        # smoothing dropped) is far outside that bound
+        wrong, _ = textbook_sw(rho, vel, p, c, dxi, J, gamma, eps=0.0)
+        assert np.abs(wrong - tp).max() > 1e-6 * scale
 
 
 def roe_jacobian(R_state, n, gamma):
