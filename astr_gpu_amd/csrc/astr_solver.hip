@@ -970,6 +970,77 @@ __global__ void k_qswap2d_plane(PtArgs a)
     a.prim[4][gi + a.g.s[2]] = p;
 }
 
+// s + e = a + b exactly (Knuth's TwoSum; plain IEEE additions)
+__host__ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
+{
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// (ah, al) += (bh, bl) in double-double arithmetic
+__host__ __device__ __forceinline__ void dd_add(double& ah, double& al, double bh, double bl)
+{
+    double s, e;
+    two_sum(ah, bh, s, e);
+    e += al + bl;
+    ah = s + e;
+    al = e - (ah - s);
+}
+
+// The per-node terms of kenergycal / enstophycal / diss_rate_cal
+// (src/statistic.F90:870-1042): rho |u|^2, rho |omega|^2 and
+// 2 mu (S:S - (div u)^2 / 3)
+__device__ __forceinline__ void stats_terms(const PtArgs& a, long gi, double& ke, double& en,
+                                            double& di)
+{
+    const double rho = a.prim[0][gi];
+    const double u = a.prim[1][gi], v = a.prim[2][gi], w = a.prim[3][gi];
+    ke = rho * (u * u + v * v + w * w);
+    const double d11 = a.dvel[0][gi], d12 = a.dvel[1][gi], d13 = a.dvel[2][gi];
+    const double d21 = a.dvel[3][gi], d22 = a.dvel[4][gi], d23 = a.dvel[5][gi];
+    const double d31 = a.dvel[6][gi], d32 = a.dvel[7][gi], d33 = a.dvel[8][gi];
+    const double o1 = d32 - d23, o2 = d13 - d31, o3 = d21 - d12;
+    en = rho * (o1 * o1 + o2 * o2 + o3 * o3);
+    const double miu = miucal(a, a.prim[5][gi]) / a.reynolds;
+    const double s12 = 0.5 * (d12 + d21), s13 = 0.5 * (d13 + d31), s23 = 0.5 * (d23 + d32);
+    const double div = d11 + d22 + d33;
+    di = 2.0 * miu * (d11 * d11 + d22 * d22 + d33 * d33 +
+                      2.0 * (s12 * s12 + s13 * s13 + s23 * s23) - (1.0 / 3.0) * (div * div));
+}
+
+// The block's double-double sums of (ke, en, di) into part[6 * blockIdx.x
+// ...]: a butterfly over each wavefront's lanes, then the wavefronts' sums
+// in LDS.  Every thread of the block takes part.
+__device__ __forceinline__ void stats_block_sum(double ke, double en, double di, double* part)
+{
+    double h[3] = {ke, en, di}, l[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double oh = __shfl_xor(h[c], off, 64), ol = __shfl_xor(l[c], off, 64);
+            dd_add(h[c], l[c], oh, ol);
+        }
+    __shared__ double ws[4][6];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 3; ++c) {
+            ws[wv][2 * c] = h[c];
+            ws[wv][2 * c + 1] = l[c];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (int)(blockDim.x + 63) >> 6;
+        for (int c = 0; c < 3; ++c) {
+            double hh = ws[0][2 * c], ll = ws[0][2 * c + 1];
+            for (int v = 1; v < nw; ++v) dd_add(hh, ll, ws[v][2 * c], ws[v][2 * c + 1]);
+            part[6 * blockIdx.x + 2 * c] = hh;
+            part[6 * blockIdx.x + 2 * c + 1] = ll;
+        }
+    }
+}
+
 // The inputs of G at one node, loaded once: every output store could alias
 // an input array, so loads interleaved with the stores would be repeated.
 struct NodeIn {
@@ -1109,7 +1180,9 @@ __device__ __forceinline__ int halo_of_image(int c, int n)
 
 // k_gflux at cell i of the interior row (j, k); hj, hk are the y and z
 // halo rows of which this row is the periodic image (or NO_HALO).
-__device__ __forceinline__ void gflux_cell(const PtArgs& a, int i, int j, int k, int hj, int hk)
+template <bool STATS>
+__device__ __forceinline__ void gflux_cell(const PtArgs& a, int i, int j, int k, int hj, int hk,
+                                           double (&st)[3])
 {
     const int n0 = a.g.n[0];
     const int c[3] = {i, j, k};
@@ -1118,6 +1191,10 @@ __device__ __forceinline__ void gflux_cell(const PtArgs& a, int i, int j, int k,
         const NodeIn nd = load_node(a, gi);
         double sg[6] = {0, 0, 0, 0, 0, 0}, qf[3] = {0, 0, 0};
         if (a.visc) sigma_of(a, gi, nd.u, sg, qf);
+        // rkfirst's statistics of this state (nodes 1..n), from the inputs
+        // already loaded (the stored primitive rho aside), before any store
+        if (STATS && i >= 1 && j >= 1 && k >= 1 && j <= a.g.n[1] && k <= a.g.n[2])
+            stats_terms(a, gi, st[0], st[1], st[2]);
         double out[15];
 #pragma unroll
         for (int e = 0; e < 3; ++e) gflux_of(a, e, a.xi[e][c[e] + HM], nd, sg, qf, out + 5 * e);
@@ -1160,6 +1237,12 @@ __device__ __forceinline__ void gflux_cell(const PtArgs& a, int i, int j, int k,
 // threads of cells that no flux sweep reads (row padding past the strips,
 // i past n in halo rows) store zeros, so each field's lines are written
 // whole.
+//
+// STATS (stage 1 of a step): the workgroup also sums the statistics terms
+// of its nodes in 1..n into a.part (double-double, 6 per workgroup), which
+// k_stats_fin reduces; the inputs are those of G, so the statistics cost
+// one more load per node (the primitive rho) instead of a pass of their own.
+template <bool STATS>
 __global__ __launch_bounds__(256) void k_gflux(PtArgs a)
 {
     const int iend = line_end_i(a.g.n[0] + HM);
@@ -1167,11 +1250,14 @@ __global__ __launch_bounds__(256) void k_gflux(PtArgs a)
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned row = t / rl;
     const unsigned nj = (unsigned)a.g.n[1] + 1;
-    if (row >= nj * (unsigned)(a.g.n[2] + 1)) return;
-    const unsigned kk = row / nj;
-    const int j = (int)(row - kk * nj), k = (int)kk;
-    gflux_cell(a, (int)(t - row * rl) - 16, j, k, halo_of_image(j, a.g.n[1]),
-               halo_of_image(k, a.g.n[2]));
+    double st[3] = {0.0, 0.0, 0.0};
+    if (row < nj * (unsigned)(a.g.n[2] + 1)) {
+        const unsigned kk = row / nj;
+        const int j = (int)(row - kk * nj), k = (int)kk;
+        gflux_cell<STATS>(a, (int)(t - row * rl) - 16, j, k, halo_of_image(j, a.g.n[1]),
+                          halo_of_image(k, a.g.n[2]), st);
+    }
+    if (STATS) stats_block_sum(st[0], st[1], st[2], a.part);
 }
 
 // bchomo of the mini-app (tgvsolver.F90:271-369): the primitives of the
@@ -1264,24 +1350,6 @@ __global__ __launch_bounds__(256) void k_prim(PtArgs a)
     }
 }
 
-// s + e = a + b exactly (Knuth's TwoSum; plain IEEE additions)
-__host__ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
-{
-    s = a + b;
-    const double bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-
-// (ah, al) += (bh, bl) in double-double arithmetic
-__host__ __device__ __forceinline__ void dd_add(double& ah, double& al, double bh, double bl)
-{
-    double s, e;
-    two_sum(ah, bh, s, e);
-    e += al + bl;
-    ah = s + e;
-    al = e - (ah - s);
-}
-
 // Partial sums of kenergycal / enstophycal / diss_rate_cal over 1..n
 // (src/statistic.F90:870-1042), one double-double (hi, lo) pair per
 // statistic and workgroup: the sums are then exact to about 1e-30 relative
@@ -1300,20 +1368,7 @@ __global__ __launch_bounds__(256) void k_stats(PtArgs a)
         const long r = p / n0;
         const int j = 1 + (int)(r % n1);
         const int k = 1 + (int)(r / n1);
-        const long gi = gidx(a.g, i, j, k);
-        const double rho = a.prim[0][gi];
-        const double u = a.prim[1][gi], v = a.prim[2][gi], w = a.prim[3][gi];
-        ke = rho * (u * u + v * v + w * w);
-        const double d11 = a.dvel[0][gi], d12 = a.dvel[1][gi], d13 = a.dvel[2][gi];
-        const double d21 = a.dvel[3][gi], d22 = a.dvel[4][gi], d23 = a.dvel[5][gi];
-        const double d31 = a.dvel[6][gi], d32 = a.dvel[7][gi], d33 = a.dvel[8][gi];
-        const double o1 = d32 - d23, o2 = d13 - d31, o3 = d21 - d12;
-        en = rho * (o1 * o1 + o2 * o2 + o3 * o3);
-        const double miu = miucal(a, a.prim[5][gi]) / a.reynolds;
-        const double s12 = 0.5 * (d12 + d21), s13 = 0.5 * (d13 + d31), s23 = 0.5 * (d23 + d32);
-        const double div = d11 + d22 + d33;
-        di = 2.0 * miu * (d11 * d11 + d22 * d22 + d33 * d33 +
-                          2.0 * (s12 * s12 + s13 * s13 + s23 * s23) - (1.0 / 3.0) * (div * div));
+        stats_terms(a, gidx(a.g, i, j, k), ke, en, di);
     }
     sh[0][threadIdx.x] = ke;
     sh[1][threadIdx.x] = en;
@@ -3073,13 +3128,30 @@ int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt, double* const* qo = nul
 
 // sigma/qflux and the flux G = Fv - F of each axis (interior and the halo
 // strips the flux sweeps read)
-int do_gflux(astr_ctx* c)
+// workgroups of k_gflux: one thread per cell of the interior (j, k) rows, i
+// from -16 to line_end_i(n + HM)
+long gflux_blocks(const Geo& g)
 {
-    // one thread per cell of the interior (j, k) rows, i from -16 to line_end_i(n + HM)
-    const long rows = (long)(c->g.n[1] + 1) * (c->g.n[2] + 1);
-    const long n = rows * (line_end_i(c->g.n[0] + HM) + 16);
-    hipLaunchKernelGGL(k_gflux, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       pt_args(c));
+    const long rows = (long)(g.n[1] + 1) * (g.n[2] + 1);
+    return (rows * (line_end_i(g.n[0] + HM) + 16) + 255) / 256;
+}
+
+// stats_slot (nullable, device memory): also the statistics of the state
+// (rkfirst), summed by the same workgroups and reduced into the slot
+int do_gflux(astr_ctx* c, double* stats_slot = nullptr)
+{
+    const long nb = gflux_blocks(c->g);
+    if (stats_slot) {
+        hipLaunchKernelGGL(k_gflux<true>, dim3((unsigned)nb), dim3(256), 0, c->stream,
+                           pt_args(c));
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_stats_fin, dim3(1), dim3(256), 0, c->stream, (const double*)c->part,
+                           nb, (double)c->g.n[0] * (double)c->g.n[1] * (double)c->g.n[2],
+                           stats_slot);
+    } else {
+        hipLaunchKernelGGL(k_gflux<false>, dim3((unsigned)nb), dim3(256), 0, c->stream,
+                           pt_args(c));
+    }
     HIPCHK(hipGetLastError());
     return ASTR_OK;
 }
@@ -3390,9 +3462,9 @@ int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed)
             if (timed) TRY(time_mark(c, 2));
             TRY(do_grad(c));
             if (timed) TRY(time_mark(c, 3));
-            TRY(do_gflux(c));
+            // rkfirst's statistics of the stage-1 state, summed by k_gflux
+            TRY(do_gflux(c, s == 1 ? stats_slot : nullptr));
             if (timed) TRY(time_mark(c, 4));
-            if (s == 1 && stats_slot) TRY(launch_stats(c, stats_slot));
             if (timed) TRY(time_mark(c, 5));
             TRY(do_flux_sweeps(c, s, dt));
             if (timed) TRY(time_mark(c, 6));
@@ -3423,11 +3495,12 @@ int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed)
         if (timed) TRY(time_mark(c, 2));
         TRY(do_grad(c));
         if (timed) TRY(time_mark(c, 3));
-        if (!upwind) TRY(do_gflux(c));
+        // rkfirst's statcal sees the stage-1 state and its gradients; the
+        // central path sums it inside k_gflux, which loads the same inputs
+        if (!upwind) TRY(do_gflux(c, s == 1 ? stats_slot : nullptr));
         else if (sensor_on(c)) TRY(do_sensor(c));
         if (timed) TRY(time_mark(c, 4));
-        // rkfirst's statcal sees the stage-1 state and its gradients
-        if (s == 1 && stats_slot) TRY(launch_stats(c, stats_slot));
+        if (upwind && s == 1 && stats_slot) TRY(launch_stats(c, stats_slot));
         if (timed) TRY(time_mark(c, 5));
         if (upwind) {
             TRY(do_upwind(c));
@@ -3962,7 +4035,10 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
         return cleanup(fail(ASTR_E_NOMEM, "metric allocation failed"));
     (void)hipMemset(c->xi, 0, sizeof(double) * 3 * c->xi_stride);
 
+    // partial sums of k_stats (one workgroup per 256 nodes) or of k_gflux's
+    // statistics (one per workgroup)
     c->npart = ((long)dims[0] * dims[1] * std::max(dims[2], 1) + 255) / 256;
+    if (dims[2] > 0) c->npart = std::max(c->npart, gflux_blocks(c->g));
     c->host_part.resize(6 * c->npart);
     if (hipMalloc(&c->part, sizeof(double) * 6 * c->npart) != hipSuccess)
         return cleanup(fail(ASTR_E_NOMEM, "partials allocation failed"));

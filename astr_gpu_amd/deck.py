@@ -17,9 +17,17 @@ Built flow types: ``tgv`` (tgvini), ``hit`` (the seeded turbulence
 generator of this package, same spectrum as the reference's) and
 ``riem2d`` (riem2dini with the example's user boundary condition).  The
 deck must be nondimensional with a generated grid (lreadgrid f), no
-restart, no species, no turbulence model and no immersed boundary; the
-other cases raise ``DeckError``.  Checkpoints and flow-field files
-(HDF5 in the reference) are not written.
+species, no turbulence model and no immersed boundary; the other cases
+raise ``DeckError``.
+
+Checkpoints: every feqchkpt steps (at the start of a step nstep with
+mod(nstep, feqchkpt) = 0, after the first, as rkfirst calls writechkpt,
+src/mainloop.F90:668-673) the state goes to outdat/flowfield.npz and
+outdat/auxiliary.txt (astr_gpu_amd.checkpoint; npz in place of the
+reference's HDF5).  A deck with lrestart t reads them back
+(readcheckpoint, src/initialisation.F90:46-50) and continues from that
+nstep; flowstate.dat is resumed as listinit does (src/utility.F90:310-362):
+the lines before that nstep are kept and the run writes on from there.
 """
 from __future__ import annotations
 
@@ -174,7 +182,6 @@ def params_from_deck(d):
     need(d["flowtype"] in FLOWTYPES, f"flow type {d['flowtype']!r} is not built ({FLOWTYPES})")
     need(d["nondimen"], "dimensional decks are not built")
     need(not d["lreadgrid"], "grids read from a file are not built (lreadgrid f)")
-    need(not d["lrestart"], "restarts read HDF5 checkpoints, which are not built")
     need(not d["limmbou"], "immersed boundaries are not built")
     need(d["num_species"] == 0, "species are not built")
     need(d["turbmode"] == "none", "turbulence models are not built")
@@ -250,10 +257,28 @@ def initial_state(d, p, seed=1):
     return ag.hit_state(p, seed)
 
 
+def resumed_lines(path, nstep):
+    """listinit with lrestart (src/utility.F90:310-362): the header and the
+    lines of flowstate.dat before the first one whose nstep is >= nstep
+    (None if the file does not exist: a new file with its header)."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        lines = f.readlines()
+    kept = lines[:1]
+    for ln in lines[1:]:
+        it = _items(ln)
+        if not it or int(it[0]) >= nstep:
+            break
+        kept.append(ln)
+    return kept
+
+
 def run_deck(casedir, out=None, maxstep=None, seed=1, device=0, log=sys.stdout, input_name=None):
     """Run a case directory; returns the path of the flowstate.dat written
     (in out, default the case directory)."""
     import astr_gpu_amd as ag
+    from astr_gpu_amd.checkpoint import read_checkpoint, write_checkpoint
 
     d = read_input(find_input(casedir, input_name))
     c = read_controller(os.path.join(casedir, "datin", "controller"))
@@ -262,21 +287,32 @@ def run_deck(casedir, out=None, maxstep=None, seed=1, device=0, log=sys.stdout, 
     p = params_from_deck(d)
     ft = d["flowtype"]
     s = ag.AstrSolver(p, device=device)
+    outdat = os.path.join(out or casedir, "outdat")
     try:
         s.upload_geom(*ag.box_metrics(p, lengths=BOX_LENGTHS[ft]))
-        st = initial_state(d, p, seed)
-        s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
         if ft == "riem2d":
             s.set_bc(tuple(d["bctype"]), ag.UDF_RIEMANN2D)
+        nstep0 = 0
+        if d["lrestart"]:
+            nstep0 = read_checkpoint(s, outdat)
+        else:
+            st = initial_state(d, p, seed)
+            s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
             s.time = 0.0
         names = ["nstep", "time"] + (["kenergy", "enstophy", "dissipation"]
                                      if ft in ("tgv", "hit") else
                                      ["maxq1", "maxq2", "maxq3", "maxq4", "maxq5"])
         path = os.path.join(out or casedir, "flowstate.dat")
         dt, feqlist = c["deltat"], max(1, c["feqlist"])
+        kept = resumed_lines(path, nstep0) if d["lrestart"] else None
         with open(path, "w") as fs:
-            fs.write(flowstate_header(names) + "\n")
-            for nstep in range(0, c["maxstep"] + 1):
+            if kept is None:
+                fs.write(flowstate_header(names) + "\n")
+            else:
+                fs.writelines(kept)
+            for nstep in range(nstep0, c["maxstep"] + 1):
+                if nstep != nstep0 and c["feqchkpt"] > 0 and nstep % c["feqchkpt"] == 0:
+                    write_checkpoint(s, outdat, nstep)
                 t = s.time
                 if ft in ("tgv", "hit"):
                     stats = s.time_integration_rk(dt, stats=True)
@@ -289,7 +325,7 @@ def run_deck(casedir, out=None, maxstep=None, seed=1, device=0, log=sys.stdout, 
                     qmx = s.qmax()
                     stats = None
                     s.time_integration_rk(dt)
-                if nstep == 0:
+                if nstep == nstep0:
                     continue  # rkfirst writes nothing on the first call
                 vals = stats if qmx is None else qmx
                 fs.write(flowstate_line(nstep, t, vals) + "\n")
