@@ -75,6 +75,9 @@ SIGNATURES = {
     "astr_batch_destroy": (ctypes.c_int, [_P]),
     "astr_batch_rk3_steps": (ctypes.c_int, [_P, _I, ctypes.c_double]),
     "astr_batch_set_mode": (ctypes.c_int, [_P, _I]),
+    "astr_batch_autotune": (ctypes.c_int, [_P, ctypes.c_double]),
+    "astr_batch_get_tuning": (ctypes.c_int, [_P, _D, _I, ctypes.POINTER(ctypes.c_int32)]),
+    "astr_batch_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
     "astr_bench_copy": (ctypes.c_int, [_P, _I, _D, ctypes.POINTER(ctypes.c_int64)]),
     "astr_get_timing": (ctypes.c_int, [_P, _D]),
     "astr_last_error": (ctypes.c_char_p, []),

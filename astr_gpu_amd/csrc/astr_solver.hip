@@ -92,10 +92,10 @@ __host__ __device__ constexpr int line_end_i(int n) { return (n / 16 + 1) * 16; 
 // and taken one node per thread: a row is a whole number of 128-byte lines,
 // so a wavefront covers whole lines only, also where it crosses from one
 // row to the next.  False past the last row.
-__device__ __forceinline__ bool row_node(const Geo& g, int& i, int& j, int& k)
+__device__ __forceinline__ bool row_node(const Geo& g, unsigned bid, int& i, int& j, int& k)
 {
     const unsigned le = (unsigned)line_end_i(g.n[0]);
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned t = bid * blockDim.x + threadIdx.x;
     const unsigned row = t / le;
     const unsigned nj = (unsigned)g.n[1] + 1;
     if (row >= nj * ((unsigned)g.n[2] + 1)) return false;
@@ -103,6 +103,10 @@ __device__ __forceinline__ bool row_node(const Geo& g, int& i, int& j, int& k)
     k = (int)(row / nj);
     j = (int)(row - (unsigned)k * nj);
     return true;
+}
+__device__ __forceinline__ bool row_node(const Geo& g, int& i, int& j, int& k)
+{
+    return row_node(g, blockIdx.x, i, j, k);
 }
 
 // store v back to f[gi] (a cell past node n that keeps its value, written so
@@ -301,8 +305,10 @@ __device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const do
     }
 }
 
+// One tile of lines and one quantity m: stage, solve, write back.  Called
+// by every thread of the workgroup with the same (tile_id, m).
 template <int L, int PASS, bool XD>
-__global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
+__device__ __forceinline__ void sweep_tile(const SweepArgs& a, int tile_id, int m)
 {
     // One LDS tile; line l stages node n-HM at tile[l*rs_in + TG + n].  The TG
     // guard slots and the tail padding keep every window read and every
@@ -310,7 +316,6 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     // The solution of a line overwrites that line's slots in place, which is
     // safe because only the half-wavefront solving a line reads it.
     extern __shared__ double tile[];
-    using C = PassCfg<PASS>;
     constexpr int OP = (PASS == PASS_FILTER || PASS == PASS_FILTER_FIELD) ? OP_FILTER : OP_DERIV;
     // XD: x-lines (dir 0), else y- or z-lines
     const int dir = XD ? 0 : a.dir;
@@ -320,9 +325,6 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     // (2L+1 steps); dim+11 <= 32*L+4
     constexpr int EMAX = 2 * L + 1;
 
-    int tile_id, m;
-    block_tile(blockIdx.x, C::NQ, a.ntile_a * a.nb, tile_id, m);
-    if (tile_id >= a.ntile_a * a.nb) return;  // padding blocks of the last group
     const int ta = tile_id % a.ntile_a;
     const int b = tile_id / a.ntile_a;
     const int a0 = ta * tx;
@@ -485,6 +487,15 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
             }
         }
     }
+}
+
+template <int L, int PASS, bool XD>
+__global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
+{
+    int tile_id, m;
+    block_tile(blockIdx.x, PassCfg<PASS>::NQ, a.ntile_a * a.nb, tile_id, m);
+    if (tile_id >= a.ntile_a * a.nb) return;  // padding blocks of the last group
+    sweep_tile<L, PASS, XD>(a, tile_id, m);
 }
 
 // ------------------------------------------------------ transposed sweeps
@@ -800,7 +811,9 @@ __device__ __forceinline__ double miucal(const PtArgs& a, double t)
 // i-average, the k-halos after the i- and j-averages, and refreshes the
 // primitives on every halo cell it wrote.  Each halo cell is computed here
 // from the unaveraged interior.  Reads interior only, writes halos only.
-__global__ void k_qswap_halo(PtArgs a)
+// block bid of the launch (a batched launch numbers each member's blocks
+// from 0)
+__device__ __forceinline__ void qswap_halo_body(const PtArgs& a, unsigned bid)
 {
     const Geo& g = a.g;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
@@ -811,7 +824,7 @@ __global__ void k_qswap_halo(PtArgs a)
     const long fk = 2L * HM * (n1 + 1) * ri;
     const long fj = 2L * HM * (n2 + 1) * ri;
     const long fi = 2L * HM * (n1 + 1) * (n2 + 1);
-    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long p = (long)bid * blockDim.x + threadIdx.x;
     if (p >= fi + fj + fk) return;
     int axis, h, ci[3];
     if (p < fk) {
@@ -866,17 +879,19 @@ __global__ void k_qswap_halo(PtArgs a)
     q2fvar(a, qv, gi);
 }
 
+__global__ __launch_bounds__(256) void k_qswap_halo(PtArgs a) { qswap_halo_body(a, blockIdx.x); }
+
 // qswap duplicate-node averages q(0)=q(n)=(q(0)+q(n))/2, in the order i, j,
 // k (src/parallel.F90:4884-4885, 5029-5030, 5168-5169), and the primitive
 // refresh of every seam node.  One thread owns one group of duplicates: the
 // canonical node has coordinates in [0,n) with at least one zero.
-__global__ void k_qswap_seam(PtArgs a)
+__device__ __forceinline__ void qswap_seam_body(const PtArgs& a, unsigned bid)
 {
     const Geo& g = a.g;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     const long tot = (long)n0 * n1 * n2;
     const long inner = (long)(n0 - 1) * (n1 - 1) * (n2 - 1);
-    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long p = (long)bid * blockDim.x + threadIdx.x;
     if (p >= tot - inner) return;
     // enumerate canonical nodes with some coordinate zero:
     //   i == 0: n1*n2 nodes; i > 0, j == 0: (n0-1)*n2; i,j > 0, k == 0
@@ -919,6 +934,8 @@ __global__ void k_qswap_seam(PtArgs a)
                 q2fvar(a, qv, gi);
             }
 }
+
+__global__ __launch_bounds__(256) void k_qswap_seam(PtArgs a) { qswap_seam_body(a, blockIdx.x); }
 
 // qswap of a two-dimensional box (km = 0), single rank: along a periodic
 // in-plane axis (src/parallel.F90:4877-4915 for i, 5019-5065 for j) the
@@ -1243,11 +1260,11 @@ __device__ __forceinline__ void gflux_cell(const PtArgs& a, int i, int j, int k,
 // k_stats_fin reduces; the inputs are those of G, so the statistics cost
 // one more load per node (the primitive rho) instead of a pass of their own.
 template <bool STATS>
-__global__ __launch_bounds__(256) void k_gflux(PtArgs a)
+__device__ __forceinline__ void gflux_body(const PtArgs& a, unsigned bid)
 {
     const int iend = line_end_i(a.g.n[0] + HM);
     const unsigned rl = (unsigned)(iend + 16);  // cells per row
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned t = bid * blockDim.x + threadIdx.x;
     const unsigned row = t / rl;
     const unsigned nj = (unsigned)a.g.n[1] + 1;
     double st[3] = {0.0, 0.0, 0.0};
@@ -1259,6 +1276,9 @@ __global__ __launch_bounds__(256) void k_gflux(PtArgs a)
     }
     if (STATS) stats_block_sum(st[0], st[1], st[2], a.part);
 }
+
+template <bool STATS>
+__global__ __launch_bounds__(256) void k_gflux(PtArgs a) { gflux_body<STATS>(a, blockIdx.x); }
 
 // bchomo of the mini-app (tgvsolver.F90:271-369): the primitives of the
 // halo faces copied from their periodic images (x, y and z faces, each over
@@ -1334,10 +1354,10 @@ __global__ void k_rk(PtArgs a)
 // updatefvar on the nodes 0..n (src/fludyna.F90:191-242), over row_node's
 // rows; the threads past node n up to the end of its 128-byte line store
 // the primitives already there, so every line is written whole.
-__global__ __launch_bounds__(256) void k_prim(PtArgs a)
+__device__ __forceinline__ void prim_body(const PtArgs& a, unsigned bid)
 {
     int i, j, k;
-    if (!row_node(a.g, i, j, k)) return;
+    if (!row_node(a.g, bid, i, j, k)) return;
     const long gi = gidx(a.g, i, j, k);
     if (i <= a.g.n[0]) {
         double qv[5];
@@ -1349,6 +1369,8 @@ __global__ __launch_bounds__(256) void k_prim(PtArgs a)
         for (int m = 0; m < 6; ++m) keep_cell(a.prim[m], gi);
     }
 }
+
+__global__ __launch_bounds__(256) void k_prim(PtArgs a) { prim_body(a, blockIdx.x); }
 
 // Partial sums of kenergycal / enstophycal / diss_rate_cal over 1..n
 // (src/statistic.F90:870-1042), one double-double (hi, lo) pair per
@@ -2538,6 +2560,102 @@ __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src,
         dst[i] = src[i];
 }
 
+// ------------------------------------------------- batched line-queue engine
+//
+// An ensemble of boxes advances through launches that each cover every
+// member.  A sweep launch takes the members whose lines along its axis
+// share one rows-per-lane count L (one template instance); its work items
+// are tiles of lines of those members, listed in a device table in the
+// order the autotuner chose.  In the persistent form a grid sized to the
+// resident workgroups pulls items from a device counter until the table is
+// exhausted; in the one-shot form workgroup b takes item b.  An item is a
+// tile with all its quantities (the workgroup solves them in turn, so the
+// inputs they share stay in its XCD's L2) or a tile and one quantity.
+
+struct QItem {
+    int member;  // index into the launch's argument array
+    int tile;    // tile of that member (the plain sweep's tile numbering)
+    int q;       // quantity, or -1: every quantity of the tile in turn
+    int pad;
+};
+
+struct QueueLaunch {
+    const SweepArgs* args;  // per member of the launch
+    const QItem* items;
+    int nitems;
+    int* counter;           // persistent form: the launch's own counter (zero at start)
+};
+
+template <int L, int PASS, bool XD>
+__global__ __launch_bounds__(WGT, 4) void k_sweep_q(QueueLaunch ql)
+{
+    __shared__ int s_item;
+    constexpr int NQ = PassCfg<PASS>::NQ;
+    int next = blockIdx.x;
+    for (;;) {
+        if (ql.counter) {
+            if (threadIdx.x == 0) s_item = atomicAdd(ql.counter, 1);
+            __syncthreads();
+            next = s_item;
+            // every thread has read the item before thread 0 fetches the next
+            __syncthreads();
+        }
+        const int item = __builtin_amdgcn_readfirstlane(next);
+        if (item >= ql.nitems) return;
+        const QItem it = ql.items[item];
+        const SweepArgs& a = ql.args[__builtin_amdgcn_readfirstlane(it.member)];
+        if (it.q >= 0) {
+            sweep_tile<L, PASS, XD>(a, it.tile, it.q);
+        } else {
+            for (int m = 0; m < NQ; ++m) {
+                sweep_tile<L, PASS, XD>(a, it.tile, m);
+                // the next quantity stages into the same LDS rows
+                __syncthreads();
+            }
+        }
+        if (!ql.counter) return;
+        __syncthreads();
+    }
+}
+
+// The pointwise kernels over every member in one launch: the members' blocks
+// are numbered consecutively (bstart[k] is member k's first block), and a
+// block runs the kernel body of its member with its own block number.
+enum PtKind : int { PT_QSWAP_HALO = 0, PT_QSWAP_SEAM = 1, PT_GFLUX = 2, PT_PRIM = 3 };
+
+struct PtBatch {
+    const PtArgs* args;
+    const unsigned* bstart;  // nmem + 1 entries
+    int nmem;
+};
+
+template <int K>
+__global__ __launch_bounds__(256) void k_pt_batch(PtBatch b)
+{
+    const unsigned bid = blockIdx.x;
+    int lo = 0, hi = b.nmem - 1;
+    while (lo < hi) {  // the last member whose first block is <= bid
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.bstart[mid] <= bid) lo = mid;
+        else hi = mid - 1;
+    }
+    const int mi = __builtin_amdgcn_readfirstlane(lo);
+    const PtArgs& a = b.args[mi];
+    const unsigned lb = bid - b.bstart[mi];
+    if constexpr (K == PT_QSWAP_HALO) qswap_halo_body(a, lb);
+    else if constexpr (K == PT_QSWAP_SEAM) qswap_seam_body(a, lb);
+    else if constexpr (K == PT_GFLUX) gflux_body<false>(a, lb);
+    else prim_body(a, lb);
+}
+
+// zero n ints (the queue counters of a step; a kernel rather than a memset,
+// so that the step can be captured into a graph)
+__global__ void k_zero_ints(int* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 }  // namespace
 
 // ====================================================================== host
@@ -2587,6 +2705,9 @@ struct astr_ctx {
     int udf = 0;                          // the user condition (ASTR_UDF_*), 0 none
     double time = 0.0;                    // simulation time (commvar time)
     int timing = 0;  // 0 off, 1 per kernel family, 2 whole calls of astr_rk3_steps
+    // bumped by the setters whose values kernels take by value (spacing,
+    // boundary conditions, metrics): a batch graph captured before is stale
+    unsigned long gen = 0;
     hipEvent_t ev[16] = {};
     double tms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -3805,9 +3926,346 @@ int objective_adj(astr_ctx* c, double* J)
     return qswap_adj(c, a);
 }
 
+
+// ---- batched line-queue engine (host)
+
+// Tuning choices of a queue launch, a bit set: QC_PERSISTENT a grid of the
+// resident workgroups pulling items from the counter (else one workgroup
+// per item); QC_INTERLEAVE items alternate between the members (else each
+// member's tiles in a run, the largest member first); QC_PER_QUANTITY an
+// item is one quantity of a tile (else all quantities of the tile in turn)
+enum : int { QC_PERSISTENT = 1, QC_INTERLEAVE = 2, QC_PER_QUANTITY = 4, QC_COUNT = 8 };
+
+struct QLaunchHost {
+    int pass, dir, stage;      // stage: the RK3 stage applied by a z flux launch, else 0
+    int L;
+    std::vector<int> members;  // batch members (contexts) of the launch
+    size_t lds;
+    int args_off = 0;          // first SweepArgs of the launch in the device array
+    int items_off = 0, nitems = 0;
+    int cfg = QC_PERSISTENT;
+    int grid = 0;
+};
+
+struct BatchQueue {
+    std::vector<QLaunchHost> launches;
+    SweepArgs* d_args = nullptr;
+    QItem* d_items = nullptr;
+    long items_cap = 0;
+    int* d_counters = nullptr;
+    PtArgs* d_pt = nullptr;
+    unsigned* d_bstart = nullptr;  // 4 kinds x (nmem + 1)
+    unsigned nblocks[4] = {0, 0, 0, 0};
+    int cus = 0;
+    int tuned[3][LMAX + 1];        // per pass (filter, grad, flux) and L; -1 untuned
+    double tune_us[3][LMAX + 1][QC_COUNT];
+    ~BatchQueue()
+    {
+        (void)hipFree(d_args);
+        (void)hipFree(d_items);
+        (void)hipFree(d_counters);
+        (void)hipFree(d_pt);
+        (void)hipFree(d_bstart);
+    }
+};
+
+int pass_slot(int pass) { return pass == PASS_FILTER ? 0 : pass == PASS_GRAD ? 1 : 2; }
+
+template <int PASS>
+int queue_launch_p(const QLaunchHost& h, const QueueLaunch& ql, hipStream_t st, bool occupancy,
+                   int* occ)
+{
+    const dim3 grid((unsigned)std::max(h.grid, 1));
+    switch (h.L) {
+#define QCASE(LL)                                                                            \
+    case LL:                                                                                 \
+        if (occupancy) {                                                                     \
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(                             \
+                occ, h.dir == 0 ? (const void*)k_sweep_q<LL, PASS, true>                     \
+                                : (const void*)k_sweep_q<LL, PASS, false>, WGT, h.lds));     \
+        } else if (h.dir == 0) {                                                             \
+            hipLaunchKernelGGL((k_sweep_q<LL, PASS, true>), grid, dim3(WGT), h.lds, st, ql);  \
+        } else {                                                                             \
+            hipLaunchKernelGGL((k_sweep_q<LL, PASS, false>), grid, dim3(WGT), h.lds, st, ql); \
+        }                                                                                    \
+        break;
+        QCASE(1) QCASE(2) QCASE(3) QCASE(4) QCASE(5) QCASE(6) QCASE(7) QCASE(8)
+        QCASE(9) QCASE(10) QCASE(11) QCASE(12) QCASE(13) QCASE(14) QCASE(15) QCASE(16)
+#undef QCASE
+    default:
+        return fail(ASTR_E_UNSUPPORTED, "line too long");
+    }
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
+}
+
+int queue_launch(const QLaunchHost& h, const QueueLaunch& ql, hipStream_t st, bool occupancy = false,
+                 int* occ = nullptr)
+{
+    switch (h.pass) {
+    case PASS_FILTER: return queue_launch_p<PASS_FILTER>(h, ql, st, occupancy, occ);
+    case PASS_GRAD: return queue_launch_p<PASS_GRAD>(h, ql, st, occupancy, occ);
+    default: return queue_launch_p<PASS_RHS>(h, ql, st, occupancy, occ);
+    }
+}
+
+int nq_of(int pass) { return pass == PASS_GRAD ? 4 : 5; }
+
+// the items of one launch in the order its configuration asks for
+void queue_items(const QLaunchHost& h, const std::vector<astr_ctx*>& ctx,
+                 const std::vector<SweepArgs>& args, std::vector<QItem>& out)
+{
+    const int nm = (int)h.members.size();
+    const int nq = nq_of(h.pass);
+    const bool perq = h.cfg & QC_PER_QUANTITY;
+    std::vector<int> ntiles(nm), order(nm);
+    for (int k = 0; k < nm; ++k) {
+        const SweepArgs& a = args[h.args_off + k];
+        ntiles[k] = a.ntile_a * a.nb;
+        order[k] = k;
+    }
+    // largest member first (its lines are the longest work of the launch)
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const Geo& gx = ctx[h.members[x]]->g;
+        const Geo& gy = ctx[h.members[y]]->g;
+        return (long)gx.n[0] * gx.n[1] * gx.n[2] > (long)gy.n[0] * gy.n[1] * gy.n[2];
+    });
+    auto push = [&](int k, int t) {
+        if (perq) {
+            for (int m = 0; m < nq; ++m) out.push_back(QItem{k, t, m, 0});
+        } else {
+            out.push_back(QItem{k, t, -1, 0});
+        }
+    };
+    if (h.cfg & QC_INTERLEAVE) {
+        int tmax = 0;
+        for (int k = 0; k < nm; ++k) tmax = std::max(tmax, ntiles[k]);
+        for (int t = 0; t < tmax; ++t)
+            for (int k : order)
+                if (t < ntiles[k]) push(k, t);
+    } else {
+        for (int k : order)
+            for (int t = 0; t < ntiles[k]; ++t) push(k, t);
+    }
+}
+
+// (re)build the item table and grids of every launch from their configurations
+int queue_configure(astr_batch* b);
+int queue_build(astr_batch* b);
+int queue_step(astr_batch* b, double dt, std::vector<hipEvent_t>* ev);
 }  // namespace
 
 // ====================================================================== C ABI
+
+struct astr_batch {
+    std::vector<astr_ctx*> ctx;
+    int device = 0;
+    hipStream_t root = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double dt = 0.0;
+    bool built = false;
+    // 0: one HIP graph per step over the members' streams; 1: direct
+    // launches on the members' streams; 2: the line-queue engine (launches
+    // over every member), one HIP graph per step
+    int mode = 0;
+    std::vector<unsigned long> gens;  // members' set-up generations the graph holds
+    BatchQueue* qe = nullptr;
+    std::vector<int> seq;             // the queue engine's launch sequence of a step
+    int nqueue = 0;                   // queue launches per step (counters)
+};
+
+namespace {
+
+// The launch templates of the queue engine: per sweep (pass, axis) one per
+// rows-per-lane class L of the members along that axis; the z flux sweep
+// once per RK3 stage (its arguments carry the stage).  seq lists a step:
+// template indices, and -1 - kind for the pointwise launches.
+int queue_build(astr_batch* b)
+{
+    delete b->qe;
+    b->qe = new BatchQueue();
+    BatchQueue* qe = b->qe;
+    const int nm = (int)b->ctx.size();
+    for (int p = 0; p < 3; ++p)
+        for (int l = 0; l <= LMAX; ++l) {
+            qe->tuned[p][l] = -1;
+            for (int k = 0; k < QC_COUNT; ++k) qe->tune_us[p][l][k] = 0.0;
+        }
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, b->device));
+    qe->cus = prop.multiProcessorCount;
+    std::vector<SweepArgs> args;
+    auto add_group = [&](int pass, int dir, int stage, std::vector<int>& ids) {
+        // one template per L class among the members of this sweep
+        std::vector<int> Ls;
+        for (int i = 0; i < nm; ++i) {
+            astr_ctx* c = b->ctx[i];
+            if (pass == PASS_FILTER && !c->p.lfilter) continue;
+            if (std::find(Ls.begin(), Ls.end(), c->L[dir]) == Ls.end()) Ls.push_back(c->L[dir]);
+        }
+        std::sort(Ls.begin(), Ls.end(), std::greater<int>());
+        for (int L : Ls) {
+            QLaunchHost h;
+            h.pass = pass;
+            h.dir = dir;
+            h.stage = stage;
+            h.L = L;
+            h.lds = lds_bytes(dir, L);
+            h.args_off = (int)args.size();
+            for (int i = 0; i < nm; ++i) {
+                astr_ctx* c = b->ctx[i];
+                if (c->L[dir] != L || (pass == PASS_FILTER && !c->p.lfilter)) continue;
+                h.members.push_back(i);
+                SweepArgs a = sweep_args(c, dir, pass);
+                if (stage > 0) {
+                    a.rk_stage = stage;
+                    a.rka = RKC[stage - 1][0];
+                    a.rkb = RKC[stage - 1][1];
+                    a.rkc = RKC[stage - 1][2];
+                }
+                args.push_back(a);
+            }
+            ids.push_back((int)qe->launches.size());
+            qe->launches.push_back(h);
+        }
+    };
+    std::vector<int> filt[3], grad[3], flux[2], fluxz[3];
+    for (int d = 0; d < 3; ++d) add_group(PASS_FILTER, d, 0, filt[d]);
+    for (int d = 0; d < 3; ++d) add_group(PASS_GRAD, d, 0, grad[d]);
+    for (int d = 0; d < 2; ++d) add_group(PASS_RHS, d, 0, flux[d]);
+    for (int s = 1; s <= 3; ++s) add_group(PASS_RHS, 2, s, fluxz[s - 1]);
+    b->seq.clear();
+    for (int s = 1; s <= 3; ++s) {
+        for (int d = 0; d < 3; ++d) b->seq.insert(b->seq.end(), filt[d].begin(), filt[d].end());
+        b->seq.push_back(-1 - PT_QSWAP_HALO);
+        b->seq.push_back(-1 - PT_QSWAP_SEAM);
+        for (int d = 0; d < 3; ++d) b->seq.insert(b->seq.end(), grad[d].begin(), grad[d].end());
+        b->seq.push_back(-1 - PT_GFLUX);
+        for (int d = 0; d < 2; ++d) b->seq.insert(b->seq.end(), flux[d].begin(), flux[d].end());
+        b->seq.insert(b->seq.end(), fluxz[s - 1].begin(), fluxz[s - 1].end());
+        b->seq.push_back(-1 - PT_PRIM);
+    }
+    b->nqueue = 0;
+    for (int k : b->seq)
+        if (k >= 0) ++b->nqueue;
+    // the time step enters the z flux arguments at capture (queue_set_dt)
+    HIPCHK(hipMalloc(&qe->d_args, sizeof(SweepArgs) * std::max<size_t>(args.size(), 1)));
+    HIPCHK(hipMemcpy(qe->d_args, args.data(), sizeof(SweepArgs) * args.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&qe->d_counters, sizeof(int) * std::max(b->nqueue, 1)));
+    // pointwise launches
+    std::vector<PtArgs> pts(nm);
+    std::vector<unsigned> bs(4 * (nm + 1), 0);
+    for (int i = 0; i < nm; ++i) {
+        astr_ctx* c = b->ctx[i];
+        pts[i] = pt_args(c);
+        const long n0 = c->g.n[0], n1 = c->g.n[1], n2 = c->g.n[2];
+        const long nh = 2L * HM * ((n1 + 1) * (n2 + 1) + (n0 + 1) * (n2 + 1) + (n0 + 1) * (n1 + 1));
+        const long ns = n0 * n1 * n2 - (n0 - 1) * (n1 - 1) * (n2 - 1);
+        const long np = (n1 + 1) * (n2 + 1) * line_end_i(c->g.n[0]);
+        const long nb[4] = {(nh + 255) / 256, (ns + 255) / 256, gflux_blocks(c->g), (np + 255) / 256};
+        for (int k = 0; k < 4; ++k) bs[k * (nm + 1) + i + 1] = bs[k * (nm + 1) + i] + (unsigned)nb[k];
+    }
+    for (int k = 0; k < 4; ++k) qe->nblocks[k] = bs[k * (nm + 1) + nm];
+    HIPCHK(hipMalloc(&qe->d_pt, sizeof(PtArgs) * nm));
+    HIPCHK(hipMemcpy(qe->d_pt, pts.data(), sizeof(PtArgs) * nm, hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&qe->d_bstart, sizeof(unsigned) * bs.size()));
+    HIPCHK(hipMemcpy(qe->d_bstart, bs.data(), sizeof(unsigned) * bs.size(), hipMemcpyHostToDevice));
+    return queue_configure(b);
+}
+
+// the z flux arguments carry dt
+int queue_set_dt(astr_batch* b, double dt)
+{
+    BatchQueue* qe = b->qe;
+    for (const QLaunchHost& h : qe->launches) {
+        if (h.stage == 0) continue;
+        const int nmem = (int)h.members.size();
+        std::vector<SweepArgs> a(nmem);
+        HIPCHK(hipMemcpy(a.data(), qe->d_args + h.args_off, sizeof(SweepArgs) * nmem,
+                         hipMemcpyDeviceToHost));
+        for (SweepArgs& x : a) x.dt = dt;
+        HIPCHK(hipMemcpy(qe->d_args + h.args_off, a.data(), sizeof(SweepArgs) * nmem,
+                         hipMemcpyHostToDevice));
+    }
+    return ASTR_OK;
+}
+
+int queue_configure(astr_batch* b)
+{
+    BatchQueue* qe = b->qe;
+    std::vector<SweepArgs> args;
+    std::vector<QItem> items;
+    for (QLaunchHost& h : qe->launches) {
+        const int t = qe->tuned[pass_slot(h.pass)][h.L];
+        h.cfg = t >= 0 ? t : QC_PERSISTENT;
+        // the launch's own arguments (host copies, for the tile counts)
+        const int nmem = (int)h.members.size();
+        args.resize((size_t)h.args_off + nmem);
+        for (int k = 0; k < nmem; ++k)
+            args[h.args_off + k] = sweep_args(b->ctx[h.members[k]], h.dir, h.pass);
+        h.items_off = (int)items.size();
+        queue_items(h, b->ctx, args, items);
+        h.nitems = (int)items.size() - h.items_off;
+        if (h.cfg & QC_PERSISTENT) {
+            int occ = 0;
+            TRY(queue_launch(h, QueueLaunch{}, nullptr, true, &occ));
+            h.grid = std::min(h.nitems, std::max(occ, 1) * qe->cus);
+        } else {
+            h.grid = h.nitems;
+        }
+    }
+    if ((long)items.size() > qe->items_cap) {
+        (void)hipFree(qe->d_items);
+        qe->d_items = nullptr;
+        qe->items_cap = 0;
+        HIPCHK(hipMalloc(&qe->d_items, sizeof(QItem) * items.size()));
+        qe->items_cap = (long)items.size();
+    }
+    HIPCHK(hipMemcpy(qe->d_items, items.data(), sizeof(QItem) * items.size(), hipMemcpyHostToDevice));
+    b->built = false;
+    return ASTR_OK;
+}
+
+// One RK3 step of every member through the queue engine on the batch's
+// root stream; ev (tuning) receives an event before and after each queue
+// launch of the sequence.
+int queue_step(astr_batch* b, std::vector<hipEvent_t>* ev)
+{
+    BatchQueue* qe = b->qe;
+    const int nm = (int)b->ctx.size();
+    hipLaunchKernelGGL(k_zero_ints, dim3((b->nqueue + 255) / 256), dim3(256), 0, b->root,
+                       qe->d_counters, b->nqueue);
+    HIPCHK(hipGetLastError());
+    int qi = 0;
+    for (int k : b->seq) {
+        if (k < 0) {
+            const int kind = -1 - k;
+            const PtBatch pb{qe->d_pt, qe->d_bstart + kind * (nm + 1), nm};
+            const dim3 grid(qe->nblocks[kind]);
+            switch (kind) {
+            case PT_QSWAP_HALO: hipLaunchKernelGGL(k_pt_batch<PT_QSWAP_HALO>, grid, dim3(256), 0, b->root, pb); break;
+            case PT_QSWAP_SEAM: hipLaunchKernelGGL(k_pt_batch<PT_QSWAP_SEAM>, grid, dim3(256), 0, b->root, pb); break;
+            case PT_GFLUX: hipLaunchKernelGGL(k_pt_batch<PT_GFLUX>, grid, dim3(256), 0, b->root, pb); break;
+            default: hipLaunchKernelGGL(k_pt_batch<PT_PRIM>, grid, dim3(256), 0, b->root, pb); break;
+            }
+            HIPCHK(hipGetLastError());
+            continue;
+        }
+        const QLaunchHost& h = qe->launches[k];
+        QueueLaunch ql{qe->d_args + h.args_off, qe->d_items + h.items_off, h.nitems,
+                       (h.cfg & QC_PERSISTENT) ? qe->d_counters + qi : nullptr};
+        if (ev) HIPCHK(hipEventRecord((*ev)[2 * qi], b->root));
+        TRY(queue_launch(h, ql, b->root));
+        if (ev) HIPCHK(hipEventRecord((*ev)[2 * qi + 1], b->root));
+        ++qi;
+    }
+    return ASTR_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -4077,6 +4535,7 @@ int astr_upload_geom(astr_ctx* c, const double* jacob, const double* dxi)
 {
     if (!c || !jacob || !dxi) return fail(ASTR_E_ARG, "null argument");
     TRY(set_dev(c));
+    ++c->gen;
     const int n[3] = {c->g.n[0], c->g.n[1], c->g.n[2]};
     const long ni = n[0] + 1 + 2 * HM, nj = n[1] + 1 + 2 * HM, nk = n[2] + 1 + 2 * HM;
     const long nt = ni * nj * nk;
@@ -4339,24 +4798,14 @@ int astr_bench_copy(astr_ctx* c, int32_t reps, double* ms_per_copy, int64_t* byt
     return ASTR_OK;
 }
 
-struct astr_batch {
-    std::vector<astr_ctx*> ctx;
-    int device = 0;
-    hipStream_t root = nullptr;
-    hipEvent_t fork = nullptr;
-    std::vector<hipEvent_t> join;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    double dt = 0.0;
-    bool built = false;
-    int mode = 0;  // 0: one HIP graph per step; 1: direct launches on the members' streams
-};
+
 
 int astr_batch_destroy(astr_batch* b)
 {
     if (!b) return ASTR_OK;
     (void)hipSetDevice(b->device);
     if (b->root) (void)hipStreamSynchronize(b->root);
+    delete b->qe;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (hipEvent_t e : b->join)
@@ -4441,10 +4890,205 @@ static int batch_capture(astr_batch* b, double dt)
     return ASTR_OK;
 }
 
+// the queue engine runs the central scheme of the main path
+static int queue_ok(astr_batch* b)
+{
+    for (astr_ctx* c : b->ctx) {
+        if (c->p.convection != 0 || c->p.variant != 0)
+            return fail(ASTR_E_UNSUPPORTED,
+                        "the line-queue engine runs the central scheme of the main path");
+        for (int d = 0; d < 3; ++d)
+            if (!c->lines_ok[d]) return fail(ASTR_E_UNSUPPORTED, "axis too long for the line sweeps");
+    }
+    return ASTR_OK;
+}
+
+static void drop_graph(astr_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->built = false;
+}
+
+// one step of the queue engine captured into a graph
+static int queue_capture(astr_batch* b, double dt)
+{
+    drop_graph(b);
+    for (astr_ctx* c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));
+    if (!b->qe) TRY(queue_build(b));
+    TRY(queue_set_dt(b, dt));
+    HIPCHK(hipStreamBeginCapture(b->root, hipStreamCaptureModeThreadLocal));
+    const int rc = queue_step(b, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(b->root, &g);
+    if (rc != ASTR_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(ASTR_E_HIP, "graph capture failed");
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->dt = dt;
+    b->built = true;
+    return ASTR_OK;
+}
+
+// members changed by a setter since the graph was captured: every member
+// is checked again, and the graph (and the queue engine's arguments) are
+// rebuilt
+static int batch_refresh(astr_batch* b)
+{
+    std::vector<unsigned long> g;
+    for (astr_ctx* c : b->ctx) {
+        TRY(ready_batch(c));
+        g.push_back(c->gen);
+    }
+    if (g != b->gens) {
+        drop_graph(b);
+        if (b->qe) {
+            // keep the tuning across the rebuild
+            int tuned[3][LMAX + 1];
+            memcpy(tuned, b->qe->tuned, sizeof tuned);
+            TRY(queue_build(b));
+            memcpy(b->qe->tuned, tuned, sizeof tuned);
+            TRY(queue_configure(b));
+        }
+        b->gens = g;
+    }
+    return ASTR_OK;
+}
+
 int astr_batch_set_mode(astr_batch* b, int32_t mode)
 {
-    if (!b || mode < 0 || mode > 1) return fail(ASTR_E_ARG, "mode must be 0 (graph) or 1 (streams)");
+    if (!b || mode < 0 || mode > 2)
+        return fail(ASTR_E_ARG, "mode must be 0 (graph), 1 (streams) or 2 (line queues)");
+    if (mode == 2) TRY(queue_ok(b));
+    if (mode != b->mode) drop_graph(b);
     b->mode = mode;
+    return ASTR_OK;
+}
+
+int astr_batch_autotune(astr_batch* b, double dt)
+{
+    if (!b) return fail(ASTR_E_ARG, "null batch");
+    HIPCHK(hipSetDevice(b->device));
+    TRY(queue_ok(b));
+    TRY(batch_refresh(b));
+    for (astr_ctx* c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));
+    if (!b->qe) TRY(queue_build(b));
+    TRY(queue_set_dt(b, dt));
+    BatchQueue* qe = b->qe;
+    const int nm = (int)b->ctx.size();
+    // the members' q and primitives, restored after every trial step
+    std::vector<double*> bk(nm, nullptr);
+    std::vector<hipEvent_t> ev(2 * (size_t)b->nqueue, nullptr);
+    int rc = ASTR_OK;
+    auto copy_state = [&](bool save) -> int {
+        for (int i = 0; i < nm && rc == ASTR_OK; ++i) {
+            astr_ctx* c = b->ctx[i];
+            const size_t fb = sizeof(double) * c->nfield;
+            for (int f = 0; f < 11; ++f) {
+                double* dev = f < 5 ? c->q[f] : c->prim[f - 5];
+                double* bak = bk[i] + (size_t)f * c->nfield;
+                if (hipMemcpyAsync(save ? bak : dev, save ? dev : bak, fb, hipMemcpyDeviceToDevice,
+                                   b->root) != hipSuccess)
+                    return fail(ASTR_E_HIP, "state copy failed");
+            }
+        }
+        return ASTR_OK;
+    };
+    for (int i = 0; i < nm && rc == ASTR_OK; ++i)
+        if (hipMalloc(&bk[i], sizeof(double) * 11 * b->ctx[i]->nfield) != hipSuccess)
+            rc = fail(ASTR_E_NOMEM, "tuning backup allocation failed");
+    for (size_t e = 0; e < ev.size() && rc == ASTR_OK; ++e)
+        if (hipEventCreate(&ev[e]) != hipSuccess) rc = fail(ASTR_E_HIP, "event");
+    if (rc == ASTR_OK) rc = copy_state(true);
+    double t[3][LMAX + 1][QC_COUNT] = {};
+    bool seen[3][LMAX + 1] = {};
+    // a warm-up step (every kernel loaded), then one step per configuration
+    for (int cfg = -1; cfg < QC_COUNT && rc == ASTR_OK; ++cfg) {
+        for (int p = 0; p < 3; ++p)
+            for (int l = 0; l <= LMAX; ++l) qe->tuned[p][l] = cfg < 0 ? QC_PERSISTENT : cfg;
+        rc = queue_configure(b);
+        if (rc == ASTR_OK) rc = queue_step(b, &ev);
+        if (rc == ASTR_OK && hipStreamSynchronize(b->root) != hipSuccess)
+            rc = fail(ASTR_E_HIP, "tuning step failed");
+        if (rc == ASTR_OK && cfg >= 0) {
+            int qi = 0;
+            for (int k : b->seq) {
+                if (k < 0) continue;
+                const QLaunchHost& h = qe->launches[k];
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, ev[2 * qi], ev[2 * qi + 1]) != hipSuccess) {
+                    rc = fail(ASTR_E_HIP, "event time");
+                    break;
+                }
+                t[pass_slot(h.pass)][h.L][cfg] += 1e3 * ms;
+                seen[pass_slot(h.pass)][h.L] = true;
+                ++qi;
+            }
+        }
+        if (rc == ASTR_OK) rc = copy_state(false);
+    }
+    if (rc == ASTR_OK && hipStreamSynchronize(b->root) != hipSuccess)
+        rc = fail(ASTR_E_HIP, "tuning restore failed");
+    for (int p = 0; p < 3; ++p)
+        for (int l = 0; l <= LMAX; ++l) {
+            qe->tuned[p][l] = -1;
+            if (!seen[p][l] || rc != ASTR_OK) continue;
+            int best = 0;
+            for (int k = 0; k < QC_COUNT; ++k) {
+                qe->tune_us[p][l][k] = t[p][l][k];
+                if (t[p][l][k] < t[p][l][best]) best = k;
+            }
+            qe->tuned[p][l] = best;
+        }
+    if (rc == ASTR_OK) rc = queue_configure(b);
+    for (double* x : bk) (void)hipFree(x);
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    drop_graph(b);
+    return rc;
+}
+
+int astr_batch_get_tuning(astr_batch* b, double* rows, int32_t cap, int32_t* nrows)
+{
+    if (!b || !nrows || (cap > 0 && !rows)) return fail(ASTR_E_ARG, "null argument");
+    int n = 0;
+    if (b->qe)
+        for (int p = 0; p < 3; ++p)
+            for (int l = 0; l <= LMAX; ++l) {
+                if (b->qe->tuned[p][l] < 0) continue;
+                if (n < cap) {
+                    double* r = rows + (size_t)n * (3 + QC_COUNT);
+                    r[0] = p;
+                    r[1] = l;
+                    r[2] = b->qe->tuned[p][l];
+                    for (int k = 0; k < QC_COUNT; ++k) r[3 + k] = b->qe->tune_us[p][l][k];
+                }
+                ++n;
+            }
+    *nrows = n;
+    return ASTR_OK;
+}
+
+int astr_batch_set_tuning(astr_batch* b, const int32_t* choice)
+{
+    if (!b || !choice) return fail(ASTR_E_ARG, "null argument");
+    TRY(queue_ok(b));
+    HIPCHK(hipSetDevice(b->device));
+    for (int p = 0; p < 3; ++p)
+        for (int l = 0; l <= LMAX; ++l)
+            if (choice[p * (LMAX + 1) + l] >= QC_COUNT)
+                return fail(ASTR_E_ARG, "configuration out of range");
+    TRY(batch_refresh(b));
+    if (!b->qe) TRY(queue_build(b));
+    for (int p = 0; p < 3; ++p)
+        for (int l = 0; l <= LMAX; ++l) b->qe->tuned[p][l] = choice[p * (LMAX + 1) + l];
+    TRY(queue_configure(b));
+    drop_graph(b);
     return ASTR_OK;
 }
 
@@ -4452,6 +5096,17 @@ int astr_batch_rk3_steps(astr_batch* b, int32_t nsteps, double dt)
 {
     if (!b || nsteps < 0) return fail(ASTR_E_ARG, "bad argument");
     HIPCHK(hipSetDevice(b->device));
+    TRY(batch_refresh(b));
+    if (b->mode == 2) {
+        if (!b->built || b->dt != dt) TRY(queue_capture(b, dt));
+        else
+            for (astr_ctx* c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));
+        for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(b->exec, b->root));
+        HIPCHK(hipStreamSynchronize(b->root));
+        for (astr_ctx* c : b->ctx)
+            for (int s = 0; s < nsteps; ++s) c->time = c->time + dt;
+        return ASTR_OK;
+    }
     if (b->mode == 1) {
         // every member on its own stream, one step of all members at a time;
         // the runtime spreads the streams over its hardware queues
@@ -4474,6 +5129,7 @@ int astr_set_spacing(astr_ctx* c, double dx, double dy, double dz)
 {
     if (!c) return fail(ASTR_E_ARG, "null context");
     if (!(dx > 0.0 && dy > 0.0 && dz > 0.0)) return fail(ASTR_E_ARG, "spacings must be positive");
+    ++c->gen;
     c->h[0] = dx;
     c->h[1] = dy;
     c->h[2] = dz;
@@ -4498,6 +5154,7 @@ int astr_set_bc(astr_ctx* c, const int32_t* bctype, int32_t udf)
     if (c->p.variant == 1) return fail(ASTR_E_UNSUPPORTED, "the mini-app variant has its own halos");
     for (int f = 0; f < 6; ++f) c->bctype[f] = bctype[f];
     c->udf = any ? udf : 0;
+    ++c->gen;
     return ASTR_OK;
 }
 

@@ -183,6 +183,29 @@ module astr_hip
       integer(c_int32_t), value :: mode
     end function astr_batch_set_mode
 
+    ! line-queue engine (mode 2): trial steps per launch configuration,
+    ! the choices kept, and recorded choices applied
+    integer(c_int) function astr_batch_autotune(batch, dt) bind(c, name='astr_batch_autotune')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      real(c_double), value :: dt
+    end function astr_batch_autotune
+
+    integer(c_int) function astr_batch_get_tuning(batch, rows, cap, nrows) &
+        bind(c, name='astr_batch_get_tuning')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: batch
+      real(c_double), intent(out) :: rows(*)
+      integer(c_int32_t), value :: cap
+      integer(c_int32_t), intent(out) :: nrows
+    end function astr_batch_get_tuning
+
+    integer(c_int) function astr_batch_set_tuning(batch, choice) bind(c, name='astr_batch_set_tuning')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: batch
+      integer(c_int32_t), intent(in) :: choice(*)
+    end function astr_batch_set_tuning
+
     ! uniform spacing of the mini-app variant (variant 1)
     integer(c_int) function astr_set_spacing(ctx, dx, dy, dz) bind(c, name='astr_set_spacing')
       import :: c_int, c_ptr, c_double

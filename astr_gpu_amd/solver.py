@@ -256,8 +256,9 @@ class AstrSolver:
 # -------------------------------------------------------------- set-up data
 
 class AstrBatch:
-    """Contexts that advance together: one HIP graph per RK3 step covering
-    every member (astr_batch_*).  The members stay usable on their own."""
+    """Contexts that advance together (astr_batch_*): one HIP graph per RK3
+    step covering every member, or the line-queue engine (set_mode(2)).  The
+    members stay usable on their own."""
 
     def __init__(self, solvers):
         self.lib = _lib.load()
@@ -274,8 +275,49 @@ class AstrBatch:
 
     def set_mode(self, mode: int) -> None:
         """0: one HIP graph per step (default); 1: direct launches on the
-        members' own streams."""
+        members' own streams; 2: the line-queue engine (every sweep and
+        pointwise kernel one launch over all members, one graph per step)."""
         check(self.lib.astr_batch_set_mode(self.b, int(mode)), "astr_batch_set_mode")
+
+    # the configuration bits of a queue launch (astr_batch_autotune)
+    TUNE_BITS = {1: "persistent grid pulling tiles from the work queue",
+                 2: "members' tiles interleaved (else in runs, largest member first)",
+                 4: "one quantity per work item (else all quantities of a tile)"}
+    KINDS = ("filter", "gradient", "flux")
+
+    def autotune(self, dt: float) -> list:
+        """Time every configuration of the line-queue launches for one step
+        each (the members' state is restored) and keep the fastest per sweep
+        kind and rows-per-lane class; returns the record of tuning()."""
+        check(self.lib.astr_batch_autotune(self.b, float(dt)), "astr_batch_autotune")
+        return self.tuning()
+
+    def tuning(self) -> list:
+        """The choices kept: one dict per (sweep kind, rows-per-lane class L)
+        with the chosen configuration, its meaning, and the trial time of
+        every configuration (microseconds per step)."""
+        n = ctypes.c_int32(0)
+        check(self.lib.astr_batch_get_tuning(self.b, None, 0, ctypes.byref(n)),
+              "astr_batch_get_tuning")
+        rows = np.zeros((max(n.value, 1), 11))
+        check(self.lib.astr_batch_get_tuning(self.b, dptr(rows), n.value, ctypes.byref(n)),
+              "astr_batch_get_tuning")
+        out = []
+        for r in rows[:n.value]:
+            cfg = int(r[2])
+            out.append({"kind": self.KINDS[int(r[0])], "L": int(r[1]), "config": cfg,
+                        "meaning": [v for b, v in self.TUNE_BITS.items() if cfg & b],
+                        "trial_us": [float(v) for v in r[3:]]})
+        return out
+
+    def set_tuning(self, record: list) -> None:
+        """Apply a record of tuning() without trials."""
+        choice = np.full(3 * 17, -1, dtype=np.int32)
+        for e in record:
+            choice[self.KINDS.index(e["kind"]) * 17 + int(e["L"])] = int(e["config"])
+        check(self.lib.astr_batch_set_tuning(
+            self.b, choice.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+            "astr_batch_set_tuning")
 
     def close(self) -> None:
         if self.b:

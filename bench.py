@@ -373,8 +373,10 @@ def ensemble_leg(nbox=48, seed=1, steps=10, warmup=1, sizes=None):
     """48 independent periodic boxes of decaying isotropic turbulence (sizes
     drawn from {64,...,192} per axis, seeded), the central scheme with
     filter and viscous terms: aggregate grid-point-RK3-steps/s of the
-    batched graph against looping the plain solver over the boxes, each
-    timed by the wall clock around its steps (both end synchronised)."""
+    batched modes (a graph over the members' streams, direct launches on
+    them, and the autotuned line-queue engine; the value is the fastest)
+    against looping the plain solver over the boxes, each timed by the wall
+    clock around its steps (all end synchronised)."""
     import astr_gpu_amd as ag
 
     dims = ag.ensemble_dims(nbox, seed, sizes or ag.solver.ENSEMBLE_SIZES)
@@ -409,8 +411,17 @@ def ensemble_leg(nbox=48, seed=1, steps=10, warmup=1, sizes=None):
     t0 = time.perf_counter()
     b.rk3_steps(steps, dt)
     el_streams = time.perf_counter() - t0
+    # the line-queue engine, after its autotuner's trial steps
+    b.set_mode(2)
+    t0 = time.perf_counter()
+    tuning = b.autotune(dt)
+    tune_s = time.perf_counter() - t0
+    b.rk3_steps(warmup, dt)
+    t0 = time.perf_counter()
+    b.rk3_steps(steps, dt)
+    el_queue = time.perf_counter() - t0
     b.close()
-    el = min(el_graph, el_streams)
+    el = min(el_graph, el_streams, el_queue)
     for s in solvers:
         s.close()
     value = pts * steps / el
@@ -422,7 +433,10 @@ def ensemble_leg(nbox=48, seed=1, steps=10, warmup=1, sizes=None):
         "boxes": [list(d) for d in dims], "points": pts, "steps": steps,
         "value": value, "loop_value": loop_value, "batched_over_loop": value / loop_value,
         "graph_value": pts * steps / el_graph, "streams_value": pts * steps / el_streams,
+        "queue_value": pts * steps / el_queue,
         "ms_per_step": 1e3 * el / steps, "loop_ms_per_step": 1e3 * el_loop / steps,
+        "queue_tuning": [{k: e[k] for k in ("kind", "L", "config", "meaning")} for e in tuning],
+        "tune_s": tune_s,
     }
 
 
@@ -439,6 +453,7 @@ def bench_ensemble(args):
                    "parallelism": "single GPU"},
         "loop_value": leg["loop_value"], "batched_over_loop": leg["batched_over_loop"],
         "graph_value": leg["graph_value"], "streams_value": leg["streams_value"],
+        "queue_value": leg["queue_value"], "queue_tuning": leg["queue_tuning"],
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))

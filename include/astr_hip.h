@@ -259,8 +259,38 @@ int astr_batch_destroy(astr_batch *b);
 int astr_batch_rk3_steps(astr_batch *b, int32_t nsteps, double dt);
 /* mode 0 (default): the graph above; mode 1: no graph, every member's
  * kernels launched directly on its own stream, all members one step at a
- * time (the runtime spreads the streams over its hardware queues). */
+ * time (the runtime spreads the streams over its hardware queues);
+ * mode 2: the line-queue engine (members on the central scheme of the main
+ * path only, else ASTR_E_UNSUPPORTED): each line sweep is one launch per
+ * rows-per-lane class over every member, whose workgroups pull tiles of
+ * lines of any member from a device work queue (persistent form) or take
+ * one tile each, and each pointwise kernel (qswap, the fluxes G, the
+ * primitives) is one launch over every member; one such step is captured
+ * into a HIP graph and replayed.  Every mode gives each member the steps
+ * astr_rk3_step gives it, bit for bit.  A setter called on a member after
+ * a graph was captured (spacing, boundary conditions, metrics) makes the
+ * next call capture again. */
 int astr_batch_set_mode(astr_batch *b, int32_t mode);
+
+/* Autotuner of the line-queue engine: runs one trial step per
+ * configuration of the sweep launches, timing every launch with HIP events,
+ * and keeps for each sweep kind (filter, gradient, flux) and rows-per-lane
+ * class L the fastest: persistent grid pulling from the queue or one
+ * workgroup per item (bit 1), the members' tiles interleaved or in runs,
+ * largest member first (bit 2), an item of one quantity or of all the
+ * quantities of a tile (bit 4).  The members' state is restored after each
+ * trial, so the call leaves every member where it was.  dt is the step the
+ * trials take. */
+int astr_batch_autotune(astr_batch *b, double dt);
+/* The tuning kept: *nrows rows (at most cap written) of 11 doubles: sweep
+ * kind (0 filter, 1 gradient, 2 flux), L, the chosen configuration, and the
+ * trial time in microseconds of configurations 0..7 (summed over the axes
+ * and stages of that kind and class). */
+int astr_batch_get_tuning(astr_batch *b, double *rows, int32_t cap, int32_t *nrows);
+/* Apply recorded choices without trials: choice[kind * 17 + L] (kind 0..2,
+ * L 0..16), each a configuration 0..7 or -1 for the default (persistent,
+ * runs, whole tiles). */
+int astr_batch_set_tuning(astr_batch *b, const int32_t *choice);
 
 /* Uniform spacing of the mini-app variant (variant 1,
  * miniapps/tgv_solver_3d/tgvsolver.F90), which divides every derivative
