@@ -826,55 +826,60 @@ __device__ __forceinline__ void qswap_halo_body(const PtArgs& a, unsigned bid)
     const long fi = 2L * HM * (n1 + 1) * (n2 + 1);
     long p = (long)bid * blockDim.x + threadIdx.x;
     if (p >= fi + fj + fk) return;
-    int axis, h, ci[3];
+    // the halo cell ci and its periodic source si: the coordinates are
+    // named one by one (an array indexed by the run-time axis would live in
+    // scratch memory)
+    int axis, h, ci0 = 0, ci1 = 0, ci2 = 0;
     if (p < fk) {
         axis = 2;
         const long r = p / ri;
-        ci[0] = (int)(p - r * ri);
-        ci[1] = (int)(r % (n1 + 1));
+        ci0 = (int)(p - r * ri);
+        ci1 = (int)(r % (n1 + 1));
         h = (int)(r / (n1 + 1));
     } else if ((p -= fk) < fj) {
         axis = 1;
         const long r = p / ri;
-        ci[0] = (int)(p - r * ri);
+        ci0 = (int)(p - r * ri);
         h = (int)(r % (2 * HM));
-        ci[2] = (int)(r / (2 * HM));
+        ci2 = (int)(r / (2 * HM));
     } else {
         p -= fj;
         axis = 0;
         h = (int)(p % (2 * HM));
         const long r = p / (2 * HM);
-        ci[1] = (int)(r % (n1 + 1));
-        ci[2] = (int)(r / (n1 + 1));
+        ci1 = (int)(r % (n1 + 1));
+        ci2 = (int)(r / (n1 + 1));
     }
-    const int dimh = g.n[axis];
-    // halo node and its periodic source
-    ci[axis] = h < HM ? -(h + 1) : dimh + (h - HM + 1);
-    int si[3] = {ci[0], ci[1], ci[2]};  // source cell
-    si[axis] = h < HM ? dimh - (h + 1) : (h - HM + 1);
+    const int dimh = axis == 0 ? n0 : axis == 1 ? n1 : n2;
+    const int hnode = h < HM ? -(h + 1) : dimh + (h - HM + 1);
+    const int snode = h < HM ? dimh - (h + 1) : (h - HM + 1);
+    int si0 = ci0, si1 = ci1, si2 = ci2;
+    if (axis == 0) { ci0 = hnode; si0 = snode; }
+    else if (axis == 1) { ci1 = hnode; si1 = snode; }
+    else { ci2 = hnode; si2 = snode; }
     double qv[5];
     // the source value after the averages of the axes before `axis`
     for (int m = 0; m < 5; ++m) {
         if (axis == 0) {
-            qv[m] = a.q[m][gidx(g, si[0], si[1], si[2])];
+            qv[m] = a.q[m][gidx(g, si0, si1, si2)];
         } else {
             // x-average (only if i is a seam node), then y-average (axis 2)
             auto xavg = [&](int j, int k) -> double {
-                const int i = si[0];
+                const int i = si0;
                 if (i == 0 || i == n0)
                     return 0.5 * (a.q[m][gidx(g, 0, j, k)] + a.q[m][gidx(g, n0, j, k)]);
                 return a.q[m][gidx(g, i, j, k)];
             };
             if (axis == 1) {
-                qv[m] = xavg(si[1], si[2]);
+                qv[m] = xavg(si1, si2);
             } else {
-                const int j = si[1];
-                if (j == 0 || j == n1) qv[m] = 0.5 * (xavg(0, si[2]) + xavg(n1, si[2]));
-                else qv[m] = xavg(j, si[2]);
+                const int j = si1;
+                if (j == 0 || j == n1) qv[m] = 0.5 * (xavg(0, si2) + xavg(n1, si2));
+                else qv[m] = xavg(j, si2);
             }
         }
     }
-    const long gi = gidx(g, ci[0], ci[1], ci[2]);
+    const long gi = gidx(g, ci0, ci1, ci2);
     for (int m = 0; m < 5; ++m) a.q[m][gi] = qv[m];
     q2fvar(a, qv, gi);
 }
@@ -2140,16 +2145,16 @@ __global__ void k_sw_field(CmpArgs a)
         t = a.iss + (int)(r % nt);
         b = a.b0 + (int)(r / nt);
     }
-    int c[3];
-    c[a.dir] = t;
-    c[a.dir == 0 ? 1 : 0] = av;
-    c[a.dir == 2 ? 1 : 2] = b;
-    const long gi = gidx(a.g, c[0], c[1], c[2]);
+    // node t along the axis, av and b along the others (in axis order)
+    const int c0 = a.dir == 0 ? t : av;
+    const int c1 = a.dir == 1 ? t : (a.dir == 0 ? av : b);
+    const int c2 = a.dir == 2 ? t : b;
+    const long gi = gidx(a.g, c0, c1, c2);
     const double vel[3] = {a.prim[1][gi], a.prim[2][gi], a.prim[3][gi]};
     double q[5];
     for (int m = 0; m < 5; ++m) q[m] = a.q[m][gi];
-    double dxi[3] = {0.0, 0.0, 0.0};
-    dxi[a.dir] = a.xi[t + HM];
+    const double xv = a.xi[t + HM];
+    const double dxi[3] = {a.dir == 0 ? xv : 0.0, a.dir == 1 ? xv : 0.0, a.dir == 2 ? xv : 0.0};
     double fp[5], fm[5];
     uwd::sw_split(a.gamma, a.mach, a.prim[0][gi], vel, a.prim[4][gi], a.prim[5][gi], q, dxi,
                   a.jac[gi], fp, fm);
@@ -2284,8 +2289,10 @@ __global__ __launch_bounds__(256) void k_uwd(UwdArgs a)
         double vel[3] = {a.prim[1][gi], a.prim[2][gi], a.prim[3][gi]};
         double q[5];
         for (int m = 0; m < 5; ++m) q[m] = a.q[m][gi];
-        double dxi[3] = {0.0, 0.0, 0.0};
-        dxi[dir] = a.xi[node + HM];
+        // dxi(dir, dir) in its slot (selects: an array indexed by the
+        // run-time axis would live in scratch memory)
+        const double xv = a.xi[node + HM];
+        const double dxi[3] = {dir == 0 ? xv : 0.0, dir == 1 ? xv : 0.0, dir == 2 ? xv : 0.0};
         double* o = sw + (l * NN + t) * 10;
         uwd::sw_split(a.gamma, a.mach, a.prim[0][gi], vel, a.prim[4][gi], a.prim[5][gi], q, dxi,
                       a.jac[gi], o, o + 5);
@@ -2315,9 +2322,9 @@ __global__ __launch_bounds__(256) void k_uwd(UwdArgs a)
             const long gr = gidx(a.g, i, j, k);
             const double vl[3] = {a.prim[1][gl], a.prim[2][gl], a.prim[3][gl]};
             const double vr[3] = {a.prim[1][gr], a.prim[2][gr], a.prim[3][gr]};
-            double dl[3] = {0.0, 0.0, 0.0}, dr[3] = {0.0, 0.0, 0.0};
-            dl[dir] = a.xi[ii + HM];
-            dr[dir] = a.xi[ii + 1 + HM];
+            const double xl = a.xi[ii + HM], xr = a.xi[ii + 1 + HM];
+            const double dl[3] = {dir == 0 ? xl : 0.0, dir == 1 ? xl : 0.0, dir == 2 ? xl : 0.0};
+            const double dr[3] = {dir == 0 ? xr : 0.0, dir == 1 ? xr : 0.0, dir == 2 ? xr : 0.0};
             r = uwd::roe_state(a.gamma, a.prim[0][gl], a.prim[4][gl], a.q[4][gl], vl, dl,
                                a.prim[0][gr], a.prim[4][gr], a.q[4][gr], vr, dr);
         }
@@ -2664,6 +2671,10 @@ struct astr_ctx {
     astr_params p;
     int device = 0;
     hipStream_t stream = nullptr;
+    // streams for sweeps that run beside those of the context's stream
+    // (the three gradient axes), with the events of fork and join
+    hipStream_t aux[2] = {nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
     Geo g{};
     long pi = 0, pj = 0, pk = 0, nfield = 0;  // padded dims, elements per field
     double* base = nullptr;
@@ -3019,7 +3030,7 @@ SweepArgs sweep_args(astr_ctx* c, int dir, int pass)
 }
 
 template <int PASS>
-int launch_sweep_p(astr_ctx* c, const SweepArgs& a)
+int launch_sweep_p(astr_ctx* c, const SweepArgs& a, hipStream_t st)
 {
     const int L = c->L[a.dir];
     const size_t lds = lds_bytes(a.dir, L);
@@ -3029,9 +3040,9 @@ int launch_sweep_p(astr_ctx* c, const SweepArgs& a)
 #define CASE(LL)                                                                        \
     case LL:                                                                            \
         if (a.dir == 0)                                                                 \
-            hipLaunchKernelGGL((k_sweep<LL, PASS, true>), grid, dim3(WGT), lds, c->stream, a); \
+            hipLaunchKernelGGL((k_sweep<LL, PASS, true>), grid, dim3(WGT), lds, st, a); \
         else                                                                            \
-            hipLaunchKernelGGL((k_sweep<LL, PASS, false>), grid, dim3(WGT), lds, c->stream, a); \
+            hipLaunchKernelGGL((k_sweep<LL, PASS, false>), grid, dim3(WGT), lds, st, a); \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
@@ -3043,18 +3054,40 @@ int launch_sweep_p(astr_ctx* c, const SweepArgs& a)
     return ASTR_OK;
 }
 
-int launch_sweep(astr_ctx* c, const SweepArgs& a)
+// st: the stream of the launch (default: the context's)
+int launch_sweep(astr_ctx* c, const SweepArgs& a, hipStream_t st = nullptr)
 {
     if (!c->lines_ok[a.dir])
         return fail(ASTR_E_UNSUPPORTED, "axis too long (or absent) for the compact line operators");
+    if (!st) st = c->stream;
     switch (a.pass) {
-    case PASS_FILTER: return launch_sweep_p<PASS_FILTER>(c, a);
-    case PASS_GRAD: return launch_sweep_p<PASS_GRAD>(c, a);
-    case PASS_RHS: return launch_sweep_p<PASS_RHS>(c, a);
-    case PASS_DERIV_FIELD: return launch_sweep_p<PASS_DERIV_FIELD>(c, a);
-    case PASS_FILTER_FIELD: return launch_sweep_p<PASS_FILTER_FIELD>(c, a);
+    case PASS_FILTER: return launch_sweep_p<PASS_FILTER>(c, a, st);
+    case PASS_GRAD: return launch_sweep_p<PASS_GRAD>(c, a, st);
+    case PASS_RHS: return launch_sweep_p<PASS_RHS>(c, a, st);
+    case PASS_DERIV_FIELD: return launch_sweep_p<PASS_DERIV_FIELD>(c, a, st);
+    case PASS_FILTER_FIELD: return launch_sweep_p<PASS_FILTER_FIELD>(c, a, st);
     }
     return fail(ASTR_E_ARG, "unknown pass");
+}
+
+// Work after fork(c, n) may run on the context's stream and on its n
+// auxiliary streams at once; join(c, n) makes the context's stream wait for
+// what the auxiliary streams were given.  All streams are of default
+// priority.
+int fork(astr_ctx* c, int n)
+{
+    HIPCHK(hipEventRecord(c->fork_ev, c->stream));
+    for (int k = 0; k < n; ++k) HIPCHK(hipStreamWaitEvent(c->aux[k], c->fork_ev, 0));
+    return ASTR_OK;
+}
+
+int join(astr_ctx* c, int n)
+{
+    for (int k = 0; k < n; ++k) {
+        HIPCHK(hipEventRecord(c->join_ev[k], c->aux[k]));
+        HIPCHK(hipStreamWaitEvent(c->stream, c->join_ev[k], 0));
+    }
+    return ASTR_OK;
 }
 
 // ---- transposed sweeps
@@ -3447,11 +3480,16 @@ int do_upwind(astr_ctx* c)
     return ASTR_OK;
 }
 
+// the three axes write disjoint outputs (dvel(:,d), dtmp(d)) from the same
+// inputs: y and z run on the auxiliary streams beside x, so that each
+// sweep's last workgroups overlap the others' instead of draining the GPU
 int do_grad(astr_ctx* c)
 {
     if (c->p.deriv_explicit) return do_grad_explicit(c);
-    for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
-    return ASTR_OK;
+    TRY(fork(c, 2));
+    for (int d = 0; d < 3; ++d)
+        TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD), d == 0 ? c->stream : c->aux[d - 1]));
+    return join(c, 2);
 }
 
 int time_mark(astr_ctx* c, int slot);
@@ -4353,6 +4391,12 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     if (set_dev(c) != ASTR_OK) return cleanup(ASTR_E_HIP);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(ASTR_E_HIP, "hipStreamCreate failed"));
+    for (int k = 0; k < 2; ++k)
+        if (hipStreamCreateWithFlags(&c->aux[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->join_ev[k], hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(ASTR_E_HIP, "hipStreamCreate failed"));
+    if (hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(ASTR_E_HIP, "hipEventCreate failed"));
     for (int d = 0; d < 3; ++d) c->g.n[d] = dims[d];
     c->pi = ((OI + dims[0] + 1 + HM) + 15) / 16 * 16;
     c->pj = dims[1] + 1 + 2 * HM;
@@ -4526,6 +4570,11 @@ int astr_destroy(astr_ctx* c)
         for (int w = 0; w < 2; ++w) (void)hipFree(c->cmp_ac[d][w]);
     for (int e = 0; e < 16; ++e)
         if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
+    for (int k = 0; k < 2; ++k) {
+        if (c->aux[k]) (void)hipStreamDestroy(c->aux[k]);
+        if (c->join_ev[k]) (void)hipEventDestroy(c->join_ev[k]);
+    }
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ASTR_OK;

@@ -140,7 +140,7 @@ def test_tgv256_stage_pieces_and_one_step_against_checker():
     # statistics bar is therefore the whole-run bar, 1e-10.
     for a, b in zip(sg, so):
         assert abs(a - b) <= 1e-10 * abs(b), (sg, so)
-    say(f"  one step {compare_state(ob, s, p, 1e-12, halo_faces=True):.2e}")
+    say(f"  one step {compare_state(ob, s, p, 1e-12, halo_faces=True, vector_scale=True):.2e}")
     s.close()
     ob.close()
 
