@@ -210,13 +210,18 @@ __device__ __forceinline__ void block_tile(int bid, int nq, int ntiles, int& til
 
 // load / store through a 32-bit byte offset from a wave-uniform base (every
 // field is below 4 GB), so the address is one VGPR added to an SGPR pair
+// (every field is global memory: the accesses say so, so that pointers read
+// from an argument table rather than the kernel arguments also give global
+// rather than flat instructions)
+typedef __attribute__((address_space(1))) const char* gcptr;
+typedef __attribute__((address_space(1))) char* gptr;
 __device__ __forceinline__ double ldb(const double* base, unsigned int boff)
 {
-    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + boff);
+    return *(__attribute__((address_space(1))) const double*)((gcptr)base + boff);
 }
 __device__ __forceinline__ void stb(double* base, unsigned int boff, double v)
 {
-    *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + boff) = v;
+    *(__attribute__((address_space(1))) double*)((gptr)base + boff) = v;
 }
 
 // Write-back of the y/z flux sweeps for the nodes n0 + nstep*e (e <
@@ -232,8 +237,66 @@ __device__ __forceinline__ void stb(double* base, unsigned int boff, double v)
 // keep: a lane of a line past the tile's valid ones (i past n, up to the
 // end of the 128-byte line) stores q unchanged, so the line of q is
 // written whole and its halo cells keep their values.
+// A sweep's arguments as one tile of quantity m sees them: the
+// quantity's pointers picked out and the axis' extents read, so that only
+// plain pointers and scalars remain (nothing indexed at run time).  The
+// sweep body takes this view whether the arguments were kernel arguments
+// (one box) or an entry of a member table (the line-queue engine): either
+// way the values sit in scalar registers.
+struct TileView {
+    int dir, dim, sd, n0, na, ntile_a, rs_in, org, s1, s2;
+    LineTab tab;
+    const FiltCoef* fc;
+    const double* src;   // the staged field of this quantity
+    double* gdst;        // gradient: dvel(m, dir) or dtmp(dir)
+    double *q, *qo, *qrhs, *qsave, *fout;
+    const double *jac, *xi;
+    int rk_stage, mini;
+    double rka, rkb, rkc, dt, h;
+};
+
+template <int PASS, bool XD, class A>
+__device__ __forceinline__ TileView tile_view(const A& a, int m)
+{
+    TileView v;
+    v.dir = XD ? 0 : a.dir;
+    v.dim = a.g.n[v.dir];
+    v.sd = (int)a.g.s[v.dir];
+    v.n0 = a.g.n[0];
+    v.na = a.na;
+    v.ntile_a = a.ntile_a;
+    v.rs_in = a.rs_in;
+    v.org = (int)a.g.org;
+    v.s1 = (int)a.g.s[1];
+    v.s2 = (int)a.g.s[2];
+    v.tab = a.tab;
+    v.fc = a.fc;
+    if constexpr (PASS == PASS_FILTER) v.src = a.q[m];
+    else if constexpr (PASS == PASS_GRAD) v.src = a.prim[m < 3 ? 1 + m : 5];
+    else if constexpr (PASS == PASS_RHS) v.src = a.gf[m];
+    else v.src = a.fin;
+    v.gdst = nullptr;
+    if constexpr (PASS == PASS_GRAD) v.gdst = m < 3 ? a.dvel[m * 3 + v.dir] : a.dtmp[v.dir];
+    const bool qm = PASS == PASS_FILTER || PASS == PASS_RHS;
+    v.q = qm ? a.q[m] : nullptr;
+    v.qo = qm ? a.qo[m] : nullptr;
+    v.qrhs = PASS == PASS_RHS ? a.qrhs[m] : nullptr;
+    v.qsave = PASS == PASS_RHS ? a.qsave[m] : nullptr;
+    v.fout = a.fout;
+    v.jac = a.jac;
+    v.xi = a.xi;
+    v.rk_stage = a.rk_stage;
+    v.mini = a.mini;
+    v.rka = a.rka;
+    v.rkb = a.rkb;
+    v.rkc = a.rkc;
+    v.dt = a.dt;
+    v.h = a.h;
+    return v;
+}
+
 template <int MODE, int EMAX>
-__device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const double* trow,
+__device__ __forceinline__ void rhs_epilogue(const TileView& a, const double* trow,
                                              int n0, int nstep, int dim, unsigned int lob,
                                              unsigned int sdb, bool keep)
 {
@@ -248,10 +311,10 @@ __device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const do
             if (e < EMAX) {
                 const int n = min(n0 + nstep * e, dim);
                 const unsigned int od = lob + (unsigned int)n * sdb;
-                r[g] = ldb(a.qrhs[m], od);
-                if constexpr (MODE != 0) q[g] = ldb(a.q[m], od);
+                r[g] = ldb(a.qrhs, od);
+                if constexpr (MODE != 0) q[g] = ldb(a.q, od);
                 if constexpr (MODE == 1 || MODE == 2) jv[g] = ldb(a.jac, od);
-                if constexpr (MODE == 2 || MODE == 4) sv[g] = ldb(a.qsave[m], od);
+                if constexpr (MODE == 2 || MODE == 4) sv[g] = ldb(a.qsave, od);
             }
         }
     };
@@ -267,7 +330,7 @@ __device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const do
                 const double o = trow[n + HM];
                 const unsigned int od = lob + (unsigned int)n * sdb;
                 if constexpr (MODE == 0) {
-                    stb(a.qrhs[m], od, cr[g] + (a.mini ? o / a.h : o));
+                    stb(a.qrhs, od, cr[g] + (a.mini ? o / a.h : o));
                 } else if constexpr (MODE == 1 || MODE == 2) {
                     // the RK3 combine of src/mainloop.F90:423-445 for this
                     // quantity; the workgroup owns these nodes of q_m
@@ -275,11 +338,11 @@ __device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const do
                     double qs;
                     if constexpr (MODE == 1) {
                         qs = cq[g] * cj[g];
-                        stb(a.qsave[m], od, qs);
+                        stb(a.qsave, od, qs);
                     } else {
                         qs = cs[g];
                     }
-                    stb(a.qo[m], od,
+                    stb(a.qo, od,
                         keep ? cq[g] : rk_combine(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], cj[g], r));
                 } else {
                     // rk3 of tgvsolver.F90:1040-1060: q = a qsave + b q + c dt qrhs
@@ -287,11 +350,11 @@ __device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const do
                     double qs;
                     if constexpr (MODE == 3) {
                         qs = cq[g];
-                        stb(a.qsave[m], od, qs);
+                        stb(a.qsave, od, qs);
                     } else {
                         qs = cs[g];
                     }
-                    stb(a.qo[m], od, keep ? cq[g] : rk_combine_mini(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], r));
+                    stb(a.qo, od, keep ? cq[g] : rk_combine_mini(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], r));
                 }
             }
         }
@@ -308,8 +371,9 @@ __device__ __forceinline__ void rhs_epilogue(const SweepArgs& a, int m, const do
 // One tile of lines and one quantity m: stage, solve, write back.  Called
 // by every thread of the workgroup with the same (tile_id, m).
 template <int L, int PASS, bool XD>
-__device__ __forceinline__ void sweep_tile(const SweepArgs& a, int tile_id, int m)
+__device__ __forceinline__ void sweep_tile(const TileView& a, const TileView& w, int tile_id)
 {
+    const LineTab& tab = a.tab;
     // One LDS tile; line l stages node n-HM at tile[l*rs_in + TG + n].  The TG
     // guard slots and the tail padding keep every window read and every
     // (clamped) staging write inside the tile, so neither needs a branch.
@@ -318,7 +382,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, int tile_id, int 
     extern __shared__ double tile[];
     constexpr int OP = (PASS == PASS_FILTER || PASS == PASS_FILTER_FIELD) ? OP_FILTER : OP_DERIV;
     // XD: x-lines (dir 0), else y- or z-lines
-    const int dir = XD ? 0 : a.dir;
+    const int dir = a.dir;
     constexpr bool xdir = XD;
     constexpr int tx = XD ? TXX : TXY;
     // staged nodes per thread: x 32 per step (L+1 steps), y/z 16 per step
@@ -328,8 +392,8 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, int tile_id, int 
     const int ta = tile_id % a.ntile_a;
     const int b = tile_id / a.ntile_a;
     const int a0 = ta * tx;
-    const int dim = a.g.n[dir];
-    const int sd = (int)a.g.s[dir];
+    const int dim = a.dim;
+    const int sd = a.sd;
     const int nvalid = min(tx, a.na - a0);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -347,9 +411,9 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, int tile_id, int 
     int lo;  // element offset of node 0 of line lc
     {
         const int av = a0 + lc;
-        if (XD) lo = (int)a.g.org + av * (int)a.g.s[1] + b * (int)a.g.s[2];
-        else if (dir == 1) lo = (int)a.g.org + av + b * (int)a.g.s[2];
-        else lo = (int)a.g.org + av + b * (int)a.g.s[1];
+        if (XD) lo = a.org + av * a.s1 + b * a.s2;
+        else if (dir == 1) lo = a.org + av + b * a.s2;
+        else lo = a.org + av + b * a.s1;
     }
     const unsigned int lob = (unsigned int)lo * 8u, sdb = (unsigned int)sd * 8u;
     double* const trow = tile + ll * a.rs_in + TG;
@@ -364,12 +428,11 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, int tile_id, int 
     // stores them back as staged
     const bool keepq = PASS == PASS_RHS && !XD && a.rk_stage != 0;
     const bool keepf = PASS == PASS_FILTER && !XD;
-    const bool lw = lvalid || (!XD && (dead || keepq || keepf) && a0 + ll < line_end_i(a.g.n[0]));
+    const bool lw = lvalid || (!XD && (dead || keepq || keepf) && a0 + ll < line_end_i(a.n0));
     unsigned int lob_w = lob;
     if (!XD && !lvalid) {
         const int av = a0 + ll;
-        const int low = dir == 1 ? (int)a.g.org + av + b * (int)a.g.s[2]
-                                 : (int)a.g.org + av + b * (int)a.g.s[1];
+        const int low = dir == 1 ? a.org + av + b * a.s2 : a.org + av + b * a.s1;
         lob_w = (unsigned int)low * 8u;
     }
     const int nlast = XD && dead ? line_end_i(dim) - 1 : dim;  // last node stored
@@ -386,7 +449,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, int tile_id, int 
     // the metric factors of the rows this lane solves (gradient pass)
     double xr[PASS == PASS_GRAD ? L : 1];
     if constexpr (PASS == PASS_GRAD) {
-        const int rb = a.tab.first + (lane & 31) * L;
+        const int rb = tab.first + (lane & 31) * L;
 #pragma unroll
         for (int t = 0; t < L; ++t) xr[t] = a.xi[min(max(rb + t, -HM), dim + HM) + HM];
     }
@@ -400,14 +463,14 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, int tile_id, int 
             const unsigned int od = lob + (unsigned int)node * sdb;
             double v;
             if constexpr (PASS == PASS_FILTER) {
-                v = ldb(a.q[m], lob_s + (unsigned int)wrap_node(node, dim) * sdb);
+                v = ldb(a.src, lob_s + (unsigned int)wrap_node(node, dim) * sdb);
             } else if constexpr (PASS == PASS_GRAD) {
-                v = ldb(a.prim[m < 3 ? 1 + m : 5], od);
+                v = ldb(a.src, od);
             } else if constexpr (PASS == PASS_RHS) {
                 // the flux of this axis, viscous minus inviscid (k_gflux)
-                v = ldb(a.gf[m], od);
+                v = ldb(a.src, od);
             } else {
-                v = ldb(a.fin, od);
+                v = ldb(a.src, od);
             }
             trow[n] = v;
         }
@@ -420,7 +483,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, int tile_id, int 
         double x[L];
         double* ln = tile + l * a.rs_in + TG;
         const double keep0 = ln[HM], keepd = ln[HM + dim];
-        solve_line<L, OP>(ln, a.tab, a.fc, x);
+        solve_line<L, OP>(ln, tab, a.fc, x);
         if constexpr (PASS == PASS_GRAD) {
             // gradcal's metric factor dxi(d,d) (src/comsolver.F90:304-318)
             // on the lane's rows, loaded with the staging
@@ -436,13 +499,13 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, int tile_id, int 
         const int sub = lane & 31;
         if (!keepf || l < nvalid) {
 #pragma unroll
-            for (int t = 0; t < L; ++t) ln[HM + a.tab.first + sub * L + t] = x[t];
+            for (int t = 0; t < L; ++t) ln[HM + tab.first + sub * L + t] = x[t];
         }
         if constexpr (OP == OP_FILTER) {
             // compact_filter keeps the boundary node (src/filter.F90:141-142);
             // the lane that wrote the node overwrites it
-            if (lo_boundary(a.tab.ntype) && sub == (0 - a.tab.first) / L) ln[HM] = keep0;
-            if (hi_boundary(a.tab.ntype) && sub == (dim - a.tab.first) / L) ln[HM + dim] = keepd;
+            if (lo_boundary(tab.ntype) && sub == (0 - tab.first) / L) ln[HM] = keep0;
+            if (hi_boundary(tab.ntype) && sub == (dim - tab.first) / L) ln[HM + dim] = keepd;
         }
     }
     sync();
@@ -452,13 +515,13 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, int tile_id, int 
         // read-modify-write of qrhs (y) or the RK3 combine (z), specialised
         // per mode so that each is straight-line code
         if (lw) {
-            const int mode = a.rk_stage == 0 ? 0 : (a.mini ? 3 : 1) + (a.rk_stage > 1);
+            const int mode = w.rk_stage == 0 ? 0 : (w.mini ? 3 : 1) + (w.rk_stage > 1);
             switch (mode) {
-            case 0: rhs_epilogue<0, EMAX>(a, m, trow, n0, nstep, dim, lob_w, sdb, false); break;
-            case 1: rhs_epilogue<1, EMAX>(a, m, trow, n0, nstep, dim, lob_w, sdb, !lvalid); break;
-            case 2: rhs_epilogue<2, EMAX>(a, m, trow, n0, nstep, dim, lob_w, sdb, !lvalid); break;
-            case 3: rhs_epilogue<3, EMAX>(a, m, trow, n0, nstep, dim, lob_w, sdb, !lvalid); break;
-            default: rhs_epilogue<4, EMAX>(a, m, trow, n0, nstep, dim, lob_w, sdb, !lvalid); break;
+            case 0: rhs_epilogue<0, EMAX>(w, trow, n0, nstep, dim, lob_w, sdb, false); break;
+            case 1: rhs_epilogue<1, EMAX>(w, trow, n0, nstep, dim, lob_w, sdb, !lvalid); break;
+            case 2: rhs_epilogue<2, EMAX>(w, trow, n0, nstep, dim, lob_w, sdb, !lvalid); break;
+            case 3: rhs_epilogue<3, EMAX>(w, trow, n0, nstep, dim, lob_w, sdb, !lvalid); break;
+            default: rhs_epilogue<4, EMAX>(w, trow, n0, nstep, dim, lob_w, sdb, !lvalid); break;
             }
         }
         return;
@@ -471,18 +534,17 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, int tile_id, int 
                 const double o = trow[n + HM];
                 const unsigned int od = lob_w + (unsigned int)n * sdb;
                 if constexpr (PASS == PASS_FILTER) {
-                    stb(a.qo[m], od, o);
+                    stb(w.qo, od, o);
                 } else if constexpr (PASS == PASS_GRAD) {
                     // gradcal: d/dxi times dxi(d,d) (src/comsolver.F90:304-318),
                     // or df/dx (tgvsolver.F90:420-470)
-                    const double gv = a.mini ? o / a.h : o;  // scaled by xi in the solve
-                    if (m < 3) stb(a.dvel[m * 3 + dir], od, gv);
-                    else stb(a.dtmp[dir], od, gv);
+                    const double gv = w.mini ? o / w.h : o;  // scaled by xi in the solve
+                    stb(w.gdst, od, gv);
                 } else if constexpr (PASS == PASS_RHS) {
                     // x: the first axis sets qrhs
-                    stb(a.qrhs[m], od, a.mini ? o / a.h : o);
+                    stb(w.qrhs, od, w.mini ? o / w.h : o);
                 } else {
-                    stb(a.fout, od, o);
+                    stb(w.fout, od, o);
                 }
             }
         }
@@ -495,7 +557,8 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     int tile_id, m;
     block_tile(blockIdx.x, PassCfg<PASS>::NQ, a.ntile_a * a.nb, tile_id, m);
     if (tile_id >= a.ntile_a * a.nb) return;  // padding blocks of the last group
-    sweep_tile<L, PASS, XD>(a, tile_id, m);
+    const TileView v = tile_view<PASS, XD>(a, m);
+    sweep_tile<L, PASS, XD>(v, v, tile_id);
 }
 
 // ------------------------------------------------------ transposed sweeps
@@ -2593,34 +2656,50 @@ struct QueueLaunch {
     int* counter;           // persistent form: the launch's own counter (zero at start)
 };
 
+// The argument table and the items are kernel parameters of their own,
+// const and __restrict__: the compiler may then load a member's arguments
+// once into scalar registers, knowing that the kernel's stores to the fields
+// do not change them (through a pointer inside a struct it would reload
+// them after every store).
 template <int L, int PASS, bool XD>
-__global__ __launch_bounds__(WGT, 4) void k_sweep_q(QueueLaunch ql)
+__global__ __launch_bounds__(WGT, 4) void k_sweep_q(const SweepArgs* __restrict__ margs,
+                                                    const QItem* __restrict__ items, int nitems,
+                                                    int* counter)
 {
     __shared__ int s_item;
     constexpr int NQ = PassCfg<PASS>::NQ;
     int next = blockIdx.x;
     for (;;) {
-        if (ql.counter) {
-            if (threadIdx.x == 0) s_item = atomicAdd(ql.counter, 1);
+        if (counter) {
+            if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
             __syncthreads();
             next = s_item;
             // every thread has read the item before thread 0 fetches the next
             __syncthreads();
         }
         const int item = __builtin_amdgcn_readfirstlane(next);
-        if (item >= ql.nitems) return;
-        const QItem it = ql.items[item];
-        const SweepArgs& a = ql.args[__builtin_amdgcn_readfirstlane(it.member)];
-        if (it.q >= 0) {
-            sweep_tile<L, PASS, XD>(a, it.tile, it.q);
-        } else {
-            for (int m = 0; m < NQ; ++m) {
-                sweep_tile<L, PASS, XD>(a, it.tile, m);
-                // the next quantity stages into the same LDS rows
-                __syncthreads();
-            }
+        if (item >= nitems) return;
+        const QItem it = items[item];
+        // the table is read through the constant address space: the loads
+        // are then invariant like those of kernel arguments (scalar, and
+        // repeated rather than kept live where registers are short)
+        typedef __attribute__((address_space(4))) const SweepArgs CArgs;
+        CArgs& a = ((CArgs*)margs)[__builtin_amdgcn_readfirstlane(it.member)];
+        // one quantity, or every quantity of the tile in turn (one call
+        // site, not unrolled: a copy of the body per quantity would not fit
+        // the registers)
+        const int mlo = it.q >= 0 ? it.q : 0, mhi = it.q >= 0 ? it.q + 1 : NQ;
+#pragma unroll 1
+        for (int m = mlo; m < mhi; ++m) {
+            // the write-back's view is read through a pointer the compiler
+            // cannot tie to the first (the table is invariant either way)
+            CArgs* pw = &a;
+            asm volatile("" : "+s"(pw));
+            sweep_tile<L, PASS, XD>(tile_view<PASS, XD>(a, m), tile_view<PASS, XD>(*pw, m), it.tile);
+            // the next quantity stages into the same LDS rows
+            __syncthreads();
         }
-        if (!ql.counter) return;
+        if (!counter) return;
         __syncthreads();
     }
 }
@@ -2636,19 +2715,21 @@ struct PtBatch {
     int nmem;
 };
 
+// (the tables are const __restrict__ kernel parameters, as in k_sweep_q)
 template <int K>
-__global__ __launch_bounds__(256) void k_pt_batch(PtBatch b)
+__global__ __launch_bounds__(256) void k_pt_batch(const PtArgs* __restrict__ margs,
+                                                  const unsigned* __restrict__ bstart, int nmem)
 {
     const unsigned bid = blockIdx.x;
-    int lo = 0, hi = b.nmem - 1;
+    int lo = 0, hi = nmem - 1;
     while (lo < hi) {  // the last member whose first block is <= bid
         const int mid = (lo + hi + 1) >> 1;
-        if (b.bstart[mid] <= bid) lo = mid;
+        if (bstart[mid] <= bid) lo = mid;
         else hi = mid - 1;
     }
     const int mi = __builtin_amdgcn_readfirstlane(lo);
-    const PtArgs& a = b.args[mi];
-    const unsigned lb = bid - b.bstart[mi];
+    const PtArgs& a = margs[mi];
+    const unsigned lb = bid - bstart[mi];
     if constexpr (K == PT_QSWAP_HALO) qswap_halo_body(a, lb);
     else if constexpr (K == PT_QSWAP_SEAM) qswap_seam_body(a, lb);
     else if constexpr (K == PT_GFLUX) gflux_body<false>(a, lb);
@@ -4022,9 +4103,11 @@ int queue_launch_p(const QLaunchHost& h, const QueueLaunch& ql, hipStream_t st, 
                 occ, h.dir == 0 ? (const void*)k_sweep_q<LL, PASS, true>                     \
                                 : (const void*)k_sweep_q<LL, PASS, false>, WGT, h.lds));     \
         } else if (h.dir == 0) {                                                             \
-            hipLaunchKernelGGL((k_sweep_q<LL, PASS, true>), grid, dim3(WGT), h.lds, st, ql);  \
+            hipLaunchKernelGGL((k_sweep_q<LL, PASS, true>), grid, dim3(WGT), h.lds, st,     \
+                               ql.args, ql.items, ql.nitems, ql.counter);                \
         } else {                                                                             \
-            hipLaunchKernelGGL((k_sweep_q<LL, PASS, false>), grid, dim3(WGT), h.lds, st, ql); \
+            hipLaunchKernelGGL((k_sweep_q<LL, PASS, false>), grid, dim3(WGT), h.lds, st,    \
+                               ql.args, ql.items, ql.nitems, ql.counter);                \
         }                                                                                    \
         break;
         QCASE(1) QCASE(2) QCASE(3) QCASE(4) QCASE(5) QCASE(6) QCASE(7) QCASE(8)
@@ -4284,10 +4367,22 @@ int queue_step(astr_batch* b, std::vector<hipEvent_t>* ev)
             const PtBatch pb{qe->d_pt, qe->d_bstart + kind * (nm + 1), nm};
             const dim3 grid(qe->nblocks[kind]);
             switch (kind) {
-            case PT_QSWAP_HALO: hipLaunchKernelGGL(k_pt_batch<PT_QSWAP_HALO>, grid, dim3(256), 0, b->root, pb); break;
-            case PT_QSWAP_SEAM: hipLaunchKernelGGL(k_pt_batch<PT_QSWAP_SEAM>, grid, dim3(256), 0, b->root, pb); break;
-            case PT_GFLUX: hipLaunchKernelGGL(k_pt_batch<PT_GFLUX>, grid, dim3(256), 0, b->root, pb); break;
-            default: hipLaunchKernelGGL(k_pt_batch<PT_PRIM>, grid, dim3(256), 0, b->root, pb); break;
+            case PT_QSWAP_HALO:
+                hipLaunchKernelGGL(k_pt_batch<PT_QSWAP_HALO>, grid, dim3(256), 0, b->root, pb.args,
+                                   pb.bstart, pb.nmem);
+                break;
+            case PT_QSWAP_SEAM:
+                hipLaunchKernelGGL(k_pt_batch<PT_QSWAP_SEAM>, grid, dim3(256), 0, b->root, pb.args,
+                                   pb.bstart, pb.nmem);
+                break;
+            case PT_GFLUX:
+                hipLaunchKernelGGL(k_pt_batch<PT_GFLUX>, grid, dim3(256), 0, b->root, pb.args,
+                                   pb.bstart, pb.nmem);
+                break;
+            default:
+                hipLaunchKernelGGL(k_pt_batch<PT_PRIM>, grid, dim3(256), 0, b->root, pb.args,
+                                   pb.bstart, pb.nmem);
+                break;
             }
             HIPCHK(hipGetLastError());
             continue;
