@@ -2656,6 +2656,16 @@ struct QueueLaunch {
     int* counter;           // persistent form: the launch's own counter (zero at start)
 };
 
+// A member's entry of a sweep argument table as the queue kernel reads it:
+// in the constant address space (address space 4) in the device pass.  The
+// host pass of this single-source file only checks the kernel's body, for
+// which the plain type stands in.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(4))) const SweepArgs CArgs;
+#else
+typedef const SweepArgs CArgs;
+#endif
+
 // The argument table and the items are kernel parameters of their own,
 // const and __restrict__: the compiler may then load a member's arguments
 // once into scalar registers, knowing that the kernel's stores to the fields
@@ -2681,9 +2691,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_q(const SweepArgs* __restrict_
         if (item >= nitems) return;
         const QItem it = items[item];
         // the table is read through the constant address space: the loads
-        // are then invariant like those of kernel arguments (scalar, and
-        // repeated rather than kept live where registers are short)
-        typedef __attribute__((address_space(4))) const SweepArgs CArgs;
+        // are then invariant like those of kernel arguments
         CArgs& a = ((CArgs*)margs)[__builtin_amdgcn_readfirstlane(it.member)];
         // one quantity, or every quantity of the tile in turn (one call
         // site, not unrolled: a copy of the body per quantity would not fit
