@@ -1094,19 +1094,19 @@ __device__ __forceinline__ void stats_terms(const PtArgs& a, long gi, double& ke
                       2.0 * (s12 * s12 + s13 * s13 + s23 * s23) - (1.0 / 3.0) * (div * div));
 }
 
-// The block's double-double sums of (ke, en, di) into part[6 * blockIdx.x
-// ...]: a butterfly over each wavefront's lanes, then the wavefronts' sums
-// in LDS.  Every thread of the block takes part.
+// The block's sums of (ke, en, di) into part[6 * blockIdx.x ...] as
+// double-double pairs: a butterfly of plain additions over each
+// wavefront's 64 lanes (its rounding, ~64 ulp of a wavefront's sum, is far
+// below what the per-step statistics are compared to), then the
+// wavefronts' sums in double-double in LDS.  Every thread of the block
+// takes part.
 __device__ __forceinline__ void stats_block_sum(double ke, double en, double di, double* part)
 {
     double h[3] = {ke, en, di}, l[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double oh = __shfl_xor(h[c], off, 64), ol = __shfl_xor(l[c], off, 64);
-            dd_add(h[c], l[c], oh, ol);
-        }
+        for (int c = 0; c < 3; ++c) h[c] += __shfl_xor(h[c], off, 64);
     __shared__ double ws[4][6];
     const int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
@@ -1482,18 +1482,23 @@ __global__ __launch_bounds__(256) void k_stats(PtArgs a)
         }
 }
 
-// The nb workgroup partials of k_stats summed in double-double by one
-// workgroup (each thread a strided share, then a tree over the threads),
-// and the means kenergycal / enstophycal / diss_rate_cal report: ke and
-// enstrophy 1/2 of the mean, dissipation the mean (roinf = uinf = l_0 = 1).
-// Written to out3 on the device, so a step's statistics need no host
-// round trip.
+// Workgroup partials of the statistics (double-double pairs, 6 per
+// partial) summed in double-double: workgroup w of the launch takes the
+// partials [w*chunk, (w+1)*chunk) of nb, each thread a strided share, then
+// a tree over the threads.  With out3 null the workgroup's sum goes to
+// part2 (6 doubles per workgroup: the first level over every k_stats or
+// k_gflux workgroup); otherwise (one workgroup) the means kenergycal /
+// enstophycal / diss_rate_cal report go to out3 on the device: ke and
+// enstrophy 1/2 of the mean, dissipation the mean (roinf = uinf = l_0 = 1),
+// so a step's statistics need no host round trip.
 __global__ __launch_bounds__(256) void k_stats_fin(const double* __restrict__ part, long nb,
+                                                  long chunk, double* __restrict__ part2,
                                                   double n3, double* __restrict__ out3)
 {
     __shared__ double sh[3][256], sl[3][256];
     double h[3] = {0.0, 0.0, 0.0}, l[3] = {0.0, 0.0, 0.0};
-    for (long b = threadIdx.x; b < nb; b += 256)
+    const long b0 = (long)blockIdx.x * chunk, b1 = min(nb, b0 + chunk);
+    for (long b = b0 + threadIdx.x; b < b1; b += 256)
         for (int c = 0; c < 3; ++c) dd_add(h[c], l[c], part[6 * b + 2 * c], part[6 * b + 2 * c + 1]);
     for (int c = 0; c < 3; ++c) {
         sh[c][threadIdx.x] = h[c];
@@ -1511,9 +1516,16 @@ __global__ __launch_bounds__(256) void k_stats_fin(const double* __restrict__ pa
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out3[0] = 0.5 * sh[0][0] / n3;
-        out3[1] = 0.5 * sh[1][0] / n3;
-        out3[2] = sh[2][0] / n3;
+        if (out3) {
+            out3[0] = 0.5 * sh[0][0] / n3;
+            out3[1] = 0.5 * sh[1][0] / n3;
+            out3[2] = sh[2][0] / n3;
+        } else {
+            for (int c = 0; c < 3; ++c) {
+                part2[6 * blockIdx.x + 2 * c] = sh[c][0];
+                part2[6 * blockIdx.x + 2 * c + 1] = sl[c][0];
+            }
+        }
     }
 }
 
@@ -2760,10 +2772,6 @@ struct astr_ctx {
     astr_params p;
     int device = 0;
     hipStream_t stream = nullptr;
-    // streams for sweeps that run beside those of the context's stream
-    // (the three gradient axes), with the events of fork and join
-    hipStream_t aux[2] = {nullptr, nullptr};
-    hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
     Geo g{};
     long pi = 0, pj = 0, pk = 0, nfield = 0;  // padded dims, elements per field
     double* base = nullptr;
@@ -3159,25 +3167,6 @@ int launch_sweep(astr_ctx* c, const SweepArgs& a, hipStream_t st = nullptr)
     return fail(ASTR_E_ARG, "unknown pass");
 }
 
-// Work after fork(c, n) may run on the context's stream and on its n
-// auxiliary streams at once; join(c, n) makes the context's stream wait for
-// what the auxiliary streams were given.  All streams are of default
-// priority.
-int fork(astr_ctx* c, int n)
-{
-    HIPCHK(hipEventRecord(c->fork_ev, c->stream));
-    for (int k = 0; k < n; ++k) HIPCHK(hipStreamWaitEvent(c->aux[k], c->fork_ev, 0));
-    return ASTR_OK;
-}
-
-int join(astr_ctx* c, int n)
-{
-    for (int k = 0; k < n; ++k) {
-        HIPCHK(hipEventRecord(c->join_ev[k], c->aux[k]));
-        HIPCHK(hipStreamWaitEvent(c->stream, c->join_ev[k], 0));
-    }
-    return ASTR_OK;
-}
 
 // ---- transposed sweeps
 
@@ -3379,6 +3368,8 @@ long gflux_blocks(const Geo& g)
     return (rows * (line_end_i(g.n[0] + HM) + 16) + 255) / 256;
 }
 
+int stats_finish(astr_ctx* c, long nb, double* dst);
+
 // stats_slot (nullable, device memory): also the statistics of the state
 // (rkfirst), summed by the same workgroups and reduced into the slot
 int do_gflux(astr_ctx* c, double* stats_slot = nullptr)
@@ -3388,9 +3379,7 @@ int do_gflux(astr_ctx* c, double* stats_slot = nullptr)
         hipLaunchKernelGGL(k_gflux<true>, dim3((unsigned)nb), dim3(256), 0, c->stream,
                            pt_args(c));
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_stats_fin, dim3(1), dim3(256), 0, c->stream, (const double*)c->part,
-                           nb, (double)c->g.n[0] * (double)c->g.n[1] * (double)c->g.n[2],
-                           stats_slot);
+        return stats_finish(c, nb, stats_slot);
     } else {
         hipLaunchKernelGGL(k_gflux<false>, dim3((unsigned)nb), dim3(256), 0, c->stream,
                            pt_args(c));
@@ -3569,16 +3558,11 @@ int do_upwind(astr_ctx* c)
     return ASTR_OK;
 }
 
-// the three axes write disjoint outputs (dvel(:,d), dtmp(d)) from the same
-// inputs: y and z run on the auxiliary streams beside x, so that each
-// sweep's last workgroups overlap the others' instead of draining the GPU
 int do_grad(astr_ctx* c)
 {
     if (c->p.deriv_explicit) return do_grad_explicit(c);
-    TRY(fork(c, 2));
-    for (int d = 0; d < 3; ++d)
-        TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD), d == 0 ? c->stream : c->aux[d - 1]));
-    return join(c, 2);
+    for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
+    return ASTR_OK;
 }
 
 int time_mark(astr_ctx* c, int slot);
@@ -3623,18 +3607,32 @@ int do_rk(astr_ctx* c, int stage, double dt)
     return launch_pt(c, k_rk, n_interior(c), a);
 }
 
-// k_stats and k_stats_fin: the three statistics of the current state into
+// the nb partials in c->part reduced to the three statistics in dst (3
+// doubles of device memory): STATS_FIN workgroups sum slices of them into
+// the second-level partials after the first npart, one workgroup sums those
+constexpr int STATS_FIN = 256;
+int stats_finish(astr_ctx* c, long nb, double* dst)
+{
+    const double n3 = (double)c->g.n[0] * (double)c->g.n[1] * (double)c->g.n[2];
+    double* part2 = c->part + 6 * c->npart;
+    const long chunk = (nb + STATS_FIN - 1) / STATS_FIN;
+    hipLaunchKernelGGL(k_stats_fin, dim3(STATS_FIN), dim3(256), 0, c->stream,
+                       (const double*)c->part, nb, chunk, part2, n3, (double*)nullptr);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_stats_fin, dim3(1), dim3(256), 0, c->stream, (const double*)part2,
+                       (long)STATS_FIN, (long)STATS_FIN, (double*)nullptr, n3, dst);
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
+}
+
+// k_stats and the finish: the three statistics of the current state into
 // dst (3 doubles of device memory), queued on the solver's stream
 int launch_stats(astr_ctx* c, double* dst)
 {
     PtArgs a = pt_args(c);
     const long n = (long)c->g.n[0] * c->g.n[1] * c->g.n[2];
     TRY(launch_pt(c, k_stats, n, a, 256));
-    const long nb = (n + 255) / 256;
-    hipLaunchKernelGGL(k_stats_fin, dim3(1), dim3(256), 0, c->stream, (const double*)c->part, nb,
-                       (double)c->g.n[0] * (double)c->g.n[1] * (double)c->g.n[2], dst);
-    HIPCHK(hipGetLastError());
-    return ASTR_OK;
+    return stats_finish(c, (n + 255) / 256, dst);
 }
 
 int do_stats(astr_ctx* c, double* out)
@@ -4494,12 +4492,7 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     if (set_dev(c) != ASTR_OK) return cleanup(ASTR_E_HIP);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(ASTR_E_HIP, "hipStreamCreate failed"));
-    for (int k = 0; k < 2; ++k)
-        if (hipStreamCreateWithFlags(&c->aux[k], hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&c->join_ev[k], hipEventDisableTiming) != hipSuccess)
-            return cleanup(fail(ASTR_E_HIP, "hipStreamCreate failed"));
-    if (hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess)
-        return cleanup(fail(ASTR_E_HIP, "hipEventCreate failed"));
+
     for (int d = 0; d < 3; ++d) c->g.n[d] = dims[d];
     c->pi = ((OI + dims[0] + 1 + HM) + 15) / 16 * 16;
     c->pj = dims[1] + 1 + 2 * HM;
@@ -4645,7 +4638,8 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     c->npart = ((long)dims[0] * dims[1] * std::max(dims[2], 1) + 255) / 256;
     if (dims[2] > 0) c->npart = std::max(c->npart, gflux_blocks(c->g));
     c->host_part.resize(6 * c->npart);
-    if (hipMalloc(&c->part, sizeof(double) * 6 * c->npart) != hipSuccess)
+    // the partials, then the second-level partials of the statistics
+    if (hipMalloc(&c->part, sizeof(double) * 6 * (c->npart + STATS_FIN)) != hipSuccess)
         return cleanup(fail(ASTR_E_NOMEM, "partials allocation failed"));
     if (hipMalloc(&c->stats_dev, sizeof(double) * 3 * STATS_SLOTS) != hipSuccess)
         return cleanup(fail(ASTR_E_NOMEM, "statistics allocation failed"));
@@ -4673,11 +4667,6 @@ int astr_destroy(astr_ctx* c)
         for (int w = 0; w < 2; ++w) (void)hipFree(c->cmp_ac[d][w]);
     for (int e = 0; e < 16; ++e)
         if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
-    for (int k = 0; k < 2; ++k) {
-        if (c->aux[k]) (void)hipStreamDestroy(c->aux[k]);
-        if (c->join_ev[k]) (void)hipEventDestroy(c->join_ev[k]);
-    }
-    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ASTR_OK;

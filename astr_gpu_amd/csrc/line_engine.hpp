@@ -17,8 +17,10 @@
 // The right-hand sides restate compact_fd_rhs (src/derivative.F90:210-306)
 // and compact_filter_rhs (src/filter.F90:156-285).  A lane loads its rows'
 // stencil window from the LDS line (node n at ln[n + HM]) into registers
-// once; the few closure rows at the line ends are evaluated wave-uniformly
-// (scalar coefficient loads) and handed to the lane that owns them.
+// once.  The filter's closure rows at the low end of a line without a
+// boundary there (the 11-point halo rows) are evaluated by lane 0 from its
+// window; the other closure rows are evaluated wave-uniformly (scalar
+// coefficient loads) and handed to the lane that owns them.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -235,6 +237,23 @@ __device__ __forceinline__ void solve_line(const double* ln, const LineTab& tb,
             if constexpr (OP == OP_DERIV) x[t] = rhs_deriv_in(w + t + HW);
             else x[t] = rhs_filter_in(w + t + HW, fc);
         }
+        if constexpr (OP == OP_FILTER && L >= 3) {
+            // The closure rows at the low end of a line without a boundary
+            // there (the 11-point ch rows of compact_filter_rhs) from lane
+            // 0's window, in the reference's summation order: rows 0..2 are
+            // its x[0..2], and the window holds their nodes first-2..first+8
+            // at w[3..13].  (The high end's rows sit at a run-time place in
+            // their owner's window and are taken from LDS below.)
+            if (!lo_boundary(tb.ntype) && sub == 0) {
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int j = 0; j <= 10; ++j) v = v + fc->ch[r][j] * w[j + 3];
+                    x[r] = v;
+                }
+            }
+        }
     }
     // the closure rows at both ends, computed wave-uniformly
     if constexpr (OP == OP_DERIV) {
@@ -248,7 +267,10 @@ __device__ __forceinline__ void solve_line(const double* ln, const LineTab& tb,
             set_row(x, r, sub, rhs_deriv_edge(ln + HM + tb.first + r, r, tb));
         }
     } else {
-        const int ne = lo_boundary(tb.ntype) ? 5 : 3;
+        // boundary closures, and the halo closure rows the windows above
+        // did not give
+        const bool lo_done = L >= 3 && !lo_boundary(tb.ntype);
+        const int ne = lo_boundary(tb.ntype) ? 5 : (lo_done ? 0 : 3);
 #pragma unroll
         for (int r = 0; r < 5; ++r)
             if (r < ne) set_row(x, r, sub, rhs_filter_edge(ln, r, tb, fc));
