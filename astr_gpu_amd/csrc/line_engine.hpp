@@ -237,12 +237,12 @@ __device__ __forceinline__ void solve_line(const double* ln, const LineTab& tb,
             if constexpr (OP == OP_DERIV) x[t] = rhs_deriv_in(w + t + HW);
             else x[t] = rhs_filter_in(w + t + HW, fc);
         }
-        if constexpr (OP == OP_FILTER && L >= 3) {
+        if constexpr (OP == OP_FILTER && L >= 4) {
             // The closure rows at the low end of a line without a boundary
             // there (the 11-point ch rows of compact_filter_rhs) from lane
             // 0's window, in the reference's summation order: rows 0..2 are
-            // its x[0..2], and the window holds their nodes first-2..first+8
-            // at w[3..13].  (The high end's rows sit at a run-time place in
+            // its x[0..2], and the window (nodes first-5..first+L+4, so
+            // L >= 4) holds their nodes first-2..first+8 at w[3..13].  (The high end's rows sit at a run-time place in
             // their owner's window and are taken from LDS below.)
             if (!lo_boundary(tb.ntype) && sub == 0) {
 #pragma unroll
@@ -269,7 +269,7 @@ __device__ __forceinline__ void solve_line(const double* ln, const LineTab& tb,
     } else {
         // boundary closures, and the halo closure rows the windows above
         // did not give
-        const bool lo_done = L >= 3 && !lo_boundary(tb.ntype);
+        const bool lo_done = L >= 4 && !lo_boundary(tb.ntype);
         const int ne = lo_boundary(tb.ntype) ? 5 : (lo_done ? 0 : 3);
 #pragma unroll
         for (int r = 0; r < 5; ++r)
