@@ -242,8 +242,9 @@ __device__ __forceinline__ void solve_line(const double* ln, const LineTab& tb,
             // there (the 11-point ch rows of compact_filter_rhs) from lane
             // 0's window, in the reference's summation order: rows 0..2 are
             // its x[0..2], and the window (nodes first-5..first+L+4, so
-            // L >= 4) holds their nodes first-2..first+8 at w[3..13].  (The high end's rows sit at a run-time place in
-            // their owner's window and are taken from LDS below.)
+            // L >= 4) holds their nodes first-2..first+8 at w[3..13].  (The
+            // high end's rows sit at a run-time place in their owner's
+            // window and are taken from LDS below.)
             if (!lo_boundary(tb.ntype) && sub == 0) {
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
