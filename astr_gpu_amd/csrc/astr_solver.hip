@@ -368,11 +368,16 @@ __device__ __forceinline__ void rhs_epilogue(const TileView& a, const double* tr
     }
 }
 
-// One tile of lines and one quantity m: stage, solve, write back.  Called
-// by every thread of the workgroup with the same (tile_id, m).
-template <int L, int PASS, bool XD>
-__device__ __forceinline__ void sweep_tile(const TileView& a, const TileView& w, int tile_id)
+// One tile of lines and quantity m: stage, solve, write back.  Called by
+// every thread of the workgroup with the same tile and quantity.  The
+// staging and the solve read the view of args, the write-back that of
+// args_w, formed after the solve (the same arguments; the queue kernel
+// hands them over through a second pointer so that the write-back's
+// pointers are read again rather than kept in registers through the solve).
+template <int L, int PASS, bool XD, class A>
+__device__ __forceinline__ void sweep_tile(const A& args, const A& args_w, int tile_id, int m)
 {
+    const TileView a = tile_view<PASS, XD>(args, m);
     const LineTab& tab = a.tab;
     // One LDS tile; line l stages node n-HM at tile[l*rs_in + TG + n].  The TG
     // guard slots and the tail padding keep every window read and every
@@ -511,6 +516,7 @@ __device__ __forceinline__ void sweep_tile(const TileView& a, const TileView& w,
     sync();
 
     // ---------------- write back nodes 0..dim
+    const TileView w = tile_view<PASS, XD>(args_w, m);
     if constexpr (PASS == PASS_RHS && !XD) {
         // read-modify-write of qrhs (y) or the RK3 combine (z), specialised
         // per mode so that each is straight-line code
@@ -557,8 +563,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep(SweepArgs a)
     int tile_id, m;
     block_tile(blockIdx.x, PassCfg<PASS>::NQ, a.ntile_a * a.nb, tile_id, m);
     if (tile_id >= a.ntile_a * a.nb) return;  // padding blocks of the last group
-    const TileView v = tile_view<PASS, XD>(a, m);
-    sweep_tile<L, PASS, XD>(v, v, tile_id);
+    sweep_tile<L, PASS, XD>(a, a, tile_id, m);
 }
 
 // ------------------------------------------------------ transposed sweeps
@@ -2715,7 +2720,7 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_q(const SweepArgs* __restrict_
             // cannot tie to the first (the table is invariant either way)
             CArgs* pw = &a;
             asm volatile("" : "+s"(pw));
-            sweep_tile<L, PASS, XD>(tile_view<PASS, XD>(a, m), tile_view<PASS, XD>(*pw, m), it.tile);
+            sweep_tile<L, PASS, XD>(a, *pw, it.tile, m);
             // the next quantity stages into the same LDS rows
             __syncthreads();
         }
