@@ -53,6 +53,7 @@ SIGNATURES = {
     "astr_rhscal": (ctypes.c_int, [_P]),
     "astr_rk3_update": (ctypes.c_int, [_P, _I, ctypes.c_double]),
     "astr_stats": (ctypes.c_int, [_P, _D]),
+    "astr_stats_device": (ctypes.c_int, [_P, _P]),
     "astr_rk3_step": (ctypes.c_int, [_P, ctypes.c_double, _D]),
     "astr_rk3_steps": (ctypes.c_int, [_P, _I, ctypes.c_double, _D]),
     "astr_download_rhs": (ctypes.c_int, [_P, _D, _D, _D]),

@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1532,6 +1533,48 @@ __global__ __launch_bounds__(256) void k_stats_fin(const double* __restrict__ pa
             }
         }
     }
+}
+
+// Statistics with a fixed number of workgroups (the allocation-free entry
+// point, astr_stats_device): workgroup b sums, in double-double, the nodes
+// 1..n taken grid-stride from its own start, then its threads in an LDS
+// tree; part[6 b ...] receives the (hi, lo) pairs.  The node index is
+// 32-bit (the entry point refuses boxes of 2^31 nodes or more).
+__global__ __launch_bounds__(256) void k_stats_gs(PtArgs a, double* __restrict__ part)
+{
+    __shared__ double sh[3][256], sl[3][256];
+    const unsigned n0 = (unsigned)a.g.n[0], n1 = (unsigned)a.g.n[1];
+    const unsigned ntot = n0 * n1 * (unsigned)a.g.n[2];
+    double h[3] = {0.0, 0.0, 0.0}, l[3] = {0.0, 0.0, 0.0};
+    for (unsigned p = blockIdx.x * blockDim.x + threadIdx.x; p < ntot; p += gridDim.x * blockDim.x) {
+        const unsigned r = p / n0;
+        const int i = 1 + (int)(p - r * n0);
+        const int j = 1 + (int)(r % n1);
+        const int k = 1 + (int)(r / n1);
+        double t[3];
+        stats_terms(a, gidx(a.g, i, j, k), t[0], t[1], t[2]);
+        for (int c = 0; c < 3; ++c) dd_add(h[c], l[c], t[c], 0.0);
+    }
+    for (int c = 0; c < 3; ++c) {
+        sh[c][threadIdx.x] = h[c];
+        sl[c][threadIdx.x] = l[c];
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int c = 0; c < 3; ++c) {
+                double hh = sh[c][threadIdx.x], ll = sl[c][threadIdx.x];
+                dd_add(hh, ll, sh[c][threadIdx.x + s], sl[c][threadIdx.x + s]);
+                sh[c][threadIdx.x] = hh;
+                sl[c][threadIdx.x] = ll;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) {
+            part[6 * blockIdx.x + 2 * c] = sh[c][0];
+            part[6 * blockIdx.x + 2 * c + 1] = sl[c][0];
+        }
 }
 
 // maxmincal's max_q (src/statistic.F90:737-763): the largest value of each
@@ -4856,6 +4899,59 @@ int astr_stats(astr_ctx* c, double* out3)
     if (!out3) return fail(ASTR_E_ARG, "null argument");
     if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "statistics of three-dimensional boxes only");
     return do_stats(c, out3);
+}
+
+// ---- statistics into device memory, allocating nothing
+//
+// The partial sums live in `scratch`, a file-scope device array of a fixed
+// size: NA_BLOCKS first-level partials and STATS_FIN second-level ones, 6
+// doubles each (110.6 KB per GPU, held until the process ends).  Its
+// device address is looked up once per device with hipGetSymbolAddress
+// (the array's name is only a host-side stand-in in host code).  Every
+// partial a call reads is written by that call first, so nothing needs
+// clearing; two calls that overlap on one GPU (two streams) race on it.
+constexpr int NA_BLOCKS = 2048;
+static __device__ __align__(256) double scratch[6 * (NA_BLOCKS + STATS_FIN)];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[64] = {};
+
+static int scratch_ptr(int device, double** ptr)
+{
+    if (device < 0 || device >= 64) return fail(ASTR_E_ARG, "device id out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        // the context's device is current (set_dev)
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[device] = p;
+    }
+    *ptr = static_cast<double*>(g_scratch_dev[device]);
+    return ASTR_OK;
+}
+
+int astr_stats_device(astr_ctx* c, double* out3_dev)
+{
+    TRY(ready(c));
+    if (!out3_dev) return fail(ASTR_E_ARG, "null argument");
+    if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "statistics of three-dimensional boxes only");
+    const double nodes = (double)c->g.n[0] * (double)c->g.n[1] * (double)c->g.n[2];
+    if (nodes >= 2147483648.0)
+        return fail(ASTR_E_UNSUPPORTED,
+                    "astr_stats_device takes boxes of fewer than 2^31 nodes (32-bit node index)");
+    double* part = nullptr;
+    TRY(scratch_ptr(c->device, &part));
+    double* part2 = part + 6 * NA_BLOCKS;
+    hipLaunchKernelGGL(k_stats_gs, dim3(NA_BLOCKS), dim3(256), 0, c->stream, pt_args(c), part);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_stats_fin, dim3(STATS_FIN), dim3(256), 0, c->stream, (const double*)part,
+                       (long)NA_BLOCKS, (long)((NA_BLOCKS + STATS_FIN - 1) / STATS_FIN), part2,
+                       nodes, (double*)nullptr);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_stats_fin, dim3(1), dim3(256), 0, c->stream, (const double*)part2,
+                       (long)STATS_FIN, (long)STATS_FIN, (double*)nullptr, nodes, out3_dev);
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
 }
 
 int astr_qmax(astr_ctx* c, double* qmax5)

@@ -100,6 +100,12 @@ module astr_hip
       real(c_double), intent(out) :: out3(3)
     end function astr_stats
 
+    ! the statistics into device memory (out3_dev), queued, allocating nothing
+    integer(c_int) function astr_stats_device(ctx, out3_dev) bind(c, name='astr_stats_device')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, out3_dev
+    end function astr_stats_device
+
     ! call time_integration_rk (src/mainloop.F90:104); stats as rkfirst
     integer(c_int) function astr_rk3_step(ctx, dt, stats) bind(c, name='astr_rk3_step')
       import :: c_int, c_ptr, c_double

@@ -130,6 +130,17 @@ int astr_rk3_update(astr_ctx *ctx, int32_t stage, double dt);
 /* kenergycal, enstophycal, diss_rate_cal (src/statistic.F90:870-1042) of
  * the current device state -> out[0..2]. */
 int astr_stats(astr_ctx *ctx, double *out3);
+/* The three statistics of astr_stats, written to out3_dev (3 doubles of
+ * device memory on the context's device) by kernels queued on the
+ * context's stream; the call returns without waiting (astr_synchronize, or
+ * an event on that stream, orders a later read).  No counterpart in the
+ * reference: it is for callers that cannot allocate or wait, such as work
+ * captured into a HIP graph or a tight serving loop.  It allocates
+ * nothing: its partial sums live in a fixed device array of the library
+ * (110.6 KB per GPU, kept until the process ends) that every call on that
+ * GPU shares, so calls that overlap on one GPU (on two streams) race.
+ * Boxes of 2^31 nodes or more are refused (ASTR_E_UNSUPPORTED). */
+int astr_stats_device(astr_ctx *ctx, double *out3_dev);
 /* maxmincal's max_q (src/statistic.F90:737-763): the largest value of each
  * q(m), m = 1..5, over the nodes 0..n of the current device state ->
  * qmax[0..4]; the columns statout writes to flowstate.dat for flow types
