@@ -60,6 +60,7 @@ SIGNATURES = {
     "astr_download_shock": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint8)]),
     "astr_deriv_field": (ctypes.c_int, [_P, _I, _D, _D]),
     "astr_filter_field": (ctypes.c_int, [_P, _I, _D, _D]),
+    "astr_flux_field": (ctypes.c_int, [_P, _I, _I, _I, _D, _D]),
     "astr_set_timing": (ctypes.c_int, [_P, _I]),
     "astr_synchronize": (ctypes.c_int, [_P]),
     "astr_set_spacing": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double]),

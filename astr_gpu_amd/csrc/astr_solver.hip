@@ -17,7 +17,7 @@
 // Per RK3 stage of the central path (src/mainloop.F90:392-476) the device
 // runs:
 //   filter x,y,z (in place)           filterq      src/comsolver.F90:514
-//   qswap halo + seam average          qswap        src/parallel.F90:4849
+//   qswap i, j, k parts                qswap        src/parallel.F90:4849
 //   gradient sweeps x,y,z              gradcal      src/comsolver.F90:244
 //   k_gflux (pointwise)                sigma, qflux and the axis fluxes
 //                                      G = Fv - F of convrsdcal6/diffrsdcal6
@@ -875,141 +875,116 @@ __device__ __forceinline__ double miucal(const PtArgs& a, double t)
     return t * sqrt(t) * a.tempconst1 / (t + a.tempconst);
 }
 
-// qswap halos (src/parallel.F90:4877-5200, single rank, periodic): the
-// reference copies the i-halos before any averaging, the j-halos after the
-// i-average, the k-halos after the i- and j-averages, and refreshes the
-// primitives on every halo cell it wrote.  Each halo cell is computed here
-// from the unaveraged interior.  Reads interior only, writes halos only.
-// block bid of the launch (a batched launch numbers each member's blocks
-// from 0)
-__device__ __forceinline__ void qswap_halo_body(const PtArgs& a, unsigned bid)
+// qswap of a periodic three-dimensional box, single rank
+// (src/parallel.F90:4877-5200), one kernel per axis in the reference's
+// order:
+//   i  duplicate nodes q(0) = q(n) = (q(0) + q(n))/2 on every interior row,
+//      then the i-halos copied from the interior before any other average
+//      (src/parallel.F90:4884-4885, 4877-4915);
+//   j  the same along j, over i and k in 0..n, from the i-averaged state;
+//   k  the same along k, over i and j in 0..n, from the i- and j-averaged
+//      state.
+// Every halo cell and duplicate node written gets its primitives refreshed
+// (q2fvar); a node that is a duplicate along several axes ends with the
+// primitives of its last average.  Within one kernel the cells read (the
+// interior sources and the duplicate pairs) and the cells written are
+// disjoint or owned by the same thread, so no thread reads what another
+// writes.
+constexpr int QSX_LANES = 64;  // threads per row of the i part
+
+__device__ __forceinline__ void store_q(const PtArgs& a, long gi, const double (&qv)[5])
 {
-    const Geo& g = a.g;
-    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    // thread order: the k-halo and j-halo cells as rows of i (i fastest,
-    // so a wavefront reads and writes consecutive nodes), then the i-halo
-    // cells, 2*HM per (j,k)
-    const long ri = n0 + 1;
-    const long fk = 2L * HM * (n1 + 1) * ri;
-    const long fj = 2L * HM * (n2 + 1) * ri;
-    const long fi = 2L * HM * (n1 + 1) * (n2 + 1);
-    long p = (long)bid * blockDim.x + threadIdx.x;
-    if (p >= fi + fj + fk) return;
-    // the halo cell ci and its periodic source si: the coordinates are
-    // named one by one (an array indexed by the run-time axis would live in
-    // scratch memory)
-    int axis, h, ci0 = 0, ci1 = 0, ci2 = 0;
-    if (p < fk) {
-        axis = 2;
-        const long r = p / ri;
-        ci0 = (int)(p - r * ri);
-        ci1 = (int)(r % (n1 + 1));
-        h = (int)(r / (n1 + 1));
-    } else if ((p -= fk) < fj) {
-        axis = 1;
-        const long r = p / ri;
-        ci0 = (int)(p - r * ri);
-        h = (int)(r % (2 * HM));
-        ci2 = (int)(r / (2 * HM));
-    } else {
-        p -= fj;
-        axis = 0;
-        h = (int)(p % (2 * HM));
-        const long r = p / (2 * HM);
-        ci1 = (int)(r % (n1 + 1));
-        ci2 = (int)(r / (n1 + 1));
-    }
-    const int dimh = axis == 0 ? n0 : axis == 1 ? n1 : n2;
-    const int hnode = h < HM ? -(h + 1) : dimh + (h - HM + 1);
-    const int snode = h < HM ? dimh - (h + 1) : (h - HM + 1);
-    int si0 = ci0, si1 = ci1, si2 = ci2;
-    if (axis == 0) { ci0 = hnode; si0 = snode; }
-    else if (axis == 1) { ci1 = hnode; si1 = snode; }
-    else { ci2 = hnode; si2 = snode; }
-    double qv[5];
-    // the source value after the averages of the axes before `axis`
-    for (int m = 0; m < 5; ++m) {
-        if (axis == 0) {
-            qv[m] = a.q[m][gidx(g, si0, si1, si2)];
-        } else {
-            // x-average (only if i is a seam node), then y-average (axis 2)
-            auto xavg = [&](int j, int k) -> double {
-                const int i = si0;
-                if (i == 0 || i == n0)
-                    return 0.5 * (a.q[m][gidx(g, 0, j, k)] + a.q[m][gidx(g, n0, j, k)]);
-                return a.q[m][gidx(g, i, j, k)];
-            };
-            if (axis == 1) {
-                qv[m] = xavg(si1, si2);
-            } else {
-                const int j = si1;
-                if (j == 0 || j == n1) qv[m] = 0.5 * (xavg(0, si2) + xavg(n1, si2));
-                else qv[m] = xavg(j, si2);
-            }
-        }
-    }
-    const long gi = gidx(g, ci0, ci1, ci2);
+#pragma unroll
     for (int m = 0; m < 5; ++m) a.q[m][gi] = qv[m];
     q2fvar(a, qv, gi);
 }
 
-__global__ __launch_bounds__(256) void k_qswap_halo(PtArgs a) { qswap_halo_body(a, blockIdx.x); }
-
-// qswap duplicate-node averages q(0)=q(n)=(q(0)+q(n))/2, in the order i, j,
-// k (src/parallel.F90:4884-4885, 5029-5030, 5168-5169), and the primitive
-// refresh of every seam node.  One thread owns one group of duplicates: the
-// canonical node has coordinates in [0,n) with at least one zero.
-__device__ __forceinline__ void qswap_seam_body(const PtArgs& a, unsigned bid)
+// i part: one wavefront per interior row (j, k).  Lanes 0-15 write the
+// 128-byte line of the nodes -16..-1 (the halo -HM..-1 from n-HM..n-1, the
+// row padding before it as zeros, so the line is written whole); lane 16
+// node 0 (alone on its line: rewriting the nodes 1..15 with their own
+// values to make that line whole measured slower, 166 against 102 us per
+// launch at 256^3); lanes 32.. node n and the cells after it to the end of
+// the line of node n+HM (the halo n+1..n+HM from 1..HM, then zeros).
+__device__ __forceinline__ void qswap_x_body(const PtArgs& a, unsigned bid)
 {
     const Geo& g = a.g;
-    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    const long tot = (long)n0 * n1 * n2;
-    const long inner = (long)(n0 - 1) * (n1 - 1) * (n2 - 1);
-    long p = (long)bid * blockDim.x + threadIdx.x;
-    if (p >= tot - inner) return;
-    // enumerate canonical nodes with some coordinate zero:
-    //   i == 0: n1*n2 nodes; i > 0, j == 0: (n0-1)*n2; i,j > 0, k == 0
-    int i, j, k;
-    const long c0 = (long)n1 * n2, c1 = (long)(n0 - 1) * n2;
-    if (p < c0) { i = 0; j = (int)(p % n1); k = (int)(p / n1); }
-    else if (p < c0 + c1) { p -= c0; i = 1 + (int)(p % (n0 - 1)); j = 0; k = (int)(p / (n0 - 1)); }
-    else { p -= c0 + c1; i = 1 + (int)(p % (n0 - 1)); j = 1 + (int)(p / (n0 - 1)); k = 0; }
-    const bool sx = i == 0, sy = j == 0, sz = k == 0;
-    // member (bx,by,bz): coordinate n if the bit is set
-    for (int m = 0; m < 5; ++m) {
-        double v[2][2][2];
-        for (int bz = 0; bz <= (int)sz; ++bz)
-            for (int by = 0; by <= (int)sy; ++by)
-                for (int bx = 0; bx <= (int)sx; ++bx)
-                    v[bz][by][bx] = a.q[m][gidx(g, bx ? n0 : i, by ? n1 : j, bz ? n2 : k)];
-        if (sx)
-            for (int bz = 0; bz <= (int)sz; ++bz)
-                for (int by = 0; by <= (int)sy; ++by)
-                    v[bz][by][0] = v[bz][by][1] = 0.5 * (v[bz][by][0] + v[bz][by][1]);
-        if (sy)
-            for (int bz = 0; bz <= (int)sz; ++bz)
-                for (int bx = 0; bx <= (int)sx; ++bx)
-                    v[bz][0][bx] = v[bz][1][bx] = 0.5 * (v[bz][0][bx] + v[bz][1][bx]);
-        if (sz)
-            for (int by = 0; by <= (int)sy; ++by)
-                for (int bx = 0; bx <= (int)sx; ++bx)
-                    v[0][by][bx] = v[1][by][bx] = 0.5 * (v[0][by][bx] + v[1][by][bx]);
-        for (int bz = 0; bz <= (int)sz; ++bz)
-            for (int by = 0; by <= (int)sy; ++by)
-                for (int bx = 0; bx <= (int)sx; ++bx)
-                    a.q[m][gidx(g, bx ? n0 : i, by ? n1 : j, bz ? n2 : k)] = v[bz][by][bx];
+    const int n0 = g.n[0];
+    const unsigned t = bid * blockDim.x + threadIdx.x;
+    const unsigned row = t / QSX_LANES, lane = t % QSX_LANES;
+    const unsigned nj = (unsigned)g.n[1] + 1;
+    if (row >= nj * ((unsigned)g.n[2] + 1)) return;
+    const int k = (int)(row / nj), j = (int)(row - (unsigned)k * nj);
+    int i;
+    if (lane < 16) i = (int)lane - 16;
+    else if (lane == 16) i = 0;
+    else if (lane >= 32 && (int)lane - 32 < line_end_i(n0 + HM) - n0) i = n0 + (int)lane - 32;
+    else return;
+    const long gi = gidx(g, i, j, k);
+    double qv[5];
+    if (i == 0 || i == n0) {
+        const long g0 = gidx(g, 0, j, k), gn = gidx(g, n0, j, k);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) qv[m] = 0.5 * (a.q[m][g0] + a.q[m][gn]);
+    } else if (i >= -HM && i <= n0 + HM) {
+        const long gs = gidx(g, i < 0 ? i + n0 : i - n0, j, k);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) qv[m] = a.q[m][gs];
+    } else {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) a.q[m][gi] = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) a.prim[m][gi] = 0.0;
+        return;
     }
-    for (int bz = 0; bz <= (int)sz; ++bz)
-        for (int by = 0; by <= (int)sy; ++by)
-            for (int bx = 0; bx <= (int)sx; ++bx) {
-                const long gi = gidx(g, bx ? n0 : i, by ? n1 : j, bz ? n2 : k);
-                double qv[5];
-                for (int m = 0; m < 5; ++m) qv[m] = a.q[m][gi];
-                q2fvar(a, qv, gi);
-            }
+    store_q(a, gi, qv);
 }
 
-__global__ __launch_bounds__(256) void k_qswap_seam(PtArgs a) { qswap_seam_body(a, blockIdx.x); }
+// j part (AX = 1) or k part (AX = 2): threads over the nodes i = 0..n of the
+// rows of each outer index o (k for the j part, j for the k part) and 2*HM+1
+// slots: slot 0 the duplicate pair (the rows at 0 and n of axis AX), slots
+// 1..HM the halo rows -1..-HM from n-1..n-HM, slots HM+1..2*HM the halo rows
+// n+1..n+HM from 1..HM.
+template <int AX>
+__device__ __forceinline__ void qswap_jk_body(const PtArgs& a, unsigned bid)
+{
+    const Geo& g = a.g;
+    const unsigned ni = (unsigned)g.n[0] + 1;
+    const unsigned t = bid * blockDim.x + threadIdx.x;
+    const unsigned r = t / ni;
+    const int i = (int)(t - r * ni);
+    const unsigned o = r / (2 * HM + 1);
+    const int slot = (int)(r - o * (2 * HM + 1));
+    const int na = g.n[AX], no = AX == 1 ? g.n[2] : g.n[1];
+    if ((int)o > no) return;
+    auto at = [&](int c) { return AX == 1 ? gidx(g, i, c, (int)o) : gidx(g, i, (int)o, c); };
+    double qv[5];
+    if (slot == 0) {
+        const long g0 = at(0), gn = at(na);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) qv[m] = 0.5 * (a.q[m][g0] + a.q[m][gn]);
+        store_q(a, g0, qv);
+        store_q(a, gn, qv);
+        return;
+    }
+    const int h = slot <= HM ? -slot : na + slot - HM;  // halo row
+    const long gs = at(h < 0 ? h + na : h - na);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) qv[m] = a.q[m][gs];
+    store_q(a, at(h), qv);
+}
+
+__global__ __launch_bounds__(256) void k_qswap_x(PtArgs a) { qswap_x_body(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_qswap_y(PtArgs a) { qswap_jk_body<1>(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_qswap_z(PtArgs a) { qswap_jk_body<2>(a, blockIdx.x); }
+
+// threads of the i, j and k parts of a box
+__host__ __device__ inline long qswap_threads(const Geo& g, int axis)
+{
+    const long n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    if (axis == 0) return (long)QSX_LANES * (n1 + 1) * (n2 + 1);
+    return (n0 + 1) * (2 * HM + 1) * (axis == 1 ? n2 + 1 : n1 + 1);
+}
 
 // qswap of a two-dimensional box (km = 0), single rank: along a periodic
 // in-plane axis (src/parallel.F90:4877-4915 for i, 5019-5065 for j) the
@@ -2245,6 +2220,7 @@ struct CmpArgs {
     double* sw[10];            // F+(1:5), F-(1:5) on iss..iee
     double* fh[10];            // interface fluxes of F+ / F-, rows first..last
     const double* ac[2];       // Thomas factors (3 per row) of the + / - system
+    int comp0, ncomp;          // k_flux_compact: components comp0..comp0+ncomp-1
 };
 
 // flux_steger_warming on the nodes iss..iee of every line of one axis,
@@ -2295,11 +2271,11 @@ __global__ void k_flux_compact(CmpArgs a)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nl = (long)a.na * (a.b1 - a.b0 + 1);
-    if (p >= nl * 10) return;
+    if (p >= nl * a.ncomp) return;
     const int av = a.a0 + (int)(p % a.na);
     const long r = p / a.na;
     const int b = a.b0 + (int)(r % (a.b1 - a.b0 + 1));
-    const int comp = (int)(r / (a.b1 - a.b0 + 1));  // 0..4 F+, 5..9 F-
+    const int comp = a.comp0 + (int)(r / (a.b1 - a.b0 + 1));  // 0..4 F+, 5..9 F-
     const bool plus = comp < 5;
     int c[3];
     c[a.dir] = 0;
@@ -2354,6 +2330,41 @@ __global__ void k_flux_compact(CmpArgs a)
         nxt = *dk - ac[3 * k + 0] * nxt;
         *dk = nxt;
     }
+}
+
+struct ReconsArgs {
+    Geo g;
+    int dir, ntype, lo, hi;  // interfaces lo..hi (node i: the interface i+1/2)
+    int plus, recon, shock;
+    double bfac;
+    const double* f;
+    double* fh;
+};
+
+// recons_exp (src/flux.F90:269-350) of one field at the interfaces lo..hi
+// of every line of an axis, one thread per interface: the stencil f(i-3:i+4)
+// for the + wind, f(i+4:i-3) for the - wind (convrsduwd's iwind8 order,
+// src/solver.F90:620-650), halos as given.  ntype 4 is taken as ntype 1 on
+// the low half of the line and 2 on the high half, as in k_uwd.
+__global__ void k_recons_field(ReconsArgs a)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int d = a.dir, dim = a.g.n[d];
+    const long ni = a.hi - a.lo + 1;
+    const long na = a.g.n[d == 0 ? 1 : 0] + 1, nb = a.g.n[d == 2 ? 1 : 2] + 1;
+    if (p >= ni * na * nb) return;
+    const int i = a.lo + (int)(p % ni);
+    const long r = p / ni;
+    int c[3];
+    c[d] = 0;
+    c[d == 0 ? 1 : 0] = (int)(r % na);
+    c[d == 2 ? 1 : 2] = (int)(r / na);
+    const long st = a.g.s[d], g0 = gidx(a.g, c[0], c[1], c[2]);
+    double f8[8];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) f8[w] = a.f[g0 + (long)(a.plus ? i - 3 + w : i + 4 - w) * st];
+    const int nte = a.ntype == 4 ? (i < dim / 2 ? 1 : 2) : a.ntype;
+    a.fh[g0 + (long)i * st] = uwd::recons_exp(f8, i, dim, nte, a.recon, a.shock != 0, a.bfac);
 }
 
 // Zero qrhs on the interior (qrhs=0 before rhscal, src/mainloop.F90:404)
@@ -2775,7 +2786,8 @@ __global__ __launch_bounds__(WGT, 4) void k_sweep_q(const SweepArgs* __restrict_
 // The pointwise kernels over every member in one launch: the members' blocks
 // are numbered consecutively (bstart[k] is member k's first block), and a
 // block runs the kernel body of its member with its own block number.
-enum PtKind : int { PT_QSWAP_HALO = 0, PT_QSWAP_SEAM = 1, PT_GFLUX = 2, PT_PRIM = 3 };
+enum PtKind : int { PT_QSWAP_X = 0, PT_QSWAP_Y = 1, PT_QSWAP_Z = 2, PT_GFLUX = 3, PT_PRIM = 4,
+                    PT_KINDS = 5 };
 
 struct PtBatch {
     const PtArgs* args;
@@ -2798,8 +2810,9 @@ __global__ __launch_bounds__(256) void k_pt_batch(const PtArgs* __restrict__ mar
     const int mi = __builtin_amdgcn_readfirstlane(lo);
     const PtArgs& a = margs[mi];
     const unsigned lb = bid - bstart[mi];
-    if constexpr (K == PT_QSWAP_HALO) qswap_halo_body(a, lb);
-    else if constexpr (K == PT_QSWAP_SEAM) qswap_seam_body(a, lb);
+    if constexpr (K == PT_QSWAP_X) qswap_x_body(a, lb);
+    else if constexpr (K == PT_QSWAP_Y) qswap_jk_body<1>(a, lb);
+    else if constexpr (K == PT_QSWAP_Z) qswap_jk_body<2>(a, lb);
     else if constexpr (K == PT_GFLUX) gflux_body<false>(a, lb);
     else prim_body(a, lb);
 }
@@ -3343,11 +3356,9 @@ int do_qswap(astr_ctx* c)
         }
         return launch_pt(c, k_qswap2d_plane, (long)(n0 + 1) * (n1 + 1), a);
     }
-    const long nh = 2L * HM * ((long)(n1 + 1) * (n2 + 1) + (long)(n0 + 1) * (n2 + 1) +
-                               (long)(n0 + 1) * (n1 + 1));
-    TRY(launch_pt(c, k_qswap_halo, nh, a));
-    const long ns = (long)n0 * n1 * n2 - (long)(n0 - 1) * (n1 - 1) * (n2 - 1);
-    TRY(launch_pt(c, k_qswap_seam, ns, a));
+    TRY(launch_pt(c, k_qswap_x, qswap_threads(c->g, 0), a));
+    TRY(launch_pt(c, k_qswap_y, qswap_threads(c->g, 1), a));
+    TRY(launch_pt(c, k_qswap_z, qswap_threads(c->g, 2), a));
     return ASTR_OK;
 }
 
@@ -3580,6 +3591,8 @@ int do_upwind(astr_ctx* c)
             }
             ca.ac[0] = c->cmp_ac[d][0];
             ca.ac[1] = c->cmp_ac[d][1];
+            ca.comp0 = 0;
+            ca.ncomp = 10;
             const long nl = (long)ca.na * (ca.b1 - ca.b0 + 1);
             const long nsw = nl * (ca.iee - ca.iss + 1);
             hipLaunchKernelGGL(k_sw_field, dim3((unsigned)((nsw + 255) / 256)), dim3(256), 0,
@@ -4128,7 +4141,7 @@ struct BatchQueue {
     int* d_counters = nullptr;
     PtArgs* d_pt = nullptr;
     unsigned* d_bstart = nullptr;  // 4 kinds x (nmem + 1)
-    unsigned nblocks[4] = {0, 0, 0, 0};
+    unsigned nblocks[PT_KINDS] = {};
     int cus = 0;
     int tuned[3][LMAX + 1];        // per pass (filter, grad, flux) and L; -1 untuned
     double tune_us[3][LMAX + 1][QC_COUNT];
@@ -4315,8 +4328,7 @@ int queue_build(astr_batch* b)
     b->seq.clear();
     for (int s = 1; s <= 3; ++s) {
         for (int d = 0; d < 3; ++d) b->seq.insert(b->seq.end(), filt[d].begin(), filt[d].end());
-        b->seq.push_back(-1 - PT_QSWAP_HALO);
-        b->seq.push_back(-1 - PT_QSWAP_SEAM);
+        for (int k = PT_QSWAP_X; k <= PT_QSWAP_Z; ++k) b->seq.push_back(-1 - k);
         for (int d = 0; d < 3; ++d) b->seq.insert(b->seq.end(), grad[d].begin(), grad[d].end());
         b->seq.push_back(-1 - PT_GFLUX);
         for (int d = 0; d < 2; ++d) b->seq.insert(b->seq.end(), flux[d].begin(), flux[d].end());
@@ -4332,18 +4344,19 @@ int queue_build(astr_batch* b)
     HIPCHK(hipMalloc(&qe->d_counters, sizeof(int) * std::max(b->nqueue, 1)));
     // pointwise launches
     std::vector<PtArgs> pts(nm);
-    std::vector<unsigned> bs(4 * (nm + 1), 0);
+    std::vector<unsigned> bs(PT_KINDS * (nm + 1), 0);
     for (int i = 0; i < nm; ++i) {
         astr_ctx* c = b->ctx[i];
         pts[i] = pt_args(c);
-        const long n0 = c->g.n[0], n1 = c->g.n[1], n2 = c->g.n[2];
-        const long nh = 2L * HM * ((n1 + 1) * (n2 + 1) + (n0 + 1) * (n2 + 1) + (n0 + 1) * (n1 + 1));
-        const long ns = n0 * n1 * n2 - (n0 - 1) * (n1 - 1) * (n2 - 1);
-        const long np = (n1 + 1) * (n2 + 1) * line_end_i(c->g.n[0]);
-        const long nb[4] = {(nh + 255) / 256, (ns + 255) / 256, gflux_blocks(c->g), (np + 255) / 256};
-        for (int k = 0; k < 4; ++k) bs[k * (nm + 1) + i + 1] = bs[k * (nm + 1) + i] + (unsigned)nb[k];
+        const long np = (long)(c->g.n[1] + 1) * (c->g.n[2] + 1) * line_end_i(c->g.n[0]);
+        const long nb[PT_KINDS] = {(qswap_threads(c->g, 0) + 255) / 256,
+                                   (qswap_threads(c->g, 1) + 255) / 256,
+                                   (qswap_threads(c->g, 2) + 255) / 256, gflux_blocks(c->g),
+                                   (np + 255) / 256};
+        for (int k = 0; k < PT_KINDS; ++k)
+            bs[k * (nm + 1) + i + 1] = bs[k * (nm + 1) + i] + (unsigned)nb[k];
     }
-    for (int k = 0; k < 4; ++k) qe->nblocks[k] = bs[k * (nm + 1) + nm];
+    for (int k = 0; k < PT_KINDS; ++k) qe->nblocks[k] = bs[k * (nm + 1) + nm];
     HIPCHK(hipMalloc(&qe->d_pt, sizeof(PtArgs) * nm));
     HIPCHK(hipMemcpy(qe->d_pt, pts.data(), sizeof(PtArgs) * nm, hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&qe->d_bstart, sizeof(unsigned) * bs.size()));
@@ -4421,12 +4434,16 @@ int queue_step(astr_batch* b, std::vector<hipEvent_t>* ev)
             const PtBatch pb{qe->d_pt, qe->d_bstart + kind * (nm + 1), nm};
             const dim3 grid(qe->nblocks[kind]);
             switch (kind) {
-            case PT_QSWAP_HALO:
-                hipLaunchKernelGGL(k_pt_batch<PT_QSWAP_HALO>, grid, dim3(256), 0, b->root, pb.args,
+            case PT_QSWAP_X:
+                hipLaunchKernelGGL(k_pt_batch<PT_QSWAP_X>, grid, dim3(256), 0, b->root, pb.args,
                                    pb.bstart, pb.nmem);
                 break;
-            case PT_QSWAP_SEAM:
-                hipLaunchKernelGGL(k_pt_batch<PT_QSWAP_SEAM>, grid, dim3(256), 0, b->root, pb.args,
+            case PT_QSWAP_Y:
+                hipLaunchKernelGGL(k_pt_batch<PT_QSWAP_Y>, grid, dim3(256), 0, b->root, pb.args,
+                                   pb.bstart, pb.nmem);
+                break;
+            case PT_QSWAP_Z:
+                hipLaunchKernelGGL(k_pt_batch<PT_QSWAP_Z>, grid, dim3(256), 0, b->root, pb.args,
                                    pb.bstart, pb.nmem);
                 break;
             case PT_GFLUX:
@@ -5482,6 +5499,69 @@ int astr_deriv_field(astr_ctx* c, int32_t dir, const double* f, double* df)
 int astr_filter_field(astr_ctx* c, int32_t dir, const double* f, double* ff)
 {
     return field_op(c, dir, f, ff, PASS_FILTER_FIELD);
+}
+
+int astr_flux_field(astr_ctx* c, int32_t dir, int32_t wind, int32_t shock, const double* f,
+                    double* fh)
+{
+    if (!c || !f || !fh) return fail(ASTR_E_ARG, "null argument");
+    if (dir < 1 || dir > ndims_of(c)) return fail(ASTR_E_ARG, "dir must be 1..3 (1..2 in 2-D)");
+    if (wind != 1 && wind != -1) return fail(ASTR_E_ARG, "wind must be +1 or -1");
+    if (c->p.convection == 0)
+        return fail(ASTR_E_UNSUPPORTED, "the interface fluxes need an upwind context (convection 1 or 2)");
+    TRY(set_dev(c));
+    const int d = dir - 1;
+    const int nts[3] = {c->p.npdci, c->p.npdcj, c->p.npdck};
+    std::vector<double> stage;
+    TRY(upload_field(c, c->scratch, f, stage));
+    HIPCHK(hipMemsetAsync(c->scratch2, 0, sizeof(double) * c->nfield, c->stream));
+    const int da = d == 0 ? 1 : 0, db = d == 2 ? 1 : 2;
+    if (c->p.convection == 2) {
+        CmpArgs ca{};
+        ca.g = c->g;
+        ca.dir = d;
+        ca.ntype = nts[d];
+        ca.ndims = ndims_of(c);
+        ca.a0 = 0;
+        ca.a1 = c->g.n[da];
+        ca.b0 = 0;
+        ca.b1 = c->g.n[db];
+        ca.na = ca.a1 + 1;
+        ca.first = c->cmp_first[d];
+        ca.last = c->cmp_last[d];
+        ca.bfac = c->p.bfacmpld;
+        ca.comp0 = wind > 0 ? 0 : 5;
+        ca.ncomp = 1;
+        ca.sw[ca.comp0] = c->scratch;
+        ca.fh[ca.comp0] = c->scratch2;
+        ca.ac[0] = c->cmp_ac[d][0];
+        ca.ac[1] = c->cmp_ac[d][1];
+        const long nl = (long)ca.na * (ca.b1 + 1);
+        hipLaunchKernelGGL(k_flux_compact, dim3((unsigned)((nl + 127) / 128)), dim3(128), 0,
+                           c->stream, ca);
+    } else {
+        ReconsArgs ra{};
+        ra.g = c->g;
+        ra.dir = d;
+        ra.ntype = nts[d];
+        // convrsduwd's interfaces is-1..ie, with is = 1 at a bounded low end
+        // and ie = n-1 at a bounded high end (src/parallel.F90:1043-1160)
+        ra.lo = (nts[d] == 1 || nts[d] == 4) ? 0 : -1;
+        ra.hi = (nts[d] == 2 || nts[d] == 4) ? c->g.n[d] - 1 : c->g.n[d];
+        ra.plus = wind > 0;
+        ra.recon = c->p.recon_schem;
+        ra.shock = shock != 0;
+        ra.bfac = c->p.bfacmpld;
+        ra.f = c->scratch;
+        ra.fh = c->scratch2;
+        const long n = (long)(ra.hi - ra.lo + 1) * (c->g.n[da] + 1) * (c->g.n[db] + 1);
+        hipLaunchKernelGGL(k_recons_field, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           c->stream, ra);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    TRY(download_field(c, c->scratch2, fh, stage, false));
+    return ASTR_OK;
 }
 
 int astr_adjoint_record(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt, double* J)

@@ -153,6 +153,17 @@ module astr_hip
       real(c_double), intent(out) :: ff(*)
     end function astr_filter_field
 
+    ! upwind interface values of one split-flux field (flux_compact or
+    ! recons_exp, by the context's convection), wind +1 / -1
+    integer(c_int) function astr_flux_field(ctx, dir, wind, shock, f, fh) &
+        bind(c, name='astr_flux_field')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: dir, wind, shock
+      real(c_double), intent(in) :: f(*)
+      real(c_double), intent(out) :: fh(*)
+    end function astr_flux_field
+
     ! transpose of the derivative (op 0) or filter (op 1) along dir
     integer(c_int) function astr_line_adjoint(ctx, op, dir, wrap, gbar, fbar) &
         bind(c, name='astr_line_adjoint')

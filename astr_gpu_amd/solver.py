@@ -252,6 +252,17 @@ class AstrSolver:
               "astr_filter_field")
         return ff
 
+    def interface_flux(self, direction: int, f, wind: int = 1, shock: bool = False) -> np.ndarray:
+        """Upwind interface values of split-flux field f along `direction`:
+        flux_compact (convection 2) or recons_exp (convection 1) of every
+        line; haloed result, entry i the interface i+1/2 (zero where no
+        interface is computed)."""
+        f = self._f(f, haloed_shape(self.p))
+        fh = np.zeros(haloed_shape(self.p), order="F")
+        check(self.lib.astr_flux_field(self.ctx, int(direction), int(wind), int(bool(shock)),
+                                       dptr(f), dptr(fh)), "astr_flux_field")
+        return fh
+
 
 # -------------------------------------------------------------- set-up data
 

@@ -182,6 +182,19 @@ int astr_deriv_field(astr_ctx *ctx, int32_t dir, const double *f, double *df);
 /* compact_filter along axis dir for every line (src/filter.F90:112-285):
  * f(-hm:...) haloed in, ff(0:im,0:jm,0:km) out.  Halos used as given. */
 int astr_filter_field(astr_ctx *ctx, int32_t dir, const double *f, double *ff);
+/* The upwind interface values of one split-flux field along axis dir for
+ * every line: f(-hm:...) haloed in (halos used as given), fh(-hm:...)
+ * haloed out, fh(i) the value at the interface i+1/2, zero elsewhere.
+ * wind +1 reconstructs F+ (stencil f(i-3:i+4)), -1 F- (f(i+4:i-3)).
+ *   convection 2: flux_compact (src/flux.F90:125-256), the 543c compact
+ *     upwind system of the context's closure, rows -1..dim (0.. at a
+ *     bounded low end, ..dim-1 at a bounded high end);
+ *   convection 1: recons_exp (src/flux.F90:269-350) with the context's
+ *     recon_schem at the interfaces is-1..ie of convrsduwd
+ *     (src/solver.F90:549-700); shock (0/1) is the lshock flag MP-LD reads.
+ * ASTR_E_UNSUPPORTED for a central context (convection 0). */
+int astr_flux_field(astr_ctx *ctx, int32_t dir, int32_t wind, int32_t shock, const double *f,
+                    double *fh);
 /* The transpose of one of the two operators above (op 0: the compact
  * derivative, 1: the filter) along dir: gbar(-hm:..) holds the adjoint of
  * the output on 0..im etc. (halo entries are ignored), fbar(-hm:..)

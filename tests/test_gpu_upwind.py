@@ -247,3 +247,59 @@ def test_upwind_periodic_steps_conserve_and_match_pieces(convection):
     assert np.all(np.isfinite(t1))
     s1.close()
     s2.close()
+
+
+def _lines(f, p, ax):
+    """(a, b, line) over the interior lines of axis ax of a haloed field."""
+    g = np.moveaxis(f, ax, 0)
+    for a in range(HM, g.shape[1] - HM):
+        for b in range(HM, g.shape[2] - HM):
+            yield a, b, np.ascontiguousarray(g[:, a, b])
+
+
+@pytest.mark.parametrize("ntype", [1, 2, 3, 4])
+@pytest.mark.parametrize("scheme", ["compact", 0, 2, 5, 6])
+def test_interface_flux_field_matches_checker(ntype, scheme):
+    """astr_flux_field against the checker's flux_compact / recons_exp on
+    every line of every axis, both winds: bit-equal."""
+    dims = (24, 20, 18)
+    compact = scheme == "compact"
+    p = ag.default_params(dims, npdci=ntype, npdcj=ntype, npdck=ntype,
+                          convection=2 if compact else 1, recon_schem=0 if compact else scheme,
+                          lchardecomp=0, lfilter=0, diffterm=0, mach=1.0)
+    s = ag.AstrSolver(p)
+    rng = np.random.default_rng(300 + ntype)
+    # a smooth part and a jump, so that the limiters act
+    f = rng.standard_normal(ag.haloed_shape(p)) * 0.1
+    f[ag.haloed_shape(p)[0] // 2:, :, :] += 1.0
+    f = np.asfortranarray(f)
+    for d in (1, 2, 3):
+        ax = d - 1
+        dim = dims[ax]
+        for wind in (1, -1):
+            shock = wind > 0
+            fh = np.moveaxis(s.interface_flux(d, f, wind=wind, shock=shock), ax, 0)
+            for a, b, line in _lines(f, p, ax):
+                got = fh[HM - 1:HM + dim + 1, a, b]  # nodes -1..dim
+                if compact:
+                    ref = po.flux_compact(ntype, line, plus=wind > 0, bfac=p.bfacmpld)
+                    assert np.array_equal(got, ref), (d, wind, a, b, nerr(got, ref))
+                    continue
+                lo = 0 if ntype in (1, 4) else -1
+                hi = dim - 1 if ntype in (2, 4) else dim
+                for i in range(lo, hi + 1):
+                    n = i + HM
+                    f8 = line[n - 3:n + 5] if wind > 0 else line[n - 3:n + 5][::-1]
+                    nte = (1 if i < dim // 2 else 2) if ntype == 4 else ntype
+                    ref = po.recons_exp(f8, i, dim, nte, scheme, shock, p.bfacmpld)
+                    assert got[i + 1] == ref, (d, wind, a, b, i, got[i + 1], ref)
+    s.close()
+
+
+def test_interface_flux_field_rejects_central_context():
+    p = ag.default_params((16, 16, 16))
+    s = ag.AstrSolver(p)
+    f = np.zeros(ag.haloed_shape(p), order="F")
+    with pytest.raises(ag.AstrError):
+        s.interface_flux(1, f)
+    s.close()
