@@ -2390,8 +2390,13 @@ __global__ void k_zero_qrhs(UwdArgs a)
 //     (negated on the last axis: qrhs=-qrhs, src/solver.F90:243).
 // x tiles are 1 line x 256 interfaces; y/z tiles 16 lines (consecutive i,
 // 128-byte rows) x 16 interfaces.
-template <bool XD, bool CMP>
-__global__ __launch_bounds__(256) void k_uwd(UwdArgs a)
+// LD: the explicit path with MP7-LD (reschem 5), whose reconstruction needs
+// 236 registers (2 waves per SIMD).  The other explicit schemes run with 4
+// waves per SIMD (128 registers; the dozen the compiler then keeps in
+// scratch cost less than the waves gained: MP7 at 2048^2 from 445 to 314 us
+// per launch).
+template <bool XD, bool CMP, bool LD = false>
+__global__ __launch_bounds__(256, CMP || LD ? 1 : 4) void k_uwd(UwdArgs a)
 {
     constexpr int TL = XD ? 1 : 16;
     constexpr int S = XD ? 256 : 16;
@@ -2510,26 +2515,41 @@ __global__ __launch_bounds__(256) void k_uwd(UwdArgs a)
         // closure at a rank interface)
         const int nte = a.ntype == 4 ? (ii < dim / 2 ? 1 : 2) : a.ntype;
         for (int m = 0; m < 5; ++m) {
-            double fp[8], fm[8];
-            if (chr) {
-                double lv[5];
-                uwd::lev_row(r, m, lv);
+            // the F+ stencil and its reconstruction, then F-'s: the empty
+            // asm keeps the second's LDS reads after the first's result, so
+            // that the two stencils are not live at once
+            double lv[5];
+            if (chr) uwd::lev_row(r, m, lv);
+            double v1, v2;
+            {
+                double fp[8];
+                if (chr) {
 #pragma unroll
-                for (int w = 0; w < 8; ++w) {
-                    const double* P = base + (s + w) * 10;      // node ii+w-3
-                    const double* M = base + (s + 7 - w) * 10 + 5;  // node ii+4-w
-                    fp[w] = lv[0] * P[0] + lv[1] * P[1] + lv[2] * P[2] + lv[3] * P[3] + lv[4] * P[4];
-                    fm[w] = lv[0] * M[0] + lv[1] * M[1] + lv[2] * M[2] + lv[3] * M[3] + lv[4] * M[4];
-                }
-            } else {
+                    for (int w = 0; w < 8; ++w) {
+                        const double* P = base + (s + w) * 10;  // node ii+w-3
+                        fp[w] = lv[0] * P[0] + lv[1] * P[1] + lv[2] * P[2] + lv[3] * P[3] + lv[4] * P[4];
+                    }
+                } else {
 #pragma unroll
-                for (int w = 0; w < 8; ++w) {
-                    fp[w] = base[(s + w) * 10 + m];
-                    fm[w] = base[(s + 7 - w) * 10 + 5 + m];
+                    for (int w = 0; w < 8; ++w) fp[w] = base[(s + w) * 10 + m];
                 }
+                v1 = uwd::recons_exp<LD>(fp, ii, dim, nte, a.recon, lsh, a.bfac);
             }
-            const double v1 = uwd::recons_exp(fp, ii, dim, nte, a.recon, lsh, a.bfac);
-            const double v2 = uwd::recons_exp(fm, ii, dim, nte, a.recon, lsh, a.bfac);
+            asm volatile("" : "+v"(v1) : : "memory");
+            {
+                double fm[8];
+                if (chr) {
+#pragma unroll
+                    for (int w = 0; w < 8; ++w) {
+                        const double* M = base + (s + 7 - w) * 10 + 5;  // node ii+4-w
+                        fm[w] = lv[0] * M[0] + lv[1] * M[1] + lv[2] * M[2] + lv[3] * M[3] + lv[4] * M[4];
+                    }
+                } else {
+#pragma unroll
+                    for (int w = 0; w < 8; ++w) fm[w] = base[(s + 7 - w) * 10 + 5 + m];
+                }
+                v2 = uwd::recons_exp<LD>(fm, ii, dim, nte, a.recon, lsh, a.bfac);
+            }
             fhc[m] = v1 + v2;
         }
         }
@@ -3609,8 +3629,13 @@ int do_upwind(astr_ctx* c)
             else
                 hipLaunchKernelGGL((k_uwd<false, true>), dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
         } else {
-            if (d == 0)
+            const bool ld = c->p.recon_schem == 5;
+            if (d == 0 && ld)
+                hipLaunchKernelGGL((k_uwd<true, false, true>), dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+            else if (d == 0)
                 hipLaunchKernelGGL((k_uwd<true, false>), dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
+            else if (ld)
+                hipLaunchKernelGGL((k_uwd<false, false, true>), dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
             else
                 hipLaunchKernelGGL((k_uwd<false, false>), dim3((unsigned)nblk), dim3(256), 0, c->stream, a);
         }
@@ -4532,9 +4557,16 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
         return fail(ASTR_E_UNSUPPORTED,
                     "two-dimensional boxes need an upwind convection, deriv_explicit 1 and npdck 3");
     // the compact line sweeps (filter, compact derivatives, central fluxes)
-    // take axes of up to 473 cells; an upwind context without the filter
-    // and with the explicit gradient never runs them
+    // take axes of up to 32*LMAX-7 = 505 cells, with an LDS tile that fits
+    // the device's per-workgroup limit (160 KB on gfx950, where the y/z
+    // tiles of the longest lines take 69 KB); an upwind context without the
+    // filter and with the explicit gradient never runs them
     const bool need_lines = p->convection == 0 || p->lfilter || !p->deriv_explicit;
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+            hipSuccess ||
+        lds_max <= 0)
+        return fail(ASTR_E_HIP, "cannot query the device's LDS per workgroup");
     bool lines_ok[3] = {false, false, false};
     for (int d = 0; d < 3; ++d) {
         if (d == 2 && two_d) continue;
@@ -4542,9 +4574,11 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
         if (nts[d] < 1 || nts[d] > 4) return fail(ASTR_E_ARG, "npdc must be 1..4");
         if (dims[d] > 1000000) return fail(ASTR_E_UNSUPPORTED, "axis longer than 10^6 cells");
         const int L = (dims[d] + 7 + LPL - 1) / LPL;
-        lines_ok[d] = L <= LMAX && lds_bytes(0, L) <= 65536 && lds_bytes(1, L) <= 65536;
+        lines_ok[d] = L <= LMAX && lds_bytes(0, L) <= (size_t)lds_max &&
+                      lds_bytes(1, L) <= (size_t)lds_max;
         if (!lines_ok[d] && need_lines)
-            return fail(ASTR_E_UNSUPPORTED, "axis longer than 473 cells for the compact line operators");
+            return fail(ASTR_E_UNSUPPORTED, "axis longer than the compact line operators take "
+                                            "(505 cells, or the device's LDS per workgroup)");
     }
     auto* c = new astr_ctx();
     c->p = *p;

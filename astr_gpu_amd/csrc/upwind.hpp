@@ -217,7 +217,11 @@ __device__ double round3(const double* u)
     return gc * (u[2] - u[0]) + u[0];
 }
 
-// recons_exp, src/flux.F90:269-350 (schemes checked at set-up)
+// recons_exp, src/flux.F90:269-350 (schemes checked at set-up).  LD false
+// leaves out the interior MP7-LD branch (reschem 5), whose registers would
+// otherwise set every scheme's occupancy: a caller with reschem 5 takes
+// LD true.
+template <bool LD = true>
 __device__ double recons_exp(const double* f, int inode, int dim, int ntype, int reschem,
                              bool shock, double bfac)
 {
@@ -242,7 +246,9 @@ __device__ double recons_exp(const double* f, int inode, int dim, int ntype, int
     case 1: return weno7(f, false);
     case 2: return weno7(f, true);
     case 3: return mp7(f);
-    case 5: return mp7ld(f, bfac, shock);
+    case 5:
+        if constexpr (LD) return mp7ld(f, bfac, shock);
+        else return __builtin_nan("");  // not reached: reschem 5 runs with LD true
     default: return round3(f + 2);
     }
 }

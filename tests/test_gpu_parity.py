@@ -79,6 +79,34 @@ def test_field_line_operators_long_lines():
     s.close()
 
 
+@pytest.mark.parametrize("ntype", [3, 4])
+def test_field_line_operators_longest_lines(ntype):
+    """505-cell y and z lines (16 rows per lane): the y/z tiles take 69 KB of
+    LDS, beyond the 64 KB of earlier GPUs; against the checker line by line."""
+    dims = (16, 505, 505) if ntype == 3 else (16, 480, 16)
+    p = ag.default_params(dims, npdci=3, npdcj=ntype, npdck=ntype)
+    s = ag.AstrSolver(p)
+    rng = np.random.default_rng(7 + ntype)
+    f = np.asfortranarray(rng.standard_normal(ag.haloed_shape(p)))
+    for d in ((2, 3) if ntype == 3 else (2,)):
+        df = s.central(d, f)
+        ref = lines_ref(lambda ln: pyoracle.df_compact(ntype, ln), f, d)
+        e = nerr(df, ref)
+        print(f"deriv ntype {ntype} dir {d} n {dims[d - 1]}: {e:.2e}", flush=True)
+        assert e < 1e-12
+        ff = s.compact_filter(d, f)
+        ref = lines_ref(lambda ln: pyoracle.compact_filter(ntype, ln), f, d)
+        e = nerr(ff, ref)
+        print(f"filter ntype {ntype} dir {d} n {dims[d - 1]}: {e:.2e}", flush=True)
+        assert e < 1e-12
+    s.close()
+
+
+def test_axis_beyond_the_line_operators_is_rejected():
+    with pytest.raises(ag.AstrError):
+        ag.AstrSolver(ag.default_params((16, 506, 16)))
+
+
 def make_pair(dims):
     ob = pyoracle.OracleBox(dims, variant=0)
     ob.init_tgv()

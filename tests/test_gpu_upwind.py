@@ -156,7 +156,7 @@ def test_upwind_rhs_two_dimensional_quadrants():
 @pytest.mark.parametrize("compact", [False, True])
 def test_upwind_long_lines_two_dimensional(compact):
     """x lines longer than one 256-interface tile (several segments per line)
-    and an axis beyond the compact line sweeps' 473-cell limit."""
+    and an axis beyond the compact line sweeps' 505-cell limit."""
     dims = (600, 40, 0)
     print(f"case long 2-D dims={dims} compact={compact}", flush=True)
     g, ref, lsh = run_both(dims, (1, 3, 3), 3, True, compact=compact)
