@@ -4562,11 +4562,6 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
     // tiles of the longest lines take 69 KB); an upwind context without the
     // filter and with the explicit gradient never runs them
     const bool need_lines = p->convection == 0 || p->lfilter || !p->deriv_explicit;
-    int lds_max = 0;
-    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
-            hipSuccess ||
-        lds_max <= 0)
-        return fail(ASTR_E_HIP, "cannot query the device's LDS per workgroup");
     bool lines_ok[3] = {false, false, false};
     for (int d = 0; d < 3; ++d) {
         if (d == 2 && two_d) continue;
@@ -4574,11 +4569,25 @@ int astr_create(const astr_params* p, int32_t device, astr_ctx** out)
         if (nts[d] < 1 || nts[d] > 4) return fail(ASTR_E_ARG, "npdc must be 1..4");
         if (dims[d] > 1000000) return fail(ASTR_E_UNSUPPORTED, "axis longer than 10^6 cells");
         const int L = (dims[d] + 7 + LPL - 1) / LPL;
-        lines_ok[d] = L <= LMAX && lds_bytes(0, L) <= (size_t)lds_max &&
-                      lds_bytes(1, L) <= (size_t)lds_max;
+        lines_ok[d] = L <= LMAX;
         if (!lines_ok[d] && need_lines)
             return fail(ASTR_E_UNSUPPORTED, "axis longer than the compact line operators take "
-                                            "(505 cells, or the device's LDS per workgroup)");
+                                            "(505 cells)");
+    }
+    // the argument checks above touch no GPU; the LDS tiles of the line
+    // sweeps must also fit the device's limit per workgroup
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+            hipSuccess ||
+        lds_max <= 0)
+        return fail(ASTR_E_HIP, "cannot query the device's LDS per workgroup");
+    for (int d = 0; d < 3; ++d) {
+        if (!lines_ok[d]) continue;
+        const int L = (dims[d] + 7 + LPL - 1) / LPL;
+        lines_ok[d] = lds_bytes(0, L) <= (size_t)lds_max && lds_bytes(1, L) <= (size_t)lds_max;
+        if (!lines_ok[d] && need_lines)
+            return fail(ASTR_E_UNSUPPORTED,
+                        "the line sweeps' LDS tiles exceed the device's limit per workgroup");
     }
     auto* c = new astr_ctx();
     c->p = *p;
