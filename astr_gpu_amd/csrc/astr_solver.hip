@@ -469,7 +469,10 @@ __device__ __forceinline__ void sweep_tile(const A& args, const A& args_w, int t
             const unsigned int od = lob + (unsigned int)node * sdb;
             double v;
             if constexpr (PASS == PASS_FILTER) {
-                v = ldb(a.src, lob_s + (unsigned int)wrap_node(node, dim) * sdb);
+                // filterq's dataswap: a periodic line reads its halos as
+                // copies of its nodes; a bounded line's closures read none
+                const int nd = tab.ntype == 3 ? wrap_node(node, dim) : node;
+                v = ldb(a.src, lob_s + (unsigned int)nd * sdb);
             } else if constexpr (PASS == PASS_GRAD) {
                 v = ldb(a.src, od);
             } else if constexpr (PASS == PASS_RHS) {
@@ -825,6 +828,7 @@ struct PtArgs {
     const double* xi[3];    // metric along each axis, xi[n+HM]
     int visc;               // include the viscous fluxes (diffterm)
     int mini;               // mini-app variant (tgvsolver.F90) flux forms
+    int per[3];             // axis d periodic (npdc 3): k_gflux writes its halo G
     const double* jac;
     double const1, const2, const5, const6;
     double reynolds, prandtl, tempconst, tempconst1;
@@ -1282,7 +1286,7 @@ __device__ __forceinline__ void gflux_cell(const PtArgs& a, int i, int j, int k,
         }
         return;
     }
-    if (i >= -HM && i <= n0 + HM) gflux_strip(a, c, 0, gi);
+    if (a.per[0] && i >= -HM && i <= n0 + HM) gflux_strip(a, c, 0, gi);
     else gflux_zero(a, 0, gi);
     if (i > n0) {
         gflux_zero(a, 1, gi);
@@ -1320,8 +1324,9 @@ __device__ __forceinline__ void gflux_body(const PtArgs& a, unsigned bid)
     if (row < nj * (unsigned)(a.g.n[2] + 1)) {
         const unsigned kk = row / nj;
         const int j = (int)(row - kk * nj), k = (int)kk;
-        gflux_cell<STATS>(a, (int)(t - row * rl) - 16, j, k, halo_of_image(j, a.g.n[1]),
-                          halo_of_image(k, a.g.n[2]), st);
+        gflux_cell<STATS>(a, (int)(t - row * rl) - 16, j, k,
+                          a.per[1] ? halo_of_image(j, a.g.n[1]) : NO_HALO,
+                          a.per[2] ? halo_of_image(k, a.g.n[2]) : NO_HALO, st);
     }
     if (STATS) stats_block_sum(st[0], st[1], st[2], a.part);
 }
@@ -3153,6 +3158,9 @@ PtArgs pt_args(astr_ctx* c)
     for (int d = 0; d < 3; ++d) a.xi[d] = c->xi + d * c->xi_stride;
     a.visc = c->p.diffterm;
     a.mini = c->p.variant == 1;
+    a.per[0] = c->p.npdci == 3;
+    a.per[1] = c->p.npdcj == 3;
+    a.per[2] = c->p.npdck == 3;
     for (int m = 0; m < 9; ++m) a.dvel[m] = c->dvel[m];
     for (int m = 0; m < 3; ++m) a.dtmp[m] = c->dtmp[m];
     a.jac = c->jac;
@@ -3376,9 +3384,11 @@ int do_qswap(astr_ctx* c)
         }
         return launch_pt(c, k_qswap2d_plane, (long)(n0 + 1) * (n1 + 1), a);
     }
-    TRY(launch_pt(c, k_qswap_x, qswap_threads(c->g, 0), a));
-    TRY(launch_pt(c, k_qswap_y, qswap_threads(c->g, 1), a));
-    TRY(launch_pt(c, k_qswap_z, qswap_threads(c->g, 2), a));
+    // each part runs on a periodic axis only (the single-rank branches of
+    // qswap copy nothing on a bounded one, src/parallel.F90:4877-4915)
+    if (c->p.npdci == 3) TRY(launch_pt(c, k_qswap_x, qswap_threads(c->g, 0), a));
+    if (c->p.npdcj == 3) TRY(launch_pt(c, k_qswap_y, qswap_threads(c->g, 1), a));
+    if (c->p.npdck == 3) TRY(launch_pt(c, k_qswap_z, qswap_threads(c->g, 2), a));
     return ASTR_OK;
 }
 
@@ -4875,7 +4885,13 @@ static int ready_state(astr_ctx* c)
     return set_dev(c);
 }
 
-// stage pieces and steps: three-dimensional periodic boxes
+bool all_periodic(const astr_ctx* c)
+{
+    return c->p.npdci == 3 && c->p.npdcj == 3 && c->p.npdck == 3;
+}
+
+// stage pieces and steps: three-dimensional boxes, bounded axes on the
+// central path of the main variant
 static int ready(astr_ctx* c)
 {
     TRY(ready_state(c));
@@ -4888,10 +4904,17 @@ static int ready(astr_ctx* c)
             return fail(ASTR_E_UNSUPPORTED, "two-dimensional boxes step without the filter (lfilter 0)");
         return ASTR_OK;
     }
+    if (!all_periodic(c) && (c->p.convection != 0 || c->p.variant != 0))
+        return fail(ASTR_E_UNSUPPORTED, "bounded axes of a three-dimensional box step on the "
+                                        "central path of the main variant only");
+    // a single box's axis is periodic (3) or bounded at both ends (4); npdc
+    // 1 and 2 are a rank's share of an axis split over several ranks, whose
+    // interface halos a neighbour fills (src/parallel.F90:1044-1110)
     for (int d = 0; d < 3; ++d) {
         const int nt = d == 0 ? c->p.npdci : d == 1 ? c->p.npdcj : c->p.npdck;
-        if (nt != 3)
-            return fail(ASTR_E_UNSUPPORTED, "three-dimensional boxes step with periodic axes only");
+        if (nt != 3 && nt != 4)
+            return fail(ASTR_E_UNSUPPORTED, "a single box steps with npdc 3 (periodic) or 4 "
+                                            "(bounded at both ends) on each axis");
     }
     return ASTR_OK;
 }
@@ -4902,6 +4925,7 @@ static int ready_batch(astr_ctx* c)
 {
     TRY(ready(c));
     if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "batched members are three-dimensional boxes");
+    if (!all_periodic(c)) return fail(ASTR_E_UNSUPPORTED, "batched members are periodic boxes");
     if (c->udf) return fail(ASTR_E_UNSUPPORTED, "batched members take no user boundary condition");
     return ASTR_OK;
 }
@@ -5612,7 +5636,7 @@ int astr_adjoint_record(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt, do
     TRY(ready(c));
     if (!J) return fail(ASTR_E_ARG, "null argument");
     if (nsteps < 0 || ckpt < 1) return fail(ASTR_E_ARG, "nsteps >= 0 and ckpt >= 1 required");
-    if (c->p.convection != 0 || c->p.variant != 0 || c->g.n[2] == 0)
+    if (c->p.convection != 0 || c->p.variant != 0 || c->g.n[2] == 0 || !all_periodic(c))
         return fail(ASTR_E_UNSUPPORTED, "the adjoint is built for the central path on periodic 3-D boxes");
     if (c->udf) return fail(ASTR_E_UNSUPPORTED, "the adjoint takes no user boundary condition");
     c->rec_nsteps = -1;

@@ -450,10 +450,11 @@ void orc_compact_filter(int ntype, int dim, double alfa, double beta_halo,
     free(s.ac);
 }
 
-/* ======================= periodic box ======================= */
+/* ======================= box (periodic, or bounded on chosen axes) ======================= */
 
 struct orc_box {
     int im, jm, km, variant;
+    int nt[3]; /* npdc{i,j,k}: 3 periodic, 1/2/4 bounded (single rank) */
     int ni, nj, nk;
     size_t n3;
     double *q[5], *rho, *vel[3], *prs, *tmp;
@@ -516,6 +517,7 @@ orc_box *orc_box_new(int im, int jm, int km, int variant, double reynolds,
 
     int dims[3] = {im, jm, km};
     for (int d = 0; d < 3; ++d) {
+        b->nt[d] = 3;
         scheme_fd(&b->fd[d], 3, dims[d]);
         scheme_filter(&b->fl[d], 3, dims[d], alfa);
     }
@@ -534,6 +536,28 @@ orc_box *orc_box_new(int im, int jm, int km, int variant, double reynolds,
 }
 
 static void free_scheme(scheme_t *s) { free(s->ac); }
+
+/* The closure of each axis (npdc{i,j,k}, src/parallel.F90:1044-1249 on a
+ * single rank): 3 periodic, 1/2/4 a boundary at 0 / at n / at both.  On a
+ * bounded axis qswap and dataswap copy nothing (their single-rank branches
+ * run only when the axis is periodic, lihomo/ljhomo/lkhomo:
+ * src/parallel.F90:4877-4915, 4170-4175), and the line operators take the
+ * axis's boundary closures.  Main variant only; returns -1 otherwise. */
+int orc_box_set_ntype(orc_box *b, int n0, int n1, int n2)
+{
+    if (b->variant != 0) return -1;
+    int nt[3] = {n0, n1, n2}, dims[3] = {b->im, b->jm, b->km};
+    for (int d = 0; d < 3; ++d)
+        if (nt[d] < 1 || nt[d] > 4) return -1;
+    for (int d = 0; d < 3; ++d) {
+        free_scheme(&b->fd[d]);
+        free_scheme(&b->fl[d]);
+        b->nt[d] = nt[d];
+        scheme_fd(&b->fd[d], nt[d], dims[d]);
+        scheme_filter(&b->fl[d], nt[d], dims[d], b->alfa);
+    }
+    return 0;
+}
 
 void orc_box_free(orc_box *b)
 {
@@ -642,7 +666,8 @@ static void halo_copy(const orc_box *b, double *a, int dir)
  * in place. */
 static void filterq_dir(orc_box *b, int dir)
 {
-    for (int m = 0; m < 5; ++m) halo_copy(b, b->q[m], dir);
+    if (b->nt[dir] == 3)
+        for (int m = 0; m < 5; ++m) halo_copy(b, b->q[m], dir);
     int dim = dimof(b, dir);
     long nl = nlines(b, dir);
     const scheme_t *s = &b->fl[dir];
@@ -682,6 +707,7 @@ void orc_box_qswap(orc_box *b)
 {
     int im = b->im, jm = b->jm, km = b->km;
     /* i direction */
+    if (b->nt[0] == 3) {
     for (int m = 0; m < 5; ++m) {
         double *a = b->q[m];
         halo_copy(b, a, 0);
@@ -696,7 +722,9 @@ void orc_box_qswap(orc_box *b)
             for (int i = im; i <= im + HM; ++i) q2fvar_pt(b, IX(b, i, j, k));
             for (int i = -HM; i <= 0; ++i) q2fvar_pt(b, IX(b, i, j, k));
         }
+    }
     /* j direction */
+    if (b->nt[1] == 3) {
     for (int m = 0; m < 5; ++m) {
         double *a = b->q[m];
         halo_copy(b, a, 1);
@@ -711,7 +739,9 @@ void orc_box_qswap(orc_box *b)
             for (int j = -HM; j <= 0; ++j) q2fvar_pt(b, IX(b, i, j, k));
             for (int j = jm; j <= jm + HM; ++j) q2fvar_pt(b, IX(b, i, j, k));
         }
+    }
     /* k direction */
+    if (b->nt[2] == 3) {
     for (int m = 0; m < 5; ++m) {
         double *a = b->q[m];
         halo_copy(b, a, 2);
@@ -726,6 +756,7 @@ void orc_box_qswap(orc_box *b)
             for (int k = -HM; k <= 0; ++k) q2fvar_pt(b, IX(b, i, j, k));
             for (int k = km; k <= km + HM; ++k) q2fvar_pt(b, IX(b, i, j, k));
         }
+    }
 }
 
 /* gradcal, src/comsolver.F90:244-497 (no species, no k-omega) */
@@ -846,9 +877,11 @@ static void diffrsdcal6(orc_box *b)
                 b->qflux[2][p] = hcc * b->dtmp[2][p] + s3 * u + s5 * v + s6 * w;
             }
     for (int d = 0; d < 3; ++d) {
+        if (b->nt[d] != 3) continue;
         for (int m = 0; m < 6; ++m) halo_copy(b, b->sigma[m], d);
     }
     for (int d = 0; d < 3; ++d) {
+        if (b->nt[d] != 3) continue;
         for (int m = 0; m < 3; ++m) halo_copy(b, b->qflux[m], d);
     }
     /* flux components per direction: rows of sigma used with dxi(d,1..3) */

@@ -75,6 +75,8 @@ void orc_box_free(orc_box *b);
  * tgvsolver.F90:133) and the filter's boundary beta (0.98 main, 1.09 mini)
  * are set by the variant; they can be overridden. */
 void orc_box_set_sutherland(orc_box *b, double s_over_tref_numerator);
+/* closure per axis (3 periodic, 1/2/4 bounded); main variant; 0 or -1 */
+int orc_box_set_ntype(orc_box *b, int n0, int n1, int n2);
 
 /* Set the Taylor-Green initial field: tgvini (src/initialisation.F90:1079-1149)
  * followed by updateq (src/initialisation.F90:30-162 / src/fludyna.F90:254)

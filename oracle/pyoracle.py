@@ -52,6 +52,8 @@ def oracle_lib():
         lib.orc_box_get_deriv.argtypes = [_V, _I, _D]
         lib.orc_box_get_haloed.argtypes = [_V, _I, _D]
         lib.orc_box_set_sutherland.argtypes = [_V, _F]
+        lib.orc_box_set_ntype.restype = _I
+        lib.orc_box_set_ntype.argtypes = [_V, _I, _I, _I]
         lib.orc_df_compact.argtypes = [_I, _I, _D, _D]
         lib.orc_diff6ec.argtypes = [_I, _I, _D, _D]
         lib.orc_compact_filter.argtypes = [_I, _I, _F, _F, _F, _D, _D]
@@ -162,18 +164,21 @@ FIELDS = {"q1": 0, "q2": 1, "q3": 2, "q4": 3, "q5": 4, "rho": 5, "u": 6, "v": 7,
 
 
 class OracleBox:
-    """Periodic box of the C restatement; variant 0 = main path, 1 = mini-app,
+    """Box of the C restatement (periodic, or on the main path bounded on the
+    axes whose npdc is 1, 2 or 4); variant 0 = main path, 1 = mini-app,
     2 = the mini-app's stage order with the main path's metric form and J
     factors."""
 
     def __init__(self, n, variant=0, reynolds=1600.0, mach=0.1, prandtl=0.72, gamma=1.4,
-                 ref_tem=273.15, alfa=0.49, lengths=(2 * math.pi,) * 3):
+                 ref_tem=273.15, alfa=0.49, lengths=(2 * math.pi,) * 3, npdc=(3, 3, 3)):
         if isinstance(n, int):
             n = (n, n, n)
         self.n = tuple(int(v) for v in n)
         self.lib = oracle_lib()
         self.h = self.lib.orc_box_new(self.n[0], self.n[1], self.n[2], variant, reynolds, mach,
                                       prandtl, gamma, ref_tem, alfa, *lengths)
+        if tuple(npdc) != (3, 3, 3) and self.lib.orc_box_set_ntype(self.h, *npdc) != 0:
+            raise ValueError(f"closures {npdc} need the main variant")
 
     def close(self):
         if self.h:
