@@ -2600,6 +2600,7 @@ struct BcArgs {
     int face[4];    // faces 1..4 (i=0, i=im, j=0, j=jm) under the user condition
     double time;    // the step's start time
     double gamma, mach, const2, const6;
+    double len[2];  // gridcube lengths of x and y (x(i) = len/n*i), for gcnscbc
 };
 
 // One boundary node as udf_bc sets it: the primitives, tmp = thermal(
@@ -2712,6 +2713,106 @@ __global__ void k_bc_riem2d_j(BcArgs a)
     }
 }
 #undef R2D_STATES
+
+// gcnscbc (bctype 23), src/bc.F90:7907-8137: the ghost-cell NSCBC of
+// Motheau, Almgren & Bell on face 2 (i = im, AX 0) or face 4 (j = jm,
+// AX 1), with prs_t = pinf, nondimensional, no species.  One thread per
+// node of the face: the LODI amplitudes from one-sided first differences
+// of the two nodes inside (deriv_1o, src/commfunc.F90:213-222, over the
+// gridcube spacing x(n-1) - x(n-2)); the incoming wave replaced by the soft
+// outflow relaxation towards pinf (or, on face 4 where the normal velocity
+// is negative, the inflow relaxations towards the freestream); projected
+// back (Poinsot & Lele 1992, Eqs. 33-36) and extrapolated to the face node,
+// whose density is thermal(p, T) and whose q is fvar2q(rho, u, p).  The
+// normal velocity, the sound speed and its square are INTEGER variables in
+// the reference (truncated toward zero), the Mach number their integer
+// quotient, and lambda(5) takes the normal velocity (the reference's
+// lambda(5:numq) = vnorm with numq = 5): all kept.  Freestream of the
+// nondimensional run: uinf = 1, vinf = winf = 0, tinf = 1, pinf = 1/const2.
+template <int AX>
+__global__ void k_bc_gcnscbc(BcArgs a)
+{
+    const int na = a.g.n[AX == 0 ? 1 : 0];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)(na + 1) * (a.g.n[2] + 1)) return;
+    const int ia = (int)(t % (na + 1)), k = (int)(t / (na + 1));
+    const int n = a.g.n[AX];
+    const int i = AX == 0 ? n : ia, j = AX == 0 ? ia : n;
+    const long gi = gidx(a.g, i, j, k);
+    const long g1 = AX == 0 ? gidx(a.g, i - 1, j, k) : gidx(a.g, i, j - 1, k);
+    const long g2 = AX == 0 ? gidx(a.g, i - 2, j, k) : gidx(a.g, i, j - 2, k);
+    const double gamma = a.gamma, pinf = 1.0 / a.const2;
+    const double uinf = 1.0, vinf = 0.0, winf = 0.0, tinf = 1.0, eps = 1.e-10;
+    const double hx = a.len[AX] / (double)n;
+    const double dx = hx * (double)(n - 1) - hx * (double)(n - 2);
+    const double span = hx * (double)n - 0.0;
+    const double t1 = a.prim[5][g1], r1 = a.prim[0][g1], p1 = a.prim[4][g1];
+    const double v1[3] = {a.prim[1][g1], a.prim[2][g1], a.prim[3][g1]};
+    const double v2[3] = {a.prim[1][g2], a.prim[2][g2], a.prim[3][g2]};
+    const int css = (int)(sqrt(t1) / a.mach);
+    if (css == 0) {
+        // the reference divides by the zero integer sound speed here (a
+        // trap in Fortran); the node is set to NaN instead
+        const double nan = __builtin_nan("");
+        for (int m = 0; m < 6; ++m) a.prim[m][gi] = nan;
+        for (int m = 0; m < 5; ++m) a.q[m][gi] = nan;
+        return;
+    }
+    const int css2 = css * css;
+    const int vnorm = (int)v1[AX];
+    const double mavg = (double)(vnorm / css);
+    double lambda[5];
+    lambda[0] = (double)(vnorm - css) + eps;
+    lambda[1] = (double)vnorm + eps;
+    lambda[2] = (double)vnorm + eps;
+    lambda[3] = (double)vnorm + eps;
+    lambda[4] = (double)vnorm + eps;
+    double dvel[3];
+    for (int c = 0; c < 3; ++c) dvel[c] = -(-v1[c] + v2[c]) / dx;
+    double dprs = -(-p1 + a.prim[4][g2]) / dx;
+    double drho = -(-r1 + a.prim[0][g2]) / dx;
+    constexpr int TG = AX == 0 ? 1 : 0;  // the in-plane tangential component
+    double lodi[5];
+    lodi[0] = lambda[0] * (dprs - r1 * (double)css * dvel[AX]);
+    lodi[1] = lambda[1] * ((double)css2 * drho - dprs);
+    lodi[2] = lambda[2] * dvel[TG];
+    lodi[3] = lambda[3] * dvel[2];
+    lodi[4] = lambda[4] * (dprs + r1 * (double)css * dvel[AX]);
+    if (AX == 0 || vnorm >= 0) {
+        const double sigma = 0.25;
+        const double kamma = sigma * (double)css * (1.0 - mavg * mavg) / span;
+        lodi[0] = kamma * (p1 - pinf);
+    } else {
+        const double eta = 0.25;
+        lodi[0] = eta * r1 * (double)css2 * (1.0 - mavg * mavg) / span * (v1[1] - vinf);
+        lodi[1] = eta * r1 * (double)css2 / gamma / span * (t1 - tinf);
+        lodi[2] = eta * (double)css / span * (v1[0] - uinf);
+        lodi[3] = eta * (double)css / span * (v1[2] - winf);
+    }
+    for (int m = 0; m < 5; ++m) lodi[m] = lodi[m] / lambda[m];
+    const double dtmp = t1 / r1 / (double)css * (-lodi[1] + 0.5 * (gamma - 1.0) * (lodi[0] + lodi[4]));
+    dprs = 0.5 * (lodi[0] + lodi[4]);
+    dvel[AX] = 1.0 / (2.0 * r1 * (double)css) * (lodi[4] - lodi[0]);
+    dvel[TG] = lodi[2];
+    dvel[2] = lodi[3];
+    const double tn = t1 + 2.0 * dx * dtmp;
+    const double pn = p1 + 2.0 * dx * dprs;
+    double vn[3];
+    for (int c = 0; c < 3; ++c) vn[c] = v1[c] + 2.0 * dx * dvel[c];
+    const double rn = pn / tn * a.const2;
+    a.prim[0][gi] = rn;
+    a.prim[1][gi] = vn[0];
+    a.prim[2][gi] = vn[1];
+    a.prim[3][gi] = vn[2];
+    a.prim[4][gi] = pn;
+    a.prim[5][gi] = tn;
+    a.q[0][gi] = rn;
+    a.q[1][gi] = rn * vn[0];
+    a.q[2][gi] = rn * vn[1];
+    a.q[3][gi] = rn * vn[2];
+    const double var1 = 0.5 * (vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2]);
+    a.q[4][gi] = pn * a.const6 + rn * var1;
+}
 
 // back to the default contraction of the rest of the library
 #pragma clang fp contract(fast)
@@ -2865,6 +2966,7 @@ struct astr_ctx {
     double* swf[10] = {};           // compact upwind: split fluxes F+, F-
     double* fhf[10] = {};           // compact upwind: their interface fluxes
     double* cmp_ac[3][2] = {};      // compact upwind Thomas factors per axis, wind
+    double len[3] = {1.0, 1.0, 0.0};  // gridcube lengths (astr_set_lengths)
     int cmp_first[3] = {}, cmp_last[3] = {};
     bool lines_ok[3] = {};          // axis fits the compact line sweeps
     double h[3] = {0.0, 0.0, 0.0};  // uniform spacing of the mini-app variant
@@ -3396,27 +3498,42 @@ int do_qswap(astr_ctx* c)
 // the 2-D Riemann example, at the context's time
 int do_boucon(astr_ctx* c)
 {
-    if (c->udf != ASTR_UDF_RIEMANN2D) return ASTR_OK;
+    const bool gcn2 = c->bctype[1] == 23, gcn4 = c->bctype[3] == 23;
+    if (c->udf != ASTR_UDF_RIEMANN2D && !gcn2 && !gcn4) return ASTR_OK;
     BcArgs a{};
     a.g = c->g;
     for (int m = 0; m < 5; ++m) a.q[m] = c->q[m];
     for (int m = 0; m < 6; ++m) a.prim[m] = c->prim[m];
-    for (int f = 0; f < 4; ++f) a.face[f] = c->bctype[f] == 0;
+    for (int f = 0; f < 4; ++f) a.face[f] = c->udf == ASTR_UDF_RIEMANN2D && c->bctype[f] == 0;
     a.time = c->time;
     a.gamma = c->p.gamma;
     a.mach = c->p.mach;
     a.const2 = c->const2;
     a.const6 = c->const6;
+    a.len[0] = c->len[0];
+    a.len[1] = c->len[1];
+    // boucon's face order (src/bc.F90:338-404): faces 1 and 2 (the user
+    // condition of both in one kernel, or gcnscbc on face 2 after face 1),
+    // then faces 3 and 4 likewise
     const long nk = c->g.n[2] + 1;
+    const long ni = (long)(c->g.n[0] + 1) * nk, nj = (long)(c->g.n[1] + 1) * nk;
     if (a.face[0] || a.face[1]) {
-        const long n = (long)(c->g.n[1] + 1) * nk;
-        hipLaunchKernelGGL(k_bc_riem2d_i, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_bc_riem2d_i, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0,
+                           c->stream, a);
+        HIPCHK(hipGetLastError());
+    }
+    if (gcn2) {
+        hipLaunchKernelGGL(k_bc_gcnscbc<0>, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0,
                            c->stream, a);
         HIPCHK(hipGetLastError());
     }
     if (a.face[2] || a.face[3]) {
-        const long n = (long)(c->g.n[0] + 1) * nk;
-        hipLaunchKernelGGL(k_bc_riem2d_j, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_bc_riem2d_j, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0,
+                           c->stream, a);
+        HIPCHK(hipGetLastError());
+    }
+    if (gcn4) {
+        hipLaunchKernelGGL(k_bc_gcnscbc<1>, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0,
                            c->stream, a);
         HIPCHK(hipGetLastError());
     }
@@ -5468,10 +5585,12 @@ int astr_set_bc(astr_ctx* c, const int32_t* bctype, int32_t udf)
     if (!c || !bctype) return fail(ASTR_E_ARG, "null argument");
     bool any = false;
     for (int f = 0; f < 6; ++f) {
-        if (bctype[f] != 0 && bctype[f] != 1)
+        if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23)
             return fail(ASTR_E_UNSUPPORTED,
-                        "bctype 1 (nothing done) and 0 (user condition) are built; the inflow, "
-                        "outflow, wall and NSCBC conditions of src/bc.F90 are not");
+                        "bctype 1 (nothing done), 0 (user condition) and 23 (gcnscbc) are built; "
+                        "the inflow, outflow, wall and other NSCBC conditions of src/bc.F90 are not");
+        if (bctype[f] == 23 && c->g.n[2] != 0)
+            return fail(ASTR_E_UNSUPPORTED, "bctype 23 is built for two-dimensional boxes");
         any = any || bctype[f] == 0;
     }
     if (udf != 0 && udf != ASTR_UDF_RIEMANN2D)
@@ -5480,6 +5599,17 @@ int astr_set_bc(astr_ctx* c, const int32_t* bctype, int32_t udf)
     if (c->p.variant == 1) return fail(ASTR_E_UNSUPPORTED, "the mini-app variant has its own halos");
     for (int f = 0; f < 6; ++f) c->bctype[f] = bctype[f];
     c->udf = any ? udf : 0;
+    ++c->gen;
+    return ASTR_OK;
+}
+
+int astr_set_lengths(astr_ctx* c, double lx, double ly, double lz)
+{
+    if (!c) return fail(ASTR_E_ARG, "null context");
+    if (!(lx > 0.0) || !(ly > 0.0) || !(lz >= 0.0)) return fail(ASTR_E_ARG, "lengths must be positive");
+    c->len[0] = lx;
+    c->len[1] = ly;
+    c->len[2] = lz;
     ++c->gen;
     return ASTR_OK;
 }

@@ -237,6 +237,14 @@ module astr_hip
       integer(c_int32_t), value :: udf
     end function astr_set_bc
 
+    ! gridcube's box lengths, for the boundary conditions that read node
+    ! coordinates (gcnscbc)
+    integer(c_int) function astr_set_lengths(ctx, lx, ly, lz) bind(c, name='astr_set_lengths')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: lx, ly, lz
+    end function astr_set_lengths
+
     integer(c_int) function astr_boucon(ctx) bind(c, name='astr_boucon')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -193,9 +193,15 @@ class AstrSolver:
     def set_bc(self, bctype=(1, 1, 1, 1, 1, 1), udf: int = 0) -> None:
         """boucon's faces (i=0, i=im, j=0, j=jm, k=0, k=km): 1 nothing done,
         0 the user condition udf (UDF_RIEMANN2D: udf_bc of the 2-D Riemann
-        example)."""
+        example), 23 gcnscbc (faces 2 and 4, two-dimensional boxes)."""
         arr = (ctypes.c_int32 * 6)(*[int(b) for b in bctype])
         check(self.lib.astr_set_bc(self.ctx, arr, int(udf)), "astr_set_bc")
+
+    def set_lengths(self, lx: float, ly: float, lz: float = 0.0) -> None:
+        """gridcube's box lengths (x(i) = lx/im*i), for the boundary
+        conditions that read node coordinates."""
+        check(self.lib.astr_set_lengths(self.ctx, float(lx), float(ly), float(lz)),
+              "astr_set_lengths")
 
     def boucon(self) -> None:
         check(self.lib.astr_boucon(self.ctx), "astr_boucon")

@@ -330,10 +330,22 @@ int astr_set_spacing(astr_ctx *ctx, double dx, double dy, double dz);
  * (examples/Riemann2D/user_define_module/userdefine_riemann2d.F90:421-668)
  * on the grid of gridcube(1,1,0): shock-moving quadrant states on faces
  * 1 and 3, zero-slope extrapolation on faces 2 and 4, the north-east state
- * in the corner (im,jm).  The other bctypes of bc.F90 (inflow, outflow,
- * walls, NSCBC) return ASTR_E_UNSUPPORTED.  Default: all faces 1. */
+ * in the corner (im,jm).  23 is gcnscbc (src/bc.F90:7907-8137, boucon's
+ * bctype 23 with prs_t = pinf): the ghost-cell NSCBC on face 2 (i=im) and
+ * face 4 (j=jm), as the reference has it (it acts on no other face);
+ * two-dimensional boxes; it reads node coordinates of gridcube's grid
+ * (astr_set_lengths).  The other bctypes of bc.F90 (inflow, outflow,
+ * walls, the other NSCBCs) return ASTR_E_UNSUPPORTED.  Default: all
+ * faces 1. */
 #define ASTR_UDF_RIEMANN2D 1
 int astr_set_bc(astr_ctx *ctx, const int32_t *bctype, int32_t udf);
+/* The box lengths of gridcube (src/gridgeneration.F90:233-267: x(i) =
+ * lx/im*i, y(j) = ly/jm*j, z(k) = lz/km*k), which replace the reference's
+ * node coordinates x(:,:,:,1:3) where a boundary condition reads them
+ * (gcnscbc's spacing x(n-1)-x(n-2) and xmax-xmin).  Default 1, 1, 0 (the
+ * Riemann example's gridcube(1,1,0)).  No counterpart call in the
+ * reference, which holds the coordinates themselves. */
+int astr_set_lengths(astr_ctx *ctx, double lx, double ly, double lz);
 /* call boucon alone (src/mainloop.F90:413), at the context's time */
 int astr_boucon(astr_ctx *ctx);
 /* The simulation time (commvar time) that boucon sees; each whole step

@@ -116,6 +116,7 @@ typedef struct {
     double bfacmpld;     /* MP-LD dissipation weight */
     double shkcrt;       /* Ducros sensor threshold */
     double gamma, mach;
+    double lx, ly, lz;   /* gridcube lengths (x(i) = lx/im*i): gcnscbc's coordinates */
 } orc_uwd;
 
 double orc_minmod2(double a, double b);
@@ -169,6 +170,13 @@ int orc_uwd_rhscal(const orc_uwd *u, int explicit_fd, const double *q, const dou
 double orc_riem2d_shockspeed(double gamma, double mach, double p1, double t1, double p2);
 void orc_riem2d_bc(const orc_uwd *u, const int *bctype, double time, double *q, double *rho,
                    double *vel, double *prs, double *tmp);
+/* gcnscbc (bctype 23) on face n = 2 (i = im) or 4 (j = jm); other faces
+ * are left alone, as in the reference */
+void orc_gcnscbc(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+                 double *tmp);
+/* boucon: faces 1..6 in order, udf_bc (bctype 0, udf 1) or gcnscbc (23) */
+void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, double *q,
+                double *rho, double *vel, double *prs, double *tmp);
 void orc_uwd_qswap(const orc_uwd *u, double *q, double *rho, double *vel, double *prs,
                    double *tmp);
 int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *bctype, int udf,

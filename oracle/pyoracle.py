@@ -264,7 +264,8 @@ class UwdCfg(ctypes.Structure):
                 ("npdc", ctypes.c_int * 3), ("recon_schem", ctypes.c_int),
                 ("lchardecomp", ctypes.c_int), ("bfacmpld", ctypes.c_double),
                 ("shkcrt", ctypes.c_double), ("gamma", ctypes.c_double),
-                ("mach", ctypes.c_double)]
+                ("mach", ctypes.c_double), ("lx", ctypes.c_double), ("ly", ctypes.c_double),
+                ("lz", ctypes.c_double)]
 
 
 _uwd_declared = False
@@ -317,6 +318,8 @@ def uwd_lib():
         lib.orc_riem2d_shockspeed.argtypes = [_F, _F, _F, _F, _F]
         lib.orc_riem2d_bc.argtypes = [P, I6, _F, _D, _D, _D, _D, _D]
         lib.orc_uwd_qswap.argtypes = [P, _D, _D, _D, _D, _D]
+        lib.orc_gcnscbc.restype = None
+        lib.orc_gcnscbc.argtypes = [P, _I, _D, _D, _D, _D, _D]
         lib.orc_uwd_rk3_step.restype = _I
         lib.orc_uwd_rk3_step.argtypes = [P, _I, _I, I6, _I, _F, _F, _D, _D, _D, _D, _D, _D, _D]
         _uwd_declared = True
@@ -384,10 +387,10 @@ class UwdBox:
     (Fortran order, halos hm = 5 on every axis; km = 0 is a 2-D box)."""
 
     def __init__(self, dims, npdc=(3, 3, 3), recon_schem=3, lchardecomp=True, bfacmpld=0.7,
-                 shkcrt=0.01, gamma=1.4, mach=1.0):
+                 shkcrt=0.01, gamma=1.4, mach=1.0, lengths=(1.0, 1.0, 0.0)):
         im, jm, km = dims
         self.cfg = UwdCfg(im, jm, km, (ctypes.c_int * 3)(*npdc), recon_schem, int(lchardecomp),
-                          bfacmpld, shkcrt, gamma, mach)
+                          bfacmpld, shkcrt, gamma, mach, *lengths)
         self.dims = (im, jm, km)
         self.hshape = (im + 1 + 2 * HM, jm + 1 + 2 * HM, km + 1 + 2 * HM)
         self.ishape = (im + 1, jm + 1, km + 1)
@@ -461,6 +464,14 @@ class UwdBox:
         o = self._state(st)
         lib.orc_riem2d_bc(ctypes.byref(self.cfg), self._faces(bctype), float(time), _p(o["q"]),
                           _p(o["rho"]), _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
+        return o
+
+    def gcnscbc(self, st, face):
+        """A copy of st after gcnscbc (bctype 23) on face 2 or 4."""
+        lib = uwd_lib()
+        o = self._state(st)
+        lib.orc_gcnscbc(ctypes.byref(self.cfg), int(face), _p(o["q"]), _p(o["rho"]), _p(o["vel"]),
+                        _p(o["prs"]), _p(o["tmp"]))
         return o
 
     def qswap(self, st):

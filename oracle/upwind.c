@@ -1250,6 +1250,139 @@ void orc_riem2d_bc(const orc_uwd *u, const int *bctype, double time, double *q, 
 #undef H
 }
 
+/* gcnscbc, src/bc.F90:7907-8137 (Motheau, Almgren & Bell's ghost-cell
+ * NSCBC), as boucon calls it for bctype 23 with prs_t = pinf
+ * (src/bc.F90:389-391), nondimensional, no species.  Face 2 (i = im) and
+ * face 4 (j = jm) only; the routine has no branch for the other faces.  At
+ * each node of the face, from the two nodes inside it (one-sided first
+ * differences, deriv_1o, src/commfunc.F90:213-222, over the gridcube
+ * spacing x(n-1) - x(n-2)):
+ *   LODI amplitudes l_k = lambda_k * (characteristic derivative), with the
+ *   normal velocity vnorm, the sound speed css = sos(T) = sqrt(T)/Ma and
+ *   css2 = css*css held in INTEGER variables as the reference declares
+ *   them (truncated toward zero), mavg = vnorm/css an integer quotient,
+ *   lambda = (vnorm-css, vnorm, vnorm, vnorm, vnorm) + 1e-10 (the
+ *   reference's lambda(5:numq) = vnorm overwrites vnorm+css when numq = 5);
+ *   the incoming wave l_1 replaced by the soft outflow relaxation
+ *   0.25 css (1 - mavg^2)/(xmax-xmin) (p - pinf), or, on face 4 where
+ *   vnorm < 0, by the inflow relaxations of v, T, u, w towards vinf, tinf,
+ *   uinf, winf (eta = 0.25);
+ *   l / lambda projected back to d(rho, T, p, u) (Poinsot & Lele 1992,
+ *   Eqs. 33-36), the boundary node extrapolated f(n) = f(n-1) + 2 dx df,
+ *   rho = thermal(p, T) = p/T*const2, q = fvar2q(rho, u, p).
+ * Nondimensional freestream: uinf = 1, vinf = winf = 0, tinf = 1,
+ * pinf = 1/const2 (refcal, src/solver.F90:100-120).  The reference's
+ * qrhs = 0 on the face is followed by rhscal's own sums on every step, so
+ * it is not restated. */
+void orc_gcnscbc(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+                 double *tmp)
+{
+    if (face != 2 && face != 4) return;
+    const shape_t s = shape_of(u);
+    const double gamma = u->gamma, mach = u->mach;
+    const double const2 = gamma * (mach * mach), const6 = 1.0 / (gamma - 1.0);
+    const double pinf = 1.0 / const2, uinf = 1.0, vinf = 0.0, winf = 0.0, tinf = 1.0;
+    const double eps = 1.e-10;
+    const int im = u->im, jm = u->jm, km = u->km;
+    const int ax = face == 2 ? 0 : 1;       /* normal axis */
+    const int n = ax == 0 ? im : jm;
+    const double len = ax == 0 ? u->lx : u->ly;
+    const double hx = len / (double)n;      /* gridcube: x(i) = len/n*i */
+    const double dx = hx * (double)(n - 1) - hx * (double)(n - 2);
+    const double span = hx * (double)n - 0.0;  /* xmax - xmin */
+    const int na = ax == 0 ? jm : im;
+    for (int k = 0; k <= km; ++k)
+        for (int a = 0; a <= na; ++a) {
+            const int i = ax == 0 ? im : a, j = ax == 0 ? a : jm;
+            const long h = hidx(&s, i, j, k);
+            const long h1 = ax == 0 ? hidx(&s, i - 1, j, k) : hidx(&s, i, j - 1, k);
+            const long h2 = ax == 0 ? hidx(&s, i - 2, j, k) : hidx(&s, i, j - 2, k);
+            const double t1 = tmp[h1], r1 = rho[h1], p1 = prs[h1];
+            const double v1[3] = {vel[h1], vel[s.nf + h1], vel[2 * s.nf + h1]};
+            const double v2[3] = {vel[h2], vel[s.nf + h2], vel[2 * s.nf + h2]};
+            const int css = (int)(sqrt(t1) / mach);
+            if (css == 0) {
+                /* the reference divides by the zero integer sound speed here
+                 * (a trap in Fortran); the node is set to NaN instead */
+                const double nan = NAN;
+                rho[h] = prs[h] = tmp[h] = nan;
+                for (int c = 0; c < 3; ++c) vel[c * s.nf + h] = nan;
+                for (int m = 0; m < 5; ++m) q[m * s.nf + h] = nan;
+                continue;
+            }
+            const int css2 = css * css;
+            const int vnorm = (int)v1[ax];
+            const double mavg = (double)(vnorm / css);
+            double lambda[5];
+            lambda[0] = (double)(vnorm - css);
+            lambda[1] = lambda[2] = lambda[3] = (double)vnorm;
+            lambda[4] = (double)vnorm;
+            for (int m = 0; m < 5; ++m) lambda[m] = lambda[m] + eps;
+            double dvel[3];
+            for (int c = 0; c < 3; ++c) dvel[c] = -(-v1[c] + v2[c]) / dx;
+            double dprs = -(-p1 + prs[h2]) / dx;
+            double drho = -(-r1 + rho[h2]) / dx;
+            double dtmp = -(-t1 + tmp[h2]) / dx;
+            (void)dtmp;
+            const int tg = ax == 0 ? 1 : 0;  /* the tangential components */
+            double lodi[5];
+            lodi[0] = lambda[0] * (dprs - r1 * (double)css * dvel[ax]);
+            lodi[1] = lambda[1] * ((double)css2 * drho - dprs);
+            lodi[2] = lambda[2] * dvel[tg];
+            lodi[3] = lambda[3] * dvel[2];
+            lodi[4] = lambda[4] * (dprs + r1 * (double)css * dvel[ax]);
+            if (face == 2 || vnorm >= 0) {
+                const double sigma = 0.25;
+                const double kamma = sigma * (double)css * (1.0 - mavg * mavg) / span;
+                lodi[0] = kamma * (p1 - pinf);
+            } else {
+                const double eta = 0.25;
+                lodi[0] = eta * r1 * (double)css2 * (1.0 - mavg * mavg) / span * (v1[1] - vinf);
+                lodi[1] = eta * r1 * (double)css2 / gamma / span * (t1 - tinf);
+                lodi[2] = eta * (double)css / span * (v1[0] - uinf);
+                lodi[3] = eta * (double)css / span * (v1[2] - winf);
+            }
+            for (int m = 0; m < 5; ++m) lodi[m] = lodi[m] / lambda[m];
+            drho = 1.0 / (double)css2 * (lodi[1] + 0.5 * (lodi[0] + lodi[4]));
+            dtmp = t1 / r1 / (double)css * (-lodi[1] + 0.5 * (gamma - 1.0) * (lodi[0] + lodi[4]));
+            dprs = 0.5 * (lodi[0] + lodi[4]);
+            dvel[ax] = 1.0 / (2.0 * r1 * (double)css) * (lodi[4] - lodi[0]);
+            dvel[tg] = lodi[2];
+            dvel[2] = lodi[3];
+            const double tn = t1 + 2.0 * dx * dtmp;
+            const double pn = p1 + 2.0 * dx * dprs;
+            double vn[3];
+            for (int c = 0; c < 3; ++c) vn[c] = v1[c] + 2.0 * dx * dvel[c];
+            const double rn = pn / tn * const2;
+            rho[h] = rn;
+            vel[h] = vn[0];
+            vel[s.nf + h] = vn[1];
+            vel[2 * s.nf + h] = vn[2];
+            prs[h] = pn;
+            tmp[h] = tn;
+            q[h] = rn;
+            q[s.nf + h] = rn * vn[0];
+            q[2 * s.nf + h] = rn * vn[1];
+            q[3 * s.nf + h] = rn * vn[2];
+            const double var1 = 0.5 * (vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2]);
+            q[4 * s.nf + h] = pn * const6 + rn * var1;
+        }
+}
+
+void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, double *q,
+                double *rho, double *vel, double *prs, double *tmp)
+{
+    for (int f = 0; f < 6; ++f) {
+        if (bctype[f] == 0 && udf == 1) {
+            int one[6] = {1, 1, 1, 1, 1, 1};
+            one[f] = 0;
+            orc_riem2d_bc(u, one, time, q, rho, vel, prs, tmp);
+        } else if (bctype[f] == 23) {
+            orc_gcnscbc(u, f + 1, q, rho, vel, prs, tmp);
+        }
+    }
+}
+
 /* qswap, single rank (src/parallel.F90:4849-5319, turbmode none): along
  * each periodic axis in the order i, j, k the halo copy, the duplicate
  * average q(0) = q(n) = (q(0)+q(n))/2 and q2fvar on the strips -hm..0 and
@@ -1321,10 +1454,11 @@ int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *
     const double const2 = u->gamma * (u->mach * u->mach), const6 = 1.0 / (u->gamma - 1.0);
     const double rk[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25},
                              {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
-    int anyudf = 0;
+    int anyudf = 0, anybc = 0;
     for (int f = 0; f < 6; ++f) {
-        if (bctype[f] != 0 && bctype[f] != 1) return 3;
+        if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23) return 3;
         anyudf |= bctype[f] == 0;
+        anybc |= bctype[f] != 1;
     }
     if (anyudf && udf != 1) return 3;
     double *qrhs = malloc(sizeof(double) * 5 * s.nint), *qsave = malloc(sizeof(double) * 5 * s.nint);
@@ -1333,7 +1467,7 @@ int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *
     unsigned char *lsh = malloc(s.nint);
     int rc = 0;
     for (int st = 0; st < 3 && !rc; ++st) {
-        if (anyudf) orc_riem2d_bc(u, bctype, time, q, rho, vel, prs, tmp);
+        if (anybc) orc_boucon(u, bctype, udf, time, q, rho, vel, prs, tmp);
         orc_uwd_qswap(u, q, rho, vel, prs, tmp);
         rc = (compact ? orc_cmp_rhscal : orc_uwd_rhscal)(u, explicit_fd, q, rho, vel, prs, tmp,
                                                          dxi, jacob, qrhs, dvel, dtmp, ssf, lsh);

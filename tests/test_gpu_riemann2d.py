@@ -179,3 +179,66 @@ def test_boundary_and_two_dimensional_rejections():
     with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
         ag.AstrBatch([s])
     s.close()
+
+
+def pulse_state(dims, mach, u=0.3, v=0.2, amp=0.05, seed=0):
+    """Uniform flow with a pressure pulse and a smooth perturbation, at a
+    sound speed sqrt(gamma p/rho) of about 1/Ma (gcnscbc truncates it to an
+    integer, which must not be 0)."""
+    shape = (dims[0] + 1 + 2 * HM, dims[1] + 1 + 2 * HM, 1 + 2 * HM)
+    rng = np.random.default_rng(seed)
+    x = np.linspace(0, 1, shape[0])[:, None, None] * np.ones(shape)
+    y = np.linspace(0, 1, shape[1])[None, :, None] * np.ones(shape)
+    pinf = 1.0 / (1.4 * mach * mach)
+    g = amp * np.exp(-((x - 0.6) ** 2 + (y - 0.7) ** 2) / 0.02)
+    f = 0.01 * np.sin(2 * np.pi * (x + 2 * y) + rng.random())
+    return po.prim_to_state(1.0 + g + f, u + g, v - 0.5 * g, 0.0 * x, pinf * (1.0 + g - f),
+                            mach=mach)
+
+
+@pytest.mark.parametrize("v", [0.2, -1.2])
+def test_gcnscbc_steps_match_checker(v):
+    """gcnscbc (bctype 23) on faces 2 and 4 through whole upwind steps against
+    the checker; v = -1.2 takes face 4's inflow branch (integer normal
+    velocity -1)."""
+    dims = (64, 60, 0)
+    mach = 0.4
+    print(f"case gcnscbc dims={dims} v={v}", flush=True)
+    p = ag.default_params(dims, npdci=4, npdcj=4, npdck=3, convection=1, deriv_explicit=1,
+                          recon_schem=3, lchardecomp=1, lfilter=0, diffterm=0, shkcrt=0.01,
+                          mach=mach)
+    st = pulse_state(dims, mach, v=v)
+    s = ag.AstrSolver(p)
+    jac, dxi = ag.box_metrics(p, lengths=(1.0, 1.0, 1.0))
+    s.upload_geom(jac, dxi)
+    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+    faces = (1, 23, 1, 23, 1, 1)
+    s.set_lengths(1.0, 1.0, 0.0)
+    s.set_bc(faces, 0)
+    box = po.UwdBox(dims, npdc=(4, 4, 3), recon_schem=3, lchardecomp=True, bfacmpld=0.7,
+                    shkcrt=0.01, gamma=p.gamma, mach=mach)
+    dxi9 = dxi.reshape(dxi.shape[:3] + (9,), order="F")
+    s.boucon()
+    compare_plane(s.download_state(), _boucon_ref(box, st, faces), "boucon")
+    dt = 0.2 / 64 * mach
+    s.steploop(5, dt)
+    g = s.download_state()
+    ref = box.rk3_steps(st, dxi9, jac, dt, nsteps=5, bctype=faces, udf=0)
+    compare_plane(g, ref, "5 steps")
+    s.close()
+
+
+def _boucon_ref(box, st, faces):
+    o = st
+    for f, b in enumerate(faces):
+        if b == 23:
+            o = box.gcnscbc(o, f + 1)
+    return o
+
+
+def test_gcnscbc_rejected_in_three_dimensions():
+    p = ag.default_params((32, 32, 32))
+    s = ag.AstrSolver(p)
+    with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
+        s.set_bc((1, 23, 1, 1, 1, 1), 0)
+    s.close()
