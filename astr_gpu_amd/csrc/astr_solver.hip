@@ -162,6 +162,9 @@ struct SweepArgs {
     // mini-app variant (tgvsolver.F90): derivatives divided by the uniform
     // spacing h instead of multiplied by the metric; RK3 without J
     int mini;
+    // PASS_GRAD on a grid with general metrics: the index-space derivative
+    // is stored as it is, and k_metric_grad combines the three axes
+    int raw;
     double h;
 };
 
@@ -252,7 +255,7 @@ struct TileView {
     double* gdst;        // gradient: dvel(m, dir) or dtmp(dir)
     double *q, *qo, *qrhs, *qsave, *fout;
     const double *jac, *xi;
-    int rk_stage, mini;
+    int rk_stage, mini, raw;
     double rka, rkb, rkc, dt, h;
 };
 
@@ -288,6 +291,7 @@ __device__ __forceinline__ TileView tile_view(const A& a, int m)
     v.xi = a.xi;
     v.rk_stage = a.rk_stage;
     v.mini = a.mini;
+    v.raw = a.raw;
     v.rka = a.rka;
     v.rkb = a.rkb;
     v.rkc = a.rkc;
@@ -496,7 +500,7 @@ __device__ __forceinline__ void sweep_tile(const A& args, const A& args_w, int t
         if constexpr (PASS == PASS_GRAD) {
             // gradcal's metric factor dxi(d,d) (src/comsolver.F90:304-318)
             // on the lane's rows, loaded with the staging
-            if (!a.mini) {
+            if (!a.mini && !a.raw) {
 #pragma unroll
                 for (int t = 0; t < L; ++t) x[t] *= xr[t];
             }
@@ -829,6 +833,10 @@ struct PtArgs {
     int visc;               // include the viscous fluxes (diffterm)
     int mini;               // mini-app variant (tgvsolver.F90) flux forms
     int per[3];             // axis d periodic (npdc 3): k_gflux writes its halo G
+    // general metrics (curv 1): dxf[3d+e] = d(xi_d)/d(x_e) at every node;
+    // else the rectilinear xi lines
+    int curv;
+    const double* dxf[9];
     const double* jac;
     double const1, const2, const5, const6;
     double reynolds, prandtl, tempconst, tempconst1;
@@ -1207,6 +1215,35 @@ __device__ __forceinline__ void gflux_of(const PtArgs& a, int d, double xn, cons
     }
 }
 
+// gflux_of with the three metric components of axis d at the node (a grid
+// with general metrics): U = dxi(d,:).u, the pressure on every momentum
+// component, and sigma and the heat flux projected on dxi(d,:)
+// (src/solver.F90:2239-2253, 2628-2639)
+__device__ __forceinline__ void gflux_of_gen(const PtArgs& a, int d, long gm, const NodeIn& n,
+                                             const double* sg, const double* qf, double* out)
+{
+    static constexpr int sidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    const double x1 = a.dxf[3 * d][gm], x2 = a.dxf[3 * d + 1][gm], x3 = a.dxf[3 * d + 2][gm];
+    const double p = n.p, jc = n.jc;
+    const double uu = x1 * n.u[0] + x2 * n.u[1] + x3 * n.u[2];
+    const double xe[3] = {x1, x2, x3};
+    double fi[5];
+    fi[0] = jc * n.q[0] * uu;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fi[1 + c] = jc * (n.q[1 + c] * uu + xe[c] * p);
+    fi[4] = jc * (n.q[4] + p) * uu;
+    out[0] = -fi[0];
+    if (!a.visc) {
+#pragma unroll
+        for (int m = 1; m < 5; ++m) out[m] = -fi[m];
+        return;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        out[1 + c] = (sg[sidx[c][0]] * x1 + sg[sidx[c][1]] * x2 + sg[sidx[c][2]] * x3) * jc - fi[1 + c];
+    out[4] = (qf[0] * x1 + qf[1] * x2 + qf[2] * x3) * jc - fi[4];
+}
+
 // G_d at a node of the halo strip of axis d (the nodes -hm..-1 and
 // n+1..n+hm of a d-line, where the flux sweep also reads G_d): the inviscid
 // part at the node itself, the viscous part (sigma, heat flux) that of its
@@ -1226,7 +1263,8 @@ __device__ __forceinline__ void gflux_strip(const PtArgs& a, const int (&c)[3], 
     // a compile-time axis keeps sg and qf in registers
     double out[5];
     const double xn = a.xi[d][c[d] + HM];
-    if (d == 0) gflux_of(a, 0, xn, nd, sg, qf, out);
+    if (a.curv) gflux_of_gen(a, d, gi, nd, sg, qf, out);
+    else if (d == 0) gflux_of(a, 0, xn, nd, sg, qf, out);
     else if (d == 1) gflux_of(a, 1, xn, nd, sg, qf, out);
     else gflux_of(a, 2, xn, nd, sg, qf, out);
 #pragma unroll
@@ -1266,21 +1304,28 @@ __device__ __forceinline__ void gflux_cell(const PtArgs& a, int i, int j, int k,
         if (STATS && i >= 1 && j >= 1 && k >= 1 && j <= a.g.n[1] && k <= a.g.n[2])
             stats_terms(a, gi, st[0], st[1], st[2]);
         double out[15];
+        if (a.curv) {
 #pragma unroll
-        for (int e = 0; e < 3; ++e) gflux_of(a, e, a.xi[e][c[e] + HM], nd, sg, qf, out + 5 * e);
+            for (int e = 0; e < 3; ++e) gflux_of_gen(a, e, gi, nd, sg, qf, out + 5 * e);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 3; ++e) gflux_of(a, e, a.xi[e][c[e] + HM], nd, sg, qf, out + 5 * e);
+        }
 #pragma unroll
         for (int k = 0; k < 15; ++k) a.gfl[k][gi] = out[k];
         if (hj != NO_HALO) {
             const long gh = gidx(a.g, i, hj, c[2]);
             const NodeIn nh = load_node(a, gh);
-            gflux_of(a, 1, a.xi[1][hj + HM], nh, sg, qf, out);
+            if (a.curv) gflux_of_gen(a, 1, gh, nh, sg, qf, out);
+            else gflux_of(a, 1, a.xi[1][hj + HM], nh, sg, qf, out);
 #pragma unroll
             for (int m = 0; m < 5; ++m) a.gfl[5 + m][gh] = out[m];
         }
         if (hk != NO_HALO) {
             const long gh = gidx(a.g, i, c[1], hk);
             const NodeIn nh = load_node(a, gh);
-            gflux_of(a, 2, a.xi[2][hk + HM], nh, sg, qf, out);
+            if (a.curv) gflux_of_gen(a, 2, gh, nh, sg, qf, out);
+            else gflux_of(a, 2, a.xi[2][hk + HM], nh, sg, qf, out);
 #pragma unroll
             for (int m = 0; m < 5; ++m) a.gfl[10 + m][gh] = out[m];
         }
@@ -1425,6 +1470,36 @@ __device__ __forceinline__ void prim_body(const PtArgs& a, unsigned bid)
 }
 
 __global__ __launch_bounds__(256) void k_prim(PtArgs a) { prim_body(a, blockIdx.x); }
+
+// gradcal's metric combination on a grid with general metrics
+// (src/comsolver.F90:304-318, 344-358, 384-398): the sweeps left the
+// index-space derivative of u_m (and T) along axis d in dvel(m, d) (dtmp(d));
+// dvel(m, e) = sum over d, in the order 1, 2, 3, of that derivative times
+// dxi(d, e).  Over row_node's rows (the cells past node n are dead).
+struct MetricArgs {
+    Geo g;
+    double* dv[12];  // dvel(0..8), dtmp(0..2)
+    const double* dxf[9];
+};
+
+__global__ __launch_bounds__(256) void k_metric_grad(MetricArgs a)
+{
+    int i, j, k;
+    if (!row_node(a.g, i, j, k)) return;
+    const long gi = gidx(a.g, i, j, k);
+    double x[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) x[m] = a.dxf[m][gi];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double r[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) r[d] = a.dv[3 * q + d][gi];
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+            a.dv[3 * q + e][gi] = 0.0 + r[0] * x[e] + r[1] * x[3 + e] + r[2] * x[6 + e];
+    }
+}
 
 // Partial sums of kenergycal / enstophycal / diss_rate_cal over 1..n
 // (src/statistic.F90:870-1042), one double-double (hi, lo) pair per
@@ -2967,6 +3042,8 @@ struct astr_ctx {
     double* fhf[10] = {};           // compact upwind: their interface fluxes
     double* cmp_ac[3][2] = {};      // compact upwind Thomas factors per axis, wind
     double len[3] = {1.0, 1.0, 0.0};  // gridcube lengths (astr_set_lengths)
+    bool curv = false;                // general metrics uploaded (dxf)
+    double* dxf[9] = {};              // dxi(d, e) per node, 3d+e
     int cmp_first[3] = {}, cmp_last[3] = {};
     bool lines_ok[3] = {};          // axis fits the compact line sweeps
     double h[3] = {0.0, 0.0, 0.0};  // uniform spacing of the mini-app variant
@@ -3260,6 +3337,8 @@ PtArgs pt_args(astr_ctx* c)
     for (int d = 0; d < 3; ++d) a.xi[d] = c->xi + d * c->xi_stride;
     a.visc = c->p.diffterm;
     a.mini = c->p.variant == 1;
+    a.curv = c->curv;
+    for (int m = 0; m < 9; ++m) a.dxf[m] = c->dxf[m];
     a.per[0] = c->p.npdci == 3;
     a.per[1] = c->p.npdcj == 3;
     a.per[2] = c->p.npdck == 3;
@@ -3313,6 +3392,7 @@ SweepArgs sweep_args(astr_ctx* c, int dir, int pass)
     for (int m = 0; m < 5; ++m) a.qsave[m] = c->qsave[m];
     a.visc = c->p.diffterm;
     a.mini = c->p.variant == 1;
+    a.raw = pass == PASS_GRAD && c->curv;
     a.h = c->h[dir];
     return a;
 }
@@ -3775,6 +3855,17 @@ int do_grad(astr_ctx* c)
 {
     if (c->p.deriv_explicit) return do_grad_explicit(c);
     for (int d = 0; d < 3; ++d) TRY(launch_sweep(c, sweep_args(c, d, PASS_GRAD)));
+    if (c->curv) {
+        MetricArgs m{};
+        m.g = c->g;
+        for (int q = 0; q < 9; ++q) m.dv[q] = c->dvel[q];
+        for (int q = 0; q < 3; ++q) m.dv[9 + q] = c->dtmp[q];
+        for (int q = 0; q < 9; ++q) m.dxf[q] = c->dxf[q];
+        const long n = (long)(c->g.n[1] + 1) * (c->g.n[2] + 1) * line_end_i(c->g.n[0]);
+        hipLaunchKernelGGL(k_metric_grad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           c->stream, m);
+        HIPCHK(hipGetLastError());
+    }
     return ASTR_OK;
 }
 
@@ -4900,6 +4991,7 @@ int astr_destroy(astr_ctx* c)
     (void)hipFree(c->lshock);
     for (int d = 0; d < 3; ++d)
         for (int w = 0; w < 2; ++w) (void)hipFree(c->cmp_ac[d][w]);
+    if (c->dxf[0]) (void)hipFree(c->dxf[0]);
     for (int e = 0; e < 16; ++e)
         if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -4915,18 +5007,20 @@ int astr_upload_geom(astr_ctx* c, const double* jacob, const double* dxi)
     const int n[3] = {c->g.n[0], c->g.n[1], c->g.n[2]};
     const long ni = n[0] + 1 + 2 * HM, nj = n[1] + 1 + 2 * HM, nk = n[2] + 1 + 2 * HM;
     const long nt = ni * nj * nk;
-    // rectilinear check over the whole haloed array
+    // rectilinear (dxi diagonal, dxi(d,d) a function of the d-th index
+    // alone, over the whole haloed array): the metric lines xi; otherwise
+    // the nine components per node (general metrics)
     std::vector<double> xi(3 * c->xi_stride, 0.0);
-    for (int d = 0; d < 3; ++d)
-        for (int e = 0; e < 3; ++e) {
+    bool rect = true;
+    for (int d = 0; d < 3 && rect; ++d)
+        for (int e = 0; e < 3 && rect; ++e) {
             const double* m = dxi + (long)(d + 3 * e) * nt;  // dxi(:,:,:,d+1,e+1)
-            for (long k = 0; k < nk; ++k)
-                for (long j = 0; j < nj; ++j)
+            for (long k = 0; k < nk && rect; ++k)
+                for (long j = 0; j < nj && rect; ++j)
                     for (long i = 0; i < ni; ++i) {
                         const double v = m[i + ni * (j + nj * k)];
                         if (d != e) {
-                            if (v != 0.0)
-                                return fail(ASTR_E_UNSUPPORTED, "curvilinear metrics (off-diagonal dxi)");
+                            if (v != 0.0) { rect = false; break; }
                             continue;
                         }
                         const long idx = d == 0 ? i : d == 1 ? j : k;
@@ -4934,12 +5028,32 @@ int astr_upload_geom(astr_ctx* c, const double* jacob, const double* dxi)
                         const bool first = (d == 0 ? (j == 0 && k == 0)
                                                    : d == 1 ? (i == 0 && k == 0) : (i == 0 && j == 0));
                         if (first) slot = v;
-                        else if (slot != v)
-                            return fail(ASTR_E_UNSUPPORTED, "dxi(d,d) varies across the other axes");
+                        else if (slot != v) { rect = false; break; }
                     }
         }
-    HIPCHK(hipMemcpy(c->xi, xi.data(), sizeof(double) * xi.size(), hipMemcpyHostToDevice));
     std::vector<double> stage;
+    if (rect) {
+        HIPCHK(hipMemcpy(c->xi, xi.data(), sizeof(double) * xi.size(), hipMemcpyHostToDevice));
+        c->curv = false;
+    } else {
+        // general metrics: the central path of the main variant on a
+        // three-dimensional box reads all nine components
+        if (c->p.convection != 0 || c->p.variant != 0 || c->g.n[2] == 0)
+            return fail(ASTR_E_UNSUPPORTED, "curvilinear metrics (off-diagonal or varying dxi) "
+                                            "are built for the central path of the main variant "
+                                            "on three-dimensional boxes");
+        if (!c->dxf[0]) {
+            double* base = nullptr;
+            if (hipMalloc(&base, sizeof(double) * c->nfield * 9) != hipSuccess)
+                return fail(ASTR_E_NOMEM, "device allocation of the metric fields failed");
+            HIPCHK(hipMemset(base, 0, sizeof(double) * c->nfield * 9));
+            for (int m = 0; m < 9; ++m) c->dxf[m] = base + (long)m * c->nfield;
+        }
+        for (int d = 0; d < 3; ++d)
+            for (int e = 0; e < 3; ++e)
+                TRY(upload_field(c, c->dxf[3 * d + e], dxi + (long)(d + 3 * e) * nt, stage));
+        c->curv = true;
+    }
     TRY(upload_field(c, c->jac, jacob, stage));
     c->have_geom = true;
     return ASTR_OK;
@@ -5043,6 +5157,7 @@ static int ready_batch(astr_ctx* c)
     TRY(ready(c));
     if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "batched members are three-dimensional boxes");
     if (!all_periodic(c)) return fail(ASTR_E_UNSUPPORTED, "batched members are periodic boxes");
+    if (c->curv) return fail(ASTR_E_UNSUPPORTED, "batched members have rectilinear metrics");
     if (c->udf) return fail(ASTR_E_UNSUPPORTED, "batched members take no user boundary condition");
     return ASTR_OK;
 }
@@ -5766,8 +5881,10 @@ int astr_adjoint_record(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt, do
     TRY(ready(c));
     if (!J) return fail(ASTR_E_ARG, "null argument");
     if (nsteps < 0 || ckpt < 1) return fail(ASTR_E_ARG, "nsteps >= 0 and ckpt >= 1 required");
-    if (c->p.convection != 0 || c->p.variant != 0 || c->g.n[2] == 0 || !all_periodic(c))
-        return fail(ASTR_E_UNSUPPORTED, "the adjoint is built for the central path on periodic 3-D boxes");
+    if (c->p.convection != 0 || c->p.variant != 0 || c->g.n[2] == 0 || !all_periodic(c) ||
+        c->curv)
+        return fail(ASTR_E_UNSUPPORTED, "the adjoint is built for the central path on periodic 3-D "
+                                        "boxes with rectilinear metrics");
     if (c->udf) return fail(ASTR_E_UNSUPPORTED, "the adjoint takes no user boundary condition");
     c->rec_nsteps = -1;
     const long nseg = nsteps > 0 ? (nsteps + ckpt - 1) / ckpt : 0;

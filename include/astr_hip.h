@@ -96,9 +96,13 @@ int astr_create(const astr_params *p, int32_t device, astr_ctx **out);
 int astr_destroy(astr_ctx *ctx);
 
 /* Metrics from gridgeom (src/geom.F90:99-917): jacob(-hm:im+hm,...) and
- * dxi(-hm:im+hm,...,1:3,1:3).  The solver keeps rectilinear metrics only:
- * off-diagonal dxi must be zero and dxi(d,d) must depend on the d-th index
- * alone; otherwise ASTR_E_UNSUPPORTED. */
+ * dxi(-hm:im+hm,...,1:3,1:3).  Rectilinear metrics (off-diagonal dxi zero,
+ * dxi(d,d) a function of the d-th index alone) are kept as one line per
+ * axis; any other (curvilinear) metrics as the nine components per node,
+ * which the central path of the main variant on three-dimensional boxes
+ * reads (gradcal's combination of the axes, the fluxes' projections on
+ * dxi(d,:)); other contexts return ASTR_E_UNSUPPORTED for them, as do the
+ * adjoint and the batch. */
 int astr_upload_geom(astr_ctx *ctx, const double *jacob, const double *dxi);
 
 /* Module state of src/commarray.F90 in and out: q(.,.,.,1:5), rho, vel

@@ -1273,6 +1273,18 @@ void orc_box_set(orc_box *b, int which, const double *in)
             for (int i = 0; i <= b->im; ++i) a[IX(b, i, j, k)] = in[c++];
 }
 
+/* The metrics gridgeom would compute for the box's grid (src/geom.F90:
+ * 99-917), haloed, Fortran order: jacob(-hm:im+hm,...) and dxi(...,1:3,1:3)
+ * with dxi(:,:,:,d,e) = d(xi_d)/d(x_e) at offset (d-1 + 3(e-1)) n3.  The
+ * box's kernels read every component (convrsdcal6, diffrsdcal6, gradcal). */
+void orc_box_set_metrics(orc_box *b, const double *jacob, const double *dxi)
+{
+    memcpy(b->jacob, jacob, sizeof(double) * b->n3);
+    for (int d = 0; d < 3; ++d)
+        for (int e = 0; e < 3; ++e)
+            memcpy(b->dxi[d * 3 + e], dxi + (size_t)(d + 3 * e) * b->n3, sizeof(double) * b->n3);
+}
+
 void orc_box_get_haloed(const orc_box *b, int which, double *out)
 {
     memcpy(out, which_field(b, which), sizeof(double) * b->n3);

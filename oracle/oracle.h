@@ -75,6 +75,9 @@ void orc_box_free(orc_box *b);
  * tgvsolver.F90:133) and the filter's boundary beta (0.98 main, 1.09 mini)
  * are set by the variant; they can be overridden. */
 void orc_box_set_sutherland(orc_box *b, double s_over_tref_numerator);
+/* metrics of the grid (haloed jacob and dxi(..,d,e) at (d-1+3(e-1))*n3);
+ * the box starts with those of its uniform grid */
+void orc_box_set_metrics(orc_box *b, const double *jacob, const double *dxi);
 /* closure per axis (3 periodic, 1/2/4 bounded); main variant; 0 or -1 */
 int orc_box_set_ntype(orc_box *b, int n0, int n1, int n2);
 

@@ -52,6 +52,8 @@ def oracle_lib():
         lib.orc_box_get_deriv.argtypes = [_V, _I, _D]
         lib.orc_box_get_haloed.argtypes = [_V, _I, _D]
         lib.orc_box_set_sutherland.argtypes = [_V, _F]
+        lib.orc_box_set_metrics.restype = None
+        lib.orc_box_set_metrics.argtypes = [_V, _D, _D]
         lib.orc_box_set_ntype.restype = _I
         lib.orc_box_set_ntype.argtypes = [_V, _I, _I, _I]
         lib.orc_df_compact.argtypes = [_I, _I, _D, _D]
@@ -208,6 +210,13 @@ class OracleBox:
         out = np.zeros(s, order="F")
         self.lib.orc_box_get_haloed(self.h, FIELDS[name], _p(out))
         return out
+
+    def set_metrics(self, jacob, dxi):
+        """Haloed jacob (ni, nj, nk) and dxi (ni, nj, nk, 3, 3), dxi[..., d, e] =
+        d(xi_d)/d(x_e)."""
+        jac = np.asfortranarray(jacob, dtype=np.float64)
+        dx = np.asfortranarray(dxi, dtype=np.float64)
+        self.lib.orc_box_set_metrics(self.h, _p(jac), _p(dx))
 
     def set(self, name, a):
         a = np.asfortranarray(a, dtype=np.float64)

@@ -371,14 +371,15 @@ def test_tgv64_100_steps_against_checker_series():
     s.close()
 
 
-def test_curvilinear_metrics_rejected():
-    p = ag.default_params(32)
-    s = ag.AstrSolver(p)
-    jac, dxi = ag.box_metrics(p)
-    dxi[10, 10, 10, 0, 1] = 1e-3
-    with pytest.raises(ag.AstrError, match="UNSUPPORTED"):
-        s.upload_geom(jac, dxi)
-    s.close()
+def test_curvilinear_metrics_rejected_where_not_built():
+    for over in (dict(convection=1, diffterm=0), dict(variant=1)):
+        p = ag.default_params(32, **over)
+        s = ag.AstrSolver(p)
+        jac, dxi = ag.box_metrics(p)
+        dxi[10, 10, 10, 0, 1] = 1e-3
+        with pytest.raises(ag.AstrError, match="UNSUPPORTED"):
+            s.upload_geom(jac, dxi)
+        s.close()
 
 
 def test_fortran_host_drives_the_solver():
