@@ -166,6 +166,10 @@ struct SweepArgs {
     // is stored as it is, and k_metric_grad combines the three axes
     int raw;
     double h;
+    // jac holds one value everywhere (a uniform grid): j0, read from no
+    // array
+    int juni;
+    double j0;
 };
 
 // The RK3 combine in the reference's operation order, without FMA
@@ -255,7 +259,8 @@ struct TileView {
     double* gdst;        // gradient: dvel(m, dir) or dtmp(dir)
     double *q, *qo, *qrhs, *qsave, *fout;
     const double *jac, *xi;
-    int rk_stage, mini, raw;
+    int rk_stage, mini, raw, juni;
+    double j0;
     double rka, rkb, rkc, dt, h;
 };
 
@@ -292,6 +297,8 @@ __device__ __forceinline__ TileView tile_view(const A& a, int m)
     v.rk_stage = a.rk_stage;
     v.mini = a.mini;
     v.raw = a.raw;
+    v.juni = a.juni;
+    v.j0 = a.j0;
     v.rka = a.rka;
     v.rkb = a.rkb;
     v.rkc = a.rkc;
@@ -318,7 +325,7 @@ __device__ __forceinline__ void rhs_epilogue(const TileView& a, const double* tr
                 const unsigned int od = lob + (unsigned int)n * sdb;
                 r[g] = ldb(a.qrhs, od);
                 if constexpr (MODE != 0) q[g] = ldb(a.q, od);
-                if constexpr (MODE == 1 || MODE == 2) jv[g] = ldb(a.jac, od);
+                if constexpr (MODE == 1 || MODE == 2) jv[g] = a.juni ? a.j0 : ldb(a.jac, od);
                 if constexpr (MODE == 2 || MODE == 4) sv[g] = ldb(a.qsave, od);
             }
         }
@@ -837,6 +844,8 @@ struct PtArgs {
     // else the rectilinear xi lines
     int curv;
     const double* dxf[9];
+    int juni;  // jac is j0 everywhere
+    double j0;
     const double* jac;
     double const1, const2, const5, const6;
     double reynolds, prandtl, tempconst, tempconst1;
@@ -1133,7 +1142,7 @@ __device__ __forceinline__ NodeIn load_node(const PtArgs& a, long gi)
 #pragma unroll
     for (int c = 0; c < 3; ++c) v.u[c] = a.prim[1 + c][gi];
     v.p = a.prim[4][gi];
-    v.jc = a.mini ? 1.0 : a.jac[gi];
+    v.jc = a.mini ? 1.0 : (a.juni ? a.j0 : a.jac[gi]);
     return v;
 }
 
@@ -3043,6 +3052,8 @@ struct astr_ctx {
     double* cmp_ac[3][2] = {};      // compact upwind Thomas factors per axis, wind
     double len[3] = {1.0, 1.0, 0.0};  // gridcube lengths (astr_set_lengths)
     bool curv = false;                // general metrics uploaded (dxf)
+    bool juni = false;                // jacob uniform (j0): read from no array
+    double j0 = 0.0;
     double* dxf[9] = {};              // dxi(d, e) per node, 3d+e
     int cmp_first[3] = {}, cmp_last[3] = {};
     bool lines_ok[3] = {};          // axis fits the compact line sweeps
@@ -3339,6 +3350,8 @@ PtArgs pt_args(astr_ctx* c)
     a.mini = c->p.variant == 1;
     a.curv = c->curv;
     for (int m = 0; m < 9; ++m) a.dxf[m] = c->dxf[m];
+    a.juni = c->juni;
+    a.j0 = c->j0;
     a.per[0] = c->p.npdci == 3;
     a.per[1] = c->p.npdcj == 3;
     a.per[2] = c->p.npdck == 3;
@@ -3393,6 +3406,8 @@ SweepArgs sweep_args(astr_ctx* c, int dir, int pass)
     a.visc = c->p.diffterm;
     a.mini = c->p.variant == 1;
     a.raw = pass == PASS_GRAD && c->curv;
+    a.juni = c->juni;
+    a.j0 = c->j0;
     a.h = c->h[dir];
     return a;
 }
@@ -5055,6 +5070,11 @@ int astr_upload_geom(astr_ctx* c, const double* jacob, const double* dxi)
         c->curv = true;
     }
     TRY(upload_field(c, c->jac, jacob, stage));
+    // a uniform grid's jacob is one value: the kernels of the central path
+    // then take it as a scalar (the array stays uploaded for the others)
+    c->juni = true;
+    c->j0 = jacob[0];
+    for (long t = 1; t < nt && c->juni; ++t) c->juni = jacob[t] == c->j0;
     c->have_geom = true;
     return ASTR_OK;
 }
