@@ -846,6 +846,10 @@ struct PtArgs {
     const double* dxf[9];
     int juni;  // jac is j0 everywhere
     double j0;
+    // k_prim leaves the density out: within a step of the central path
+    // nothing reads it before the last stage's k_prim (the statistics read
+    // it at the next step's first stage)
+    int skip_rho;
     const double* jac;
     double const1, const2, const5, const6;
     double reynolds, prandtl, tempconst, tempconst1;
@@ -876,12 +880,13 @@ __device__ __forceinline__ void interior_ijk(const Geo& g, long p, int& i, int& 
 // without FMA contraction (in the reference's operation order, as the
 // upwind checker does)
 #pragma clang fp contract(off)
-__device__ __forceinline__ void q2fvar(const PtArgs& a, const double qv[5], long gi)
+__device__ __forceinline__ void q2fvar(const PtArgs& a, const double qv[5], long gi,
+                                       bool write_rho = true)
 {
     const double rho = qv[0];
     const double u = qv[1] / rho, v = qv[2] / rho, w = qv[3] / rho;
     const double p = (qv[4] - 0.5 * rho * (u * u + v * v + w * w)) / a.const6;
-    a.prim[0][gi] = rho;
+    if (write_rho) a.prim[0][gi] = rho;
     a.prim[1][gi] = u;
     a.prim[2][gi] = v;
     a.prim[3][gi] = w;
@@ -1471,10 +1476,11 @@ __device__ __forceinline__ void prim_body(const PtArgs& a, unsigned bid)
         double qv[5];
 #pragma unroll
         for (int m = 0; m < 5; ++m) qv[m] = a.q[m][gi];
-        q2fvar(a, qv, gi);
+        q2fvar(a, qv, gi, !a.skip_rho);
     } else {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) keep_cell(a.prim[m], gi);
+        for (int m = 0; m < 6; ++m)
+            if (m > 0 || !a.skip_rho) keep_cell(a.prim[m], gi);
     }
 }
 
@@ -3540,11 +3546,14 @@ int launch_pt(astr_ctx* c, void (*k)(PtArgs), long nthreads, const PtArgs& a, in
 }
 
 // k_prim over the interior rows cut at the end of the line holding node n
-int launch_prim(astr_ctx* c)
+// (skip_rho: the density left as it is)
+int launch_prim(astr_ctx* c, bool skip_rho = false)
 {
     const long n = (long)(c->g.n[1] + 1) * (c->g.n[2] + 1) * line_end_i(c->g.n[0]);
     const dim3 grid((unsigned)((n + 255) / 256));
-    hipLaunchKernelGGL(k_prim, grid, dim3(256), 0, c->stream, pt_args(c));
+    PtArgs a = pt_args(c);
+    a.skip_rho = skip_rho;
+    hipLaunchKernelGGL(k_prim, grid, dim3(256), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     return ASTR_OK;
 }
@@ -4014,6 +4023,8 @@ int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed)
     // interface fluxes), 6 update (central: primitives; upwind: RK3 combine
     // and primitives)
     const bool upwind = c->p.convection != 0;
+    const bool bc_reads_prim =
+        c->udf == ASTR_UDF_RIEMANN2D || c->bctype[1] == 23 || c->bctype[3] == 23;
     if (c->p.variant == 1) {
         // rk3 of tgvsolver.F90:994-1082: bchomo, rhscal (gradcal, convection,
         // diffusion by df/dx), stage-1 statistics and qsave, the RK3 combine
@@ -4074,7 +4085,9 @@ int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed)
         }
         if (timed) TRY(time_mark(c, 6));
         if (upwind) TRY(do_rk(c, s, dt));
-        else TRY(launch_prim(c));
+        // without a boundary kernel, the density of stages 1 and 2 is read by
+        // nothing before stage 3's k_prim writes it
+        else TRY(launch_prim(c, s < 3 && !bc_reads_prim));
         if (timed) {
             TRY(time_mark(c, 7));
             HIPCHK(hipEventSynchronize(c->ev[7]));
