@@ -220,17 +220,24 @@ __device__ __forceinline__ void block_tile(int bid, int nq, int ntiles, int& til
 // field is below 4 GB), so the address is one VGPR added to an SGPR pair
 // (every field is global memory: the accesses say so, so that pointers read
 // from an argument table rather than the kernel arguments also give global
-// rather than flat instructions)
+// rather than flat instructions).  Both are streaming (nontemporal): a
+// sweep reads each staged node once and its outputs are read back from HBM
+// by a later launch, so neither is worth a place in the caches.  Streaming
+// sweep accesses, k_prim's and k_gflux's streaming stores took a 256^3 step
+// from 15.0 to 14.2 ms; streaming loads in k_gflux, whose halo strips
+// re-read their periodic images, cost it 60 us per launch and are not used.
 typedef __attribute__((address_space(1))) const char* gcptr;
 typedef __attribute__((address_space(1))) char* gptr;
 __device__ __forceinline__ double ldb(const double* base, unsigned int boff)
 {
-    return *(__attribute__((address_space(1))) const double*)((gcptr)base + boff);
+    return __builtin_nontemporal_load((__attribute__((address_space(1))) const double*)((gcptr)base + boff));
 }
 __device__ __forceinline__ void stb(double* base, unsigned int boff, double v)
 {
-    *(__attribute__((address_space(1))) double*)((gptr)base + boff) = v;
+    __builtin_nontemporal_store(v, (__attribute__((address_space(1))) double*)((gptr)base + boff));
 }
+// streaming store of one double of a pointwise kernel
+__device__ __forceinline__ void st_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
 
 // Write-back of the y/z flux sweeps for the nodes n0 + nstep*e (e <
 // EMAX) of one lane, given the derivative o of G in the LDS row trow:
@@ -886,12 +893,12 @@ __device__ __forceinline__ void q2fvar(const PtArgs& a, const double qv[5], long
     const double rho = qv[0];
     const double u = qv[1] / rho, v = qv[2] / rho, w = qv[3] / rho;
     const double p = (qv[4] - 0.5 * rho * (u * u + v * v + w * w)) / a.const6;
-    if (write_rho) a.prim[0][gi] = rho;
-    a.prim[1][gi] = u;
-    a.prim[2][gi] = v;
-    a.prim[3][gi] = w;
-    a.prim[4][gi] = p;
-    a.prim[5][gi] = p / rho * a.const2;
+    if (write_rho) st_nt(&a.prim[0][gi], rho);
+    st_nt(&a.prim[1][gi], u);
+    st_nt(&a.prim[2][gi], v);
+    st_nt(&a.prim[3][gi], w);
+    st_nt(&a.prim[4][gi], p);
+    st_nt(&a.prim[5][gi], p / rho * a.const2);
 }
 #pragma clang fp contract(fast)
 
@@ -1282,13 +1289,13 @@ __device__ __forceinline__ void gflux_strip(const PtArgs& a, const int (&c)[3], 
     else if (d == 1) gflux_of(a, 1, xn, nd, sg, qf, out);
     else gflux_of(a, 2, xn, nd, sg, qf, out);
 #pragma unroll
-    for (int m = 0; m < 5; ++m) a.gfl[5 * d + m][gi] = out[m];
+    for (int m = 0; m < 5; ++m) st_nt(&a.gfl[5 * d + m][gi], out[m]);
 }
 
 __device__ __forceinline__ void gflux_zero(const PtArgs& a, int d, long gi)
 {
 #pragma unroll
-    for (int m = 0; m < 5; ++m) a.gfl[5 * d + m][gi] = 0.0;
+    for (int m = 0; m < 5; ++m) st_nt(&a.gfl[5 * d + m][gi], 0.0);
 }
 
 // The halo node along an axis of extent n whose periodic image is node c
@@ -1326,14 +1333,14 @@ __device__ __forceinline__ void gflux_cell(const PtArgs& a, int i, int j, int k,
             for (int e = 0; e < 3; ++e) gflux_of(a, e, a.xi[e][c[e] + HM], nd, sg, qf, out + 5 * e);
         }
 #pragma unroll
-        for (int k = 0; k < 15; ++k) a.gfl[k][gi] = out[k];
+        for (int k = 0; k < 15; ++k) st_nt(&a.gfl[k][gi], out[k]);
         if (hj != NO_HALO) {
             const long gh = gidx(a.g, i, hj, c[2]);
             const NodeIn nh = load_node(a, gh);
             if (a.curv) gflux_of_gen(a, 1, gh, nh, sg, qf, out);
             else gflux_of(a, 1, a.xi[1][hj + HM], nh, sg, qf, out);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) a.gfl[5 + m][gh] = out[m];
+            for (int m = 0; m < 5; ++m) st_nt(&a.gfl[5 + m][gh], out[m]);
         }
         if (hk != NO_HALO) {
             const long gh = gidx(a.g, i, c[1], hk);
@@ -1341,7 +1348,7 @@ __device__ __forceinline__ void gflux_cell(const PtArgs& a, int i, int j, int k,
             if (a.curv) gflux_of_gen(a, 2, gh, nh, sg, qf, out);
             else gflux_of(a, 2, a.xi[2][hk + HM], nh, sg, qf, out);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) a.gfl[10 + m][gh] = out[m];
+            for (int m = 0; m < 5; ++m) st_nt(&a.gfl[10 + m][gh], out[m]);
         }
         return;
     }
