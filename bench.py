@@ -51,13 +51,16 @@ FAMILY_DOUBLES = {
     "filter": 10.0,   # q(1:5) in and out, per axis
     "qswap": None,    # boundary planes only
     "grad": 8.0,      # u, v, w, T in; dvel(.,d), dtmp(d) out
-    # dvel 9, dtmp 3, T, u, v, w, q 5, p, J in; the axis fluxes G (3 x 5) out
-    "gflux": 38.0,
-    "stats": None,    # once per step (14 doubles per node), not per stage
-    # G(d,1:5) in; x: qrhs out (10); y: qrhs in and out (15); z: qrhs, J,
-    # q, qsave in, q out (26; stage 1: qsave out instead of in)
-    "flux": (10.0 + 15.0 + 26.0) / 3.0,
-    "prim": 11.0,     # q 5 in; rho, u, v, w, p, T out
+    # dvel 9, dtmp 3, T, u, v, w, q 5, p in; the axis fluxes G (3 x 5) out
+    # (a grid whose Jacobian varies also reads J; the box's is one scalar)
+    "gflux": 37.0,
+    "stats": None,    # inside stage 1's k_gflux: one more load (rho) per step
+    # G(d,1:5) in; x: qrhs out (10); y: qrhs in and out (15); z: qrhs, q,
+    # qsave in, q out (25; stage 1: qsave out instead of in; J a scalar)
+    "flux": (10.0 + 15.0 + 25.0) / 3.0,
+    # q 5 in; u, v, w, p, T out, and rho at the last stage (no kernel reads
+    # the density in between)
+    "prim": (10.0 + 10.0 + 11.0) / 3.0,
 }
 FAMILY_KERNEL = {"filter": "k_sweep<L,0> (filter)", "grad": "k_sweep<L,1> (gradient)",
                  "gflux": "k_gflux", "flux": "k_sweep<L,2> (flux derivatives, RK3 in z)",
@@ -65,7 +68,7 @@ FAMILY_KERNEL = {"filter": "k_sweep<L,0> (filter)", "grad": "k_sweep<L,1> (gradi
 FAMILY_LAUNCHES = {"filter": 3, "qswap": 2, "grad": 3, "gflux": 1, "stats": 0, "flux": 3,
                    "prim": 1}
 FAMILY_NAMES = ["filter", "qswap", "grad", "gflux", "stats", "flux", "prim"]
-STATS_DOUBLES = 14.0
+STATS_DOUBLES = 1.0  # the statistics' rho, loaded by stage 1's k_gflux
 # the reference's own pass structure: 176 doubles per point per stage
 REF_BYTES_PER_PT_STEP = 4224.0
 # whole-step algorithmic bytes per node: three stages of the kernels above
