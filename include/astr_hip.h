@@ -336,9 +336,12 @@ int astr_set_spacing(astr_ctx *ctx, double dx, double dy, double dz);
  * 1 and 3, zero-slope extrapolation on faces 2 and 4, the north-east state
  * in the corner (im,jm).  23 is gcnscbc (src/bc.F90:7907-8137, boucon's
  * bctype 23 with prs_t = pinf): the ghost-cell NSCBC on face 2 (i=im) and
- * face 4 (j=jm), as the reference has it (it acts on no other face);
- * two-dimensional boxes; it reads node coordinates of gridcube's grid
- * (astr_set_lengths).  The other bctypes of bc.F90 (inflow, outflow,
+ * face 4 (j=jm), as the reference has it (it acts on no other face),
+ * over every k: two-dimensional boxes on the upwind path and
+ * three-dimensional boxes on the central path (a bounded x or y axis);
+ * batched members and the adjoint take no boundary condition
+ * (ASTR_E_UNSUPPORTED there).  It reads node coordinates of gridcube's
+ * grid (astr_set_lengths).  The other bctypes of bc.F90 (inflow, outflow,
  * walls, the other NSCBCs) return ASTR_E_UNSUPPORTED.  Default: all
  * faces 1. */
 #define ASTR_UDF_RIEMANN2D 1

@@ -467,6 +467,7 @@ struct orc_box {
     double const1, const2, const3, const4, const5, const6, const7;
     double tempconst, tempconst1, roinf, uinf, pinf, tinf;
     double lx, ly, lz, dx, dy, dz;
+    int bc[6]; /* bctype of faces 1..6: 0 nothing done here, 23 gcnscbc */
 };
 
 #define IX(b, i, j, k) ((size_t)((i) + HM) + (size_t)(b)->ni * ((size_t)((j) + HM) + (size_t)(b)->nj * (size_t)((k) + HM)))
@@ -557,6 +558,39 @@ int orc_box_set_ntype(orc_box *b, int n0, int n1, int n2)
         scheme_filter(&b->fl[d], nt[d], dims[d], b->alfa);
     }
     return 0;
+}
+
+int orc_box_set_bc(orc_box *b, const int *bctype)
+{
+    if (b->variant != 0) return -1;
+    for (int f = 0; f < 6; ++f)
+        if (bctype[f] != 0 && bctype[f] != 23) return -1;
+    for (int f = 0; f < 6; ++f) b->bc[f] = bctype[f];
+    return 0;
+}
+
+/* boucon (src/bc.F90:327-411) with gcnscbc (src/bc.F90:7907-8140) on faces
+ * 2 and 4, in that order: the restatement of upwind.c (orc_gcnscbc) run on
+ * this box's fields (the same haloed layout), its lengths the spans of the
+ * gridcube coordinates x(i) = lx/im*i, y(j) = ly/jm*j */
+static void box_boucon(orc_box *b)
+{
+    if (b->bc[1] != 23 && b->bc[3] != 23) return;
+    orc_uwd u;
+    memset(&u, 0, sizeof u);
+    u.im = b->im; u.jm = b->jm; u.km = b->km;
+    u.gamma = b->gamma; u.mach = b->mach;
+    u.lx = b->lx; u.ly = b->ly; u.lz = b->lz;
+    const size_t n = b->n3;
+    double *q = malloc(5 * n * sizeof(double)), *vel = malloc(3 * n * sizeof(double));
+    for (int m = 0; m < 5; ++m) memcpy(q + m * n, b->q[m], n * sizeof(double));
+    for (int c = 0; c < 3; ++c) memcpy(vel + c * n, b->vel[c], n * sizeof(double));
+    if (b->bc[1] == 23) orc_gcnscbc(&u, 2, q, b->rho, vel, b->prs, b->tmp);
+    if (b->bc[3] == 23) orc_gcnscbc(&u, 4, q, b->rho, vel, b->prs, b->tmp);
+    for (int m = 0; m < 5; ++m) memcpy(b->q[m], q + m * n, n * sizeof(double));
+    for (int c = 0; c < 3; ++c) memcpy(b->vel[c], vel + c * n, n * sizeof(double));
+    free(q);
+    free(vel);
 }
 
 void orc_box_free(orc_box *b)
@@ -1310,6 +1344,7 @@ void orc_box_step(orc_box *b, double dt, double *stats)
         for (int s = 0; s < 3; ++s) {
             orc_box_filterq(b);
             zero_qrhs(b);
+            box_boucon(b);  /* boucon before qswap, src/mainloop.F90:413-415 */
             orc_box_qswap(b);
             orc_box_rhscal(b);
             if (s == 0) {

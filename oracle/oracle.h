@@ -80,6 +80,9 @@ void orc_box_set_sutherland(orc_box *b, double s_over_tref_numerator);
 void orc_box_set_metrics(orc_box *b, const double *jacob, const double *dxi);
 /* closure per axis (3 periodic, 1/2/4 bounded); main variant; 0 or -1 */
 int orc_box_set_ntype(orc_box *b, int n0, int n1, int n2);
+/* bctype of faces 1..6 for the main variant's steps: 0 (nothing done) or
+ * 23 (gcnscbc, src/bc.F90:7907-8140, acting on faces 2 and 4); 0 or -1 */
+int orc_box_set_bc(orc_box *b, const int *bctype);
 
 /* Set the Taylor-Green initial field: tgvini (src/initialisation.F90:1079-1149)
  * followed by updateq (src/initialisation.F90:30-162 / src/fludyna.F90:254)

@@ -56,6 +56,8 @@ def oracle_lib():
         lib.orc_box_set_metrics.argtypes = [_V, _D, _D]
         lib.orc_box_set_ntype.restype = _I
         lib.orc_box_set_ntype.argtypes = [_V, _I, _I, _I]
+        lib.orc_box_set_bc.restype = _I
+        lib.orc_box_set_bc.argtypes = [_V, ctypes.POINTER(ctypes.c_int)]
         lib.orc_df_compact.argtypes = [_I, _I, _D, _D]
         lib.orc_diff6ec.argtypes = [_I, _I, _D, _D]
         lib.orc_compact_filter.argtypes = [_I, _I, _F, _F, _F, _D, _D]
@@ -192,6 +194,14 @@ class OracleBox:
             self.close()
         except Exception:
             pass
+
+    def set_bc(self, bctype):
+        """Boundary condition of faces 1..6 for the steps: 0 (nothing done)
+        or 23 (gcnscbc, acting on faces 2 and 4 before each stage's qswap);
+        the main variant only."""
+        arr = (ctypes.c_int * 6)(*[int(v) for v in bctype])
+        if self.lib.orc_box_set_bc(self.h, arr) != 0:
+            raise ValueError(f"bctype {tuple(bctype)}: 0 or 23 per face, main variant")
 
     @property
     def shape(self):

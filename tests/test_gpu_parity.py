@@ -136,6 +136,52 @@ def test_bounded_axes_central_steps_match_checker(npdc):
     s.close()
 
 
+def test_gcnscbc_central_steps_match_checker():
+    """gcnscbc (bctype 23) on the x-high and y-high faces of a three-dimensional
+    box (bounded x and y, periodic z) through whole RK3 steps of the central
+    path against the checker: statistics at 1e-10, every node and the
+    periodic axis' halo faces at 1e-10 after 3 steps.  Ma 0.4 keeps the
+    integer sound speed gcnscbc takes (sqrt(T)/Ma near 2.5) away from a
+    truncation boundary."""
+    from test_oracle_bounded import GCN_FACES, pulse
+
+    dims, mach = (32, 28, 24), 0.4
+    print(f"case gcnscbc central dims={dims}", flush=True)
+    ob = pyoracle.OracleBox(dims, variant=0, npdc=(4, 4, 3), mach=mach)
+    ob.set_bc(GCN_FACES)
+    pulse(ob, mach)
+    st = ob.state_haloed()
+    p = ag.default_params(dims, npdci=4, npdcj=4, npdck=3, mach=mach)
+    s = ag.AstrSolver(p)
+    s.upload_geom(*ag.box_metrics(p))
+    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+    s.set_lengths(2 * np.pi, 2 * np.pi, 2 * np.pi)
+    s.set_bc(tuple(23 if f == 23 else 1 for f in GCN_FACES), 0)
+    dt = 0.01
+    for n in range(3):
+        so = ob.step(dt, stats=True)
+        sg = s.time_integration_rk(dt, stats=True)
+        for a, b in zip(sg, so):
+            assert abs(a - b) <= 1e-10 * abs(b), (n, sg, so)
+    print(f"  3 steps {compare_state(ob, s, p, 1e-10, halo_faces=True, vector_scale=True):.2e}")
+    s.close()
+
+
+def test_gcnscbc_rejected_by_batch_and_adjoint():
+    dims = (16, 16, 16)
+    p = ag.default_params(dims)
+    s = ag.AstrSolver(p)
+    s.upload_geom(*ag.box_metrics(p))
+    st = ag.tgv_state(p)
+    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+    s.set_bc((1, 23, 1, 1, 1, 1), 0)
+    with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
+        s.adjoint_enstrophy(2, 0.001, 2)
+    with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
+        ag.AstrBatch([s])
+    s.close()
+
+
 def test_bounded_axes_rejected_where_not_built():
     dims = (32, 32, 32)
     for over in (dict(npdci=4, convection=1, diffterm=0), dict(npdcj=4, variant=1),

@@ -234,11 +234,3 @@ def _boucon_ref(box, st, faces):
         if b == 23:
             o = box.gcnscbc(o, f + 1)
     return o
-
-
-def test_gcnscbc_rejected_in_three_dimensions():
-    p = ag.default_params((32, 32, 32))
-    s = ag.AstrSolver(p)
-    with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
-        s.set_bc((1, 23, 1, 1, 1, 1), 0)
-    s.close()
