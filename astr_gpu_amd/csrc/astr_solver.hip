@@ -170,6 +170,11 @@ struct SweepArgs {
     // array
     int juni;
     double j0;
+    // the RK3 combine with qsave holding the stage-1 state q itself rather
+    // than q J: stage 1 stores no qsave (its state stays in the buffer it
+    // was read from while the result goes to qo), stages 2-3 form q J from
+    // it (the same product, rounded alike)
+    int qsraw;
 };
 
 // The RK3 combine in the reference's operation order, without FMA
@@ -254,6 +259,7 @@ __device__ __forceinline__ void stc(double* base, unsigned int boff, double v)
 //   MODE 0  qrhs += o (y axis, or z when no RK3 combine follows)
 //   MODE 1  z axis, RK3 stage 1: qsave = q J, then the combine
 //   MODE 2  z axis, RK3 stages 2-3: the combine with the saved qsave
+//   (qsraw: stage 1 stores nothing, stages 2-3 take qsave J)
 //   MODE 3/4 the mini-app's combine without J (stage 1 / stages 2-3)
 // The operands of a batch of RB nodes are loaded before the stores of the
 // previous batch (a store to q or qrhs could alias a later load, so the
@@ -276,7 +282,7 @@ struct TileView {
     double* gdst;        // gradient: dvel(m, dir) or dtmp(dir)
     double *q, *qo, *qrhs, *qsave, *fout;
     const double *jac, *xi;
-    int rk_stage, mini, raw, juni;
+    int rk_stage, mini, raw, juni, qsraw;
     double j0;
     double rka, rkb, rkc, dt, h;
 };
@@ -315,6 +321,7 @@ __device__ __forceinline__ TileView tile_view(const A& a, int m)
     v.mini = a.mini;
     v.raw = a.raw;
     v.juni = a.juni;
+    v.qsraw = a.qsraw;
     v.j0 = a.j0;
     v.rka = a.rka;
     v.rkb = a.rkb;
@@ -367,9 +374,9 @@ __device__ __forceinline__ void rhs_epilogue(const TileView& a, const double* tr
                     double qs;
                     if constexpr (MODE == 1) {
                         qs = cq[g] * cj[g];
-                        stb(a.qsave, od, qs);
+                        if (!a.qsraw) stb(a.qsave, od, qs);
                     } else {
-                        qs = cs[g];
+                        qs = a.qsraw ? cs[g] * cj[g] : cs[g];
                     }
                     stb(a.qo, od,
                         keep ? cq[g] : rk_combine(a.rka, a.rkb, a.rkc, a.dt, qs, cq[g], cj[g], r));
@@ -3669,13 +3676,15 @@ int do_boucon(astr_ctx* c)
 // primitives.
 const double RKC[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
 
-int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt, double* const* qo = nullptr)
+int do_flux_sweeps(astr_ctx* c, int rk_stage, double dt, double* const* qo = nullptr,
+                   bool qsraw = false)
 {
     for (int d = 0; d < 3; ++d) {
         SweepArgs a = sweep_args(c, d, PASS_RHS);
         if (qo)
             for (int m = 0; m < 5; ++m) a.qo[m] = qo[m];
         if (d == 2 && rk_stage > 0) {
+            a.qsraw = qsraw;
             a.rk_stage = rk_stage;
             a.rka = RKC[rk_stage - 1][0];
             a.rkb = RKC[rk_stage - 1][1];
@@ -4034,7 +4043,13 @@ int time_mark(astr_ctx* c, int slot)
     return ASTR_OK;
 }
 
-int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed)
+// alt (whole steps of one box, central path): stage 1's RK3 combine writes
+// its result into the qsave buffer and leaves the stage's state where it
+// was read, and the two buffers swap roles (q <-> qsave) for the rest of the
+// step, so no qsave = q J is stored (5 fields per step fewer).  The
+// buffers then alternate from step to step; a generation bump makes a
+// batch that holds this context rebuild its launches.
+int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed, bool alt = false)
 {
     // timing families: 0 filter, 1 boucon and qswap, 2 grad, 3 central:
     // sigma and the axis fluxes (k_gflux), upwind: the shock sensor,
@@ -4099,6 +4114,12 @@ int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed)
         if (timed) TRY(time_mark(c, 5));
         if (upwind) {
             TRY(do_upwind(c));
+        } else if (alt) {
+            TRY(do_flux_sweeps(c, s, dt, s == 1 ? c->qsave : nullptr, true));
+            if (s == 1) {
+                for (int m = 0; m < 5; ++m) std::swap(c->q[m], c->qsave[m]);
+                ++c->gen;
+            }
         } else {
             TRY(do_flux_sweeps(c, s, dt));
         }
@@ -5362,7 +5383,7 @@ static int rk3_steps(astr_ctx* c, int32_t nsteps, double dt, double* stats, bool
     int done = 0;
     for (int s = 0; s < nsteps; ++s) {
         const int slot = s - done;
-        TRY(do_step(c, dt, stats ? c->stats_dev + 3 * slot : nullptr, c->timing == 1));
+        TRY(do_step(c, dt, stats ? c->stats_dev + 3 * slot : nullptr, c->timing == 1, true));
         if (stats && (slot + 1 == STATS_SLOTS || s + 1 == nsteps)) {
             HIPCHK(hipMemcpyAsync(stats + 3 * (long)done, c->stats_dev,
                                   sizeof(double) * 3 * (slot + 1), hipMemcpyDeviceToHost,

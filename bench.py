@@ -56,8 +56,9 @@ FAMILY_DOUBLES = {
     "gflux": 37.0,
     "stats": None,    # inside stage 1's k_gflux: one more load (rho) per step
     # G(d,1:5) in; x: qrhs out (10); y: qrhs in and out (15); z: qrhs, q,
-    # qsave in, q out (25; stage 1: qsave out instead of in; J a scalar)
-    "flux": (10.0 + 15.0 + 25.0) / 3.0,
+    # qsave in, q out (25 at stages 2 and 3; 20 at stage 1, where q and
+    # qsave swap roles instead of storing qsave; J a scalar)
+    "flux": (10.0 + 15.0 + (20.0 + 25.0 + 25.0) / 3.0) / 3.0,
     # q 5 in; u, v, w, p, T out, and rho at the last stage (no kernel reads
     # the density in between)
     "prim": (10.0 + 10.0 + 11.0) / 3.0,
