@@ -1761,27 +1761,82 @@ __device__ __forceinline__ void q2fvar_adj(const AdjArgs& a, const double* q, co
     qbv[4] += q4b;
 }
 
-// qswap's refresh of the primitives reversed at the halo cells (one axis
-// outside 0..n) and the seam nodes (interior, some coordinate 0 or n), at
-// q after qswap: q-bar collects the primitive-bar there, which is zeroed
-// (the forward overwrote those primitives).  One thread per node of the
-// padded box (x over the (i,j) plane, y over k).
+// The nodes whose primitives qswap refreshes: the halo cells with one
+// axis outside 0..n (the 2*HM slabs of each axis over the other two axes'
+// 0..n), then the seam nodes (every coordinate in 0..n, some 0 or n), each
+// seam node listed once, under its first such axis.  r-th node in c;
+// consecutive r walk along i where the slab or seam allows.
+__host__ __device__ inline long q2f_adj_count(const Geo& g)
+{
+    const long e0 = g.n[0] + 1L, e1 = g.n[1] + 1L, e2 = g.n[2] + 1L;
+    return 2L * HM * (e1 * e2 + e0 * e2 + e0 * e1) + 2 * e1 * e2 + 2 * (e0 - 2) * e2 +
+           2 * (e0 - 2) * (e1 - 2);
+}
+
+__host__ __device__ inline bool q2f_adj_node(const Geo& g, long r, int (&c)[3])
+{
+    const long ext[3] = {g.n[0] + 1L, g.n[1] + 1L, g.n[2] + 1L};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
+        const long ns = 2L * HM * ext[oa] * ext[ob];
+        if (r < ns) {
+            int h;
+            if (d == 0) {  // the slab cells of a row lie along i
+                h = (int)(r % (2 * HM));
+                r /= 2 * HM;
+                c[oa] = (int)(r % ext[oa]);
+                r /= ext[oa];
+            } else {
+                c[oa] = (int)(r % ext[oa]);
+                r /= ext[oa];
+                h = (int)(r % (2 * HM));
+                r /= 2 * HM;
+            }
+            c[ob] = (int)r;
+            c[d] = h < HM ? -(h + 1) : g.n[d] + (h - HM + 1);
+            return true;
+        }
+        r -= ns;
+    }
+    const long ni = ext[0] - 2, nj = ext[1] - 2;
+    long ns = 2 * ext[1] * ext[2];  // i = 0 or n, j and k in 0..n
+    if (r < ns) {
+        c[0] = (r & 1) ? g.n[0] : 0;
+        r >>= 1;
+        c[1] = (int)(r % ext[1]);
+        c[2] = (int)(r / ext[1]);
+        return true;
+    }
+    r -= ns;
+    ns = 2 * ni * ext[2];  // j = 0 or n, i in 1..n-1, k in 0..n
+    if (r < ns) {
+        c[0] = 1 + (int)(r % ni);
+        r /= ni;
+        c[1] = (r & 1) ? g.n[1] : 0;
+        c[2] = (int)(r >> 1);
+        return true;
+    }
+    r -= ns;
+    ns = 2 * ni * nj;  // k = 0 or n, i and j in 1..n-1
+    if (r < ns) {
+        c[0] = 1 + (int)(r % ni);
+        r /= ni;
+        c[2] = (r & 1) ? g.n[2] : 0;
+        c[1] = 1 + (int)(r >> 1);
+        return true;
+    }
+    return false;
+}
+
+// qswap's refresh of the primitives reversed at those nodes, at q after
+// qswap: q-bar collects the primitive-bar there, which is zeroed (the
+// forward overwrote those primitives).  One thread per node.
 __global__ __launch_bounds__(256) void k_q2f_adj(AdjArgs a)
 {
-    const unsigned pw = (unsigned)(a.g.n[0] + 1 + 2 * HM);
-    const unsigned ph = (unsigned)(a.g.n[1] + 1 + 2 * HM);
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= pw * ph) return;
-    const unsigned jj = t / pw;
-    const int c[3] = {(int)(t - jj * pw) - HM, (int)jj - HM, (int)blockIdx.y - HM};
-    int nout = 0;
-    bool seam = false;
-#pragma unroll
-    for (int e = 0; e < 3; ++e) {
-        if (c[e] < 0 || c[e] > a.g.n[e]) ++nout;
-        else if (c[e] == 0 || c[e] == a.g.n[e]) seam = true;
-    }
-    if (nout > 1 || (nout == 0 && !seam)) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int c[3];
+    if (!q2f_adj_node(a.g, t, c)) return;
     const long gi = gidx(a.g, c[0], c[1], c[2]);
     // every operand before the first store (a store could alias a later load)
     double q[5], pbv[6], qbv[5];
@@ -1873,16 +1928,24 @@ __device__ __forceinline__ void inviscid_adj(int d, double xn, double jc, const 
     pbv[4] += pbar;
 }
 
-// G on the halo strips of axis d: the inviscid part at the halo node itself
-// (its viscous part, taken at the periodic image, is reversed by
-// k_gflux_adj_int).  One thread per halo node, i fastest within a row.
+// G on the halo strips of every axis d: the inviscid part at the halo node
+// itself (its viscous part, taken at the periodic image, is reversed by
+// k_gflux_adj_int).  One thread per halo node, the x strips first, then y,
+// then z (disjoint node sets), i fastest within a row.
+__host__ __device__ inline long gflux_adj_halo_count(const Geo& g, int d)
+{
+    const long e0 = g.n[0] + 1L, e1 = g.n[1] + 1L, e2 = g.n[2] + 1L;
+    return 2L * HM * (d == 0 ? e1 * e2 : d == 1 ? e0 * e2 : e0 * e1);
+}
+
 __global__ void k_gflux_adj_halo(AdjArgs a)
 {
-    const int d = a.axis;
     const long ext[3] = {a.g.n[0] + 1, a.g.n[1] + 1, a.g.n[2] + 1};
-    const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= 2L * HM * ext[oa] * ext[ob]) return;
+    int d = 0;
+    while (d < 3 && r >= gflux_adj_halo_count(a.g, d)) r -= gflux_adj_halo_count(a.g, d++);
+    if (d == 3) return;
+    const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
     int c[3], h;
     if (d == 0) {
         // x strips: the 2 HM halo nodes of a row are the contiguous ones
@@ -2067,10 +2130,20 @@ __global__ void k_qswap_adj_line(AdjArgs a)
     const long nl = ext[oa] * ext[ob];
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= (2 * HM + 1) * 5 * nl) return;
-    const int op = (int)(r / (5 * nl));
-    r -= op * 5 * nl;
-    const int m = (int)(r / nl);
-    r -= m * nl;
+    int op, m;
+    if (d == 0) {
+        // x lines: the pieces of one row are neighbours (the same lines)
+        op = (int)(r % (2 * HM + 1));
+        r /= 2 * HM + 1;
+        m = (int)(r % 5);
+        r /= 5;
+    } else {
+        // y and z lines: neighbouring threads take neighbouring i
+        op = (int)(r / (5 * nl));
+        r -= op * 5 * nl;
+        m = (int)(r / nl);
+        r -= m * nl;
+    }
     int c[3];
     c[oa] = (int)(r % ext[oa]);
     c[ob] = (int)(r / ext[oa]);
@@ -4248,12 +4321,6 @@ AdjArgs adj_args(astr_ctx* c, double* const* q, double* const* prim, double* con
     return a;
 }
 
-dim3 padded_grid(const astr_ctx* c)
-{
-    const long plane = (long)(c->g.n[0] + 1 + 2 * HM) * (c->g.n[1] + 1 + 2 * HM);
-    return dim3((unsigned)((plane + 255) / 256), (unsigned)(c->g.n[2] + 1 + 2 * HM));
-}
-
 // a row_node kernel over the interior rows
 template <class K>
 int launch_rows(astr_ctx* c, K kernel, const AdjArgs& a)
@@ -4362,8 +4429,7 @@ int grad_adj(astr_ctx* c)
 // halo and seam nodes, then the copies and averages in the order z, y, x
 int qswap_adj(astr_ctx* c, AdjArgs a)
 {
-    hipLaunchKernelGGL(k_q2f_adj, padded_grid(c), dim3(256), 0, c->stream, a);
-    HIPCHK(hipGetLastError());
+    TRY(launch_adj(c, k_q2f_adj, q2f_adj_count(c->g), a));
     const long ext[3] = {c->g.n[0] + 1L, c->g.n[1] + 1L, c->g.n[2] + 1L};
     for (int d = 2; d >= 0; --d) {
         a.axis = d;
@@ -4391,12 +4457,10 @@ int stage_adj(astr_ctx* c, const StageStore& st, int s, double dt)
         TRY(launch_sweep_t(c, d, OP_DERIV, 5, f.rb, f.gb + 5 * d, nullptr, false, false));
     // the axis fluxes, at the stored gradients
     TRY(launch_rows(c, k_gflux_adj_int, a));
-    const long ext[3] = {c->g.n[0] + 1L, c->g.n[1] + 1L, c->g.n[2] + 1L};
-    for (int d = 0; d < 3; ++d) {
-        a.axis = d;
-        const long nh = 2L * HM * (d == 0 ? ext[1] * ext[2] : d == 1 ? ext[0] * ext[2] : ext[0] * ext[1]);
-        TRY(launch_adj(c, k_gflux_adj_halo, nh, a));
-    }
+    TRY(launch_adj(c, k_gflux_adj_halo,
+                   gflux_adj_halo_count(c->g, 0) + gflux_adj_halo_count(c->g, 1) +
+                       gflux_adj_halo_count(c->g, 2),
+                   a));
     TRY(grad_adj(c));
     TRY(qswap_adj(c, a));
     // filterq: the filters of z, y, x transposed, in place
@@ -5986,7 +6050,7 @@ int astr_adjoint_record(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt, do
         if (seg == nseg - 1) {
             TRY(segment_fwd(c, nstates, 3 * (t1 - t0), dt));
         } else {
-            for (long t = t0; t < t1; ++t) TRY(do_step(c, dt, nullptr, false));
+            for (long t = t0; t < t1; ++t) TRY(do_step(c, dt, nullptr, false, true));
         }
     }
     TRY(save_state(c, nseg));
