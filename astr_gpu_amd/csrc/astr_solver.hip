@@ -2351,8 +2351,9 @@ __device__ __forceinline__ double diff6ec_at(const double* base, long stride, in
 
 // gradcal with the explicit 6th-order derivative (difschm '643e'),
 // src/comsolver.F90:244-497: dvel(m,d) = d/dxi_d(u_m) * dxi(d,d)
-// (streaming stores, as in k_rk and k_zero_qrhs: 2048^2 Riemann step
-// 4.01 -> 3.92 ms; k_uwd's qrhs stores stay cached: streaming them gained nothing)
+// (streaming stores, as in k_rk: 2048^2 Riemann step 4.01 -> 3.89 ms; the
+// qrhs that k_zero_qrhs clears and k_uwd accumulates stays cached, which
+// measured 40 us per step faster than streaming it)
 __global__ void k_grad_explicit(GradXArgs a)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2570,7 +2571,7 @@ __global__ void k_zero_qrhs(UwdArgs a)
     int ci, cj, ck;
     split3(p, ni, nj, ci, cj, ck);
     const long gi = gidx(a.g, ci, cj, ck);
-    for (int m = 0; m < 5; ++m) st_nt(&a.qrhs[m][gi], 0.0);
+    for (int m = 0; m < 5; ++m) a.qrhs[m][gi] = 0.0;
 }
 
 // convrsduwd along one axis, src/solver.F90:549-1202 (no species, crinod
