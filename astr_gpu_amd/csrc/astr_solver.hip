@@ -1773,57 +1773,78 @@ __host__ __device__ inline long q2f_adj_count(const Geo& g)
            2 * (e0 - 2) * (e1 - 2);
 }
 
-__host__ __device__ inline bool q2f_adj_node(const Geo& g, long r, int (&c)[3])
+// the r-th cell of the 2*HM halo slabs of axis D (over the other two axes'
+// 0..n) as (i, j, k); x slabs walk the 2*HM cells of a row first, y and z
+// slabs walk i first
+template <int D>
+__host__ __device__ inline void q2f_adj_slab(const Geo& g, long r, int& i, int& j, int& k)
 {
-    const long ext[3] = {g.n[0] + 1L, g.n[1] + 1L, g.n[2] + 1L};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
-        const long ns = 2L * HM * ext[oa] * ext[ob];
-        if (r < ns) {
-            int h;
-            if (d == 0) {  // the slab cells of a row lie along i
-                h = (int)(r % (2 * HM));
-                r /= 2 * HM;
-                c[oa] = (int)(r % ext[oa]);
-                r /= ext[oa];
-            } else {
-                c[oa] = (int)(r % ext[oa]);
-                r /= ext[oa];
-                h = (int)(r % (2 * HM));
-                r /= 2 * HM;
-            }
-            c[ob] = (int)r;
-            c[d] = h < HM ? -(h + 1) : g.n[d] + (h - HM + 1);
-            return true;
-        }
-        r -= ns;
+    constexpr int OA = D == 0 ? 1 : 0, OB = D == 2 ? 1 : 2;
+    const long ea = g.n[OA] + 1L;
+    int h, ca, cb;
+    if (D == 0) {
+        h = (int)(r % (2 * HM));
+        r /= 2 * HM;
+        ca = (int)(r % ea);
+        cb = (int)(r / ea);
+    } else {
+        ca = (int)(r % ea);
+        r /= ea;
+        h = (int)(r % (2 * HM));
+        cb = (int)(r / (2 * HM));
     }
-    const long ni = ext[0] - 2, nj = ext[1] - 2;
-    long ns = 2 * ext[1] * ext[2];  // i = 0 or n, j and k in 0..n
+    const int cd = h < HM ? -(h + 1) : g.n[D] + (h - HM + 1);
+    i = D == 0 ? cd : ca;
+    j = D == 1 ? cd : (D == 0 ? ca : cb);
+    k = D == 2 ? cd : cb;
+}
+
+__host__ __device__ inline bool q2f_adj_node(const Geo& g, long r, int& i, int& j, int& k)
+{
+    const long e0 = g.n[0] + 1L, e1 = g.n[1] + 1L, e2 = g.n[2] + 1L;
+    long ns = 2L * HM * e1 * e2;
     if (r < ns) {
-        c[0] = (r & 1) ? g.n[0] : 0;
-        r >>= 1;
-        c[1] = (int)(r % ext[1]);
-        c[2] = (int)(r / ext[1]);
+        q2f_adj_slab<0>(g, r, i, j, k);
         return true;
     }
     r -= ns;
-    ns = 2 * ni * ext[2];  // j = 0 or n, i in 1..n-1, k in 0..n
+    ns = 2L * HM * e0 * e2;
     if (r < ns) {
-        c[0] = 1 + (int)(r % ni);
+        q2f_adj_slab<1>(g, r, i, j, k);
+        return true;
+    }
+    r -= ns;
+    ns = 2L * HM * e0 * e1;
+    if (r < ns) {
+        q2f_adj_slab<2>(g, r, i, j, k);
+        return true;
+    }
+    r -= ns;
+    const long ni = e0 - 2, nj = e1 - 2;
+    ns = 2 * e1 * e2;  // i = 0 or n, j and k in 0..n
+    if (r < ns) {
+        i = (r & 1) ? g.n[0] : 0;
+        r >>= 1;
+        j = (int)(r % e1);
+        k = (int)(r / e1);
+        return true;
+    }
+    r -= ns;
+    ns = 2 * ni * e2;  // j = 0 or n, i in 1..n-1, k in 0..n
+    if (r < ns) {
+        i = 1 + (int)(r % ni);
         r /= ni;
-        c[1] = (r & 1) ? g.n[1] : 0;
-        c[2] = (int)(r >> 1);
+        j = (r & 1) ? g.n[1] : 0;
+        k = (int)(r >> 1);
         return true;
     }
     r -= ns;
     ns = 2 * ni * nj;  // k = 0 or n, i and j in 1..n-1
     if (r < ns) {
-        c[0] = 1 + (int)(r % ni);
+        i = 1 + (int)(r % ni);
         r /= ni;
-        c[2] = (r & 1) ? g.n[2] : 0;
-        c[1] = 1 + (int)(r >> 1);
+        k = (r & 1) ? g.n[2] : 0;
+        j = 1 + (int)(r >> 1);
         return true;
     }
     return false;
@@ -1835,9 +1856,9 @@ __host__ __device__ inline bool q2f_adj_node(const Geo& g, long r, int (&c)[3])
 __global__ __launch_bounds__(256) void k_q2f_adj(AdjArgs a)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int c[3];
-    if (!q2f_adj_node(a.g, t, c)) return;
-    const long gi = gidx(a.g, c[0], c[1], c[2]);
+    int ci, cj, ck;
+    if (!q2f_adj_node(a.g, t, ci, cj, ck)) return;
+    const long gi = gidx(a.g, ci, cj, ck);
     // every operand before the first store (a store could alias a later load)
     double q[5], pbv[6], qbv[5];
 #pragma unroll
@@ -1938,49 +1959,55 @@ __host__ __device__ inline long gflux_adj_halo_count(const Geo& g, int d)
     return 2L * HM * (d == 0 ? e1 * e2 : d == 1 ? e0 * e2 : e0 * e1);
 }
 
-__global__ void k_gflux_adj_halo(AdjArgs a)
+// the halo node r of axis D's strips (a compile-time axis keeps every
+// index and array in registers)
+template <int D>
+__device__ __forceinline__ void gflux_adj_halo_node(const AdjArgs& a, long r)
 {
-    const long ext[3] = {a.g.n[0] + 1, a.g.n[1] + 1, a.g.n[2] + 1};
-    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int d = 0;
-    while (d < 3 && r >= gflux_adj_halo_count(a.g, d)) r -= gflux_adj_halo_count(a.g, d++);
-    if (d == 3) return;
-    const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
-    int c[3], h;
-    if (d == 0) {
+    constexpr int OA = D == 0 ? 1 : 0, OB = D == 2 ? 1 : 2;
+    const long ea = a.g.n[OA] + 1L, eb = a.g.n[OB] + 1L;
+    int h, ca, cb;
+    if (D == 0) {
         // x strips: the 2 HM halo nodes of a row are the contiguous ones
         h = (int)(r % (2 * HM));
         r /= 2 * HM;
-        c[oa] = (int)(r % ext[oa]);
-        c[ob] = (int)(r / ext[oa]);
+        ca = (int)(r % ea);
+        cb = (int)(r / ea);
     } else {
         // y/z strips: consecutive i
-        c[oa] = (int)(r % ext[oa]);
-        r /= ext[oa];
-        c[ob] = (int)(r % ext[ob]);
-        h = (int)(r / ext[ob]);
+        ca = (int)(r % ea);
+        r /= ea;
+        cb = (int)(r % eb);
+        h = (int)(r / eb);
     }
-    c[d] = h < HM ? -(h + 1) : a.g.n[d] + (h - HM + 1);
-    const long gi = gidx(a.g, c[0], c[1], c[2]);
+    const int cd = h < HM ? -(h + 1) : a.g.n[D] + (h - HM + 1);
+    const int ci = D == 0 ? cd : ca, cj = D == 1 ? cd : (D == 0 ? ca : cb), ck = D == 2 ? cd : cb;
+    const long gi = gidx(a.g, ci, cj, ck);
     double gbv[5], qv[5], qbv[5], pbv[6];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        gbv[m] = a.gb[5 * d + m][gi];
+        gbv[m] = a.gb[5 * D + m][gi];
         qv[m] = a.q[m][gi];
         qbv[m] = a.qb[m][gi];
     }
 #pragma unroll
     for (int k = 0; k < 6; ++k) pbv[k] = a.pb[k][gi];
-    // a compile-time axis keeps the arrays in registers
-    const double xn = a.xi[d][c[d] + HM], jc = a.jac[gi], pf = a.prim[4][gi];
-    const double ud = a.prim[1 + d][gi];
-    if (d == 0) inviscid_adj(0, xn, jc, qv, pf, ud, gbv, qbv, pbv);
-    else if (d == 1) inviscid_adj(1, xn, jc, qv, pf, ud, gbv, qbv, pbv);
-    else inviscid_adj(2, xn, jc, qv, pf, ud, gbv, qbv, pbv);
+    const double xn = a.xi[D][cd + HM], jc = a.jac[gi], pf = a.prim[4][gi];
+    const double ud = a.prim[1 + D][gi];
+    inviscid_adj(D, xn, jc, qv, pf, ud, gbv, qbv, pbv);
 #pragma unroll
     for (int m = 0; m < 5; ++m) a.qb[m][gi] = qbv[m];
-    a.pb[1 + d][gi] = pbv[1 + d];
+    a.pb[1 + D][gi] = pbv[1 + D];
     a.pb[4][gi] = pbv[4];
+}
+
+__global__ void k_gflux_adj_halo(AdjArgs a)
+{
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n0 = gflux_adj_halo_count(a.g, 0), n1 = gflux_adj_halo_count(a.g, 1);
+    if (r < n0) gflux_adj_halo_node<0>(a, r);
+    else if ((r -= n0) < n1) gflux_adj_halo_node<1>(a, r);
+    else if ((r -= n1) < gflux_adj_halo_count(a.g, 2)) gflux_adj_halo_node<2>(a, r);
 }
 
 // G on the interior reversed at one node: the inviscid parts of its three
