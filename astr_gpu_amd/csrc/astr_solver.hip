@@ -1761,104 +1761,28 @@ __device__ __forceinline__ void q2fvar_adj(const AdjArgs& a, const double* q, co
     qbv[4] += q4b;
 }
 
-// The nodes whose primitives qswap refreshes: the halo cells with one
-// axis outside 0..n (the 2*HM slabs of each axis over the other two axes'
-// 0..n), then the seam nodes (every coordinate in 0..n, some 0 or n), each
-// seam node listed once, under its first such axis.  r-th node in c;
-// consecutive r walk along i where the slab or seam allows.
-__host__ __device__ inline long q2f_adj_count(const Geo& g)
-{
-    const long e0 = g.n[0] + 1L, e1 = g.n[1] + 1L, e2 = g.n[2] + 1L;
-    return 2L * HM * (e1 * e2 + e0 * e2 + e0 * e1) + 2 * e1 * e2 + 2 * (e0 - 2) * e2 +
-           2 * (e0 - 2) * (e1 - 2);
-}
-
-// the r-th cell of the 2*HM halo slabs of axis D (over the other two axes'
-// 0..n) as (i, j, k); x slabs walk the 2*HM cells of a row first, y and z
-// slabs walk i first
-template <int D>
-__host__ __device__ inline void q2f_adj_slab(const Geo& g, long r, int& i, int& j, int& k)
-{
-    constexpr int OA = D == 0 ? 1 : 0, OB = D == 2 ? 1 : 2;
-    const long ea = g.n[OA] + 1L;
-    int h, ca, cb;
-    if (D == 0) {
-        h = (int)(r % (2 * HM));
-        r /= 2 * HM;
-        ca = (int)(r % ea);
-        cb = (int)(r / ea);
-    } else {
-        ca = (int)(r % ea);
-        r /= ea;
-        h = (int)(r % (2 * HM));
-        cb = (int)(r / (2 * HM));
-    }
-    const int cd = h < HM ? -(h + 1) : g.n[D] + (h - HM + 1);
-    i = D == 0 ? cd : ca;
-    j = D == 1 ? cd : (D == 0 ? ca : cb);
-    k = D == 2 ? cd : cb;
-}
-
-__host__ __device__ inline bool q2f_adj_node(const Geo& g, long r, int& i, int& j, int& k)
-{
-    const long e0 = g.n[0] + 1L, e1 = g.n[1] + 1L, e2 = g.n[2] + 1L;
-    long ns = 2L * HM * e1 * e2;
-    if (r < ns) {
-        q2f_adj_slab<0>(g, r, i, j, k);
-        return true;
-    }
-    r -= ns;
-    ns = 2L * HM * e0 * e2;
-    if (r < ns) {
-        q2f_adj_slab<1>(g, r, i, j, k);
-        return true;
-    }
-    r -= ns;
-    ns = 2L * HM * e0 * e1;
-    if (r < ns) {
-        q2f_adj_slab<2>(g, r, i, j, k);
-        return true;
-    }
-    r -= ns;
-    const long ni = e0 - 2, nj = e1 - 2;
-    ns = 2 * e1 * e2;  // i = 0 or n, j and k in 0..n
-    if (r < ns) {
-        i = (r & 1) ? g.n[0] : 0;
-        r >>= 1;
-        j = (int)(r % e1);
-        k = (int)(r / e1);
-        return true;
-    }
-    r -= ns;
-    ns = 2 * ni * e2;  // j = 0 or n, i in 1..n-1, k in 0..n
-    if (r < ns) {
-        i = 1 + (int)(r % ni);
-        r /= ni;
-        j = (r & 1) ? g.n[1] : 0;
-        k = (int)(r >> 1);
-        return true;
-    }
-    r -= ns;
-    ns = 2 * ni * nj;  // k = 0 or n, i and j in 1..n-1
-    if (r < ns) {
-        i = 1 + (int)(r % ni);
-        r /= ni;
-        k = (r & 1) ? g.n[2] : 0;
-        j = 1 + (int)(r >> 1);
-        return true;
-    }
-    return false;
-}
-
-// qswap's refresh of the primitives reversed at those nodes, at q after
-// qswap: q-bar collects the primitive-bar there, which is zeroed (the
-// forward overwrote those primitives).  One thread per node.
+// qswap's refresh of the primitives reversed at the halo cells (one axis
+// outside 0..n) and the seam nodes (interior, some coordinate 0 or n), at
+// q after qswap: q-bar collects the primitive-bar there, which is zeroed
+// (the forward overwrote those primitives).  One thread per node of the
+// padded box (x over the (i,j) plane, y over k).
 __global__ __launch_bounds__(256) void k_q2f_adj(AdjArgs a)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int ci, cj, ck;
-    if (!q2f_adj_node(a.g, t, ci, cj, ck)) return;
-    const long gi = gidx(a.g, ci, cj, ck);
+    const unsigned pw = (unsigned)(a.g.n[0] + 1 + 2 * HM);
+    const unsigned ph = (unsigned)(a.g.n[1] + 1 + 2 * HM);
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= pw * ph) return;
+    const unsigned jj = t / pw;
+    const int c[3] = {(int)(t - jj * pw) - HM, (int)jj - HM, (int)blockIdx.y - HM};
+    int nout = 0;
+    bool seam = false;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        if (c[e] < 0 || c[e] > a.g.n[e]) ++nout;
+        else if (c[e] == 0 || c[e] == a.g.n[e]) seam = true;
+    }
+    if (nout > 1 || (nout == 0 && !seam)) return;
+    const long gi = gidx(a.g, c[0], c[1], c[2]);
     // every operand before the first store (a store could alias a later load)
     double q[5], pbv[6], qbv[5];
 #pragma unroll
@@ -4349,6 +4273,12 @@ AdjArgs adj_args(astr_ctx* c, double* const* q, double* const* prim, double* con
     return a;
 }
 
+dim3 padded_grid(const astr_ctx* c)
+{
+    const long plane = (long)(c->g.n[0] + 1 + 2 * HM) * (c->g.n[1] + 1 + 2 * HM);
+    return dim3((unsigned)((plane + 255) / 256), (unsigned)(c->g.n[2] + 1 + 2 * HM));
+}
+
 // a row_node kernel over the interior rows
 template <class K>
 int launch_rows(astr_ctx* c, K kernel, const AdjArgs& a)
@@ -4457,7 +4387,8 @@ int grad_adj(astr_ctx* c)
 // halo and seam nodes, then the copies and averages in the order z, y, x
 int qswap_adj(astr_ctx* c, AdjArgs a)
 {
-    TRY(launch_adj(c, k_q2f_adj, q2f_adj_count(c->g), a));
+    hipLaunchKernelGGL(k_q2f_adj, padded_grid(c), dim3(256), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
     const long ext[3] = {c->g.n[0] + 1L, c->g.n[1] + 1L, c->g.n[2] + 1L};
     for (int d = 2; d >= 0; --d) {
         a.axis = d;
