@@ -675,8 +675,10 @@ __device__ __forceinline__ double bt_interior(const double* ln, int j, const C10
 // staging map and XCD-grouped block ids): stage g, solve A^T per
 // half-wavefront, form B^T w in place, write the adjoint of the line's
 // input.
+// 5 waves per SIMD where the registers allow it without spilling (L <= 6:
+// the 128^3 adjoint's reversal 221 -> 212 ms), else 4
 template <int L, int OP, bool XD>
-__global__ __launch_bounds__(WGT, 4) void k_sweep_t(TSweepArgs a)
+__global__ __launch_bounds__(WGT, L <= 6 ? 5 : 4) void k_sweep_t(TSweepArgs a)
 {
     extern __shared__ double tile[];
     const int dir = XD ? 0 : a.dir;
