@@ -56,6 +56,36 @@ def test_batch_graph_replays_equal_box_by_box_steps(convection, mode):
         s.close()
 
 
+@pytest.mark.parametrize("mode", [0, 2])
+def test_batch_members_stepped_alone_in_between(mode):
+    """Whole steps of one box alternate its q and qsave buffers (stage 1
+    writes its result into the other one); a batch holding the box rebuilds
+    its launches after them.  Plain steps on a member between batched steps
+    (an odd number, so the buffers have swapped) must leave every member
+    bit-equal to boxes stepped alone throughout."""
+    dims = [(32, 32, 32), (40, 36, 32)]
+    print(f"case batch interleaved mode={mode} dims={dims}", flush=True)
+    eager = [make_box(d, 20 + i) for i, d in enumerate(dims)]
+    mixed = [make_box(d, 20 + i) for i, d in enumerate(dims)]
+    b = ag.AstrBatch(mixed)
+    b.set_mode(mode)
+    dt = 0.004
+    for rnd in range(2):
+        b.rk3_steps(1, dt)
+        mixed[0].steploop(1, dt)
+        mixed[1].steploop(3, dt)
+        b.rk3_steps(1, dt)
+        eager[0].steploop(3, dt)
+        eager[1].steploop(5, dt)
+        for k, (se, sm) in enumerate(zip(eager, mixed)):
+            a, g = se.download_state(), sm.download_state()
+            for nm in a:
+                assert np.array_equal(a[nm], g[nm]), (rnd, k, nm)
+    b.close()
+    for s in eager + mixed:
+        s.close()
+
+
 def test_batch_rejects_mixed_or_unready_members():
     s1 = make_box((32, 32, 32), 1)
     p = ag.default_params((32, 32, 32))
