@@ -136,20 +136,22 @@ def test_bounded_axes_central_steps_match_checker(npdc):
     s.close()
 
 
-def test_gcnscbc_central_steps_match_checker():
+@pytest.mark.parametrize("v", [0.2, -1.2])
+def test_gcnscbc_central_steps_match_checker(v):
     """gcnscbc (bctype 23) on the x-high and y-high faces of a three-dimensional
     box (bounded x and y, periodic z) through whole RK3 steps of the central
     path against the checker: statistics at 1e-10, every node and the
     periodic axis' halo faces at 1e-10 after 3 steps.  Ma 0.4 keeps the
     integer sound speed gcnscbc takes (sqrt(T)/Ma near 2.5) away from a
-    truncation boundary."""
+    truncation boundary; v = -1.2 takes face 4's inflow branch (integer
+    normal velocity -1)."""
     from test_oracle_bounded import GCN_FACES, pulse
 
     dims, mach = (32, 28, 24), 0.4
-    print(f"case gcnscbc central dims={dims}", flush=True)
+    print(f"case gcnscbc central dims={dims} v={v}", flush=True)
     ob = pyoracle.OracleBox(dims, variant=0, npdc=(4, 4, 3), mach=mach)
     ob.set_bc(GCN_FACES)
-    pulse(ob, mach)
+    pulse(ob, mach, v=v)
     st = ob.state_haloed()
     p = ag.default_params(dims, npdci=4, npdcj=4, npdck=3, mach=mach)
     s = ag.AstrSolver(p)
