@@ -2957,6 +2957,56 @@ __global__ void k_bc_gcnscbc(BcArgs a)
     a.q[4][gi] = pn * a.const6 + rn * var1;
 }
 
+// zeroextrap (src/bc.F90:2331-2627), bctype 50, on the face of axis AX at its
+// low (HI false) or high end, over 0..n of the other two axes: velocity and
+// pressure of the boundary node by extrapolate_2o with dv = 0 from its two
+// inner neighbours (src/commfunc.F90:277-285: v0 = (4 v1 - v2 - 2 dv)/3),
+// and on the x faces the temperature too, the density from thermal(p, T) =
+// p/T*const2; on the y and z faces the density, the temperature from
+// thermal(p, rho) = p/rho*const2; q from fvar2q.  In the reference's
+// operation order, like the rest of this region.
+template <int AX, bool HI>
+__global__ void k_bc_zeroextrap(BcArgs a)
+{
+    constexpr int OA = AX == 0 ? 1 : 0, OB = AX == 2 ? 1 : 2;
+    const long ea = a.g.n[OA] + 1L;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ea * (a.g.n[OB] + 1L)) return;
+    const int ca = (int)(t % ea), cb = (int)(t / ea);
+    const int n = a.g.n[AX];
+    const int c0 = HI ? n : 0, c1 = HI ? n - 1 : 1, c2 = HI ? n - 2 : 2;
+    auto at = [&](int cd) {
+        const int i = AX == 0 ? cd : ca;
+        const int j = AX == 1 ? cd : (AX == 0 ? ca : cb);
+        const int k = AX == 2 ? cd : cb;
+        return gidx(a.g, i, j, k);
+    };
+    const long gi = at(c0), g1 = at(c1), g2 = at(c2);
+    const double n1d3 = 1.0 / 3.0;
+    double ve[3];
+    for (int m = 0; m < 3; ++m)
+        ve[m] = n1d3 * (4.0 * a.prim[1 + m][g1] - a.prim[1 + m][g2] - 2.0 * 0.0);
+    const double pe = n1d3 * (4.0 * a.prim[4][g1] - a.prim[4][g2] - 2.0 * 0.0);
+    double re, te;
+    if (AX == 0) {
+        te = n1d3 * (4.0 * a.prim[5][g1] - a.prim[5][g2] - 2.0 * 0.0);
+        re = pe / te * a.const2;
+    } else {
+        re = n1d3 * (4.0 * a.prim[0][g1] - a.prim[0][g2] - 2.0 * 0.0);
+        te = pe / re * a.const2;
+    }
+    a.prim[0][gi] = re;
+    for (int m = 0; m < 3; ++m) a.prim[1 + m][gi] = ve[m];
+    a.prim[4][gi] = pe;
+    a.prim[5][gi] = te;
+    a.q[0][gi] = re;
+    a.q[1][gi] = re * ve[0];
+    a.q[2][gi] = re * ve[1];
+    a.q[3][gi] = re * ve[2];
+    const double var1 = 0.5 * (ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
+    a.q[4][gi] = pe * a.const6 + re * var1;
+}
+
 // back to the default contraction of the rest of the library
 #pragma clang fp contract(fast)
 
@@ -3653,10 +3703,29 @@ int do_qswap(astr_ctx* c)
 
 // boucon (src/bc.F90:327-411) for the faces under the user condition of
 // the 2-D Riemann example, at the context's time
+// zeroextrap on face f (0..5), one thread per node of the face
+static int launch_zeroextrap(astr_ctx* c, const BcArgs& a, int f)
+{
+    const int ax = f / 2, oa = ax == 0 ? 1 : 0, ob = ax == 2 ? 1 : 2;
+    const long n = (c->g.n[oa] + 1L) * (c->g.n[ob] + 1L);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    switch (f) {
+    case 0: hipLaunchKernelGGL((k_bc_zeroextrap<0, false>), grid, dim3(256), 0, c->stream, a); break;
+    case 1: hipLaunchKernelGGL((k_bc_zeroextrap<0, true>), grid, dim3(256), 0, c->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_bc_zeroextrap<1, false>), grid, dim3(256), 0, c->stream, a); break;
+    case 3: hipLaunchKernelGGL((k_bc_zeroextrap<1, true>), grid, dim3(256), 0, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((k_bc_zeroextrap<2, false>), grid, dim3(256), 0, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_bc_zeroextrap<2, true>), grid, dim3(256), 0, c->stream, a); break;
+    }
+    HIPCHK(hipGetLastError());
+    return ASTR_OK;
+}
+
 int do_boucon(astr_ctx* c)
 {
-    const bool gcn2 = c->bctype[1] == 23, gcn4 = c->bctype[3] == 23;
-    if (c->udf != ASTR_UDF_RIEMANN2D && !gcn2 && !gcn4) return ASTR_OK;
+    bool any = c->udf == ASTR_UDF_RIEMANN2D;
+    for (int f = 0; f < 6; ++f) any = any || c->bctype[f] == 50 || c->bctype[f] == 23;
+    if (!any) return ASTR_OK;
     BcArgs a{};
     a.g = c->g;
     for (int m = 0; m < 5; ++m) a.q[m] = c->q[m];
@@ -3669,30 +3738,33 @@ int do_boucon(astr_ctx* c)
     a.const6 = c->const6;
     a.len[0] = c->len[0];
     a.len[1] = c->len[1];
-    // boucon's face order (src/bc.F90:338-404): faces 1 and 2 (the user
-    // condition of both in one kernel, or gcnscbc on face 2 after face 1),
-    // then faces 3 and 4 likewise
+    // boucon's face order (src/bc.F90:338-404), face by face 1..6: the user
+    // condition of faces 1 and 2 (and of 3 and 4) in one kernel (it is never
+    // combined with zeroextrap), gcnscbc on faces 2 and 4, zeroextrap on any
     const long nk = c->g.n[2] + 1;
     const long ni = (long)(c->g.n[0] + 1) * nk, nj = (long)(c->g.n[1] + 1) * nk;
-    if (a.face[0] || a.face[1]) {
-        hipLaunchKernelGGL(k_bc_riem2d_i, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0,
-                           c->stream, a);
-        HIPCHK(hipGetLastError());
-    }
-    if (gcn2) {
-        hipLaunchKernelGGL(k_bc_gcnscbc<0>, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0,
-                           c->stream, a);
-        HIPCHK(hipGetLastError());
-    }
-    if (a.face[2] || a.face[3]) {
-        hipLaunchKernelGGL(k_bc_riem2d_j, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0,
-                           c->stream, a);
-        HIPCHK(hipGetLastError());
-    }
-    if (gcn4) {
-        hipLaunchKernelGGL(k_bc_gcnscbc<1>, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0,
-                           c->stream, a);
-        HIPCHK(hipGetLastError());
+    for (int f = 0; f < 6; ++f) {
+        if (f == 0 && (a.face[0] || a.face[1])) {
+            hipLaunchKernelGGL(k_bc_riem2d_i, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0,
+                               c->stream, a);
+            HIPCHK(hipGetLastError());
+        }
+        if (f == 2 && (a.face[2] || a.face[3])) {
+            hipLaunchKernelGGL(k_bc_riem2d_j, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0,
+                               c->stream, a);
+            HIPCHK(hipGetLastError());
+        }
+        if (c->bctype[f] == 50) TRY(launch_zeroextrap(c, a, f));
+        if (c->bctype[f] == 23 && f == 1) {
+            hipLaunchKernelGGL(k_bc_gcnscbc<0>, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0,
+                               c->stream, a);
+            HIPCHK(hipGetLastError());
+        }
+        if (c->bctype[f] == 23 && f == 3) {
+            hipLaunchKernelGGL(k_bc_gcnscbc<1>, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0,
+                               c->stream, a);
+            HIPCHK(hipGetLastError());
+        }
     }
     return ASTR_OK;
 }
@@ -4084,8 +4156,8 @@ int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed, bool alt = f
     // interface fluxes), 6 update (central: primitives; upwind: RK3 combine
     // and primitives)
     const bool upwind = c->p.convection != 0;
-    const bool bc_reads_prim =
-        c->udf == ASTR_UDF_RIEMANN2D || c->bctype[1] == 23 || c->bctype[3] == 23;
+    bool bc_reads_prim = c->udf == ASTR_UDF_RIEMANN2D;
+    for (int f = 0; f < 6; ++f) bc_reads_prim = bc_reads_prim || c->bctype[f] == 23 || c->bctype[f] == 50;
     if (c->p.variant == 1) {
         // rk3 of tgvsolver.F90:994-1082: bchomo, rhscal (gradcal, convection,
         // diffusion by df/dx), stage-1 statistics and qsave, the RK3 combine
@@ -5221,8 +5293,14 @@ bool all_periodic(const astr_ctx* c)
 
 // stage pieces and steps: three-dimensional boxes, bounded axes on the
 // central path of the main variant
-// gcnscbc on face 2 or 4 (the faces the reference's gcnscbc acts on)
-static bool has_gcnscbc(const astr_ctx* c) { return c->bctype[1] == 23 || c->bctype[3] == 23; }
+// a boundary condition of bc.F90 on some face: gcnscbc (it acts on faces 2
+// and 4 only) or zeroextrap
+static bool has_face_bc(const astr_ctx* c)
+{
+    for (int f = 0; f < 6; ++f)
+        if ((c->bctype[f] == 23 && (f == 1 || f == 3)) || c->bctype[f] == 50) return true;
+    return false;
+}
 
 static int ready(astr_ctx* c)
 {
@@ -5259,7 +5337,7 @@ static int ready_batch(astr_ctx* c)
     if (c->g.n[2] == 0) return fail(ASTR_E_UNSUPPORTED, "batched members are three-dimensional boxes");
     if (!all_periodic(c)) return fail(ASTR_E_UNSUPPORTED, "batched members are periodic boxes");
     if (c->curv) return fail(ASTR_E_UNSUPPORTED, "batched members have rectilinear metrics");
-    if (c->udf || has_gcnscbc(c))
+    if (c->udf || has_face_bc(c))
         return fail(ASTR_E_UNSUPPORTED, "batched members take no boundary condition");
     return ASTR_OK;
 }
@@ -5800,14 +5878,20 @@ int astr_set_spacing(astr_ctx* c, double dx, double dy, double dz)
 int astr_set_bc(astr_ctx* c, const int32_t* bctype, int32_t udf)
 {
     if (!c || !bctype) return fail(ASTR_E_ARG, "null argument");
-    bool any = false;
+    bool any = false, ext = false;
     for (int f = 0; f < 6; ++f) {
-        if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23)
+        if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50)
             return fail(ASTR_E_UNSUPPORTED,
-                        "bctype 1 (nothing done), 0 (user condition) and 23 (gcnscbc) are built; "
-                        "the inflow, outflow, wall and other NSCBC conditions of src/bc.F90 are not");
+                        "bctype 1 (nothing done), 0 (user condition), 23 (gcnscbc) and 50 "
+                        "(zeroextrap) are built; the inflow, outflow, wall and other NSCBC "
+                        "conditions of src/bc.F90 are not");
+        if (bctype[f] == 50 && f >= 4 && c->g.n[2] == 0)
+            return fail(ASTR_E_UNSUPPORTED, "zeroextrap on the k faces needs a three-dimensional box");
         any = any || bctype[f] == 0;
+        ext = ext || bctype[f] == 50;
     }
+    if (any && ext)
+        return fail(ASTR_E_UNSUPPORTED, "zeroextrap is not combined with a user condition");
     if (udf != 0 && udf != ASTR_UDF_RIEMANN2D)
         return fail(ASTR_E_ARG, "udf must be 0 (none) or ASTR_UDF_RIEMANN2D");
     if (any && udf == 0) return fail(ASTR_E_ARG, "bctype 0 needs a user condition (udf)");
@@ -5985,7 +6069,7 @@ int astr_adjoint_record(astr_ctx* c, int32_t nsteps, double dt, int32_t ckpt, do
         c->curv)
         return fail(ASTR_E_UNSUPPORTED, "the adjoint is built for the central path on periodic 3-D "
                                         "boxes with rectilinear metrics");
-    if (c->udf || has_gcnscbc(c))
+    if (c->udf || has_face_bc(c))
         return fail(ASTR_E_UNSUPPORTED, "the adjoint takes no boundary condition");
     c->rec_nsteps = -1;
     const long nseg = nsteps > 0 ? (nsteps + ckpt - 1) / ckpt : 0;

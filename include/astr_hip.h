@@ -341,9 +341,13 @@ int astr_set_spacing(astr_ctx *ctx, double dx, double dy, double dz);
  * three-dimensional boxes on the central path (a bounded x or y axis);
  * batched members and the adjoint take no boundary condition
  * (ASTR_E_UNSUPPORTED there).  It reads node coordinates of gridcube's
- * grid (astr_set_lengths).  The other bctypes of bc.F90 (inflow, outflow,
- * walls, the other NSCBCs) return ASTR_E_UNSUPPORTED.  Default: all
- * faces 1. */
+ * grid (astr_set_lengths).  50 is zeroextrap (src/bc.F90:2331-2627): the
+ * boundary node from its two inner neighbours by extrapolate_2o with
+ * dv = 0, on any face (the k faces of three-dimensional boxes only), not
+ * combined with a user condition; batched members and the adjoint take
+ * none.  Faces act in the reference's order 1..6.  The other bctypes of
+ * bc.F90 (inflow, outflow, walls, the other NSCBCs) return
+ * ASTR_E_UNSUPPORTED.  Default: all faces 1. */
 #define ASTR_UDF_RIEMANN2D 1
 int astr_set_bc(astr_ctx *ctx, const int32_t *bctype, int32_t udf);
 /* The box lengths of gridcube (src/gridgeneration.F90:233-267: x(i) =

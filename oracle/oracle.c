@@ -564,18 +564,21 @@ int orc_box_set_bc(orc_box *b, const int *bctype)
 {
     if (b->variant != 0) return -1;
     for (int f = 0; f < 6; ++f)
-        if (bctype[f] != 0 && bctype[f] != 23) return -1;
+        if (bctype[f] != 0 && bctype[f] != 23 && bctype[f] != 50) return -1;
     for (int f = 0; f < 6; ++f) b->bc[f] = bctype[f];
     return 0;
 }
 
-/* boucon (src/bc.F90:327-411) with gcnscbc (src/bc.F90:7907-8140) on faces
- * 2 and 4, in that order: the restatement of upwind.c (orc_gcnscbc) run on
- * this box's fields (the same haloed layout), its lengths the spans of the
- * gridcube coordinates x(i) = lx/im*i, y(j) = ly/jm*j */
+/* boucon (src/bc.F90:327-411) face by face in the order 1..6: gcnscbc
+ * (src/bc.F90:7907-8140, faces 2 and 4) and zeroextrap (src/bc.F90:
+ * 2331-2627), the restatements of upwind.c run on this box's fields (the
+ * same haloed layout); gcnscbc's coordinates are the gridcube's x(i) =
+ * lx/im*i, y(j) = ly/jm*j */
 static void box_boucon(orc_box *b)
 {
-    if (b->bc[1] != 23 && b->bc[3] != 23) return;
+    int any = 0;
+    for (int f = 0; f < 6; ++f) any |= b->bc[f] != 0;
+    if (!any) return;
     orc_uwd u;
     memset(&u, 0, sizeof u);
     u.im = b->im; u.jm = b->jm; u.km = b->km;
@@ -585,8 +588,10 @@ static void box_boucon(orc_box *b)
     double *q = malloc(5 * n * sizeof(double)), *vel = malloc(3 * n * sizeof(double));
     for (int m = 0; m < 5; ++m) memcpy(q + m * n, b->q[m], n * sizeof(double));
     for (int c = 0; c < 3; ++c) memcpy(vel + c * n, b->vel[c], n * sizeof(double));
-    if (b->bc[1] == 23) orc_gcnscbc(&u, 2, q, b->rho, vel, b->prs, b->tmp);
-    if (b->bc[3] == 23) orc_gcnscbc(&u, 4, q, b->rho, vel, b->prs, b->tmp);
+    for (int f = 0; f < 6; ++f) {
+        if (b->bc[f] == 23) orc_gcnscbc(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
+        else if (b->bc[f] == 50) orc_zeroextrap(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
+    }
     for (int m = 0; m < 5; ++m) memcpy(b->q[m], q + m * n, n * sizeof(double));
     for (int c = 0; c < 3; ++c) memcpy(b->vel[c], vel + c * n, n * sizeof(double));
     free(q);

@@ -196,12 +196,12 @@ class OracleBox:
             pass
 
     def set_bc(self, bctype):
-        """Boundary condition of faces 1..6 for the steps: 0 (nothing done)
-        or 23 (gcnscbc, acting on faces 2 and 4 before each stage's qswap);
-        the main variant only."""
+        """Boundary condition of faces 1..6 for the steps, applied in face
+        order before each stage's qswap: 0 (nothing done), 23 (gcnscbc,
+        acting on faces 2 and 4) or 50 (zeroextrap); the main variant only."""
         arr = (ctypes.c_int * 6)(*[int(v) for v in bctype])
         if self.lib.orc_box_set_bc(self.h, arr) != 0:
-            raise ValueError(f"bctype {tuple(bctype)}: 0 or 23 per face, main variant")
+            raise ValueError(f"bctype {tuple(bctype)}: 0, 23 or 50 per face, main variant")
 
     @property
     def shape(self):
@@ -339,6 +339,8 @@ def uwd_lib():
         lib.orc_uwd_qswap.argtypes = [P, _D, _D, _D, _D, _D]
         lib.orc_gcnscbc.restype = None
         lib.orc_gcnscbc.argtypes = [P, _I, _D, _D, _D, _D, _D]
+        lib.orc_zeroextrap.restype = None
+        lib.orc_zeroextrap.argtypes = [P, _I, _D, _D, _D, _D, _D]
         lib.orc_uwd_rk3_step.restype = _I
         lib.orc_uwd_rk3_step.argtypes = [P, _I, _I, I6, _I, _F, _F, _D, _D, _D, _D, _D, _D, _D]
         _uwd_declared = True
@@ -491,6 +493,14 @@ class UwdBox:
         o = self._state(st)
         lib.orc_gcnscbc(ctypes.byref(self.cfg), int(face), _p(o["q"]), _p(o["rho"]), _p(o["vel"]),
                         _p(o["prs"]), _p(o["tmp"]))
+        return o
+
+    def zeroextrap(self, st, face):
+        """A copy of st after zeroextrap (bctype 50) on face 1..6."""
+        lib = uwd_lib()
+        o = self._state(st)
+        lib.orc_zeroextrap(ctypes.byref(self.cfg), int(face), _p(o["q"]), _p(o["rho"]),
+                           _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
         return o
 
     def qswap(self, st):

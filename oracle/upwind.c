@@ -1369,6 +1369,61 @@ void orc_gcnscbc(const orc_uwd *u, int face, double *q, double *rho, double *vel
         }
 }
 
+/* zeroextrap (src/bc.F90:2331-2627), bctype 50, on face 1..6 (i = 0, i = im,
+ * j = 0, j = jm, k = 0, k = km) over 0..n of the other two axes: the
+ * boundary node's velocity and pressure from the two inner nodes by
+ * extrapolate_2o with dv = 0 (src/commfunc.F90:277-285: v0 = (4 v1 - v2 -
+ * 2 dv)/3, the value at the face of the parabola with zero slope there), and
+ * on faces 1-2 the temperature too, the density from thermal(p, T) =
+ * p/T*const2; on faces 3-6 the density, the temperature from thermal(p, rho)
+ * = p/rho*const2; q = fvar2q.  The reference's qrhs = 0 on the face is
+ * followed by rhscal's own sums, so it is not restated. */
+void orc_zeroextrap(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+                    double *tmp)
+{
+    if (face < 1 || face > 6) return;
+    const shape_t s = shape_of(u);
+    const double gamma = u->gamma, mach = u->mach;
+    const double const2 = gamma * (mach * mach), const6 = 1.0 / (gamma - 1.0);
+    const double n1d3 = 1.0 / 3.0;
+    const int ax = (face - 1) / 2, hi = (face - 1) % 2;
+    const int n[3] = {u->im, u->jm, u->km};
+    const int oa = ax == 0 ? 1 : 0, ob = ax == 2 ? 1 : 2;
+    for (int b = 0; b <= n[ob]; ++b)
+        for (int a = 0; a <= n[oa]; ++a) {
+            int c[3], c1[3], c2[3];
+            c[oa] = c1[oa] = c2[oa] = a;
+            c[ob] = c1[ob] = c2[ob] = b;
+            c[ax] = hi ? n[ax] : 0;
+            c1[ax] = hi ? n[ax] - 1 : 1;
+            c2[ax] = hi ? n[ax] - 2 : 2;
+            const long h = hidx(&s, c[0], c[1], c[2]);
+            const long h1 = hidx(&s, c1[0], c1[1], c1[2]), h2 = hidx(&s, c2[0], c2[1], c2[2]);
+            double ve[3];
+            for (int m = 0; m < 3; ++m)
+                ve[m] = n1d3 * (4.0 * vel[m * s.nf + h1] - vel[m * s.nf + h2] - 2.0 * 0.0);
+            const double pe = n1d3 * (4.0 * prs[h1] - prs[h2] - 2.0 * 0.0);
+            double re, te;
+            if (ax == 0) {
+                te = n1d3 * (4.0 * tmp[h1] - tmp[h2] - 2.0 * 0.0);
+                re = pe / te * const2;
+            } else {
+                re = n1d3 * (4.0 * rho[h1] - rho[h2] - 2.0 * 0.0);
+                te = pe / re * const2;
+            }
+            prs[h] = pe;
+            tmp[h] = te;
+            rho[h] = re;
+            for (int m = 0; m < 3; ++m) vel[m * s.nf + h] = ve[m];
+            q[h] = re;
+            q[s.nf + h] = re * ve[0];
+            q[2 * s.nf + h] = re * ve[1];
+            q[3 * s.nf + h] = re * ve[2];
+            const double var1 = 0.5 * (ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
+            q[4 * s.nf + h] = pe * const6 + re * var1;
+        }
+}
+
 void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, double *q,
                 double *rho, double *vel, double *prs, double *tmp)
 {
@@ -1379,6 +1434,8 @@ void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, doubl
             orc_riem2d_bc(u, one, time, q, rho, vel, prs, tmp);
         } else if (bctype[f] == 23) {
             orc_gcnscbc(u, f + 1, q, rho, vel, prs, tmp);
+        } else if (bctype[f] == 50) {
+            orc_zeroextrap(u, f + 1, q, rho, vel, prs, tmp);
         }
     }
 }
@@ -1456,7 +1513,7 @@ int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *
                              {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
     int anyudf = 0, anybc = 0;
     for (int f = 0; f < 6; ++f) {
-        if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23) return 3;
+        if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50) return 3;
         anyudf |= bctype[f] == 0;
         anybc |= bctype[f] != 1;
     }

@@ -169,6 +169,36 @@ def test_gcnscbc_central_steps_match_checker(v):
     s.close()
 
 
+def test_zeroextrap_central_steps_match_checker():
+    """zeroextrap (bctype 50) on the four faces of the bounded x and y axes of
+    a three-dimensional box (periodic z) through whole RK3 steps of the
+    central path against the checker, applied face by face before each
+    stage's qswap: statistics at 1e-10, every node and the periodic axis'
+    halo faces at 1e-10 after 3 steps."""
+    from test_oracle_bounded import pulse
+
+    dims, mach = (32, 28, 24), 0.4
+    faces = (50, 50, 50, 50, 0, 0)
+    print(f"case zeroextrap central dims={dims}", flush=True)
+    ob = pyoracle.OracleBox(dims, variant=0, npdc=(4, 4, 3), mach=mach)
+    ob.set_bc(faces)
+    pulse(ob, mach)
+    st = ob.state_haloed()
+    p = ag.default_params(dims, npdci=4, npdcj=4, npdck=3, mach=mach)
+    s = ag.AstrSolver(p)
+    s.upload_geom(*ag.box_metrics(p))
+    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+    s.set_bc(tuple(f if f else 1 for f in faces), 0)
+    dt = 0.01
+    for n in range(3):
+        so = ob.step(dt, stats=True)
+        sg = s.time_integration_rk(dt, stats=True)
+        for a, b in zip(sg, so):
+            assert abs(a - b) <= 1e-10 * abs(b), (n, sg, so)
+    print(f"  3 steps {compare_state(ob, s, p, 1e-10, halo_faces=True, vector_scale=True):.2e}")
+    s.close()
+
+
 def test_gcnscbc_rejected_by_batch_and_adjoint():
     dims = (16, 16, 16)
     p = ag.default_params(dims)

@@ -233,4 +233,48 @@ def _boucon_ref(box, st, faces):
     for f, b in enumerate(faces):
         if b == 23:
             o = box.gcnscbc(o, f + 1)
+        elif b == 50:
+            o = box.zeroextrap(o, f + 1)
     return o
+
+
+@pytest.mark.parametrize("faces", [(50, 50, 50, 50, 1, 1), (50, 23, 50, 23, 1, 1)])
+def test_zeroextrap_steps_match_checker(faces):
+    """zeroextrap (bctype 50) on the four faces of a 2-D box, alone and with
+    gcnscbc on faces 2 and 4 (the reference's face order 1..6), through
+    boucon and whole upwind steps against the checker."""
+    dims = (64, 60, 0)
+    mach = 0.4
+    print(f"case zeroextrap dims={dims} faces={faces}", flush=True)
+    p = ag.default_params(dims, npdci=4, npdcj=4, npdck=3, convection=1, deriv_explicit=1,
+                          recon_schem=3, lchardecomp=1, lfilter=0, diffterm=0, shkcrt=0.01,
+                          mach=mach)
+    st = pulse_state(dims, mach)
+    s = ag.AstrSolver(p)
+    jac, dxi = ag.box_metrics(p, lengths=(1.0, 1.0, 1.0))
+    s.upload_geom(jac, dxi)
+    s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+    s.set_lengths(1.0, 1.0, 0.0)
+    s.set_bc(faces, 0)
+    box = po.UwdBox(dims, npdc=(4, 4, 3), recon_schem=3, lchardecomp=True, bfacmpld=0.7,
+                    shkcrt=0.01, gamma=p.gamma, mach=mach)
+    dxi9 = dxi.reshape(dxi.shape[:3] + (9,), order="F")
+    s.boucon()
+    compare_plane(s.download_state(), _boucon_ref(box, st, faces), "boucon")
+    dt = 0.2 / 64 * mach
+    s.steploop(5, dt)
+    g = s.download_state()
+    ref = box.rk3_steps(st, dxi9, jac, dt, nsteps=5, bctype=faces, udf=0)
+    compare_plane(g, ref, "5 steps")
+    s.close()
+
+
+def test_zeroextrap_rejected_where_not_built():
+    p = ag.default_params((32, 32, 0), npdci=4, npdcj=4, npdck=3, convection=1,
+                          deriv_explicit=1, lfilter=0, diffterm=0)
+    s = ag.AstrSolver(p)
+    with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
+        s.set_bc((50, 0, 1, 1, 1, 1), ag.UDF_RIEMANN2D)  # with a user condition
+    with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
+        s.set_bc((1, 1, 1, 1, 50, 1), 0)  # a k face of a 2-D box
+    s.close()

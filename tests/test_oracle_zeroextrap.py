@@ -1,0 +1,87 @@
+"""The checker's zeroextrap (bctype 50, src/bc.F90:2331-2627): the boundary
+node of a face from its two inner neighbours by extrapolate_2o with dv = 0
+(src/commfunc.F90:277-285), v0 = (4 v1 - v2)/3.  That is the value at the
+face of the parabola with zero slope there, so a field c + d r^2 (r the
+node distance from the face) comes back as c on the face, which a linear
+extrapolation (c + 2d/3 for it), a wrong weight or the wrong neighbour
+fails.  Faces 1-2 extrapolate the temperature and take the density from
+the equation of state, faces 3-6 the other way round."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+HM = 5
+GAMMA, MACH = 1.4, 0.4
+
+
+def parabolic_state(dims, face, c, d):
+    """Primitives c[k] + d[k] r^2 in the distance r (in nodes) from the given
+    face, constant along the face; every other node keeps that law too."""
+    shape = tuple(n + 1 + 2 * HM for n in dims)
+    ax, hi = (face - 1) // 2, (face - 1) % 2
+    idx = np.arange(shape[ax]) - HM
+    r = (dims[ax] - idx) if hi else idx
+    sh = [1, 1, 1]
+    sh[ax] = shape[ax]
+    r = r.reshape(sh).astype(np.float64)
+    prim = [np.broadcast_to(c[k] + d[k] * r * r, shape).copy() for k in range(5)]
+    return po.prim_to_state(*prim, gamma=GAMMA, mach=MACH), r
+
+
+@pytest.mark.parametrize("face", [1, 2, 3, 4])
+def test_zeroextrap_returns_the_zero_slope_parabola(face):
+    dims = (12, 10, 0)
+    box = po.UwdBox(dims, npdc=(4, 4, 3), gamma=GAMMA, mach=MACH)
+    c = (1.1, 0.3, -0.2, 0.05, 1.0 / (GAMMA * MACH * MACH))
+    d = (0.01, 0.02, -0.015, 0.005, 0.03)
+    st, _ = parabolic_state(dims, face, c, d)
+    o = box.zeroextrap(st, face)
+    ax, hi = (face - 1) // 2, (face - 1) % 2
+    sl = [slice(HM, HM + n + 1) for n in dims]
+    sl[ax] = HM + (dims[ax] if hi else 0)
+    sl = tuple(sl)
+    const2 = GAMMA * MACH * MACH
+    u = [o["vel"][sl + (m,)] for m in range(3)]
+    p, rho, t = o["prs"][sl], o["rho"][sl], o["tmp"][sl]
+    for m in range(3):
+        assert np.max(np.abs(u[m] - c[1 + m])) <= 1e-14, (face, m)
+    assert np.max(np.abs(p - c[4])) <= 1e-14 * c[4]
+    if face <= 2:
+        # the temperature extrapolated (its own law is not a parabola in r,
+        # so only the equation of state is checked), the density from it
+        assert np.max(np.abs(rho - p / t * const2)) <= 1e-14
+    else:
+        assert np.max(np.abs(rho - c[0])) <= 1e-14
+        assert np.max(np.abs(t - p / rho * const2)) <= 1e-14
+    q = o["q"][sl]
+    assert np.array_equal(q[..., 0], rho)
+    for m in range(3):
+        assert np.max(np.abs(q[..., 1 + m] - rho * u[m])) <= 1e-15
+    e = p / (GAMMA - 1.0) + 0.5 * rho * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2)
+    assert np.max(np.abs(q[..., 4] - e)) <= 1e-13 * np.max(np.abs(e))
+    # every other node untouched
+    mask = np.ones(st["prs"].shape, dtype=bool)
+    mask[sl] = False
+    for nm in ("prs", "rho", "tmp"):
+        assert np.array_equal(o[nm][mask], st[nm][mask]), nm
+
+
+def test_zeroextrap_is_not_a_linear_extrapolation():
+    """A linear law c + b r is extrapolated to c + 2b/3, not c."""
+    dims = (12, 10, 0)
+    box = po.UwdBox(dims, npdc=(4, 4, 3), gamma=GAMMA, mach=MACH)
+    shape = tuple(n + 1 + 2 * HM for n in dims)
+    r = (np.arange(shape[0]) - HM).reshape(-1, 1, 1).astype(np.float64) * np.ones(shape)
+    st = po.prim_to_state(np.full(shape, 1.0), 0.2 + 0.03 * r, np.zeros(shape), np.zeros(shape),
+                          np.full(shape, 1.0 / (GAMMA * MACH * MACH)), gamma=GAMMA, mach=MACH)
+    o = box.zeroextrap(st, 1)
+    u0 = o["vel"][HM, HM:HM + dims[1] + 1, HM, 0]
+    assert np.max(np.abs(u0 - (0.2 + 0.02))) <= 1e-15
+
+
+def test_central_box_accepts_zeroextrap():
+    ob = po.OracleBox((16, 12, 10), variant=0, npdc=(4, 4, 3))
+    ob.set_bc((50, 50, 50, 50, 0, 0))
+    with pytest.raises(ValueError):
+        ob.set_bc((51, 0, 0, 0, 0, 0))
