@@ -2742,7 +2742,7 @@ struct BcArgs {
     double* prim[6];
     int face[4];    // faces 1..4 (i=0, i=im, j=0, j=jm) under the user condition
     double time;    // the step's start time
-    double gamma, mach, const2, const6;
+    double gamma, mach, const1, const2, const6;
     double len[2];  // gridcube lengths of x and y (x(i) = len/n*i), for gcnscbc
 };
 
@@ -2957,18 +2957,29 @@ __global__ void k_bc_gcnscbc(BcArgs a)
     a.q[4][gi] = pn * a.const6 + rn * var1;
 }
 
-// zeroextrap (src/bc.F90:2331-2627), bctype 50, on the face of axis AX at its
-// low (HI false) or high end, over 0..n of the other two axes: velocity and
+// zeroextrap (src/bc.F90:2331-2627, bctype 50; SYM false) and symmetry
+// (src/bc.F90:2639-2971, bctype 60; SYM true) on the face of axis AX at its
+// low (HI false) or high end, over 0..n of the other two axes.  Velocity and
 // pressure of the boundary node by extrapolate_2o with dv = 0 from its two
-// inner neighbours (src/commfunc.F90:277-285: v0 = (4 v1 - v2 - 2 dv)/3),
-// and on the x faces the temperature too, the density from thermal(p, T) =
-// p/T*const2; on the y and z faces the density, the temperature from
-// thermal(p, rho) = p/rho*const2; q from fvar2q.  In the reference's
-// operation order, like the rest of this region.
-template <int AX, bool HI>
-__global__ void k_bc_zeroextrap(BcArgs a)
+// inner neighbours (src/commfunc.F90:277-285: v0 = (4 v1 - v2 - 2 dv)/3).
+//   zeroextrap: on the x faces the temperature too and the density from
+//   thermal(p, T) = p/T*const2, on the y and z faces the density and the
+//   temperature from thermal(p, rho) = p/rho*const2; q by fvar2q's pressure
+//   form.
+//   symmetry (rectilinear grids): the velocity normal to the face zero
+//   (faces 1, 2 and 6 project the extrapolated velocity through the boundary
+//   normal dxi(d,:)/|dxi(d,:)| = +-e_d, which leaves exactly that; faces
+//   3-5 set it); the temperature extrapolated and the density from p/T,
+//   except on face 6 (the density extrapolated, T from p/rho); q by fvar2q's
+//   temperature form rho*(T*const1 + |u|^2/2) on faces 1, 2, 4 and 6, its
+//   pressure form on faces 3 and 5 (src/fludyna.F90:312-376).
+// In the reference's operation order, like the rest of this region.
+template <int AX, bool HI, bool SYM>
+__global__ void k_bc_extrap(BcArgs a)
 {
     constexpr int OA = AX == 0 ? 1 : 0, OB = AX == 2 ? 1 : 2;
+    constexpr bool RHO_EXT = SYM ? (AX == 2 && HI) : AX != 0;
+    constexpr bool TFORM = SYM && (AX == 0 || HI);
     const long ea = a.g.n[OA] + 1L;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ea * (a.g.n[OB] + 1L)) return;
@@ -2986,14 +2997,15 @@ __global__ void k_bc_zeroextrap(BcArgs a)
     double ve[3];
     for (int m = 0; m < 3; ++m)
         ve[m] = n1d3 * (4.0 * a.prim[1 + m][g1] - a.prim[1 + m][g2] - 2.0 * 0.0);
+    if (SYM) ve[AX] = 0.0;
     const double pe = n1d3 * (4.0 * a.prim[4][g1] - a.prim[4][g2] - 2.0 * 0.0);
     double re, te;
-    if (AX == 0) {
-        te = n1d3 * (4.0 * a.prim[5][g1] - a.prim[5][g2] - 2.0 * 0.0);
-        re = pe / te * a.const2;
-    } else {
+    if (RHO_EXT) {
         re = n1d3 * (4.0 * a.prim[0][g1] - a.prim[0][g2] - 2.0 * 0.0);
         te = pe / re * a.const2;
+    } else {
+        te = n1d3 * (4.0 * a.prim[5][g1] - a.prim[5][g2] - 2.0 * 0.0);
+        re = pe / te * a.const2;
     }
     a.prim[0][gi] = re;
     for (int m = 0; m < 3; ++m) a.prim[1 + m][gi] = ve[m];
@@ -3004,7 +3016,7 @@ __global__ void k_bc_zeroextrap(BcArgs a)
     a.q[2][gi] = re * ve[1];
     a.q[3][gi] = re * ve[2];
     const double var1 = 0.5 * (ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
-    a.q[4][gi] = pe * a.const6 + re * var1;
+    a.q[4][gi] = TFORM ? re * (te * a.const1 + var1) : pe * a.const6 + re * var1;
 }
 
 // back to the default contraction of the rest of the library
@@ -3703,19 +3715,21 @@ int do_qswap(astr_ctx* c)
 
 // boucon (src/bc.F90:327-411) for the faces under the user condition of
 // the 2-D Riemann example, at the context's time
-// zeroextrap on face f (0..5), one thread per node of the face
-static int launch_zeroextrap(astr_ctx* c, const BcArgs& a, int f)
+// zeroextrap (sym false) or symmetry (sym true) on face f (0..5), one
+// thread per node of the face
+template <bool SYM>
+static int launch_extrap(astr_ctx* c, const BcArgs& a, int f)
 {
     const int ax = f / 2, oa = ax == 0 ? 1 : 0, ob = ax == 2 ? 1 : 2;
     const long n = (c->g.n[oa] + 1L) * (c->g.n[ob] + 1L);
     const dim3 grid((unsigned)((n + 255) / 256));
     switch (f) {
-    case 0: hipLaunchKernelGGL((k_bc_zeroextrap<0, false>), grid, dim3(256), 0, c->stream, a); break;
-    case 1: hipLaunchKernelGGL((k_bc_zeroextrap<0, true>), grid, dim3(256), 0, c->stream, a); break;
-    case 2: hipLaunchKernelGGL((k_bc_zeroextrap<1, false>), grid, dim3(256), 0, c->stream, a); break;
-    case 3: hipLaunchKernelGGL((k_bc_zeroextrap<1, true>), grid, dim3(256), 0, c->stream, a); break;
-    case 4: hipLaunchKernelGGL((k_bc_zeroextrap<2, false>), grid, dim3(256), 0, c->stream, a); break;
-    default: hipLaunchKernelGGL((k_bc_zeroextrap<2, true>), grid, dim3(256), 0, c->stream, a); break;
+    case 0: hipLaunchKernelGGL((k_bc_extrap<0, false, SYM>), grid, dim3(256), 0, c->stream, a); break;
+    case 1: hipLaunchKernelGGL((k_bc_extrap<0, true, SYM>), grid, dim3(256), 0, c->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_bc_extrap<1, false, SYM>), grid, dim3(256), 0, c->stream, a); break;
+    case 3: hipLaunchKernelGGL((k_bc_extrap<1, true, SYM>), grid, dim3(256), 0, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((k_bc_extrap<2, false, SYM>), grid, dim3(256), 0, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_bc_extrap<2, true, SYM>), grid, dim3(256), 0, c->stream, a); break;
     }
     HIPCHK(hipGetLastError());
     return ASTR_OK;
@@ -3724,7 +3738,8 @@ static int launch_zeroextrap(astr_ctx* c, const BcArgs& a, int f)
 int do_boucon(astr_ctx* c)
 {
     bool any = c->udf == ASTR_UDF_RIEMANN2D;
-    for (int f = 0; f < 6; ++f) any = any || c->bctype[f] == 50 || c->bctype[f] == 23;
+    for (int f = 0; f < 6; ++f)
+        any = any || c->bctype[f] == 50 || c->bctype[f] == 60 || c->bctype[f] == 23;
     if (!any) return ASTR_OK;
     BcArgs a{};
     a.g = c->g;
@@ -3734,13 +3749,15 @@ int do_boucon(astr_ctx* c)
     a.time = c->time;
     a.gamma = c->p.gamma;
     a.mach = c->p.mach;
+    a.const1 = c->const1;
     a.const2 = c->const2;
     a.const6 = c->const6;
     a.len[0] = c->len[0];
     a.len[1] = c->len[1];
     // boucon's face order (src/bc.F90:338-404), face by face 1..6: the user
     // condition of faces 1 and 2 (and of 3 and 4) in one kernel (it is never
-    // combined with zeroextrap), gcnscbc on faces 2 and 4, zeroextrap on any
+    // combined with zeroextrap or symmetry), gcnscbc on faces 2 and 4,
+    // zeroextrap and symmetry on any
     const long nk = c->g.n[2] + 1;
     const long ni = (long)(c->g.n[0] + 1) * nk, nj = (long)(c->g.n[1] + 1) * nk;
     for (int f = 0; f < 6; ++f) {
@@ -3754,7 +3771,8 @@ int do_boucon(astr_ctx* c)
                                c->stream, a);
             HIPCHK(hipGetLastError());
         }
-        if (c->bctype[f] == 50) TRY(launch_zeroextrap(c, a, f));
+        if (c->bctype[f] == 50) TRY(launch_extrap<false>(c, a, f));
+        if (c->bctype[f] == 60) TRY(launch_extrap<true>(c, a, f));
         if (c->bctype[f] == 23 && f == 1) {
             hipLaunchKernelGGL(k_bc_gcnscbc<0>, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0,
                                c->stream, a);
@@ -4157,7 +4175,9 @@ int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed, bool alt = f
     // and primitives)
     const bool upwind = c->p.convection != 0;
     bool bc_reads_prim = c->udf == ASTR_UDF_RIEMANN2D;
-    for (int f = 0; f < 6; ++f) bc_reads_prim = bc_reads_prim || c->bctype[f] == 23 || c->bctype[f] == 50;
+    for (int f = 0; f < 6; ++f)
+        bc_reads_prim = bc_reads_prim || c->bctype[f] == 23 || c->bctype[f] == 50 ||
+                        c->bctype[f] == 60;
     if (c->p.variant == 1) {
         // rk3 of tgvsolver.F90:994-1082: bchomo, rhscal (gradcal, convection,
         // diffusion by df/dx), stage-1 statistics and qsave, the RK3 combine
@@ -5294,17 +5314,23 @@ bool all_periodic(const astr_ctx* c)
 // stage pieces and steps: three-dimensional boxes, bounded axes on the
 // central path of the main variant
 // a boundary condition of bc.F90 on some face: gcnscbc (it acts on faces 2
-// and 4 only) or zeroextrap
+// and 4 only), zeroextrap or symmetry
 static bool has_face_bc(const astr_ctx* c)
 {
     for (int f = 0; f < 6; ++f)
-        if ((c->bctype[f] == 23 && (f == 1 || f == 3)) || c->bctype[f] == 50) return true;
+        if ((c->bctype[f] == 23 && (f == 1 || f == 3)) || c->bctype[f] == 50 ||
+            c->bctype[f] == 60)
+            return true;
     return false;
 }
 
 static int ready(astr_ctx* c)
 {
     TRY(ready_state(c));
+    if (c->curv)
+        for (int f = 0; f < 6; ++f)
+            if (c->bctype[f] == 60)
+                return fail(ASTR_E_UNSUPPORTED, "symmetry (bctype 60) needs rectilinear metrics");
     if (c->p.variant == 1 && !c->have_h)
         return fail(ASTR_E_STATE, "the mini-app variant needs astr_set_spacing first");
     if (c->g.n[2] == 0) {
@@ -5880,18 +5906,22 @@ int astr_set_bc(astr_ctx* c, const int32_t* bctype, int32_t udf)
     if (!c || !bctype) return fail(ASTR_E_ARG, "null argument");
     bool any = false, ext = false;
     for (int f = 0; f < 6; ++f) {
-        if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50)
+        if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50 &&
+            bctype[f] != 60)
             return fail(ASTR_E_UNSUPPORTED,
-                        "bctype 1 (nothing done), 0 (user condition), 23 (gcnscbc) and 50 "
-                        "(zeroextrap) are built; the inflow, outflow, wall and other NSCBC "
-                        "conditions of src/bc.F90 are not");
-        if (bctype[f] == 50 && f >= 4 && c->g.n[2] == 0)
-            return fail(ASTR_E_UNSUPPORTED, "zeroextrap on the k faces needs a three-dimensional box");
+                        "bctype 1 (nothing done), 0 (user condition), 23 (gcnscbc), 50 "
+                        "(zeroextrap) and 60 (symmetry) are built; the inflow, outflow, wall and "
+                        "other NSCBC conditions of src/bc.F90 are not");
+        const bool ex = bctype[f] == 50 || bctype[f] == 60;
+        if (ex && f >= 4 && c->g.n[2] == 0)
+            return fail(ASTR_E_UNSUPPORTED,
+                        "zeroextrap and symmetry on the k faces need a three-dimensional box");
         any = any || bctype[f] == 0;
-        ext = ext || bctype[f] == 50;
+        ext = ext || ex;
     }
     if (any && ext)
-        return fail(ASTR_E_UNSUPPORTED, "zeroextrap is not combined with a user condition");
+        return fail(ASTR_E_UNSUPPORTED,
+                    "zeroextrap and symmetry are not combined with a user condition");
     if (udf != 0 && udf != ASTR_UDF_RIEMANN2D)
         return fail(ASTR_E_ARG, "udf must be 0 (none) or ASTR_UDF_RIEMANN2D");
     if (any && udf == 0) return fail(ASTR_E_ARG, "bctype 0 needs a user condition (udf)");

@@ -193,8 +193,8 @@ class AstrSolver:
     def set_bc(self, bctype=(1, 1, 1, 1, 1, 1), udf: int = 0) -> None:
         """boucon's faces (i=0, i=im, j=0, j=jm, k=0, k=km): 1 nothing done,
         0 the user condition udf (UDF_RIEMANN2D: udf_bc of the 2-D Riemann
-        example), 23 gcnscbc (acting on faces 2 and 4), 50 zeroextrap (any
-        face; not with a user condition)."""
+        example), 23 gcnscbc (acting on faces 2 and 4), 50 zeroextrap and 60
+        symmetry (any face; not with a user condition)."""
         arr = (ctypes.c_int32 * 6)(*[int(b) for b in bctype])
         check(self.lib.astr_set_bc(self.ctx, arr, int(udf)), "astr_set_bc")
 

@@ -343,9 +343,10 @@ int astr_set_spacing(astr_ctx *ctx, double dx, double dy, double dz);
  * (ASTR_E_UNSUPPORTED there).  It reads node coordinates of gridcube's
  * grid (astr_set_lengths).  50 is zeroextrap (src/bc.F90:2331-2627): the
  * boundary node from its two inner neighbours by extrapolate_2o with
- * dv = 0, on any face (the k faces of three-dimensional boxes only), not
- * combined with a user condition; batched members and the adjoint take
- * none.  Faces act in the reference's order 1..6.  The other bctypes of
+ * dv = 0; 60 is symmetry (src/bc.F90:2639-2971): the same with the velocity
+ * normal to the face zero (rectilinear metrics only).  Both on any face
+ * (the k faces of three-dimensional boxes only), not combined with a user
+ * condition; batched members and the adjoint take none.  Faces act in the reference's order 1..6.  The other bctypes of
  * bc.F90 (inflow, outflow, walls, the other NSCBCs) return
  * ASTR_E_UNSUPPORTED.  Default: all faces 1. */
 #define ASTR_UDF_RIEMANN2D 1

@@ -564,16 +564,17 @@ int orc_box_set_bc(orc_box *b, const int *bctype)
 {
     if (b->variant != 0) return -1;
     for (int f = 0; f < 6; ++f)
-        if (bctype[f] != 0 && bctype[f] != 23 && bctype[f] != 50) return -1;
+        if (bctype[f] != 0 && bctype[f] != 23 && bctype[f] != 50 && bctype[f] != 60) return -1;
     for (int f = 0; f < 6; ++f) b->bc[f] = bctype[f];
     return 0;
 }
 
 /* boucon (src/bc.F90:327-411) face by face in the order 1..6: gcnscbc
- * (src/bc.F90:7907-8140, faces 2 and 4) and zeroextrap (src/bc.F90:
- * 2331-2627), the restatements of upwind.c run on this box's fields (the
- * same haloed layout); gcnscbc's coordinates are the gridcube's x(i) =
- * lx/im*i, y(j) = ly/jm*j */
+ * (src/bc.F90:7907-8140, faces 2 and 4), zeroextrap (src/bc.F90:2331-2627)
+ * and symmetry (src/bc.F90:2639-2971, the box's rectilinear metrics), the
+ * restatements of upwind.c run on this box's fields (the same haloed
+ * layout); gcnscbc's coordinates are the gridcube's x(i) = lx/im*i,
+ * y(j) = ly/jm*j */
 static void box_boucon(orc_box *b)
 {
     int any = 0;
@@ -591,6 +592,7 @@ static void box_boucon(orc_box *b)
     for (int f = 0; f < 6; ++f) {
         if (b->bc[f] == 23) orc_gcnscbc(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
         else if (b->bc[f] == 50) orc_zeroextrap(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
+        else if (b->bc[f] == 60) orc_symmetry(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
     }
     for (int m = 0; m < 5; ++m) memcpy(b->q[m], q + m * n, n * sizeof(double));
     for (int c = 0; c < 3; ++c) memcpy(b->vel[c], vel + c * n, n * sizeof(double));

@@ -81,8 +81,9 @@ void orc_box_set_metrics(orc_box *b, const double *jacob, const double *dxi);
 /* closure per axis (3 periodic, 1/2/4 bounded); main variant; 0 or -1 */
 int orc_box_set_ntype(orc_box *b, int n0, int n1, int n2);
 /* bctype of faces 1..6 for the main variant's steps: 0 (nothing done), 23
- * (gcnscbc, src/bc.F90:7907-8140, acting on faces 2 and 4) or 50
- * (zeroextrap, src/bc.F90:2331-2627); applied in face order; 0 or -1 */
+ * (gcnscbc, src/bc.F90:7907-8140, acting on faces 2 and 4), 50
+ * (zeroextrap, src/bc.F90:2331-2627) or 60 (symmetry, src/bc.F90:
+ * 2639-2971, rectilinear grids); applied in face order; 0 or -1 */
 int orc_box_set_bc(orc_box *b, const int *bctype);
 
 /* Set the Taylor-Green initial field: tgvini (src/initialisation.F90:1079-1149)
@@ -185,6 +186,10 @@ void orc_gcnscbc(const orc_uwd *u, int face, double *q, double *rho, double *vel
  * neighbours by extrapolate_2o with dv = 0 (src/bc.F90:2331-2627) */
 void orc_zeroextrap(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
                     double *tmp);
+/* symmetry (bctype 60) on face 1..6 of a rectilinear grid: zeroextrap's
+ * extrapolation with the normal velocity zero (src/bc.F90:2639-2971) */
+void orc_symmetry(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+                  double *tmp);
 /* boucon: faces 1..6 in order, udf_bc (bctype 0, udf 1) or gcnscbc (23) */
 void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, double *q,
                 double *rho, double *vel, double *prs, double *tmp);

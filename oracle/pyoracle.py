@@ -198,10 +198,11 @@ class OracleBox:
     def set_bc(self, bctype):
         """Boundary condition of faces 1..6 for the steps, applied in face
         order before each stage's qswap: 0 (nothing done), 23 (gcnscbc,
-        acting on faces 2 and 4) or 50 (zeroextrap); the main variant only."""
+        acting on faces 2 and 4), 50 (zeroextrap) or 60 (symmetry); the main
+        variant only."""
         arr = (ctypes.c_int * 6)(*[int(v) for v in bctype])
         if self.lib.orc_box_set_bc(self.h, arr) != 0:
-            raise ValueError(f"bctype {tuple(bctype)}: 0, 23 or 50 per face, main variant")
+            raise ValueError(f"bctype {tuple(bctype)}: 0, 23, 50 or 60 per face, main variant")
 
     @property
     def shape(self):
@@ -341,6 +342,8 @@ def uwd_lib():
         lib.orc_gcnscbc.argtypes = [P, _I, _D, _D, _D, _D, _D]
         lib.orc_zeroextrap.restype = None
         lib.orc_zeroextrap.argtypes = [P, _I, _D, _D, _D, _D, _D]
+        lib.orc_symmetry.restype = None
+        lib.orc_symmetry.argtypes = [P, _I, _D, _D, _D, _D, _D]
         lib.orc_uwd_rk3_step.restype = _I
         lib.orc_uwd_rk3_step.argtypes = [P, _I, _I, I6, _I, _F, _F, _D, _D, _D, _D, _D, _D, _D]
         _uwd_declared = True
@@ -501,6 +504,14 @@ class UwdBox:
         o = self._state(st)
         lib.orc_zeroextrap(ctypes.byref(self.cfg), int(face), _p(o["q"]), _p(o["rho"]),
                            _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
+        return o
+
+    def symmetry(self, st, face):
+        """A copy of st after symmetry (bctype 60) on face 1..6."""
+        lib = uwd_lib()
+        o = self._state(st)
+        lib.orc_symmetry(ctypes.byref(self.cfg), int(face), _p(o["q"]), _p(o["rho"]),
+                         _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
         return o
 
     def qswap(self, st):

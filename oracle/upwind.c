@@ -1424,6 +1424,67 @@ void orc_zeroextrap(const orc_uwd *u, int face, double *q, double *rho, double *
         }
 }
 
+/* symmetry (src/bc.F90:2639-2971), bctype 60, on face 1..6 of a rectilinear
+ * grid: the boundary node from its two inner neighbours by extrapolate_2o
+ * with dv = 0 like zeroextrap, its velocity normal to the face zero (faces
+ * 1, 2 and 6 project the extrapolated velocity on the face through the
+ * boundary normal bnorm = dxi(d,:)/|dxi(d,:)| of src/geom.F90:700-790,
+ * which is +-e_d on such a grid, so the normal component is exactly zero
+ * and the others are kept; faces 3, 4 and 5 set it to 0); the temperature
+ * extrapolated and the density from thermal(p, T) = p/T*const2, except on
+ * face 6, which extrapolates the density and takes T = p/rho*const2; q by
+ * fvar2q's temperature form q5 = rho*(T*const1 + |u|^2/2) on faces 1, 2, 4
+ * and 6 and its pressure form q5 = p*const6 + rho*|u|^2/2 on faces 3 and 5,
+ * as the reference calls it (src/fludyna.F90:312-376). */
+void orc_symmetry(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+                  double *tmp)
+{
+    if (face < 1 || face > 6) return;
+    const shape_t s = shape_of(u);
+    const double gamma = u->gamma, mach = u->mach;
+    const double const1 = 1.0 / (gamma * (gamma - 1.0) * (mach * mach));
+    const double const2 = gamma * (mach * mach), const6 = 1.0 / (gamma - 1.0);
+    const double n1d3 = 1.0 / 3.0;
+    const int ax = (face - 1) / 2, hi = (face - 1) % 2;
+    const int rho_ext = face == 6, tform = face != 3 && face != 5;
+    const int n[3] = {u->im, u->jm, u->km};
+    const int oa = ax == 0 ? 1 : 0, ob = ax == 2 ? 1 : 2;
+    for (int b = 0; b <= n[ob]; ++b)
+        for (int a = 0; a <= n[oa]; ++a) {
+            int c[3], c1[3], c2[3];
+            c[oa] = c1[oa] = c2[oa] = a;
+            c[ob] = c1[ob] = c2[ob] = b;
+            c[ax] = hi ? n[ax] : 0;
+            c1[ax] = hi ? n[ax] - 1 : 1;
+            c2[ax] = hi ? n[ax] - 2 : 2;
+            const long h = hidx(&s, c[0], c[1], c[2]);
+            const long h1 = hidx(&s, c1[0], c1[1], c1[2]), h2 = hidx(&s, c2[0], c2[1], c2[2]);
+            double ve[3];
+            for (int m = 0; m < 3; ++m)
+                ve[m] = n1d3 * (4.0 * vel[m * s.nf + h1] - vel[m * s.nf + h2] - 2.0 * 0.0);
+            ve[ax] = 0.0;
+            const double pe = n1d3 * (4.0 * prs[h1] - prs[h2] - 2.0 * 0.0);
+            double re, te;
+            if (rho_ext) {
+                re = n1d3 * (4.0 * rho[h1] - rho[h2] - 2.0 * 0.0);
+                te = pe / re * const2;
+            } else {
+                te = n1d3 * (4.0 * tmp[h1] - tmp[h2] - 2.0 * 0.0);
+                re = pe / te * const2;
+            }
+            prs[h] = pe;
+            tmp[h] = te;
+            rho[h] = re;
+            for (int m = 0; m < 3; ++m) vel[m * s.nf + h] = ve[m];
+            q[h] = re;
+            q[s.nf + h] = re * ve[0];
+            q[2 * s.nf + h] = re * ve[1];
+            q[3 * s.nf + h] = re * ve[2];
+            const double var1 = 0.5 * (ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
+            q[4 * s.nf + h] = tform ? re * (te * const1 + var1) : pe * const6 + re * var1;
+        }
+}
+
 void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, double *q,
                 double *rho, double *vel, double *prs, double *tmp)
 {
@@ -1436,6 +1497,8 @@ void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, doubl
             orc_gcnscbc(u, f + 1, q, rho, vel, prs, tmp);
         } else if (bctype[f] == 50) {
             orc_zeroextrap(u, f + 1, q, rho, vel, prs, tmp);
+        } else if (bctype[f] == 60) {
+            orc_symmetry(u, f + 1, q, rho, vel, prs, tmp);
         }
     }
 }
@@ -1513,7 +1576,9 @@ int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *
                              {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
     int anyudf = 0, anybc = 0;
     for (int f = 0; f < 6; ++f) {
-        if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50) return 3;
+        if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50 &&
+            bctype[f] != 60)
+            return 3;
         anyudf |= bctype[f] == 0;
         anybc |= bctype[f] != 1;
     }

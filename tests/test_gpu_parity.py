@@ -169,17 +169,18 @@ def test_gcnscbc_central_steps_match_checker(v):
     s.close()
 
 
-def test_zeroextrap_central_steps_match_checker():
-    """zeroextrap (bctype 50) on the four faces of the bounded x and y axes of
-    a three-dimensional box (periodic z) through whole RK3 steps of the
-    central path against the checker, applied face by face before each
-    stage's qswap: statistics at 1e-10, every node and the periodic axis'
-    halo faces at 1e-10 after 3 steps."""
+@pytest.mark.parametrize("faces", [(50, 50, 50, 50, 0, 0), (60, 60, 60, 60, 0, 0),
+                                   (60, 50, 50, 60, 0, 0)])
+def test_zeroextrap_central_steps_match_checker(faces):
+    """zeroextrap (bctype 50) and symmetry (60) on the four faces of the
+    bounded x and y axes of a three-dimensional box (periodic z) through
+    whole RK3 steps of the central path against the checker, applied face by
+    face before each stage's qswap: statistics at 1e-10, every node and the
+    periodic axis' halo faces at 1e-10 after 3 steps."""
     from test_oracle_bounded import pulse
 
     dims, mach = (32, 28, 24), 0.4
-    faces = (50, 50, 50, 50, 0, 0)
-    print(f"case zeroextrap central dims={dims}", flush=True)
+    print(f"case zeroextrap/symmetry central dims={dims} faces={faces}", flush=True)
     ob = pyoracle.OracleBox(dims, variant=0, npdc=(4, 4, 3), mach=mach)
     ob.set_bc(faces)
     pulse(ob, mach)

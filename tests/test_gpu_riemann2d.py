@@ -235,14 +235,17 @@ def _boucon_ref(box, st, faces):
             o = box.gcnscbc(o, f + 1)
         elif b == 50:
             o = box.zeroextrap(o, f + 1)
+        elif b == 60:
+            o = box.symmetry(o, f + 1)
     return o
 
 
-@pytest.mark.parametrize("faces", [(50, 50, 50, 50, 1, 1), (50, 23, 50, 23, 1, 1)])
+@pytest.mark.parametrize("faces", [(50, 50, 50, 50, 1, 1), (50, 23, 50, 23, 1, 1),
+                                   (60, 60, 60, 60, 1, 1), (60, 23, 50, 60, 1, 1)])
 def test_zeroextrap_steps_match_checker(faces):
-    """zeroextrap (bctype 50) on the four faces of a 2-D box, alone and with
-    gcnscbc on faces 2 and 4 (the reference's face order 1..6), through
-    boucon and whole upwind steps against the checker."""
+    """zeroextrap (bctype 50) and symmetry (60) on the faces of a 2-D box,
+    alone and with gcnscbc on faces 2 and 4 (the reference's face order
+    1..6), through boucon and whole upwind steps against the checker."""
     dims = (64, 60, 0)
     mach = 0.4
     print(f"case zeroextrap dims={dims} faces={faces}", flush=True)

@@ -1,11 +1,14 @@
-"""The checker's zeroextrap (bctype 50, src/bc.F90:2331-2627): the boundary
+"""The checker's zeroextrap (bctype 50, src/bc.F90:2331-2627) and symmetry
+(bctype 60, src/bc.F90:2639-2971).  zeroextrap: the boundary
 node of a face from its two inner neighbours by extrapolate_2o with dv = 0
 (src/commfunc.F90:277-285), v0 = (4 v1 - v2)/3.  That is the value at the
 face of the parabola with zero slope there, so a field c + d r^2 (r the
 node distance from the face) comes back as c on the face, which a linear
 extrapolation (c + 2d/3 for it), a wrong weight or the wrong neighbour
 fails.  Faces 1-2 extrapolate the temperature and take the density from
-the equation of state, faces 3-6 the other way round."""
+the equation of state, faces 3-6 the other way round.  symmetry: the same
+extrapolation with the velocity normal to the face zero and the others
+kept, on a rectilinear grid."""
 import numpy as np
 import pytest
 
@@ -85,3 +88,36 @@ def test_central_box_accepts_zeroextrap():
     ob.set_bc((50, 50, 50, 50, 0, 0))
     with pytest.raises(ValueError):
         ob.set_bc((51, 0, 0, 0, 0, 0))
+
+
+@pytest.mark.parametrize("face", [1, 2, 3, 4])
+def test_symmetry_zeroes_the_normal_velocity(face):
+    dims = (12, 10, 0)
+    box = po.UwdBox(dims, npdc=(4, 4, 3), gamma=GAMMA, mach=MACH)
+    c = (1.1, 0.3, -0.2, 0.05, 1.0 / (GAMMA * MACH * MACH))
+    d = (0.01, 0.02, -0.015, 0.005, 0.03)
+    st, _ = parabolic_state(dims, face, c, d)
+    o = box.symmetry(st, face)
+    ax, hi = (face - 1) // 2, (face - 1) % 2
+    sl = [slice(HM, HM + n + 1) for n in dims]
+    sl[ax] = HM + (dims[ax] if hi else 0)
+    sl = tuple(sl)
+    const2 = GAMMA * MACH * MACH
+    u = [o["vel"][sl + (m,)] for m in range(3)]
+    p, rho, t = o["prs"][sl], o["rho"][sl], o["tmp"][sl]
+    for m in range(3):
+        want = 0.0 if m == ax else c[1 + m]
+        assert np.max(np.abs(u[m] - want)) <= 1e-14, (face, m)
+    assert np.max(np.abs(p - c[4])) <= 1e-14 * c[4]
+    assert np.max(np.abs(rho - p / t * const2)) <= 1e-14
+    q = o["q"][sl]
+    assert np.array_equal(q[..., 0], rho)
+    assert np.all(q[..., 1 + ax] == 0.0)
+    # both of fvar2q's energy forms give the same value to round-off
+    e = p / (GAMMA - 1.0) + 0.5 * rho * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2)
+    assert np.max(np.abs(q[..., 4] - e)) <= 1e-13 * np.max(np.abs(e))
+
+
+def test_central_box_accepts_symmetry():
+    ob = po.OracleBox((16, 12, 10), variant=0, npdc=(4, 4, 3))
+    ob.set_bc((60, 60, 50, 60, 0, 0))
