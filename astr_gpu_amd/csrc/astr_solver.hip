@@ -2744,7 +2744,11 @@ struct BcArgs {
     double time;    // the step's start time
     double gamma, mach, const1, const2, const6;
     double len[2];  // gridcube lengths of x and y (x(i) = len/n*i), for gcnscbc
+    double tw[6];   // wall temperature of each face (noslip)
 };
+
+// the boundary conditions of k_bc_extrap
+enum { BCX_ZEROEXTRAP = 0, BCX_SYMMETRY = 1, BCX_NOSLIP = 2, BCX_ADIABATIC = 3 };
 
 // One boundary node as udf_bc sets it: the primitives, tmp = thermal(
 // pressure, density) = p/rho*const2 and q from fvar2q_sca's pressure form,
@@ -2957,8 +2961,9 @@ __global__ void k_bc_gcnscbc(BcArgs a)
     a.q[4][gi] = pn * a.const6 + rn * var1;
 }
 
-// zeroextrap (src/bc.F90:2331-2627, bctype 50; SYM false) and symmetry
-// (src/bc.F90:2639-2971, bctype 60; SYM true) on the face of axis AX at its
+// zeroextrap (src/bc.F90:2331-2627, bctype 50; MODE BCX_ZEROEXTRAP), symmetry
+// (src/bc.F90:2639-2971, bctype 60; BCX_SYMMETRY) and the walls (bctypes 41,
+// 42; BCX_NOSLIP, BCX_ADIABATIC) on the face of axis AX at its
 // low (HI false) or high end, over 0..n of the other two axes.  Velocity and
 // pressure of the boundary node by extrapolate_2o with dv = 0 from its two
 // inner neighbours (src/commfunc.F90:277-285: v0 = (4 v1 - v2 - 2 dv)/3).
@@ -2974,11 +2979,16 @@ __global__ void k_bc_gcnscbc(BcArgs a)
 //   temperature form rho*(T*const1 + |u|^2/2) on faces 1, 2, 4 and 6, its
 //   pressure form on faces 3 and 5 (src/fludyna.F90:312-376).
 // In the reference's operation order, like the rest of this region.
-template <int AX, bool HI, bool SYM>
+//   walls (MODE 2 noslip, src/bc.F90:6281-6595; MODE 3 noslip_adibatic,
+//   :6611-6919; faces 1-4): the node at rest, the pressure num1d3*(4 p1 -
+//   p2), the temperature the face's wall temperature (2) or extrapolated
+//   alike (3), the density from p/T; q by fvar2q's pressure form.
+template <int AX, bool HI, int MODE>
 __global__ void k_bc_extrap(BcArgs a)
 {
+    constexpr bool SYM = MODE == BCX_SYMMETRY, WALL = MODE >= BCX_NOSLIP;
     constexpr int OA = AX == 0 ? 1 : 0, OB = AX == 2 ? 1 : 2;
-    constexpr bool RHO_EXT = SYM ? (AX == 2 && HI) : AX != 0;
+    constexpr bool RHO_EXT = WALL ? false : (SYM ? (AX == 2 && HI) : AX != 0);
     constexpr bool TFORM = SYM && (AX == 0 || HI);
     const long ea = a.g.n[OA] + 1L;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2996,11 +3006,14 @@ __global__ void k_bc_extrap(BcArgs a)
     const double n1d3 = 1.0 / 3.0;
     double ve[3];
     for (int m = 0; m < 3; ++m)
-        ve[m] = n1d3 * (4.0 * a.prim[1 + m][g1] - a.prim[1 + m][g2] - 2.0 * 0.0);
+        ve[m] = WALL ? 0.0 : n1d3 * (4.0 * a.prim[1 + m][g1] - a.prim[1 + m][g2] - 2.0 * 0.0);
     if (SYM) ve[AX] = 0.0;
     const double pe = n1d3 * (4.0 * a.prim[4][g1] - a.prim[4][g2] - 2.0 * 0.0);
     double re, te;
-    if (RHO_EXT) {
+    if (MODE == BCX_NOSLIP) {
+        te = a.tw[2 * AX + (HI ? 1 : 0)];
+        re = pe / te * a.const2;
+    } else if (RHO_EXT) {
         re = n1d3 * (4.0 * a.prim[0][g1] - a.prim[0][g2] - 2.0 * 0.0);
         te = pe / re * a.const2;
     } else {
@@ -3172,6 +3185,7 @@ struct astr_ctx {
     double* fhf[10] = {};           // compact upwind: their interface fluxes
     double* cmp_ac[3][2] = {};      // compact upwind Thomas factors per axis, wind
     double len[3] = {1.0, 1.0, 0.0};  // gridcube lengths (astr_set_lengths)
+    double twall[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0};  // wall temperatures (astr_set_twall)
     bool curv = false;                // general metrics uploaded (dxf)
     bool juni = false;                // jacob uniform (j0): read from no array
     double j0 = 0.0;
@@ -3715,21 +3729,21 @@ int do_qswap(astr_ctx* c)
 
 // boucon (src/bc.F90:327-411) for the faces under the user condition of
 // the 2-D Riemann example, at the context's time
-// zeroextrap (sym false) or symmetry (sym true) on face f (0..5), one
-// thread per node of the face
-template <bool SYM>
+// k_bc_extrap's condition MODE on face f (0..5), one thread per node of the
+// face
+template <int MODE>
 static int launch_extrap(astr_ctx* c, const BcArgs& a, int f)
 {
     const int ax = f / 2, oa = ax == 0 ? 1 : 0, ob = ax == 2 ? 1 : 2;
     const long n = (c->g.n[oa] + 1L) * (c->g.n[ob] + 1L);
     const dim3 grid((unsigned)((n + 255) / 256));
     switch (f) {
-    case 0: hipLaunchKernelGGL((k_bc_extrap<0, false, SYM>), grid, dim3(256), 0, c->stream, a); break;
-    case 1: hipLaunchKernelGGL((k_bc_extrap<0, true, SYM>), grid, dim3(256), 0, c->stream, a); break;
-    case 2: hipLaunchKernelGGL((k_bc_extrap<1, false, SYM>), grid, dim3(256), 0, c->stream, a); break;
-    case 3: hipLaunchKernelGGL((k_bc_extrap<1, true, SYM>), grid, dim3(256), 0, c->stream, a); break;
-    case 4: hipLaunchKernelGGL((k_bc_extrap<2, false, SYM>), grid, dim3(256), 0, c->stream, a); break;
-    default: hipLaunchKernelGGL((k_bc_extrap<2, true, SYM>), grid, dim3(256), 0, c->stream, a); break;
+    case 0: hipLaunchKernelGGL((k_bc_extrap<0, false, MODE>), grid, dim3(256), 0, c->stream, a); break;
+    case 1: hipLaunchKernelGGL((k_bc_extrap<0, true, MODE>), grid, dim3(256), 0, c->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_bc_extrap<1, false, MODE>), grid, dim3(256), 0, c->stream, a); break;
+    case 3: hipLaunchKernelGGL((k_bc_extrap<1, true, MODE>), grid, dim3(256), 0, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((k_bc_extrap<2, false, MODE>), grid, dim3(256), 0, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_bc_extrap<2, true, MODE>), grid, dim3(256), 0, c->stream, a); break;
     }
     HIPCHK(hipGetLastError());
     return ASTR_OK;
@@ -3739,7 +3753,8 @@ int do_boucon(astr_ctx* c)
 {
     bool any = c->udf == ASTR_UDF_RIEMANN2D;
     for (int f = 0; f < 6; ++f)
-        any = any || c->bctype[f] == 50 || c->bctype[f] == 60 || c->bctype[f] == 23;
+        any = any || c->bctype[f] == 50 || c->bctype[f] == 60 || c->bctype[f] == 23 ||
+              c->bctype[f] == 41 || c->bctype[f] == 42;
     if (!any) return ASTR_OK;
     BcArgs a{};
     a.g = c->g;
@@ -3754,6 +3769,7 @@ int do_boucon(astr_ctx* c)
     a.const6 = c->const6;
     a.len[0] = c->len[0];
     a.len[1] = c->len[1];
+    for (int f = 0; f < 6; ++f) a.tw[f] = c->twall[f];
     // boucon's face order (src/bc.F90:338-404), face by face 1..6: the user
     // condition of faces 1 and 2 (and of 3 and 4) in one kernel (it is never
     // combined with zeroextrap or symmetry), gcnscbc on faces 2 and 4,
@@ -3771,8 +3787,10 @@ int do_boucon(astr_ctx* c)
                                c->stream, a);
             HIPCHK(hipGetLastError());
         }
-        if (c->bctype[f] == 50) TRY(launch_extrap<false>(c, a, f));
-        if (c->bctype[f] == 60) TRY(launch_extrap<true>(c, a, f));
+        if (c->bctype[f] == 50) TRY(launch_extrap<BCX_ZEROEXTRAP>(c, a, f));
+        if (c->bctype[f] == 60) TRY(launch_extrap<BCX_SYMMETRY>(c, a, f));
+        if (c->bctype[f] == 41) TRY(launch_extrap<BCX_NOSLIP>(c, a, f));
+        if (c->bctype[f] == 42) TRY(launch_extrap<BCX_ADIABATIC>(c, a, f));
         if (c->bctype[f] == 23 && f == 1) {
             hipLaunchKernelGGL(k_bc_gcnscbc<0>, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0,
                                c->stream, a);
@@ -4177,7 +4195,7 @@ int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed, bool alt = f
     bool bc_reads_prim = c->udf == ASTR_UDF_RIEMANN2D;
     for (int f = 0; f < 6; ++f)
         bc_reads_prim = bc_reads_prim || c->bctype[f] == 23 || c->bctype[f] == 50 ||
-                        c->bctype[f] == 60;
+                        c->bctype[f] == 60 || c->bctype[f] == 41 || c->bctype[f] == 42;
     if (c->p.variant == 1) {
         // rk3 of tgvsolver.F90:994-1082: bchomo, rhscal (gradcal, convection,
         // diffusion by df/dx), stage-1 statistics and qsave, the RK3 combine
@@ -5314,12 +5332,12 @@ bool all_periodic(const astr_ctx* c)
 // stage pieces and steps: three-dimensional boxes, bounded axes on the
 // central path of the main variant
 // a boundary condition of bc.F90 on some face: gcnscbc (it acts on faces 2
-// and 4 only), zeroextrap or symmetry
+// and 4 only), zeroextrap, symmetry or a noslip wall
 static bool has_face_bc(const astr_ctx* c)
 {
     for (int f = 0; f < 6; ++f)
         if ((c->bctype[f] == 23 && (f == 1 || f == 3)) || c->bctype[f] == 50 ||
-            c->bctype[f] == 60)
+            c->bctype[f] == 60 || c->bctype[f] == 41 || c->bctype[f] == 42)
             return true;
     return false;
 }
@@ -5906,13 +5924,17 @@ int astr_set_bc(astr_ctx* c, const int32_t* bctype, int32_t udf)
     if (!c || !bctype) return fail(ASTR_E_ARG, "null argument");
     bool any = false, ext = false;
     for (int f = 0; f < 6; ++f) {
+        const bool wall = bctype[f] == 41 || bctype[f] == 42;
         if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50 &&
-            bctype[f] != 60)
+            bctype[f] != 60 && !wall)
             return fail(ASTR_E_UNSUPPORTED,
                         "bctype 1 (nothing done), 0 (user condition), 23 (gcnscbc), 50 "
-                        "(zeroextrap) and 60 (symmetry) are built; the inflow, outflow, wall and "
-                        "other NSCBC conditions of src/bc.F90 are not");
-        const bool ex = bctype[f] == 50 || bctype[f] == 60;
+                        "(zeroextrap), 60 (symmetry), 41 and 42 (noslip walls) are built; the "
+                        "inflow, outflow, far-field, slip-wall and other NSCBC conditions of "
+                        "src/bc.F90 are not");
+        if (wall && f >= 4)
+            return fail(ASTR_E_UNSUPPORTED, "noslip walls act on faces 1-4 (src/bc.F90:6281-6919)");
+        const bool ex = bctype[f] == 50 || bctype[f] == 60 || wall;
         if (ex && f >= 4 && c->g.n[2] == 0)
             return fail(ASTR_E_UNSUPPORTED,
                         "zeroextrap and symmetry on the k faces need a three-dimensional box");
@@ -5921,13 +5943,23 @@ int astr_set_bc(astr_ctx* c, const int32_t* bctype, int32_t udf)
     }
     if (any && ext)
         return fail(ASTR_E_UNSUPPORTED,
-                    "zeroextrap and symmetry are not combined with a user condition");
+                    "zeroextrap, symmetry and walls are not combined with a user condition");
     if (udf != 0 && udf != ASTR_UDF_RIEMANN2D)
         return fail(ASTR_E_ARG, "udf must be 0 (none) or ASTR_UDF_RIEMANN2D");
     if (any && udf == 0) return fail(ASTR_E_ARG, "bctype 0 needs a user condition (udf)");
     if (c->p.variant == 1) return fail(ASTR_E_UNSUPPORTED, "the mini-app variant has its own halos");
     for (int f = 0; f < 6; ++f) c->bctype[f] = bctype[f];
     c->udf = any ? udf : 0;
+    ++c->gen;
+    return ASTR_OK;
+}
+
+int astr_set_twall(astr_ctx* c, const double* twall)
+{
+    if (!c || !twall) return fail(ASTR_E_ARG, "null argument");
+    for (int f = 0; f < 6; ++f)
+        if (!(twall[f] > 0.0)) return fail(ASTR_E_ARG, "wall temperatures must be positive");
+    for (int f = 0; f < 6; ++f) c->twall[f] = twall[f];
     ++c->gen;
     return ASTR_OK;
 }

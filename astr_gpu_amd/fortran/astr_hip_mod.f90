@@ -245,6 +245,13 @@ module astr_hip
       real(c_double), value :: lx, ly, lz
     end function astr_set_lengths
 
+    ! wall temperature of each face for bctype 41 (twall of src/bc.F90:29)
+    integer(c_int) function astr_set_twall(ctx, twall) bind(c, name='astr_set_twall')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: twall(6)
+    end function astr_set_twall
+
     integer(c_int) function astr_boucon(ctx) bind(c, name='astr_boucon')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -194,9 +194,16 @@ class AstrSolver:
         """boucon's faces (i=0, i=im, j=0, j=jm, k=0, k=km): 1 nothing done,
         0 the user condition udf (UDF_RIEMANN2D: udf_bc of the 2-D Riemann
         example), 23 gcnscbc (acting on faces 2 and 4), 50 zeroextrap and 60
-        symmetry (any face; not with a user condition)."""
+        symmetry (any face), 41 and 42 noslip walls (isothermal at set_twall's
+        temperature, adiabatic; faces 1-4); none of these with a user
+        condition."""
         arr = (ctypes.c_int32 * 6)(*[int(b) for b in bctype])
         check(self.lib.astr_set_bc(self.ctx, arr, int(udf)), "astr_set_bc")
+
+    def set_twall(self, twall) -> None:
+        """Wall temperature of each face for bctype 41 (noslip)."""
+        arr = (ctypes.c_double * 6)(*[float(t) for t in twall])
+        check(self.lib.astr_set_twall(self.ctx, arr), "astr_set_twall")
 
     def set_lengths(self, lx: float, ly: float, lz: float = 0.0) -> None:
         """gridcube's box lengths (x(i) = lx/im*i), for the boundary

@@ -345,12 +345,21 @@ int astr_set_spacing(astr_ctx *ctx, double dx, double dy, double dz);
  * boundary node from its two inner neighbours by extrapolate_2o with
  * dv = 0; 60 is symmetry (src/bc.F90:2639-2971): the same with the velocity
  * normal to the face zero (rectilinear metrics only).  Both on any face
- * (the k faces of three-dimensional boxes only), not combined with a user
- * condition; batched members and the adjoint take none.  Faces act in the reference's order 1..6.  The other bctypes of
+ * (the k faces of three-dimensional boxes only).  41 and 42 are the noslip
+ * walls (src/bc.F90:6281-6919): at rest, the pressure extrapolated, the
+ * temperature the face's wall temperature (41, astr_set_twall) or
+ * extrapolated (42), on faces 1-4 as the reference has them.  None of these
+ * is combined with a user condition; batched members and the adjoint take
+ * none.  Faces act in the reference's order 1..6.  The other bctypes of
  * bc.F90 (inflow, outflow, walls, the other NSCBCs) return
  * ASTR_E_UNSUPPORTED.  Default: all faces 1. */
 #define ASTR_UDF_RIEMANN2D 1
 int astr_set_bc(astr_ctx *ctx, const int32_t *bctype, int32_t udf);
+/* The wall temperature of each face for bctype 41 (twall of src/bc.F90:29,
+ * read with the face's bctype by readinput, src/readwrite.F90:565).
+ * Default 1 on every face.  No counterpart call in the reference: the deck
+ * holds it. */
+int astr_set_twall(astr_ctx *ctx, const double *twall6);
 /* The box lengths of gridcube (src/gridgeneration.F90:233-267: x(i) =
  * lx/im*i, y(j) = ly/jm*j, z(k) = lz/km*k), which replace the reference's
  * node coordinates x(:,:,:,1:3) where a boundary condition reads them

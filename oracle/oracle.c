@@ -467,7 +467,8 @@ struct orc_box {
     double const1, const2, const3, const4, const5, const6, const7;
     double tempconst, tempconst1, roinf, uinf, pinf, tinf;
     double lx, ly, lz, dx, dy, dz;
-    int bc[6]; /* bctype of faces 1..6: 0 nothing done here, 23 gcnscbc */
+    int bc[6]; /* bctype of faces 1..6: 0 nothing done here, else boucon's */
+    double twall[6];
 };
 
 #define IX(b, i, j, k) ((size_t)((i) + HM) + (size_t)(b)->ni * ((size_t)((j) + HM) + (size_t)(b)->nj * (size_t)((k) + HM)))
@@ -488,6 +489,7 @@ orc_box *orc_box_new(int im, int jm, int km, int variant, double reynolds,
                      double alfa, double lx, double ly, double lz)
 {
     orc_box *b = calloc(1, sizeof(orc_box));
+    for (int f = 0; f < 6; ++f) b->twall[f] = 1.0;
     b->im = im; b->jm = jm; b->km = km; b->variant = variant;
     b->ni = im + 1 + 2 * HM; b->nj = jm + 1 + 2 * HM; b->nk = km + 1 + 2 * HM;
     b->n3 = (size_t)b->ni * b->nj * b->nk;
@@ -564,17 +566,25 @@ int orc_box_set_bc(orc_box *b, const int *bctype)
 {
     if (b->variant != 0) return -1;
     for (int f = 0; f < 6; ++f)
-        if (bctype[f] != 0 && bctype[f] != 23 && bctype[f] != 50 && bctype[f] != 60) return -1;
+        if (bctype[f] != 0 && bctype[f] != 23 && bctype[f] != 50 && bctype[f] != 60 &&
+            !((bctype[f] == 41 || bctype[f] == 42) && f < 4))
+            return -1;
     for (int f = 0; f < 6; ++f) b->bc[f] = bctype[f];
     return 0;
 }
 
 /* boucon (src/bc.F90:327-411) face by face in the order 1..6: gcnscbc
  * (src/bc.F90:7907-8140, faces 2 and 4), zeroextrap (src/bc.F90:2331-2627)
- * and symmetry (src/bc.F90:2639-2971, the box's rectilinear metrics), the
- * restatements of upwind.c run on this box's fields (the same haloed
+ * symmetry (src/bc.F90:2639-2971, the box's rectilinear metrics) and the
+ * walls (src/bc.F90:6281-6919), the restatements of upwind.c run on this
+ * box's fields (the same haloed
  * layout); gcnscbc's coordinates are the gridcube's x(i) = lx/im*i,
  * y(j) = ly/jm*j */
+void orc_box_set_twall(orc_box *b, const double *twall)
+{
+    for (int f = 0; f < 6; ++f) b->twall[f] = twall[f];
+}
+
 static void box_boucon(orc_box *b)
 {
     int any = 0;
@@ -585,6 +595,7 @@ static void box_boucon(orc_box *b)
     u.im = b->im; u.jm = b->jm; u.km = b->km;
     u.gamma = b->gamma; u.mach = b->mach;
     u.lx = b->lx; u.ly = b->ly; u.lz = b->lz;
+    for (int f = 0; f < 6; ++f) u.twall[f] = b->twall[f];
     const size_t n = b->n3;
     double *q = malloc(5 * n * sizeof(double)), *vel = malloc(3 * n * sizeof(double));
     for (int m = 0; m < 5; ++m) memcpy(q + m * n, b->q[m], n * sizeof(double));
@@ -593,6 +604,8 @@ static void box_boucon(orc_box *b)
         if (b->bc[f] == 23) orc_gcnscbc(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
         else if (b->bc[f] == 50) orc_zeroextrap(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
         else if (b->bc[f] == 60) orc_symmetry(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
+        else if (b->bc[f] == 41 || b->bc[f] == 42)
+            orc_wall(&u, f + 1, b->bc[f] == 41, q, b->rho, vel, b->prs, b->tmp);
     }
     for (int m = 0; m < 5; ++m) memcpy(b->q[m], q + m * n, n * sizeof(double));
     for (int c = 0; c < 3; ++c) memcpy(b->vel[c], vel + c * n, n * sizeof(double));

@@ -82,9 +82,12 @@ void orc_box_set_metrics(orc_box *b, const double *jacob, const double *dxi);
 int orc_box_set_ntype(orc_box *b, int n0, int n1, int n2);
 /* bctype of faces 1..6 for the main variant's steps: 0 (nothing done), 23
  * (gcnscbc, src/bc.F90:7907-8140, acting on faces 2 and 4), 50
- * (zeroextrap, src/bc.F90:2331-2627) or 60 (symmetry, src/bc.F90:
- * 2639-2971, rectilinear grids); applied in face order; 0 or -1 */
+ * (zeroextrap, src/bc.F90:2331-2627), 60 (symmetry, src/bc.F90:
+ * 2639-2971, rectilinear grids), 41 or 42 (noslip, noslip_adibatic,
+ * src/bc.F90:6281-6919, faces 1-4); applied in face order; 0 or -1 */
 int orc_box_set_bc(orc_box *b, const int *bctype);
+/* wall temperature of each face for bctype 41 (twall of src/bc.F90:29) */
+void orc_box_set_twall(orc_box *b, const double *twall);
 
 /* Set the Taylor-Green initial field: tgvini (src/initialisation.F90:1079-1149)
  * followed by updateq (src/initialisation.F90:30-162 / src/fludyna.F90:254)
@@ -125,6 +128,7 @@ typedef struct {
     double shkcrt;       /* Ducros sensor threshold */
     double gamma, mach;
     double lx, ly, lz;   /* gridcube lengths (x(i) = lx/im*i): gcnscbc's coordinates */
+    double twall[6];     /* wall temperature of each face (bctype 41) */
 } orc_uwd;
 
 double orc_minmod2(double a, double b);
@@ -190,6 +194,10 @@ void orc_zeroextrap(const orc_uwd *u, int face, double *q, double *rho, double *
  * extrapolation with the normal velocity zero (src/bc.F90:2639-2971) */
 void orc_symmetry(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
                   double *tmp);
+/* noslip (bctype 41, wall temperature u->twall[face-1]) and noslip_adibatic
+ * (42) on face 1..4 (src/bc.F90:6281-6919) */
+void orc_wall(const orc_uwd *u, int face, int isothermal, double *q, double *rho, double *vel,
+              double *prs, double *tmp);
 /* boucon: faces 1..6 in order, udf_bc (bctype 0, udf 1) or gcnscbc (23) */
 void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, double *q,
                 double *rho, double *vel, double *prs, double *tmp);

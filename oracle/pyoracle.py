@@ -58,6 +58,8 @@ def oracle_lib():
         lib.orc_box_set_ntype.argtypes = [_V, _I, _I, _I]
         lib.orc_box_set_bc.restype = _I
         lib.orc_box_set_bc.argtypes = [_V, ctypes.POINTER(ctypes.c_int)]
+        lib.orc_box_set_twall.restype = None
+        lib.orc_box_set_twall.argtypes = [_V, ctypes.POINTER(ctypes.c_double)]
         lib.orc_df_compact.argtypes = [_I, _I, _D, _D]
         lib.orc_diff6ec.argtypes = [_I, _I, _D, _D]
         lib.orc_compact_filter.argtypes = [_I, _I, _F, _F, _F, _D, _D]
@@ -195,14 +197,19 @@ class OracleBox:
         except Exception:
             pass
 
+    def set_twall(self, twall):
+        """Wall temperature of each face for bctype 41 (default 1)."""
+        self.lib.orc_box_set_twall(self.h, (ctypes.c_double * 6)(*[float(t) for t in twall]))
+
     def set_bc(self, bctype):
         """Boundary condition of faces 1..6 for the steps, applied in face
         order before each stage's qswap: 0 (nothing done), 23 (gcnscbc,
-        acting on faces 2 and 4), 50 (zeroextrap) or 60 (symmetry); the main
-        variant only."""
+        acting on faces 2 and 4), 50 (zeroextrap), 60 (symmetry), 41 or 42
+        (noslip isothermal or adiabatic, faces 1-4); the main variant only."""
         arr = (ctypes.c_int * 6)(*[int(v) for v in bctype])
         if self.lib.orc_box_set_bc(self.h, arr) != 0:
-            raise ValueError(f"bctype {tuple(bctype)}: 0, 23, 50 or 60 per face, main variant")
+            raise ValueError(f"bctype {tuple(bctype)}: 0, 23, 50, 60 (41, 42 on faces 1-4) per "
+                             f"face, main variant")
 
     @property
     def shape(self):
@@ -285,7 +292,7 @@ class UwdCfg(ctypes.Structure):
                 ("lchardecomp", ctypes.c_int), ("bfacmpld", ctypes.c_double),
                 ("shkcrt", ctypes.c_double), ("gamma", ctypes.c_double),
                 ("mach", ctypes.c_double), ("lx", ctypes.c_double), ("ly", ctypes.c_double),
-                ("lz", ctypes.c_double)]
+                ("lz", ctypes.c_double), ("twall", ctypes.c_double * 6)]
 
 
 _uwd_declared = False
@@ -344,6 +351,8 @@ def uwd_lib():
         lib.orc_zeroextrap.argtypes = [P, _I, _D, _D, _D, _D, _D]
         lib.orc_symmetry.restype = None
         lib.orc_symmetry.argtypes = [P, _I, _D, _D, _D, _D, _D]
+        lib.orc_wall.restype = None
+        lib.orc_wall.argtypes = [P, _I, _I, _D, _D, _D, _D, _D]
         lib.orc_uwd_rk3_step.restype = _I
         lib.orc_uwd_rk3_step.argtypes = [P, _I, _I, I6, _I, _F, _F, _D, _D, _D, _D, _D, _D, _D]
         _uwd_declared = True
@@ -411,10 +420,12 @@ class UwdBox:
     (Fortran order, halos hm = 5 on every axis; km = 0 is a 2-D box)."""
 
     def __init__(self, dims, npdc=(3, 3, 3), recon_schem=3, lchardecomp=True, bfacmpld=0.7,
-                 shkcrt=0.01, gamma=1.4, mach=1.0, lengths=(1.0, 1.0, 0.0)):
+                 shkcrt=0.01, gamma=1.4, mach=1.0, lengths=(1.0, 1.0, 0.0),
+                 twall=(1.0,) * 6):
         im, jm, km = dims
         self.cfg = UwdCfg(im, jm, km, (ctypes.c_int * 3)(*npdc), recon_schem, int(lchardecomp),
-                          bfacmpld, shkcrt, gamma, mach, *lengths)
+                          bfacmpld, shkcrt, gamma, mach, *lengths,
+                          (ctypes.c_double * 6)(*[float(t) for t in twall]))
         self.dims = (im, jm, km)
         self.hshape = (im + 1 + 2 * HM, jm + 1 + 2 * HM, km + 1 + 2 * HM)
         self.ishape = (im + 1, jm + 1, km + 1)
@@ -504,6 +515,15 @@ class UwdBox:
         o = self._state(st)
         lib.orc_zeroextrap(ctypes.byref(self.cfg), int(face), _p(o["q"]), _p(o["rho"]),
                            _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
+        return o
+
+    def wall(self, st, face, isothermal):
+        """A copy of st after noslip (bctype 41, isothermal, at the face's
+        twall) or noslip_adibatic (42) on face 1..4."""
+        lib = uwd_lib()
+        o = self._state(st)
+        lib.orc_wall(ctypes.byref(self.cfg), int(face), int(bool(isothermal)), _p(o["q"]),
+                     _p(o["rho"]), _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
         return o
 
     def symmetry(self, st, face):

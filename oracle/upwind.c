@@ -1485,6 +1485,51 @@ void orc_symmetry(const orc_uwd *u, int face, double *q, double *rho, double *ve
         }
 }
 
+/* noslip (src/bc.F90:6281-6595), bctype 41, and noslip_adibatic
+ * (src/bc.F90:6611-6919), bctype 42, on face 1..4 (the reference stops on
+ * faces 5 and 6): the boundary node at rest, its pressure num1d3*(4 p1 -
+ * p2) from the two inner neighbours, its temperature the face's wall
+ * temperature (41) or extrapolated alike (42), the density from
+ * thermal(p, T) = p/T*const2 and q by fvar2q's pressure form.  The wall
+ * blowing and suction of face 3 (datin/wallbs.dat) is not read: without
+ * that file the reference's wall velocity is zero too. */
+void orc_wall(const orc_uwd *u, int face, int isothermal, double *q, double *rho, double *vel,
+              double *prs, double *tmp)
+{
+    if (face < 1 || face > 4) return;
+    const shape_t s = shape_of(u);
+    const double gamma = u->gamma, mach = u->mach;
+    const double const2 = gamma * (mach * mach), const6 = 1.0 / (gamma - 1.0);
+    const double n1d3 = 1.0 / 3.0;
+    const int ax = (face - 1) / 2, hi = (face - 1) % 2;
+    const int n[3] = {u->im, u->jm, u->km};
+    const int oa = ax == 0 ? 1 : 0, ob = 2;
+    for (int b = 0; b <= n[ob]; ++b)
+        for (int a = 0; a <= n[oa]; ++a) {
+            int c[3], c1[3], c2[3];
+            c[oa] = c1[oa] = c2[oa] = a;
+            c[ob] = c1[ob] = c2[ob] = b;
+            c[ax] = hi ? n[ax] : 0;
+            c1[ax] = hi ? n[ax] - 1 : 1;
+            c2[ax] = hi ? n[ax] - 2 : 2;
+            const long h = hidx(&s, c[0], c[1], c[2]);
+            const long h1 = hidx(&s, c1[0], c1[1], c1[2]), h2 = hidx(&s, c2[0], c2[1], c2[2]);
+            const double pe = n1d3 * (4.0 * prs[h1] - prs[h2]);
+            const double te = isothermal ? u->twall[face - 1] : n1d3 * (4.0 * tmp[h1] - tmp[h2]);
+            const double re = pe / te * const2;
+            prs[h] = pe;
+            tmp[h] = te;
+            rho[h] = re;
+            for (int m = 0; m < 3; ++m) vel[m * s.nf + h] = 0.0;
+            q[h] = re;
+            q[s.nf + h] = re * 0.0;
+            q[2 * s.nf + h] = re * 0.0;
+            q[3 * s.nf + h] = re * 0.0;
+            const double var1 = 0.5 * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+            q[4 * s.nf + h] = pe * const6 + re * var1;
+        }
+}
+
 void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, double *q,
                 double *rho, double *vel, double *prs, double *tmp)
 {
@@ -1499,6 +1544,8 @@ void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, doubl
             orc_zeroextrap(u, f + 1, q, rho, vel, prs, tmp);
         } else if (bctype[f] == 60) {
             orc_symmetry(u, f + 1, q, rho, vel, prs, tmp);
+        } else if (bctype[f] == 41 || bctype[f] == 42) {
+            orc_wall(u, f + 1, bctype[f] == 41, q, rho, vel, prs, tmp);
         }
     }
 }
@@ -1577,7 +1624,7 @@ int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *
     int anyudf = 0, anybc = 0;
     for (int f = 0; f < 6; ++f) {
         if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50 &&
-            bctype[f] != 60)
+            bctype[f] != 60 && bctype[f] != 41 && bctype[f] != 42)
             return 3;
         anyudf |= bctype[f] == 0;
         anybc |= bctype[f] != 1;

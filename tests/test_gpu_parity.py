@@ -170,10 +170,11 @@ def test_gcnscbc_central_steps_match_checker(v):
 
 
 @pytest.mark.parametrize("faces", [(50, 50, 50, 50, 0, 0), (60, 60, 60, 60, 0, 0),
-                                   (60, 50, 50, 60, 0, 0)])
+                                   (60, 50, 50, 60, 0, 0), (41, 41, 42, 42, 0, 0)])
 def test_zeroextrap_central_steps_match_checker(faces):
-    """zeroextrap (bctype 50) and symmetry (60) on the four faces of the
-    bounded x and y axes of a three-dimensional box (periodic z) through
+    """zeroextrap (bctype 50), symmetry (60) and the noslip walls (41, 42) on
+    the four faces of the bounded x and y axes of a three-dimensional box
+    (periodic z; the walls make it a viscous channel) through
     whole RK3 steps of the central path against the checker, applied face by
     face before each stage's qswap: statistics at 1e-10, every node and the
     periodic axis' halo faces at 1e-10 after 3 steps."""
@@ -181,14 +182,17 @@ def test_zeroextrap_central_steps_match_checker(faces):
 
     dims, mach = (32, 28, 24), 0.4
     print(f"case zeroextrap/symmetry central dims={dims} faces={faces}", flush=True)
+    tw = (1.05, 0.95, 1.1, 0.9, 1.0, 1.0)
     ob = pyoracle.OracleBox(dims, variant=0, npdc=(4, 4, 3), mach=mach)
     ob.set_bc(faces)
+    ob.set_twall(tw)
     pulse(ob, mach)
     st = ob.state_haloed()
     p = ag.default_params(dims, npdci=4, npdcj=4, npdck=3, mach=mach)
     s = ag.AstrSolver(p)
     s.upload_geom(*ag.box_metrics(p))
     s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
+    s.set_twall(tw)
     s.set_bc(tuple(f if f else 1 for f in faces), 0)
     dt = 0.01
     for n in range(3):

@@ -237,15 +237,19 @@ def _boucon_ref(box, st, faces):
             o = box.zeroextrap(o, f + 1)
         elif b == 60:
             o = box.symmetry(o, f + 1)
+        elif b in (41, 42):
+            o = box.wall(o, f + 1, b == 41)
     return o
 
 
 @pytest.mark.parametrize("faces", [(50, 50, 50, 50, 1, 1), (50, 23, 50, 23, 1, 1),
-                                   (60, 60, 60, 60, 1, 1), (60, 23, 50, 60, 1, 1)])
+                                   (60, 60, 60, 60, 1, 1), (60, 23, 50, 60, 1, 1),
+                                   (41, 42, 41, 42, 1, 1), (50, 41, 60, 42, 1, 1)])
 def test_zeroextrap_steps_match_checker(faces):
-    """zeroextrap (bctype 50) and symmetry (60) on the faces of a 2-D box,
-    alone and with gcnscbc on faces 2 and 4 (the reference's face order
-    1..6), through boucon and whole upwind steps against the checker."""
+    """zeroextrap (bctype 50), symmetry (60) and the noslip walls (41 at its
+    face's wall temperature, 42) on the faces of a 2-D box, alone, mixed and
+    with gcnscbc on faces 2 and 4 (the reference's face order 1..6), through
+    boucon and whole upwind steps against the checker."""
     dims = (64, 60, 0)
     mach = 0.4
     print(f"case zeroextrap dims={dims} faces={faces}", flush=True)
@@ -258,9 +262,11 @@ def test_zeroextrap_steps_match_checker(faces):
     s.upload_geom(jac, dxi)
     s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
     s.set_lengths(1.0, 1.0, 0.0)
+    tw = (1.05, 0.95, 1.1, 0.9, 1.0, 1.0)
+    s.set_twall(tw)
     s.set_bc(faces, 0)
     box = po.UwdBox(dims, npdc=(4, 4, 3), recon_schem=3, lchardecomp=True, bfacmpld=0.7,
-                    shkcrt=0.01, gamma=p.gamma, mach=mach)
+                    shkcrt=0.01, gamma=p.gamma, mach=mach, twall=tw)
     dxi9 = dxi.reshape(dxi.shape[:3] + (9,), order="F")
     s.boucon()
     compare_plane(s.download_state(), _boucon_ref(box, st, faces), "boucon")
@@ -280,4 +286,8 @@ def test_zeroextrap_rejected_where_not_built():
         s.set_bc((50, 0, 1, 1, 1, 1), ag.UDF_RIEMANN2D)  # with a user condition
     with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
         s.set_bc((1, 1, 1, 1, 50, 1), 0)  # a k face of a 2-D box
+    with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
+        s.set_bc((1, 1, 1, 1, 41, 1), 0)  # walls act on faces 1-4
+    with pytest.raises(ag.AstrError, match="ASTR_E_ARG"):
+        s.set_twall((1.0, 0.0, 1.0, 1.0, 1.0, 1.0))
     s.close()

@@ -121,3 +121,48 @@ def test_symmetry_zeroes_the_normal_velocity(face):
 def test_central_box_accepts_symmetry():
     ob = po.OracleBox((16, 12, 10), variant=0, npdc=(4, 4, 3))
     ob.set_bc((60, 60, 50, 60, 0, 0))
+
+
+@pytest.mark.parametrize("face,bct", [(1, 41), (2, 42), (3, 41), (4, 42)])
+def test_walls_hold_the_fluid_at_rest(face, bct):
+    """noslip (41) and noslip_adibatic (42): the wall node at rest, its
+    pressure the zero-slope parabola's face value, its temperature the face's
+    wall temperature (41) or extrapolated alike (42), the density from the
+    equation of state and the energy p/(gamma-1)."""
+    dims = (12, 10, 0)
+    tw = (1.3, 1.1, 0.9, 1.2, 1.0, 1.0)
+    box = po.UwdBox(dims, npdc=(4, 4, 3), gamma=GAMMA, mach=MACH, twall=tw)
+    c = (1.1, 0.3, -0.2, 0.05, 1.0 / (GAMMA * MACH * MACH))
+    d = (0.01, 0.02, -0.015, 0.005, 0.03)
+    st, _ = parabolic_state(dims, face, c, d)
+    o = box.wall(st, face, bct == 41)
+    ax, hi = (face - 1) // 2, (face - 1) % 2
+    sl = [slice(HM, HM + n + 1) for n in dims]
+    sl[ax] = HM + (dims[ax] if hi else 0)
+    sl = tuple(sl)
+    const2 = GAMMA * MACH * MACH
+    for m in range(3):
+        assert np.all(o["vel"][sl + (m,)] == 0.0)
+    p, rho, t = o["prs"][sl], o["rho"][sl], o["tmp"][sl]
+    assert np.max(np.abs(p - c[4])) <= 1e-14 * c[4]
+    if bct == 41:
+        assert np.all(t == tw[face - 1])
+    else:
+        # the extrapolated temperature of the parabolic state's own T
+        t1 = st["tmp"][tuple(HM + (dims[ax] - 1 if hi else 1) if k == ax else s_
+                             for k, s_ in enumerate(sl))]
+        t2 = st["tmp"][tuple(HM + (dims[ax] - 2 if hi else 2) if k == ax else s_
+                             for k, s_ in enumerate(sl))]
+        assert np.max(np.abs(t - (4.0 * t1 - t2) / 3.0)) <= 1e-14
+    assert np.max(np.abs(rho - p / t * const2)) <= 1e-14
+    q = o["q"][sl]
+    assert np.array_equal(q[..., 0], rho)
+    assert np.all(q[..., 1:4] == 0.0)
+    assert np.max(np.abs(q[..., 4] - p / (GAMMA - 1.0))) <= 1e-14 * np.max(p)
+
+
+def test_walls_only_on_faces_1_to_4():
+    ob = po.OracleBox((16, 12, 10), variant=0, npdc=(4, 4, 4))
+    ob.set_bc((41, 42, 41, 42, 0, 0))
+    with pytest.raises(ValueError):
+        ob.set_bc((0, 0, 0, 0, 41, 0))
