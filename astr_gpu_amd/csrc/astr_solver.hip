@@ -2745,6 +2745,7 @@ struct BcArgs {
     double gamma, mach, const1, const2, const6;
     double len[2];  // gridcube lengths of x and y (x(i) = len/n*i), for gcnscbc
     double tw[6];   // wall temperature of each face (noslip)
+    double dt;      // the step's time step (outflow)
 };
 
 // the boundary conditions of k_bc_extrap
@@ -3032,6 +3033,65 @@ __global__ void k_bc_extrap(BcArgs a)
     a.q[4][gi] = TFORM ? re * (te * a.const1 + var1) : pe * a.const6 + re * var1;
 }
 
+// outflow (src/bc.F90:3379-3592), bctype 21, on face 2 (AX 0) or face 4
+// (AX 1), the faces it acts on.  Face 2 copies velocity, pressure and
+// temperature of node n-1 (the reference's zero-order branch, always
+// taken), the density from thermal(p, T) = p/T*const2, q by fvar2q's
+// temperature form.  Face 4 extrapolates velocity, pressure, temperature
+// and density by extrapolate_2o with dv = 0; where the node's own normal
+// velocity is at least its sound speed css = sqrt(T)/Ma it takes them as
+// they are, else its pressure relaxes towards pinf = 1/const2,
+// p = (p + alpha dt pinf + rho css (ve - v))/(1 + alpha dt), alpha = 0.25,
+// with the node's own p, rho and v and the step's dt, and the density from
+// (p, Te); then T = p/rho*const2 and q by the pressure form.
+template <int AX>
+__global__ void k_bc_outflow(BcArgs a)
+{
+    const long ea = (AX == 0 ? a.g.n[1] : a.g.n[0]) + 1L;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ea * (a.g.n[2] + 1L)) return;
+    const int ca = (int)(t % ea), k = (int)(t / ea);
+    const int n = a.g.n[AX];
+    auto at = [&](int cd) { return AX == 0 ? gidx(a.g, cd, ca, k) : gidx(a.g, ca, cd, k); };
+    const long gi = at(n), g1 = at(n - 1), g2 = at(n - 2);
+    const double n1d3 = 1.0 / 3.0;
+    double ve[3], pn, rn, tn;
+    if (AX == 0) {
+        for (int m = 0; m < 3; ++m) ve[m] = a.prim[1 + m][g1];
+        pn = a.prim[4][g1];
+        tn = a.prim[5][g1];
+        rn = pn / tn * a.const2;
+    } else {
+        const double pinf = 1.0 / a.const2, alpha = 0.25;
+        const double css = sqrt(a.prim[5][gi]) / a.mach;
+        const double ub = a.prim[2][gi];
+        for (int m = 0; m < 3; ++m)
+            ve[m] = n1d3 * (4.0 * a.prim[1 + m][g1] - a.prim[1 + m][g2] - 2.0 * 0.0);
+        const double pe = n1d3 * (4.0 * a.prim[4][g1] - a.prim[4][g2] - 2.0 * 0.0);
+        const double te = n1d3 * (4.0 * a.prim[5][g1] - a.prim[5][g2] - 2.0 * 0.0);
+        const double roe = n1d3 * (4.0 * a.prim[0][g1] - a.prim[0][g2] - 2.0 * 0.0);
+        if (ub >= css) {
+            pn = pe;
+            rn = roe;
+        } else {
+            pn = (a.prim[4][gi] + alpha * a.dt * pinf + a.prim[0][gi] * css * (ve[1] - ub)) /
+                 (1.0 + alpha * a.dt);
+            rn = pn / te * a.const2;
+        }
+        tn = pn / rn * a.const2;
+    }
+    a.prim[0][gi] = rn;
+    for (int m = 0; m < 3; ++m) a.prim[1 + m][gi] = ve[m];
+    a.prim[4][gi] = pn;
+    a.prim[5][gi] = tn;
+    a.q[0][gi] = rn;
+    a.q[1][gi] = rn * ve[0];
+    a.q[2][gi] = rn * ve[1];
+    a.q[3][gi] = rn * ve[2];
+    const double var1 = 0.5 * (ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
+    a.q[4][gi] = AX == 0 ? rn * (tn * a.const1 + var1) : pn * a.const6 + rn * var1;
+}
+
 // back to the default contraction of the rest of the library
 #pragma clang fp contract(fast)
 
@@ -3186,6 +3246,7 @@ struct astr_ctx {
     double* cmp_ac[3][2] = {};      // compact upwind Thomas factors per axis, wind
     double len[3] = {1.0, 1.0, 0.0};  // gridcube lengths (astr_set_lengths)
     double twall[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0};  // wall temperatures (astr_set_twall)
+    double bc_dt = 0.0;  // time step of the latest whole step, which outflow reads
     bool curv = false;                // general metrics uploaded (dxf)
     bool juni = false;                // jacob uniform (j0): read from no array
     double j0 = 0.0;
@@ -3754,7 +3815,7 @@ int do_boucon(astr_ctx* c)
     bool any = c->udf == ASTR_UDF_RIEMANN2D;
     for (int f = 0; f < 6; ++f)
         any = any || c->bctype[f] == 50 || c->bctype[f] == 60 || c->bctype[f] == 23 ||
-              c->bctype[f] == 41 || c->bctype[f] == 42;
+              c->bctype[f] == 41 || c->bctype[f] == 42 || c->bctype[f] == 21;
     if (!any) return ASTR_OK;
     BcArgs a{};
     a.g = c->g;
@@ -3770,6 +3831,7 @@ int do_boucon(astr_ctx* c)
     a.len[0] = c->len[0];
     a.len[1] = c->len[1];
     for (int f = 0; f < 6; ++f) a.tw[f] = c->twall[f];
+    a.dt = c->bc_dt;
     // boucon's face order (src/bc.F90:338-404), face by face 1..6: the user
     // condition of faces 1 and 2 (and of 3 and 4) in one kernel (it is never
     // combined with zeroextrap or symmetry), gcnscbc on faces 2 and 4,
@@ -3790,6 +3852,16 @@ int do_boucon(astr_ctx* c)
         if (c->bctype[f] == 50) TRY(launch_extrap<BCX_ZEROEXTRAP>(c, a, f));
         if (c->bctype[f] == 60) TRY(launch_extrap<BCX_SYMMETRY>(c, a, f));
         if (c->bctype[f] == 41) TRY(launch_extrap<BCX_NOSLIP>(c, a, f));
+        if (c->bctype[f] == 21 && f == 1) {
+            hipLaunchKernelGGL(k_bc_outflow<0>, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0,
+                               c->stream, a);
+            HIPCHK(hipGetLastError());
+        }
+        if (c->bctype[f] == 21 && f == 3) {
+            hipLaunchKernelGGL(k_bc_outflow<1>, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0,
+                               c->stream, a);
+            HIPCHK(hipGetLastError());
+        }
         if (c->bctype[f] == 42) TRY(launch_extrap<BCX_ADIABATIC>(c, a, f));
         if (c->bctype[f] == 23 && f == 1) {
             hipLaunchKernelGGL(k_bc_gcnscbc<0>, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0,
@@ -4186,6 +4258,7 @@ int time_mark(astr_ctx* c, int slot)
 // batch that holds this context rebuild its launches.
 int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed, bool alt = false)
 {
+    c->bc_dt = dt;  // boucon's outflow reads the step's deltat
     // timing families: 0 filter, 1 boucon and qswap, 2 grad, 3 central:
     // sigma and the axis fluxes (k_gflux), upwind: the shock sensor,
     // 4 stats, 5 flux sweeps (central: with the RK3 combine; upwind: the
@@ -4195,7 +4268,8 @@ int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed, bool alt = f
     bool bc_reads_prim = c->udf == ASTR_UDF_RIEMANN2D;
     for (int f = 0; f < 6; ++f)
         bc_reads_prim = bc_reads_prim || c->bctype[f] == 23 || c->bctype[f] == 50 ||
-                        c->bctype[f] == 60 || c->bctype[f] == 41 || c->bctype[f] == 42;
+                        c->bctype[f] == 60 || c->bctype[f] == 41 || c->bctype[f] == 42 ||
+                        c->bctype[f] == 21;
     if (c->p.variant == 1) {
         // rk3 of tgvsolver.F90:994-1082: bchomo, rhscal (gradcal, convection,
         // diffusion by df/dx), stage-1 statistics and qsave, the RK3 combine
@@ -5337,7 +5411,7 @@ static bool has_face_bc(const astr_ctx* c)
 {
     for (int f = 0; f < 6; ++f)
         if ((c->bctype[f] == 23 && (f == 1 || f == 3)) || c->bctype[f] == 50 ||
-            c->bctype[f] == 60 || c->bctype[f] == 41 || c->bctype[f] == 42)
+            c->bctype[f] == 60 || c->bctype[f] == 41 || c->bctype[f] == 42 || c->bctype[f] == 21)
             return true;
     return false;
 }
@@ -5430,6 +5504,7 @@ int astr_rk3_update(astr_ctx* c, int32_t stage, double dt)
     TRY(ready(c));
     if (c->p.variant == 1) return fail(ASTR_E_UNSUPPORTED, "the mini-app variant runs whole steps only");
     if (stage < 1 || stage > 3) return fail(ASTR_E_ARG, "stage must be 1..3");
+    c->bc_dt = dt;  // the next boucon's outflow reads it
     return do_rk(c, stage, dt);
 }
 
@@ -5926,15 +6001,17 @@ int astr_set_bc(astr_ctx* c, const int32_t* bctype, int32_t udf)
     for (int f = 0; f < 6; ++f) {
         const bool wall = bctype[f] == 41 || bctype[f] == 42;
         if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50 &&
-            bctype[f] != 60 && !wall)
+            bctype[f] != 60 && bctype[f] != 21 && !wall)
             return fail(ASTR_E_UNSUPPORTED,
                         "bctype 1 (nothing done), 0 (user condition), 23 (gcnscbc), 50 "
-                        "(zeroextrap), 60 (symmetry), 41 and 42 (noslip walls) are built; the "
-                        "inflow, outflow, far-field, slip-wall and other NSCBC conditions of "
+                        "(zeroextrap), 60 (symmetry), 41 and 42 (noslip walls), 21 (outflow) are "
+                        "built; the inflow, far-field, slip-wall and other NSCBC conditions of "
                         "src/bc.F90 are not");
         if (wall && f >= 4)
             return fail(ASTR_E_UNSUPPORTED, "noslip walls act on faces 1-4 (src/bc.F90:6281-6919)");
-        const bool ex = bctype[f] == 50 || bctype[f] == 60 || wall;
+        if (bctype[f] == 21 && f != 1 && f != 3)
+            return fail(ASTR_E_UNSUPPORTED, "outflow acts on faces 2 and 4 (src/bc.F90:3379-3592)");
+        const bool ex = bctype[f] == 50 || bctype[f] == 60 || wall || bctype[f] == 21;
         if (ex && f >= 4 && c->g.n[2] == 0)
             return fail(ASTR_E_UNSUPPORTED,
                         "zeroextrap and symmetry on the k faces need a three-dimensional box");

@@ -348,9 +348,12 @@ int astr_set_spacing(astr_ctx *ctx, double dx, double dy, double dz);
  * (the k faces of three-dimensional boxes only).  41 and 42 are the noslip
  * walls (src/bc.F90:6281-6919): at rest, the pressure extrapolated, the
  * temperature the face's wall temperature (41, astr_set_twall) or
- * extrapolated (42), on faces 1-4 as the reference has them.  None of these
- * is combined with a user condition; batched members and the adjoint take
- * none.  Faces act in the reference's order 1..6.  The other bctypes of
+ * extrapolated (42), on faces 1-4 as the reference has them.  21 is outflow
+ * (src/bc.F90:3379-3592) on faces 2 and 4: face 2 copies node n-1, face 4
+ * extrapolates and, below the sound speed, relaxes the pressure towards
+ * pinf at the rate 0.25 dt (dt of the latest step or astr_rk3_update).
+ * None of these is combined with a user condition; batched members and the
+ * adjoint take none.  Faces act in the reference's order 1..6.  The other bctypes of
  * bc.F90 (inflow, outflow, walls, the other NSCBCs) return
  * ASTR_E_UNSUPPORTED.  Default: all faces 1. */
 #define ASTR_UDF_RIEMANN2D 1

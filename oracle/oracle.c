@@ -567,7 +567,8 @@ int orc_box_set_bc(orc_box *b, const int *bctype)
     if (b->variant != 0) return -1;
     for (int f = 0; f < 6; ++f)
         if (bctype[f] != 0 && bctype[f] != 23 && bctype[f] != 50 && bctype[f] != 60 &&
-            !((bctype[f] == 41 || bctype[f] == 42) && f < 4))
+            !((bctype[f] == 41 || bctype[f] == 42) && f < 4) &&
+            !(bctype[f] == 21 && (f == 1 || f == 3)))
             return -1;
     for (int f = 0; f < 6; ++f) b->bc[f] = bctype[f];
     return 0;
@@ -585,7 +586,7 @@ void orc_box_set_twall(orc_box *b, const double *twall)
     for (int f = 0; f < 6; ++f) b->twall[f] = twall[f];
 }
 
-static void box_boucon(orc_box *b)
+static void box_boucon(orc_box *b, double dt)
 {
     int any = 0;
     for (int f = 0; f < 6; ++f) any |= b->bc[f] != 0;
@@ -596,6 +597,7 @@ static void box_boucon(orc_box *b)
     u.gamma = b->gamma; u.mach = b->mach;
     u.lx = b->lx; u.ly = b->ly; u.lz = b->lz;
     for (int f = 0; f < 6; ++f) u.twall[f] = b->twall[f];
+    u.deltat = dt;
     const size_t n = b->n3;
     double *q = malloc(5 * n * sizeof(double)), *vel = malloc(3 * n * sizeof(double));
     for (int m = 0; m < 5; ++m) memcpy(q + m * n, b->q[m], n * sizeof(double));
@@ -606,6 +608,7 @@ static void box_boucon(orc_box *b)
         else if (b->bc[f] == 60) orc_symmetry(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
         else if (b->bc[f] == 41 || b->bc[f] == 42)
             orc_wall(&u, f + 1, b->bc[f] == 41, q, b->rho, vel, b->prs, b->tmp);
+        else if (b->bc[f] == 21) orc_outflow(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
     }
     for (int m = 0; m < 5; ++m) memcpy(b->q[m], q + m * n, n * sizeof(double));
     for (int c = 0; c < 3; ++c) memcpy(b->vel[c], vel + c * n, n * sizeof(double));
@@ -1364,7 +1367,7 @@ void orc_box_step(orc_box *b, double dt, double *stats)
         for (int s = 0; s < 3; ++s) {
             orc_box_filterq(b);
             zero_qrhs(b);
-            box_boucon(b);  /* boucon before qswap, src/mainloop.F90:413-415 */
+            box_boucon(b, dt);  /* boucon before qswap, src/mainloop.F90:413-415 */
             orc_box_qswap(b);
             orc_box_rhscal(b);
             if (s == 0) {

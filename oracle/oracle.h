@@ -84,7 +84,8 @@ int orc_box_set_ntype(orc_box *b, int n0, int n1, int n2);
  * (gcnscbc, src/bc.F90:7907-8140, acting on faces 2 and 4), 50
  * (zeroextrap, src/bc.F90:2331-2627), 60 (symmetry, src/bc.F90:
  * 2639-2971, rectilinear grids), 41 or 42 (noslip, noslip_adibatic,
- * src/bc.F90:6281-6919, faces 1-4); applied in face order; 0 or -1 */
+ * src/bc.F90:6281-6919, faces 1-4) or 21 (outflow, src/bc.F90:3379-3592,
+ * faces 2 and 4); applied in face order; 0 or -1 */
 int orc_box_set_bc(orc_box *b, const int *bctype);
 /* wall temperature of each face for bctype 41 (twall of src/bc.F90:29) */
 void orc_box_set_twall(orc_box *b, const double *twall);
@@ -129,6 +130,7 @@ typedef struct {
     double gamma, mach;
     double lx, ly, lz;   /* gridcube lengths (x(i) = lx/im*i): gcnscbc's coordinates */
     double twall[6];     /* wall temperature of each face (bctype 41) */
+    double deltat;       /* the step's time step, which outflow (bctype 21) reads */
 } orc_uwd;
 
 double orc_minmod2(double a, double b);
@@ -194,6 +196,10 @@ void orc_zeroextrap(const orc_uwd *u, int face, double *q, double *rho, double *
  * extrapolation with the normal velocity zero (src/bc.F90:2639-2971) */
 void orc_symmetry(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
                   double *tmp);
+/* outflow (bctype 21) on face 2 or 4 (src/bc.F90:3379-3592; u->deltat the
+ * step's time step) */
+void orc_outflow(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+                 double *tmp);
 /* noslip (bctype 41, wall temperature u->twall[face-1]) and noslip_adibatic
  * (42) on face 1..4 (src/bc.F90:6281-6919) */
 void orc_wall(const orc_uwd *u, int face, int isothermal, double *q, double *rho, double *vel,

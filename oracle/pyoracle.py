@@ -292,7 +292,8 @@ class UwdCfg(ctypes.Structure):
                 ("lchardecomp", ctypes.c_int), ("bfacmpld", ctypes.c_double),
                 ("shkcrt", ctypes.c_double), ("gamma", ctypes.c_double),
                 ("mach", ctypes.c_double), ("lx", ctypes.c_double), ("ly", ctypes.c_double),
-                ("lz", ctypes.c_double), ("twall", ctypes.c_double * 6)]
+                ("lz", ctypes.c_double), ("twall", ctypes.c_double * 6),
+                ("deltat", ctypes.c_double)]
 
 
 _uwd_declared = False
@@ -353,6 +354,8 @@ def uwd_lib():
         lib.orc_symmetry.argtypes = [P, _I, _D, _D, _D, _D, _D]
         lib.orc_wall.restype = None
         lib.orc_wall.argtypes = [P, _I, _I, _D, _D, _D, _D, _D]
+        lib.orc_outflow.restype = None
+        lib.orc_outflow.argtypes = [P, _I, _D, _D, _D, _D, _D]
         lib.orc_uwd_rk3_step.restype = _I
         lib.orc_uwd_rk3_step.argtypes = [P, _I, _I, I6, _I, _F, _F, _D, _D, _D, _D, _D, _D, _D]
         _uwd_declared = True
@@ -425,7 +428,7 @@ class UwdBox:
         im, jm, km = dims
         self.cfg = UwdCfg(im, jm, km, (ctypes.c_int * 3)(*npdc), recon_schem, int(lchardecomp),
                           bfacmpld, shkcrt, gamma, mach, *lengths,
-                          (ctypes.c_double * 6)(*[float(t) for t in twall]))
+                          (ctypes.c_double * 6)(*[float(t) for t in twall]), 0.0)
         self.dims = (im, jm, km)
         self.hshape = (im + 1 + 2 * HM, jm + 1 + 2 * HM, km + 1 + 2 * HM)
         self.ishape = (im + 1, jm + 1, km + 1)
@@ -515,6 +518,16 @@ class UwdBox:
         o = self._state(st)
         lib.orc_zeroextrap(ctypes.byref(self.cfg), int(face), _p(o["q"]), _p(o["rho"]),
                            _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
+        return o
+
+    def outflow(self, st, face, dt):
+        """A copy of st after outflow (bctype 21) on face 2 or 4, with the
+        step's time step dt."""
+        lib = uwd_lib()
+        o = self._state(st)
+        self.cfg.deltat = float(dt)
+        lib.orc_outflow(ctypes.byref(self.cfg), int(face), _p(o["q"]), _p(o["rho"]),
+                        _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
         return o
 
     def wall(self, st, face, isothermal):

@@ -1485,6 +1485,78 @@ void orc_symmetry(const orc_uwd *u, int face, double *q, double *rho, double *ve
         }
 }
 
+/* outflow (src/bc.F90:3379-3592), bctype 21, on face 2 or 4 (the faces it
+ * acts on).  Face 2 copies the velocity, pressure and temperature of node
+ * n-1 (the reference's zero-order branch, taken always), the density from
+ * thermal(p, T) = p/T*const2 and q by fvar2q's temperature form.  Face 4
+ * extrapolates velocity, pressure, temperature and density by
+ * extrapolate_2o with dv = 0; where the node's own normal velocity v is at
+ * least its sound speed css = sos(T) = sqrt(T)/Ma it takes them as they
+ * are, else its pressure relaxes towards pinf, p = (p + alpha dt pinf +
+ * rho css (ve - v))/(1 + alpha dt) with alpha = 0.25, the node's own p, rho
+ * and v and the step's dt, and its density comes from (p, Te); on either
+ * branch T = thermal(p, rho) = p/rho*const2 afterwards and q by the pressure
+ * form.  pinf = 1/const2 (the nondimensional freestream). */
+void orc_outflow(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+                 double *tmp)
+{
+    if (face != 2 && face != 4) return;
+    const shape_t s = shape_of(u);
+    const double gamma = u->gamma, mach = u->mach;
+    const double const1 = 1.0 / (gamma * (gamma - 1.0) * (mach * mach));
+    const double const2 = gamma * (mach * mach), const6 = 1.0 / (gamma - 1.0);
+    const double pinf = 1.0 / const2, alpha = 0.25, dt = u->deltat;
+    const double n1d3 = 1.0 / 3.0;
+    const int ax = face == 2 ? 0 : 1;
+    const int n[3] = {u->im, u->jm, u->km};
+    const int oa = ax == 0 ? 1 : 0, ob = 2;
+    for (int b = 0; b <= n[ob]; ++b)
+        for (int a = 0; a <= n[oa]; ++a) {
+            int c[3], c1[3], c2[3];
+            c[oa] = c1[oa] = c2[oa] = a;
+            c[ob] = c1[ob] = c2[ob] = b;
+            c[ax] = n[ax];
+            c1[ax] = n[ax] - 1;
+            c2[ax] = n[ax] - 2;
+            const long h = hidx(&s, c[0], c[1], c[2]);
+            const long h1 = hidx(&s, c1[0], c1[1], c1[2]), h2 = hidx(&s, c2[0], c2[1], c2[2]);
+            double ve[3], pn, rn, tn;
+            if (face == 2) {
+                for (int m = 0; m < 3; ++m) ve[m] = vel[m * s.nf + h1];
+                pn = prs[h1];
+                tn = tmp[h1];
+                rn = pn / tn * const2;
+            } else {
+                const double css = sqrt(tmp[h]) / mach;
+                const double ub = vel[s.nf + h];
+                for (int m = 0; m < 3; ++m)
+                    ve[m] = n1d3 * (4.0 * vel[m * s.nf + h1] - vel[m * s.nf + h2] - 2.0 * 0.0);
+                const double pe = n1d3 * (4.0 * prs[h1] - prs[h2] - 2.0 * 0.0);
+                const double te = n1d3 * (4.0 * tmp[h1] - tmp[h2] - 2.0 * 0.0);
+                const double roe = n1d3 * (4.0 * rho[h1] - rho[h2] - 2.0 * 0.0);
+                if (ub >= css) {
+                    pn = pe;
+                    rn = roe;
+                } else {
+                    pn = (prs[h] + alpha * dt * pinf + rho[h] * css * (ve[1] - vel[s.nf + h])) /
+                         (1.0 + alpha * dt);
+                    rn = pn / te * const2;
+                }
+                tn = pn / rn * const2;
+            }
+            prs[h] = pn;
+            tmp[h] = tn;
+            rho[h] = rn;
+            for (int m = 0; m < 3; ++m) vel[m * s.nf + h] = ve[m];
+            q[h] = rn;
+            q[s.nf + h] = rn * ve[0];
+            q[2 * s.nf + h] = rn * ve[1];
+            q[3 * s.nf + h] = rn * ve[2];
+            const double var1 = 0.5 * (ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
+            q[4 * s.nf + h] = face == 2 ? rn * (tn * const1 + var1) : pn * const6 + rn * var1;
+        }
+}
+
 /* noslip (src/bc.F90:6281-6595), bctype 41, and noslip_adibatic
  * (src/bc.F90:6611-6919), bctype 42, on face 1..4 (the reference stops on
  * faces 5 and 6): the boundary node at rest, its pressure num1d3*(4 p1 -
@@ -1546,6 +1618,8 @@ void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, doubl
             orc_symmetry(u, f + 1, q, rho, vel, prs, tmp);
         } else if (bctype[f] == 41 || bctype[f] == 42) {
             orc_wall(u, f + 1, bctype[f] == 41, q, rho, vel, prs, tmp);
+        } else if (bctype[f] == 21) {
+            orc_outflow(u, f + 1, q, rho, vel, prs, tmp);
         }
     }
 }
@@ -1624,7 +1698,7 @@ int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *
     int anyudf = 0, anybc = 0;
     for (int f = 0; f < 6; ++f) {
         if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50 &&
-            bctype[f] != 60 && bctype[f] != 41 && bctype[f] != 42)
+            bctype[f] != 60 && bctype[f] != 41 && bctype[f] != 42 && bctype[f] != 21)
             return 3;
         anyudf |= bctype[f] == 0;
         anybc |= bctype[f] != 1;
@@ -1636,7 +1710,11 @@ int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *
     unsigned char *lsh = malloc(s.nint);
     int rc = 0;
     for (int st = 0; st < 3 && !rc; ++st) {
-        if (anybc) orc_boucon(u, bctype, udf, time, q, rho, vel, prs, tmp);
+        if (anybc) {
+            orc_uwd ud = *u;  /* boucon's outflow reads the step's deltat */
+            ud.deltat = dt;
+            orc_boucon(&ud, bctype, udf, time, q, rho, vel, prs, tmp);
+        }
         orc_uwd_qswap(u, q, rho, vel, prs, tmp);
         rc = (compact ? orc_cmp_rhscal : orc_uwd_rhscal)(u, explicit_fd, q, rho, vel, prs, tmp,
                                                          dxi, jacob, qrhs, dvel, dtmp, ssf, lsh);

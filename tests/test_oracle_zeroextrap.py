@@ -166,3 +166,69 @@ def test_walls_only_on_faces_1_to_4():
     ob.set_bc((41, 42, 41, 42, 0, 0))
     with pytest.raises(ValueError):
         ob.set_bc((0, 0, 0, 0, 41, 0))
+
+
+def test_outflow_face2_copies_the_inner_node():
+    """outflow (21) on face 2: the zero-order branch the reference always takes
+    (velocity, pressure and temperature of node n-1, the density from the
+    equation of state)."""
+    dims = (12, 10, 0)
+    box = po.UwdBox(dims, npdc=(4, 4, 3), gamma=GAMMA, mach=MACH)
+    c = (1.1, 0.3, -0.2, 0.05, 1.0 / (GAMMA * MACH * MACH))
+    d = (0.01, 0.02, -0.015, 0.005, 0.03)
+    st, _ = parabolic_state(dims, 2, c, d)
+    o = box.outflow(st, 2, 0.01)
+    b, i1 = HM + dims[0], HM + dims[0] - 1
+    J = slice(HM, HM + dims[1] + 1)
+    for nm in ("prs", "tmp"):
+        assert np.array_equal(o[nm][b, J, HM], st[nm][i1, J, HM]), nm
+    for m in range(3):
+        assert np.array_equal(o["vel"][b, J, HM, m], st["vel"][i1, J, HM, m])
+    const2 = GAMMA * MACH * MACH
+    assert np.max(np.abs(o["rho"][b, J, HM] - o["prs"][b, J, HM] / o["tmp"][b, J, HM] * const2)) \
+        <= 1e-14
+
+
+def _face4(dims, v0):
+    """A state whose face-4 node (j = jm) holds the velocity v0 normal to the
+    face while the inner nodes hold the zero-slope parabola."""
+    c = (1.1, 0.3, 0.2, 0.05, 1.0 / (GAMMA * MACH * MACH))
+    d = (0.01, 0.02, -0.015, 0.005, 0.03)
+    st, _ = parabolic_state(dims, 4, c, d)
+    b = HM + dims[1]
+    st["vel"][:, b, :, 1] = v0
+    return st, c, b
+
+
+def test_outflow_face4_supersonic_takes_the_extrapolation():
+    dims = (12, 10, 0)
+    box = po.UwdBox(dims, npdc=(4, 4, 3), gamma=GAMMA, mach=MACH)
+    st, c, b = _face4(dims, 3.0)  # sqrt(T)/Ma is about 2.4 here
+    o = box.outflow(st, 4, 0.01)
+    I = slice(HM, HM + dims[0] + 1)
+    assert np.max(np.abs(o["prs"][I, b, HM] - c[4])) <= 1e-14 * c[4]
+    assert np.max(np.abs(o["rho"][I, b, HM] - c[0])) <= 1e-14
+    assert np.max(np.abs(o["vel"][I, b, HM, 1] - c[2])) <= 1e-14
+    const2 = GAMMA * MACH * MACH
+    assert np.max(np.abs(o["tmp"][I, b, HM] - o["prs"][I, b, HM] / o["rho"][I, b, HM] * const2)) \
+        <= 1e-14
+
+
+def test_outflow_face4_subsonic_relaxes_towards_pinf():
+    """Below the sound speed the new pressure keeps the outgoing characteristic
+    p - rho c v of the node (dt = 0: p_new - rho c ve = p - rho c v, with the
+    node's own rho, c = sqrt(T)/Ma and v), and relaxes towards pinf as dt
+    grows (dt -> infinity: p_new -> pinf)."""
+    dims = (12, 10, 0)
+    box = po.UwdBox(dims, npdc=(4, 4, 3), gamma=GAMMA, mach=MACH)
+    st, c, b = _face4(dims, 0.1)
+    I = slice(HM, HM + dims[0] + 1)
+    p0, r0, t0 = st["prs"][I, b, HM], st["rho"][I, b, HM], st["tmp"][I, b, HM]
+    css = np.sqrt(t0) / MACH
+    o = box.outflow(st, 4, 0.0)
+    ve = o["vel"][I, b, HM, 1]
+    assert np.max(np.abs((o["prs"][I, b, HM] - r0 * css * ve) - (p0 - r0 * css * 0.1))) \
+        <= 1e-13 * np.max(p0)
+    o = box.outflow(st, 4, 1e12)
+    pinf = 1.0 / (GAMMA * MACH * MACH)
+    assert np.max(np.abs(o["prs"][I, b, HM] - pinf)) <= 1e-9 * pinf
