@@ -352,6 +352,8 @@ int astr_set_spacing(astr_ctx *ctx, double dx, double dy, double dz);
  * (src/bc.F90:3379-3592) on faces 2 and 4: face 2 copies node n-1, face 4
  * extrapolates and, below the sound speed, relaxes the pressure towards
  * pinf at the rate 0.25 dt (dt of the latest step or astr_rk3_update).
+ * 31 is fixbc (src/bc.F90:7562-7648): faces 3 and 4 held at its two fixed
+ * states, the other faces left alone.
  * None of these is combined with a user condition; batched members and the
  * adjoint take none.  Faces act in the reference's order 1..6.  The other bctypes of
  * bc.F90 (inflow, outflow, walls, the other NSCBCs) return

@@ -200,6 +200,9 @@ void orc_symmetry(const orc_uwd *u, int face, double *q, double *rho, double *ve
  * step's time step) */
 void orc_outflow(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
                  double *tmp);
+/* fixbc (bctype 31): faces 3 and 4 held at fixed states (src/bc.F90:7562-7648) */
+void orc_fixbc(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+               double *tmp);
 /* noslip (bctype 41, wall temperature u->twall[face-1]) and noslip_adibatic
  * (42) on face 1..4 (src/bc.F90:6281-6919) */
 void orc_wall(const orc_uwd *u, int face, int isothermal, double *q, double *rho, double *vel,

@@ -1557,6 +1557,33 @@ void orc_outflow(const orc_uwd *u, int face, double *q, double *rho, double *vel
         }
 }
 
+/* fixbc (src/bc.F90:7562-7648), bctype 31: face 3 held at rho 2, p 1 and
+ * face 4 at rho 1, p 2.5, both at rest, T = thermal(rho, p) = p/rho*const2,
+ * q by fvar2q's pressure form; the other faces are left alone. */
+void orc_fixbc(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+               double *tmp)
+{
+    if (face != 3 && face != 4) return;
+    const shape_t s = shape_of(u);
+    const double const2 = u->gamma * (u->mach * u->mach), const6 = 1.0 / (u->gamma - 1.0);
+    const int j = face == 3 ? 0 : u->jm;
+    const double rf = face == 3 ? 2.0 : 1.0, pf = face == 3 ? 1.0 : 2.5;
+    for (int k = 0; k <= u->km; ++k)
+        for (int i = 0; i <= u->im; ++i) {
+            const long h = hidx(&s, i, j, k);
+            rho[h] = rf;
+            for (int m = 0; m < 3; ++m) vel[m * s.nf + h] = 0.0;
+            prs[h] = pf;
+            tmp[h] = pf / rf * const2;
+            q[h] = rf;
+            q[s.nf + h] = rf * 0.0;
+            q[2 * s.nf + h] = rf * 0.0;
+            q[3 * s.nf + h] = rf * 0.0;
+            const double var1 = 0.5 * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+            q[4 * s.nf + h] = pf * const6 + rf * var1;
+        }
+}
+
 /* noslip (src/bc.F90:6281-6595), bctype 41, and noslip_adibatic
  * (src/bc.F90:6611-6919), bctype 42, on face 1..4 (the reference stops on
  * faces 5 and 6): the boundary node at rest, its pressure num1d3*(4 p1 -
@@ -1620,6 +1647,8 @@ void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, doubl
             orc_wall(u, f + 1, bctype[f] == 41, q, rho, vel, prs, tmp);
         } else if (bctype[f] == 21) {
             orc_outflow(u, f + 1, q, rho, vel, prs, tmp);
+        } else if (bctype[f] == 31) {
+            orc_fixbc(u, f + 1, q, rho, vel, prs, tmp);
         }
     }
 }
@@ -1698,7 +1727,8 @@ int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *
     int anyudf = 0, anybc = 0;
     for (int f = 0; f < 6; ++f) {
         if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50 &&
-            bctype[f] != 60 && bctype[f] != 41 && bctype[f] != 42 && bctype[f] != 21)
+            bctype[f] != 60 && bctype[f] != 41 && bctype[f] != 42 && bctype[f] != 21 &&
+            bctype[f] != 31)
             return 3;
         anyudf |= bctype[f] == 0;
         anybc |= bctype[f] != 1;

@@ -241,13 +241,15 @@ def _boucon_ref(box, st, faces):
             o = box.wall(o, f + 1, b == 41)
         elif b == 21:
             o = box.outflow(o, f + 1, 0.0)
+        elif b == 31:
+            o = box.fixbc(o, f + 1)
     return o
 
 
 @pytest.mark.parametrize("faces", [(50, 50, 50, 50, 1, 1), (50, 23, 50, 23, 1, 1),
                                    (60, 60, 60, 60, 1, 1), (60, 23, 50, 60, 1, 1),
                                    (41, 42, 41, 42, 1, 1), (50, 41, 60, 42, 1, 1),
-                                   (50, 21, 50, 21, 1, 1)])
+                                   (50, 21, 50, 21, 1, 1), (50, 50, 31, 31, 1, 1)])
 def test_zeroextrap_steps_match_checker(faces):
     """zeroextrap (bctype 50), symmetry (60), the noslip walls (41 at its
     face's wall temperature, 42) and outflow (21, faces 2 and 4) on the faces

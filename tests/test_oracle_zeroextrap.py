@@ -232,3 +232,25 @@ def test_outflow_face4_subsonic_relaxes_towards_pinf():
     o = box.outflow(st, 4, 1e12)
     pinf = 1.0 / (GAMMA * MACH * MACH)
     assert np.max(np.abs(o["prs"][I, b, HM] - pinf)) <= 1e-9 * pinf
+
+
+def test_fixbc_holds_its_two_states():
+    """fixbc (31): face 3 at rho 2, p 1 and face 4 at rho 1, p 2.5, at rest,
+    T = p/rho*const2; faces 1 and 2 left alone."""
+    dims = (12, 10, 0)
+    box = po.UwdBox(dims, npdc=(4, 4, 3), gamma=GAMMA, mach=MACH)
+    c = (1.1, 0.3, -0.2, 0.05, 1.0 / (GAMMA * MACH * MACH))
+    d = (0.01, 0.02, -0.015, 0.005, 0.03)
+    st, _ = parabolic_state(dims, 3, c, d)
+    const2 = GAMMA * MACH * MACH
+    I = slice(HM, HM + dims[0] + 1)
+    for face, j, r, p in ((3, HM, 2.0, 1.0), (4, HM + dims[1], 1.0, 2.5)):
+        o = box.fixbc(st, face)
+        assert np.all(o["rho"][I, j, HM] == r) and np.all(o["prs"][I, j, HM] == p)
+        assert np.all(o["vel"][I, j, HM, :] == 0.0)
+        assert np.max(np.abs(o["tmp"][I, j, HM] - p / r * const2)) <= 1e-15
+        assert np.max(np.abs(o["q"][I, j, HM, 4] - p / (GAMMA - 1.0))) <= 1e-15 * p
+    for face in (1, 2):
+        o = box.fixbc(st, face)
+        for nm in ("q", "rho", "vel", "prs", "tmp"):
+            assert np.array_equal(o[nm], st[nm])
