@@ -117,11 +117,20 @@ def read_input(path):
     d["num_species"] = int(r[0])
     r = nxt()
     d["turbmode"], d["iomode"] = r[0], r[1] if len(r) > 1 else ""
-    bct = []
-    for _ in range(6):
+    # src/readwrite.F90:561-575: bctype per face, 41 followed by twall,
+    # 411 by xslip and twall, 421 by xslip
+    bct, twall = [], [1.0] * 6
+    for n in range(6):
         r = nxt()
-        bct.append(int(r[0]))  # twall, xslip, ... of some types follow
+        bct.append(int(r[0]))
+        if bct[-1] == 41 and len(r) > 1:
+            twall[n] = _real(r[1])
+        elif bct[-1] == 411 and len(r) > 2:
+            d["xslip"], twall[n] = _real(r[1]), _real(r[2])
+        elif bct[-1] == 421 and len(r) > 1:
+            d["xslip"] = _real(r[1])
     d["bctype"] = bct
+    d["twall"] = twall
     d["ninit"] = int(nxt()[0])
     r = nxt()
     d["spg"] = [int(v) for v in r[:6]]
@@ -167,6 +176,9 @@ CONVECTION = {"643c": 0, "543e": 1, "543c": 2}
 GRADIENT = {"643c": 0, "643e": 1}
 FLOWTYPES = ("tgv", "hit", "riem2d")
 BOX_LENGTHS = {"tgv": (2 * math.pi,) * 3, "hit": (2 * math.pi,) * 3, "riem2d": (1.0, 1.0, 1.0)}
+# the conditions of src/bc.F90 the library applies (gcnscbc, zeroextrap,
+# symmetry, the noslip walls, outflow, fixbc)
+BUILT_BCTYPES = {23, 50, 60, 41, 42, 21, 31}
 
 
 def params_from_deck(d):
@@ -196,10 +208,11 @@ def params_from_deck(d):
     if two_d:
         npdc[2] = 3
     for n, b in enumerate(d["bctype"]):
-        if b == 1:
+        if b == 1 or (b == 0 and d["flowtype"] == "riem2d"):
             continue
-        need(b == 0 and d["flowtype"] == "riem2d",
-             f"bctype {b} on face {n + 1} is not built (1, and 0 for riem2d)")
+        need(b in BUILT_BCTYPES,
+             f"bctype {b} on face {n + 1} is not built ({sorted(BUILT_BCTYPES)}, 1, and 0 "
+             f"for riem2d)")
     return ag.default_params(dims, npdci=npdc[0], npdcj=npdc[1], npdck=npdc[2],
                              convection=CONVECTION[d["conschm"]],
                              deriv_explicit=GRADIENT[d["difschm"]],
@@ -290,8 +303,11 @@ def run_deck(casedir, out=None, maxstep=None, seed=1, device=0, log=sys.stdout, 
     outdat = os.path.join(out or casedir, "outdat")
     try:
         s.upload_geom(*ag.box_metrics(p, lengths=BOX_LENGTHS[ft]))
-        if ft == "riem2d":
-            s.set_bc(tuple(d["bctype"]), ag.UDF_RIEMANN2D)
+        if any(b != 1 for b in d["bctype"]):
+            lx, ly, lz = BOX_LENGTHS[ft]
+            s.set_lengths(lx, ly, lz if p.km > 0 else 0.0)
+            s.set_twall(d["twall"])
+            s.set_bc(tuple(d["bctype"]), ag.UDF_RIEMANN2D if ft == "riem2d" else 0)
         nstep0 = 0
         if d["lrestart"]:
             nstep0 = read_checkpoint(s, outdat)

@@ -53,7 +53,7 @@ def test_riemann_deck():
 
 def test_unbuilt_deck_settings_are_rejected(tmp_path):
     src = os.path.join(DECKS, "tgv", "datin", "input.tgv")
-    for old, new, what in (("\n1\n1\n1 \n1\n1\n1\n", "\n41, 1.0\n1\n1 \n1\n1\n1\n", "bctype"),
+    for old, new, what in (("\n1\n1\n1 \n1\n1\n1\n", "\n411, 0.5, 1.0\n1\n1 \n1\n1\n1\n", "bctype"),
                            ("643c, 643c, rk3", "643c, 643c, rk4", "rk3"),
                            ("\ntgv\n", "\nchannel\n", "flow type")):
         text = open(src).read()
@@ -63,6 +63,23 @@ def test_unbuilt_deck_settings_are_rejected(tmp_path):
         (case / "datin" / "input.tgv").write_text(text.replace(old, new))
         with pytest.raises(deck.DeckError, match=what):
             deck.params_from_deck(deck.read_input(deck.find_input(str(case))))
+
+
+def test_deck_walls_and_their_temperatures(tmp_path):
+    """readinput's bc lines (src/readwrite.F90:561-575): bctype 41 carries its
+    face's wall temperature; the built conditions pass params_from_deck."""
+    src = os.path.join(DECKS, "tgv", "datin", "input.tgv")
+    text = open(src).read()
+    old = "\n1\n1\n1 \n1\n1\n1\n"
+    assert old in text
+    case = tmp_path / "walls"
+    (case / "datin").mkdir(parents=True)
+    (case / "datin" / "input.tgv").write_text(
+        text.replace(old, "\n41, 1.25\n42\n50 \n60\n1\n1\n"))
+    d = deck.read_input(deck.find_input(str(case)))
+    assert d["bctype"] == [41, 42, 50, 60, 1, 1]
+    assert d["twall"][0] == 1.25 and d["twall"][1:] == [1.0] * 5
+    deck.params_from_deck(d)
 
 
 def test_fortran_e_known_strings():
