@@ -2749,7 +2749,14 @@ struct BcArgs {
 };
 
 // the boundary conditions of k_bc_extrap
-enum { BCX_ZEROEXTRAP = 0, BCX_SYMMETRY = 1, BCX_NOSLIP = 2, BCX_ADIABATIC = 3, BCX_FIX = 4 };
+enum {
+    BCX_ZEROEXTRAP = 0,
+    BCX_SYMMETRY = 1,
+    BCX_NOSLIP = 2,
+    BCX_ADIABATIC = 3,
+    BCX_FIX = 4,
+    BCX_SLIPADIAB = 5
+};
 
 // One boundary node as udf_bc sets it: the primitives, tmp = thermal(
 // pressure, density) = p/rho*const2 and q from fvar2q_sca's pressure form,
@@ -2980,6 +2987,9 @@ __global__ void k_bc_gcnscbc(BcArgs a)
 //   temperature form rho*(T*const1 + |u|^2/2) on faces 1, 2, 4 and 6, its
 //   pressure form on faces 3 and 5 (src/fludyna.F90:312-376).
 // In the reference's operation order, like the rest of this region.
+//   slipadibwall (MODE 5, src/bc.F90:7114-7313, faces 3 and 4): pressure,
+//   temperature and u extrapolated, w = 0, v = 0 on face 3 and extrapolated
+//   on face 4, the density from p/T; q by the pressure form.
 //   fixbc (MODE 4, src/bc.F90:7562-7648): the fixed states of faces 3
 //   (rho 2, p 1) and 4 (rho 1, p 2.5) at rest, T = p/rho*const2.
 //   walls (MODE 2 noslip, src/bc.F90:6281-6595; MODE 3 noslip_adibatic,
@@ -2990,8 +3000,9 @@ template <int AX, bool HI, int MODE>
 __global__ void k_bc_extrap(BcArgs a)
 {
     constexpr bool SYM = MODE == BCX_SYMMETRY, WALL = MODE == BCX_NOSLIP || MODE == BCX_ADIABATIC;
+    constexpr bool SLIP = MODE == BCX_SLIPADIAB;
     constexpr int OA = AX == 0 ? 1 : 0, OB = AX == 2 ? 1 : 2;
-    constexpr bool RHO_EXT = WALL ? false : (SYM ? (AX == 2 && HI) : AX != 0);
+    constexpr bool RHO_EXT = (WALL || SLIP) ? false : (SYM ? (AX == 2 && HI) : AX != 0);
     constexpr bool TFORM = SYM && (AX == 0 || HI);
     const long ea = a.g.n[OA] + 1L;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -3011,6 +3022,12 @@ __global__ void k_bc_extrap(BcArgs a)
     for (int m = 0; m < 3; ++m)
         ve[m] = WALL ? 0.0 : n1d3 * (4.0 * a.prim[1 + m][g1] - a.prim[1 + m][g2] - 2.0 * 0.0);
     if (SYM) ve[AX] = 0.0;
+    if (SLIP) {
+        // slipadibwall: u extrapolated, w = 0; v = 0 on face 3, extrapolated
+        // on face 4
+        if (!HI) ve[1] = 0.0;
+        ve[2] = 0.0;
+    }
     const double pe = n1d3 * (4.0 * a.prim[4][g1] - a.prim[4][g2] - 2.0 * 0.0);
     double re, te;
     if (MODE == BCX_FIX) {
@@ -3835,7 +3852,7 @@ int do_boucon(astr_ctx* c)
     for (int f = 0; f < 6; ++f)
         any = any || c->bctype[f] == 50 || c->bctype[f] == 60 || c->bctype[f] == 23 ||
               c->bctype[f] == 41 || c->bctype[f] == 42 || c->bctype[f] == 21 ||
-              c->bctype[f] == 31;
+              c->bctype[f] == 31 || c->bctype[f] == 421;
     if (!any) return ASTR_OK;
     BcArgs a{};
     a.g = c->g;
@@ -3873,6 +3890,7 @@ int do_boucon(astr_ctx* c)
         if (c->bctype[f] == 60) TRY(launch_extrap<BCX_SYMMETRY>(c, a, f));
         if (c->bctype[f] == 41) TRY(launch_extrap<BCX_NOSLIP>(c, a, f));
         if (c->bctype[f] == 31 && (f == 2 || f == 3)) TRY(launch_extrap<BCX_FIX>(c, a, f));
+        if (c->bctype[f] == 421) TRY(launch_extrap<BCX_SLIPADIAB>(c, a, f));
         if (c->bctype[f] == 21 && f == 1) {
             hipLaunchKernelGGL(k_bc_outflow<0>, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0,
                                c->stream, a);
@@ -4290,7 +4308,7 @@ int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed, bool alt = f
     for (int f = 0; f < 6; ++f)
         bc_reads_prim = bc_reads_prim || c->bctype[f] == 23 || c->bctype[f] == 50 ||
                         c->bctype[f] == 60 || c->bctype[f] == 41 || c->bctype[f] == 42 ||
-                        c->bctype[f] == 21 || c->bctype[f] == 31;
+                        c->bctype[f] == 21 || c->bctype[f] == 31 || c->bctype[f] == 421;
     if (c->p.variant == 1) {
         // rk3 of tgvsolver.F90:994-1082: bchomo, rhscal (gradcal, convection,
         // diffusion by df/dx), stage-1 statistics and qsave, the RK3 combine
@@ -5433,7 +5451,7 @@ static bool has_face_bc(const astr_ctx* c)
     for (int f = 0; f < 6; ++f)
         if ((c->bctype[f] == 23 && (f == 1 || f == 3)) || c->bctype[f] == 50 ||
             c->bctype[f] == 60 || c->bctype[f] == 41 || c->bctype[f] == 42 || c->bctype[f] == 21 ||
-            c->bctype[f] == 31)
+            c->bctype[f] == 31 || c->bctype[f] == 421)
             return true;
     return false;
 }
@@ -6023,18 +6041,21 @@ int astr_set_bc(astr_ctx* c, const int32_t* bctype, int32_t udf)
     for (int f = 0; f < 6; ++f) {
         const bool wall = bctype[f] == 41 || bctype[f] == 42;
         if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50 &&
-            bctype[f] != 60 && bctype[f] != 21 && bctype[f] != 31 && !wall)
+            bctype[f] != 60 && bctype[f] != 21 && bctype[f] != 31 && bctype[f] != 421 && !wall)
             return fail(ASTR_E_UNSUPPORTED,
                         "bctype 1 (nothing done), 0 (user condition), 23 (gcnscbc), 50 "
-                        "(zeroextrap), 60 (symmetry), 41 and 42 (noslip walls), 21 (outflow), 31 "
-                        "(fixbc) are built; the inflow, far-field, slip-wall and other NSCBC "
-                        "conditions of src/bc.F90 are not");
+                        "(zeroextrap), 60 (symmetry), 41 and 42 (noslip walls), 421 (adiabatic "
+                        "slip wall), 21 (outflow), 31 (fixbc) are built; the inflow, far-field, "
+                        "isothermal slip-wall and other NSCBC conditions of src/bc.F90 are not");
         if (wall && f >= 4)
             return fail(ASTR_E_UNSUPPORTED, "noslip walls act on faces 1-4 (src/bc.F90:6281-6919)");
         if (bctype[f] == 21 && f != 1 && f != 3)
             return fail(ASTR_E_UNSUPPORTED, "outflow acts on faces 2 and 4 (src/bc.F90:3379-3592)");
+        if (bctype[f] == 421 && f != 2 && f != 3)
+            return fail(ASTR_E_UNSUPPORTED,
+                        "the adiabatic slip wall acts on faces 3 and 4 (src/bc.F90:7114-7313)");
         const bool ex = bctype[f] == 50 || bctype[f] == 60 || wall || bctype[f] == 21 ||
-                        bctype[f] == 31;
+                        bctype[f] == 31 || bctype[f] == 421;
         if (ex && f >= 4 && c->g.n[2] == 0)
             return fail(ASTR_E_UNSUPPORTED,
                         "zeroextrap and symmetry on the k faces need a three-dimensional box");

@@ -178,7 +178,7 @@ FLOWTYPES = ("tgv", "hit", "riem2d")
 BOX_LENGTHS = {"tgv": (2 * math.pi,) * 3, "hit": (2 * math.pi,) * 3, "riem2d": (1.0, 1.0, 1.0)}
 # the conditions of src/bc.F90 the library applies (gcnscbc, zeroextrap,
 # symmetry, the noslip walls, outflow, fixbc)
-BUILT_BCTYPES = {23, 50, 60, 41, 42, 21, 31}
+BUILT_BCTYPES = {23, 50, 60, 41, 42, 21, 31, 421}
 
 
 def params_from_deck(d):

@@ -196,7 +196,8 @@ class AstrSolver:
         example), 23 gcnscbc (acting on faces 2 and 4), 50 zeroextrap and 60
         symmetry (any face), 41 and 42 noslip walls (isothermal at set_twall's
         temperature, adiabatic; faces 1-4), 21 outflow (faces 2 and 4), 31
-        fixbc; none of these with a user condition."""
+        fixbc, 421 the adiabatic slip wall (faces 3 and 4); none of these
+        with a user condition."""
         arr = (ctypes.c_int32 * 6)(*[int(b) for b in bctype])
         check(self.lib.astr_set_bc(self.ctx, arr, int(udf)), "astr_set_bc")
 

@@ -353,10 +353,13 @@ int astr_set_spacing(astr_ctx *ctx, double dx, double dy, double dz);
  * extrapolates and, below the sound speed, relaxes the pressure towards
  * pinf at the rate 0.25 dt (dt of the latest step or astr_rk3_update).
  * 31 is fixbc (src/bc.F90:7562-7648): faces 3 and 4 held at its two fixed
- * states, the other faces left alone.
+ * states, the other faces left alone.  421 is slipadibwall
+ * (src/bc.F90:7114-7313) on faces 3 and 4: pressure, temperature and u
+ * extrapolated, w = 0, v = 0 on face 3 and extrapolated on face 4, the
+ * density from p/T (its unused wall blowing and k-omega branches aside).
  * None of these is combined with a user condition; batched members and the
  * adjoint take none.  Faces act in the reference's order 1..6.  The other bctypes of
- * bc.F90 (inflow, outflow, walls, the other NSCBCs) return
+ * bc.F90 (inflow, far field, the isothermal slip wall, the other NSCBCs) return
  * ASTR_E_UNSUPPORTED.  Default: all faces 1. */
 #define ASTR_UDF_RIEMANN2D 1
 int astr_set_bc(astr_ctx *ctx, const int32_t *bctype, int32_t udf);

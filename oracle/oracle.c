@@ -568,7 +568,8 @@ int orc_box_set_bc(orc_box *b, const int *bctype)
     for (int f = 0; f < 6; ++f)
         if (bctype[f] != 0 && bctype[f] != 23 && bctype[f] != 50 && bctype[f] != 60 &&
             !((bctype[f] == 41 || bctype[f] == 42) && f < 4) &&
-            !(bctype[f] == 21 && (f == 1 || f == 3)) && bctype[f] != 31)
+            !(bctype[f] == 21 && (f == 1 || f == 3)) && bctype[f] != 31 &&
+            !(bctype[f] == 421 && (f == 2 || f == 3)))
             return -1;
     for (int f = 0; f < 6; ++f) b->bc[f] = bctype[f];
     return 0;
@@ -610,6 +611,7 @@ static void box_boucon(orc_box *b, double dt)
             orc_wall(&u, f + 1, b->bc[f] == 41, q, b->rho, vel, b->prs, b->tmp);
         else if (b->bc[f] == 21) orc_outflow(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
         else if (b->bc[f] == 31) orc_fixbc(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
+        else if (b->bc[f] == 421) orc_slipadib(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
     }
     for (int m = 0; m < 5; ++m) memcpy(b->q[m], q + m * n, n * sizeof(double));
     for (int c = 0; c < 3; ++c) memcpy(b->vel[c], vel + c * n, n * sizeof(double));

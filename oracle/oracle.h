@@ -200,6 +200,9 @@ void orc_symmetry(const orc_uwd *u, int face, double *q, double *rho, double *ve
  * step's time step) */
 void orc_outflow(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
                  double *tmp);
+/* slipadibwall (bctype 421) on face 3 or 4 (src/bc.F90:7114-7313) */
+void orc_slipadib(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+                  double *tmp);
 /* fixbc (bctype 31): faces 3 and 4 held at fixed states (src/bc.F90:7562-7648) */
 void orc_fixbc(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
                double *tmp);

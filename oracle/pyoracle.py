@@ -358,6 +358,8 @@ def uwd_lib():
         lib.orc_outflow.argtypes = [P, _I, _D, _D, _D, _D, _D]
         lib.orc_fixbc.restype = None
         lib.orc_fixbc.argtypes = [P, _I, _D, _D, _D, _D, _D]
+        lib.orc_slipadib.restype = None
+        lib.orc_slipadib.argtypes = [P, _I, _D, _D, _D, _D, _D]
         lib.orc_uwd_rk3_step.restype = _I
         lib.orc_uwd_rk3_step.argtypes = [P, _I, _I, I6, _I, _F, _F, _D, _D, _D, _D, _D, _D, _D]
         _uwd_declared = True
@@ -530,6 +532,14 @@ class UwdBox:
         self.cfg.deltat = float(dt)
         lib.orc_outflow(ctypes.byref(self.cfg), int(face), _p(o["q"]), _p(o["rho"]),
                         _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
+        return o
+
+    def slipadib(self, st, face):
+        """A copy of st after slipadibwall (bctype 421) on face 3 or 4."""
+        lib = uwd_lib()
+        o = self._state(st)
+        lib.orc_slipadib(ctypes.byref(self.cfg), int(face), _p(o["q"]), _p(o["rho"]),
+                         _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
         return o
 
     def fixbc(self, st, face):

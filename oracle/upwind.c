@@ -1557,6 +1557,42 @@ void orc_outflow(const orc_uwd *u, int face, double *q, double *rho, double *vel
         }
 }
 
+/* slipadibwall (src/bc.F90:7114-7313), bctype 421, on face 3 or 4 (the
+ * faces it acts on): pressure, temperature and u by num1d3*(4 f1 - f2), w
+ * zero, v zero on face 3 and extrapolated alike on face 4; the density from
+ * thermal(p, T) = p/T*const2 and q by fvar2q's pressure form. */
+void orc_slipadib(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+                  double *tmp)
+{
+    if (face != 3 && face != 4) return;
+    const shape_t s = shape_of(u);
+    const double const2 = u->gamma * (u->mach * u->mach), const6 = 1.0 / (u->gamma - 1.0);
+    const double n1d3 = 1.0 / 3.0;
+    const int hi = face == 4;
+    for (int k = 0; k <= u->km; ++k)
+        for (int i = 0; i <= u->im; ++i) {
+            const long h = hidx(&s, i, hi ? u->jm : 0, k);
+            const long h1 = hidx(&s, i, hi ? u->jm - 1 : 1, k), h2 = hidx(&s, i, hi ? u->jm - 2 : 2, k);
+            const double pe = n1d3 * (4.0 * prs[h1] - prs[h2]);
+            const double te = n1d3 * (4.0 * tmp[h1] - tmp[h2]);
+            double ve[3];
+            ve[0] = n1d3 * (4.0 * vel[h1] - vel[h2]);
+            ve[1] = hi ? n1d3 * (4.0 * vel[s.nf + h1] - vel[s.nf + h2]) : 0.0;
+            ve[2] = 0.0;
+            const double re = pe / te * const2;
+            prs[h] = pe;
+            tmp[h] = te;
+            rho[h] = re;
+            for (int m = 0; m < 3; ++m) vel[m * s.nf + h] = ve[m];
+            q[h] = re;
+            q[s.nf + h] = re * ve[0];
+            q[2 * s.nf + h] = re * ve[1];
+            q[3 * s.nf + h] = re * ve[2];
+            const double var1 = 0.5 * (ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
+            q[4 * s.nf + h] = pe * const6 + re * var1;
+        }
+}
+
 /* fixbc (src/bc.F90:7562-7648), bctype 31: face 3 held at rho 2, p 1 and
  * face 4 at rho 1, p 2.5, both at rest, T = thermal(rho, p) = p/rho*const2,
  * q by fvar2q's pressure form; the other faces are left alone. */
@@ -1649,6 +1685,8 @@ void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, doubl
             orc_outflow(u, f + 1, q, rho, vel, prs, tmp);
         } else if (bctype[f] == 31) {
             orc_fixbc(u, f + 1, q, rho, vel, prs, tmp);
+        } else if (bctype[f] == 421) {
+            orc_slipadib(u, f + 1, q, rho, vel, prs, tmp);
         }
     }
 }
@@ -1728,7 +1766,7 @@ int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *
     for (int f = 0; f < 6; ++f) {
         if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50 &&
             bctype[f] != 60 && bctype[f] != 41 && bctype[f] != 42 && bctype[f] != 21 &&
-            bctype[f] != 31)
+            bctype[f] != 31 && bctype[f] != 421)
             return 3;
         anyudf |= bctype[f] == 0;
         anybc |= bctype[f] != 1;

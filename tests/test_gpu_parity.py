@@ -171,7 +171,8 @@ def test_gcnscbc_central_steps_match_checker(v):
 
 @pytest.mark.parametrize("faces", [(50, 50, 50, 50, 0, 0), (60, 60, 60, 60, 0, 0),
                                    (60, 50, 50, 60, 0, 0), (41, 41, 42, 42, 0, 0),
-                                   (50, 21, 50, 21, 0, 0), (50, 50, 31, 31, 0, 0)])
+                                   (50, 21, 50, 21, 0, 0), (50, 50, 31, 31, 0, 0),
+                                   (50, 21, 421, 421, 0, 0)])
 def test_zeroextrap_central_steps_match_checker(faces):
     """zeroextrap (bctype 50), symmetry (60) and the noslip walls (41, 42) on
     the four faces of the bounded x and y axes of a three-dimensional box

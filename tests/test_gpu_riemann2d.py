@@ -243,16 +243,20 @@ def _boucon_ref(box, st, faces):
             o = box.outflow(o, f + 1, 0.0)
         elif b == 31:
             o = box.fixbc(o, f + 1)
+        elif b == 421:
+            o = box.slipadib(o, f + 1)
     return o
 
 
 @pytest.mark.parametrize("faces", [(50, 50, 50, 50, 1, 1), (50, 23, 50, 23, 1, 1),
                                    (60, 60, 60, 60, 1, 1), (60, 23, 50, 60, 1, 1),
                                    (41, 42, 41, 42, 1, 1), (50, 41, 60, 42, 1, 1),
-                                   (50, 21, 50, 21, 1, 1), (50, 50, 31, 31, 1, 1)])
+                                   (50, 21, 50, 21, 1, 1), (50, 50, 31, 31, 1, 1),
+                                   (50, 21, 421, 421, 1, 1)])
 def test_zeroextrap_steps_match_checker(faces):
     """zeroextrap (bctype 50), symmetry (60), the noslip walls (41 at its
-    face's wall temperature, 42) and outflow (21, faces 2 and 4) on the faces
+    face's wall temperature, 42), the adiabatic slip wall (421, faces 3 and
+    4), fixbc (31) and outflow (21, faces 2 and 4) on the faces
     of a 2-D box, alone, mixed and with gcnscbc on faces 2 and 4 (the
     reference's face order 1..6), through boucon and whole upwind steps
     against the checker (boucon alone runs with the time step of no step, 0)."""
@@ -294,6 +298,8 @@ def test_zeroextrap_rejected_where_not_built():
         s.set_bc((1, 1, 1, 1, 50, 1), 0)  # a k face of a 2-D box
     with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
         s.set_bc((1, 1, 1, 1, 41, 1), 0)  # walls act on faces 1-4
+    with pytest.raises(ag.AstrError, match="ASTR_E_UNSUPPORTED"):
+        s.set_bc((421, 1, 1, 1, 1, 1), 0)  # the slip wall acts on faces 3 and 4
     with pytest.raises(ag.AstrError, match="ASTR_E_ARG"):
         s.set_twall((1.0, 0.0, 1.0, 1.0, 1.0, 1.0))
     s.close()

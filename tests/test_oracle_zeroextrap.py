@@ -254,3 +254,56 @@ def test_fixbc_holds_its_two_states():
         o = box.fixbc(st, face)
         for nm in ("q", "rho", "vel", "prs", "tmp"):
             assert np.array_equal(o[nm], st[nm])
+
+
+@pytest.mark.parametrize("face", [3, 4])
+def test_slipadib_wall_keeps_the_tangential_velocity(face):
+    """slipadibwall (421): pressure, temperature and u by the zero-slope
+    parabola's face value, w zero, v zero on face 3 and extrapolated alike on
+    face 4, the density from the equation of state (not extrapolated) and
+    the energy by the pressure form; faces 1 and 2 left alone."""
+    dims = (12, 10, 0)
+    box = po.UwdBox(dims, npdc=(4, 4, 3), gamma=GAMMA, mach=MACH)
+    c = (1.1, 0.3, -0.2, 0.05, 1.0 / (GAMMA * MACH * MACH))
+    d = (0.01, 0.02, -0.015, 0.005, 0.03)
+    st, _ = parabolic_state(dims, face, c, d)
+    o = box.slipadib(st, face)
+    hi = face == 4
+    I = slice(HM, HM + dims[0] + 1)
+    j0 = HM + (dims[1] if hi else 0)
+    j1, j2 = (j0 - 1, j0 - 2) if hi else (j0 + 1, j0 + 2)
+    const2 = GAMMA * MACH * MACH
+    u = [o["vel"][I, j0, HM, m] for m in range(3)]
+    p, rho, t = o["prs"][I, j0, HM], o["rho"][I, j0, HM], o["tmp"][I, j0, HM]
+    assert np.max(np.abs(u[0] - c[1])) <= 1e-14
+    if hi:
+        assert np.max(np.abs(u[1] - c[2])) <= 1e-14
+    else:
+        assert np.all(u[1] == 0.0)
+    assert np.all(u[2] == 0.0)
+    assert np.max(np.abs(p - c[4])) <= 1e-14 * c[4]
+    t1, t2 = st["tmp"][I, j1, HM], st["tmp"][I, j2, HM]
+    assert np.max(np.abs(t - (4.0 * t1 - t2) / 3.0)) <= 1e-14
+    assert np.max(np.abs(rho - p / t * const2)) <= 1e-14
+    # the parabolic density is not what the wall takes
+    assert np.min(np.abs(rho - c[0])) > 1e-6
+    q = o["q"][I, j0, HM]
+    assert np.array_equal(q[..., 0], rho)
+    for m in range(3):
+        assert np.array_equal(q[..., 1 + m], rho * u[m])
+    ke = 0.5 * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2)
+    assert np.max(np.abs(q[..., 4] - (p / (GAMMA - 1.0) + rho * ke))) <= 1e-14 * np.max(p)
+    for f in (1, 2):
+        o = box.slipadib(st, f)
+        for nm in ("q", "rho", "vel", "prs", "tmp"):
+            assert np.array_equal(o[nm], st[nm])
+
+
+def test_slipadib_wall_only_on_faces_3_and_4():
+    ob = po.OracleBox((16, 12, 10), variant=0, npdc=(4, 4, 3))
+    ob.set_bc((0, 0, 421, 421, 0, 0))
+    for f in (0, 1, 4, 5):
+        bc = [0] * 6
+        bc[f] = 421
+        with pytest.raises(ValueError):
+            ob.set_bc(tuple(bc))
