@@ -67,6 +67,7 @@ SIGNATURES = {
     "astr_set_bc": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
     "astr_set_lengths": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "astr_set_twall": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "astr_set_xslip": (ctypes.c_int, [_P, ctypes.c_double]),
     "astr_line_adjoint": (ctypes.c_int, [_P, _I, _I, _I, _D, _D]),
     "astr_adjoint_enstrophy": (ctypes.c_int, [_P, _I, ctypes.c_double, _I, _D, _D, _D, _D]),
     "astr_adjoint_record": (ctypes.c_int, [_P, _I, ctypes.c_double, _I, _D]),

@@ -2746,6 +2746,7 @@ struct BcArgs {
     double len[2];  // gridcube lengths of x and y (x(i) = len/n*i), for gcnscbc
     double tw[6];   // wall temperature of each face (noslip)
     double dt;      // the step's time step (outflow)
+    double xslip;   // slip-nonslip point of the isothermal slip wall
 };
 
 // the boundary conditions of k_bc_extrap
@@ -2755,7 +2756,8 @@ enum {
     BCX_NOSLIP = 2,
     BCX_ADIABATIC = 3,
     BCX_FIX = 4,
-    BCX_SLIPADIAB = 5
+    BCX_SLIPADIAB = 5,
+    BCX_SLIPISOT = 6
 };
 
 // One boundary node as udf_bc sets it: the primitives, tmp = thermal(
@@ -2990,6 +2992,9 @@ __global__ void k_bc_gcnscbc(BcArgs a)
 //   slipadibwall (MODE 5, src/bc.F90:7114-7313, faces 3 and 4): pressure,
 //   temperature and u extrapolated, w = 0, v = 0 on face 3 and extrapolated
 //   on face 4, the density from p/T; q by the pressure form.
+//   slipisotwall (MODE 6, src/bc.F90:6921-7112) on face 3: the slip wall's
+//   node where x <= xslip, else at rest at the face's wall temperature; its
+//   face 4 is the noslip wall's (MODE 2).
 //   fixbc (MODE 4, src/bc.F90:7562-7648): the fixed states of faces 3
 //   (rho 2, p 1) and 4 (rho 1, p 2.5) at rest, T = p/rho*const2.
 //   walls (MODE 2 noslip, src/bc.F90:6281-6595; MODE 3 noslip_adibatic,
@@ -3000,9 +3005,9 @@ template <int AX, bool HI, int MODE>
 __global__ void k_bc_extrap(BcArgs a)
 {
     constexpr bool SYM = MODE == BCX_SYMMETRY, WALL = MODE == BCX_NOSLIP || MODE == BCX_ADIABATIC;
-    constexpr bool SLIP = MODE == BCX_SLIPADIAB;
+    constexpr bool SLIP = MODE == BCX_SLIPADIAB, SLIPT = MODE == BCX_SLIPISOT;
     constexpr int OA = AX == 0 ? 1 : 0, OB = AX == 2 ? 1 : 2;
-    constexpr bool RHO_EXT = (WALL || SLIP) ? false : (SYM ? (AX == 2 && HI) : AX != 0);
+    constexpr bool RHO_EXT = (WALL || SLIP || SLIPT) ? false : (SYM ? (AX == 2 && HI) : AX != 0);
     constexpr bool TFORM = SYM && (AX == 0 || HI);
     const long ea = a.g.n[OA] + 1L;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -3022,11 +3027,19 @@ __global__ void k_bc_extrap(BcArgs a)
     for (int m = 0; m < 3; ++m)
         ve[m] = WALL ? 0.0 : n1d3 * (4.0 * a.prim[1 + m][g1] - a.prim[1 + m][g2] - 2.0 * 0.0);
     if (SYM) ve[AX] = 0.0;
-    if (SLIP) {
+    if (SLIP || SLIPT) {
         // slipadibwall: u extrapolated, w = 0; v = 0 on face 3, extrapolated
-        // on face 4
+        // on face 4.  slipisotwall (face 3 only): the same where the node's
+        // gridcube x = lx/im*i is at most xslip, else at rest (the unread wall
+        // blowing zero) at the face's wall temperature
         if (!HI) ve[1] = 0.0;
         ve[2] = 0.0;
+    }
+    bool stick = false;
+    if (SLIPT) {
+        const double xn = a.len[0] / (double)a.g.n[0] * (double)ca;
+        stick = !(xn <= a.xslip);
+        if (stick) ve[0] = 0.0;
     }
     const double pe = n1d3 * (4.0 * a.prim[4][g1] - a.prim[4][g2] - 2.0 * 0.0);
     double re, te;
@@ -3047,7 +3060,7 @@ __global__ void k_bc_extrap(BcArgs a)
         const double var0 = 0.5 * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
         a.q[4][gi] = pf * a.const6 + re * var0;
         return;
-    } else if (MODE == BCX_NOSLIP) {
+    } else if (MODE == BCX_NOSLIP || stick) {
         te = a.tw[2 * AX + (HI ? 1 : 0)];
         re = pe / te * a.const2;
     } else if (RHO_EXT) {
@@ -3283,6 +3296,7 @@ struct astr_ctx {
     double len[3] = {1.0, 1.0, 0.0};  // gridcube lengths (astr_set_lengths)
     double twall[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0};  // wall temperatures (astr_set_twall)
     double bc_dt = 0.0;  // time step of the latest whole step, which outflow reads
+    double xslip = 0.0;  // slip-nonslip point (astr_set_xslip)
     bool curv = false;                // general metrics uploaded (dxf)
     bool juni = false;                // jacob uniform (j0): read from no array
     double j0 = 0.0;
@@ -3852,7 +3866,7 @@ int do_boucon(astr_ctx* c)
     for (int f = 0; f < 6; ++f)
         any = any || c->bctype[f] == 50 || c->bctype[f] == 60 || c->bctype[f] == 23 ||
               c->bctype[f] == 41 || c->bctype[f] == 42 || c->bctype[f] == 21 ||
-              c->bctype[f] == 31 || c->bctype[f] == 421;
+              c->bctype[f] == 31 || c->bctype[f] == 421 || c->bctype[f] == 411;
     if (!any) return ASTR_OK;
     BcArgs a{};
     a.g = c->g;
@@ -3867,6 +3881,7 @@ int do_boucon(astr_ctx* c)
     a.const6 = c->const6;
     a.len[0] = c->len[0];
     a.len[1] = c->len[1];
+    a.xslip = c->xslip;
     for (int f = 0; f < 6; ++f) a.tw[f] = c->twall[f];
     a.dt = c->bc_dt;
     // boucon's face order (src/bc.F90:338-404), face by face 1..6: the user
@@ -3891,6 +3906,8 @@ int do_boucon(astr_ctx* c)
         if (c->bctype[f] == 41) TRY(launch_extrap<BCX_NOSLIP>(c, a, f));
         if (c->bctype[f] == 31 && (f == 2 || f == 3)) TRY(launch_extrap<BCX_FIX>(c, a, f));
         if (c->bctype[f] == 421) TRY(launch_extrap<BCX_SLIPADIAB>(c, a, f));
+        if (c->bctype[f] == 411 && f == 2) TRY(launch_extrap<BCX_SLIPISOT>(c, a, f));
+        if (c->bctype[f] == 411 && f == 3) TRY(launch_extrap<BCX_NOSLIP>(c, a, f));
         if (c->bctype[f] == 21 && f == 1) {
             hipLaunchKernelGGL(k_bc_outflow<0>, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0,
                                c->stream, a);
@@ -4308,7 +4325,8 @@ int do_step(astr_ctx* c, double dt, double* stats_slot, bool timed, bool alt = f
     for (int f = 0; f < 6; ++f)
         bc_reads_prim = bc_reads_prim || c->bctype[f] == 23 || c->bctype[f] == 50 ||
                         c->bctype[f] == 60 || c->bctype[f] == 41 || c->bctype[f] == 42 ||
-                        c->bctype[f] == 21 || c->bctype[f] == 31 || c->bctype[f] == 421;
+                        c->bctype[f] == 21 || c->bctype[f] == 31 || c->bctype[f] == 421 ||
+                        c->bctype[f] == 411;
     if (c->p.variant == 1) {
         // rk3 of tgvsolver.F90:994-1082: bchomo, rhscal (gradcal, convection,
         // diffusion by df/dx), stage-1 statistics and qsave, the RK3 combine
@@ -5451,7 +5469,7 @@ static bool has_face_bc(const astr_ctx* c)
     for (int f = 0; f < 6; ++f)
         if ((c->bctype[f] == 23 && (f == 1 || f == 3)) || c->bctype[f] == 50 ||
             c->bctype[f] == 60 || c->bctype[f] == 41 || c->bctype[f] == 42 || c->bctype[f] == 21 ||
-            c->bctype[f] == 31 || c->bctype[f] == 421)
+            c->bctype[f] == 31 || c->bctype[f] == 421 || c->bctype[f] == 411)
             return true;
     return false;
 }
@@ -6041,21 +6059,22 @@ int astr_set_bc(astr_ctx* c, const int32_t* bctype, int32_t udf)
     for (int f = 0; f < 6; ++f) {
         const bool wall = bctype[f] == 41 || bctype[f] == 42;
         if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50 &&
-            bctype[f] != 60 && bctype[f] != 21 && bctype[f] != 31 && bctype[f] != 421 && !wall)
+            bctype[f] != 60 && bctype[f] != 21 && bctype[f] != 31 && bctype[f] != 421 &&
+            bctype[f] != 411 && !wall)
             return fail(ASTR_E_UNSUPPORTED,
                         "bctype 1 (nothing done), 0 (user condition), 23 (gcnscbc), 50 "
-                        "(zeroextrap), 60 (symmetry), 41 and 42 (noslip walls), 421 (adiabatic "
-                        "slip wall), 21 (outflow), 31 (fixbc) are built; the inflow, far-field, "
-                        "isothermal slip-wall and other NSCBC conditions of src/bc.F90 are not");
+                        "(zeroextrap), 60 (symmetry), 41 and 42 (noslip walls), 411 and 421 "
+                        "(slip walls), 21 (outflow), 31 (fixbc) are built; the inflow, far-field "
+                        "and other NSCBC conditions of src/bc.F90 are not");
         if (wall && f >= 4)
             return fail(ASTR_E_UNSUPPORTED, "noslip walls act on faces 1-4 (src/bc.F90:6281-6919)");
         if (bctype[f] == 21 && f != 1 && f != 3)
             return fail(ASTR_E_UNSUPPORTED, "outflow acts on faces 2 and 4 (src/bc.F90:3379-3592)");
-        if (bctype[f] == 421 && f != 2 && f != 3)
+        if ((bctype[f] == 421 || bctype[f] == 411) && f != 2 && f != 3)
             return fail(ASTR_E_UNSUPPORTED,
-                        "the adiabatic slip wall acts on faces 3 and 4 (src/bc.F90:7114-7313)");
+                        "the slip walls act on faces 3 and 4 (src/bc.F90:6921-7313)");
         const bool ex = bctype[f] == 50 || bctype[f] == 60 || wall || bctype[f] == 21 ||
-                        bctype[f] == 31 || bctype[f] == 421;
+                        bctype[f] == 31 || bctype[f] == 421 || bctype[f] == 411;
         if (ex && f >= 4 && c->g.n[2] == 0)
             return fail(ASTR_E_UNSUPPORTED,
                         "zeroextrap and symmetry on the k faces need a three-dimensional box");
@@ -6082,6 +6101,14 @@ int astr_set_twall(astr_ctx* c, const double* twall)
         if (!(twall[f] > 0.0)) return fail(ASTR_E_ARG, "wall temperatures must be positive");
     for (int f = 0; f < 6; ++f) c->twall[f] = twall[f];
     ++c->gen;
+    return ASTR_OK;
+}
+
+int astr_set_xslip(astr_ctx* c, double xslip)
+{
+    if (!c) return fail(ASTR_E_ARG, "null context");
+    if (!std::isfinite(xslip)) return fail(ASTR_E_ARG, "xslip must be finite");
+    c->xslip = xslip;
     return ASTR_OK;
 }
 

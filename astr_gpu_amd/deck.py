@@ -178,7 +178,7 @@ FLOWTYPES = ("tgv", "hit", "riem2d")
 BOX_LENGTHS = {"tgv": (2 * math.pi,) * 3, "hit": (2 * math.pi,) * 3, "riem2d": (1.0, 1.0, 1.0)}
 # the conditions of src/bc.F90 the library applies (gcnscbc, zeroextrap,
 # symmetry, the noslip walls, outflow, fixbc)
-BUILT_BCTYPES = {23, 50, 60, 41, 42, 21, 31, 421}
+BUILT_BCTYPES = {23, 50, 60, 41, 42, 21, 31, 421, 411}
 
 
 def params_from_deck(d):
@@ -307,6 +307,8 @@ def run_deck(casedir, out=None, maxstep=None, seed=1, device=0, log=sys.stdout, 
             lx, ly, lz = BOX_LENGTHS[ft]
             s.set_lengths(lx, ly, lz if p.km > 0 else 0.0)
             s.set_twall(d["twall"])
+            if "xslip" in d:
+                s.set_xslip(d["xslip"])
             s.set_bc(tuple(d["bctype"]), ag.UDF_RIEMANN2D if ft == "riem2d" else 0)
         nstep0 = 0
         if d["lrestart"]:

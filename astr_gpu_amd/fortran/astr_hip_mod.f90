@@ -252,6 +252,13 @@ module astr_hip
       real(c_double), intent(in) :: twall(6)
     end function astr_set_twall
 
+    ! slip-nonslip point of bctype 411 (xslip of src/bc.F90:29)
+    integer(c_int) function astr_set_xslip(ctx, xslip) bind(c, name='astr_set_xslip')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: xslip
+    end function astr_set_xslip
+
     integer(c_int) function astr_boucon(ctx) bind(c, name='astr_boucon')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

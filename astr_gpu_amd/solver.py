@@ -196,8 +196,9 @@ class AstrSolver:
         example), 23 gcnscbc (acting on faces 2 and 4), 50 zeroextrap and 60
         symmetry (any face), 41 and 42 noslip walls (isothermal at set_twall's
         temperature, adiabatic; faces 1-4), 21 outflow (faces 2 and 4), 31
-        fixbc, 421 the adiabatic slip wall (faces 3 and 4); none of these
-        with a user condition."""
+        fixbc, 421 and 411 the adiabatic and isothermal slip walls (faces 3
+        and 4; 411 slips where x <= set_xslip's point); none of these with a
+        user condition."""
         arr = (ctypes.c_int32 * 6)(*[int(b) for b in bctype])
         check(self.lib.astr_set_bc(self.ctx, arr, int(udf)), "astr_set_bc")
 
@@ -205,6 +206,10 @@ class AstrSolver:
         """Wall temperature of each face for bctype 41 (noslip)."""
         arr = (ctypes.c_double * 6)(*[float(t) for t in twall])
         check(self.lib.astr_set_twall(self.ctx, arr), "astr_set_twall")
+
+    def set_xslip(self, xslip: float) -> None:
+        """The slip-nonslip point of bctype 411 (the isothermal slip wall)."""
+        check(self.lib.astr_set_xslip(self.ctx, float(xslip)), "astr_set_xslip")
 
     def set_lengths(self, lx: float, ly: float, lz: float = 0.0) -> None:
         """gridcube's box lengths (x(i) = lx/im*i), for the boundary

@@ -357,9 +357,12 @@ int astr_set_spacing(astr_ctx *ctx, double dx, double dy, double dz);
  * (src/bc.F90:7114-7313) on faces 3 and 4: pressure, temperature and u
  * extrapolated, w = 0, v = 0 on face 3 and extrapolated on face 4, the
  * density from p/T (its unused wall blowing and k-omega branches aside).
+ * 411 is slipisotwall (src/bc.F90:6921-7112): on face 3 the same where the
+ * node's gridcube x = lx/im*i is at most xslip (astr_set_xslip), else at
+ * rest at the face's wall temperature; on face 4 the isothermal noslip wall.
  * None of these is combined with a user condition; batched members and the
  * adjoint take none.  Faces act in the reference's order 1..6.  The other bctypes of
- * bc.F90 (inflow, far field, the isothermal slip wall, the other NSCBCs) return
+ * bc.F90 (inflow, far field, the other NSCBCs) return
  * ASTR_E_UNSUPPORTED.  Default: all faces 1. */
 #define ASTR_UDF_RIEMANN2D 1
 int astr_set_bc(astr_ctx *ctx, const int32_t *bctype, int32_t udf);
@@ -368,6 +371,10 @@ int astr_set_bc(astr_ctx *ctx, const int32_t *bctype, int32_t udf);
  * Default 1 on every face.  No counterpart call in the reference: the deck
  * holds it. */
 int astr_set_twall(astr_ctx *ctx, const double *twall6);
+/* The slip-nonslip point xslip of bctype 411 (src/bc.F90:29, read with the
+ * face's bctype by readinput, src/readwrite.F90:569).  Default 0.  No
+ * counterpart call in the reference: the deck holds it. */
+int astr_set_xslip(astr_ctx *ctx, double xslip);
 /* The box lengths of gridcube (src/gridgeneration.F90:233-267: x(i) =
  * lx/im*i, y(j) = ly/jm*j, z(k) = lz/km*k), which replace the reference's
  * node coordinates x(:,:,:,1:3) where a boundary condition reads them

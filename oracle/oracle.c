@@ -469,6 +469,7 @@ struct orc_box {
     double lx, ly, lz, dx, dy, dz;
     int bc[6]; /* bctype of faces 1..6: 0 nothing done here, else boucon's */
     double twall[6];
+    double xslip;
 };
 
 #define IX(b, i, j, k) ((size_t)((i) + HM) + (size_t)(b)->ni * ((size_t)((j) + HM) + (size_t)(b)->nj * (size_t)((k) + HM)))
@@ -569,7 +570,7 @@ int orc_box_set_bc(orc_box *b, const int *bctype)
         if (bctype[f] != 0 && bctype[f] != 23 && bctype[f] != 50 && bctype[f] != 60 &&
             !((bctype[f] == 41 || bctype[f] == 42) && f < 4) &&
             !(bctype[f] == 21 && (f == 1 || f == 3)) && bctype[f] != 31 &&
-            !(bctype[f] == 421 && (f == 2 || f == 3)))
+            !((bctype[f] == 421 || bctype[f] == 411) && (f == 2 || f == 3)))
             return -1;
     for (int f = 0; f < 6; ++f) b->bc[f] = bctype[f];
     return 0;
@@ -587,6 +588,8 @@ void orc_box_set_twall(orc_box *b, const double *twall)
     for (int f = 0; f < 6; ++f) b->twall[f] = twall[f];
 }
 
+void orc_box_set_xslip(orc_box *b, double xslip) { b->xslip = xslip; }
+
 static void box_boucon(orc_box *b, double dt)
 {
     int any = 0;
@@ -599,6 +602,7 @@ static void box_boucon(orc_box *b, double dt)
     u.lx = b->lx; u.ly = b->ly; u.lz = b->lz;
     for (int f = 0; f < 6; ++f) u.twall[f] = b->twall[f];
     u.deltat = dt;
+    u.xslip = b->xslip;
     const size_t n = b->n3;
     double *q = malloc(5 * n * sizeof(double)), *vel = malloc(3 * n * sizeof(double));
     for (int m = 0; m < 5; ++m) memcpy(q + m * n, b->q[m], n * sizeof(double));
@@ -612,6 +616,7 @@ static void box_boucon(orc_box *b, double dt)
         else if (b->bc[f] == 21) orc_outflow(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
         else if (b->bc[f] == 31) orc_fixbc(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
         else if (b->bc[f] == 421) orc_slipadib(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
+        else if (b->bc[f] == 411) orc_slipisot(&u, f + 1, q, b->rho, vel, b->prs, b->tmp);
     }
     for (int m = 0; m < 5; ++m) memcpy(b->q[m], q + m * n, n * sizeof(double));
     for (int c = 0; c < 3; ++c) memcpy(b->vel[c], vel + c * n, n * sizeof(double));

@@ -89,6 +89,8 @@ int orc_box_set_ntype(orc_box *b, int n0, int n1, int n2);
 int orc_box_set_bc(orc_box *b, const int *bctype);
 /* wall temperature of each face for bctype 41 (twall of src/bc.F90:29) */
 void orc_box_set_twall(orc_box *b, const double *twall);
+/* slip-nonslip point of bctype 411 (xslip of src/bc.F90:29) */
+void orc_box_set_xslip(orc_box *b, double xslip);
 
 /* Set the Taylor-Green initial field: tgvini (src/initialisation.F90:1079-1149)
  * followed by updateq (src/initialisation.F90:30-162 / src/fludyna.F90:254)
@@ -131,6 +133,7 @@ typedef struct {
     double lx, ly, lz;   /* gridcube lengths (x(i) = lx/im*i): gcnscbc's coordinates */
     double twall[6];     /* wall temperature of each face (bctype 41) */
     double deltat;       /* the step's time step, which outflow (bctype 21) reads */
+    double xslip;        /* slip-nonslip point of bctype 411 */
 } orc_uwd;
 
 double orc_minmod2(double a, double b);
@@ -200,6 +203,9 @@ void orc_symmetry(const orc_uwd *u, int face, double *q, double *rho, double *ve
  * step's time step) */
 void orc_outflow(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
                  double *tmp);
+/* slipisotwall (bctype 411) on face 3 or 4 (src/bc.F90:6921-7112) */
+void orc_slipisot(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+                  double *tmp);
 /* slipadibwall (bctype 421) on face 3 or 4 (src/bc.F90:7114-7313) */
 void orc_slipadib(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
                   double *tmp);

@@ -60,6 +60,8 @@ def oracle_lib():
         lib.orc_box_set_bc.argtypes = [_V, ctypes.POINTER(ctypes.c_int)]
         lib.orc_box_set_twall.restype = None
         lib.orc_box_set_twall.argtypes = [_V, ctypes.POINTER(ctypes.c_double)]
+        lib.orc_box_set_xslip.restype = None
+        lib.orc_box_set_xslip.argtypes = [_V, ctypes.c_double]
         lib.orc_df_compact.argtypes = [_I, _I, _D, _D]
         lib.orc_diff6ec.argtypes = [_I, _I, _D, _D]
         lib.orc_compact_filter.argtypes = [_I, _I, _F, _F, _F, _D, _D]
@@ -201,6 +203,10 @@ class OracleBox:
         """Wall temperature of each face for bctype 41 (default 1)."""
         self.lib.orc_box_set_twall(self.h, (ctypes.c_double * 6)(*[float(t) for t in twall]))
 
+    def set_xslip(self, xslip):
+        """Slip-nonslip point of bctype 411 (default 0)."""
+        self.lib.orc_box_set_xslip(self.h, float(xslip))
+
     def set_bc(self, bctype):
         """Boundary condition of faces 1..6 for the steps, applied in face
         order before each stage's qswap: 0 (nothing done), 23 (gcnscbc,
@@ -293,7 +299,7 @@ class UwdCfg(ctypes.Structure):
                 ("shkcrt", ctypes.c_double), ("gamma", ctypes.c_double),
                 ("mach", ctypes.c_double), ("lx", ctypes.c_double), ("ly", ctypes.c_double),
                 ("lz", ctypes.c_double), ("twall", ctypes.c_double * 6),
-                ("deltat", ctypes.c_double)]
+                ("deltat", ctypes.c_double), ("xslip", ctypes.c_double)]
 
 
 _uwd_declared = False
@@ -358,6 +364,8 @@ def uwd_lib():
         lib.orc_outflow.argtypes = [P, _I, _D, _D, _D, _D, _D]
         lib.orc_fixbc.restype = None
         lib.orc_fixbc.argtypes = [P, _I, _D, _D, _D, _D, _D]
+        lib.orc_slipisot.restype = None
+        lib.orc_slipisot.argtypes = [P, _I, _D, _D, _D, _D, _D]
         lib.orc_slipadib.restype = None
         lib.orc_slipadib.argtypes = [P, _I, _D, _D, _D, _D, _D]
         lib.orc_uwd_rk3_step.restype = _I
@@ -428,11 +436,12 @@ class UwdBox:
 
     def __init__(self, dims, npdc=(3, 3, 3), recon_schem=3, lchardecomp=True, bfacmpld=0.7,
                  shkcrt=0.01, gamma=1.4, mach=1.0, lengths=(1.0, 1.0, 0.0),
-                 twall=(1.0,) * 6):
+                 twall=(1.0,) * 6, xslip=0.0):
         im, jm, km = dims
         self.cfg = UwdCfg(im, jm, km, (ctypes.c_int * 3)(*npdc), recon_schem, int(lchardecomp),
                           bfacmpld, shkcrt, gamma, mach, *lengths,
-                          (ctypes.c_double * 6)(*[float(t) for t in twall]), 0.0)
+                          (ctypes.c_double * 6)(*[float(t) for t in twall]), 0.0,
+                          float(xslip))
         self.dims = (im, jm, km)
         self.hshape = (im + 1 + 2 * HM, jm + 1 + 2 * HM, km + 1 + 2 * HM)
         self.ishape = (im + 1, jm + 1, km + 1)
@@ -532,6 +541,14 @@ class UwdBox:
         self.cfg.deltat = float(dt)
         lib.orc_outflow(ctypes.byref(self.cfg), int(face), _p(o["q"]), _p(o["rho"]),
                         _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
+        return o
+
+    def slipisot(self, st, face):
+        """A copy of st after slipisotwall (bctype 411) on face 3 or 4."""
+        lib = uwd_lib()
+        o = self._state(st)
+        lib.orc_slipisot(ctypes.byref(self.cfg), int(face), _p(o["q"]), _p(o["rho"]),
+                         _p(o["vel"]), _p(o["prs"]), _p(o["tmp"]))
         return o
 
     def slipadib(self, st, face):

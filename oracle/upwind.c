@@ -1593,6 +1593,51 @@ void orc_slipadib(const orc_uwd *u, int face, double *q, double *rho, double *ve
         }
 }
 
+/* slipisotwall (src/bc.F90:6921-7112), bctype 411, on face 3 or 4: on face
+ * 3 the pressure by num1d3*(4 p1 - p2) and, where the node's gridcube
+ * x = lx/im*i is at most xslip, u and T extrapolated alike with v = 0, else
+ * the node at rest (the wall blowing of datin/wallbs.dat, unread, zero) at
+ * the face's wall temperature; w = 0.  Face 4 is the isothermal noslip wall.
+ * The density from p/T*const2, q by fvar2q's pressure form. */
+void orc_slipisot(const orc_uwd *u, int face, double *q, double *rho, double *vel, double *prs,
+                  double *tmp)
+{
+    if (face == 4) {
+        orc_wall(u, 4, 1, q, rho, vel, prs, tmp);
+        return;
+    }
+    if (face != 3) return;
+    const shape_t s = shape_of(u);
+    const double const2 = u->gamma * (u->mach * u->mach), const6 = 1.0 / (u->gamma - 1.0);
+    const double n1d3 = 1.0 / 3.0;
+    for (int k = 0; k <= u->km; ++k)
+        for (int i = 0; i <= u->im; ++i) {
+            const long h = hidx(&s, i, 0, k), h1 = hidx(&s, i, 1, k), h2 = hidx(&s, i, 2, k);
+            const double pe = n1d3 * (4.0 * prs[h1] - prs[h2]);
+            const double x = u->lx / (double)u->im * (double)i;
+            double ue, te;
+            if (x <= u->xslip) {
+                ue = n1d3 * (4.0 * vel[h1] - vel[h2]);
+                te = n1d3 * (4.0 * tmp[h1] - tmp[h2]);
+            } else {
+                ue = 0.0;
+                te = u->twall[2];
+            }
+            const double re = pe / te * const2;
+            prs[h] = pe;
+            tmp[h] = te;
+            rho[h] = re;
+            vel[h] = ue;
+            vel[s.nf + h] = 0.0;
+            vel[2 * s.nf + h] = 0.0;
+            q[h] = re;
+            q[s.nf + h] = re * ue;
+            q[2 * s.nf + h] = re * 0.0;
+            q[3 * s.nf + h] = re * 0.0;
+            q[4 * s.nf + h] = pe * const6 + re * (0.5 * (ue * ue + 0.0 * 0.0 + 0.0 * 0.0));
+        }
+}
+
 /* fixbc (src/bc.F90:7562-7648), bctype 31: face 3 held at rho 2, p 1 and
  * face 4 at rho 1, p 2.5, both at rest, T = thermal(rho, p) = p/rho*const2,
  * q by fvar2q's pressure form; the other faces are left alone. */
@@ -1687,6 +1732,8 @@ void orc_boucon(const orc_uwd *u, const int *bctype, int udf, double time, doubl
             orc_fixbc(u, f + 1, q, rho, vel, prs, tmp);
         } else if (bctype[f] == 421) {
             orc_slipadib(u, f + 1, q, rho, vel, prs, tmp);
+        } else if (bctype[f] == 411) {
+            orc_slipisot(u, f + 1, q, rho, vel, prs, tmp);
         }
     }
 }
@@ -1766,7 +1813,7 @@ int orc_uwd_rk3_step(const orc_uwd *u, int compact, int explicit_fd, const int *
     for (int f = 0; f < 6; ++f) {
         if (bctype[f] != 0 && bctype[f] != 1 && bctype[f] != 23 && bctype[f] != 50 &&
             bctype[f] != 60 && bctype[f] != 41 && bctype[f] != 42 && bctype[f] != 21 &&
-            bctype[f] != 31 && bctype[f] != 421)
+            bctype[f] != 31 && bctype[f] != 421 && bctype[f] != 411)
             return 3;
         anyudf |= bctype[f] == 0;
         anybc |= bctype[f] != 1;

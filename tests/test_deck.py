@@ -53,7 +53,7 @@ def test_riemann_deck():
 
 def test_unbuilt_deck_settings_are_rejected(tmp_path):
     src = os.path.join(DECKS, "tgv", "datin", "input.tgv")
-    for old, new, what in (("\n1\n1\n1 \n1\n1\n1\n", "\n411, 0.5, 1.0\n1\n1 \n1\n1\n1\n", "bctype"),
+    for old, new, what in (("\n1\n1\n1 \n1\n1\n1\n", "\n11\n1\n1 \n1\n1\n1\n", "bctype"),
                            ("643c, 643c, rk3", "643c, 643c, rk4", "rk3"),
                            ("\ntgv\n", "\nchannel\n", "flow type")):
         text = open(src).read()

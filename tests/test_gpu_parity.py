@@ -11,6 +11,7 @@ Tolerances (written per test):
   * whole steps: KE and enstrophy within 1e-10 relative, primitive fields
     within 1e-9 absolute (the bar the reference sets for 100 RK3 steps).
 """
+import math
 import os
 
 import numpy as np
@@ -172,7 +173,7 @@ def test_gcnscbc_central_steps_match_checker(v):
 @pytest.mark.parametrize("faces", [(50, 50, 50, 50, 0, 0), (60, 60, 60, 60, 0, 0),
                                    (60, 50, 50, 60, 0, 0), (41, 41, 42, 42, 0, 0),
                                    (50, 21, 50, 21, 0, 0), (50, 50, 31, 31, 0, 0),
-                                   (50, 21, 421, 421, 0, 0)])
+                                   (50, 21, 421, 421, 0, 0), (50, 50, 411, 411, 0, 0)])
 def test_zeroextrap_central_steps_match_checker(faces):
     """zeroextrap (bctype 50), symmetry (60) and the noslip walls (41, 42) on
     the four faces of the bounded x and y axes of a three-dimensional box
@@ -188,6 +189,7 @@ def test_zeroextrap_central_steps_match_checker(faces):
     ob = pyoracle.OracleBox(dims, variant=0, npdc=(4, 4, 3), mach=mach)
     ob.set_bc(faces)
     ob.set_twall(tw)
+    ob.set_xslip(math.pi)  # x(16) = 2 pi/32*16 is pi exactly: it slips
     pulse(ob, mach)
     st = ob.state_haloed()
     p = ag.default_params(dims, npdci=4, npdcj=4, npdck=3, mach=mach)
@@ -195,6 +197,8 @@ def test_zeroextrap_central_steps_match_checker(faces):
     s.upload_geom(*ag.box_metrics(p))
     s.upload_state(st["q"], st["rho"], st["vel"], st["prs"], st["tmp"])
     s.set_twall(tw)
+    s.set_lengths(2 * math.pi, 2 * math.pi, 2 * math.pi)  # the checker box's
+    s.set_xslip(math.pi)
     s.set_bc(tuple(f if f else 1 for f in faces), 0)
     dt = 0.01
     for n in range(3):

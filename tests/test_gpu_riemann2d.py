@@ -245,6 +245,8 @@ def _boucon_ref(box, st, faces):
             o = box.fixbc(o, f + 1)
         elif b == 421:
             o = box.slipadib(o, f + 1)
+        elif b == 411:
+            o = box.slipisot(o, f + 1)
     return o
 
 
@@ -252,11 +254,12 @@ def _boucon_ref(box, st, faces):
                                    (60, 60, 60, 60, 1, 1), (60, 23, 50, 60, 1, 1),
                                    (41, 42, 41, 42, 1, 1), (50, 41, 60, 42, 1, 1),
                                    (50, 21, 50, 21, 1, 1), (50, 50, 31, 31, 1, 1),
-                                   (50, 21, 421, 421, 1, 1)])
+                                   (50, 21, 421, 421, 1, 1), (50, 21, 411, 411, 1, 1)])
 def test_zeroextrap_steps_match_checker(faces):
     """zeroextrap (bctype 50), symmetry (60), the noslip walls (41 at its
     face's wall temperature, 42), the adiabatic slip wall (421, faces 3 and
-    4), fixbc (31) and outflow (21, faces 2 and 4) on the faces
+    4), the isothermal one (411, slipping up to x = 0.5), fixbc (31) and
+    outflow (21, faces 2 and 4) on the faces
     of a 2-D box, alone, mixed and with gcnscbc on faces 2 and 4 (the
     reference's face order 1..6), through boucon and whole upwind steps
     against the checker (boucon alone runs with the time step of no step, 0)."""
@@ -274,9 +277,10 @@ def test_zeroextrap_steps_match_checker(faces):
     s.set_lengths(1.0, 1.0, 0.0)
     tw = (1.05, 0.95, 1.1, 0.9, 1.0, 1.0)
     s.set_twall(tw)
+    s.set_xslip(0.5)
     s.set_bc(faces, 0)
     box = po.UwdBox(dims, npdc=(4, 4, 3), recon_schem=3, lchardecomp=True, bfacmpld=0.7,
-                    shkcrt=0.01, gamma=p.gamma, mach=mach, twall=tw)
+                    shkcrt=0.01, gamma=p.gamma, mach=mach, twall=tw, xslip=0.5)
     dxi9 = dxi.reshape(dxi.shape[:3] + (9,), order="F")
     s.boucon()
     compare_plane(s.download_state(), _boucon_ref(box, st, faces), "boucon")

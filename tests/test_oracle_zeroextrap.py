@@ -307,3 +307,43 @@ def test_slipadib_wall_only_on_faces_3_and_4():
         bc[f] = 421
         with pytest.raises(ValueError):
             ob.set_bc(tuple(bc))
+
+
+def test_slipisot_wall_slips_up_to_xslip():
+    """slipisotwall (411) on face 3: where the node's gridcube x = lx/im*i is
+    at most xslip, the slip wall's node (u and T on the zero-slope parabola,
+    v = w = 0); past it, at rest at the face's wall temperature; the pressure
+    on the parabola and the density from the equation of state throughout.
+    Face 4 is the isothermal noslip wall."""
+    dims = (12, 10, 0)
+    tw = (1.3, 1.1, 0.9, 1.2, 1.0, 1.0)
+    xslip = 0.5  # nodes 0..6 slip, x(6) = 0.5 exactly
+    box = po.UwdBox(dims, npdc=(4, 4, 3), gamma=GAMMA, mach=MACH, twall=tw, xslip=xslip)
+    c = (1.1, 0.3, -0.2, 0.05, 1.0 / (GAMMA * MACH * MACH))
+    d = (0.01, 0.02, -0.015, 0.005, 0.03)
+    st, _ = parabolic_state(dims, 3, c, d)
+    o = box.slipisot(st, 3)
+    const2 = GAMMA * MACH * MACH
+    I = slice(HM, HM + dims[0] + 1)
+    slip = np.arange(dims[0] + 1) <= 6
+    u = o["vel"][I, HM, HM, :]
+    p, rho, t = o["prs"][I, HM, HM], o["rho"][I, HM, HM], o["tmp"][I, HM, HM]
+    assert np.max(np.abs(p - c[4])) <= 1e-14 * c[4]
+    assert np.max(np.abs(u[slip, 0] - c[1])) <= 1e-14
+    assert np.all(u[~slip, 0] == 0.0) and np.all(u[:, 1:] == 0.0)
+    t1, t2 = st["tmp"][I, HM + 1, HM], st["tmp"][I, HM + 2, HM]
+    assert np.max(np.abs(t[slip] - (4.0 * t1[slip] - t2[slip]) / 3.0)) <= 1e-14
+    assert np.all(t[~slip] == tw[2])
+    assert np.max(np.abs(rho - p / t * const2)) <= 1e-14
+    q = o["q"][I, HM, HM]
+    assert np.array_equal(q[..., 1], rho * u[:, 0])
+    assert np.max(np.abs(q[..., 4] - (p / (GAMMA - 1.0) + 0.5 * rho * u[:, 0] ** 2))) \
+        <= 1e-14 * np.max(p)
+    st4, _ = parabolic_state(dims, 4, c, d)
+    a, b = box.slipisot(st4, 4), box.wall(st4, 4, True)
+    for nm in ("q", "rho", "vel", "prs", "tmp"):
+        assert np.array_equal(a[nm], b[nm])
+    ob = po.OracleBox((16, 12, 10), variant=0, npdc=(4, 4, 3))
+    ob.set_bc((0, 0, 411, 411, 0, 0))
+    with pytest.raises(ValueError):
+        ob.set_bc((411, 0, 0, 0, 0, 0))
